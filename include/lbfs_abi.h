@@ -1,0 +1,159 @@
+/*
+ * C ABI of the MI355X LBFS + IBM + spring-network membrane time step.
+ *
+ * Plain pointers and sizes only.  All device memory is owned by the context;
+ * the caller owns every host buffer it passes in.  Every call returns an int
+ * status (LBFS_OK or a negative LBFS_E_* code) and never exits the process;
+ * lbfs_last_error() gives the message of the last failure on a context.
+ * One context = one HIP stream; call a context from one host thread at a time.
+ *
+ * Cell ids at this boundary are the caller's (the mesh description's) ids;
+ * the library keeps its own tile-blocked order internally.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   gpu_solver::General_Purpose_Solver_mk_i   Solver_gpu.h:28-31, Solver_gpu.cu:178-1275
+ *   device array set-up / marshalling          Solver_gpu.cu:394-742, 1821-2151
+ *   per-stage kernels                          LBFS.hpp:44-96, immersed_boundary_method.hpp:11-49,
+ *                                              spring_network.hpp:18-106
+ */
+#ifndef LBFS_ABI_H
+#define LBFS_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define LBFS_OK 0
+#define LBFS_E_INVALID_ARG -1
+#define LBFS_E_HIP -2
+#define LBFS_E_NONFINITE -3
+#define LBFS_E_UNSUPPORTED -4
+#define LBFS_E_NO_DEVICE -5
+#define LBFS_E_STATE -6
+
+/* ---- lattices ---- */
+#define LBFS_D3Q15 0 /* the reference lattice (Solver_gpu.cu:1407-1445) */
+#define LBFS_D3Q19 1
+#define LBFS_D2Q9 2
+
+/* ---- gradient reconstruction (gradients.cpp:255-265) ---- */
+#define LBFS_GRAD_HYBRID 0
+#define LBFS_GRAD_LS 1
+#define LBFS_GRAD_GG 2
+
+/* ---- boundary condition types (preprocessor.cpp:471-493) ---- */
+#define LBFS_BC_DIRICHLET 1
+#define LBFS_BC_NEUMANN 2
+
+/* ---- semantics flags for reference quirks (lbfs_params.flags) ---- */
+#define LBFS_FLAG_UPWIND_BAND 1      /* tie band |e.n| <= 0.001 dt_f, LBFS.cu:349-352 */
+#define LBFS_FLAG_NEUMANN_SIGNED 2   /* Neumann ghost gradient (1 - n), gradients.cpp:412-414 */
+#define LBFS_FLAG_GHOST_DIR_GPU 4    /* Dirichlet ghost gradient along owner->ghost, LBFS.cu:1965-1985 */
+#define LBFS_FLAG_LS_DET_COMPAT 16   /* LS determinant as written in gradients.cpp:282-284 */
+#define LBFS_FLAG_CLOSED_FACES 32    /* face area |sum of triangle normals| / 2 instead of the sum of
+                                        triangle areas (unstructured_mesh.cpp:1193-1196): cells close exactly */
+
+/* ---- IBM kernels ---- */
+#define LBFS_IBM_LINEAR2 0 /* 2-point trilinear on the logical grid, immersed_boundary_method.cu:14-97, 947-1024 */
+#define LBFS_IBM_COS4 1    /* 4-point cosine on physical distances, renormalised */
+
+/*
+ * Hexahedral mesh in OpenFOAM polyMesh form (unstructured_mesh.cpp:670-941):
+ * internal faces first, then boundary faces.  Quad faces, vertices ordered so
+ * that the normal points from owner to neighbour.  Periodic wrap faces are
+ * internal faces whose neighbour image is translated by face_shift.
+ */
+typedef struct {
+    int n_points;
+    const double *points;        /* [3 n_points] */
+    int n_cells;
+    int n_faces;
+    int n_internal;
+    const int *face_vertices;    /* [4 n_faces] */
+    const int *owner;            /* [n_faces] */
+    const int *neighbour;        /* [n_internal] */
+    const double *face_shift;    /* [3 n_faces] or NULL */
+    const int *bc_rho_type;      /* [n_faces - n_internal] */
+    const int *bc_vel_type;      /* [n_faces - n_internal] */
+    const double *bc_values;     /* [4 (n_faces - n_internal)] rho, u, v, w */
+    /* logical structure (cell id = i + j nx + k nx ny) used for tiling and IBM; 0 if unknown */
+    int nx, ny, nz;
+    double origin[3];
+    double spacing;              /* nominal cell size (IBM stencil unit) */
+} lbfs_mesh_desc;
+
+typedef struct {
+    int lattice;
+    int gradient_type;
+    int flags;
+    double visc;        /* kinematic viscosity (global_variables.cpp:24) */
+    double gamma;       /* preconditioning gamma, 1 = none */
+    double fneq_min;    /* face dt cap 3 nu / (gamma fneq_min), <= 0 disables */
+    double dt;          /* time-marching step */
+    double rho0;        /* reference density; the device stores rho - rho0 */
+    double body_force[3];
+} lbfs_params;
+
+typedef struct lbfs_ctx lbfs_ctx;
+
+/* context lifetime: Solver_gpu.cu:394-742 (set-up), no counterpart for destroy */
+int lbfs_create(const lbfs_mesh_desc *mesh, const lbfs_params *params, lbfs_ctx **out);
+int lbfs_destroy(lbfs_ctx *ctx);
+const char *lbfs_last_error(const lbfs_ctx *ctx);
+int lbfs_sync(lbfs_ctx *ctx);
+
+/* fluid state in caller cell order, FP64 host arrays of n_cells
+ * (Solution::clone(double4*) Solution.cpp:308-319, Solver_gpu.cu:2138-2151) */
+int lbfs_set_state(lbfs_ctx *ctx, const double *rho, const double *u, const double *v, const double *w);
+int lbfs_get_state(lbfs_ctx *ctx, double *rho, double *u, double *v, double *w);
+
+/* whole time steps: the time loop body of Solver_gpu.cu:749-1246 */
+int lbfs_step(lbfs_ctx *ctx, int nsteps);
+
+/* per-stage entry points (parity tests) */
+int lbfs_stage_gradients(lbfs_ctx *ctx, double *grad /* [12 n_cells] or NULL */);   /* LBFS.cu:2177-2240 */
+int lbfs_stage_face_flux(lbfs_ctx *ctx, double *flux /* [4 n_faces] */);           /* LBFS.cu:2574-2810 */
+int lbfs_stage_residual(lbfs_ctx *ctx, double *res /* [4 n_cells] */);             /* LBFS.cu:2242-2281 */
+
+/* cell force density added to the momentum residual (Solver_gpu.cu:834-846, 1008-1010) */
+int lbfs_get_cell_force(lbfs_ctx *ctx, double *f /* [3 n_cells] */);
+
+/* diagnostics: residual RMS (residuals.cpp:114-132) of the last step and FP64 mass */
+typedef struct {
+    double rho_rms, u_rms, v_rms, w_rms;
+    double total_mass;
+    double max_speed;
+    int finite;
+} lbfs_diag;
+int lbfs_reduce_diagnostics(lbfs_ctx *ctx, double ref_rho, double ref_u, lbfs_diag *out);
+
+/* ---- measurement (no counterpart in the reference; used by bench.py) ----
+ * HIP events recorded on the context's stream.  lbfs_time_steps: wall time of
+ * nsteps whole steps.  lbfs_time_kernels: average duration per launch of each
+ * kernel of the step, names[i] / ms[i] for i < *count (at most max_count). */
+int lbfs_time_steps(lbfs_ctx *ctx, int nsteps, double *ms_total);
+int lbfs_time_kernels(lbfs_ctx *ctx, int nsteps, int max_count, const char **names, double *ms, int *count);
+/* device bytes touched per launch by the model of each kernel (same order as lbfs_time_kernels) */
+int lbfs_model_bytes(lbfs_ctx *ctx, int max_count, double *bytes, int *count);
+
+/* ---- host-side mesh helpers (no device work) ---- */
+/* counts of a generated nx*ny*nz hex box (unstructured_mesh.cpp:239-666) */
+int lbfs_mesh_hex_counts(int nx, int ny, int nz, const int *periodic /* [3] */,
+                         int *n_points, int *n_faces, int *n_internal);
+/* seeded jittered hex box; patch ids of boundary faces: 0 left 1 right 2 bottom 3 top 4 back 5 front */
+int lbfs_mesh_hex_generate(int nx, int ny, int nz, const double *lengths /* [3] */, const double *origin /* [3] */,
+                           const int *periodic /* [3] */, double jitter, uint64_t seed,
+                           double *points, int *face_vertices, int *owner, int *neighbour,
+                           double *face_shift, int *patch);
+/* FP64 geometry of a mesh (unstructured_mesh.cpp:1592-1692) */
+int lbfs_mesh_geometry(const lbfs_mesh_desc *mesh, double visc, double gamma, double fneq_min, int flags,
+                       double *area, double *normal, double *face_centroid, double *face_dt,
+                       double *centroid /* [3 (n_cells + n_bc)] */, double *volume);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,47 @@
+// The opaque context behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "fluid_kernels.hpp"
+#include "lbfs_abi.h"
+#include "mesh_host.hpp"
+
+struct lbfs_ctx {
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    lbfs_params params{};
+    int mesh_nx = 0, mesh_ny = 0, mesh_nz = 0;
+    double mesh_origin[3] = {0, 0, 0};
+    double mesh_spacing = 0.0;
+
+    lbfs::Geometry geo;
+    lbfs::DeviceLayout lay;
+    lbfs::FluidArgs args{};
+
+    float4 *st[2] = {nullptr, nullptr};
+    int cur = 0;
+    float4 *grad = nullptr, *res = nullptr, *face_scratch = nullptr;
+    double *diag_part = nullptr;
+    float *static_force = nullptr;  // [3 n] body force + closure term
+    float *cell_force = nullptr;    // [3 n] static force + spread IBM force
+    std::vector<void *> allocs;
+
+    int fail(int code, const char *fmt, ...);
+    int enqueue_step();
+    void release_extensions();
+
+    // The step as an ordered list of kernel stages (each enqueues one launch
+    // on `stream`), with the modelled device bytes of that launch.
+    struct Stage {
+        const char *name;
+        std::function<int()> run;
+        double model_bytes;
+    };
+    std::vector<Stage> step_stages();
+};

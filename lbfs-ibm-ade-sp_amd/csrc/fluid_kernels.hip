@@ -1,0 +1,333 @@
+// LBFS fluid kernels for gfx950 (FP32 working precision, density stored as
+// rho - rho0).
+//
+//   k_gradients      per-cell least-squares gradients of (rho, u, v, w)
+//                    (LBFS.cu:2177-2240, ghost rows of LBFS.cu:1933-2048 folded in)
+//   k_flux_update    one workgroup per tile of cells: every face touching the
+//                    tile is reconstructed once into LDS (LBFS.cu:2574-2810),
+//                    then each cell gathers its six face fluxes in a fixed
+//                    order and advances (LBFS.cu:2242-2281, 2089-2172).  No
+//                    atomics: results are bitwise repeatable.
+//
+// Face reconstruction works on deviations from the rest state: with
+// g_k = feq_k - w_k rho0, the interface density deviation and momentum are
+// sum g_k and sum e_k g_k, and the flux of the interface equilibrium is taken
+// from its closed-form moments (rho cs^2 I + rho u u / gamma), which the
+// discrete sums of the reference reproduce exactly on these lattices.
+#include <hip/hip_runtime.h>
+
+#include "fluid_kernels.hpp"
+#include "lattice.hpp"
+
+namespace lbfs {
+
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks)
+{
+    // consecutive tile ids land on the same XCD (8 XCDs, round-robin dispatch)
+    const int per = (nblocks + 7) >> 3;
+    const int xcd = bid & 7, idx = bid >> 3;
+    const int full = nblocks - (per - 1) * 8;  // XCDs that get `per` blocks
+    int base = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
+    return base + idx;
+}
+
+// ---------------------------------------------------------------------------
+// gradients
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gradients(FluidArgs a, const float4 *__restrict__ st,
+                                                   float4 *__restrict__ grad)
+{
+    const int nblk = gridDim.x;
+    const int b = xcd_remap(blockIdx.x, nblk);
+    const int i = b * 256 + threadIdx.x;
+    const int n = a.n_cells;
+    if (i >= n) return;
+    const float4 si = st[i];
+    float R[12];
+#pragma unroll
+    for (int t = 0; t < 12; t++) R[t] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const int j = a.stencil[(size_t)k * n + i];
+        float dr, du, dv, dw;
+        if (j >= 0) {
+            const float4 sj = st[j];
+            dr = sj.x - si.x; du = sj.y - si.y; dv = sj.z - si.z; dw = sj.w - si.w;
+        } else {
+            // ghost: Dirichlet mirror 2 bc - owner, Neumann copy (Solution.cpp:759-833)
+            const int bc = -1 - j;
+            const int2 ty = a.bc_type[bc];
+            const float4 bv = a.bc_val[bc];
+            dr = ty.x == 1 ? 2.0f * (bv.x - si.x) : 0.0f;
+            du = ty.y == 1 ? 2.0f * (bv.y - si.y) : 0.0f;
+            dv = ty.y == 1 ? 2.0f * (bv.z - si.z) : 0.0f;
+            dw = ty.y == 1 ? 2.0f * (bv.w - si.w) : 0.0f;
+        }
+        const float wx = a.gw[(size_t)(3 * k) * n + i];
+        const float wy = a.gw[(size_t)(3 * k + 1) * n + i];
+        const float wz = a.gw[(size_t)(3 * k + 2) * n + i];
+        R[0] += wx * dr; R[1] += wy * dr; R[2] += wz * dr;
+        R[3] += wx * du; R[4] += wy * du; R[5] += wz * du;
+        R[6] += wx * dv; R[7] += wy * dv; R[8] += wz * dv;
+        R[9] += wx * dw; R[10] += wy * dw; R[11] += wz * dw;
+    }
+    grad[3 * (size_t)i + 0] = make_float4(R[0], R[1], R[2], R[3]);
+    grad[3 * (size_t)i + 1] = make_float4(R[4], R[5], R[6], R[7]);
+    grad[3 * (size_t)i + 2] = make_float4(R[8], R[9], R[10], R[11]);
+}
+
+// ---------------------------------------------------------------------------
+// face reconstruction and flux
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void load_grad(const float4 *__restrict__ grad, int i, float g[12])
+{
+    const float4 a = grad[3 * (size_t)i], b = grad[3 * (size_t)i + 1], c = grad[3 * (size_t)i + 2];
+    g[0] = a.x; g[1] = a.y; g[2] = a.z; g[3] = a.w;
+    g[4] = b.x; g[5] = b.y; g[6] = b.z; g[7] = b.w;
+    g[8] = c.x; g[9] = c.y; g[10] = c.z; g[11] = c.w;
+}
+
+template <class L>
+__device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__restrict__ st,
+                                            const float4 *__restrict__ grad, int tf)
+{
+    const float4 fa = a.fA[tf], fb = a.fB[tf], fcv = a.fC[tf];
+    const int2 on = a.fI[tf];
+    const int o = on.x, nb = on.y;
+    const bool ghost = nb >= a.n_cells;
+    const float nx = fa.x, ny = fa.y, nz = fa.z, A = fa.w, dt = fb.w;
+    const float4 so = st[o];
+    float go[12], gn[12];
+    load_grad(grad, o, go);
+    const float sov[4] = {so.x, so.y, so.z, so.w};
+    float po[4], pn[4], p0[4];
+    if (!ghost) {
+        const float4 sn = st[nb];
+        load_grad(grad, nb, gn);
+        const float snv[4] = {sn.x, sn.y, sn.z, sn.w};
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            po[v] = sov[v] - (fb.x * go[3 * v] + fb.y * go[3 * v + 1] + fb.z * go[3 * v + 2]);
+            pn[v] = snv[v] - (fcv.x * gn[3 * v] + fcv.y * gn[3 * v + 1] + fcv.z * gn[3 * v + 2]);
+            p0[v] = 0.5f * (po[v] + pn[v]);
+        }
+    } else {
+        // ghost gradient (gradients.cpp:393-505 / LBFS.cu:1933-2048) and face
+        // value from the boundary condition (Solver.cpp:855-900)
+        const int bc = nb - a.n_cells;
+        const int2 ty = a.bc_type[bc];
+        const float4 bv = a.bc_val[bc];
+        const float bvv[4] = {bv.x, bv.y, bv.z, bv.w};
+        const float d1n = sqrtf(fb.x * fb.x + fb.y * fb.y + fb.z * fb.z);
+        const float inv_dmag = 2.0f / d1n;  // ghost centroid sits halfway to the face
+        float dir[3];
+        if (a.flags & LBFS_FLAG_GHOST_DIR_GPU) { dir[0] = -fb.x / d1n; dir[1] = -fb.y / d1n; dir[2] = -fb.z / d1n; }
+        else { dir[0] = nx; dir[1] = ny; dir[2] = nz; }
+        const float nv[3] = {nx, ny, nz};
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int type = v == 0 ? ty.x : ty.y;
+            if (type == 1) {
+                const float dphi = 2.0f * (bvv[v] - sov[v]) * inv_dmag;
+#pragma unroll
+                for (int c = 0; c < 3; c++) gn[3 * v + c] = dphi * dir[c];
+                p0[v] = bvv[v];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const float fac = (a.flags & LBFS_FLAG_NEUMANN_SIGNED) ? 1.0f - nv[c] : 1.0f - fabsf(nv[c]);
+                    gn[3 * v + c] = go[3 * v + c] * fac;
+                }
+                p0[v] = sov[v] - (fb.x * gn[3 * v] + fb.y * gn[3 * v + 1] + fb.z * gn[3 * v + 2]);
+            }
+            po[v] = p0[v];
+            pn[v] = p0[v];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 12; t++) { go[t] *= dt; gn[t] *= dt; }
+    const float rho0 = a.rho0, ig = a.inv_gamma;
+    const bool band = (a.flags & LBFS_FLAG_UPWIND_BAND) != 0;
+    const float bw = 0.001f * dt;
+    float S0, S1x = 0.0f, S1y = 0.0f, S1z = 0.0f, Tx = 0.0f, Ty = 0.0f, Tz = 0.0f;
+    {
+        const float uu = p0[1] * p0[1] + p0[2] * p0[2] + p0[3] * p0[3];
+        const float P = -1.5f * uu * ig;
+        S0 = L::w(0) * (p0[0] + (rho0 + p0[0]) * P);
+    }
+#pragma unroll
+    for (int k = 1; k < L::Q; k++) {
+        constexpr int dummy = 0;
+        (void)dummy;
+        const int ex = L::ex[k], ey = L::ey[k], ez = L::ez[k];
+        const float en = ex * nx + ey * ny + ez * nz;
+        float ph[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const float ego = ex * go[3 * v] + ey * go[3 * v + 1] + ez * go[3 * v + 2];
+            const float egn = ex * gn[3 * v] + ey * gn[3 * v + 1] + ez * gn[3 * v + 2];
+            const float vo = po[v] - ego, vn = pn[v] - egn;
+            if (!band) {
+                ph[v] = en < 0.0f ? vn : vo;
+            } else {
+                const float vt = p0[v] - (ghost ? egn : 0.5f * (ego + egn));
+                ph[v] = en > bw ? vo : (en < -bw ? vn : vt);
+            }
+        }
+        const float eu = ex * ph[1] + ey * ph[2] + ez * ph[3];
+        const float uu = ph[1] * ph[1] + ph[2] * ph[2] + ph[3] * ph[3];
+        const float P = 3.0f * eu + (4.5f * eu * eu - 1.5f * uu) * ig;
+        const float gk = L::w(k) * (ph[0] + (rho0 + ph[0]) * P);
+        S0 += gk;
+        S1x += ex * gk; S1y += ey * gk; S1z += ez * gk;
+        const float eng = en * gk;
+        Tx += ex * eng; Ty += ey * eng; Tz += ez * eng;
+    }
+    const float rho_f = rho0 + S0;
+    const float mn = S1x * nx + S1y * ny + S1z * nz;
+    const float vf = a.visc_factor ? a.visc_factor[o] : 1.0f;
+    const float tau = a.visc3_over_gamma * vf / dt;
+    const float c1 = 1.0f - tau;
+    const float third = S0 * (1.0f / 3.0f);
+    const float q = mn / rho_f * ig;
+    float4 r;
+    r.x = mn * A;
+    r.y = A * (c1 * (third * nx + S1x * q) + tau * Tx);
+    r.z = A * (c1 * (third * ny + S1y * q) + tau * Ty);
+    r.w = A * (c1 * (third * nz + S1z * q) + tau * Tz);
+    return r;
+}
+
+template <class L>
+__global__ __launch_bounds__(256) void k_flux_update(FluidArgs a, const float4 *__restrict__ st,
+                                                     const float4 *__restrict__ grad, float4 *__restrict__ st_out,
+                                                     float4 *__restrict__ res_out, int do_update)
+{
+    extern __shared__ float4 lds_flux[];
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int f0 = a.tile_face_start[t], f1 = a.tile_face_start[t + 1];
+    for (int f = f0 + threadIdx.x; f < f1; f += blockDim.x) lds_flux[f - f0] = face_flux<L>(a, st, grad, f);
+    __syncthreads();
+    const int c0 = a.tile_cell_start[t], c1 = a.tile_cell_start[t + 1];
+    for (int i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
+        const uint32_t *sl = reinterpret_cast<const uint32_t *>(a.cell_slots) + 3 * (size_t)i;
+        const uint32_t s01 = sl[0], s23 = sl[1], s45 = sl[2];
+        const uint32_t s[6] = {s01 & 0xffffu, s01 >> 16, s23 & 0xffffu, s23 >> 16, s45 & 0xffffu, s45 >> 16};
+        float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            const float4 fl = lds_flux[s[k] & 0x7fffu];
+            const float sg = (s[k] & 0x8000u) ? -1.0f : 1.0f;
+            r0 += sg * fl.x; r1 += sg * fl.y; r2 += sg * fl.z; r3 += sg * fl.w;
+        }
+        const float iv = a.inv_vol[i];
+        r0 *= iv; r1 *= iv; r2 *= iv; r3 *= iv;
+        if (res_out) res_out[i] = make_float4(r0, r1, r2, r3);
+        if (!do_update) continue;
+        float fx = a.body_force[0], fy = a.body_force[1], fz = a.body_force[2];
+        if (a.cell_force) {
+            const float *cf = a.cell_force + 3 * (size_t)i;
+            fx += cf[0]; fy += cf[1]; fz += cf[2];
+        }
+        r1 += fx; r2 += fy; r3 += fz;
+        const float4 s0 = st[i];
+        const float dt = a.dt;
+        const float drho = s0.x + dt * r0;
+        const float rho = a.rho0 + drho;
+        const float inv_rho = 1.0f / rho;
+        float4 sn;
+        sn.x = drho;
+        sn.y = s0.y + dt * (r1 - s0.y * r0) * inv_rho;
+        sn.z = s0.z + dt * (r2 - s0.z * r0) * inv_rho;
+        sn.w = s0.w + dt * (r3 - s0.w * r0) * inv_rho;
+        st_out[i] = sn;
+    }
+}
+
+template <class L>
+__global__ __launch_bounds__(256) void k_face_flux_all(FluidArgs a, const float4 *__restrict__ st,
+                                                       const float4 *__restrict__ grad, float4 *__restrict__ out,
+                                                       int n_tile_faces)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < n_tile_faces) out[f] = face_flux<L>(a, st, grad, f);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 diagnostics: per-block partial sums (fixed order), summed on the host
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_diag_partials(int n, float rho0, const float4 *__restrict__ st,
+                                                       const float4 *__restrict__ res, const float *__restrict__ inv_vol,
+                                                       double *__restrict__ part)
+{
+    __shared__ double sh[6][256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        const float4 s = st[i];
+        const float4 r = res[i];
+        v[0] = (double)r.x * r.x;
+        v[1] = (double)r.y * r.y;
+        v[2] = (double)r.z * r.z;
+        v[3] = (double)r.w * r.w;
+        v[4] = ((double)rho0 + (double)s.x) / (double)inv_vol[i];
+        const double sp = sqrt((double)s.y * s.y + (double)s.z * s.z + (double)s.w * s.w);
+        v[5] = (isfinite(s.x) && isfinite(s.y) && isfinite(s.z) && isfinite(s.w)) ? sp : INFINITY;
+    }
+    for (int t = 0; t < 6; t++) sh[t][threadIdx.x] = v[t];
+    __syncthreads();
+    for (int stp = 128; stp > 0; stp >>= 1) {
+        if ((int)threadIdx.x < stp) {
+            for (int t = 0; t < 5; t++) sh[t][threadIdx.x] += sh[t][threadIdx.x + stp];
+            sh[5][threadIdx.x] = fmax(sh[5][threadIdx.x], sh[5][threadIdx.x + stp]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[6 * (size_t)blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_gradients(const FluidArgs &a, const float4 *st, float4 *grad, hipStream_t s)
+{
+    const int blocks = (a.n_cells + 255) / 256;
+    hipLaunchKernelGGL(k_gradients, dim3(blocks), dim3(256), 0, s, a, st, grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_flux_update(const FluidArgs &a, const float4 *st, const float4 *grad, float4 *st_out,
+                              float4 *res_out, int do_update, hipStream_t s)
+{
+    const size_t lds = sizeof(float4) * (size_t)a.max_tile_faces;
+    switch (a.lattice) {
+    case 0: hipLaunchKernelGGL(k_flux_update<D3Q15>, dim3(a.n_tiles), dim3(256), lds, s, a, st, grad, st_out, res_out, do_update); break;
+    case 1: hipLaunchKernelGGL(k_flux_update<D3Q19>, dim3(a.n_tiles), dim3(256), lds, s, a, st, grad, st_out, res_out, do_update); break;
+    case 2: hipLaunchKernelGGL(k_flux_update<D2Q9>, dim3(a.n_tiles), dim3(256), lds, s, a, st, grad, st_out, res_out, do_update); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_face_flux_all(const FluidArgs &a, const float4 *st, const float4 *grad, float4 *out, int n_tile_faces,
+                                hipStream_t s)
+{
+    const int blocks = (n_tile_faces + 255) / 256;
+    switch (a.lattice) {
+    case 0: hipLaunchKernelGGL(k_face_flux_all<D3Q15>, dim3(blocks), dim3(256), 0, s, a, st, grad, out, n_tile_faces); break;
+    case 1: hipLaunchKernelGGL(k_face_flux_all<D3Q19>, dim3(blocks), dim3(256), 0, s, a, st, grad, out, n_tile_faces); break;
+    case 2: hipLaunchKernelGGL(k_face_flux_all<D2Q9>, dim3(blocks), dim3(256), 0, s, a, st, grad, out, n_tile_faces); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_diag_partials(int n, float rho0, const float4 *st, const float4 *res, const float *inv_vol, double *part,
+                                hipStream_t s)
+{
+    const int blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_diag_partials, dim3(blocks), dim3(256), 0, s, n, rho0, st, res, inv_vol, part);
+    return hipGetLastError();
+}
+
+}  // namespace lbfs

@@ -1,0 +1,41 @@
+// Device-side argument block and launchers of the fluid kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "lbfs_abi.h"
+
+namespace lbfs {
+
+struct FluidArgs {
+    int n_cells, n_bc, n_tiles, max_tile_faces;
+    int lattice, flags;
+    float rho0, inv_gamma, visc3_over_gamma, dt;
+    float body_force[3];
+    // per cell (internal tile order)
+    const int *stencil;            // [6 n] SoA
+    const float *gw;               // [18 n] SoA
+    const float *inv_vol;          // [n]
+    const int16_t *cell_slots;     // [6 n]
+    const float *visc_factor;      // [n] or null
+    const float *cell_force;       // [3 n] or null (IBM force density)
+    // tiles
+    const int *tile_cell_start, *tile_face_start;
+    // tile faces
+    const float4 *fA, *fB, *fC;
+    const int2 *fI;
+    // boundary conditions
+    const int2 *bc_type;           // (rho type, vel type)
+    const float4 *bc_val;          // (rho - rho0, u, v, w)
+};
+
+hipError_t launch_gradients(const FluidArgs &a, const float4 *st, float4 *grad, hipStream_t s);
+hipError_t launch_flux_update(const FluidArgs &a, const float4 *st, const float4 *grad, float4 *st_out,
+                              float4 *res_out, int do_update, hipStream_t s);
+hipError_t launch_face_flux_all(const FluidArgs &a, const float4 *st, const float4 *grad, float4 *out,
+                                int n_tile_faces, hipStream_t s);
+hipError_t launch_diag_partials(int n, float rho0, const float4 *st, const float4 *res, const float *inv_vol,
+                                double *part, hipStream_t s);
+
+}  // namespace lbfs

@@ -1,0 +1,386 @@
+// C ABI implementation: context set-up, state transfer and the time step.
+#include "lbfs_abi.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "context.hpp"
+#include "fluid_kernels.hpp"
+#include "mesh_host.hpp"
+
+using namespace lbfs;
+
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return ctx->fail(LBFS_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+int lbfs_ctx::fail(int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+}
+
+template <class T>
+static int dev_upload(lbfs_ctx *ctx, T **dst, const T *src, size_t n)
+{
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, n * sizeof(T) + 16));
+    ctx->allocs.push_back(p);
+    if (src) HIPCHK(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    *dst = (T *)p;
+    return LBFS_OK;
+}
+
+static thread_local std::string g_create_error;
+
+extern "C" const char *lbfs_last_error(const lbfs_ctx *ctx)
+{
+    if (!ctx) return g_create_error.c_str();
+    return ctx->err.c_str();
+}
+
+static int fluid_setup(lbfs_ctx *ctx, const lbfs_mesh_desc &m, const lbfs_params &p)
+{
+    std::string err;
+    int rc = compute_geometry(m, p.visc, p.gamma, p.fneq_min, (p.flags & LBFS_FLAG_CLOSED_FACES) != 0, ctx->geo, err);
+    if (rc) return ctx->fail(rc, "geometry: %s", err.c_str());
+    std::vector<double> lhs, rhs;
+    rc = compute_ls(ctx->geo, p.gradient_type, (p.flags & LBFS_FLAG_LS_DET_COMPAT) != 0, lhs, rhs, err);
+    if (rc) return ctx->fail(rc, "least squares: %s", err.c_str());
+    rc = build_layout(m, ctx->geo, lhs, rhs, p.rho0, p.body_force, ctx->lay, err);
+    if (rc) return ctx->fail(rc, "layout: %s", err.c_str());
+    const DeviceLayout &d = ctx->lay;
+    const int n = d.n_cells, nbc = d.n_bc;
+    FluidArgs &a = ctx->args;
+    memset(&a, 0, sizeof(a));
+    a.n_cells = n;
+    a.n_bc = nbc;
+    a.n_tiles = d.n_tiles;
+    a.max_tile_faces = d.max_tile_faces;
+    a.lattice = p.lattice;
+    a.flags = p.flags;
+    a.rho0 = (float)p.rho0;
+    a.inv_gamma = (float)(1.0 / p.gamma);
+    a.visc3_over_gamma = (float)(p.visc * 3.0 / p.gamma);
+    a.dt = (float)p.dt;
+    for (int t = 0; t < 3; t++) a.body_force[t] = (float)p.body_force[t];
+    int *stencil; float *gw, *inv_vol; int16_t *slots; int *tcs, *tfs;
+    float4 *fA, *fB, *fC; int2 *fI, *bct; float4 *bcv;
+    if ((rc = dev_upload(ctx, &stencil, d.stencil.data(), d.stencil.size()))) return rc;
+    if ((rc = dev_upload(ctx, &gw, d.gw.data(), d.gw.size()))) return rc;
+    if ((rc = dev_upload(ctx, &inv_vol, d.inv_vol.data(), d.inv_vol.size()))) return rc;
+    if ((rc = dev_upload(ctx, &slots, d.cell_slots.data(), d.cell_slots.size()))) return rc;
+    if ((rc = dev_upload(ctx, &tcs, d.tile_cell_start.data(), d.tile_cell_start.size()))) return rc;
+    if ((rc = dev_upload(ctx, &tfs, d.tile_face_start.data(), d.tile_face_start.size()))) return rc;
+    const size_t ntf = d.face_ref.size();
+    if ((rc = dev_upload(ctx, &fA, (const float4 *)d.fA.data(), ntf))) return rc;
+    if ((rc = dev_upload(ctx, &fB, (const float4 *)d.fB.data(), ntf))) return rc;
+    if ((rc = dev_upload(ctx, &fC, (const float4 *)d.fC.data(), ntf))) return rc;
+    if ((rc = dev_upload(ctx, &fI, (const int2 *)d.fI.data(), ntf))) return rc;
+    std::vector<int2> bt(nbc > 0 ? nbc : 1);
+    std::vector<float4> bvals(nbc > 0 ? nbc : 1);
+    for (int b = 0; b < nbc; b++) {
+        const int rt = m.bc_rho_type ? m.bc_rho_type[b] : LBFS_BC_NEUMANN;
+        const int vt = m.bc_vel_type ? m.bc_vel_type[b] : LBFS_BC_NEUMANN;
+        if ((rt != 1 && rt != 2) || (vt != 1 && vt != 2))
+            return ctx->fail(LBFS_E_UNSUPPORTED, "boundary face %d: only Dirichlet (1) and Neumann (2) are supported", b);
+        bt[b] = make_int2(rt, vt);
+        const double *bv = m.bc_values ? m.bc_values + 4 * (size_t)b : nullptr;
+        bvals[b] = make_float4(bv ? (float)(bv[0] - p.rho0) : 0.0f, bv ? (float)bv[1] : 0.0f, bv ? (float)bv[2] : 0.0f,
+                               bv ? (float)bv[3] : 0.0f);
+    }
+    if ((rc = dev_upload(ctx, &bct, bt.data(), bt.size()))) return rc;
+    if ((rc = dev_upload(ctx, &bcv, bvals.data(), bvals.size()))) return rc;
+    a.stencil = stencil; a.gw = gw; a.inv_vol = inv_vol; a.cell_slots = slots;
+    a.tile_cell_start = tcs; a.tile_face_start = tfs;
+    a.fA = fA; a.fB = fB; a.fC = fC; a.fI = fI;
+    a.bc_type = bct; a.bc_val = bcv;
+    a.visc_factor = nullptr;
+    // cell force density: body force + closure term, IBM forces are added on top
+    float *sf, *cf;
+    if ((rc = dev_upload(ctx, &sf, d.static_force.data(), d.static_force.size()))) return rc;
+    if ((rc = dev_upload(ctx, &cf, d.static_force.data(), d.static_force.size()))) return rc;
+    ctx->static_force = sf;
+    ctx->cell_force = cf;
+    a.cell_force = cf;
+    for (int t = 0; t < 3; t++) a.body_force[t] = 0.0f;
+    // state buffers
+    if ((rc = dev_upload<float4>(ctx, &ctx->st[0], nullptr, n))) return rc;
+    if ((rc = dev_upload<float4>(ctx, &ctx->st[1], nullptr, n))) return rc;
+    if ((rc = dev_upload<float4>(ctx, &ctx->grad, nullptr, 3 * (size_t)n))) return rc;
+    if ((rc = dev_upload<float4>(ctx, &ctx->res, nullptr, n))) return rc;
+    if ((rc = dev_upload<float4>(ctx, &ctx->face_scratch, nullptr, ntf))) return rc;
+    if ((rc = dev_upload<double>(ctx, &ctx->diag_part, nullptr, 6 * (size_t)((n + 255) / 256)))) return rc;
+    HIPCHK(hipMemsetAsync(ctx->st[0], 0, sizeof(float4) * n, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->st[1], 0, sizeof(float4) * n, ctx->stream));
+    ctx->cur = 0;
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_create(const lbfs_mesh_desc *mesh, const lbfs_params *params, lbfs_ctx **out)
+{
+    g_create_error.clear();
+    if (!out) return LBFS_E_INVALID_ARG;
+    *out = nullptr;
+    if (!mesh || !params) { g_create_error = "null mesh or params"; return LBFS_E_INVALID_ARG; }
+    if (params->lattice < 0 || params->lattice > 2) { g_create_error = "unknown lattice"; return LBFS_E_INVALID_ARG; }
+    if (!(params->gamma > 0.0) || !(params->visc >= 0.0) || !(params->dt > 0.0)) {
+        g_create_error = "gamma and dt must be positive, visc non-negative";
+        return LBFS_E_INVALID_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        g_create_error = "no HIP device available";
+        return LBFS_E_NO_DEVICE;
+    }
+    lbfs_ctx *ctx = new lbfs_ctx();
+    ctx->params = *params;
+    ctx->mesh_nx = mesh->nx; ctx->mesh_ny = mesh->ny; ctx->mesh_nz = mesh->nz;
+    for (int t = 0; t < 3; t++) ctx->mesh_origin[t] = mesh->origin[t];
+    ctx->mesh_spacing = mesh->spacing;
+    if (hipGetDevice(&ctx->device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_create_error = "could not create a HIP stream";
+        delete ctx;
+        return LBFS_E_HIP;
+    }
+    int rc = fluid_setup(ctx, *mesh, *params);
+    if (rc) {
+        g_create_error = ctx->err;
+        lbfs_destroy(ctx);
+        return rc;
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        g_create_error = "stream synchronisation failed during set-up";
+        lbfs_destroy(ctx);
+        return LBFS_E_HIP;
+    }
+    *out = ctx;
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_destroy(lbfs_ctx *ctx)
+{
+    if (!ctx) return LBFS_E_INVALID_ARG;
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    ctx->release_extensions();
+    for (void *p : ctx->allocs) hipFree(p);
+    ctx->allocs.clear();
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_sync(lbfs_ctx *ctx)
+{
+    if (!ctx) return LBFS_E_INVALID_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_set_state(lbfs_ctx *ctx, const double *rho, const double *u, const double *v, const double *w)
+{
+    if (!ctx || !rho || !u || !v || !w) return LBFS_E_INVALID_ARG;
+    const int n = ctx->lay.n_cells;
+    std::vector<float4> h(n);
+    for (int i = 0; i < n; i++) {
+        const int c = ctx->lay.perm[i];
+        if (!std::isfinite(rho[c]) || !std::isfinite(u[c]) || !std::isfinite(v[c]) || !std::isfinite(w[c]))
+            return ctx->fail(LBFS_E_NONFINITE, "non-finite state at cell %d", c);
+        h[i] = make_float4((float)(rho[c] - ctx->params.rho0), (float)u[c], (float)v[c], (float)w[c]);
+    }
+    HIPCHK(hipMemcpyAsync(ctx->st[ctx->cur], h.data(), sizeof(float4) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_get_state(lbfs_ctx *ctx, double *rho, double *u, double *v, double *w)
+{
+    if (!ctx || !rho || !u || !v || !w) return LBFS_E_INVALID_ARG;
+    const int n = ctx->lay.n_cells;
+    std::vector<float4> h(n);
+    HIPCHK(hipMemcpyAsync(h.data(), ctx->st[ctx->cur], sizeof(float4) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; i++) {
+        const int c = ctx->lay.perm[i];
+        rho[c] = ctx->params.rho0 + (double)h[i].x;
+        u[c] = h[i].y; v[c] = h[i].z; w[c] = h[i].w;
+    }
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_step(lbfs_ctx *ctx, int nsteps)
+{
+    if (!ctx || nsteps < 0) return LBFS_E_INVALID_ARG;
+    for (int s = 0; s < nsteps; s++) {
+        int rc = ctx->enqueue_step();
+        if (rc) return rc;
+    }
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_stage_gradients(lbfs_ctx *ctx, double *grad_out)
+{
+    if (!ctx) return LBFS_E_INVALID_ARG;
+    HIPCHK(launch_gradients(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->stream));
+    if (!grad_out) return LBFS_OK;
+    const int n = ctx->lay.n_cells;
+    std::vector<float> h(12 * (size_t)n);
+    HIPCHK(hipMemcpyAsync(h.data(), ctx->grad, sizeof(float) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; i++) {
+        const int c = ctx->lay.perm[i];
+        for (int t = 0; t < 12; t++) grad_out[12 * (size_t)c + t] = h[12 * (size_t)i + t];
+    }
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_stage_face_flux(lbfs_ctx *ctx, double *flux_out)
+{
+    if (!ctx || !flux_out) return LBFS_E_INVALID_ARG;
+    const int ntf = (int)ctx->lay.face_ref.size();
+    HIPCHK(launch_gradients(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->stream));
+    HIPCHK(launch_face_flux_all(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->face_scratch, ntf, ctx->stream));
+    std::vector<float4> h(ntf);
+    HIPCHK(hipMemcpyAsync(h.data(), ctx->face_scratch, sizeof(float4) * ntf, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // the device flux leaves out the rest pressure rho0/3 n A (it cancels per
+    // cell); it is added back here so the result is the reference face flux
+    const double p0 = ctx->params.rho0 / 3.0;
+    for (int f = 0; f < ctx->geo.n_faces; f++) {
+        const float4 r = h[ctx->lay.face_first[f]];
+        const double A = ctx->geo.area[f];
+        const double *n = &ctx->geo.normal[3 * (size_t)f];
+        flux_out[4 * (size_t)f] = r.x;
+        flux_out[4 * (size_t)f + 1] = r.y + p0 * n[0] * A;
+        flux_out[4 * (size_t)f + 2] = r.z + p0 * n[1] * A;
+        flux_out[4 * (size_t)f + 3] = r.w + p0 * n[2] * A;
+    }
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_stage_residual(lbfs_ctx *ctx, double *res_out)
+{
+    if (!ctx || !res_out) return LBFS_E_INVALID_ARG;
+    const int n = ctx->lay.n_cells;
+    HIPCHK(launch_gradients(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->stream));
+    HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, nullptr, ctx->res, 0, ctx->stream));
+    std::vector<float4> h(n);
+    HIPCHK(hipMemcpyAsync(h.data(), ctx->res, sizeof(float4) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // add back the rest-pressure closure term -rho0/3 sum_f n A / V (zero on closed cells)
+    const Geometry &g = ctx->geo;
+    std::vector<double> S(3 * (size_t)n, 0.0);
+    for (int f = 0; f < g.n_faces; f++)
+        for (int t = 0; t < 3; t++) {
+            const double v = g.normal[3 * (size_t)f + t] * g.area[f];
+            S[3 * (size_t)g.owner[f] + t] += v;
+            if (g.nbr[f] < n) S[3 * (size_t)g.nbr[f] + t] -= v;
+        }
+    const double p0 = ctx->params.rho0 / 3.0;
+    for (int i = 0; i < n; i++) {
+        const int c = ctx->lay.perm[i];
+        res_out[4 * (size_t)c] = h[i].x;
+        res_out[4 * (size_t)c + 1] = h[i].y - p0 * S[3 * (size_t)c] / g.vol[c];
+        res_out[4 * (size_t)c + 2] = h[i].z - p0 * S[3 * (size_t)c + 1] / g.vol[c];
+        res_out[4 * (size_t)c + 3] = h[i].w - p0 * S[3 * (size_t)c + 2] / g.vol[c];
+    }
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_get_cell_force(lbfs_ctx *ctx, double *f_out)
+{
+    if (!ctx || !f_out) return LBFS_E_INVALID_ARG;
+    const int n = ctx->lay.n_cells;
+    std::vector<float> cf(3 * (size_t)n), sf(3 * (size_t)n);
+    HIPCHK(hipMemcpyAsync(cf.data(), ctx->cell_force, sizeof(float) * cf.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(sf.data(), ctx->static_force, sizeof(float) * sf.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // body force plus the spread IBM force (the closure term is not a force of the reference)
+    for (int i = 0; i < n; i++) {
+        const int c = ctx->lay.perm[i];
+        for (int t = 0; t < 3; t++)
+            f_out[3 * (size_t)c + t] = ctx->params.body_force[t] + ((double)cf[3 * (size_t)i + t] - (double)sf[3 * (size_t)i + t]);
+    }
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_reduce_diagnostics(lbfs_ctx *ctx, double ref_rho, double ref_u, lbfs_diag *out)
+{
+    if (!ctx || !out || !(ref_rho > 0.0) || !(ref_u > 0.0)) return LBFS_E_INVALID_ARG;
+    const int n = ctx->lay.n_cells, nb = (n + 255) / 256;
+    HIPCHK(launch_gradients(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->stream));
+    HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, nullptr, ctx->res, 0, ctx->stream));
+    HIPCHK(launch_diag_partials(n, ctx->args.rho0, ctx->st[ctx->cur], ctx->res, ctx->args.inv_vol, ctx->diag_part,
+                                ctx->stream));
+    std::vector<double> h(6 * (size_t)nb);
+    HIPCHK(hipMemcpyAsync(h.data(), ctx->diag_part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double s[5] = {0, 0, 0, 0, 0}, mx = 0.0;
+    for (int b = 0; b < nb; b++) {
+        for (int t = 0; t < 5; t++) s[t] += h[6 * (size_t)b + t];
+        mx = std::fmax(mx, h[6 * (size_t)b + 5]);
+    }
+    out->rho_rms = std::sqrt(s[0] / n) / ref_rho;
+    out->u_rms = std::sqrt(s[1] / n) / (ref_rho * ref_u);
+    out->v_rms = std::sqrt(s[2] / n) / (ref_rho * ref_u);
+    out->w_rms = std::sqrt(s[3] / n) / (ref_rho * ref_u);
+    out->total_mass = s[4];
+    out->max_speed = mx;
+    out->finite = std::isfinite(mx) && std::isfinite(s[0]) && std::isfinite(s[1]) ? 1 : 0;
+    return out->finite ? LBFS_OK : ctx->fail(LBFS_E_NONFINITE, "non-finite fluid state");
+}
+
+// ---------------------------------------------------------------------------
+// host-side mesh helpers
+// ---------------------------------------------------------------------------
+extern "C" int lbfs_mesh_hex_counts(int nx, int ny, int nz, const int *periodic, int *n_points, int *n_faces,
+                                    int *n_internal)
+{
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !periodic || !n_points || !n_faces || !n_internal) return LBFS_E_INVALID_ARG;
+    const HexCounts c = hex_counts(nx, ny, nz, periodic);
+    *n_points = c.n_points; *n_faces = c.n_faces; *n_internal = c.n_internal;
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_mesh_hex_generate(int nx, int ny, int nz, const double *lengths, const double *origin,
+                                      const int *periodic, double jitter, uint64_t seed, double *points,
+                                      int *face_vertices, int *owner, int *neighbour, double *face_shift, int *patch)
+{
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !lengths || !origin || !periodic || !points || !face_vertices || !owner ||
+        !neighbour || !face_shift || !patch || jitter < 0.0 || jitter >= 0.5)
+        return LBFS_E_INVALID_ARG;
+    hex_generate(nx, ny, nz, lengths, origin, periodic, jitter, seed, points, face_vertices, owner, neighbour, face_shift,
+                 patch);
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_mesh_geometry(const lbfs_mesh_desc *mesh, double visc, double gamma, double fneq_min, int flags,
+                                  double *area, double *normal, double *face_centroid, double *face_dt,
+                                  double *centroid, double *volume)
+{
+    if (!mesh || !area || !normal || !face_centroid || !face_dt || !centroid || !volume) return LBFS_E_INVALID_ARG;
+    Geometry g;
+    std::string err;
+    int rc = compute_geometry(*mesh, visc, gamma, fneq_min, (flags & LBFS_FLAG_CLOSED_FACES) != 0, g, err);
+    if (rc) { g_create_error = err; return rc; }
+    std::copy(g.area.begin(), g.area.end(), area);
+    std::copy(g.normal.begin(), g.normal.end(), normal);
+    std::copy(g.fc.begin(), g.fc.end(), face_centroid);
+    std::copy(g.fdt.begin(), g.fdt.end(), face_dt);
+    std::copy(g.cc.begin(), g.cc.end(), centroid);
+    std::copy(g.vol.begin(), g.vol.end(), volume);
+    return LBFS_OK;
+}

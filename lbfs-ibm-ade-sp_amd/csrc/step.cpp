@@ -1,0 +1,112 @@
+// Time-step orchestration behind lbfs_step: the loop body of
+// Solver_gpu.cu:749-1246 as an ordered list of kernel stages, plus the
+// event-based measurement entry points.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "context.hpp"
+
+using namespace lbfs;
+
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return ctx->fail(LBFS_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages()
+{
+    std::vector<Stage> s;
+    lbfs_ctx *ctx = this;
+    const double n = lay.n_cells, nf = geo.n_faces;
+    // pass G: state 16 + stencil 24 + RHS 72 + LS matrix 36 + gradients written 48 (B per cell)
+    s.push_back({"gradients", [ctx]() -> int {
+                     HIPCHK(launch_gradients(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->stream));
+                     return LBFS_OK;
+                 },
+                 196.0 * n});
+    // pass F: state 16 + gradients 48 + volume/centroid 16 + force 12 + state written 16 per cell,
+    // plus owner/nb 8, normal 12, face centroid 12, area 4, dt 4 per face
+    s.push_back({"flux_update", [ctx]() -> int {
+                     HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->st[ctx->cur ^ 1], nullptr,
+                                               1, ctx->stream));
+                     ctx->cur ^= 1;
+                     return LBFS_OK;
+                 },
+                 108.0 * n + 40.0 * nf});
+    return s;
+}
+
+int lbfs_ctx::enqueue_step()
+{
+    for (auto &st : step_stages()) {
+        int rc = st.run();
+        if (rc) return rc;
+    }
+    return LBFS_OK;
+}
+
+void lbfs_ctx::release_extensions() {}
+
+extern "C" int lbfs_time_steps(lbfs_ctx *ctx, int nsteps, double *ms_total)
+{
+    if (!ctx || nsteps <= 0 || !ms_total) return LBFS_E_INVALID_ARG;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, ctx->stream));
+    for (int s = 0; s < nsteps; s++) {
+        int rc = ctx->enqueue_step();
+        if (rc) return rc;
+    }
+    HIPCHK(hipEventRecord(b, ctx->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    *ms_total = ms;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_time_kernels(lbfs_ctx *ctx, int nsteps, int max_count, const char **names, double *ms, int *count)
+{
+    if (!ctx || nsteps <= 0 || !ms || !count || max_count <= 0) return LBFS_E_INVALID_ARG;
+    std::vector<lbfs_ctx::Stage> stages = ctx->step_stages();
+    const int ns = (int)stages.size();
+    std::vector<hipEvent_t> ev((size_t)nsteps * (ns + 1));
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    for (int s = 0; s < nsteps; s++) {
+        for (int k = 0; k < ns; k++) {
+            HIPCHK(hipEventRecord(ev[(size_t)s * (ns + 1) + k], ctx->stream));
+            int rc = stages[k].run();
+            if (rc) return rc;
+        }
+        HIPCHK(hipEventRecord(ev[(size_t)s * (ns + 1) + ns], ctx->stream));
+    }
+    HIPCHK(hipEventSynchronize(ev.back()));
+    std::vector<double> acc(ns, 0.0);
+    for (int s = 0; s < nsteps; s++)
+        for (int k = 0; k < ns; k++) {
+            float t = 0.0f;
+            HIPCHK(hipEventElapsedTime(&t, ev[(size_t)s * (ns + 1) + k], ev[(size_t)s * (ns + 1) + k + 1]));
+            acc[k] += t;
+        }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    *count = ns < max_count ? ns : max_count;
+    for (int k = 0; k < *count; k++) {
+        ms[k] = acc[k] / nsteps;
+        if (names) names[k] = stages[k].name;
+    }
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_model_bytes(lbfs_ctx *ctx, int max_count, double *bytes, int *count)
+{
+    if (!ctx || !bytes || !count || max_count <= 0) return LBFS_E_INVALID_ARG;
+    std::vector<lbfs_ctx::Stage> stages = ctx->step_stages();
+    *count = (int)stages.size() < max_count ? (int)stages.size() : max_count;
+    for (int k = 0; k < *count; k++) bytes[k] = stages[k].model_bytes;
+    return LBFS_OK;
+}

@@ -1,0 +1,50 @@
+"""Seeded synthetic workloads of the benchmark configurations.
+
+  channel_rbc     128x64x64 jittered hex channel (+-10 %, seed 2), x-periodic,
+                  no-slip walls in y and z, D3Q19 FP32, body-force Poiseuille
+                  flow, one biconcave RBC of 2562 nodes / 5120 triangles
+  tgv_2d          64x64 periodic quads (+-5 %, seed 1), one z layer, D2Q9,
+                  Taylor-Green vortex at Re = 100, U0 = 0.01
+"""
+import numpy as np
+
+from . import abi
+
+# duct flow: nu = 1/6 (relaxation factor 1 at dt_f = 0.5), dt = 0.25
+CHANNEL_VISC = 1.0 / 6.0
+CHANNEL_DT = 0.25
+
+
+def duct_body_force(u_max, ny, nz, visc):
+    """Body force giving centre-line velocity u_max in a rectangular duct of
+    ny x nz cells (series solution of the Poiseuille problem, 50 odd terms)."""
+    a, b = ny / 2.0, nz / 2.0
+    s = 0.0
+    for n in range(1, 100, 2):
+        s += (-1) ** ((n - 1) // 2) / n ** 3 * (1.0 - 1.0 / np.cosh(n * np.pi * b / (2 * a)))
+    # u_c = 16 a^2 G / (mu pi^3) * s  with G the pressure gradient per unit volume
+    return u_max * visc * np.pi ** 3 / (16.0 * a * a * s)
+
+
+def channel(nx=128, ny=64, nz=64, jitter=0.1, seed=2, lattice=abi.D3Q19, u_max=0.01, flags=0):
+    mesh = abi.HexMesh(nx, ny, nz, periodic=(1, 0, 0), jitter=jitter, seed=seed)
+    f = duct_body_force(u_max, ny, nz, CHANNEL_VISC)
+    params = abi.make_params(lattice=lattice, visc=CHANNEL_VISC, dt=CHANNEL_DT, rho0=1.0, flags=flags,
+                             body_force=(f, 0.0, 0.0))
+    n = mesh.n_cells
+    state = (np.ones(n), np.zeros(n), np.zeros(n), np.zeros(n))
+    return mesh, params, state
+
+
+def tgv_2d(n=64, jitter=0.05, seed=1, u0=0.01, re=100.0, lattice=abi.D2Q9, dt=0.25):
+    mesh = abi.HexMesh(n, n, 1, periodic=(1, 1, 1), jitter=jitter, seed=seed)
+    visc = u0 * n / re
+    params = abi.make_params(lattice=lattice, visc=visc, dt=dt, rho0=1.0)
+    g = mesh.geometry()
+    c = g["centroid"][: 3 * mesh.n_cells].reshape(-1, 3)
+    k = 2 * np.pi / n
+    x, y = c[:, 0], c[:, 1]
+    u = -u0 * np.cos(k * x) * np.sin(k * y)
+    v = u0 * np.sin(k * x) * np.cos(k * y)
+    rho = 1.0 - 0.75 * u0 * u0 * (np.cos(2 * k * x) + np.cos(2 * k * y))
+    return mesh, params, (rho, u, v, np.zeros_like(u))

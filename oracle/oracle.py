@@ -1,0 +1,191 @@
+"""ctypes wrapper of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the CPU
+baseline leg of bench.py as the checker.  The product library never imports
+or calls anything here.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+OR_D3Q15, OR_D3Q19, OR_D2Q9 = 0, 1, 2
+FLAG_UPWIND_BAND, FLAG_NEUMANN_SIGNED, FLAG_GHOST_DIR_GPU, FLAG_LITERAL_D3Q15 = 1, 2, 4, 8
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class OrMesh(C.Structure):
+    _fields_ = [
+        ("n_cells", C.c_int), ("n_faces", C.c_int), ("n_int", C.c_int), ("n_bc", C.c_int),
+        ("owner", _ip), ("nbr", _ip), ("shift", _dp), ("area", _dp), ("normal", _dp), ("fc", _dp),
+        ("fdt", _dp), ("cc", _dp), ("vol", _dp), ("gcell", _ip), ("gface", _ip), ("grole", _ip),
+        ("lhs", _dp), ("rhs", _dp), ("bc_rho_type", _ip), ("bc_vel_type", _ip), ("bc_val", _dp),
+        ("visc_factor", _dp),
+    ]
+
+
+class OrParams(C.Structure):
+    _fields_ = [("lattice", C.c_int), ("flags", C.c_int), ("visc", C.c_double), ("gamma", C.c_double),
+                ("dt", C.c_double)]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run oracle/build.py")
+        lib = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        lib.or_geometry.restype = C.c_int
+        lib.or_geometry.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _ip, _ip, _ip, _dp, C.c_double, C.c_double,
+                                    C.c_double, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _ip]
+        lib.or_ls_setup.restype = C.c_int
+        lib.or_ls_setup.argtypes = [C.POINTER(OrMesh), C.c_int, C.c_int, _dp, _dp]
+        lib.or_lattice.restype = C.c_int
+        lib.or_lattice.argtypes = [C.c_int, _ip, _dp]
+        lib.or_feq.restype = None
+        lib.or_feq.argtypes = [C.c_int, C.c_double, _dp, C.c_double, _dp]
+        om, op = C.POINTER(OrMesh), C.POINTER(OrParams)
+        lib.or_bc_update.argtypes = [om, _dp]
+        lib.or_gradients.argtypes = [om, op, _dp, _dp]
+        lib.or_face_flux.argtypes = [om, op, _dp, _dp, _dp]
+        lib.or_residual.argtypes = [om, _dp, _dp]
+        lib.or_update.argtypes = [om, op, _dp, _dp, _dp, _dp]
+        lib.or_fluid_step.restype = C.c_int
+        lib.or_fluid_step.argtypes = [om, op, _dp, _dp, _dp, _dp, _dp, _dp]
+        lib.or_residual_rms.argtypes = [C.c_int, _dp, C.c_double, C.c_double, _dp]
+        for name in ("or_bc_update", "or_gradients", "or_face_flux", "or_residual", "or_update", "or_residual_rms"):
+            getattr(lib, name).restype = None
+        _lib = lib
+    return _lib
+
+
+def ptr(a, kind=_dp):
+    if a is None:
+        return None
+    return a.ctypes.data_as(kind)
+
+
+def lattice(kind):
+    e = np.zeros(81, dtype=np.int32)
+    w = np.zeros(27)
+    q = load().or_lattice(kind, ptr(e, _ip), ptr(w))
+    return e[: 3 * q].reshape(q, 3).copy(), w[:q].copy()
+
+
+def feq(kind, rho, u, gamma=1.0):
+    out = np.zeros(27)
+    uu = np.ascontiguousarray(u, dtype=np.float64)
+    load().or_feq(kind, float(rho), ptr(uu), float(gamma), ptr(out))
+    q = {OR_D3Q15: 15, OR_D3Q19: 19, OR_D2Q9: 9}[kind]
+    return out[:q].copy()
+
+
+class OracleFluid:
+    """FP64 restatement of the reference fluid step on a polyMesh-form mesh.
+
+    `mesh` is any object with the polyMesh arrays of lbfs_amd.abi.HexMesh
+    (points, face_vertices, owner, neighbour, face_shift, bc_* arrays).
+    """
+
+    def __init__(self, mesh, lattice_kind, visc, dt, gamma=1.0, fneq_min=0.0, flags=0, grad_type=0,
+                 det_compat=False, visc_factor=None, closed=False):
+        lib = load()
+        self.mesh = mesh
+        nc, nf, ni = mesh.n_cells, mesh.n_faces, mesh.n_internal
+        nb = nf - ni
+        self.nc, self.nf, self.ni, self.nb = nc, nf, ni, nb
+        self.area = np.zeros(nf)
+        self.normal = np.zeros(3 * nf)
+        self.fc = np.zeros(3 * nf)
+        self.fdt = np.zeros(nf)
+        self.cc = np.zeros(3 * (nc + nb))
+        self.vol = np.zeros(nc)
+        self.gcell = np.zeros(6 * nc, dtype=np.int32)
+        self.gface = np.zeros(6 * nc, dtype=np.int32)
+        self.grole = np.zeros(6 * nc, dtype=np.int32)
+        self.nbr = np.zeros(nf, dtype=np.int32)
+        self.shift = np.ascontiguousarray(mesh.face_shift, dtype=np.float64)
+        rc = lib.or_geometry(nc, nf, ni, ptr(mesh.points), ptr(mesh.face_vertices, _ip), ptr(mesh.owner, _ip),
+                             ptr(mesh.neighbour, _ip), ptr(self.shift), visc, gamma, fneq_min, 1 if closed else 0, ptr(self.area),
+                             ptr(self.normal), ptr(self.fc), ptr(self.fdt), ptr(self.cc), ptr(self.vol),
+                             ptr(self.gcell, _ip), ptr(self.gface, _ip), ptr(self.grole, _ip), ptr(self.nbr, _ip))
+        if rc != 0:
+            raise RuntimeError("or_geometry failed: %d" % rc)
+        self.owner = np.ascontiguousarray(mesh.owner, dtype=np.int32)
+        self.lhs = np.zeros(9 * nc)
+        self.rhs = np.zeros(18 * nc)
+        self.bc_rho_type = np.ascontiguousarray(mesh.bc_rho_type, dtype=np.int32)
+        self.bc_vel_type = np.ascontiguousarray(mesh.bc_vel_type, dtype=np.int32)
+        self.bc_val = np.ascontiguousarray(mesh.bc_values, dtype=np.float64)
+        self.visc_factor = None if visc_factor is None else np.ascontiguousarray(visc_factor, dtype=np.float64)
+        self.m = OrMesh()
+        m = self.m
+        m.n_cells, m.n_faces, m.n_int, m.n_bc = nc, nf, ni, nb
+        m.owner, m.nbr, m.shift = ptr(self.owner, _ip), ptr(self.nbr, _ip), ptr(self.shift)
+        m.area, m.normal, m.fc, m.fdt = ptr(self.area), ptr(self.normal), ptr(self.fc), ptr(self.fdt)
+        m.cc, m.vol = ptr(self.cc), ptr(self.vol)
+        m.gcell, m.gface, m.grole = ptr(self.gcell, _ip), ptr(self.gface, _ip), ptr(self.grole, _ip)
+        m.lhs, m.rhs = ptr(self.lhs), ptr(self.rhs)
+        m.bc_rho_type, m.bc_vel_type, m.bc_val = ptr(self.bc_rho_type, _ip), ptr(self.bc_vel_type, _ip), ptr(self.bc_val)
+        m.visc_factor = ptr(self.visc_factor)
+        rc = lib.or_ls_setup(C.byref(m), grad_type, 1 if det_compat else 0, ptr(self.lhs), ptr(self.rhs))
+        if rc != 0:
+            raise RuntimeError("or_ls_setup failed")
+        self.p = OrParams(lattice_kind, flags, visc, gamma, dt)
+        self.st = np.zeros(4 * (nc + nb))
+        self.cons = np.zeros(4 * nc)
+        self.g = np.zeros(12 * (nc + nb))
+        self.flux = np.zeros(4 * nf)
+        self.res = np.zeros(4 * nc)
+
+    def set_state(self, rho, u, v, w):
+        """Primitive state; the conserved buffer is (rho, rho u) (momentum
+        initialised consistently, unlike Solver.cpp:195 for rho0 != 1)."""
+        nc = self.nc
+        self.st[0:4 * nc:4], self.st[1:4 * nc:4], self.st[2:4 * nc:4], self.st[3:4 * nc:4] = rho, u, v, w
+        self.cons[0::4] = rho
+        self.cons[1::4] = np.asarray(rho) * u
+        self.cons[2::4] = np.asarray(rho) * v
+        self.cons[3::4] = np.asarray(rho) * w
+
+    def state(self):
+        nc = self.nc
+        s = self.st[: 4 * nc]
+        return s[0::4].copy(), s[1::4].copy(), s[2::4].copy(), s[3::4].copy()
+
+    def gradients(self):
+        lib = load()
+        lib.or_bc_update(C.byref(self.m), ptr(self.st))
+        lib.or_gradients(C.byref(self.m), C.byref(self.p), ptr(self.st), ptr(self.g))
+        return self.g[: 12 * self.nc].copy()
+
+    def face_flux(self):
+        self.gradients()
+        load().or_face_flux(C.byref(self.m), C.byref(self.p), ptr(self.st), ptr(self.g), ptr(self.flux))
+        return self.flux.copy()
+
+    def residual(self):
+        self.face_flux()
+        load().or_residual(C.byref(self.m), ptr(self.flux), ptr(self.res))
+        return self.res.copy()
+
+    def step(self, n=1, force=None):
+        lib = load()
+        fp = None
+        if force is not None:
+            force = np.ascontiguousarray(force, dtype=np.float64)
+            fp = ptr(force)
+        for _ in range(n):
+            rc = lib.or_fluid_step(C.byref(self.m), C.byref(self.p), ptr(self.st), ptr(self.cons), fp, ptr(self.g),
+                                   ptr(self.flux), ptr(self.res))
+            if rc != 0:
+                raise FloatingPointError("oracle state became non-finite")

@@ -1,0 +1,74 @@
+"""CPU-side checks of the C ABI library: it loads, exports every symbol the
+headers declare, and its host-only mesh helpers behave (no device work)."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cases
+from lbfs_amd import abi
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(REPO, "include", "*.h")):
+        src = open(h).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        for m in re.finditer(r"\b(lbfs_[a-z0-9_]+)\s*\(", src):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_declared_symbols():
+    lib = abi.load()
+    names = declared_symbols()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_create_without_device_fails_cleanly(has_gpu):
+    if has_gpu:
+        pytest.skip("a device is present")
+    m = cases.channel_mesh(4, 4, 4)
+    with pytest.raises(abi.LbfsError, match="no HIP device"):
+        abi.Solver(m, abi.make_params())
+
+
+@pytest.mark.parametrize("periodic", [(0, 0, 0), (1, 0, 0), (1, 1, 1)])
+def test_hex_counts_and_topology(periodic):
+    nx, ny, nz = 5, 4, 3
+    m = abi.HexMesh(nx, ny, nz, periodic=periodic, jitter=0.1, seed=9)
+    # every cell has exactly six faces
+    cnt = np.zeros(m.n_cells, dtype=int)
+    np.add.at(cnt, m.owner, 1)
+    np.add.at(cnt, m.neighbour[: m.n_internal], 1)
+    assert np.all(cnt == 6)
+    # periodic wraps carry the box length, other faces no shift
+    sh = m.face_shift.reshape(-1, 3)
+    for a in range(3):
+        n_wrap = np.sum(np.abs(sh[:, a] - m.lengths[a]) < 1e-12)
+        assert n_wrap == (m.n_cells // [nx, ny, nz][a] if periodic[a] else 0)
+    # boundary patches only on non-periodic axes
+    pats = set(m.bc_patch[: m.n_bc].tolist()) if m.n_bc else set()
+    for a in range(3):
+        assert ((2 * a) in pats) == (not periodic[a])
+
+
+def test_jitter_is_seeded_and_bounded():
+    a = abi.HexMesh(6, 5, 4, jitter=0.1, seed=3)
+    b = abi.HexMesh(6, 5, 4, jitter=0.1, seed=3)
+    c = abi.HexMesh(6, 5, 4, jitter=0.1, seed=4)
+    assert np.array_equal(a.points, b.points)
+    assert not np.array_equal(a.points, c.points)
+    p = a.points.reshape(-1, 3)
+    ijk = np.stack(np.meshgrid(np.arange(7), np.arange(6), np.arange(5), indexing="ij"), -1)
+    ijk = ijk.transpose(2, 1, 0, 3).reshape(-1, 3)
+    assert np.abs(p - ijk).max() <= 0.1 + 1e-6
+    # FP32-representable coordinates
+    assert np.array_equal(p.astype(np.float32).astype(np.float64), p)

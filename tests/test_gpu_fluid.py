@@ -1,0 +1,144 @@
+"""GPU parity of the fluid path: HIP kernels (FP32, through the C ABI) against
+the FP64 oracle on the same seeded meshes and states.
+
+Tolerances (stated per test) are relative L2 norms; the north-star bar for
+the full step is 1e-5 on velocity after N steps.
+"""
+import numpy as np
+import pytest
+
+import cases
+from lbfs_amd import abi
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+VISC, DT = 1.0 / 6.0, 0.25
+
+
+def pair(mesh, lat_a, lat_o, flags=0, visc=VISC, dt=DT, body=(0.0, 0.0, 0.0), grad_type=0):
+    closed = bool(flags & abi.FLAG_CLOSED_FACES)
+    det = bool(flags & abi.FLAG_LS_DET_COMPAT)
+    oflags = flags & (abi.FLAG_UPWIND_BAND | abi.FLAG_NEUMANN_SIGNED | abi.FLAG_GHOST_DIR_GPU)
+    s = abi.Solver(mesh, abi.make_params(lattice=lat_a, visc=visc, dt=dt, flags=flags, body_force=body,
+                                         gradient_type=grad_type))
+    o = orc.OracleFluid(mesh, lat_o, visc=visc, dt=dt, flags=oflags, closed=closed, det_compat=det,
+                        grad_type=grad_type)
+    return s, o
+
+
+def test_gradients_parity():
+    m = cases.channel_mesh(16, 12, 10)
+    s, o = pair(m, abi.D3Q19, orc.OR_D3Q19)
+    st = cases.smooth_state(m, seed=11)
+    s.set_state(*st)
+    o.set_state(*st)
+    gs, go = s.gradients(), o.gradients()
+    g_s, g_o = gs.reshape(-1, 4, 3), go.reshape(-1, 4, 3)
+    for v in range(4):
+        assert cases.rel_l2(g_s[:, v], g_o[:, v]) < 2e-5, v
+
+
+@pytest.mark.parametrize("lat_a,lat_o", cases.LAT_PAIRS)
+@pytest.mark.parametrize("flags", [0, abi.FLAG_UPWIND_BAND | abi.FLAG_GHOST_DIR_GPU, abi.FLAG_NEUMANN_SIGNED])
+def test_face_flux_parity(lat_a, lat_o, flags):
+    m = cases.channel_mesh(12, 10, 8)
+    s, o = pair(m, lat_a, lat_o, flags=flags)
+    st = cases.smooth_state(m, seed=12)
+    s.set_state(*st)
+    o.set_state(*st)
+    fs, fo = s.face_flux().reshape(-1, 4), o.face_flux().reshape(-1, 4)
+    # mass flux compared to its own scale; momentum flux carries rho0/3 n A
+    assert cases.rel_l2(fs[:, 0], fo[:, 0]) < 1e-5
+    for v in range(1, 4):
+        d = fs[:, v] - fo[:, v]
+        assert np.sqrt(np.mean(d * d)) < 2e-7, (v, np.abs(d).max())
+
+
+@pytest.mark.parametrize("closed", [False, True])
+def test_residual_parity(closed):
+    m = cases.channel_mesh(12, 10, 8)
+    s, o = pair(m, abi.D3Q19, orc.OR_D3Q19, flags=abi.FLAG_CLOSED_FACES if closed else 0)
+    st = cases.smooth_state(m, seed=13)
+    s.set_state(*st)
+    o.set_state(*st)
+    rs, ro = s.residual().reshape(-1, 4), o.residual().reshape(-1, 4)
+    for v in range(4):
+        assert cases.rel_l2(rs[:, v], ro[:, v]) < 1e-4, v
+
+
+@pytest.mark.parametrize("lat_a,lat_o", cases.LAT_PAIRS)
+@pytest.mark.parametrize("flags", [0, abi.FLAG_CLOSED_FACES | abi.FLAG_UPWIND_BAND, abi.FLAG_LS_DET_COMPAT])
+def test_channel_step_parity(lat_a, lat_o, flags):
+    """Body-force channel flow from a perturbed state, 100 steps: relative L2
+    of (u, v, w) <= 1e-5 and of rho - rho0 <= 1e-4 against FP64."""
+    m = cases.channel_mesh(16, 12, 10)
+    body = (2e-5, 0.0, 0.0)
+    s, o = pair(m, lat_a, lat_o, flags=flags, body=body)
+    st = cases.smooth_state(m, seed=14)
+    s.set_state(*st)
+    o.set_state(*st)
+    n = m.n_cells
+    force = np.tile(np.array(body), n)
+    steps = 100
+    s.step(steps)
+    o.step(steps, force=force)
+    rs, us, vs, ws = s.get_state()
+    ro, uo, vo, wo = o.state()
+    vel_err = cases.rel_l2(np.concatenate([us, vs, ws]), np.concatenate([uo, vo, wo]))
+    rho_err = cases.rel_l2(rs - 1.0, ro - 1.0)
+    assert vel_err < 1e-5, vel_err
+    assert rho_err < 1e-4, rho_err
+
+
+def test_taylor_green_2d_parity_and_decay():
+    """Config-1 style: D2Q9 on a jittered 2D periodic mesh, 400 steps."""
+    N = 32
+    m = abi.HexMesh(N, N, 1, periodic=(1, 1, 1), jitter=0.05, seed=1)
+    nu = 0.01 * N / 100.0
+    s, o = pair(m, abi.D2Q9, orc.OR_D2Q9, visc=nu)
+    st = cases.tgv_state(m, 0.01)
+    s.set_state(*st)
+    o.set_state(*st)
+    s.step(400)
+    o.step(400)
+    rs, us, vs, _ = s.get_state()
+    ro, uo, vo, _ = o.state()
+    assert cases.rel_l2(np.concatenate([us, vs]), np.concatenate([uo, vo])) < 1e-5
+    k = 2 * np.pi / N
+    decay = np.exp(-2 * nu * k * k * DT * 400)
+    u0 = st[1]
+    assert abs(np.dot(us, u0) / np.dot(u0, u0) / decay - 1.0) < 0.02
+
+
+def test_bitwise_repeatable():
+    m = cases.channel_mesh(16, 12, 10)
+    outs = []
+    for _ in range(2):
+        s = abi.Solver(m, abi.make_params(lattice=abi.D3Q19, visc=VISC, dt=DT, body_force=(1e-5, 0, 0)))
+        s.set_state(*cases.smooth_state(m, seed=15))
+        s.step(20)
+        outs.append(np.concatenate(s.get_state()))
+        s.close()
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_mass_conserved_periodic():
+    m = abi.HexMesh(16, 16, 16, periodic=(1, 1, 1), jitter=0.1, seed=21)
+    s = abi.Solver(m, abi.make_params(lattice=abi.D3Q19, visc=VISC, dt=DT))
+    s.set_state(*cases.smooth_state(m, seed=16))
+    d0 = s.diagnostics()
+    s.step(200)
+    d1 = s.diagnostics()
+    assert d1["finite"] == 1
+    assert abs(d1["total_mass"] - d0["total_mass"]) / d0["total_mass"] < 1e-6
+
+
+def test_invalid_arguments_return_codes():
+    m = cases.channel_mesh(4, 4, 4)
+    with pytest.raises(abi.LbfsError):
+        abi.Solver(m, abi.make_params(lattice=7))
+    s = abi.Solver(m, abi.make_params())
+    bad = np.full(m.n_cells, np.nan)
+    with pytest.raises(abi.LbfsError, match="non-finite"):
+        s.set_state(bad, bad, bad, bad)

@@ -1,0 +1,211 @@
+"""Pins of the FP64 fluid oracle (CPU only).
+
+The reference ships no tests, fixtures or golden vectors for this path, so
+each oracle function is pinned by what the mathematics fixes: equilibrium
+moments, the D3Q15 -> D2Q9 projection, a hand-expanded restatement of the
+reference's D3Q15 formulas, exactness of least squares for linear fields,
+discrete conservation, free-stream preservation and the Taylor-Green decay.
+"""
+import numpy as np
+import pytest
+
+import cases
+from lbfs_amd import abi
+from oracle import oracle as orc
+
+
+@pytest.mark.parametrize("kind", [orc.OR_D3Q15, orc.OR_D3Q19, orc.OR_D2Q9])
+@pytest.mark.parametrize("gamma", [1.0, 0.35])
+def test_equilibrium_moments(kind, gamma):
+    """sum f = rho, sum e f = rho u, sum e e f = rho/3 I + rho u u / gamma."""
+    e, w = orc.lattice(kind)
+    assert abs(w.sum() - 1.0) < 1e-15
+    rng = np.random.default_rng(kind)
+    for _ in range(5):
+        rho = 1.0 + 0.01 * rng.standard_normal()
+        u = 0.05 * rng.standard_normal(3)
+        if kind == orc.OR_D2Q9:
+            u[2] = 0.0
+        f = orc.feq(kind, rho, u, gamma)
+        assert abs(f.sum() - rho) < 1e-14
+        np.testing.assert_allclose(e.T @ f, rho * u, atol=1e-15)
+        M = (e[:, :, None] * e[:, None, :] * f[:, None, None]).sum(0)
+        exp = rho / 3.0 * np.eye(3) + rho * np.outer(u, u) / gamma
+        dims = 2 if kind == orc.OR_D2Q9 else 3
+        np.testing.assert_allclose(M[:dims, :dims], exp[:dims, :dims], atol=1e-15)
+
+
+def test_d3q15_projects_onto_d2q9():
+    """With w = 0, summing D3Q15 equilibria over their z partners gives D2Q9."""
+    e15, _ = orc.lattice(orc.OR_D3Q15)
+    e9, _ = orc.lattice(orc.OR_D2Q9)
+    u = np.array([0.013, -0.021, 0.0])
+    f15 = orc.feq(orc.OR_D3Q15, 1.002, u)
+    f9 = orc.feq(orc.OR_D2Q9, 1.002, u)
+    for k, ek in enumerate(e9):
+        sel = [j for j, ej in enumerate(e15) if ej[0] == ek[0] and ej[1] == ek[1]]
+        assert abs(f15[sel].sum() - f9[k]) < 1e-16
+
+
+def test_geometry_matches_product():
+    """The product's FP64 host geometry equals the oracle's restatement."""
+    for closed in (False, True):
+        m = cases.channel_mesh(6, 5, 4)
+        g = m.geometry(flags=abi.FLAG_CLOSED_FACES if closed else 0)
+        o = orc.OracleFluid(m, orc.OR_D3Q19, visc=1 / 6, dt=0.25, closed=closed)
+        assert np.max(np.abs(g["area"] - o.area)) < 1e-13
+        assert np.max(np.abs(g["normal"] - o.normal)) < 1e-13
+        assert np.max(np.abs(g["face_centroid"] - o.fc)) < 1e-12
+        assert np.max(np.abs(g["centroid"] - o.cc)) < 1e-12
+        assert np.max(np.abs(g["volume"] - o.vol)) < 1e-12
+        assert abs(o.vol.sum() - 6 * 5 * 4) < 1e-9
+
+
+def test_generated_mesh_is_watertight():
+    """Closed-face cells have zero net area vector; volumes sum to the box."""
+    m = abi.HexMesh(5, 4, 3, periodic=(1, 1, 1), jitter=0.1, seed=7)
+    o = orc.OracleFluid(m, orc.OR_D3Q15, visc=0.1, dt=0.25, closed=True)
+    nA = o.normal.reshape(-1, 3) * o.area[:, None]
+    S = np.zeros((m.n_cells, 3))
+    np.add.at(S, o.owner, nA)
+    np.add.at(S, o.nbr[: m.n_internal], -nA[: m.n_internal])
+    assert np.abs(S).max() < 1e-13
+    assert abs(o.vol.sum() - 60.0) < 1e-10
+
+
+def test_reference_face_area_does_not_close():
+    """With the reference's face area (sum of triangle areas) warped faces
+    leave a net area vector per cell (kept on the GPU as a cell force)."""
+    m = cases.channel_mesh(5, 4, 3)
+    o = orc.OracleFluid(m, orc.OR_D3Q15, visc=0.1, dt=0.25, closed=False)
+    nA = o.normal.reshape(-1, 3) * o.area[:, None]
+    S = np.zeros((m.n_cells, 3))
+    np.add.at(S, o.owner, nA)
+    np.add.at(S, o.nbr[: m.n_internal], -nA[: m.n_internal])
+    assert np.abs(S).max() > 1e-4
+
+
+def _linear_field(o, coef):
+    c = o.cc.reshape(-1, 3)
+    return coef[0] + c @ coef[1:]
+
+
+def test_least_squares_exact_for_linear_fields():
+    """Interior cells reproduce the gradient of a linear field exactly
+    (gradients.cpp:214-380 with the determinant fixed)."""
+    m = abi.HexMesh(6, 6, 6, periodic=(0, 0, 0), jitter=0.1, seed=3)
+    o = orc.OracleFluid(m, orc.OR_D3Q15, visc=0.1, dt=0.25, grad_type=1)
+    coefs = [np.array([1.0, 0.3, -0.2, 0.7]), np.array([0.0, 1.0, 2.0, -1.0]),
+             np.array([0.0, -0.5, 0.1, 0.4]), np.array([0.0, 0.2, 0.2, 0.9])]
+    fields = [_linear_field(o, cf) for cf in coefs]
+    nc = m.n_cells
+    o.st[:] = 0.0
+    for v in range(4):
+        o.st[v::4][: nc + o.nb] = fields[v][: nc + o.nb]
+    # interior cells only: ghost values come from the boundary condition
+    import ctypes
+    orc.load().or_gradients(ctypes.byref(o.m), ctypes.byref(o.p), orc.ptr(o.st), orc.ptr(o.g))
+    g = o.g[: 12 * nc].reshape(nc, 4, 3)
+    interior = np.all(o.gcell.reshape(nc, 6) < nc, axis=1)
+    assert interior.sum() > 0
+    for v in range(4):
+        np.testing.assert_allclose(g[interior, v, :], np.broadcast_to(coefs[v][1:], (interior.sum(), 3)),
+                                   atol=1e-12)
+
+
+def test_determinant_compat_only_differs_off_orthogonal():
+    """The reference's determinant (gradients.cpp:282-284) is exact on
+    orthogonal meshes and differs once cells are skewed."""
+    mo = abi.HexMesh(4, 4, 4, periodic=(1, 1, 1), jitter=0.0, seed=1)
+    a = orc.OracleFluid(mo, orc.OR_D3Q15, visc=0.1, dt=0.25, grad_type=1, det_compat=False)
+    b = orc.OracleFluid(mo, orc.OR_D3Q15, visc=0.1, dt=0.25, grad_type=1, det_compat=True)
+    np.testing.assert_allclose(a.lhs, b.lhs, atol=1e-15)
+    mj = abi.HexMesh(4, 4, 4, periodic=(1, 1, 1), jitter=0.1, seed=1)
+    a = orc.OracleFluid(mj, orc.OR_D3Q15, visc=0.1, dt=0.25, grad_type=1, det_compat=False)
+    b = orc.OracleFluid(mj, orc.OR_D3Q15, visc=0.1, dt=0.25, grad_type=1, det_compat=True)
+    assert np.max(np.abs(a.lhs - b.lhs)) > 1e-6
+
+
+def test_literal_d3q15_matches_table_driven():
+    """The table-driven lattice path equals the hand-expanded D3Q15 formulas
+    of Solver.cpp:655-1374 face by face."""
+    m = cases.channel_mesh(6, 5, 4)
+    st = cases.smooth_state(m, seed=4)
+    a = orc.OracleFluid(m, orc.OR_D3Q15, visc=0.05, dt=0.25)
+    b = orc.OracleFluid(m, orc.OR_D3Q15, visc=0.05, dt=0.25, flags=orc.FLAG_LITERAL_D3Q15)
+    a.set_state(*st)
+    b.set_state(*st)
+    fa, fb = a.face_flux(), b.face_flux()
+    assert np.max(np.abs(fa - fb)) < 1e-14 * max(1.0, np.abs(fa).max())
+
+
+@pytest.mark.parametrize("kind", [orc.OR_D3Q15, orc.OR_D3Q19])
+def test_discrete_conservation_periodic(kind):
+    """Face-scatter antisymmetry: sum_i V_i R_i = 0 for mass and momentum."""
+    m = abi.HexMesh(5, 4, 4, periodic=(1, 1, 1), jitter=0.1, seed=5)
+    o = orc.OracleFluid(m, kind, visc=0.05, dt=0.25)
+    o.set_state(*cases.smooth_state(m, seed=1))
+    r = o.residual().reshape(-1, 4)
+    tot = (r * o.vol[:, None]).sum(0)
+    assert np.abs(tot).max() < 1e-15 * m.n_cells
+
+
+@pytest.mark.parametrize("kind", [orc.OR_D3Q15, orc.OR_D3Q19])
+def test_free_stream_preserved(kind):
+    """A uniform moving state stays uniform on a closed jittered periodic mesh."""
+    m = abi.HexMesh(5, 4, 4, periodic=(1, 1, 1), jitter=0.1, seed=6)
+    o = orc.OracleFluid(m, kind, visc=0.05, dt=0.25, closed=True)
+    n = m.n_cells
+    o.set_state(np.full(n, 1.001), np.full(n, 0.02), np.full(n, -0.01), np.full(n, 0.005))
+    o.step(3)
+    r, u, v, w = o.state()
+    assert np.max(np.abs(r - 1.001)) < 1e-14
+    assert np.max(np.abs(u - 0.02)) < 1e-14 and np.max(np.abs(v + 0.01)) < 1e-14
+
+
+def test_rest_state_drifts_with_reference_face_area():
+    """The reference's face area makes a fluid at rest move on warped meshes
+    (the closure defect), and stays at rest with closed faces."""
+    m = abi.HexMesh(5, 4, 4, periodic=(1, 1, 1), jitter=0.1, seed=6)
+    n = m.n_cells
+    for closed, moves in ((False, True), (True, False)):
+        o = orc.OracleFluid(m, orc.OR_D3Q15, visc=0.05, dt=0.25, closed=closed)
+        o.set_state(np.ones(n), np.zeros(n), np.zeros(n), np.zeros(n))
+        o.step(1)
+        _, u, _, _ = o.state()
+        assert (np.abs(u).max() > 1e-6) == moves
+
+
+def _tgv_runs(flags, N=24, steps=160, dt=0.25):
+    m = abi.HexMesh(N, N, 1, periodic=(1, 1, 1), jitter=0.05, seed=1)
+    U0, Re = 0.01, 100.0
+    nu = U0 * N / Re
+    runs = {}
+    for kind in (orc.OR_D3Q15, orc.OR_D2Q9):
+        o = orc.OracleFluid(m, kind, visc=nu, dt=dt, flags=flags)
+        o.set_state(*cases.tgv_state(m, U0))
+        o.step(steps)
+        runs[kind] = o.state()
+    return m, nu, runs
+
+
+def test_taylor_green_d3q15_projects_exactly_with_tie_band():
+    """Config-1 style TGV on a jittered 2D mesh (one z layer, z-periodic).
+    With the tie band (LBFS.cu:349-352) the +-z points of D3Q15 take the face
+    average, and D3Q15 then reproduces D2Q9 to round-off (projection identity)."""
+    _, _, runs = _tgv_runs(orc.FLAG_UPWIND_BAND)
+    for a, b in zip(runs[orc.OR_D3Q15], runs[orc.OR_D2Q9]):
+        assert np.max(np.abs(a - b)) < 1e-13
+
+
+def test_taylor_green_decays_at_viscous_rate():
+    """Strict upwinding (Solver.cpp): both lattices decay as exp(-2 nu k^2 t)."""
+    m, nu, runs = _tgv_runs(0)
+    N, dt, steps = 24, 0.25, 160
+    k = 2 * np.pi / N
+    decay = np.exp(-2 * nu * k * k * dt * steps)
+    u0 = cases.tgv_state(m, 0.01)[1]
+    for kind in (orc.OR_D3Q15, orc.OR_D2Q9):
+        _, u, _, _ = runs[kind]
+        amp = np.dot(u, u0) / np.dot(u0, u0)
+        assert abs(amp / decay - 1.0) < 0.02, (kind, amp, decay)
