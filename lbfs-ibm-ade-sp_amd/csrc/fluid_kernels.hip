@@ -88,9 +88,18 @@ __device__ __forceinline__ void load_grad(const float4 *__restrict__ grad, int i
 }
 
 template <class L>
+constexpr bool opposite_pairs()
+{
+    for (int k = 1; k + 1 < L::Q; k += 2)
+        if (L::ex[k] != -L::ex[k + 1] || L::ey[k] != -L::ey[k + 1] || L::ez[k] != -L::ez[k + 1]) return false;
+    return (L::Q % 2) == 1;
+}
+
+template <class L, bool BAND>
 __device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__restrict__ st,
                                             const float4 *__restrict__ grad, int tf)
 {
+    static_assert(opposite_pairs<L>(), "lattice directions must come in opposite pairs after the rest point");
     const float4 fa = a.fA[tf], fb = a.fB[tf], fcv = a.fC[tf];
     const int2 on = a.fI[tf];
     const int o = on.x, nb = on.y;
@@ -147,7 +156,6 @@ __device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__
 #pragma unroll
     for (int t = 0; t < 12; t++) { go[t] *= dt; gn[t] *= dt; }
     const float rho0 = a.rho0, ig = a.inv_gamma;
-    const bool band = (a.flags & LBFS_FLAG_UPWIND_BAND) != 0;
     const float bw = 0.001f * dt;
     float S0, S1x = 0.0f, S1y = 0.0f, S1z = 0.0f, Tx = 0.0f, Ty = 0.0f, Tz = 0.0f;
     {
@@ -155,50 +163,58 @@ __device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__
         const float P = -1.5f * uu * ig;
         S0 = L::w(0) * (p0[0] + (rho0 + p0[0]) * P);
     }
+    // equilibrium deviation g = w (drho + rho P) of one lattice point
+    auto point = [&](const float ph[4], int ex, int ey, int ez, float w) -> float {
+        const float eu = ex * ph[1] + ey * ph[2] + ez * ph[3];
+        const float uu = ph[1] * ph[1] + ph[2] * ph[2] + ph[3] * ph[3];
+        const float P = eu * (3.0f + 4.5f * ig * eu) - 1.5f * ig * uu;
+        return w * (ph[0] + (rho0 + ph[0]) * P);
+    };
+    // directions come in pairs (e, -e): one e.n and one e.grad per pair
 #pragma unroll
-    for (int k = 1; k < L::Q; k++) {
-        constexpr int dummy = 0;
-        (void)dummy;
+    for (int k = 1; k < L::Q; k += 2) {
         const int ex = L::ex[k], ey = L::ey[k], ez = L::ez[k];
         const float en = ex * nx + ey * ny + ez * nz;
-        float ph[4];
+        float pk[4], pb[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const float ego = ex * go[3 * v] + ey * go[3 * v + 1] + ez * go[3 * v + 2];
             const float egn = ex * gn[3 * v] + ey * gn[3 * v + 1] + ez * gn[3 * v + 2];
-            const float vo = po[v] - ego, vn = pn[v] - egn;
-            if (!band) {
-                ph[v] = en < 0.0f ? vn : vo;
+            if (!BAND) {
+                // point x_f - e dt sits on the neighbour side when e.n < 0 (ties to the owner)
+                pk[v] = en < 0.0f ? pn[v] - egn : po[v] - ego;
+                pb[v] = en > 0.0f ? pn[v] + egn : po[v] + ego;
             } else {
-                const float vt = p0[v] - (ghost ? egn : 0.5f * (ego + egn));
-                ph[v] = en > bw ? vo : (en < -bw ? vn : vt);
+                const float avg = ghost ? egn : 0.5f * (ego + egn);
+                pk[v] = en > bw ? po[v] - ego : (en < -bw ? pn[v] - egn : p0[v] - avg);
+                pb[v] = en < -bw ? po[v] + ego : (en > bw ? pn[v] + egn : p0[v] + avg);
             }
         }
-        const float eu = ex * ph[1] + ey * ph[2] + ez * ph[3];
-        const float uu = ph[1] * ph[1] + ph[2] * ph[2] + ph[3] * ph[3];
-        const float P = 3.0f * eu + (4.5f * eu * eu - 1.5f * uu) * ig;
-        const float gk = L::w(k) * (ph[0] + (rho0 + ph[0]) * P);
-        S0 += gk;
-        S1x += ex * gk; S1y += ey * gk; S1z += ez * gk;
-        const float eng = en * gk;
+        const float gk = point(pk, ex, ey, ez, L::w(k));
+        const float gb = point(pb, -ex, -ey, -ez, L::w(k + 1));
+        const float gs = gk + gb, gd = gk - gb;
+        S0 += gs;
+        S1x += ex * gd; S1y += ey * gd; S1z += ez * gd;
+        const float eng = en * gs;
         Tx += ex * eng; Ty += ey * eng; Tz += ez * eng;
     }
     const float rho_f = rho0 + S0;
     const float mn = S1x * nx + S1y * ny + S1z * nz;
     const float vf = a.visc_factor ? a.visc_factor[o] : 1.0f;
-    const float tau = a.visc3_over_gamma * vf / dt;
+    const float tau = a.visc3_over_gamma * vf * __builtin_amdgcn_rcpf(dt);
     const float c1 = 1.0f - tau;
     const float third = S0 * (1.0f / 3.0f);
-    const float q = mn / rho_f * ig;
+    const float q = mn * __builtin_amdgcn_rcpf(rho_f) * ig;
     float4 r;
     r.x = mn * A;
     r.y = A * (c1 * (third * nx + S1x * q) + tau * Tx);
     r.z = A * (c1 * (third * ny + S1y * q) + tau * Ty);
-    r.w = A * (c1 * (third * nz + S1z * q) + tau * Tz);
+    // a planar lattice carries no z momentum flux (its sum e_z e_z w is zero)
+    r.w = L::dims == 3 ? A * (c1 * (third * nz + S1z * q) + tau * Tz) : 0.0f;
     return r;
 }
 
-template <class L>
+template <class L, bool BAND>
 __global__ __launch_bounds__(256) void k_flux_update(FluidArgs a, const float4 *__restrict__ st,
                                                      const float4 *__restrict__ grad, float4 *__restrict__ st_out,
                                                      float4 *__restrict__ res_out, int do_update)
@@ -206,7 +222,7 @@ __global__ __launch_bounds__(256) void k_flux_update(FluidArgs a, const float4 *
     extern __shared__ float4 lds_flux[];
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int f0 = a.tile_face_start[t], f1 = a.tile_face_start[t + 1];
-    for (int f = f0 + threadIdx.x; f < f1; f += blockDim.x) lds_flux[f - f0] = face_flux<L>(a, st, grad, f);
+    for (int f = f0 + threadIdx.x; f < f1; f += blockDim.x) lds_flux[f - f0] = face_flux<L, BAND>(a, st, grad, f);
     __syncthreads();
     const int c0 = a.tile_cell_start[t], c1 = a.tile_cell_start[t + 1];
     for (int i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
@@ -244,13 +260,13 @@ __global__ __launch_bounds__(256) void k_flux_update(FluidArgs a, const float4 *
     }
 }
 
-template <class L>
+template <class L, bool BAND>
 __global__ __launch_bounds__(256) void k_face_flux_all(FluidArgs a, const float4 *__restrict__ st,
                                                        const float4 *__restrict__ grad, float4 *__restrict__ out,
                                                        int n_tile_faces)
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f < n_tile_faces) out[f] = face_flux<L>(a, st, grad, f);
+    if (f < n_tile_faces) out[f] = face_flux<L, BAND>(a, st, grad, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -296,30 +312,53 @@ hipError_t launch_gradients(const FluidArgs &a, const float4 *st, float4 *grad, 
     return hipGetLastError();
 }
 
+// instantiate a kernel template for the context's lattice and tie-band mode
+template <template <class, bool> class K, class... Args>
+static hipError_t dispatch(const FluidArgs &a, dim3 grid, size_t lds, hipStream_t s, Args... args)
+{
+    const bool band = (a.flags & LBFS_FLAG_UPWIND_BAND) != 0;
+    switch (a.lattice * 2 + (band ? 1 : 0)) {
+    case 0: K<D3Q15, false>::launch(grid, lds, s, a, args...); break;
+    case 1: K<D3Q15, true>::launch(grid, lds, s, a, args...); break;
+    case 2: K<D3Q19, false>::launch(grid, lds, s, a, args...); break;
+    case 3: K<D3Q19, true>::launch(grid, lds, s, a, args...); break;
+    case 4: K<D2Q9, false>::launch(grid, lds, s, a, args...); break;
+    case 5: K<D2Q9, true>::launch(grid, lds, s, a, args...); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class L, bool BAND>
+struct FluxUpdate {
+    static void launch(dim3 g, size_t lds, hipStream_t s, const FluidArgs &a, const float4 *st, const float4 *grad,
+                       float4 *st_out, float4 *res_out, int do_update)
+    {
+        hipLaunchKernelGGL((k_flux_update<L, BAND>), g, dim3(256), lds, s, a, st, grad, st_out, res_out, do_update);
+    }
+};
+
+template <class L, bool BAND>
+struct FaceFluxAll {
+    static void launch(dim3 g, size_t lds, hipStream_t s, const FluidArgs &a, const float4 *st, const float4 *grad,
+                       float4 *out, int n)
+    {
+        hipLaunchKernelGGL((k_face_flux_all<L, BAND>), g, dim3(256), lds, s, a, st, grad, out, n);
+    }
+};
+
 hipError_t launch_flux_update(const FluidArgs &a, const float4 *st, const float4 *grad, float4 *st_out,
                               float4 *res_out, int do_update, hipStream_t s)
 {
     const size_t lds = sizeof(float4) * (size_t)a.max_tile_faces;
-    switch (a.lattice) {
-    case 0: hipLaunchKernelGGL(k_flux_update<D3Q15>, dim3(a.n_tiles), dim3(256), lds, s, a, st, grad, st_out, res_out, do_update); break;
-    case 1: hipLaunchKernelGGL(k_flux_update<D3Q19>, dim3(a.n_tiles), dim3(256), lds, s, a, st, grad, st_out, res_out, do_update); break;
-    case 2: hipLaunchKernelGGL(k_flux_update<D2Q9>, dim3(a.n_tiles), dim3(256), lds, s, a, st, grad, st_out, res_out, do_update); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    return dispatch<FluxUpdate>(a, dim3(a.n_tiles), lds, s, st, grad, st_out, res_out, do_update);
 }
 
 hipError_t launch_face_flux_all(const FluidArgs &a, const float4 *st, const float4 *grad, float4 *out, int n_tile_faces,
                                 hipStream_t s)
 {
     const int blocks = (n_tile_faces + 255) / 256;
-    switch (a.lattice) {
-    case 0: hipLaunchKernelGGL(k_face_flux_all<D3Q15>, dim3(blocks), dim3(256), 0, s, a, st, grad, out, n_tile_faces); break;
-    case 1: hipLaunchKernelGGL(k_face_flux_all<D3Q19>, dim3(blocks), dim3(256), 0, s, a, st, grad, out, n_tile_faces); break;
-    case 2: hipLaunchKernelGGL(k_face_flux_all<D2Q9>, dim3(blocks), dim3(256), 0, s, a, st, grad, out, n_tile_faces); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    return dispatch<FaceFluxAll>(a, dim3(blocks), 0, s, st, grad, out, n_tile_faces);
 }
 
 hipError_t launch_diag_partials(int n, float rho0, const float4 *st, const float4 *res, const float *inv_vol, double *part,

@@ -8,6 +8,7 @@ namespace lbfs {
 struct D3Q15 {
     static constexpr int Q = 15;
     static constexpr int id = 0;
+    static constexpr int dims = 3;
     static constexpr int ex[15] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1};
     static constexpr int ey[15] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, -1, 1, 1, -1};
     static constexpr int ez[15] = {0, 0, 0, 0, 0, 1, -1, 1, -1, -1, 1, 1, -1, 1, -1};
@@ -17,6 +18,7 @@ struct D3Q15 {
 struct D3Q19 {
     static constexpr int Q = 19;
     static constexpr int id = 1;
+    static constexpr int dims = 3;
     static constexpr int ex[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
     static constexpr int ey[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
     static constexpr int ez[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
@@ -26,6 +28,7 @@ struct D3Q19 {
 struct D2Q9 {
     static constexpr int Q = 9;
     static constexpr int id = 2;
+    static constexpr int dims = 2;
     static constexpr int ex[9] = {0, 1, -1, 0, 0, 1, -1, 1, -1};
     static constexpr int ey[9] = {0, 0, 0, 1, -1, 1, -1, -1, 1};
     static constexpr int ez[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};

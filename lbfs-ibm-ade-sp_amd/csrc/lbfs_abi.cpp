@@ -60,7 +60,7 @@ static int fluid_setup(lbfs_ctx *ctx, const lbfs_mesh_desc &m, const lbfs_params
     std::vector<double> lhs, rhs;
     rc = compute_ls(ctx->geo, p.gradient_type, (p.flags & LBFS_FLAG_LS_DET_COMPAT) != 0, lhs, rhs, err);
     if (rc) return ctx->fail(rc, "least squares: %s", err.c_str());
-    rc = build_layout(m, ctx->geo, lhs, rhs, p.rho0, p.body_force, ctx->lay, err);
+    rc = build_layout(m, ctx->geo, lhs, rhs, p.rho0, p.body_force, p.lattice == LBFS_D2Q9 ? 2 : 3, ctx->lay, err);
     if (rc) return ctx->fail(rc, "layout: %s", err.c_str());
     const DeviceLayout &d = ctx->lay;
     const int n = d.n_cells, nbc = d.n_bc;
@@ -259,6 +259,7 @@ extern "C" int lbfs_stage_face_flux(lbfs_ctx *ctx, double *flux_out)
     // the device flux leaves out the rest pressure rho0/3 n A (it cancels per
     // cell); it is added back here so the result is the reference face flux
     const double p0 = ctx->params.rho0 / 3.0;
+    const double p0z = ctx->params.lattice == LBFS_D2Q9 ? 0.0 : p0;
     for (int f = 0; f < ctx->geo.n_faces; f++) {
         const float4 r = h[ctx->lay.face_first[f]];
         const double A = ctx->geo.area[f];
@@ -266,7 +267,7 @@ extern "C" int lbfs_stage_face_flux(lbfs_ctx *ctx, double *flux_out)
         flux_out[4 * (size_t)f] = r.x;
         flux_out[4 * (size_t)f + 1] = r.y + p0 * n[0] * A;
         flux_out[4 * (size_t)f + 2] = r.z + p0 * n[1] * A;
-        flux_out[4 * (size_t)f + 3] = r.w + p0 * n[2] * A;
+        flux_out[4 * (size_t)f + 3] = r.w + p0z * n[2] * A;
     }
     return LBFS_OK;
 }
@@ -290,12 +291,13 @@ extern "C" int lbfs_stage_residual(lbfs_ctx *ctx, double *res_out)
             if (g.nbr[f] < n) S[3 * (size_t)g.nbr[f] + t] -= v;
         }
     const double p0 = ctx->params.rho0 / 3.0;
+    const double p0z = ctx->params.lattice == LBFS_D2Q9 ? 0.0 : p0;
     for (int i = 0; i < n; i++) {
         const int c = ctx->lay.perm[i];
         res_out[4 * (size_t)c] = h[i].x;
         res_out[4 * (size_t)c + 1] = h[i].y - p0 * S[3 * (size_t)c] / g.vol[c];
         res_out[4 * (size_t)c + 2] = h[i].z - p0 * S[3 * (size_t)c + 1] / g.vol[c];
-        res_out[4 * (size_t)c + 3] = h[i].w - p0 * S[3 * (size_t)c + 2] / g.vol[c];
+        res_out[4 * (size_t)c + 3] = h[i].w - p0z * S[3 * (size_t)c + 2] / g.vol[c];
     }
     return LBFS_OK;
 }

@@ -290,7 +290,7 @@ int compute_ls(const Geometry &g, int grad_type, bool det_compat, std::vector<do
 // tile-blocked device layout
 // ---------------------------------------------------------------------------
 int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<double> &lhs,
-                 const std::vector<double> &rhs, double rho0, const double body_force[3], DeviceLayout &d,
+                 const std::vector<double> &rhs, double rho0, const double body_force[3], int dims, DeviceLayout &d,
                  std::string &err)
 {
     const int nc = g.n_cells;
@@ -395,7 +395,8 @@ int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<d
         for (int i = 0; i < nc; i++) {
             const int c = d.perm[i];
             for (int t = 0; t < 3; t++)
-                d.static_force[3 * (size_t)i + t] = (float)(body_force[t] - rho0 / 3.0 * S[3 * (size_t)c + t] / g.vol[c]);
+                d.static_force[3 * (size_t)i + t] =
+                    (float)(body_force[t] - (t < dims ? rho0 / 3.0 * S[3 * (size_t)c + t] / g.vol[c] : 0.0));
         }
     }
     d.bc_owner.assign(g.n_bc, 0);

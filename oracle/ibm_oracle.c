@@ -1,0 +1,104 @@
+/*
+ * FP64 CPU oracle: immersed-boundary interpolation and spreading.
+ *
+ * TEST INFRASTRUCTURE ONLY (see lbfs_oracle.h).
+ *
+ * OR_IBM_LINEAR2 restates the active reference kernels
+ * interpolate_velocities_on_nodes (immersed_boundary_method.cu:14-97) and
+ * spread_forces_on_structured_grid (immersed_boundary_method.cu:947-1024):
+ * x_ref = (X - origin)/h, cells round(x_ref)-1 .. round(x_ref), weight
+ * 1 - |i + 0.5 - x_ref| per axis on the logical grid, spreading F w / h^3.
+ * Extension: periodic axes wrap the cell index instead of dropping it.
+ *
+ * OR_IBM_COS4 is the 4-point cosine kernel phi(r) = (1 + cos(pi r / 2)) / 4
+ * of the inactive reference kernels (immersed_boundary_method.cu:100-165,
+ * 1028-1091) made 3D, evaluated on physical centroid distances r = (x_c - X)/h
+ * over the 4x4x4 logical stencil and renormalised to sum 1; spreading uses
+ * the same weights, F w / V_c, so sum_c f_c V_c = F and interpolation and
+ * spreading are adjoint.
+ */
+#include "lbfs_oracle.h"
+
+#include <math.h>
+
+#define OR_PI 3.14159265358979323846
+
+static int wrap(int i, int n, int periodic, int *ok)
+{
+    if (i >= 0 && i < n) return i;
+    if (!periodic) { *ok = 0; return 0; }
+    i %= n;
+    if (i < 0) i += n;
+    return i;
+}
+
+/* stencil of node X: up to 64 (cell id, weight) pairs; returns count */
+static int stencil(int kernel, const or_grid *g, const double *X, int *cells, double *wts)
+{
+    const int n3[3] = {g->nx, g->ny, g->nz};
+    int base[3], S = kernel == OR_IBM_LINEAR2 ? 2 : 4, a, i, j, k, cnt = 0;
+    double xr[3], wsum = 0.0;
+    for (a = 0; a < 3; a++) {
+        xr[a] = (X[a] - g->origin[a]) / g->h;
+        base[a] = kernel == OR_IBM_LINEAR2 ? (int)round(xr[a]) - 1 : (int)floor(xr[a] - 1.5);
+    }
+    for (i = 0; i < S; i++)
+        for (j = 0; j < S; j++)
+            for (k = 0; k < S; k++) {
+                int idx[3] = {base[0] + i, base[1] + j, base[2] + k}, w3[3], ok = 1, c;
+                double w = 1.0;
+                for (a = 0; a < 3; a++) w3[a] = wrap(idx[a], n3[a], g->periodic[a], &ok);
+                if (!ok) continue;
+                c = w3[0] + w3[1] * g->nx + w3[2] * g->nx * g->ny;
+                for (a = 0; a < 3; a++) {
+                    double r;
+                    if (kernel == OR_IBM_LINEAR2) {
+                        r = idx[a] + 0.5 - xr[a];
+                        w *= 1.0 - fabs(r);
+                    } else {
+                        double xc = g->cc[3 * c + a];
+                        /* periodic image of the centroid nearest to the node */
+                        if (g->periodic[a]) xc += (idx[a] - w3[a]) * (g->L[a] / n3[a]);
+                        r = (xc - X[a]) / g->h;
+                        w *= fabs(r) < 2.0 ? 0.25 * (1.0 + cos(OR_PI * r / 2.0)) : 0.0;
+                    }
+                }
+                cells[cnt] = c;
+                wts[cnt] = w;
+                wsum += w;
+                cnt++;
+            }
+    if (kernel == OR_IBM_COS4 && wsum > 0.0)
+        for (i = 0; i < cnt; i++) wts[i] /= wsum;
+    return cnt;
+}
+
+int or_ibm_interpolate(int kernel, const or_grid *g, int n_nodes, const double *X, const double *st, double *U)
+{
+    int n, t, a;
+    int cells[64];
+    double w[64];
+    for (n = 0; n < n_nodes; n++) {
+        int cnt = stencil(kernel, g, X + 3 * n, cells, w);
+        for (a = 0; a < 3; a++) U[3 * n + a] = 0.0;
+        for (t = 0; t < cnt; t++)
+            for (a = 0; a < 3; a++) U[3 * n + a] += st[4 * cells[t] + 1 + a] * w[t];
+    }
+    return 0;
+}
+
+int or_ibm_spread(int kernel, const or_grid *g, int n_nodes, const double *X, const double *F, double *f_cell)
+{
+    int n, t, a;
+    int cells[64];
+    double w[64];
+    const double ih3 = 1.0 / (g->h * g->h * g->h);
+    for (n = 0; n < n_nodes; n++) {
+        int cnt = stencil(kernel, g, X + 3 * n, cells, w);
+        for (t = 0; t < cnt; t++) {
+            double s = kernel == OR_IBM_LINEAR2 ? ih3 : 1.0 / g->vol[cells[t]];
+            for (a = 0; a < 3; a++) f_cell[3 * cells[t] + a] += F[3 * n + a] * w[t] * s;
+        }
+    }
+    return 0;
+}

@@ -87,12 +87,19 @@ void or_residual_rms(int n_cells, const double *res, double ref_rho, double ref_
 /* ---- immersed boundary (immersed_boundary_method.cu) ---- */
 #define OR_IBM_LINEAR2 0  /* 2-point trilinear on the logical grid (immersed_boundary_method.cu:14-97, 947-1024) */
 #define OR_IBM_COS4 1     /* 4-point cosine on physical centroid distances, renormalised */
-int or_ibm_interpolate(int kernel, int n_nodes, const double *X, int nx, int ny, int nz,
-                       const double *origin, double h, const double *cc, const double *st, int n_cells,
+/* logical grid: cell id i + j nx + k nx ny, nominal spacing h from `origin`;
+ * periodic axes wrap (lengths L); cc / st / vol are in cell-id order */
+typedef struct {
+    int nx, ny, nz;
+    int periodic[3];
+    double origin[3], L[3], h;
+    const double *cc;   /* [3 n] centroids */
+    const double *vol;  /* [n] */
+} or_grid;
+int or_ibm_interpolate(int kernel, const or_grid *g, int n_nodes, const double *X, const double *st,
                        double *U);
-int or_ibm_spread(int kernel, int n_nodes, const double *X, const double *F, int nx, int ny, int nz,
-                  const double *origin, double h, const double *cc, const double *vol, int n_cells,
-                  double *f_cell);
+int or_ibm_spread(int kernel, const or_grid *g, int n_nodes, const double *X, const double *F,
+                  double *f_cell /* [3 n], accumulated */);
 
 /* ---- spring-network membrane (spring_network.cu) ---- */
 typedef struct {

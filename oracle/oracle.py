@@ -34,6 +34,27 @@ class OrParams(C.Structure):
                 ("dt", C.c_double)]
 
 
+class OrGrid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("periodic", C.c_int * 3),
+                ("origin", C.c_double * 3), ("L", C.c_double * 3), ("h", C.c_double), ("cc", _dp), ("vol", _dp)]
+
+
+class OrMemParams(C.Structure):
+    _fields_ = [("shear_modulus", C.c_double), ("wlc_ratio", C.c_double), ("volume_modulus", C.c_double),
+                ("local_area_modulus", C.c_double), ("global_area_modulus", C.c_double),
+                ("global_bending_modulus", C.c_double), ("local_bending_modulus", C.c_double),
+                ("viscosity", C.c_double), ("curvature_ratio", C.c_double), ("volume_ratio", C.c_double),
+                ("sphere_reference", C.c_int), ("mad_compat", C.c_int)]
+
+
+class OrMembrane(C.Structure):
+    _fields_ = [("n_nodes", C.c_int), ("n_tri", C.c_int), ("n_edges", C.c_int), ("tri", _ip), ("spring", _ip),
+                ("contour_length", _dp), ("spring_constant", _dp), ("curvature_0", _dp), ("spring_angle_0", _dp),
+                ("spring_area_0", _dp), ("curvature", _dp), ("spring_area", _dp),
+                ("total_volume", C.c_double), ("total_volume_0", C.c_double), ("total_area", C.c_double),
+                ("total_area_0", C.c_double), ("mad", C.c_double), ("mad_0", C.c_double)]
+
+
 _lib = None
 
 
@@ -64,6 +85,20 @@ def load():
         lib.or_residual_rms.argtypes = [C.c_int, _dp, C.c_double, C.c_double, _dp]
         for name in ("or_bc_update", "or_gradients", "or_face_flux", "or_residual", "or_update", "or_residual_rms"):
             getattr(lib, name).restype = None
+        og = C.POINTER(OrGrid)
+        lib.or_ibm_interpolate.restype = C.c_int
+        lib.or_ibm_interpolate.argtypes = [C.c_int, og, C.c_int, _dp, _dp, _dp]
+        lib.or_ibm_spread.restype = C.c_int
+        lib.or_ibm_spread.argtypes = [C.c_int, og, C.c_int, _dp, _dp, _dp]
+        mbp, mpp = C.POINTER(OrMembrane), C.POINTER(OrMemParams)
+        lib.or_build_springs.restype = C.c_int
+        lib.or_build_springs.argtypes = [C.c_int, C.c_int, _ip, _ip]
+        lib.or_membrane_setup.restype = None
+        lib.or_membrane_setup.argtypes = [mbp, mpp, _dp, _dp, _dp, _dp]
+        lib.or_membrane_forces.restype = None
+        lib.or_membrane_forces.argtypes = [mbp, mpp, _dp, _dp, C.c_int, _dp]
+        lib.or_membrane_globals.restype = None
+        lib.or_membrane_globals.argtypes = [mbp, _dp, _dp, _dp]
         _lib = lib
     return _lib
 
