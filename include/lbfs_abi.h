@@ -130,6 +130,46 @@ typedef struct {
 } lbfs_diag;
 int lbfs_reduce_diagnostics(lbfs_ctx *ctx, double ref_rho, double ref_u, lbfs_diag *out);
 
+/* ---- spring-network membranes and immersed-boundary coupling ----
+ * One descriptor per membrane (lagrangian_object, lagrangian_object.h:17-141;
+ * parameters parsed in preprocessor.cpp:120-401).  Node positions are
+ * absolute.  The three shapes are those of Solver_gpu.cu:659-732: contour
+ * lengths from the stress-free shape, reference angles from the curvature
+ * shape, positions from the initial shape (NULL = the initial shape). */
+typedef struct {
+    int n_nodes;
+    int n_tri;
+    const double *nodes;              /* [3 n_nodes] initial shape */
+    const double *nodes_stress_free;  /* or NULL */
+    const double *nodes_curvature;    /* or NULL */
+    const int *tri;                   /* [3 n_tri], outward winding */
+    double shear_modulus, wlc_ratio;
+    double volume_modulus, local_area_modulus, global_area_modulus;
+    double global_bending_modulus, local_bending_modulus;
+    double viscosity;                 /* membrane viscosity (Osay dissipative term) */
+    double curvature_ratio, volume_ratio;
+    int sphere_reference;             /* 1: curvature_0 = curvature_ratio / 3.265483301 */
+    int mad_compat;                   /* 1: MAD over the first n_nodes edges (Solver_gpu.cu:2355) */
+} lbfs_membrane_desc;
+
+/* attach membranes (replaces any attached before); requires the mesh's logical
+ * structure (nx, ny, nz, origin, spacing); ibm_kernel = LBFS_IBM_*.
+ * Set-up of Solver_gpu.cu:659-732; per step run_spring_network_functions_atomics
+ * (Solver_gpu.cu:2311-2361), interpolate/advect (Solver_gpu.cu:808-828) and
+ * spreading (Solver_gpu.cu:899-915) join the fluid step. */
+int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, int count, int ibm_kernel);
+/* concatenated node state of all membranes (positions absolute) */
+int lbfs_get_membrane_nodes(lbfs_ctx *ctx, double *X, double *U, double *F);
+/* per membrane: V, A, MAD, V0, A0, MAD0 (FP64, lagrangian_object.cpp:601-650) */
+int lbfs_get_membrane_globals(lbfs_ctx *ctx, double *out6);
+
+/* host-side membrane meshes: frequency-n geodesic sphere (10 n^2 + 2 nodes) */
+int lbfs_mesh_icosphere(int frequency, double *nodes, int *tri);
+/* Evans-Fung biconcave disc of diameter d0 from a unit sphere (symmetry axis z) */
+int lbfs_mesh_evans_fung(int n_nodes, const double *unit_nodes, double d0, double *nodes);
+/* springs (a,b,c,d), edge a-c, of a closed triangulation (lagrangian_object.cpp:322-490) */
+int lbfs_mesh_springs(int n_nodes, int n_tri, const int *tri, int *springs, int *n_springs);
+
 /* ---- measurement (no counterpart in the reference; used by bench.py) ----
  * HIP events recorded on the context's stream.  lbfs_time_steps: wall time of
  * nsteps whole steps.  lbfs_time_kernels: average duration per launch of each

@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "coupling.hpp"
 #include "fluid_kernels.hpp"
 #include "lbfs_abi.h"
 #include "mesh_host.hpp"
@@ -31,6 +32,14 @@ struct lbfs_ctx {
     float *static_force = nullptr;  // [3 n] body force + closure term
     float *cell_force = nullptr;    // [3 n] static force + spread IBM force
     std::vector<void *> allocs;
+
+    // membranes and IBM coupling (lbfs_add_membranes)
+    bool have_membranes = false;
+    lbfs::MemArgs mem{};
+    lbfs::IbmArgs ibm{};
+    std::vector<int> mem_node_start;
+    std::vector<double> mem_origin;
+    std::vector<void *> coupling_allocs;
 
     int fail(int code, const char *fmt, ...);
     int enqueue_step();

@@ -1,0 +1,277 @@
+// Host side of the membrane / IBM coupling: set-up of the device arrays of all
+// membranes (Solver_gpu.cu:473-611, 659-732 for one membrane; here any number,
+// each with its own parameters and connectivity offsets) and state readback.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "context.hpp"
+#include "coupling.hpp"
+#include "membrane_host.hpp"
+
+using namespace lbfs;
+
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return ctx->fail(LBFS_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+static int cp_alloc(lbfs_ctx *ctx, T **dst, const T *src, size_t n)
+{
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, (n > 0 ? n : 1) * sizeof(T) + 16));
+    ctx->coupling_allocs.push_back(p);
+    if (src && n) HIPCHK(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    else HIPCHK(hipMemset(p, 0, (n > 0 ? n : 1) * sizeof(T)));
+    *dst = (T *)p;
+    return LBFS_OK;
+}
+
+void lbfs_ctx::release_extensions()
+{
+    for (void *p : coupling_allocs) (void)hipFree(p);
+    coupling_allocs.clear();
+    have_membranes = false;
+}
+
+extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, int count, int ibm_kernel)
+{
+    if (!ctx || !mem || count <= 0) return LBFS_E_INVALID_ARG;
+    if (ibm_kernel != LBFS_IBM_LINEAR2 && ibm_kernel != LBFS_IBM_COS4)
+        return ctx->fail(LBFS_E_INVALID_ARG, "unknown IBM kernel %d", ibm_kernel);
+    if (ctx->mesh_nx <= 0 || ctx->mesh_ny <= 0 || ctx->mesh_nz <= 0 || !(ctx->mesh_spacing > 0.0) ||
+        (long long)ctx->mesh_nx * ctx->mesh_ny * ctx->mesh_nz != ctx->lay.n_cells)
+        return ctx->fail(LBFS_E_UNSUPPORTED, "membranes need a mesh with a logical i,j,k structure and spacing");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->release_extensions();
+    // restore the force field without IBM contributions
+    HIPCHK(hipMemcpy(ctx->cell_force, ctx->static_force, sizeof(float) * 3 * (size_t)ctx->lay.n_cells,
+                     hipMemcpyDeviceToDevice));
+    std::vector<MembraneSetup> ms(count);
+    std::string err;
+    for (int i = 0; i < count; i++) {
+        int rc = membrane_setup(mem[i], ms[i], err);
+        if (rc) return ctx->fail(rc, "membrane %d: %s", i, err.c_str());
+    }
+    // concatenated host arrays
+    std::vector<float4> X, origin(count);
+    std::vector<int> node_mem, mem_tri_start(1, 0), mem_edge_start(1, 0), mem_node_start(1, 0), edge_mem;
+    std::vector<int4> tri, edges;
+    std::vector<float4> edge_const, edge_state;
+    std::vector<float2> edge_area0;
+    std::vector<MemParamDev> params(count);
+    std::vector<double> glob(8 * (size_t)count, 0.0);
+    for (int i = 0; i < count; i++) {
+        const MembraneSetup &s = ms[i];
+        const int nb = mem_node_start.back(), eb = mem_edge_start.back();
+        double c[3] = {0, 0, 0};
+        for (int n = 0; n < s.n_nodes; n++)
+            for (int a = 0; a < 3; a++) c[a] += s.X[3 * (size_t)n + a] / s.n_nodes;
+        origin[i] = make_float4((float)c[0], (float)c[1], (float)c[2], 0.0f);
+        const double oc[3] = {(double)origin[i].x, (double)origin[i].y, (double)origin[i].z};
+        for (int n = 0; n < s.n_nodes; n++) {
+            X.push_back(make_float4((float)(s.X[3 * (size_t)n] - oc[0]), (float)(s.X[3 * (size_t)n + 1] - oc[1]),
+                                    (float)(s.X[3 * (size_t)n + 2] - oc[2]), 0.0f));
+            node_mem.push_back(i);
+        }
+        for (int t = 0; t < s.n_tri; t++)
+            tri.push_back(make_int4(nb + s.tri[3 * t], nb + s.tri[3 * t + 1], nb + s.tri[3 * t + 2], i));
+        for (int e = 0; e < s.n_edges; e++) {
+            const int *q = &s.springs[4 * (size_t)e];
+            edges.push_back(make_int4(nb + q[0], nb + q[1], nb + q[2], nb + q[3]));
+            edge_mem.push_back(i);
+            edge_const.push_back(make_float4((float)s.contour_length[e], (float)s.spring_constant[e], (float)s.curvature_0[e], 0.0f));
+            edge_area0.push_back(make_float2((float)s.spring_area_0[2 * (size_t)e], (float)s.spring_area_0[2 * (size_t)e + 1]));
+            edge_state.push_back(make_float4((float)s.curvature[e], (float)s.spring_area[2 * (size_t)e],
+                                             (float)s.spring_area[2 * (size_t)e + 1], 0.0f));
+        }
+        mem_node_start.push_back(nb + s.n_nodes);
+        mem_tri_start.push_back(mem_tri_start.back() + s.n_tri);
+        mem_edge_start.push_back(eb + s.n_edges);
+        const double r = mem[i].wlc_ratio;
+        MemParamDev &p = params[i];
+        p.spring_c = (float)(0.25 * r * r / ((r - 1.0) * (r - 1.0)) - 0.25 + 1.0 / r);
+        p.volume_modulus = (float)mem[i].volume_modulus;
+        p.local_area_modulus = (float)mem[i].local_area_modulus;
+        p.global_area_modulus = (float)mem[i].global_area_modulus;
+        p.global_bending = (float)mem[i].global_bending_modulus;
+        p.local_bending = (float)mem[i].local_bending_modulus;
+        p.viscosity = (float)mem[i].viscosity;
+        p.n_mad = mem[i].mad_compat ? std::min(s.n_nodes, s.n_edges) : s.n_edges;
+        double *gl = &glob[8 * (size_t)i];
+        gl[0] = s.V; gl[1] = s.A; gl[2] = s.MAD; gl[3] = s.V0; gl[4] = s.A0; gl[5] = s.MAD0; gl[6] = s.A;
+    }
+    const int nn = (int)X.size(), ne = (int)edges.size();
+    // node -> edge-slot incidences in (edge, slot) order
+    std::vector<int> inc_start(nn + 1, 0), inc(4 * (size_t)ne);
+    for (int e = 0; e < ne; e++) {
+        const int q[4] = {edges[e].x, edges[e].y, edges[e].z, edges[e].w};
+        for (int j = 0; j < 4; j++) inc_start[q[j] + 1]++;
+    }
+    for (int n = 0; n < nn; n++) inc_start[n + 1] += inc_start[n];
+    {
+        std::vector<int> fill(inc_start.begin(), inc_start.end() - 1);
+        for (int e = 0; e < ne; e++) {
+            const int q[4] = {edges[e].x, edges[e].y, edges[e].z, edges[e].w};
+            for (int j = 0; j < 4; j++) inc[fill[q[j]]++] = 4 * e + j;
+        }
+    }
+    MemArgs &m = ctx->mem;
+    memset(&m, 0, sizeof(m));
+    m.n_mem = count; m.n_nodes = nn; m.n_tri = (int)tri.size(); m.n_edges = ne;
+    int rc;
+    float4 *dX, *dU, *dF, *dorig, *dec, *des, *def;
+    int *dnm, *dts, *des0, *dem, *dis, *dinc;
+    int4 *dtri, *dedg;
+    float2 *dea;
+    MemParamDev *dpar;
+    double *dgl;
+    if ((rc = cp_alloc(ctx, &dX, X.data(), nn))) return rc;
+    if ((rc = cp_alloc<float4>(ctx, &dU, nullptr, nn))) return rc;
+    if ((rc = cp_alloc<float4>(ctx, &dF, nullptr, nn))) return rc;
+    if ((rc = cp_alloc(ctx, &dorig, origin.data(), count))) return rc;
+    if ((rc = cp_alloc(ctx, &dnm, node_mem.data(), nn))) return rc;
+    if ((rc = cp_alloc(ctx, &dtri, tri.data(), tri.size()))) return rc;
+    if ((rc = cp_alloc(ctx, &dts, mem_tri_start.data(), mem_tri_start.size()))) return rc;
+    if ((rc = cp_alloc(ctx, &des0, mem_edge_start.data(), mem_edge_start.size()))) return rc;
+    if ((rc = cp_alloc(ctx, &dedg, edges.data(), ne))) return rc;
+    if ((rc = cp_alloc(ctx, &dem, edge_mem.data(), ne))) return rc;
+    if ((rc = cp_alloc(ctx, &dec, edge_const.data(), ne))) return rc;
+    if ((rc = cp_alloc(ctx, &dea, edge_area0.data(), ne))) return rc;
+    if ((rc = cp_alloc(ctx, &des, edge_state.data(), ne))) return rc;
+    if ((rc = cp_alloc<float4>(ctx, &def, nullptr, 4 * (size_t)ne))) return rc;
+    if ((rc = cp_alloc(ctx, &dis, inc_start.data(), inc_start.size()))) return rc;
+    if ((rc = cp_alloc(ctx, &dinc, inc.data(), inc.size()))) return rc;
+    if ((rc = cp_alloc(ctx, &dpar, params.data(), count))) return rc;
+    if ((rc = cp_alloc(ctx, &dgl, glob.data(), glob.size()))) return rc;
+    m.X = dX; m.U = dU; m.F = dF; m.node_mem = dnm; m.origin = dorig; m.tri = dtri;
+    m.mem_tri_start = dts; m.mem_edge_start = des0; m.edges = dedg; m.edge_mem = dem;
+    m.edge_const = dec; m.edge_area0 = dea; m.edge_state = des; m.edge_force = def;
+    m.inc_start = dis; m.inc = dinc; m.params = dpar; m.glob = dgl;
+    ctx->mem_node_start = mem_node_start;
+    ctx->mem_origin.assign(3 * (size_t)count, 0.0);
+    for (int i = 0; i < count; i++) {
+        ctx->mem_origin[3 * (size_t)i] = origin[i].x;
+        ctx->mem_origin[3 * (size_t)i + 1] = origin[i].y;
+        ctx->mem_origin[3 * (size_t)i + 2] = origin[i].z;
+    }
+    // IBM grid
+    IbmArgs &g = ctx->ibm;
+    memset(&g, 0, sizeof(g));
+    g.kernel = ibm_kernel;
+    const int nl[3] = {ctx->mesh_nx, ctx->mesh_ny, ctx->mesh_nz};
+    for (int a = 0; a < 3; a++) {
+        g.n[a] = nl[a];
+        g.origin[a] = (float)ctx->mesh_origin[a];
+        g.L[a] = (float)(nl[a] * ctx->mesh_spacing);
+        g.per[a] = 0;
+        g.tn[a] = (nl[a] + 3) / 4;
+    }
+    for (int f = 0; f < ctx->geo.n_faces; f++)
+        for (int a = 0; a < 3; a++)
+            if (ctx->geo.shift[3 * (size_t)f + a] != 0.0) {
+                g.per[a] = 1;
+                g.L[a] = (float)std::fabs(ctx->geo.shift[3 * (size_t)f + a]);
+            }
+    g.h = (float)ctx->mesh_spacing;
+    g.inv_h = (float)(1.0 / ctx->mesh_spacing);
+    g.n_bins = g.tn[0] * g.tn[1] * g.tn[2];
+    g.dt = (float)ctx->params.dt;
+    const int nc = ctx->lay.n_cells;
+    std::vector<float4> cen(nc);
+    for (int i = 0; i < nc; i++) {
+        const int c = ctx->lay.perm[i];
+        cen[i] = make_float4((float)ctx->geo.cc[3 * (size_t)c], (float)ctx->geo.cc[3 * (size_t)c + 1],
+                             (float)ctx->geo.cc[3 * (size_t)c + 2], (float)ctx->geo.vol[c]);
+    }
+    int *dinv, *bk, *br, *bc, *bs, *so, *tf;
+    float4 *dcen;
+    float *ws;
+    if ((rc = cp_alloc(ctx, &dinv, ctx->lay.inv.data(), nc))) return rc;
+    if ((rc = cp_alloc(ctx, &dcen, cen.data(), nc))) return rc;
+    if ((rc = cp_alloc<int>(ctx, &bk, nullptr, nn))) return rc;
+    if ((rc = cp_alloc<int>(ctx, &br, nullptr, nn))) return rc;
+    if ((rc = cp_alloc<int>(ctx, &bc, nullptr, g.n_bins))) return rc;
+    if ((rc = cp_alloc<int>(ctx, &bs, nullptr, g.n_bins + 1))) return rc;
+    if ((rc = cp_alloc<int>(ctx, &so, nullptr, nn))) return rc;
+    if ((rc = cp_alloc<int>(ctx, &tf, nullptr, g.n_bins))) return rc;
+    if ((rc = cp_alloc<float>(ctx, &ws, nullptr, nn))) return rc;
+    g.inv = dinv; g.cen = dcen; g.bin_key = bk; g.bin_rank = br; g.bin_count = bc; g.bin_start = bs;
+    g.sorted = so; g.tile_flag = tf; g.wsum_inv = ws;
+    g.static_force = ctx->static_force;
+    g.cell_force = ctx->cell_force;
+    HIPCHK(hipDeviceSynchronize());
+    ctx->have_membranes = true;
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_get_membrane_nodes(lbfs_ctx *ctx, double *X, double *U, double *F)
+{
+    if (!ctx) return LBFS_E_INVALID_ARG;
+    if (!ctx->have_membranes) return ctx->fail(LBFS_E_STATE, "no membranes attached");
+    const int nn = ctx->mem.n_nodes;
+    std::vector<float4> hx(nn), hu(nn), hf(nn);
+    std::vector<int> nm(nn);
+    HIPCHK(hipMemcpyAsync(hx.data(), ctx->mem.X, sizeof(float4) * nn, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hu.data(), ctx->mem.U, sizeof(float4) * nn, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hf.data(), ctx->mem.F, sizeof(float4) * nn, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    int mi = 0;
+    for (int n = 0; n < nn; n++) {
+        while (n >= ctx->mem_node_start[mi + 1]) mi++;
+        const double *o = &ctx->mem_origin[3 * (size_t)mi];
+        if (X) { X[3 * (size_t)n] = o[0] + hx[n].x; X[3 * (size_t)n + 1] = o[1] + hx[n].y; X[3 * (size_t)n + 2] = o[2] + hx[n].z; }
+        if (U) { U[3 * (size_t)n] = hu[n].x; U[3 * (size_t)n + 1] = hu[n].y; U[3 * (size_t)n + 2] = hu[n].z; }
+        if (F) { F[3 * (size_t)n] = hf[n].x; F[3 * (size_t)n + 1] = hf[n].y; F[3 * (size_t)n + 2] = hf[n].z; }
+    }
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_get_membrane_globals(lbfs_ctx *ctx, double *out6)
+{
+    if (!ctx || !out6) return LBFS_E_INVALID_ARG;
+    if (!ctx->have_membranes) return ctx->fail(LBFS_E_STATE, "no membranes attached");
+    const int nm = ctx->mem.n_mem;
+    std::vector<double> g(8 * (size_t)nm);
+    HIPCHK(hipMemcpyAsync(g.data(), ctx->mem.glob, sizeof(double) * g.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < nm; i++)
+        for (int q = 0; q < 6; q++) out6[6 * (size_t)i + q] = g[8 * (size_t)i + q];
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_mesh_icosphere(int frequency, double *nodes, int *tri)
+{
+    if (frequency < 1 || frequency > 64 || !nodes || !tri) return LBFS_E_INVALID_ARG;
+    std::vector<double> X;
+    std::vector<int> T;
+    icosphere(frequency, X, T);
+    std::copy(X.begin(), X.end(), nodes);
+    std::copy(T.begin(), T.end(), tri);
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_mesh_evans_fung(int n_nodes, const double *unit_nodes, double d0, double *nodes)
+{
+    if (n_nodes <= 0 || !unit_nodes || !nodes || !(d0 > 0.0)) return LBFS_E_INVALID_ARG;
+    std::vector<double> u(unit_nodes, unit_nodes + 3 * (size_t)n_nodes), out;
+    evans_fung(u, d0, out);
+    std::copy(out.begin(), out.end(), nodes);
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_mesh_springs(int n_nodes, int n_tri, const int *tri, int *springs, int *n_springs)
+{
+    if (n_nodes <= 0 || n_tri <= 0 || !tri || !springs || !n_springs) return LBFS_E_INVALID_ARG;
+    std::vector<int> T(tri, tri + 3 * (size_t)n_tri), S;
+    std::string err;
+    int rc = build_springs(n_nodes, T, S, err);
+    if (rc) return rc;
+    std::copy(S.begin(), S.end(), springs);
+    *n_springs = (int)S.size() / 4;
+    return LBFS_OK;
+}

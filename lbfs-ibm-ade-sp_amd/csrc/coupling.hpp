@@ -1,0 +1,60 @@
+// Device-side argument blocks and launchers of the membrane and IBM kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lbfs_abi.h"
+
+namespace lbfs {
+
+struct MemParamDev {
+    float spring_c;           // WLC constant making F(L0) = 0
+    float volume_modulus, local_area_modulus, global_area_modulus;
+    float global_bending, local_bending, viscosity;
+    int n_mad;                // edges summed for MAD
+};
+
+struct MemArgs {
+    int n_mem, n_nodes, n_tri, n_edges;
+    float4 *X;                // node position relative to the membrane origin
+    float4 *U;                // node velocity (interpolated)
+    float4 *F;                // node force
+    const int *node_mem;
+    const float4 *origin;     // per membrane
+    const int4 *tri;          // a, b, c, membrane
+    const int *mem_tri_start, *mem_edge_start;
+    const int4 *edges;        // a, b, c, d (edge a-c)
+    const int *edge_mem;
+    const float4 *edge_const; // contour length, spring constant, curvature_0, 0
+    const float2 *edge_area0; // |N1|, |N2| of the reference state
+    float4 *edge_state;       // curvature, |N1|, |N2|, 0 (feeds the MAD of the next step)
+    float4 *edge_force;       // 4 per edge: forces on a, b, c, d
+    const int *inc_start, *inc;  // node -> (edge * 4 + slot) incidences
+    const MemParamDev *params;
+    double *glob;             // 8 per membrane: V, A, MAD, V0, A0, MAD0, A_prev, 0
+};
+
+struct IbmArgs {
+    int kernel;               // LBFS_IBM_*
+    int n[3], per[3];
+    float origin[3], L[3], h, inv_h;
+    int tn[3], n_bins;        // 4x4x4 bins of the logical grid
+    float dt;
+    const int *inv;           // caller cell id (i + j nx + k nx ny) -> internal id
+    const float4 *cen;        // internal order: centroid xyz, volume
+    int *bin_key, *bin_rank, *bin_count, *bin_start, *sorted;
+    float *wsum_inv;          // per node: 1 / sum of cosine weights at the advected position
+    int *tile_flag;           // per bin tile: holds IBM force this step
+    const float *static_force;
+    float *cell_force;
+};
+
+hipError_t launch_ibm_interp(const MemArgs &m, const IbmArgs &g, const float4 *st, hipStream_t s);
+hipError_t launch_ibm_scan(const IbmArgs &g, hipStream_t s);
+hipError_t launch_ibm_place(const MemArgs &m, const IbmArgs &g, hipStream_t s);
+hipError_t launch_mem_globals(const MemArgs &m, hipStream_t s);
+hipError_t launch_mem_edges(const MemArgs &m, hipStream_t s);
+hipError_t launch_mem_nodes(const MemArgs &m, hipStream_t s);
+hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s);
+
+}  // namespace lbfs

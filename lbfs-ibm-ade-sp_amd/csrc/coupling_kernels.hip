@@ -1,0 +1,496 @@
+// Immersed-boundary coupling and spring-network membrane kernels for gfx950.
+//
+//   k_ibm_interp   per node: fluid velocity at the node (immersed_boundary_method.cu:14-97
+//                  or the 4-point cosine kernel), explicit advection X += U dt
+//                  (immersed_boundary_method.cu:234-264), bin key of the advected node
+//   k_ibm_scan     exclusive scan of the bin counts (one workgroup), counts cleared
+//   k_ibm_place    nodes placed in bin order
+//   k_mem_globals  per membrane (one workgroup): volume, area (FP64) and the MAD of
+//                  the previous force call (spring_network.cu:1186-1238,
+//                  lagrangian_object.cpp:601-650)
+//   k_mem_edges    per spring: all membrane force terms (spring_network.cu:613-971),
+//                  written per edge and slot (no atomics)
+//   k_mem_nodes    per node: sum of its edge slots in a fixed order
+//   k_ibm_spread   per 4x4x4 bin tile: each cell gathers the forces of the nodes
+//                  binned around it (immersed_boundary_method.cu:947-1024); the sum
+//                  is accumulated in 64-bit fixed point so its value does not depend
+//                  on the order the nodes are visited in
+#include <hip/hip_runtime.h>
+
+#include "coupling.hpp"
+
+namespace lbfs {
+
+namespace {
+
+constexpr double FIX_SCALE = 1099511627776.0;  // 2^40
+constexpr float PI_F = 3.14159265358979f;
+
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ __forceinline__ f3 ld3(float4 v) { return {v.x, v.y, v.z}; }
+__device__ __forceinline__ float cmp(f3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+// ---------------------------------------------------------------------------
+// IBM stencil helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int stencil_size(const IbmArgs &g) { return g.kernel == LBFS_IBM_LINEAR2 ? 2 : 4; }
+
+__device__ __forceinline__ void node_base(const IbmArgs &g, const float P[3], float xr[3], int base[3])
+{
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        xr[a] = (P[a] - g.origin[a]) * g.inv_h;
+        base[a] = g.kernel == LBFS_IBM_LINEAR2 ? (int)roundf(xr[a]) - 1 : (int)floorf(xr[a] - 1.5f);
+    }
+}
+
+// wrap a logical index into the grid; false if it lies outside a wall
+__device__ __forceinline__ bool wrap_index(const IbmArgs &g, int a, int i, int &w)
+{
+    const int n = g.n[a];
+    if (i >= 0 && i < n) { w = i; return true; }
+    if (!g.per[a]) return false;
+    w = ((i % n) + n) % n;
+    return true;
+}
+
+// unnormalised weight of the cell with unwrapped index idx / wrapped index w
+__device__ __forceinline__ float cell_weight(const IbmArgs &g, const int idx[3], const int w[3], int internal,
+                                             const float P[3], const float xr[3])
+{
+    float wt = 1.0f;
+    if (g.kernel == LBFS_IBM_LINEAR2) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) wt *= 1.0f - fabsf((float)idx[a] + 0.5f - xr[a]);
+    } else {
+        const float4 c = g.cen[internal];
+        const float cc[3] = {c.x, c.y, c.z};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float xc = cc[a] + (float)(idx[a] - w[a]) * (g.L[a] / (float)g.n[a]);
+            const float r = (xc - P[a]) * g.inv_h;
+            wt *= fabsf(r) < 2.0f ? 0.25f * (1.0f + cosf(PI_F * 0.5f * r)) : 0.0f;
+        }
+    }
+    return wt;
+}
+
+__device__ __forceinline__ int cell_internal(const IbmArgs &g, const int w[3])
+{
+    return g.inv[w[0] + g.n[0] * (w[1] + g.n[1] * w[2])];
+}
+
+__device__ __forceinline__ int bin_coord(const IbmArgs &g, int a, int base)
+{
+    const int n = g.n[a];
+    int b = base;
+    if (g.per[a]) b = ((b % n) + n) % n;
+    else b = b < 0 ? 0 : (b >= n ? n - 1 : b);
+    return b >> 2;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// IBM: interpolation, advection, binning
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const float4 *__restrict__ st)
+{
+    const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nd >= m.n_nodes) return;
+    const float4 o = m.origin[m.node_mem[nd]];
+    float4 xl = m.X[nd];
+    float P[3] = {o.x + xl.x, o.y + xl.y, o.z + xl.z};
+    float xr[3];
+    int base[3];
+    node_base(g, P, xr, base);
+    const int S = stencil_size(g);
+    float ux = 0.0f, uy = 0.0f, uz = 0.0f, ws = 0.0f;
+    for (int i = 0; i < S; i++)
+        for (int j = 0; j < S; j++)
+            for (int k = 0; k < S; k++) {
+                const int idx[3] = {base[0] + i, base[1] + j, base[2] + k};
+                int w[3];
+                if (!wrap_index(g, 0, idx[0], w[0]) || !wrap_index(g, 1, idx[1], w[1]) || !wrap_index(g, 2, idx[2], w[2]))
+                    continue;
+                const int ic = cell_internal(g, w);
+                const float wt = cell_weight(g, idx, w, ic, P, xr);
+                const float4 s = st[ic];
+                ux += wt * s.y; uy += wt * s.z; uz += wt * s.w;
+                ws += wt;
+            }
+    if (g.kernel == LBFS_IBM_COS4 && ws > 0.0f) {
+        const float iw = 1.0f / ws;
+        ux *= iw; uy *= iw; uz *= iw;
+    }
+    m.U[nd] = make_float4(ux, uy, uz, 0.0f);
+    xl.x += ux * g.dt; xl.y += uy * g.dt; xl.z += uz * g.dt;
+    m.X[nd] = xl;
+    // advected node: normalisation of its spreading weights and its bin
+    P[0] = o.x + xl.x; P[1] = o.y + xl.y; P[2] = o.z + xl.z;
+    node_base(g, P, xr, base);
+    if (g.kernel == LBFS_IBM_COS4) {
+        float s2 = 0.0f;
+        for (int i = 0; i < S; i++)
+            for (int j = 0; j < S; j++)
+                for (int k = 0; k < S; k++) {
+                    const int idx[3] = {base[0] + i, base[1] + j, base[2] + k};
+                    int w[3];
+                    if (!wrap_index(g, 0, idx[0], w[0]) || !wrap_index(g, 1, idx[1], w[1]) || !wrap_index(g, 2, idx[2], w[2]))
+                        continue;
+                    s2 += cell_weight(g, idx, w, cell_internal(g, w), P, xr);
+                }
+        g.wsum_inv[nd] = s2 > 0.0f ? 1.0f / s2 : 0.0f;
+    } else {
+        g.wsum_inv[nd] = 1.0f;
+    }
+    const int key = bin_coord(g, 0, base[0]) + g.tn[0] * (bin_coord(g, 1, base[1]) + g.tn[1] * bin_coord(g, 2, base[2]));
+    g.bin_key[nd] = key;
+    g.bin_rank[nd] = atomicAdd(&g.bin_count[key], 1);
+}
+
+__global__ __launch_bounds__(1024) void k_ibm_scan(IbmArgs g)
+{
+    __shared__ int part[1024];
+    const int nb = g.n_bins, per = (nb + 1023) / 1024;
+    const int b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
+    int s = 0;
+    for (int b = b0; b < b1; b++) s += g.bin_count[b];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    // Hillis-Steele inclusive scan of the 1024 partial sums
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0;
+    for (int b = b0; b < b1; b++) {
+        g.bin_start[b] = run;
+        run += g.bin_count[b];
+        g.bin_count[b] = 0;
+    }
+    if (threadIdx.x == 1023) g.bin_start[nb] = part[1023];
+}
+
+__global__ __launch_bounds__(256) void k_ibm_place(MemArgs m, IbmArgs g)
+{
+    const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nd >= m.n_nodes) return;
+    g.sorted[g.bin_start[g.bin_key[nd]] + g.bin_rank[nd]] = nd;
+}
+
+// ---------------------------------------------------------------------------
+// IBM: spreading, one 64-thread workgroup per 4x4x4 tile of cells
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_ibm_spread(MemArgs m, IbmArgs g)
+{
+    const int t = blockIdx.x;
+    const int tc[3] = {t % g.tn[0], (t / g.tn[0]) % g.tn[1], t / (g.tn[0] * g.tn[1])};
+    __shared__ int src_lo[8], src_hi[8];
+    __shared__ int total;
+    if (threadIdx.x < 8) {
+        const int d[3] = {(int)(threadIdx.x & 1) - 1, (int)((threadIdx.x >> 1) & 1) - 1, (int)(threadIdx.x >> 2) - 1};
+        int bc[3];
+        bool ok = true;
+        for (int a = 0; a < 3; a++) {
+            int v = tc[a] + d[a];
+            if (v < 0 || v >= g.tn[a]) {
+                if (!g.per[a] || g.tn[a] == 1) { ok = false; break; }
+                v = (v + g.tn[a]) % g.tn[a];
+            }
+            bc[a] = v;
+        }
+        int lo = 0, hi = 0;
+        if (ok) {
+            const int b = bc[0] + g.tn[0] * (bc[1] + g.tn[1] * bc[2]);
+            lo = g.bin_start[b];
+            hi = g.bin_start[b + 1];
+        }
+        src_lo[threadIdx.x] = lo;
+        src_hi[threadIdx.x] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int q = 0; q < 8; q++) s += src_hi[q] - src_lo[q];
+        total = s;
+    }
+    __syncthreads();
+    const int l[3] = {(int)(threadIdx.x & 3), (int)((threadIdx.x >> 2) & 3), (int)(threadIdx.x >> 4)};
+    const int w[3] = {4 * tc[0] + l[0], 4 * tc[1] + l[1], 4 * tc[2] + l[2]};
+    const bool valid = w[0] < g.n[0] && w[1] < g.n[1] && w[2] < g.n[2];
+    const int ic = valid ? cell_internal(g, w) : 0;
+    const int flag = g.tile_flag[t];
+    if (total == 0) {
+        if (flag) {
+            if (valid)
+                for (int a = 0; a < 3; a++) g.cell_force[3 * (size_t)ic + a] = g.static_force[3 * (size_t)ic + a];
+            __syncthreads();
+            if (threadIdx.x == 0) g.tile_flag[t] = 0;
+        }
+        return;
+    }
+    long long acc[3] = {0, 0, 0};
+    const int S = stencil_size(g);
+    if (valid) {
+        for (int q = 0; q < 8; q++)
+            for (int p = src_lo[q]; p < src_hi[q]; p++) {
+                const int nd = g.sorted[p];
+                const float4 o = m.origin[m.node_mem[nd]];
+                const float4 xl = m.X[nd];
+                const float P[3] = {o.x + xl.x, o.y + xl.y, o.z + xl.z};
+                float xr[3];
+                int base[3];
+                node_base(g, P, xr, base);
+                int idx[3];
+                bool in = true;
+                for (int a = 0; a < 3; a++) {
+                    int d = w[a] - base[a];
+                    if (g.per[a]) d = ((d % g.n[a]) + g.n[a]) % g.n[a];
+                    if (d < 0 || d >= S) { in = false; break; }
+                    idx[a] = base[a] + d;
+                }
+                if (!in) continue;
+                const float wt = cell_weight(g, idx, w, ic, P, xr) * g.wsum_inv[nd];
+                const float4 F = m.F[nd];
+                acc[0] += llrint((double)(F.x * wt) * FIX_SCALE);
+                acc[1] += llrint((double)(F.y * wt) * FIX_SCALE);
+                acc[2] += llrint((double)(F.z * wt) * FIX_SCALE);
+            }
+        const float sc = g.kernel == LBFS_IBM_LINEAR2 ? g.inv_h * g.inv_h * g.inv_h : 1.0f / g.cen[ic].w;
+        for (int a = 0; a < 3; a++)
+            g.cell_force[3 * (size_t)ic + a] = g.static_force[3 * (size_t)ic + a] + (float)((double)acc[a] / FIX_SCALE) * sc;
+    }
+    if (threadIdx.x == 0 && !flag) g.tile_flag[t] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// membrane
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mem_globals(MemArgs m)
+{
+    const int mem = blockIdx.x;
+    __shared__ double sh[3][256];
+    double V = 0.0, A = 0.0, M = 0.0;
+    for (int t = m.mem_tri_start[mem] + threadIdx.x; t < m.mem_tri_start[mem + 1]; t += blockDim.x) {
+        const int4 q = m.tri[t];
+        const f3 a = ld3(m.X[q.x]), b = ld3(m.X[q.y]), c = ld3(m.X[q.z]);
+        const f3 N = cross(b - a, c - a);
+        A += 0.5 * (double)sqrtf(dot(N, N));
+        V += (double)dot(N, (a + b + c) * (1.0f / 3.0f)) / 6.0;
+    }
+    const int e0 = m.mem_edge_start[mem];
+    const int e1 = e0 + m.params[mem].n_mad;
+    for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+        const float4 s = m.edge_state[e];
+        M += (double)s.x * ((double)s.y + (double)s.z) / 3.0;
+    }
+    sh[0][threadIdx.x] = V;
+    sh[1][threadIdx.x] = A;
+    sh[2][threadIdx.x] = M;
+    __syncthreads();
+    for (int stp = 128; stp > 0; stp >>= 1) {
+        if ((int)threadIdx.x < stp)
+            for (int q = 0; q < 3; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + stp];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double *gl = m.glob + 8 * (size_t)mem;
+        gl[2] = sh[2][0] / gl[6];  // MAD of the previous call, over the area it was computed with
+        gl[0] = sh[0][0];
+        gl[1] = sh[1][0];
+        gl[6] = sh[1][0];
+    }
+}
+
+namespace {
+// derivatives of N = (q - p) x (r - p) and of N . centroid for the nine
+// coordinates p.xyz q.xyz r.xyz: dN = e_k x (next - prev) (spring_network.cu:1348-1412)
+__device__ __forceinline__ void tri_derivs(f3 p, f3 q, f3 r, f3 N, f3 dN[9], float dV[9])
+{
+    const f3 v[3] = {p, q, r};
+    const f3 cen = (p + q + r) * (1.0f / 3.0f);
+#pragma unroll
+    for (int vt = 0; vt < 3; vt++) {
+        const f3 dd = v[(vt + 1) % 3] - v[(vt + 2) % 3];
+        const f3 dk[3] = {mk(0.0f, -dd.z, dd.y), mk(dd.z, 0.0f, -dd.x), mk(-dd.y, dd.x, 0.0f)};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            dN[3 * vt + k] = dk[k];
+            dV[3 * vt + k] = cmp(N, k) * (1.0f / 3.0f) + dot(dk[k], cen);
+        }
+    }
+}
+}  // namespace
+
+__global__ __launch_bounds__(128) void k_mem_edges(MemArgs m)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.n_edges) return;
+    const int mem = m.edge_mem[e];
+    const MemParamDev p = m.params[mem];
+    const double *gl = m.glob + 8 * (size_t)mem;
+    // global coefficients in FP64 (volume and area constraints, MAD)
+    const float vol_coef = (float)(-(double)p.volume_modulus * (gl[0] / gl[3] - 1.0) / 6.0);
+    const float area_coef = (float)(-(double)p.global_area_modulus * (gl[1] / gl[4] - 1.0));
+    const float mad_coef = (float)(-2.0 * (double)p.global_bending * (gl[2] - gl[5]) / gl[4]);
+    const int4 q = m.edges[e];
+    const f3 A = ld3(m.X[q.x]), B = ld3(m.X[q.y]), C = ld3(m.X[q.z]), D = ld3(m.X[q.w]);
+    const f3 l = C - A;
+    const float length = sqrtf(dot(l, l));
+    const f3 N1 = cross(B - A, C - A), N2 = cross(C - A, D - A);
+    const float n1 = sqrtf(dot(N1, N1)), n2 = sqrtf(dot(N2, N2));
+    f3 dN1[9], dN2[9];
+    float dV1[9], dV2[9];
+    tri_derivs(A, B, C, N1, dN1, dV1);
+    tri_derivs(C, D, A, N2, dN2, dV2);
+    const float in1 = 1.0f / n1, in2 = 1.0f / n2;
+    float dn1[9], dn2[9];
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+        dn1[j] = dot(dN1[j], N1) * in1;
+        dn2[j] = dot(dN2[j], N2) * in2;
+    }
+    // slots A(0-2) B(3-5) C(6-8) D(9-11); triangle 2 is (C, D, A)
+    float d_area[12], d_vol[12], d_ang[12];
+#pragma unroll
+    for (int j = 0; j < 9; j++) { d_area[j] = dn1[j]; d_vol[j] = dV1[j]; }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        d_area[9 + j] = dn2[3 + j];
+        d_vol[9 + j] = dV2[3 + j];
+        d_area[6 + j] += dn2[j];
+        d_vol[6 + j] += dV2[j];
+        d_area[j] += dn2[6 + j];
+        d_vol[j] += dV2[6 + j];
+    }
+#pragma unroll
+    for (int j = 0; j < 12; j++) { d_area[j] *= (1.0f / 6.0f); d_vol[j] *= (1.0f / 18.0f); }
+    float cs = dot(N1, N2) * in1 * in2;
+    cs = fminf(fmaxf(cs, -1.0f), 1.0f);
+    const f3 c1 = (A + B + C) * (1.0f / 3.0f), c2 = (A + C + D) * (1.0f / 3.0f);
+    const float dir = dot(N1, c2 - c1);
+    const float angle = acosf(cs) * (float)((dir < 0.0f) - (dir > 0.0f));
+    const float il = 1.0f / length;
+    const f3 dL = l * il;
+    {
+        const f3 h1 = cross(N1 * in1, dL), h2 = cross(dL, N2 * in2);
+        float a2[9];
+#pragma unroll
+        for (int j = 0; j < 9; j++) {
+            d_ang[j] = -in1 * dot(h1, dN1[j]);
+            a2[j] = -in2 * dot(h2, dN2[j]);
+        }
+        d_ang[9] = a2[3]; d_ang[10] = a2[4]; d_ang[11] = a2[5];
+        d_ang[6] += a2[0]; d_ang[7] += a2[1]; d_ang[8] += a2[2];
+        d_ang[0] += a2[6]; d_ang[1] += a2[7]; d_ang[2] += a2[8];
+    }
+    float d_curve[12];
+#pragma unroll
+    for (int j = 0; j < 12; j++) d_curve[j] = d_ang[j] * length;
+    d_curve[0] -= dL.x * angle; d_curve[1] -= dL.y * angle; d_curve[2] -= dL.z * angle;
+    d_curve[6] += dL.x * angle; d_curve[7] += dL.y * angle; d_curve[8] += dL.z * angle;
+    f3 Fo[4] = {mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
+    // local and global area: one opposite node per triangle
+    const float2 a0 = m.edge_area0[e];
+    const float ca = -p.local_area_modulus * (n1 / a0.x - 1.0f) + area_coef;
+    const float cb = -p.local_area_modulus * (n2 / a0.y - 1.0f) + area_coef;
+    Fo[1] = Fo[1] + mk(dn1[3], dn1[4], dn1[5]) * ca;
+    Fo[3] = Fo[3] + mk(dn2[3], dn2[4], dn2[5]) * cb;
+    // volume
+#pragma unroll
+    for (int j = 0; j < 4; j++) Fo[j] = Fo[j] + mk(d_vol[3 * j], d_vol[3 * j + 1], d_vol[3 * j + 2]) * vol_coef;
+    // membrane viscosity
+    {
+        const float4 va = m.U[q.x], vc = m.U[q.z];
+        const f3 vca = mk(vc.x - va.x, vc.y - va.y, vc.z - va.z);
+        const f3 vf = dL * (-dot(dL, vca) * 4.0f / 1.7320508075688772f * p.viscosity);
+        Fo[2] = Fo[2] + vf;
+        Fo[0] = Fo[0] - vf;
+    }
+    // bending
+    const float4 ec = m.edge_const[e];
+    const float kappa = angle * length / ((n1 + n2) * (1.0f / 6.0f));
+    const float dk0 = ec.z - kappa;
+    const float lbc = -0.5f * p.local_bending * dk0 * dk0;
+    const float ccf = -p.global_bending * (kappa - ec.z) + mad_coef;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        Fo[j] = Fo[j] + mk(d_area[3 * j], d_area[3 * j + 1], d_area[3 * j + 2]) * lbc;
+        Fo[j] = Fo[j] + mk(d_curve[3 * j], d_curve[3 * j + 1], d_curve[3 * j + 2]) * ccf;
+    }
+    // worm-like chain spring
+    {
+        const float x = length / ec.x;
+        const float om = 1.0f - x;
+        const float fc = -ec.y * (0.25f / (om * om) - 0.25f + x - p.spring_c);
+        Fo[2] = Fo[2] + dL * fc;
+        Fo[0] = Fo[0] - dL * fc;
+    }
+    m.edge_state[e] = make_float4(kappa, n1, n2, 0.0f);
+#pragma unroll
+    for (int j = 0; j < 4; j++) m.edge_force[4 * (size_t)e + j] = make_float4(Fo[j].x, Fo[j].y, Fo[j].z, 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_mem_nodes(MemArgs m)
+{
+    const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nd >= m.n_nodes) return;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    for (int j = m.inc_start[nd]; j < m.inc_start[nd + 1]; j++) {
+        const float4 f = m.edge_force[m.inc[j]];
+        fx += f.x; fy += f.y; fz += f.z;
+    }
+    m.F[nd] = make_float4(fx, fy, fz, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_ibm_interp(const MemArgs &m, const IbmArgs &g, const float4 *st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ibm_interp, dim3((m.n_nodes + 255) / 256), dim3(256), 0, s, m, g, st);
+    return hipGetLastError();
+}
+hipError_t launch_ibm_scan(const IbmArgs &g, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ibm_scan, dim3(1), dim3(1024), 0, s, g);
+    return hipGetLastError();
+}
+hipError_t launch_ibm_place(const MemArgs &m, const IbmArgs &g, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ibm_place, dim3((m.n_nodes + 255) / 256), dim3(256), 0, s, m, g);
+    return hipGetLastError();
+}
+hipError_t launch_mem_globals(const MemArgs &m, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mem_globals, dim3(m.n_mem), dim3(256), 0, s, m);
+    return hipGetLastError();
+}
+hipError_t launch_mem_edges(const MemArgs &m, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mem_edges, dim3((m.n_edges + 127) / 128), dim3(128), 0, s, m);
+    return hipGetLastError();
+}
+hipError_t launch_mem_nodes(const MemArgs &m, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mem_nodes, dim3((m.n_nodes + 255) / 256), dim3(256), 0, s, m);
+    return hipGetLastError();
+}
+hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ibm_spread, dim3(g.n_bins), dim3(64), 0, s, m, g);
+    return hipGetLastError();
+}
+
+}  // namespace lbfs

@@ -20,6 +20,47 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages()
     std::vector<Stage> s;
     lbfs_ctx *ctx = this;
     const double n = lay.n_cells, nf = geo.n_faces;
+    if (have_membranes) {
+        // IBM interpolation from the state at the start of the step and node
+        // advection, then membrane forces at the advected nodes and spreading
+        // (Solver_gpu.cu:808-915): X 12 r/w, U 12 w, per-node stencil data
+        const double nn = mem.n_nodes, ne = mem.n_edges, nt = mem.n_tri;
+        s.push_back({"ibm_interp", [ctx]() -> int {
+                         HIPCHK(launch_ibm_interp(ctx->mem, ctx->ibm, ctx->st[ctx->cur], ctx->stream));
+                         return LBFS_OK;
+                     },
+                     60.0 * nn});
+        s.push_back({"ibm_scan", [ctx]() -> int {
+                         HIPCHK(launch_ibm_scan(ctx->ibm, ctx->stream));
+                         return LBFS_OK;
+                     },
+                     8.0 * ibm.n_bins});
+        s.push_back({"ibm_place", [ctx]() -> int {
+                         HIPCHK(launch_ibm_place(ctx->mem, ctx->ibm, ctx->stream));
+                         return LBFS_OK;
+                     },
+                     16.0 * nn});
+        s.push_back({"mem_globals", [ctx]() -> int {
+                         HIPCHK(launch_mem_globals(ctx->mem, ctx->stream));
+                         return LBFS_OK;
+                     },
+                     32.0 * nt + 16.0 * ne});
+        s.push_back({"mem_edges", [ctx]() -> int {
+                         HIPCHK(launch_mem_edges(ctx->mem, ctx->stream));
+                         return LBFS_OK;
+                     },
+                     52.0 * ne});
+        s.push_back({"mem_nodes", [ctx]() -> int {
+                         HIPCHK(launch_mem_nodes(ctx->mem, ctx->stream));
+                         return LBFS_OK;
+                     },
+                     36.0 * nn});
+        s.push_back({"ibm_spread", [ctx]() -> int {
+                         HIPCHK(launch_ibm_spread(ctx->mem, ctx->ibm, ctx->stream));
+                         return LBFS_OK;
+                     },
+                     36.0 * 64.0 * nn / 16.0});
+    }
     // pass G: state 16 + stencil 24 + RHS 72 + LS matrix 36 + gradients written 48 (B per cell)
     s.push_back({"gradients", [ctx]() -> int {
                      HIPCHK(launch_gradients(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->stream));
@@ -46,8 +87,6 @@ int lbfs_ctx::enqueue_step()
     }
     return LBFS_OK;
 }
-
-void lbfs_ctx::release_extensions() {}
 
 extern "C" int lbfs_time_steps(lbfs_ctx *ctx, int nsteps, double *ms_total)
 {

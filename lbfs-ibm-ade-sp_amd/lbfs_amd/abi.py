@@ -47,6 +47,27 @@ class Diag(C.Structure):
     ]
 
 
+class MembraneDesc(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int), ("n_tri", C.c_int), ("nodes", _dp), ("nodes_stress_free", _dp),
+        ("nodes_curvature", _dp), ("tri", _ip),
+        ("shear_modulus", C.c_double), ("wlc_ratio", C.c_double), ("volume_modulus", C.c_double),
+        ("local_area_modulus", C.c_double), ("global_area_modulus", C.c_double),
+        ("global_bending_modulus", C.c_double), ("local_bending_modulus", C.c_double),
+        ("viscosity", C.c_double), ("curvature_ratio", C.c_double), ("volume_ratio", C.c_double),
+        ("sphere_reference", C.c_int), ("mad_compat", C.c_int),
+    ]
+
+
+MEMBRANE_PARAM_NAMES = ("shear_modulus", "wlc_ratio", "volume_modulus", "local_area_modulus", "global_area_modulus",
+                        "global_bending_modulus", "local_bending_modulus", "viscosity", "curvature_ratio",
+                        "volume_ratio", "sphere_reference", "mad_compat")
+
+# lattice-unit membrane parameters for a 20-cell RBC in nu = 1/6 flow
+RBC_PARAMS = dict(shear_modulus=5e-3, wlc_ratio=2.2, volume_modulus=0.1, local_area_modulus=5e-3,
+                  global_area_modulus=0.1, global_bending_modulus=1e-3, local_bending_modulus=0.0, viscosity=0.0,
+                  curvature_ratio=1.0, volume_ratio=1.0, sphere_reference=0, mad_compat=0)
+
 _lib = None
 
 
@@ -71,6 +92,12 @@ def load():
             "lbfs_stage_residual": (C.c_int, [vp, _dp]),
             "lbfs_reduce_diagnostics": (C.c_int, [vp, C.c_double, C.c_double, C.POINTER(Diag)]),
             "lbfs_get_cell_force": (C.c_int, [vp, _dp]),
+            "lbfs_add_membranes": (C.c_int, [vp, C.POINTER(MembraneDesc), C.c_int, C.c_int]),
+            "lbfs_get_membrane_nodes": (C.c_int, [vp, _dp, _dp, _dp]),
+            "lbfs_get_membrane_globals": (C.c_int, [vp, _dp]),
+            "lbfs_mesh_icosphere": (C.c_int, [C.c_int, _dp, _ip]),
+            "lbfs_mesh_evans_fung": (C.c_int, [C.c_int, _dp, C.c_double, _dp]),
+            "lbfs_mesh_springs": (C.c_int, [C.c_int, C.c_int, _ip, _ip, _ip]),
             "lbfs_time_steps": (C.c_int, [vp, C.c_int, _dp]),
             "lbfs_time_kernels": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_char_p), _dp, _ip]),
             "lbfs_model_bytes": (C.c_int, [vp, C.c_int, _dp, _ip]),
@@ -174,6 +201,68 @@ class HexMesh:
         return out
 
 
+def icosphere(frequency):
+    """Frequency-n geodesic unit sphere: (nodes [N,3], triangles [T,3])."""
+    n, t = 10 * frequency * frequency + 2, 20 * frequency * frequency
+    x = np.zeros(3 * n)
+    tri = np.zeros(3 * t, dtype=np.int32)
+    check(load().lbfs_mesh_icosphere(int(frequency), ptr(x), ptr(tri, _ip)))
+    return x.reshape(n, 3), tri.reshape(t, 3)
+
+
+def evans_fung(unit_nodes, d0):
+    u = np.ascontiguousarray(unit_nodes, dtype=np.float64).reshape(-1)
+    out = np.zeros_like(u)
+    check(load().lbfs_mesh_evans_fung(len(u) // 3, ptr(u), float(d0), ptr(out)))
+    return out.reshape(-1, 3)
+
+
+def springs(n_nodes, tri):
+    t = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1)
+    out = np.zeros(4 * (3 * (len(t) // 3) // 2 + 1), dtype=np.int32)
+    ns = C.c_int()
+    check(load().lbfs_mesh_springs(int(n_nodes), len(t) // 3, ptr(t, _ip), ptr(out, _ip), C.byref(ns)))
+    return out[: 4 * ns.value].reshape(-1, 4)
+
+
+def rotation(axis, angle):
+    a = np.asarray(axis, dtype=np.float64)
+    a = a / np.linalg.norm(a)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(angle) * K + (1 - np.cos(angle)) * K @ K
+
+
+def rbc(frequency=16, d0=20.0, center=(0.0, 0.0, 0.0), rot=None):
+    """Evans-Fung red cell on a frequency-n geodesic sphere, rotated and placed."""
+    u, tri = icosphere(frequency)
+    x = evans_fung(u, d0)
+    if rot is not None:
+        x = x @ np.asarray(rot).T
+    return x + np.asarray(center, dtype=np.float64), tri
+
+
+class Membrane:
+    """Host-side description of one membrane (keeps its arrays alive)."""
+
+    def __init__(self, nodes, tri, nodes_stress_free=None, nodes_curvature=None, **params):
+        self.nodes = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1)
+        self.tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1)
+        self.sf = None if nodes_stress_free is None else np.ascontiguousarray(nodes_stress_free, dtype=np.float64).reshape(-1)
+        self.cv = None if nodes_curvature is None else np.ascontiguousarray(nodes_curvature, dtype=np.float64).reshape(-1)
+        p = dict(RBC_PARAMS)
+        p.update(params)
+        self.params = p
+
+    def desc(self):
+        d = MembraneDesc()
+        d.n_nodes, d.n_tri = len(self.nodes) // 3, len(self.tri) // 3
+        d.nodes, d.tri = ptr(self.nodes), ptr(self.tri, _ip)
+        d.nodes_stress_free, d.nodes_curvature = ptr(self.sf), ptr(self.cv)
+        for k in MEMBRANE_PARAM_NAMES:
+            setattr(d, k, self.params[k])
+        return d
+
+
 def make_params(lattice=D3Q19, visc=1.0 / 6.0, dt=0.5, rho0=1.0, gamma=1.0, fneq_min=0.0, flags=0,
                 gradient_type=GRAD_HYBRID, body_force=(0.0, 0.0, 0.0)):
     p = Params()
@@ -236,6 +325,23 @@ class Solver:
         r = np.zeros(4 * self.n)
         check(load().lbfs_stage_residual(self.ctx, ptr(r)), self.ctx)
         return r
+
+    def add_membranes(self, membranes, ibm_kernel=IBM_COS4):
+        self._membranes = list(membranes)
+        arr = (MembraneDesc * len(self._membranes))(*[m.desc() for m in self._membranes])
+        self._mem_arr = arr
+        check(load().lbfs_add_membranes(self.ctx, arr, len(self._membranes), int(ibm_kernel)), self.ctx)
+        self.n_mem_nodes = sum(len(m.nodes) // 3 for m in self._membranes)
+
+    def membrane_nodes(self):
+        X, U, F = (np.zeros(3 * self.n_mem_nodes) for _ in range(3))
+        check(load().lbfs_get_membrane_nodes(self.ctx, ptr(X), ptr(U), ptr(F)), self.ctx)
+        return X.reshape(-1, 3), U.reshape(-1, 3), F.reshape(-1, 3)
+
+    def membrane_globals(self):
+        g = np.zeros(6 * len(self._membranes))
+        check(load().lbfs_get_membrane_globals(self.ctx, ptr(g)), self.ctx)
+        return g.reshape(-1, 6)
 
     def cell_force(self):
         f = np.zeros(3 * self.n)

@@ -224,3 +224,104 @@ class OracleFluid:
                                    ptr(self.flux), ptr(self.res))
             if rc != 0:
                 raise FloatingPointError("oracle state became non-finite")
+
+
+MEM_KEYS = ("shear_modulus", "wlc_ratio", "volume_modulus", "local_area_modulus", "global_area_modulus",
+            "global_bending_modulus", "local_bending_modulus", "viscosity", "curvature_ratio", "volume_ratio",
+            "sphere_reference", "mad_compat")
+
+
+def build_springs(n_nodes, tri):
+    t = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1)
+    out = np.zeros(4 * (3 * (len(t) // 3) // 2 + 1), dtype=np.int32)
+    ns = load().or_build_springs(int(n_nodes), len(t) // 3, ptr(t, _ip), ptr(out, _ip))
+    return out[: 4 * ns].reshape(-1, 4)
+
+
+class OracleMembrane:
+    """FP64 spring-network membrane (setup + per-step forces)."""
+
+    def __init__(self, nodes, tri, params, nodes_stress_free=None, nodes_curvature=None):
+        lib = load()
+        self.X = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1).copy()
+        self.tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1)
+        n = len(self.X) // 3
+        self.n = n
+        self.spring = build_springs(n, self.tri).reshape(-1)
+        ne = len(self.spring) // 4
+        self.ne = ne
+        self.arr = {k: np.zeros(ne) for k in ("contour_length", "spring_constant", "curvature_0", "spring_angle_0",
+                                               "curvature")}
+        self.arr["spring_area_0"] = np.zeros(2 * ne)
+        self.arr["spring_area"] = np.zeros(2 * ne)
+        mb = OrMembrane()
+        mb.n_nodes, mb.n_tri, mb.n_edges = n, len(self.tri) // 3, ne
+        mb.tri, mb.spring = ptr(self.tri, _ip), ptr(self.spring, _ip)
+        for k, a in self.arr.items():
+            setattr(mb, k, ptr(a))
+        self.mb = mb
+        mp = OrMemParams()
+        for k in MEM_KEYS:
+            setattr(mp, k, params[k])
+        self.mp = mp
+        sf = self.X if nodes_stress_free is None else np.ascontiguousarray(nodes_stress_free, dtype=np.float64).reshape(-1)
+        cv = sf if nodes_curvature is None else np.ascontiguousarray(nodes_curvature, dtype=np.float64).reshape(-1)
+        self.F = np.zeros(3 * n)
+        lib.or_membrane_setup(C.byref(mb), C.byref(mp), ptr(sf), ptr(cv), ptr(self.X), ptr(self.F))
+
+    def forces(self, X=None, V=None):
+        X = self.X if X is None else np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+        V = np.zeros(3 * self.n) if V is None else np.ascontiguousarray(V, dtype=np.float64).reshape(-1)
+        F = np.zeros(3 * self.n)
+        load().or_membrane_forces(C.byref(self.mb), C.byref(self.mp), ptr(X), ptr(V), 0, ptr(F))
+        return F.reshape(-1, 3)
+
+    def globals(self):
+        m = self.mb
+        return np.array([m.total_volume, m.total_area, m.mad, m.total_volume_0, m.total_area_0, m.mad_0])
+
+
+class OracleCoupled:
+    """FP64 coupled step in the order of Solver_gpu.cu:808-1021: interpolate
+    the start-of-step velocity to the nodes and advect them, membrane forces
+    at the advected nodes, spread onto the body force, fluid step."""
+
+    def __init__(self, fluid, mesh, membranes, ibm_kernel, body_force=(0.0, 0.0, 0.0)):
+        self.fluid = fluid
+        self.mems = membranes
+        self.kernel = ibm_kernel
+        g = OrGrid()
+        g.nx, g.ny, g.nz = mesh.nx, mesh.ny, mesh.nz
+        for a in range(3):
+            g.periodic[a] = int(mesh.periodic[a])
+            g.origin[a] = mesh.origin[a]
+            g.L[a] = mesh.lengths[a]
+        g.h = mesh.spacing
+        self.cc = fluid.cc[: 3 * fluid.nc].copy()
+        self.vol = fluid.vol.copy()
+        g.cc, g.vol = ptr(self.cc), ptr(self.vol)
+        self.g = g
+        self.body = np.asarray(body_force, dtype=np.float64)
+        self.last_force = None
+
+    def interpolate(self, X):
+        U = np.zeros_like(X)
+        load().or_ibm_interpolate(self.kernel, C.byref(self.g), len(X) // 3, ptr(X), ptr(self.fluid.st), ptr(U))
+        return U
+
+    def spread(self, X, F, f_cell):
+        load().or_ibm_spread(self.kernel, C.byref(self.g), len(X) // 3, ptr(X), ptr(F), ptr(f_cell))
+
+    def step(self, n=1):
+        nc = self.fluid.nc
+        for _ in range(n):
+            f_cell = np.tile(self.body, nc)
+            for m in self.mems:
+                U = self.interpolate(m.X)
+                m.X += U * self.fluid.p.dt
+                F = m.forces(m.X, U).reshape(-1)
+                m.F = F
+                m.U = U
+                self.spread(m.X, F, f_cell)
+            self.last_force = f_cell
+            self.fluid.step(1, force=f_cell)

@@ -1,0 +1,219 @@
+"""Pins of the FP64 membrane and IBM oracle (CPU only).
+
+The reference has no tests for spring_network.cu or the IBM kernels; the
+oracle is pinned by translation/rotation invariance, the zero-force reference
+state, forces equal to minus the gradient of the energies they derive from
+(finite differences), an independent dihedral-angle evaluation for the
+bending term, and partition of unity / conservation / adjointness of the IBM
+weights.
+"""
+import numpy as np
+import pytest
+
+import cases
+from lbfs_amd import abi
+from oracle import oracle as orc
+
+NO_BEND = dict(abi.RBC_PARAMS, global_bending_modulus=0.0, local_bending_modulus=0.0, viscosity=0.0)
+
+
+def small_rbc(freq=4, d0=8.0, seed=0, noise=0.0):
+    x, t = abi.rbc(freq, d0)
+    if noise:
+        x = x + noise * np.random.default_rng(seed).standard_normal(x.shape)
+    return x, t
+
+
+def test_springs_match_and_cover_every_edge_once():
+    for freq in (2, 4, 8):
+        x, t = abi.icosphere(freq)
+        sp = abi.springs(len(x), t)
+        assert np.array_equal(sp, orc.build_springs(len(x), t))
+        assert len(sp) * 2 == 3 * len(t)
+        edges = {tuple(sorted((a, c))) for a, b, c, d in sp}
+        assert len(edges) == len(sp)
+        tris = {tuple(np.roll(tt, -np.argmin(tt))) for tt in t}
+        for a, b, c, d in sp[:50]:
+            for tri in ((a, b, c), (a, c, d)):
+                assert tuple(np.roll(tri, -np.argmin(tri))) in tris
+
+
+def test_geodesic_counts_and_outward_winding():
+    for freq in (1, 3, 8, 11, 16):
+        x, t = abi.icosphere(freq)
+        assert len(x) == 10 * freq * freq + 2 and len(t) == 20 * freq * freq
+        assert np.allclose(np.linalg.norm(x, axis=1), 1.0)
+        N = np.cross(x[t[:, 1]] - x[t[:, 0]], x[t[:, 2]] - x[t[:, 0]])
+        assert np.all(np.einsum("ij,ij->i", N, x[t].mean(1)) > 0)
+
+
+def test_forces_are_translation_and_rotation_free():
+    x0, t = small_rbc()
+    params = dict(abi.RBC_PARAMS, local_bending_modulus=2e-3, viscosity=0.05)
+    m = orc.OracleMembrane(x0, t, params)
+    rng = np.random.default_rng(1)
+    X = x0 + 0.05 * rng.standard_normal(x0.shape)
+    V = 0.01 * rng.standard_normal(x0.shape)
+    F = m.forces(X, V)
+    assert np.abs(F.sum(0)).max() < 1e-13
+    assert np.abs(np.cross(X, F).sum(0)).max() < 1e-12
+
+
+def test_reference_state_is_force_free_without_bending():
+    x0, t = small_rbc()
+    m = orc.OracleMembrane(x0, t, NO_BEND)
+    assert np.abs(m.forces()).max() < 1e-14
+
+
+def _energy(m, X, params, n0):
+    """WLC + local area + global area + volume energies whose negative
+    gradients are the force terms of spring_network.cu:613-971."""
+    X = X.reshape(-1, 3)
+    sp = m.spring.reshape(-1, 4)
+    L = np.linalg.norm(X[sp[:, 2]] - X[sp[:, 0]], axis=1)
+    r = params["wlc_ratio"]
+    Cc = 0.25 * r * r / (r - 1) ** 2 - 0.25 + 1 / r
+    Lm, ks = m.arr["contour_length"], m.arr["spring_constant"]
+    x = L / Lm
+    e_wlc = np.sum(ks * Lm * (0.25 / (1 - x) - x / 4 + x * x / 2 - Cc * x))
+    t = m.tri.reshape(-1, 3)
+    N = np.cross(X[t[:, 1]] - X[t[:, 0]], X[t[:, 2]] - X[t[:, 0]])
+    n = np.linalg.norm(N, axis=1)
+    e_loc = np.sum(params["local_area_modulus"] * (n - n0) ** 2 / (2 * n0))
+    A, A0 = 0.5 * n.sum(), m.mb.total_area_0
+    e_glob = params["global_area_modulus"] * (A - A0) ** 2 / A0
+    V = np.sum(np.einsum("ij,ij->i", N, X[t].mean(1))) / 6.0
+    V0 = m.mb.total_volume_0
+    e_vol = params["volume_modulus"] * (V - V0) ** 2 / (12.0 * V0)
+    return e_wlc + e_loc + e_glob + e_vol
+
+
+def test_conservative_forces_are_energy_gradients():
+    x0, t = small_rbc(freq=3, d0=6.0)
+    params = dict(NO_BEND, volume_ratio=0.9)
+    m = orc.OracleMembrane(x0, t, params)
+    tt = t.reshape(-1, 3)
+    n0 = np.linalg.norm(np.cross(x0[tt[:, 1]] - x0[tt[:, 0]], x0[tt[:, 2]] - x0[tt[:, 0]]), axis=1)
+    X = x0 + 0.03 * np.random.default_rng(2).standard_normal(x0.shape)
+    F = m.forces(X).reshape(-1)
+    h = 1e-6
+    Xf = X.reshape(-1).copy()
+    for j in np.random.default_rng(3).choice(len(Xf), 25, replace=False):
+        xp, xm = Xf.copy(), Xf.copy()
+        xp[j] += h
+        xm[j] -= h
+        g = (_energy(m, xp, params, n0) - _energy(m, xm, params, n0)) / (2 * h)
+        assert abs(-g - F[j]) < 1e-7 + 1e-5 * abs(F[j]), (j, -g, F[j])
+
+
+def _dihedral(X, sp):
+    """Signed dihedral angle of each spring with the convention of
+    spring_network.cu:752-773 (convex positive)."""
+    A, B, Cn, D = (X[sp[:, k]] for k in range(4))
+    N1 = np.cross(B - A, Cn - A)
+    N2 = np.cross(Cn - A, D - A)
+    cs = np.clip(np.einsum("ij,ij->i", N1, N2) / (np.linalg.norm(N1, axis=1) * np.linalg.norm(N2, axis=1)), -1, 1)
+    d = np.einsum("ij,ij->i", N1, (A + Cn + D) / 3 - (A + B + Cn) / 3)
+    return np.arccos(cs) * ((d < 0).astype(float) - (d > 0).astype(float))
+
+
+def test_bending_force_is_gradient_of_theta_length_with_frozen_coefficients():
+    """F = -sum_e c_e d(theta_e L_e)/dX with c_e = k (kappa_e - kappa0_e) taken
+    at the evaluation point (the curvature term of spring_network.cu:899-916)."""
+    x0, t = small_rbc(freq=3, d0=6.0)
+    k = 2e-3
+    params = dict(NO_BEND, global_bending_modulus=k, shear_modulus=0.0, volume_modulus=0.0,
+                  local_area_modulus=0.0, global_area_modulus=0.0)
+    m = orc.OracleMembrane(x0, t, params)
+    X = x0 + 0.02 * np.random.default_rng(4).standard_normal(x0.shape)
+    F = m.forces(X).reshape(-1)
+    c = k * (m.arr["curvature"] - m.arr["curvature_0"])
+    sp = m.spring.reshape(-1, 4)
+
+    def G(xf):
+        Xr = xf.reshape(-1, 3)
+        L = np.linalg.norm(Xr[sp[:, 2]] - Xr[sp[:, 0]], axis=1)
+        return np.sum(c * _dihedral(Xr, sp) * L)
+
+    Xf = X.reshape(-1).copy()
+    h = 1e-6
+    for j in np.random.default_rng(5).choice(len(Xf), 25, replace=False):
+        xp, xm = Xf.copy(), Xf.copy()
+        xp[j] += h
+        xm[j] -= h
+        g = (G(xp) - G(xm)) / (2 * h)
+        assert abs(-g - F[j]) < 1e-9 + 1e-5 * abs(F[j]), (j, -g, F[j])
+
+
+def test_spring_constant_formula():
+    """k_s = 4 r L0 mu / (sqrt(3) (0.5 (1 - 1/r)^-3 + 1)), L_max = r L0
+    (spring_network.cu:516-604, 1600-1615)."""
+    x0, t = small_rbc()
+    p = abi.RBC_PARAMS
+    m = orc.OracleMembrane(x0, t, p)
+    sp = m.spring.reshape(-1, 4)
+    L0 = np.linalg.norm(x0[sp[:, 2]] - x0[sp[:, 0]], axis=1)
+    r = p["wlc_ratio"]
+    ks = 4 * r * L0 * p["shear_modulus"] / (np.sqrt(3) * (0.5 * (1 - 1 / r) ** -3 + 1))
+    np.testing.assert_allclose(m.arr["spring_constant"], ks, rtol=1e-14)
+    np.testing.assert_allclose(m.arr["contour_length"], r * L0, rtol=1e-14)
+
+
+def _ibm_setup(kernel, periodic=(1, 0, 0)):
+    mesh = abi.HexMesh(12, 10, 10, periodic=periodic, jitter=0.1, seed=8)
+    fl = orc.OracleFluid(mesh, orc.OR_D3Q19, visc=1 / 6, dt=0.25)
+    return mesh, fl, orc.OracleCoupled(fl, mesh, [], kernel)
+
+
+@pytest.mark.parametrize("kernel", [abi.IBM_LINEAR2, abi.IBM_COS4])
+def test_ibm_partition_of_unity_and_linear_exactness(kernel):
+    mesh, fl, cp = _ibm_setup(kernel)
+    rng = np.random.default_rng(6)
+    X = np.column_stack([rng.uniform(0, 12, 30), rng.uniform(3, 7, 30), rng.uniform(3, 7, 30)]).reshape(-1)
+    n = mesh.n_cells
+    # constant field: interpolated exactly by both kernels
+    fl.st[:] = 0.0
+    fl.st[1:4 * n:4] = 0.7
+    U = cp.interpolate(X).reshape(-1, 3)
+    np.testing.assert_allclose(U[:, 0], 0.7, rtol=1e-14)
+    if kernel == abi.IBM_LINEAR2:
+        # trilinear reproduces linear functions of the logical cell centres exactly
+        i = np.arange(n)
+        cx = (i % 12) + 0.5
+        fl.st[1:4 * n:4] = 0.3 + 0.1 * cx
+        U = cp.interpolate(X).reshape(-1, 3)
+        Xr = X.reshape(-1, 3)
+        inner = (Xr[:, 0] > 1.0) & (Xr[:, 0] < 11.0)
+        np.testing.assert_allclose(U[inner, 0], 0.3 + 0.1 * Xr[inner, 0], rtol=1e-13)
+
+
+def test_ibm_cos4_conserves_momentum_and_is_adjoint():
+    mesh, fl, cp = _ibm_setup(abi.IBM_COS4)
+    rng = np.random.default_rng(7)
+    X = np.column_stack([rng.uniform(0, 12, 20), rng.uniform(3, 7, 20), rng.uniform(3, 7, 20)]).reshape(-1)
+    F = rng.standard_normal(60)
+    f = np.zeros(3 * mesh.n_cells)
+    cp.spread(X, F, f)
+    np.testing.assert_allclose((f.reshape(-1, 3) * fl.vol[:, None]).sum(0), F.reshape(-1, 3).sum(0), rtol=1e-12)
+    # adjointness: sum_n U_n . F_n = sum_c u_c . f_c V_c
+    n = mesh.n_cells
+    u = rng.standard_normal((n, 3))
+    fl.st[:] = 0.0
+    for a in range(3):
+        fl.st[1 + a:4 * n:4] = u[:, a]
+    U = cp.interpolate(X)
+    lhs = np.dot(U, F)
+    rhs = np.sum(u * f.reshape(-1, 3) * fl.vol[:, None])
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+def test_ibm_periodic_wrap_is_seamless():
+    """A node near x = 0 and its periodic image near x = L interpolate alike."""
+    mesh, fl, cp = _ibm_setup(abi.IBM_COS4)
+    n = mesh.n_cells
+    c, _ = cases.centroids(mesh)
+    fl.st[:] = 0.0
+    fl.st[1:4 * n:4] = np.cos(2 * np.pi * c[:, 0] / 12.0)
+    X1 = np.array([0.3, 5.0, 5.0])
+    X2 = np.array([12.3, 5.0, 5.0])
+    assert abs(cp.interpolate(X1)[0] - cp.interpolate(X2)[0]) < 1e-14
