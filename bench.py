@@ -52,6 +52,17 @@ def cpu_model():
     return "unknown"
 
 
+def timed(solver, steps, warmup):
+    """Wall time per step of `steps` whole steps after `warmup`, synchronised
+    on both sides (inputs resident in HBM)."""
+    solver.step(warmup)
+    solver.sync()
+    t0 = time.perf_counter()
+    solver.step(steps)
+    solver.sync()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,6 +70,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU design is documented, not built); "
@@ -66,41 +78,61 @@ def main():
         sys.exit(2)
 
     from lbfs_amd import abi, workloads
-    print("bench: building workload", file=sys.stderr, flush=True)
+    log = lambda m: print("bench: " + m, file=sys.stderr, flush=True)  # noqa: E731
+
+    # --- config 2: 128x64x64 channel, one 2562-node RBC (the headline step) ---
+    log("building the 128x64x64 channel")
     mesh, params, state = workloads.channel()
+    n = mesh.n_cells
+    plain = abi.Solver(mesh, params)
+    plain.set_state(*state)
+    ms_plain = timed(plain, args.steps, args.warmup)
+    plain.close()
+    log("plain fluid step %.4f ms" % ms_plain)
+
     solver = abi.Solver(mesh, params)
     solver.set_state(*state)
-    n = mesh.n_cells
-    print("bench: warmup %d steps" % args.warmup, file=sys.stderr, flush=True)
-    solver.step(args.warmup)
-    solver.sync()
-
-    # timed region: K whole steps, inputs resident in HBM, synchronised on both sides
-    solver.sync()
-    t0 = time.perf_counter()
-    solver.step(args.steps)
-    solver.sync()
-    wall = time.perf_counter() - t0
-    ev_ms = solver.time_steps(args.steps)
-    ms_per_step = wall * 1e3 / args.steps
-    mlups = n * args.steps / wall / 1e6
-    print("bench: %d steps in %.3f s (events %.3f ms/step)" % (args.steps, wall, ev_ms / args.steps),
-          file=sys.stderr, flush=True)
-
-    # dominant kernel: modelled bytes per launch / average launch time
+    solver.add_membranes(workloads.single_rbc(mesh), ibm_kernel=abi.IBM_COS4)
+    ms_step = timed(solver, args.steps, args.warmup)
+    mlups = n / (ms_step * 1e-3) / 1e6
+    ev_ms = solver.time_steps(args.steps) / args.steps
+    log("RBC step %.4f ms (events %.4f ms)" % (ms_step, ev_ms))
     kms = solver.time_kernels(min(args.steps, 50))
     names = list(kms.keys())
     mbytes = solver.model_bytes()
     step_bytes = float(np.sum(mbytes))
     dom = max(range(len(names)), key=lambda i: kms[names[i]])
     achieved = mbytes[dom] / (kms[names[dom]] * 1e-3) / 1e9
-    step_gbps = step_bytes / (ms_per_step * 1e-3) / 1e9
+    step_gbps = step_bytes / (ms_step * 1e-3) / 1e9
     diag = solver.diagnostics(ref_rho=1.0, ref_u=0.01)
+    mglob = solver.membrane_globals()[0]
     solver.close()
+
+    # --- config 3: 256x128x128 suspension of 256 RBCs of mixed resolution ---
+    batched = None
+    if not args.no_batched:
+        log("building the 256-cell suspension")
+        m3, p3, s3, mems, d0 = workloads.suspension()
+        b_plain = abi.Solver(m3, p3)
+        b_plain.set_state(*s3)
+        ms3_plain = timed(b_plain, max(args.steps // 4, 10), 5)
+        b_plain.close()
+        b = abi.Solver(m3, p3)
+        b.set_state(*s3)
+        b.add_membranes(mems, ibm_kernel=abi.IBM_COS4)
+        ms3 = timed(b, max(args.steps // 4, 10), 5)
+        k3 = b.time_kernels(10)
+        b.close()
+        nodes = sum(len(mm.nodes) // 3 for mm in mems)
+        batched = {"workload": "256x128x128 jittered hex, 256 Evans-Fung cells (d0 %.1f, %d nodes), 20%% hematocrit, "
+                               "D3Q19" % (d0, nodes),
+                   "ms_per_step": ms3, "plain_ms_per_step": ms3_plain, "ratio_vs_plain": ms3 / ms3_plain,
+                   "mlups": m3.n_cells / (ms3 * 1e-3) / 1e6, "kernels_ms": k3}
+        log("suspension step %.4f ms (plain %.4f ms)" % (ms3, ms3_plain))
 
     cpu = None
     if not args.no_cpu_baseline:
-        print("bench: CPU baseline (FP64 oracle, %d steps)" % args.cpu_steps, file=sys.stderr, flush=True)
+        log("CPU baseline (FP64 oracle, %d steps)" % args.cpu_steps)
         cpu_mlups, cpu_s = cpu_baseline(mesh, params, args.cpu_steps)
         cpu = {"value": cpu_mlups, "unit": "MLUPS", "cores": 1, "kind": "port",
                "sample": "%d fluid steps of the same 128x64x64 D3Q19 mesh, FP64, single thread, %.1f s, host %s"
@@ -113,22 +145,28 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic (seeded jittered hex mesh, rest state + body force)",
-        "config": {"workload": "channel 128x64x64 jittered hex (+-10%, seed 2), x-periodic, walls y/z, "
-                               "D3Q19, body-force Poiseuille, fluid step",
+        "data": "synthetic (seeded jittered hex mesh, rest state + body force, seeded RBC shapes)",
+        "config": {"workload": "channel 128x64x64 jittered hex (+-10%, seed 2), x-periodic, walls y/z, D3Q19, "
+                               "body-force Poiseuille, one Evans-Fung RBC 2562 nodes / 5120 triangles, 4-point IBM",
                    "cells": n, "faces": mesh.n_faces, "lattice": "D3Q19"},
         "achieved_hbm_gbps_model": step_gbps,
+        "plain_ms_per_step": ms_plain,
+        "plain_mlups": n / (ms_plain * 1e-3) / 1e6,
+        "rbc_step_vs_plain": ms_step / ms_plain,
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
                      "model_bytes_per_launch": mbytes[dom], "avg_launch_ms": kms[names[dom]]},
         "kernels_ms": kms,
-        "event_ms_per_step": ev_ms / args.steps,
+        "event_ms_per_step": ev_ms,
         "max_speed": diag["max_speed"],
+        "membrane_volume_rel_change": mglob[0] / mglob[3] - 1.0,
+        "batched": batched,
+        "adjoint": None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -113,6 +113,10 @@ int lbfs_get_state(lbfs_ctx *ctx, double *rho, double *u, double *v, double *w);
 /* whole time steps: the time loop body of Solver_gpu.cu:749-1246 */
 int lbfs_step(lbfs_ctx *ctx, int nsteps);
 
+/* options (no counterpart in the reference) */
+#define LBFS_OPT_GRAPH 1 /* 1 (default): replay the step from a captured hipGraph, 0: launch kernels one by one */
+int lbfs_set_option(lbfs_ctx *ctx, int option, int value);
+
 /* per-stage entry points (parity tests) */
 int lbfs_stage_gradients(lbfs_ctx *ctx, double *grad /* [12 n_cells] or NULL */);   /* LBFS.cu:2177-2240 */
 int lbfs_stage_face_flux(lbfs_ctx *ctx, double *flux /* [4 n_faces] */);           /* LBFS.cu:2574-2810 */

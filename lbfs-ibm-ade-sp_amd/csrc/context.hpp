@@ -41,8 +41,14 @@ struct lbfs_ctx {
     std::vector<double> mem_origin;
     std::vector<void *> coupling_allocs;
 
+    // hipGraph of two consecutive steps starting from cur == 0
+    bool use_graph = true;
+    hipGraphExec_t graph_exec = nullptr;
+    void drop_graph();
+
     int fail(int code, const char *fmt, ...);
     int enqueue_step();
+    int enqueue_steps(int nsteps);
     void release_extensions();
 
     // The step as an ordered list of kernel stages (each enqueues one launch

@@ -47,6 +47,7 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
         (long long)ctx->mesh_nx * ctx->mesh_ny * ctx->mesh_nz != ctx->lay.n_cells)
         return ctx->fail(LBFS_E_UNSUPPORTED, "membranes need a mesh with a logical i,j,k structure and spacing");
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->drop_graph();
     ctx->release_extensions();
     // restore the force field without IBM contributions
     HIPCHK(hipMemcpy(ctx->cell_force, ctx->static_force, sizeof(float) * 3 * (size_t)ctx->lay.n_cells,

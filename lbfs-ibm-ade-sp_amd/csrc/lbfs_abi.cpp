@@ -174,7 +174,8 @@ extern "C" int lbfs_create(const lbfs_mesh_desc *mesh, const lbfs_params *params
 extern "C" int lbfs_destroy(lbfs_ctx *ctx)
 {
     if (!ctx) return LBFS_E_INVALID_ARG;
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->drop_graph();
     ctx->release_extensions();
     for (void *p : ctx->allocs) hipFree(p);
     ctx->allocs.clear();
@@ -221,14 +222,74 @@ extern "C" int lbfs_get_state(lbfs_ctx *ctx, double *rho, double *u, double *v, 
     return LBFS_OK;
 }
 
-extern "C" int lbfs_step(lbfs_ctx *ctx, int nsteps)
+void lbfs_ctx::drop_graph()
 {
-    if (!ctx || nsteps < 0) return LBFS_E_INVALID_ARG;
-    for (int s = 0; s < nsteps; s++) {
-        int rc = ctx->enqueue_step();
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    graph_exec = nullptr;
+}
+
+// capture two consecutive steps (state ping-pong back to buffer 0) into a graph
+static int capture_graph(lbfs_ctx *ctx)
+{
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = ctx->enqueue_step();
+    if (!rc) rc = ctx->enqueue_step();
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return ctx->fail(LBFS_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        ctx->graph_exec = nullptr;
+        return ctx->fail(LBFS_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    return LBFS_OK;
+}
+
+int lbfs_ctx::enqueue_steps(int nsteps)
+{
+    lbfs_ctx *ctx = this;
+    int s = 0;
+    if (use_graph && nsteps >= 2) {
+        if (cur != 0) {
+            int rc = enqueue_step();
+            if (rc) return rc;
+            s++;
+        }
+        if (!graph_exec) {
+            const int saved = cur;
+            int rc = capture_graph(this);
+            cur = saved;  // capture only recorded the launches
+            if (rc) return rc;
+        }
+        for (; s + 2 <= nsteps; s += 2) HIPCHK(hipGraphLaunch(graph_exec, stream));
+    }
+    for (; s < nsteps; s++) {
+        int rc = enqueue_step();
         if (rc) return rc;
     }
     return LBFS_OK;
+}
+
+extern "C" int lbfs_step(lbfs_ctx *ctx, int nsteps)
+{
+    if (!ctx || nsteps < 0) return LBFS_E_INVALID_ARG;
+    return ctx->enqueue_steps(nsteps);
+}
+
+extern "C" int lbfs_set_option(lbfs_ctx *ctx, int option, int value)
+{
+    if (!ctx) return LBFS_E_INVALID_ARG;
+    if (option == LBFS_OPT_GRAPH) {
+        ctx->use_graph = value != 0;
+        ctx->drop_graph();
+        return LBFS_OK;
+    }
+    return ctx->fail(LBFS_E_INVALID_ARG, "unknown option %d", option);
 }
 
 extern "C" int lbfs_stage_gradients(lbfs_ctx *ctx, double *grad_out)

@@ -95,10 +95,8 @@ extern "C" int lbfs_time_steps(lbfs_ctx *ctx, int nsteps, double *ms_total)
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
     HIPCHK(hipEventRecord(a, ctx->stream));
-    for (int s = 0; s < nsteps; s++) {
-        int rc = ctx->enqueue_step();
-        if (rc) return rc;
-    }
+    int rc = ctx->enqueue_steps(nsteps);
+    if (rc) return rc;
     HIPCHK(hipEventRecord(b, ctx->stream));
     HIPCHK(hipEventSynchronize(b));
     float ms = 0.0f;

@@ -16,6 +16,7 @@ GRAD_HYBRID, GRAD_LS, GRAD_GG = 0, 1, 2
 BC_DIRICHLET, BC_NEUMANN = 1, 2
 FLAG_UPWIND_BAND, FLAG_NEUMANN_SIGNED, FLAG_GHOST_DIR_GPU, FLAG_LS_DET_COMPAT, FLAG_CLOSED_FACES = 1, 2, 4, 16, 32
 IBM_LINEAR2, IBM_COS4 = 0, 1
+OPT_GRAPH = 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -92,6 +93,7 @@ def load():
             "lbfs_stage_residual": (C.c_int, [vp, _dp]),
             "lbfs_reduce_diagnostics": (C.c_int, [vp, C.c_double, C.c_double, C.POINTER(Diag)]),
             "lbfs_get_cell_force": (C.c_int, [vp, _dp]),
+            "lbfs_set_option": (C.c_int, [vp, C.c_int, C.c_int]),
             "lbfs_add_membranes": (C.c_int, [vp, C.POINTER(MembraneDesc), C.c_int, C.c_int]),
             "lbfs_get_membrane_nodes": (C.c_int, [vp, _dp, _dp, _dp]),
             "lbfs_get_membrane_globals": (C.c_int, [vp, _dp]),
@@ -325,6 +327,10 @@ class Solver:
         r = np.zeros(4 * self.n)
         check(load().lbfs_stage_residual(self.ctx, ptr(r)), self.ctx)
         return r
+
+    def set_graph(self, on):
+        """Replay steps from a captured hipGraph (default) or launch eagerly."""
+        check(load().lbfs_set_option(self.ctx, OPT_GRAPH, 1 if on else 0), self.ctx)
 
     def add_membranes(self, membranes, ibm_kernel=IBM_COS4):
         self._membranes = list(membranes)

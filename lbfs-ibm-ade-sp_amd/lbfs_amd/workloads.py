@@ -36,6 +36,56 @@ def channel(nx=128, ny=64, nz=64, jitter=0.1, seed=2, lattice=abi.D3Q19, u_max=0
     return mesh, params, state
 
 
+def single_rbc(mesh, frequency=16, d0=20.0, params=None):
+    """One Evans-Fung cell (2562 nodes / 5120 triangles at frequency 16) at the
+    channel centre, symmetry axis along z."""
+    c = mesh.origin + 0.5 * mesh.lengths
+    x, tri = abi.rbc(frequency, d0, center=c)
+    return [abi.Membrane(x, tri, **(params or {}))]
+
+
+# geodesic frequencies of the three resolutions: 642 (f=8), 1212 (f=11) and
+# 2562 (f=16) nodes; 1280 is not a geodesic node count, 1212 is the nearest
+SUSPENSION_FREQS = (8, 11, 16)
+
+
+def suspension(nx=256, ny=128, nz=128, n_cells=256, hematocrit=0.2, seed=3, jitter=0.1, lattice=abi.D3Q19,
+               u_max=0.01, params=None):
+    """Batched suspension: n_cells Evans-Fung cells of seeded mixed resolution
+    in a jittered channel at the given hematocrit.  Cells sit in an 8 x 4 x 8
+    arrangement of slots (one cell per slot, non-overlapping by construction)
+    with seeded position offsets, in-plane rotation and a tilt up to 0.1 rad."""
+    mesh = abi.HexMesh(nx, ny, nz, periodic=(1, 0, 0), jitter=jitter, seed=seed)
+    f = duct_body_force(u_max, ny, nz, CHANNEL_VISC)
+    pr = abi.make_params(lattice=lattice, visc=CHANNEL_VISC, dt=CHANNEL_DT, rho0=1.0, body_force=(f, 0.0, 0.0))
+    # diameter giving the hematocrit: the Evans-Fung volume scales as d0^3
+    u, t = abi.icosphere(16)
+    x20 = abi.evans_fung(u, 20.0)
+    tri = t
+    N = np.cross(x20[tri[:, 1]] - x20[tri[:, 0]], x20[tri[:, 2]] - x20[tri[:, 0]])
+    v20 = np.sum(np.einsum("ij,ij->i", N, x20[tri].mean(1))) / 6.0
+    d0 = 20.0 * (hematocrit * nx * ny * nz / n_cells / v20) ** (1.0 / 3.0)
+    slots = (8, 4, 8)
+    if slots[0] * slots[1] * slots[2] < n_cells:
+        raise ValueError("too many cells for the slot arrangement")
+    sz = np.array([nx / slots[0], ny / slots[1], nz / slots[2]])
+    rng = np.random.default_rng(seed)
+    mems = []
+    for c in range(n_cells):
+        i, j, k = c % slots[0], (c // slots[0]) % slots[1], c // (slots[0] * slots[1])
+        freq = SUSPENSION_FREQS[rng.integers(0, 3)]
+        rot = abi.rotation((0, 0, 1), rng.uniform(0, 2 * np.pi)) @ abi.rotation(
+            (1, 0, 0), rng.uniform(-0.1, 0.1))
+        x, tri = abi.rbc(freq, d0, rot=rot)
+        ext = x.max(0) - x.min(0)
+        slack = np.maximum(sz - ext - 4.0, 0.0)  # keep 2 cells between neighbours and walls
+        centre = (np.array([i, j, k]) + 0.5) * sz + rng.uniform(-0.5, 0.5, 3) * slack
+        mems.append(abi.Membrane(x + centre, tri, **(params or {})))
+    n = mesh.n_cells
+    state = (np.ones(n), np.zeros(n), np.zeros(n), np.zeros(n))
+    return mesh, pr, state, mems, d0
+
+
 def tgv_2d(n=64, jitter=0.05, seed=1, u0=0.01, re=100.0, lattice=abi.D2Q9, dt=0.25):
     mesh = abi.HexMesh(n, n, 1, periodic=(1, 1, 1), jitter=jitter, seed=seed)
     visc = u0 * n / re
