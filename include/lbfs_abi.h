@@ -154,6 +154,8 @@ typedef struct {
     double curvature_ratio, volume_ratio;
     int sphere_reference;             /* 1: curvature_0 = curvature_ratio / 3.265483301 */
     int mad_compat;                   /* 1: MAD over the first n_nodes edges (Solver_gpu.cu:2355) */
+    double internal_viscosity_ratio;  /* interior / exterior viscosity; 1 leaves the field at 1
+                                         (Solver_gpu.cu:884-889, lagrangian_object.h:85) */
 } lbfs_membrane_desc;
 
 /* attach membranes (replaces any attached before); requires the mesh's logical

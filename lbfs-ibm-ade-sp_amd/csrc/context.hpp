@@ -40,6 +40,9 @@ struct lbfs_ctx {
     std::vector<int> mem_node_start;
     std::vector<double> mem_origin;
     std::vector<void *> coupling_allocs;
+    float *visc_factor = nullptr;   // [n] internal viscosity field, null while every ratio is 1
+    float delta_h = 0.1f;           // smoothing width of the field (unstructured_mesh.cpp:263-264)
+    long long t_step = 0;           // steps since the membranes were attached
 
     // hipGraph of two consecutive steps starting from cur == 0
     bool use_graph = true;
@@ -58,5 +61,8 @@ struct lbfs_ctx {
         std::function<int()> run;
         double model_bytes;
     };
-    std::vector<Stage> step_stages();
+    std::vector<Stage> step_stages(bool with_visc = false);
+    // the internal viscosity field is rebuilt at t = 1 (mod 10000) (Solver_gpu.cu:884-889)
+    bool visc_due(long long t) const { return visc_factor && (t % 10000) == 1; }
+    int run_stages(bool with_visc);
 };

@@ -36,6 +36,8 @@ void lbfs_ctx::release_extensions()
     for (void *p : coupling_allocs) (void)hipFree(p);
     coupling_allocs.clear();
     have_membranes = false;
+    visc_factor = nullptr;
+    args.visc_factor = nullptr;
 }
 
 extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, int count, int ibm_kernel)
@@ -103,6 +105,7 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
         p.local_bending = (float)mem[i].local_bending_modulus;
         p.viscosity = (float)mem[i].viscosity;
         p.n_mad = mem[i].mad_compat ? std::min(s.n_nodes, s.n_edges) : s.n_edges;
+        p.visc_ratio = mem[i].internal_viscosity_ratio > 0.0 ? (float)mem[i].internal_viscosity_ratio : 1.0f;
         double *gl = &glob[8 * (size_t)i];
         gl[0] = s.V; gl[1] = s.A; gl[2] = s.MAD; gl[3] = s.V0; gl[4] = s.A0; gl[5] = s.MAD0; gl[6] = s.A;
     }
@@ -149,6 +152,12 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     if ((rc = cp_alloc(ctx, &dinc, inc.data(), inc.size()))) return rc;
     if ((rc = cp_alloc(ctx, &dpar, params.data(), count))) return rc;
     if ((rc = cp_alloc(ctx, &dgl, glob.data(), glob.size()))) return rc;
+    int *dns;
+    float4 *dbb;
+    if ((rc = cp_alloc(ctx, &dns, mem_node_start.data(), mem_node_start.size()))) return rc;
+    if ((rc = cp_alloc<float4>(ctx, &dbb, nullptr, 2 * (size_t)count))) return rc;
+    m.mem_node_start = dns;
+    m.bbox = dbb;
     m.X = dX; m.U = dU; m.F = dF; m.node_mem = dnm; m.origin = dorig; m.tri = dtri;
     m.mem_tri_start = dts; m.mem_edge_start = des0; m.edges = dedg; m.edge_mem = dem;
     m.edge_const = dec; m.edge_area0 = dea; m.edge_state = des; m.edge_force = def;
@@ -205,6 +214,19 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     g.sorted = so; g.tile_flag = tf; g.wsum_inv = ws;
     g.static_force = ctx->static_force;
     g.cell_force = ctx->cell_force;
+    g.n_cells = nc;
+    // internal viscosity field, only when some membrane has a ratio other than 1
+    bool any_ratio = false;
+    for (int i = 0; i < count; i++) any_ratio |= params[i].visc_ratio != 1.0f;
+    if (any_ratio) {
+        std::vector<float> ones(nc, 1.0f);
+        float *vf;
+        if ((rc = cp_alloc(ctx, &vf, ones.data(), nc))) return rc;
+        ctx->visc_factor = vf;
+        ctx->args.visc_factor = vf;
+    }
+    ctx->delta_h = (float)std::min(0.5 * ctx->mesh_spacing, 0.1);
+    ctx->t_step = 0;
     HIPCHK(hipDeviceSynchronize());
     ctx->have_membranes = true;
     return LBFS_OK;

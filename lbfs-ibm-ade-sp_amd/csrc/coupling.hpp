@@ -12,6 +12,7 @@ struct MemParamDev {
     float volume_modulus, local_area_modulus, global_area_modulus;
     float global_bending, local_bending, viscosity;
     int n_mad;                // edges summed for MAD
+    float visc_ratio;         // internal viscosity ratio (1 = off)
 };
 
 struct MemArgs {
@@ -22,7 +23,7 @@ struct MemArgs {
     const int *node_mem;
     const float4 *origin;     // per membrane
     const int4 *tri;          // a, b, c, membrane
-    const int *mem_tri_start, *mem_edge_start;
+    const int *mem_tri_start, *mem_edge_start, *mem_node_start;
     const int4 *edges;        // a, b, c, d (edge a-c)
     const int *edge_mem;
     const float4 *edge_const; // contour length, spring constant, curvature_0, 0
@@ -32,10 +33,12 @@ struct MemArgs {
     const int *inc_start, *inc;  // node -> (edge * 4 + slot) incidences
     const MemParamDev *params;
     double *glob;             // 8 per membrane: V, A, MAD, V0, A0, MAD0, A_prev, 0
+    float4 *bbox;             // 2 per membrane: lower and upper corner of the nodes
 };
 
 struct IbmArgs {
     int kernel;               // LBFS_IBM_*
+    int n_cells;
     int n[3], per[3];
     float origin[3], L[3], h, inv_h;
     int tn[3], n_bins;        // 4x4x4 bins of the logical grid
@@ -56,5 +59,7 @@ hipError_t launch_mem_globals(const MemArgs &m, hipStream_t s);
 hipError_t launch_mem_edges(const MemArgs &m, hipStream_t s);
 hipError_t launch_mem_nodes(const MemArgs &m, hipStream_t s);
 hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s);
+// internal viscosity field (immersed_boundary_method.cu:389-527)
+hipError_t launch_internal_visc(const MemArgs &m, const IbmArgs &g, float delta_h, float *visc_factor, hipStream_t s);
 
 }  // namespace lbfs

@@ -494,3 +494,94 @@ hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s)
 }
 
 }  // namespace lbfs
+
+namespace lbfs {
+
+// ---------------------------------------------------------------------------
+// internal viscosity (immersed_boundary_method.cu:389-527, Solver_gpu.cu:2153-2208)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mem_bbox(MemArgs m)
+{
+    const int *mem_node_start = m.mem_node_start;
+    const int mem = blockIdx.x;
+    __shared__ float lo[3][256], hi[3][256];
+    float l[3] = {1e30f, 1e30f, 1e30f}, h[3] = {-1e30f, -1e30f, -1e30f};
+    const float4 o = m.origin[mem];
+    for (int n = mem_node_start[mem] + threadIdx.x; n < mem_node_start[mem + 1]; n += blockDim.x) {
+        const float4 x = m.X[n];
+        const float p[3] = {o.x + x.x, o.y + x.y, o.z + x.z};
+        for (int a = 0; a < 3; a++) { l[a] = fminf(l[a], p[a]); h[a] = fmaxf(h[a], p[a]); }
+    }
+    for (int a = 0; a < 3; a++) { lo[a][threadIdx.x] = l[a]; hi[a][threadIdx.x] = h[a]; }
+    __syncthreads();
+    for (int stp = 128; stp > 0; stp >>= 1) {
+        if ((int)threadIdx.x < stp)
+            for (int a = 0; a < 3; a++) {
+                lo[a][threadIdx.x] = fminf(lo[a][threadIdx.x], lo[a][threadIdx.x + stp]);
+                hi[a][threadIdx.x] = fmaxf(hi[a][threadIdx.x], hi[a][threadIdx.x + stp]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        m.bbox[2 * mem] = make_float4(lo[0][0], lo[1][0], lo[2][0], 0.0f);
+        m.bbox[2 * mem + 1] = make_float4(hi[0][0], hi[1][0], hi[2][0], 0.0f);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_internal_visc(MemArgs m, IbmArgs g, float delta_h, float *visc_factor)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n_cells) return;
+    const float4 c = g.cen[i];
+    float out = 1.0f;
+    bool any = false;
+    for (int mem = 0; mem < m.n_mem; mem++) {
+        const float r = m.params[mem].visc_ratio;
+        if (r == 1.0f) continue;
+        const float4 lo = m.bbox[2 * mem], hi = m.bbox[2 * mem + 1];
+        const float xl = lo.x - 2.0f * delta_h, xh = hi.x + 2.0f * delta_h;
+        if (c.x < xl || c.x > xh || c.y < lo.y - 2.0f * delta_h || c.y > hi.y + 2.0f * delta_h ||
+            c.z < lo.z - 2.0f * delta_h || c.z > hi.z + 2.0f * delta_h)
+            continue;
+        const float4 o = m.origin[mem];
+        const float pc0 = c.x * c.y - xh * c.y, pc1 = c.x * c.z - xh * c.z, pc2 = c.x - xh;
+        int count = 0;
+        float dmin = 1e9f;
+        for (int t = m.mem_tri_start[mem]; t < m.mem_tri_start[mem + 1]; t++) {
+            const int4 q = m.tri[t];
+            const int ids[3] = {q.x, q.y, q.z};
+            float p[3][3];
+            for (int k = 0; k < 3; k++) {
+                const float4 x = m.X[ids[k]];
+                p[k][0] = o.x + x.x; p[k][1] = o.y + x.y; p[k][2] = o.z + x.z;
+            }
+            if (!(p[0][0] > c.x && p[1][0] > c.x && p[2][0] > c.x)) continue;
+            float side[3];
+            for (int k = 0; k < 3; k++) {
+                const float *u = p[k], *v = p[(k + 1) % 3];
+                const float e0 = u[1] * v[2] - v[1] * u[2], e1 = u[2] - v[2], e2 = v[1] - u[1];
+                side[k] = pc0 * e1 + pc1 * e2 + pc2 * e0;
+            }
+            if ((side[0] < 0 && side[1] < 0 && side[2] < 0) || (side[0] > 0 && side[1] > 0 && side[2] > 0)) count++;
+            for (int k = 0; k < 3; k++) {
+                const float dx = c.x - p[k][0], dy = c.y - p[k][1], dz = c.z - p[k][2];
+                dmin = fminf(dmin, sqrtf(dx * dx + dy * dy + dz * dz));
+            }
+        }
+        const float factor = 2.0f * fminf(0.5f, dmin * 0.5f / delta_h + 1.0f / PI_F * sinf(PI_F * dmin / delta_h) * 0.5f);
+        const float v = 0.5f * ((count & 1) ? (r + 1.0f) + (r - 1.0f) * factor : (r + 1.0f) + (1.0f - r) * factor);
+        // several membranes: keep the value furthest from 1
+        if (!any || fabsf(v - 1.0f) > fabsf(out - 1.0f)) out = v;
+        any = true;
+    }
+    visc_factor[i] = out;
+}
+
+hipError_t launch_internal_visc(const MemArgs &m, const IbmArgs &g, float delta_h, float *visc_factor, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mem_bbox, dim3(m.n_mem), dim3(256), 0, s, m);
+    hipLaunchKernelGGL(k_internal_visc, dim3((g.n_cells + 255) / 256), dim3(256), 0, s, m, g, delta_h, visc_factor);
+    return hipGetLastError();
+}
+
+}  // namespace lbfs

@@ -233,8 +233,8 @@ static int capture_graph(lbfs_ctx *ctx)
 {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int rc = ctx->enqueue_step();
-    if (!rc) rc = ctx->enqueue_step();
+    int rc = ctx->run_stages(false);
+    if (!rc) rc = ctx->run_stages(false);
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
@@ -266,7 +266,24 @@ int lbfs_ctx::enqueue_steps(int nsteps)
             cur = saved;  // capture only recorded the launches
             if (rc) return rc;
         }
-        for (; s + 2 <= nsteps; s += 2) HIPCHK(hipGraphLaunch(graph_exec, stream));
+        // graph pairs, falling back to single eager steps where the internal
+        // viscosity field is due
+        while (s + 2 <= nsteps) {
+            if (visc_due(t_step) || visc_due(t_step + 1)) {
+                int rc = enqueue_step();
+                if (rc) return rc;
+                s++;
+                if (cur != 0) {
+                    rc = enqueue_step();
+                    if (rc) return rc;
+                    s++;
+                }
+                continue;
+            }
+            HIPCHK(hipGraphLaunch(graph_exec, stream));
+            t_step += 2;
+            s += 2;
+        }
     }
     for (; s < nsteps; s++) {
         int rc = enqueue_step();

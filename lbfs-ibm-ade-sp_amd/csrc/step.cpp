@@ -15,7 +15,7 @@ using namespace lbfs;
         if (e_ != hipSuccess) return ctx->fail(LBFS_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
-std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages()
+std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
 {
     std::vector<Stage> s;
     lbfs_ctx *ctx = this;
@@ -55,6 +55,12 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages()
                          return LBFS_OK;
                      },
                      36.0 * nn});
+        if (with_visc)
+            s.push_back({"internal_visc", [ctx]() -> int {
+                             HIPCHK(launch_internal_visc(ctx->mem, ctx->ibm, ctx->delta_h, ctx->visc_factor, ctx->stream));
+                             return LBFS_OK;
+                         },
+                         8.0 * n});
         s.push_back({"ibm_spread", [ctx]() -> int {
                          HIPCHK(launch_ibm_spread(ctx->mem, ctx->ibm, ctx->stream));
                          return LBFS_OK;
@@ -79,13 +85,20 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages()
     return s;
 }
 
-int lbfs_ctx::enqueue_step()
+int lbfs_ctx::run_stages(bool with_visc)
 {
-    for (auto &st : step_stages()) {
+    for (auto &st : step_stages(with_visc)) {
         int rc = st.run();
         if (rc) return rc;
     }
     return LBFS_OK;
+}
+
+int lbfs_ctx::enqueue_step()
+{
+    int rc = run_stages(visc_due(t_step));
+    if (!rc) t_step++;
+    return rc;
 }
 
 extern "C" int lbfs_time_steps(lbfs_ctx *ctx, int nsteps, double *ms_total)

@@ -56,18 +56,19 @@ class MembraneDesc(C.Structure):
         ("local_area_modulus", C.c_double), ("global_area_modulus", C.c_double),
         ("global_bending_modulus", C.c_double), ("local_bending_modulus", C.c_double),
         ("viscosity", C.c_double), ("curvature_ratio", C.c_double), ("volume_ratio", C.c_double),
-        ("sphere_reference", C.c_int), ("mad_compat", C.c_int),
+        ("sphere_reference", C.c_int), ("mad_compat", C.c_int), ("internal_viscosity_ratio", C.c_double),
     ]
 
 
 MEMBRANE_PARAM_NAMES = ("shear_modulus", "wlc_ratio", "volume_modulus", "local_area_modulus", "global_area_modulus",
                         "global_bending_modulus", "local_bending_modulus", "viscosity", "curvature_ratio",
-                        "volume_ratio", "sphere_reference", "mad_compat")
+                        "volume_ratio", "sphere_reference", "mad_compat", "internal_viscosity_ratio")
 
 # lattice-unit membrane parameters for a 20-cell RBC in nu = 1/6 flow
 RBC_PARAMS = dict(shear_modulus=5e-3, wlc_ratio=2.2, volume_modulus=0.1, local_area_modulus=5e-3,
                   global_area_modulus=0.1, global_bending_modulus=1e-3, local_bending_modulus=0.0, viscosity=0.0,
-                  curvature_ratio=1.0, volume_ratio=1.0, sphere_reference=0, mad_compat=0)
+                  curvature_ratio=1.0, volume_ratio=1.0, sphere_reference=0, mad_compat=0,
+                  internal_viscosity_ratio=1.0)
 
 _lib = None
 

@@ -102,3 +102,61 @@ int or_ibm_spread(int kernel, const or_grid *g, int n_nodes, const double *X, co
     }
     return 0;
 }
+
+/*
+ * Internal viscosity field: update_interior_viscosities_plucker
+ * (immersed_boundary_method.cu:389-527) driven by gpu_solver::update_viscosities
+ * (Solver_gpu.cu:2153-2208).  Cells whose centroid lies in the node bounding box
+ * widened by 2 delta_h cast a ray in +x; every triangle with all three nodes at
+ * larger x is tested with Plucker side operators (immersed_boundary_method.cu:
+ * 357-384); odd crossings = inside.  factor = 2 min(0.5, d/(2 delta_h) +
+ * sin(pi d / delta_h) / (2 pi)) with d the distance to the nearest tested node;
+ * visc = ((r + 1) +- (r - 1) factor) / 2.  Cells outside the box keep 1.
+ * Deviation: cell 0 is not skipped (the reference tests i > 0).
+ */
+void or_internal_viscosity(int n_cells, const double *cc, int n_nodes, const double *X, int n_tri, const int *tri,
+                           double ratio, double delta_h, double *visc_factor)
+{
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    int i, t, a;
+    for (i = 0; i < n_nodes; i++)
+        for (a = 0; a < 3; a++) {
+            if (X[3 * i + a] < lo[a]) lo[a] = X[3 * i + a];
+            if (X[3 * i + a] > hi[a]) hi[a] = X[3 * i + a];
+        }
+    for (a = 0; a < 3; a++) { lo[a] -= 2.0 * delta_h; hi[a] += 2.0 * delta_h; }
+    for (i = 0; i < n_cells; i++) {
+        const double *c = cc + 3 * i;
+        double pc[3], dmin = 1e9, factor, visc;
+        int count = 0;
+        if (c[0] < lo[0] || c[0] > hi[0] || c[1] < lo[1] || c[1] > hi[1] || c[2] < lo[2] || c[2] > hi[2]) {
+            visc_factor[i] = 1.0;
+            continue;
+        }
+        /* Plucker coordinates of the ray from the centroid to x = hi[0] */
+        pc[0] = c[0] * c[1] - hi[0] * c[1];
+        pc[1] = c[0] * c[2] - hi[0] * c[2];
+        pc[2] = c[0] - hi[0];
+        for (t = 0; t < n_tri; t++) {
+            const double *p[3] = {X + 3 * tri[3 * t], X + 3 * tri[3 * t + 1], X + 3 * tri[3 * t + 2]};
+            double side[3];
+            int k;
+            if (!(p[0][0] > c[0] && p[1][0] > c[0] && p[2][0] > c[0])) continue;
+            for (k = 0; k < 3; k++) {
+                const double *u = p[k], *v = p[(k + 1) % 3];
+                double e0 = u[1] * v[2] - v[1] * u[2], e1 = u[2] - v[2], e2 = v[1] - u[1];
+                side[k] = pc[0] * e1 + pc[1] * e2 + pc[2] * e0;
+            }
+            if ((side[0] < 0 && side[1] < 0 && side[2] < 0) || (side[0] > 0 && side[1] > 0 && side[2] > 0)) count++;
+            for (k = 0; k < 3; k++) {
+                double d = sqrt((c[0] - p[k][0]) * (c[0] - p[k][0]) + (c[1] - p[k][1]) * (c[1] - p[k][1]) +
+                                (c[2] - p[k][2]) * (c[2] - p[k][2]));
+                if (d < dmin) dmin = d;
+            }
+        }
+        factor = 2.0 * fmin(0.5, dmin * 0.5 / delta_h + 1.0 / OR_PI * sin(OR_PI * dmin / delta_h) * 0.5);
+        if (count % 2) visc = (ratio + 1.0) + (ratio - 1.0) * factor;
+        else visc = (ratio + 1.0) + (1.0 - ratio) * factor;
+        visc_factor[i] = visc * 0.5;
+    }
+}

@@ -99,6 +99,8 @@ def load():
         lib.or_membrane_forces.argtypes = [mbp, mpp, _dp, _dp, C.c_int, _dp]
         lib.or_membrane_globals.restype = None
         lib.or_membrane_globals.argtypes = [mbp, _dp, _dp, _dp]
+        lib.or_internal_viscosity.restype = None
+        lib.or_internal_viscosity.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, _ip, C.c_double, C.c_double, _dp]
         _lib = lib
     return _lib
 
@@ -243,6 +245,7 @@ class OracleMembrane:
 
     def __init__(self, nodes, tri, params, nodes_stress_free=None, nodes_curvature=None):
         lib = load()
+        self.visc_ratio = float(params.get("internal_viscosity_ratio", 1.0))
         self.X = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1).copy()
         self.tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1)
         n = len(self.X) // 3
@@ -303,6 +306,25 @@ class OracleCoupled:
         self.g = g
         self.body = np.asarray(body_force, dtype=np.float64)
         self.last_force = None
+        self.t = 0
+        self.delta_h = min(0.5 * mesh.spacing, 0.1)
+        self.ratios = [getattr(m, "visc_ratio", 1.0) for m in membranes]
+        self.visc = None
+
+    def internal_viscosity(self):
+        """Field of or_internal_viscosity; with several membranes the value
+        furthest from 1 wins."""
+        nc = self.fluid.nc
+        out = np.ones(nc)
+        for m, r in zip(self.mems, self.ratios):
+            if r == 1.0:
+                continue
+            vf = np.zeros(nc)
+            load().or_internal_viscosity(nc, ptr(self.cc), m.n, ptr(m.X), len(m.tri) // 3, ptr(m.tri, _ip), r,
+                                         self.delta_h, ptr(vf))
+            take = np.abs(vf - 1.0) > np.abs(out - 1.0)
+            out[take] = vf[take]
+        return out
 
     def interpolate(self, X):
         U = np.zeros_like(X)
@@ -322,6 +344,12 @@ class OracleCoupled:
                 F = m.forces(m.X, U).reshape(-1)
                 m.F = F
                 m.U = U
-                self.spread(m.X, F, f_cell)
+            if self.t % 10000 == 1 and any(r != 1.0 for r in self.ratios):
+                self.visc = self.internal_viscosity()
+                self.fluid.visc_factor = self.visc
+                self.fluid.m.visc_factor = ptr(self.visc)
+            for m in self.mems:
+                self.spread(m.X, m.F, f_cell)
             self.last_force = f_cell
             self.fluid.step(1, force=f_cell)
+            self.t += 1

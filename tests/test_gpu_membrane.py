@@ -13,16 +13,17 @@ VISC, DT = 1.0 / 6.0, 0.25
 PARAMS = dict(abi.RBC_PARAMS, local_bending_modulus=1e-3, viscosity=0.02, volume_ratio=0.95)
 
 
-def coupled_pair(kernel, steps, freq=4, d0=8.0, body=(2e-5, 0.0, 0.0), seed=21, lattice=(abi.D3Q19, orc.OR_D3Q19)):
+def coupled_pair(kernel, steps, freq=4, d0=8.0, body=(2e-5, 0.0, 0.0), seed=21, lattice=(abi.D3Q19, orc.OR_D3Q19),
+                 params=PARAMS):
     mesh = cases.channel_mesh(24, 16, 16, seed=seed)
     x0, tri = abi.rbc(freq, d0, center=(12.0, 8.0, 8.0), rot=abi.rotation((1, 1, 0), 0.4))
     s = abi.Solver(mesh, abi.make_params(lattice=lattice[0], visc=VISC, dt=DT, body_force=body))
     st = cases.smooth_state(mesh, seed=seed, amp=0.005)
     s.set_state(*st)
-    s.add_membranes([abi.Membrane(x0, tri, **PARAMS)], ibm_kernel=kernel)
+    s.add_membranes([abi.Membrane(x0, tri, **params)], ibm_kernel=kernel)
     fl = orc.OracleFluid(mesh, lattice[1], visc=VISC, dt=DT)
     fl.set_state(*st)
-    mo = orc.OracleMembrane(x0, tri, PARAMS)
+    mo = orc.OracleMembrane(x0, tri, params)
     cp = orc.OracleCoupled(fl, mesh, [mo], kernel, body_force=body)
     s.step(steps)
     cp.step(steps)
@@ -59,6 +60,19 @@ def test_coupled_step_parity(kernel):
     assert pos_err < 1e-5, pos_err
     assert disp_err < 1e-4, disp_err
     assert vel_err < 1e-5, vel_err
+
+
+def test_internal_viscosity_field_and_step():
+    """Viscosity ratio 5 inside the cell: the field built at t = 1
+    (Solver_gpu.cu:884-889) and the following steps match the oracle."""
+    p = dict(PARAMS, internal_viscosity_ratio=5.0)
+    s, cp, mo, x0 = coupled_pair(abi.IBM_COS4, 6, params=p)
+    assert cp.visc is not None and np.any(cp.visc > 1.0)
+    rs, us, vs, ws = s.get_state()
+    ro, uo, vo, wo = cp.fluid.state()
+    assert cases.rel_l2(np.concatenate([us, vs, ws]), np.concatenate([uo, vo, wo])) < 1e-5
+    X, _, _ = s.membrane_nodes()
+    assert cases.rel_l2(X - x0, mo.X.reshape(-1, 3) - x0) < 1e-4
 
 
 def test_coupled_step_bitwise_repeatable():

@@ -217,3 +217,20 @@ def test_ibm_periodic_wrap_is_seamless():
     X1 = np.array([0.3, 5.0, 5.0])
     X2 = np.array([12.3, 5.0, 5.0])
     assert abs(cp.interpolate(X1)[0] - cp.interpolate(X2)[0]) < 1e-14
+
+
+def test_internal_viscosity_marks_interior():
+    """Plucker ray parity (immersed_boundary_method.cu:389-527): cells well
+    inside a closed membrane get the ratio, cells in the box but outside get 1."""
+    mesh = abi.HexMesh(16, 16, 16, periodic=(0, 0, 0), jitter=0.1, seed=9)
+    fl = orc.OracleFluid(mesh, orc.OR_D3Q19, visc=1 / 6, dt=0.25)
+    u, t = abi.icosphere(6)
+    x = 5.0 * u + 8.0
+    m = orc.OracleMembrane(x, t, dict(abi.RBC_PARAMS, internal_viscosity_ratio=5.0))
+    cp = orc.OracleCoupled(fl, mesh, [m], abi.IBM_COS4)
+    vf = cp.internal_viscosity()
+    c = fl.cc[: 3 * mesh.n_cells].reshape(-1, 3)
+    r = np.linalg.norm(c - 8.0, axis=1)
+    assert np.all(vf[r < 4.0] == 5.0)
+    assert np.all(vf[r > 6.0] == 1.0)
+    assert np.all((vf >= 1.0) & (vf <= 5.0))
