@@ -194,6 +194,18 @@ int lbfs_mesh_hex_generate(int nx, int ny, int nz, const double *lengths /* [3] 
                            const int *periodic /* [3] */, double jitter, uint64_t seed,
                            double *points, int *face_vertices, int *owner, int *neighbour,
                            double *face_shift, int *patch);
+/* OpenFOAM polyMesh ASCII files in `dir` (points, faces, owner, neighbour,
+ * boundary): import_openfoam_mesh (unstructured_mesh.cpp:670-941) and
+ * import_openfoam_BC_tags (unstructured_mesh.cpp:2533-2608).  Quad faces only.
+ * face_patch: patch index of each face (-1 for internal faces). */
+int lbfs_polymesh_counts(const char *dir, int *n_points, int *n_cells, int *n_faces, int *n_internal, int *n_patches);
+int lbfs_polymesh_read(const char *dir, double *points, int *face_vertices, int *owner, int *neighbour, int *face_patch,
+                       double *face_shift /* [3 n_faces] or NULL; periodic translations from dir/lbfsFaceShift */);
+int lbfs_polymesh_patch(const char *dir, int index, char *name, int name_len, char *type, int type_len);
+int lbfs_polymesh_write(const char *dir, int n_points, const double *points, int n_cells, int n_faces, int n_internal,
+                        const int *face_vertices, const int *owner, const int *neighbour, int n_patches,
+                        const char *const *patch_names, const char *const *patch_types, const int *patch_start,
+                        const int *patch_size, const double *face_shift /* or NULL */);
 /* FP64 geometry of a mesh (unstructured_mesh.cpp:1592-1692) */
 int lbfs_mesh_geometry(const lbfs_mesh_desc *mesh, double visc, double gamma, double fneq_min, int flags,
                        double *area, double *normal, double *face_centroid, double *face_dt,

@@ -109,6 +109,11 @@ def load():
                                                  _dp, _ip, _ip, _ip, _dp, _ip]),
             "lbfs_mesh_geometry": (C.c_int, [C.POINTER(MeshDesc), C.c_double, C.c_double, C.c_double, C.c_int,
                                              _dp, _dp, _dp, _dp, _dp, _dp]),
+            "lbfs_polymesh_counts": (C.c_int, [C.c_char_p, _ip, _ip, _ip, _ip, _ip]),
+            "lbfs_polymesh_read": (C.c_int, [C.c_char_p, _dp, _ip, _ip, _ip, _ip, _dp]),
+            "lbfs_polymesh_patch": (C.c_int, [C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int]),
+            "lbfs_polymesh_write": (C.c_int, [C.c_char_p, C.c_int, _dp, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip,
+                                              C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), _ip, _ip, _dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -134,8 +139,15 @@ def check(rc, ctx=None):
         raise LbfsError("lbfs call failed (%d): %s" % (rc, msg.decode() if msg else ""))
 
 
+PATCH_NAMES = ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")
+
+
 class HexMesh:
-    """Seeded jittered hex box in polyMesh form (generated by the native helper)."""
+    """Seeded jittered hex box in polyMesh form (generated by the native helper),
+    or a hexahedral OpenFOAM polyMesh read from disk (read_polymesh)."""
+
+    patch_names = PATCH_NAMES
+    patch_types = ("wall",) * 6
 
     def __init__(self, nx, ny, nz, lengths=None, origin=(0.0, 0.0, 0.0), periodic=(1, 0, 0), jitter=0.1, seed=2):
         lib = load()
@@ -158,13 +170,78 @@ class HexMesh:
                                          float(jitter), int(seed) & (2 ** 64 - 1), ptr(self.points),
                                          ptr(self.face_vertices, _ip), ptr(self.owner, _ip), ptr(self.neighbour, _ip),
                                          ptr(self.face_shift), ptr(self.patch, _ip)))
+        self._default_bc()
+
+    def _default_bc(self):
         self.bc_patch = self.patch[self.n_internal:].copy()
         # default: no-slip Dirichlet velocity, Neumann density on every wall patch
         self.bc_rho_type = np.full(max(self.n_bc, 1), BC_NEUMANN, dtype=np.int32)
         self.bc_vel_type = np.full(max(self.n_bc, 1), BC_DIRICHLET, dtype=np.int32)
         self.bc_values = np.zeros(4 * max(self.n_bc, 1))
         self.bc_values[0::4] = 1.0
-        self.spacing = float(self.lengths[0] / nx)
+        self.spacing = float(self.lengths[0] / self.nx) if self.nx else 1.0
+
+    @classmethod
+    def read_polymesh(cls, directory, nx=0, ny=0, nz=0, origin=(0.0, 0.0, 0.0), lengths=None):
+        """Read constant/polyMesh-style ASCII files (points, faces, owner,
+        neighbour, boundary, optional lbfsFaceShift).  nx, ny, nz give the
+        logical cell numbering when the mesh has one (needed for tiling and
+        membranes); 0 leaves it unknown."""
+        lib = load()
+        d = os.fsencode(str(directory))
+        cnt = [C.c_int() for _ in range(5)]
+        check(lib.lbfs_polymesh_counts(d, *[C.byref(c) for c in cnt]))
+        m = cls.__new__(cls)
+        m.n_points, m.n_cells, m.n_faces, m.n_internal, n_patch = [c.value for c in cnt]
+        m.n_bc = m.n_faces - m.n_internal
+        m.nx, m.ny, m.nz = nx, ny, nz
+        if nx and nx * ny * nz != m.n_cells:
+            raise LbfsError("logical size %dx%dx%d does not match %d cells" % (nx, ny, nz, m.n_cells))
+        m.origin = np.array(origin, dtype=np.float64)
+        m.lengths = np.array(lengths if lengths is not None else (nx, ny, nz), dtype=np.float64)
+        m.points = np.zeros(3 * m.n_points)
+        m.face_vertices = np.zeros(4 * m.n_faces, dtype=np.int32)
+        m.owner = np.zeros(m.n_faces, dtype=np.int32)
+        m.neighbour = np.zeros(max(m.n_internal, 1), dtype=np.int32)
+        m.patch = np.full(m.n_faces, -1, dtype=np.int32)
+        m.face_shift = np.zeros(3 * m.n_faces)
+        check(lib.lbfs_polymesh_read(d, ptr(m.points), ptr(m.face_vertices, _ip), ptr(m.owner, _ip),
+                                     ptr(m.neighbour, _ip), ptr(m.patch, _ip), ptr(m.face_shift)))
+        names, types = [], []
+        for k in range(n_patch):
+            nb, tb = C.create_string_buffer(256), C.create_string_buffer(256)
+            check(lib.lbfs_polymesh_patch(d, k, nb, 256, tb, 256))
+            names.append(nb.value.decode())
+            types.append(tb.value.decode())
+        m.patch_names, m.patch_types = tuple(names), tuple(types)
+        m.periodic = np.array([np.any(m.face_shift.reshape(-1, 3)[:, t] != 0.0) for t in range(3)], dtype=np.int32)
+        m._default_bc()
+        return m
+
+    def write_polymesh(self, directory):
+        """Write the mesh as ASCII polyMesh files; periodic wrap faces stay
+        internal faces and their translations go to lbfsFaceShift."""
+        lib = load()
+        os.makedirs(directory, exist_ok=True)
+        bp = self.patch[self.n_internal:]
+        starts, sizes = [], []
+        for k in range(len(self.patch_names)):
+            idx = np.nonzero(bp == k)[0]
+            if len(idx) and not np.array_equal(idx, np.arange(idx[0], idx[0] + len(idx))):
+                raise LbfsError("patch %d faces are not contiguous" % k)
+            starts.append(self.n_internal + (int(idx[0]) if len(idx) else 0))
+            sizes.append(len(idx))
+        keep = list(range(len(self.patch_names)))   # empty patches kept so patch ids survive a round trip
+        n = len(keep)
+        names = (C.c_char_p * max(n, 1))(*[self.patch_names[k].encode() for k in keep])
+        types = (C.c_char_p * max(n, 1))(*[self.patch_types[k].encode() for k in keep])
+        st = np.array([starts[k] for k in keep] or [0], dtype=np.int32)
+        sz = np.array([sizes[k] for k in keep] or [0], dtype=np.int32)
+        shift = self.face_shift if np.any(self.face_shift != 0.0) else None
+        check(lib.lbfs_polymesh_write(os.fsencode(str(directory)), self.n_points, ptr(self.points), self.n_cells,
+                                      self.n_faces, self.n_internal, ptr(self.face_vertices, _ip), ptr(self.owner, _ip),
+                                      ptr(self.neighbour, _ip), n, names, types, ptr(st, _ip), ptr(sz, _ip),
+                                      ptr(shift)))
 
     def set_patch_bc(self, patch, rho_type, vel_type, values):
         sel = self.bc_patch == patch

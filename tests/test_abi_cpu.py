@@ -72,3 +72,32 @@ def test_jitter_is_seeded_and_bounded():
     assert np.abs(p - ijk).max() <= 0.1 + 1e-6
     # FP32-representable coordinates
     assert np.array_equal(p.astype(np.float32).astype(np.float64), p)
+
+
+def test_polymesh_round_trip(tmp_path):
+    """A generated mesh written as ASCII polyMesh files and read back is the
+    same mesh: points to the last bit, faces, owner/neighbour, patches and
+    periodic translations, and the same product geometry."""
+    m = abi.HexMesh(6, 5, 4, periodic=(1, 0, 1), jitter=0.15, seed=9)
+    m.write_polymesh(tmp_path)
+    r = abi.HexMesh.read_polymesh(tmp_path, 6, 5, 4)
+    assert (r.n_points, r.n_cells, r.n_faces, r.n_internal) == (m.n_points, m.n_cells, m.n_faces, m.n_internal)
+    assert np.array_equal(r.points, m.points)
+    assert np.array_equal(r.face_vertices, m.face_vertices)
+    assert np.array_equal(r.owner, m.owner)
+    assert np.array_equal(r.neighbour[: m.n_internal], m.neighbour[: m.n_internal])
+    assert np.array_equal(r.face_shift, m.face_shift)
+    assert np.array_equal(r.patch, m.patch)
+    assert r.patch_names == abi.PATCH_NAMES and list(r.periodic) == [1, 0, 1]
+    ga, gb = m.geometry(), r.geometry()
+    for k in ga:
+        assert np.array_equal(ga[k], gb[k]), k
+
+
+def test_polymesh_rejects_non_quad(tmp_path):
+    m = abi.HexMesh(2, 2, 2, periodic=(0, 0, 0), jitter=0.0)
+    m.write_polymesh(tmp_path)
+    txt = (tmp_path / "faces").read_text().replace("4(", "3(", 1)
+    (tmp_path / "faces").write_text(txt)
+    with pytest.raises(abi.LbfsError):
+        abi.HexMesh.read_polymesh(tmp_path)
