@@ -47,6 +47,11 @@ extern "C" {
 /* ---- boundary condition types (preprocessor.cpp:471-493) ---- */
 #define LBFS_BC_DIRICHLET 1
 #define LBFS_BC_NEUMANN 2
+#define LBFS_BC_PARABOLIC 9  /* velocity only: duct inlet 16 u_max (H/2+y)(H/2-y)(H/2+z)(H/2-z)/H^4 at the
+                                ghost centroid, u_max = bc u value, H = channel_diameter (LBFS.cu:1892-1906) */
+#define LBFS_BC_SHEAR 10     /* velocity only: u = y * bc u value at the ghost centroid (LBFS.cu:1907-1918) */
+#define LBFS_BC_EMPTY 11     /* rho and velocity: face of an "empty" patch, no flux (Solver.cpp:308-310);
+                                the reference marks these by patch name, this ABI by type */
 
 /* ---- semantics flags for reference quirks (lbfs_params.flags) ---- */
 #define LBFS_FLAG_UPWIND_BAND 1      /* tie band |e.n| <= 0.001 dt_f, LBFS.cu:349-352 */
@@ -95,6 +100,7 @@ typedef struct {
     double dt;          /* time-marching step */
     double rho0;        /* reference density; the device stores rho - rho0 */
     double body_force[3];
+    double channel_diameter; /* H of LBFS_BC_PARABOLIC (domain.Y, Solver_gpu.cu:935); 0 if unused */
 } lbfs_params;
 
 typedef struct lbfs_ctx lbfs_ctx;

@@ -24,6 +24,7 @@ struct lbfs_ctx {
     lbfs::Geometry geo;
     lbfs::DeviceLayout lay;
     lbfs::FluidArgs args{};
+    std::vector<int> bc_vel_type;   // caller velocity BC type per boundary face
 
     float4 *st[2] = {nullptr, nullptr};
     int cur = 0;

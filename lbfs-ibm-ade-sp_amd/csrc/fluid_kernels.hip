@@ -54,14 +54,16 @@ __global__ __launch_bounds__(256) void k_gradients(FluidArgs a, const float4 *__
             const float4 sj = st[j];
             dr = sj.x - si.x; du = sj.y - si.y; dv = sj.z - si.z; dw = sj.w - si.w;
         } else {
-            // ghost: Dirichlet mirror 2 bc - owner, Neumann copy (Solution.cpp:759-833)
+            // ghost: Dirichlet mirror 2 bc - owner (velocity codes 1 and 3),
+            // Neumann and empty copy (Solution.cpp:759-833, LBFS.cu:1843-1928)
             const int bc = -1 - j;
             const int2 ty = a.bc_type[bc];
             const float4 bv = a.bc_val[bc];
+            const bool mv = ty.y == 1 || ty.y == 3;
             dr = ty.x == 1 ? 2.0f * (bv.x - si.x) : 0.0f;
-            du = ty.y == 1 ? 2.0f * (bv.y - si.y) : 0.0f;
-            dv = ty.y == 1 ? 2.0f * (bv.z - si.z) : 0.0f;
-            dw = ty.y == 1 ? 2.0f * (bv.w - si.w) : 0.0f;
+            du = mv ? 2.0f * (bv.y - si.y) : 0.0f;
+            dv = mv ? 2.0f * (bv.z - si.z) : 0.0f;
+            dw = mv ? 2.0f * (bv.w - si.w) : 0.0f;
         }
         const float wx = a.gw[(size_t)(3 * k) * n + i];
         const float wy = a.gw[(size_t)(3 * k + 1) * n + i];
@@ -125,6 +127,7 @@ __device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__
         // value from the boundary condition (Solver.cpp:855-900)
         const int bc = nb - a.n_cells;
         const int2 ty = a.bc_type[bc];
+        if (ty.y == LBFS_BC_EMPTY) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // Solver.cpp:308-310
         const float4 bv = a.bc_val[bc];
         const float bvv[4] = {bv.x, bv.y, bv.z, bv.w};
         const float d1n = sqrtf(fb.x * fb.x + fb.y * fb.y + fb.z * fb.z);
@@ -136,11 +139,13 @@ __device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int type = v == 0 ? ty.x : ty.y;
-            if (type == 1) {
+            if (type == 1 || type == 3) {
                 const float dphi = 2.0f * (bvv[v] - sov[v]) * inv_dmag;
 #pragma unroll
                 for (int c = 0; c < 3; c++) gn[3 * v + c] = dphi * dir[c];
-                p0[v] = bvv[v];
+                // code 3 (parabolic/shear targets) extrapolates the face value
+                // with the ghost gradient (LBFS.cu:358-378)
+                p0[v] = type == 1 ? bvv[v] : sov[v] - (fb.x * gn[3 * v] + fb.y * gn[3 * v + 1] + fb.z * gn[3 * v + 2]);
             } else {
 #pragma unroll
                 for (int c = 0; c < 3; c++) {

@@ -91,15 +91,40 @@ static int fluid_setup(lbfs_ctx *ctx, const lbfs_mesh_desc &m, const lbfs_params
     if ((rc = dev_upload(ctx, &fC, (const float4 *)d.fC.data(), ntf))) return rc;
     if ((rc = dev_upload(ctx, &fI, (const int2 *)d.fI.data(), ntf))) return rc;
     std::vector<int2> bt(nbc > 0 ? nbc : 1);
+    ctx->bc_vel_type.assign(nbc, LBFS_BC_NEUMANN);
     std::vector<float4> bvals(nbc > 0 ? nbc : 1);
     for (int b = 0; b < nbc; b++) {
         const int rt = m.bc_rho_type ? m.bc_rho_type[b] : LBFS_BC_NEUMANN;
         const int vt = m.bc_vel_type ? m.bc_vel_type[b] : LBFS_BC_NEUMANN;
-        if ((rt != 1 && rt != 2) || (vt != 1 && vt != 2))
-            return ctx->fail(LBFS_E_UNSUPPORTED, "boundary face %d: only Dirichlet (1) and Neumann (2) are supported", b);
-        bt[b] = make_int2(rt, vt);
+        const bool rt_ok = rt == LBFS_BC_DIRICHLET || rt == LBFS_BC_NEUMANN || rt == LBFS_BC_EMPTY;
+        const bool vt_ok = vt == LBFS_BC_DIRICHLET || vt == LBFS_BC_NEUMANN || vt == LBFS_BC_PARABOLIC ||
+                           vt == LBFS_BC_SHEAR || vt == LBFS_BC_EMPTY;
+        if (!rt_ok || !vt_ok)
+            return ctx->fail(LBFS_E_UNSUPPORTED, "boundary face %d: unsupported BC types (rho %d, vel %d)", b, rt, vt);
+        if ((rt == LBFS_BC_EMPTY) != (vt == LBFS_BC_EMPTY))
+            return ctx->fail(LBFS_E_INVALID_ARG, "boundary face %d: empty must be set for rho and velocity", b);
+        if ((vt == LBFS_BC_PARABOLIC) && !(p.channel_diameter > 0.0))
+            return ctx->fail(LBFS_E_INVALID_ARG, "parabolic velocity BC needs channel_diameter > 0");
         const double *bv = m.bc_values ? m.bc_values + 4 * (size_t)b : nullptr;
-        bvals[b] = make_float4(bv ? (float)(bv[0] - p.rho0) : 0.0f, bv ? (float)bv[1] : 0.0f, bv ? (float)bv[2] : 0.0f,
+        double u = bv ? bv[1] : 0.0;
+        // device codes: rho 1 mirror / 2 copy; velocity 1 mirror with the BC
+        // value at the face, 3 mirror about a target evaluated once at the
+        // ghost centroid with the face value extrapolated (LBFS.cu:1892-1918,
+        // 358-378), 2 copy, LBFS_BC_EMPTY no flux
+        int dvt = vt;
+        if (vt == LBFS_BC_PARABOLIC || vt == LBFS_BC_SHEAR) {
+            const double *c = &ctx->geo.cc[3 * ((size_t)n + b)];
+            if (vt == LBFS_BC_PARABOLIC) {
+                const double H = p.channel_diameter, H2 = 0.5 * H, H4 = H * H * H * H;
+                u = 16.0 * u * (H2 + c[1]) * (H2 - c[1]) * (H2 + c[2]) * (H2 - c[2]) / H4;
+            } else {
+                u = c[1] * u;
+            }
+            dvt = 3;
+        }
+        bt[b] = make_int2(rt == LBFS_BC_DIRICHLET ? 1 : 2, dvt);
+        ctx->bc_vel_type[b] = vt;
+        bvals[b] = make_float4(bv ? (float)(bv[0] - p.rho0) : 0.0f, (float)u, bv ? (float)bv[2] : 0.0f,
                                bv ? (float)bv[3] : 0.0f);
     }
     if ((rc = dev_upload(ctx, &bct, bt.data(), bt.size()))) return rc;
@@ -340,7 +365,8 @@ extern "C" int lbfs_stage_face_flux(lbfs_ctx *ctx, double *flux_out)
     const double p0z = ctx->params.lattice == LBFS_D2Q9 ? 0.0 : p0;
     for (int f = 0; f < ctx->geo.n_faces; f++) {
         const float4 r = h[ctx->lay.face_first[f]];
-        const double A = ctx->geo.area[f];
+        const bool empty = f >= ctx->geo.n_int && ctx->bc_vel_type[f - ctx->geo.n_int] == LBFS_BC_EMPTY;
+        const double A = empty ? 0.0 : ctx->geo.area[f];
         const double *n = &ctx->geo.normal[3 * (size_t)f];
         flux_out[4 * (size_t)f] = r.x;
         flux_out[4 * (size_t)f + 1] = r.y + p0 * n[0] * A;
@@ -361,13 +387,7 @@ extern "C" int lbfs_stage_residual(lbfs_ctx *ctx, double *res_out)
     HIPCHK(hipStreamSynchronize(ctx->stream));
     // add back the rest-pressure closure term -rho0/3 sum_f n A / V (zero on closed cells)
     const Geometry &g = ctx->geo;
-    std::vector<double> S(3 * (size_t)n, 0.0);
-    for (int f = 0; f < g.n_faces; f++)
-        for (int t = 0; t < 3; t++) {
-            const double v = g.normal[3 * (size_t)f + t] * g.area[f];
-            S[3 * (size_t)g.owner[f] + t] += v;
-            if (g.nbr[f] < n) S[3 * (size_t)g.nbr[f] + t] -= v;
-        }
+    const std::vector<double> &S = ctx->lay.closure;
     const double p0 = ctx->params.rho0 / 3.0;
     const double p0z = ctx->params.lattice == LBFS_D2Q9 ? 0.0 : p0;
     for (int i = 0; i < n; i++) {

@@ -382,8 +382,11 @@ int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<d
     // cells; with the reference's face areas (sum of triangle areas) warped
     // faces leave a defect, kept here exactly in FP64.
     {
-        std::vector<double> S(3 * (size_t)nc, 0.0);
+        std::vector<double> &S = d.closure;
+        S.assign(3 * (size_t)nc, 0.0);
         for (int f = 0; f < g.n_faces; f++) {
+            // faces of empty patches carry no flux at all
+            if (f >= g.n_int && m.bc_vel_type && m.bc_vel_type[f - g.n_int] == LBFS_BC_EMPTY) continue;
             const double A = g.area[f];
             for (int t = 0; t < 3; t++) {
                 const double v = g.normal[3 * (size_t)f + t] * A;

@@ -61,6 +61,7 @@ struct DeviceLayout {
     std::vector<float> gw;        // 18 * n, SoA [(k*3+c)*n + i], W_k = LHS^-1 RHS_k
     std::vector<float> inv_vol;   // n
     std::vector<float> static_force; // 3 * n: body force + pressure closure defect of the cell
+    std::vector<double> closure;  // 3 * n, caller order: sum of n A over the faces that carry flux
     std::vector<int16_t> cell_slots; // 6 * n, AoS [i*6+k]: tile-local face slot, sign in bit 15 (1 = owner)
     // per tile-face (duplicated across tiles)
     std::vector<float> fA;        // 4 per face: normal xyz, area

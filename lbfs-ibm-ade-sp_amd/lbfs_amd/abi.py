@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "liblbfs_amd.so")
 
 D3Q15, D3Q19, D2Q9 = 0, 1, 2
 GRAD_HYBRID, GRAD_LS, GRAD_GG = 0, 1, 2
-BC_DIRICHLET, BC_NEUMANN = 1, 2
+BC_DIRICHLET, BC_NEUMANN, BC_PARABOLIC, BC_SHEAR, BC_EMPTY = 1, 2, 9, 10, 11
 FLAG_UPWIND_BAND, FLAG_NEUMANN_SIGNED, FLAG_GHOST_DIR_GPU, FLAG_LS_DET_COMPAT, FLAG_CLOSED_FACES = 1, 2, 4, 16, 32
 IBM_LINEAR2, IBM_COS4 = 0, 1
 OPT_GRAPH = 1
@@ -37,7 +37,7 @@ class Params(C.Structure):
     _fields_ = [
         ("lattice", C.c_int), ("gradient_type", C.c_int), ("flags", C.c_int),
         ("visc", C.c_double), ("gamma", C.c_double), ("fneq_min", C.c_double), ("dt", C.c_double),
-        ("rho0", C.c_double), ("body_force", C.c_double * 3),
+        ("rho0", C.c_double), ("body_force", C.c_double * 3), ("channel_diameter", C.c_double),
     ]
 
 
@@ -216,6 +216,9 @@ class HexMesh:
         m.patch_names, m.patch_types = tuple(names), tuple(types)
         m.periodic = np.array([np.any(m.face_shift.reshape(-1, 3)[:, t] != 0.0) for t in range(3)], dtype=np.int32)
         m._default_bc()
+        for k, t in enumerate(m.patch_types):
+            if t == "empty":
+                m.set_patch_bc(k, BC_EMPTY, BC_EMPTY, (1.0, 0.0, 0.0, 0.0))
         return m
 
     def write_polymesh(self, directory):
@@ -344,10 +347,11 @@ class Membrane:
 
 
 def make_params(lattice=D3Q19, visc=1.0 / 6.0, dt=0.5, rho0=1.0, gamma=1.0, fneq_min=0.0, flags=0,
-                gradient_type=GRAD_HYBRID, body_force=(0.0, 0.0, 0.0)):
+                gradient_type=GRAD_HYBRID, body_force=(0.0, 0.0, 0.0), channel_diameter=0.0):
     p = Params()
     p.lattice, p.gradient_type, p.flags = lattice, gradient_type, flags
     p.visc, p.gamma, p.fneq_min, p.dt, p.rho0 = visc, gamma, fneq_min, dt, rho0
+    p.channel_diameter = channel_diameter
     for t in range(3):
         p.body_force[t] = body_force[t]
     return p

@@ -302,19 +302,45 @@ int or_ls_setup(const or_mesh *m, int grad_type, int det_compat, double *lhs, do
 /* fluid stages                                                        */
 /* ------------------------------------------------------------------ */
 
-/* Ghost states: Solution.cpp:759-833 (types 1 Dirichlet mirror 2 bc - owner,
- * 2 Neumann copy with dx = 0). */
-void or_bc_update(const or_mesh *m, double *st)
+/* Dirichlet value of variable v (0 rho, 1-3 velocity) at boundary b: the BC
+ * value, or for the GPU-only velocity types the target evaluated at the
+ * ghost centroid, LBFS.cu:1892-1918: parabolic duct inlet
+ * u = 16 u_max (H/2 + y)(H/2 - y)(H/2 + z)(H/2 - z) / H^4, shear u = y u_s. */
+static double bc_target(const or_mesh *m, const or_params *p, int b, int v)
+{
+    const double *bc = m->bc_val + 4 * b;
+    const double *c = m->cc + 3 * (m->n_cells + b);
+    int type = v == 0 ? m->bc_rho_type[b] : m->bc_vel_type[b];
+    if (v == 1 && type == OR_BC_PARABOLIC) {
+        double H = p->channel_diameter, H2 = 0.5 * H, H4 = H * H * H * H;
+        return 16.0 * bc[1] * (H2 + c[1]) * (H2 - c[1]) * (H2 + c[2]) * (H2 - c[2]) / H4;
+    }
+    if (v == 1 && type == OR_BC_SHEAR) return c[1] * bc[1];
+    return bc[v];
+}
+
+/* the ghost mirrors a Dirichlet value for these types */
+static int bc_mirrored(const or_mesh *m, int b, int v)
+{
+    int type = v == 0 ? m->bc_rho_type[b] : m->bc_vel_type[b];
+    if (v == 0) return type == OR_BC_DIRICHLET;
+    return type == OR_BC_DIRICHLET || type == OR_BC_PARABOLIC || type == OR_BC_SHEAR;
+}
+
+/* Ghost states: Solution.cpp:759-833 and LBFS.cu:1843-1928 (types 1, 9, 10
+ * mirror 2 bc - owner, Neumann and empty copy the owner with dx = 0). */
+void or_bc_update(const or_mesh *m, const or_params *p, double *st)
 {
     int b, v;
     for (b = 0; b < m->n_bc; b++) {
         int j = m->n_cells + b, o = m->owner[m->n_int + b];
-        const double *bc = m->bc_val + 4 * b;
-        if (m->bc_rho_type[b] == 1) st[4 * j] = bc[0] - (st[4 * o] - bc[0]);
-        else st[4 * j] = st[4 * o];
-        for (v = 1; v < 4; v++) {
-            if (m->bc_vel_type[b] == 1) st[4 * j + v] = bc[v] - (st[4 * o + v] - bc[v]);
-            else st[4 * j + v] = st[4 * o + v];
+        for (v = 0; v < 4; v++) {
+            if (bc_mirrored(m, b, v)) {
+                double t = bc_target(m, p, b, v);
+                st[4 * j + v] = t - (st[4 * o + v] - t);
+            } else {
+                st[4 * j + v] = st[4 * o + v];
+            }
         }
     }
 }
@@ -355,9 +381,8 @@ void or_gradients(const or_mesh *m, const or_params *p, const double *st, double
         dmag = v_norm(d);
         for (t = 0; t < 3; t++) dir[t] = (p->flags & OR_FLAG_GHOST_DIR_GPU) ? d[t] / dmag : n[t];
         for (v = 0; v < 4; v++) {
-            int type = v == 0 ? m->bc_rho_type[b] : m->bc_vel_type[b];
-            if (type == 1) {
-                double dphi = 2.0 * (m->bc_val[4 * b + v] - st[4 * o + v]);
+            if (bc_mirrored(m, b, v)) {
+                double dphi = 2.0 * (bc_target(m, p, b, v) - st[4 * o + v]);
                 for (t = 0; t < 3; t++) g[12 * j + 3 * v + t] = dphi / dmag * dir[t];
             } else {
                 for (t = 0; t < 3; t++) {
@@ -402,12 +427,20 @@ void or_face_flux(const or_mesh *m, const or_params *p, const double *st, const 
             t1[t] = m->cc[3 * i + t] - m->fc[3 * f + t];
             t2[t] = m->cc[3 * nb + t] + (s ? s[t] : 0.0) - m->fc[3 * f + t];
         }
+        if (ghost && m->bc_vel_type[nb - m->n_cells] == OR_BC_EMPTY) {
+            /* empty patch: no flux (Solver.cpp:308-310) */
+            for (v = 0; v < 4; v++) flux[4 * f + v] = 0.0;
+            continue;
+        }
         if (ghost) {
             int b = nb - m->n_cells;
             const double *bc = m->bc_val + 4 * b;
             for (v = 0; v < 4; v++) {
+                /* only plain Dirichlet substitutes the BC value at the face;
+                 * the parabolic/shear types extrapolate with the ghost
+                 * gradient (LBFS.cu:358-378) */
                 int type = v == 0 ? m->bc_rho_type[b] : m->bc_vel_type[b];
-                if (type == 1) phi0[v] = bc[v];
+                if (type == OR_BC_DIRICHLET) phi0[v] = bc[v];
                 else phi0[v] = st[4 * i + v] - v_dot(t1, gn + 3 * v);
                 phi_i[v] = phi_nb[v] = phi0[v];
             }
@@ -506,7 +539,7 @@ int or_fluid_step(const or_mesh *m, const or_params *p, double *st, double *cons
                   double *g, double *flux, double *res)
 {
     int i;
-    or_bc_update(m, st);
+    or_bc_update(m, p, st);
     or_gradients(m, p, st, g);
     or_face_flux(m, p, st, g, flux);
     or_residual(m, flux, res);

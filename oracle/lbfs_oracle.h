@@ -57,7 +57,16 @@ typedef struct {
     double visc;     /* kinematic viscosity nu */
     double gamma;    /* preconditioning gamma (1 = none) */
     double dt;       /* time-marching step */
+    double channel_diameter; /* H of the parabolic (9) and shear (10) velocity BCs, domain.Y in the reference */
 } or_params;
+
+/* boundary condition types (preprocessor.cpp:469-491); OR_BC_EMPTY marks the
+ * faces of an "empty" patch, which carry no flux (Solver.cpp:308-310) */
+#define OR_BC_DIRICHLET 1
+#define OR_BC_NEUMANN 2
+#define OR_BC_PARABOLIC 9
+#define OR_BC_SHEAR 10
+#define OR_BC_EMPTY 11
 
 /* ---- geometry and setup (unstructured_mesh.cpp, gradients.cpp) ---- */
 int or_geometry(int n_cells, int n_faces, int n_int, const double *pts, const int *fverts,
@@ -73,7 +82,7 @@ int or_lattice(int lattice, int *e /* [27*3] */, double *w /* [27] */);
 void or_feq(int lattice, double rho, const double *u, double gamma, double *feq);
 
 /* ---- fluid stages (Solver.cpp CPU path; state AoS st[4 i + {rho,u,v,w}]) ---- */
-void or_bc_update(const or_mesh *m, double *st);
+void or_bc_update(const or_mesh *m, const or_params *p, double *st);
 void or_gradients(const or_mesh *m, const or_params *p, const double *st, double *g /* [12 n_total] */);
 void or_face_flux(const or_mesh *m, const or_params *p, const double *st, const double *g,
                   double *flux /* [4 n_faces]: mass, mom x, y, z */);

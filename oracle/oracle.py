@@ -31,7 +31,10 @@ class OrMesh(C.Structure):
 
 class OrParams(C.Structure):
     _fields_ = [("lattice", C.c_int), ("flags", C.c_int), ("visc", C.c_double), ("gamma", C.c_double),
-                ("dt", C.c_double)]
+                ("dt", C.c_double), ("channel_diameter", C.c_double)]
+
+
+OR_BC_DIRICHLET, OR_BC_NEUMANN, OR_BC_PARABOLIC, OR_BC_SHEAR, OR_BC_EMPTY = 1, 2, 9, 10, 11
 
 
 class OrGrid(C.Structure):
@@ -75,7 +78,7 @@ def load():
         lib.or_feq.restype = None
         lib.or_feq.argtypes = [C.c_int, C.c_double, _dp, C.c_double, _dp]
         om, op = C.POINTER(OrMesh), C.POINTER(OrParams)
-        lib.or_bc_update.argtypes = [om, _dp]
+        lib.or_bc_update.argtypes = [om, op, _dp]
         lib.or_gradients.argtypes = [om, op, _dp, _dp]
         lib.or_face_flux.argtypes = [om, op, _dp, _dp, _dp]
         lib.or_residual.argtypes = [om, _dp, _dp]
@@ -134,7 +137,7 @@ class OracleFluid:
     """
 
     def __init__(self, mesh, lattice_kind, visc, dt, gamma=1.0, fneq_min=0.0, flags=0, grad_type=0,
-                 det_compat=False, visc_factor=None, closed=False):
+                 det_compat=False, visc_factor=None, closed=False, channel_diameter=0.0):
         lib = load()
         self.mesh = mesh
         nc, nf, ni = mesh.n_cells, mesh.n_faces, mesh.n_internal
@@ -177,7 +180,7 @@ class OracleFluid:
         rc = lib.or_ls_setup(C.byref(m), grad_type, 1 if det_compat else 0, ptr(self.lhs), ptr(self.rhs))
         if rc != 0:
             raise RuntimeError("or_ls_setup failed")
-        self.p = OrParams(lattice_kind, flags, visc, gamma, dt)
+        self.p = OrParams(lattice_kind, flags, visc, gamma, dt, channel_diameter)
         self.st = np.zeros(4 * (nc + nb))
         self.cons = np.zeros(4 * nc)
         self.g = np.zeros(12 * (nc + nb))
@@ -201,7 +204,7 @@ class OracleFluid:
 
     def gradients(self):
         lib = load()
-        lib.or_bc_update(C.byref(self.m), ptr(self.st))
+        lib.or_bc_update(C.byref(self.m), C.byref(self.p), ptr(self.st))
         lib.or_gradients(C.byref(self.m), C.byref(self.p), ptr(self.st), ptr(self.g))
         return self.g[: 12 * self.nc].copy()
 

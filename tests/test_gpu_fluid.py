@@ -142,3 +142,52 @@ def test_invalid_arguments_return_codes():
     bad = np.full(m.n_cells, np.nan)
     with pytest.raises(abi.LbfsError, match="non-finite"):
         s.set_state(bad, bad, bad, bad)
+
+
+def test_profile_and_empty_boundaries_parity():
+    """Parabolic duct inlet (9) and shear (10) velocity BCs on a jittered duct,
+    and a one-layer mesh with empty front/back patches: 60 steps, relative L2
+    of velocity <= 1e-5 against FP64, face fluxes of empty faces exactly 0."""
+    H = 8.0
+    m = abi.HexMesh(12, 8, 8, lengths=(12.0, H, H), origin=(0.0, -H / 2, -H / 2), periodic=(0, 0, 0),
+                    jitter=0.1, seed=6)
+    m.set_patch_bc(0, abi.BC_NEUMANN, abi.BC_PARABOLIC, (1.0, 0.01, 0.0, 0.0))
+    m.set_patch_bc(1, abi.BC_DIRICHLET, abi.BC_SHEAR, (1.0, 0.001, 0.0, 0.0))
+    s = abi.Solver(m, abi.make_params(lattice=abi.D3Q19, visc=VISC, dt=DT, channel_diameter=H))
+    o = orc.OracleFluid(m, orc.OR_D3Q19, visc=VISC, dt=DT, channel_diameter=H)
+    st = cases.smooth_state(m, seed=15)
+    s.set_state(*st)
+    o.set_state(*st)
+    np.testing.assert_allclose(s.face_flux().reshape(-1, 4), o.face_flux().reshape(-1, 4), atol=2e-6)
+    s.step(60)
+    o.step(60)
+    _, us, vs, ws = s.get_state()
+    _, uo, vo, wo = o.state()
+    assert cases.rel_l2(np.concatenate([us, vs, ws]), np.concatenate([uo, vo, wo])) < 1e-5
+
+    m2 = abi.HexMesh(16, 16, 1, periodic=(1, 1, 0), jitter=0.05, seed=1)
+    for patch in (4, 5):
+        m2.set_patch_bc(patch, abi.BC_EMPTY, abi.BC_EMPTY, (1.0, 0.0, 0.0, 0.0))
+    s2 = abi.Solver(m2, abi.make_params(lattice=abi.D3Q15, visc=0.0024, dt=DT))
+    o2 = orc.OracleFluid(m2, orc.OR_D3Q15, visc=0.0024, dt=DT)
+    st2 = cases.tgv_state(m2, 0.01)
+    s2.set_state(*st2)
+    o2.set_state(*st2)
+    fl = s2.face_flux().reshape(-1, 4)
+    assert np.all(fl[m2.n_internal:] == 0.0)
+    s2.step(60)
+    o2.step(60)
+    _, us, vs, ws = s2.get_state()
+    _, uo, vo, wo = o2.state()
+    assert cases.rel_l2(np.concatenate([us, vs, ws]), np.concatenate([uo, vo, wo])) < 1e-5
+
+
+def test_unsupported_bc_types_rejected():
+    m = cases.channel_mesh(6, 5, 4)
+    m.set_patch_bc(2, abi.BC_NEUMANN, 4, (1.0, 0.0, 0.0, 0.0))   # parabolic-N: not on the GPU path
+    with pytest.raises(abi.LbfsError):
+        abi.Solver(m, abi.make_params())
+    m = cases.channel_mesh(6, 5, 4)
+    m.set_patch_bc(2, abi.BC_NEUMANN, abi.BC_PARABOLIC, (1.0, 0.01, 0.0, 0.0))
+    with pytest.raises(abi.LbfsError):
+        abi.Solver(m, abi.make_params())          # parabolic without channel_diameter

@@ -176,8 +176,11 @@ def test_rest_state_drifts_with_reference_face_area():
         assert (np.abs(u).max() > 1e-6) == moves
 
 
-def _tgv_runs(flags, N=24, steps=160, dt=0.25):
-    m = abi.HexMesh(N, N, 1, periodic=(1, 1, 1), jitter=0.05, seed=1)
+def _tgv_runs(flags, N=24, steps=160, dt=0.25, empty=False, jitter=0.05):
+    m = abi.HexMesh(N, N, 1, periodic=(1, 1, 0 if empty else 1), jitter=jitter, seed=1)
+    if empty:
+        for patch in (4, 5):
+            m.set_patch_bc(patch, abi.BC_EMPTY, abi.BC_EMPTY, (1.0, 0.0, 0.0, 0.0))
     U0, Re = 0.01, 100.0
     nu = U0 * N / Re
     runs = {}
@@ -209,3 +212,89 @@ def test_taylor_green_decays_at_viscous_rate():
         _, u, _, _ = runs[kind]
         amp = np.dot(u, u0) / np.dot(u0, u0)
         assert abs(amp / decay - 1.0) < 0.02, (kind, amp, decay)
+
+
+def test_empty_patches_carry_no_flux_and_project_onto_d2q9():
+    """C1's cross-check form: one z layer with empty front/back patches
+    (Solver.cpp:308-310).  Their faces carry exactly zero flux and mass is
+    conserved to round-off on the jittered mesh; on an orthogonal mesh (where
+    the ghost rows leave no z gradient) D3Q15 with the tie band reproduces
+    D2Q9 to round-off and w stays zero."""
+    m = abi.HexMesh(24, 24, 1, periodic=(1, 1, 0), jitter=0.05, seed=1)
+    for patch in (4, 5):
+        m.set_patch_bc(patch, abi.BC_EMPTY, abi.BC_EMPTY, (1.0, 0.0, 0.0, 0.0))
+    o = orc.OracleFluid(m, orc.OR_D3Q15, visc=0.0024, dt=0.25)
+    o.set_state(*cases.tgv_state(m, 0.01))
+    fl = o.face_flux().reshape(-1, 4)
+    assert np.all(fl[m.n_internal:] == 0.0) and np.abs(fl[: m.n_internal]).max() > 1e-4
+    res = o.residual().reshape(-1, 4)
+    assert abs(np.dot(res[:, 0], o.vol)) < 1e-12 * np.abs(res[:, 0]).sum()
+    _, _, runs = _tgv_runs(orc.FLAG_UPWIND_BAND, steps=40, empty=True, jitter=0.0)
+    for a, b in zip(runs[orc.OR_D3Q15], runs[orc.OR_D2Q9]):
+        assert np.max(np.abs(a - b)) < 1e-13
+    assert np.max(np.abs(runs[orc.OR_D3Q15][3])) < 1e-15
+
+
+def _duct(nx=4, n=5, H=5.0, jitter=0.0):
+    m = abi.HexMesh(nx, n, n, lengths=(float(nx), H, H), origin=(0.0, -H / 2, -H / 2), periodic=(0, 0, 0),
+                    jitter=jitter, seed=4)
+    m.set_patch_bc(0, abi.BC_NEUMANN, abi.BC_PARABOLIC, (1.0, 0.02, 0.0, 0.0))
+    m.set_patch_bc(1, abi.BC_DIRICHLET, abi.BC_SHEAR, (1.0, 0.003, 0.0, 0.0))
+    return m
+
+
+def test_parabolic_and_shear_ghost_targets():
+    """Ghost states of the GPU-only velocity BCs (LBFS.cu:1892-1918) mirror a
+    target taken at the ghost centroid: the parabolic inlet peaks at u_max on
+    the duct axis, vanishes on the walls and is even in y and z; the shear
+    target is linear in y."""
+    H = 5.0
+    m = _duct(H=H)
+    o = orc.OracleFluid(m, orc.OR_D3Q19, visc=1 / 6, dt=0.25, channel_diameter=H)
+    n = m.n_cells
+    rng = np.random.default_rng(5)
+    o.set_state(1.0 + 1e-3 * rng.standard_normal(n), 0.01 * rng.standard_normal(n), np.zeros(n), np.zeros(n))
+    o.gradients()
+    st = o.st.reshape(-1, 4)
+    cc = o.cc.reshape(-1, 3)
+    for patch, typ in ((0, "parabolic"), (1, "shear")):
+        bcs = np.nonzero(m.bc_patch[: m.n_bc] == patch)[0]
+        j = n + bcs
+        own = m.owner[m.n_internal + bcs]
+        target = 0.5 * (st[j, 1] + st[own, 1])
+        y, z = cc[j, 1], cc[j, 2]
+        if typ == "parabolic":
+            centre = np.argmin(y * y + z * z)
+            assert abs(y[centre]) < 1e-12 and abs(z[centre]) < 1e-12
+            assert abs(target[centre] - 0.02) < 1e-15
+            # separable and even: t(y, z) t(0, 0) = t(y, 0) t(0, z)
+            for a in range(len(j)):
+                ty = target[np.argmin(np.abs(y - y[a]) + np.abs(z))]
+                tz = target[np.argmin(np.abs(y) + np.abs(z - z[a]))]
+                assert abs(target[a] * 0.02 - ty * tz) < 1e-15
+                mirror = np.argmin(np.abs(y + y[a]) + np.abs(z - z[a]))
+                assert abs(target[mirror] - target[a]) < 1e-15
+            # extrapolated to the wall |y| = H/2 the profile is zero: quadratic through 3 rows
+            row = np.abs(z) < 1e-12
+            coef = np.polyfit(y[row], target[row], 2)
+            assert abs(np.polyval(coef, H / 2)) < 1e-14 and abs(np.polyval(coef, -H / 2)) < 1e-14
+        else:
+            coef = np.polyfit(y, target, 1)
+            assert abs(coef[0] - 0.003) < 1e-14 and abs(coef[1]) < 1e-14
+        # v, w mirror the BC values (zero)
+        assert np.max(np.abs(st[j, 2] + st[own, 2])) < 1e-15
+
+
+def test_extrapolated_face_values_for_profile_bcs():
+    """At parabolic/shear faces the face state is extrapolated with the ghost
+    gradient (LBFS.cu:358-378), not set to the target: the mass flux of a
+    uniform rest state through such a face is not forced to the target."""
+    H = 5.0
+    m = _duct(H=H, jitter=0.1)
+    n = m.n_cells
+    o = orc.OracleFluid(m, orc.OR_D3Q19, visc=1 / 6, dt=0.25, channel_diameter=H)
+    o.set_state(np.ones(n), np.zeros(n), np.zeros(n), np.zeros(n))
+    fl = o.face_flux().reshape(-1, 4)
+    inlet = m.n_internal + np.nonzero(m.bc_patch[: m.n_bc] == 0)[0]
+    # with the owner at rest, the extrapolated face velocity is the ghost-gradient step toward the target
+    assert np.all(np.abs(fl[inlet, 0]) > 0)
