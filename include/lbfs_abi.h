@@ -175,6 +175,11 @@ int lbfs_get_membrane_nodes(lbfs_ctx *ctx, double *X, double *U, double *F);
 /* per membrane: V, A, MAD, V0, A0, MAD0 (FP64, lagrangian_object.cpp:601-650) */
 int lbfs_get_membrane_globals(lbfs_ctx *ctx, double *out6);
 
+/* output-step membrane energies, 7 per membrane: WLC spring, local area,
+ * local bending, global area, volume, ADE, total (Solver_gpu.cu:1060-1139,
+ * spring_network.cu:1687-1768); evaluated from the last force call's state */
+int lbfs_get_membrane_energies(lbfs_ctx *ctx, double *out /* [7 n_membranes] */);
+
 /* host-side membrane meshes: frequency-n geodesic sphere (10 n^2 + 2 nodes) */
 int lbfs_mesh_icosphere(int frequency, double *nodes, int *tri);
 /* Evans-Fung biconcave disc of diameter d0 from a unit sphere (symmetry axis z) */

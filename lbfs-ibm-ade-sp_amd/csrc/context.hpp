@@ -39,6 +39,7 @@ struct lbfs_ctx {
     lbfs::MemArgs mem{};
     lbfs::IbmArgs ibm{};
     std::vector<int> mem_node_start;
+    double *mem_energy = nullptr;   // [7 n_mem] output of lbfs_get_membrane_energies
     std::vector<double> mem_origin;
     std::vector<void *> coupling_allocs;
     float *visc_factor = nullptr;   // [n] internal viscosity field, null while every ratio is 1

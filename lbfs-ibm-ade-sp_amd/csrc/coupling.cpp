@@ -36,6 +36,7 @@ void lbfs_ctx::release_extensions()
     for (void *p : coupling_allocs) (void)hipFree(p);
     coupling_allocs.clear();
     have_membranes = false;
+    mem_energy = nullptr;
     visc_factor = nullptr;
     args.visc_factor = nullptr;
 }
@@ -98,6 +99,7 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
         const double r = mem[i].wlc_ratio;
         MemParamDev &p = params[i];
         p.spring_c = (float)(0.25 * r * r / ((r - 1.0) * (r - 1.0)) - 0.25 + 1.0 / r);
+        p.wlc_x0 = (float)(1.0 / r);
         p.volume_modulus = (float)mem[i].volume_modulus;
         p.local_area_modulus = (float)mem[i].local_area_modulus;
         p.global_area_modulus = (float)mem[i].global_area_modulus;
@@ -156,6 +158,7 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     float4 *dbb;
     if ((rc = cp_alloc(ctx, &dns, mem_node_start.data(), mem_node_start.size()))) return rc;
     if ((rc = cp_alloc<float4>(ctx, &dbb, nullptr, 2 * (size_t)count))) return rc;
+    if ((rc = cp_alloc<double>(ctx, &ctx->mem_energy, nullptr, 7 * (size_t)count))) return rc;
     m.mem_node_start = dns;
     m.bbox = dbb;
     m.X = dX; m.U = dU; m.F = dF; m.node_mem = dnm; m.origin = dorig; m.tri = dtri;
@@ -264,6 +267,17 @@ extern "C" int lbfs_get_membrane_globals(lbfs_ctx *ctx, double *out6)
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (int i = 0; i < nm; i++)
         for (int q = 0; q < 6; q++) out6[6 * (size_t)i + q] = g[8 * (size_t)i + q];
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_get_membrane_energies(lbfs_ctx *ctx, double *out7)
+{
+    if (!ctx || !out7) return LBFS_E_INVALID_ARG;
+    if (!ctx->have_membranes) return ctx->fail(LBFS_E_STATE, "no membranes attached");
+    const int nm = ctx->mem.n_mem;
+    HIPCHK(launch_mem_energy(ctx->mem, ctx->mem_energy, ctx->stream));
+    HIPCHK(hipMemcpyAsync(out7, ctx->mem_energy, sizeof(double) * 7 * (size_t)nm, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return LBFS_OK;
 }
 

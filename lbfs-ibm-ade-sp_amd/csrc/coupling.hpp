@@ -13,6 +13,7 @@ struct MemParamDev {
     float global_bending, local_bending, viscosity;
     int n_mad;                // edges summed for MAD
     float visc_ratio;         // internal viscosity ratio (1 = off)
+    float wlc_x0;             // 1 / wlc_ratio: rest length over contour length
 };
 
 struct MemArgs {
@@ -61,5 +62,7 @@ hipError_t launch_mem_nodes(const MemArgs &m, hipStream_t s);
 hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s);
 // internal viscosity field (immersed_boundary_method.cu:389-527)
 hipError_t launch_internal_visc(const MemArgs &m, const IbmArgs &g, float delta_h, float *visc_factor, hipStream_t s);
+// output-step energies, 7 doubles per membrane (Solver_gpu.cu:1060-1139)
+hipError_t launch_mem_energy(const MemArgs &m, double *out, hipStream_t s);
 
 }  // namespace lbfs

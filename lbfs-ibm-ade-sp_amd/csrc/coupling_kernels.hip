@@ -577,6 +577,68 @@ __global__ __launch_bounds__(256) void k_internal_visc(MemArgs m, IbmArgs g, flo
     visc_factor[i] = out;
 }
 
+// Output-step energies of each membrane (one workgroup per membrane, FP64
+// sums over its edges): WLC springs (get_spring_energy, spring_network.cu:1721-1768,
+// halved as there), local area (get_area_energy :1687-1698, scaled by the
+// global area modulus as there), local bending (get_bending_energy :1703-1715,
+// edge e divided by entry e of the flat |N| array as the launch passes it),
+// then global area, volume and ADE from the globals (Solver_gpu.cu:1060-1139).
+// out: spring, local area, local bending, global area, volume, ADE, total.
+__global__ __launch_bounds__(256) void k_mem_energy(MemArgs m, double *out)
+{
+    const int mem = blockIdx.x;
+    const MemParamDev p = m.params[mem];
+    const double x0 = p.wlc_x0;
+    const double c1 = 0.25 / (1.0 - x0) / (1.0 - x0) - 0.25 + x0;
+    const double c2 = x0 * x0 * (3.0 - 2.0 * x0) / (1.0 - x0) - 4.0 * c1 * x0;
+    __shared__ double sh[3][256];
+    double es = 0.0, ela = 0.0, elb = 0.0;
+    const int e0 = m.mem_edge_start[mem], e1 = m.mem_edge_start[mem + 1];
+    for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+        const int4 q = m.edges[e];
+        const f3 l = ld3(m.X[q.z]) - ld3(m.X[q.x]);
+        const float4 kc = m.edge_const[e];
+        const float4 st = m.edge_state[e];
+        const float2 a0 = m.edge_area0[e];
+        const double Lm = kc.x, x = (double)sqrtf(dot(l, l)) / Lm;
+        es += 0.5 * (double)kc.y * fabs(0.25 * Lm * (x * x * (3.0 - 2.0 * x) / (1.0 - x) - 4.0 * x * c1 - c2));
+        const double d1 = (double)st.y - a0.x, d2 = (double)st.z - a0.y;
+        ela += (double)p.global_area_modulus * 0.5 * (d1 * d1 / a0.x + d2 * d2 / a0.y);
+        const int flat = e - e0;                       // entry of the membrane's 2E |N| array
+        const float4 sf = m.edge_state[e0 + (flat >> 1)];
+        const double nflat = (flat & 1) ? sf.z : sf.y;
+        const double dk = (double)st.x - kc.z;
+        elb += 0.5 * (double)p.local_bending * dk * dk / nflat;
+    }
+    sh[0][threadIdx.x] = es;
+    sh[1][threadIdx.x] = ela;
+    sh[2][threadIdx.x] = elb;
+    __syncthreads();
+    for (int stp = 128; stp > 0; stp >>= 1) {
+        if ((int)threadIdx.x < stp)
+            for (int k = 0; k < 3; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + stp];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double *gl = m.glob + 8 * (size_t)mem;
+        const double dv = gl[0] - gl[3], da = gl[1] - gl[4], dm = gl[2] - gl[5];
+        double *o = out + 7 * (size_t)mem;
+        o[0] = sh[0][0];
+        o[1] = sh[1][0];
+        o[2] = sh[2][0];
+        o[3] = 0.5 * (double)p.global_area_modulus * da * da / gl[4];
+        o[4] = 0.5 * (double)p.volume_modulus * dv * dv / gl[3];
+        o[5] = 0.5 * (double)p.global_bending * dm * dm / gl[4];
+        o[6] = o[0] + o[1] + o[2] + o[3] + o[4] + o[5];
+    }
+}
+
+hipError_t launch_mem_energy(const MemArgs &m, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mem_energy, dim3(m.n_mem), dim3(256), 0, s, m, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_internal_visc(const MemArgs &m, const IbmArgs &g, float delta_h, float *visc_factor, hipStream_t s)
 {
     hipLaunchKernelGGL(k_mem_bbox, dim3(m.n_mem), dim3(256), 0, s, m);

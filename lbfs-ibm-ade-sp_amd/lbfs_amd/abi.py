@@ -98,6 +98,7 @@ def load():
             "lbfs_add_membranes": (C.c_int, [vp, C.POINTER(MembraneDesc), C.c_int, C.c_int]),
             "lbfs_get_membrane_nodes": (C.c_int, [vp, _dp, _dp, _dp]),
             "lbfs_get_membrane_globals": (C.c_int, [vp, _dp]),
+            "lbfs_get_membrane_energies": (C.c_int, [vp, _dp]),
             "lbfs_mesh_icosphere": (C.c_int, [C.c_int, _dp, _ip]),
             "lbfs_mesh_evans_fung": (C.c_int, [C.c_int, _dp, C.c_double, _dp]),
             "lbfs_mesh_springs": (C.c_int, [C.c_int, C.c_int, _ip, _ip, _ip]),
@@ -430,6 +431,13 @@ class Solver:
         g = np.zeros(6 * len(self._membranes))
         check(load().lbfs_get_membrane_globals(self.ctx, ptr(g)), self.ctx)
         return g.reshape(-1, 6)
+
+    def membrane_energies(self):
+        """Per membrane: spring, local area, local bending, global area,
+        volume, ADE, total."""
+        out = np.zeros(7 * len(self._membranes))
+        check(load().lbfs_get_membrane_energies(self.ctx, ptr(out)), self.ctx)
+        return out.reshape(-1, 7)
 
     def cell_force(self):
         f = np.zeros(3 * self.n)

@@ -135,6 +135,8 @@ void or_membrane_setup(or_membrane *mb, const or_mem_params *mp, const double *X
 void or_membrane_forces(or_membrane *mb, const or_mem_params *mp, const double *X, const double *V,
                         int initials, double *F);
 void or_membrane_globals(or_membrane *mb, const double *X, double *vol, double *area);
+void or_membrane_energies(const or_membrane *mb, const or_mem_params *mp, const double *X,
+                          double *out /* [7]: spring, local area, local bending, global area, volume, ADE, total */);
 
 #ifdef __cplusplus
 }

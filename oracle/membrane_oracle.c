@@ -331,3 +331,44 @@ void or_membrane_setup(or_membrane *mb, const or_mem_params *mp, const double *X
     or_membrane_forces(mb, mp, X_initial, V0, 1, F_initial);
     free(V0);
 }
+
+/*
+ * Output-step energies (Solver_gpu.cu:1060-1139): volume, global area and
+ * ADE from the globals; local area (get_area_energy, spring_network.cu:1687-1698,
+ * which the reference scales with the global area modulus), local bending
+ * (get_bending_energy :1703-1715, launched with the flat |N| array as the
+ * nodal area, so edge e divides by entry e of the 2E array) and WLC springs
+ * (get_spring_energy :1721-1768, launched with 1 / wlc_ratio).  The reference
+ * adds only the first n_tet_Blocks block partials of the edge sums; here the
+ * sums run over every edge.
+ * out: spring, local area, local bending, global area, volume, ADE, total.
+ */
+void or_membrane_energies(const or_membrane *mb, const or_mem_params *mp, const double *X, double *out)
+{
+    const double x0 = 1.0 / mp->wlc_ratio;
+    const double e1 = 0.25 / (1.0 - x0) / (1.0 - x0) - 0.25 + x0;
+    const double e2 = x0 * x0 * (3.0 - 2.0 * x0) / (1.0 - x0) - 4.0 * e1 * x0;
+    double dv = mb->total_volume - mb->total_volume_0, da = mb->total_area - mb->total_area_0, dm = mb->mad - mb->mad_0;
+    double e_spring = 0.0, e_la = 0.0, e_lb = 0.0;
+    int e;
+    for (e = 0; e < mb->n_edges; e++) {
+        const int *s = mb->spring + 4 * e;
+        d3 l = sub(ld(X, s[2]), ld(X, s[0]));
+        double Lm = mb->contour_length[e], x = sqrt(dot(l, l)) / Lm;
+        double a1 = mb->spring_area[2 * e] - mb->spring_area_0[2 * e];
+        double a2 = mb->spring_area[2 * e + 1] - mb->spring_area_0[2 * e + 1];
+        double dk = mb->curvature[e] - mb->curvature_0[e];
+        e_la += mp->global_area_modulus * 0.5 *
+                (a1 * a1 / mb->spring_area_0[2 * e] + a2 * a2 / mb->spring_area_0[2 * e + 1]);
+        e_lb += 0.5 * mp->local_bending_modulus * dk * dk / mb->spring_area[e];
+        e_spring += 0.5 * mb->spring_constant[e] *
+                    fabs(0.25 * Lm * (x * x * (3.0 - 2.0 * x) / (1.0 - x) - 4.0 * x * e1 - e2));
+    }
+    out[0] = e_spring;
+    out[1] = e_la;
+    out[2] = e_lb;
+    out[3] = 0.5 * mp->global_area_modulus * da * da / mb->total_area_0;
+    out[4] = 0.5 * mp->volume_modulus * dv * dv / mb->total_volume_0;
+    out[5] = 0.5 * mp->global_bending_modulus * dm * dm / mb->total_area_0;
+    out[6] = out[0] + out[1] + out[2] + out[3] + out[4] + out[5];
+}

@@ -102,6 +102,8 @@ def load():
         lib.or_membrane_forces.argtypes = [mbp, mpp, _dp, _dp, C.c_int, _dp]
         lib.or_membrane_globals.restype = None
         lib.or_membrane_globals.argtypes = [mbp, _dp, _dp, _dp]
+        lib.or_membrane_energies.restype = None
+        lib.or_membrane_energies.argtypes = [mbp, mpp, _dp, _dp]
         lib.or_internal_viscosity.restype = None
         lib.or_internal_viscosity.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, _ip, C.c_double, C.c_double, _dp]
         _lib = lib
@@ -285,6 +287,14 @@ class OracleMembrane:
     def globals(self):
         m = self.mb
         return np.array([m.total_volume, m.total_area, m.mad, m.total_volume_0, m.total_area_0, m.mad_0])
+
+    def energies(self, X=None):
+        """spring, local area, local bending, global area, volume, ADE, total
+        from the state of the last force call."""
+        X = self.X if X is None else np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+        out = np.zeros(7)
+        load().or_membrane_energies(C.byref(self.mb), C.byref(self.mp), ptr(X), ptr(out))
+        return out
 
 
 class OracleCoupled:

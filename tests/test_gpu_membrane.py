@@ -140,3 +140,14 @@ def test_graph_replays_match_eager_launches():
         sg, se = np.concatenate(g.get_state()), np.concatenate(e.get_state())
         assert np.array_equal(sg, se), rep
         assert np.array_equal(g.membrane_nodes()[0], e.membrane_nodes()[0]), rep
+
+
+def test_membrane_energies_match_oracle():
+    """Output-step energies after 20 coupled steps (Solver_gpu.cu:1060-1139)
+    against the oracle's, per term (FP32 state, FP64 sums)."""
+    p = dict(PARAMS, local_bending_modulus=2e-3, global_bending_modulus=5e-4)
+    s, cp, mo, x0 = coupled_pair(abi.IBM_COS4, 20, params=p)
+    e = s.membrane_energies()[0]
+    eo = mo.energies()
+    assert np.all(eo[[0, 1, 6]] > 0)
+    np.testing.assert_allclose(e, eo, rtol=2e-3, atol=1e-12 * eo[6])

@@ -234,3 +234,48 @@ def test_internal_viscosity_marks_interior():
     assert np.all(vf[r < 4.0] == 5.0)
     assert np.all(vf[r > 6.0] == 1.0)
     assert np.all((vf >= 1.0) & (vf <= 5.0))
+
+
+def test_energies_vanish_at_rest_and_spring_energy_matches_wlc_force():
+    """Output-step energies (Solver_gpu.cu:1060-1139): every term is zero in
+    the stress-free reference state; with only WLC springs active, the
+    spring energy's gradient is half the negative WLC force (the reference
+    halves the sum, counting each spring twice), checked by central
+    differences against the force routine; the global terms equal their
+    closed forms under a uniform scaling."""
+    x0, t = small_rbc(freq=3, d0=6.0)
+    params = dict(NO_BEND, volume_ratio=1.0)
+    m = orc.OracleMembrane(x0, t, params)
+    e = m.energies()
+    assert np.abs(e).max() < 1e-14
+    wlc_only = dict(params, volume_modulus=0.0, local_area_modulus=0.0, global_area_modulus=0.0)
+    w = orc.OracleMembrane(x0, t, wlc_only)
+    X = x0 + 0.05 * np.random.default_rng(7).standard_normal(x0.shape)
+    F = w.forces(X).reshape(-1)
+    Xf = X.reshape(-1)
+    h = 1e-6
+    for j in np.random.default_rng(8).choice(len(Xf), 12, replace=False):
+        xp, xm = Xf.copy(), Xf.copy()
+        xp[j] += h
+        xm[j] -= h
+        w.forces(xp)
+        ep = w.energies(xp)[0]
+        w.forces(xm)
+        em = w.energies(xm)[0]
+        g = (ep - em) / (2 * h)
+        assert abs(g + 0.5 * F[j]) < 1e-8 + 1e-5 * abs(F[j]), (j, g, F[j])
+    # closed-form global terms after a uniform scaling
+    sc = 1.05
+    Xs = x0 * sc
+    m.forces(Xs)
+    e = m.energies(Xs)
+    g = m.globals()
+    V, A, V0, A0 = g[0], g[1], g[3], g[4]
+    assert abs(V / V0 - sc ** 3) < 1e-12 and abs(A / A0 - sc ** 2) < 1e-12
+    assert abs(e[4] - 0.5 * params["volume_modulus"] * (V - V0) ** 2 / V0) < 1e-15
+    assert abs(e[3] - 0.5 * params["global_area_modulus"] * (A - A0) ** 2 / A0) < 1e-15
+    # local area term: every |N| scales by sc^2
+    n0 = m.arr["spring_area_0"]
+    exp_la = params["global_area_modulus"] * 0.5 * np.sum((sc * sc - 1) ** 2 * n0)
+    assert abs(e[1] - exp_la) < 1e-12 * exp_la
+    assert abs(e[6] - e[:6].sum()) < 1e-15
