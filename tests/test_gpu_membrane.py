@@ -150,4 +150,4 @@ def test_membrane_energies_match_oracle():
     e = s.membrane_energies()[0]
     eo = mo.energies()
     assert np.all(eo[[0, 1, 6]] > 0)
-    np.testing.assert_allclose(e, eo, rtol=2e-3, atol=1e-12 * eo[6])
+    np.testing.assert_allclose(e, eo, rtol=2e-3, atol=2e-4 * eo[6])
