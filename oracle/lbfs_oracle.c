@@ -522,16 +522,57 @@ void or_residual(const or_mesh *m, const double *flux, double *res)
 /* Forward-Euler conservative update (one Runge-Kutta stage, RungeKutta.h:12-14,
  * Solver.cpp:405-438; force added to the momentum residual as on the GPU,
  * Solver_gpu.cu:1008-1010 / LBFS.cu:2089-2172).  cons holds (rho, rho u). */
+/*
+ * Local time steps (get_cfl_device, LBFS.cu:2871-2936; areas of
+ * populate_cfl_areas, Solver_gpu.cu:1657-1690): with A_t = sum over the
+ * cell's faces of |n_t| A / 2 and c = U sqrt(1 - gamma + (gamma / sqrt 3 / U)^2),
+ * lambda = (U + c)(A_x + A_y + A_z) / gamma + 4 (2 nu / (gamma rho V)) |A|^2 and
+ * dt_i = CFL V / lambda.  OR_TS_MIN replaces every dt_i by their minimum
+ * (min<128> + min_block, Solver_gpu.cu:773-783); OR_TS_CONSTANT uses dt.
+ */
+void or_local_dt(const or_mesh *m, const or_params *p, const double *st, double *dt_cell)
+{
+    int i, f, t;
+    double U = p->max_velocity, g = p->gamma, mn = 1e300;
+    double c = U * sqrt(1.0 - g + pow(g / sqrt(3.0) / U, 2.0));
+    double *Ac = (double *)calloc(3 * (size_t)m->n_cells, sizeof(double));
+    for (f = 0; f < m->n_faces; f++) {
+        int o = m->owner[f], nb = m->nbr[f];
+        for (t = 0; t < 3; t++) {
+            double a = fabs(m->normal[3 * f + t] * m->area[f]);
+            Ac[3 * o + t] += 0.5 * a;
+            if (nb < m->n_cells) Ac[3 * nb + t] += 0.5 * a;
+        }
+    }
+    for (i = 0; i < m->n_cells; i++) {
+        const double *A = Ac + 3 * i;
+        double lam = (U + c) * (A[0] + A[1] + A[2]) / g;
+        double ve = 2.0 * p->visc / g / st[4 * i] / m->vol[i] * (A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
+        dt_cell[i] = p->dt * m->vol[i] / (lam + 4.0 * ve);
+        if (dt_cell[i] < mn) mn = dt_cell[i];
+    }
+    if (p->time_stepping == OR_TS_MIN)
+        for (i = 0; i < m->n_cells; i++) dt_cell[i] = mn;
+    free(Ac);
+}
+
 void or_update(const or_mesh *m, const or_params *p, const double *res, const double *force,
                double *cons, double *st)
 {
     int i, v;
+    double *dtc = NULL;
+    if (p->time_stepping == OR_TS_LOCAL || p->time_stepping == OR_TS_MIN) {
+        dtc = (double *)malloc(sizeof(double) * (size_t)m->n_cells);
+        or_local_dt(m, p, st, dtc);
+    }
     for (i = 0; i < m->n_cells; i++) {
-        cons[4 * i] += p->dt * res[4 * i];
-        for (v = 1; v < 4; v++) cons[4 * i + v] += p->dt * (res[4 * i + v] + (force ? force[3 * i + v - 1] : 0.0));
+        double dt = dtc ? dtc[i] : p->dt;
+        cons[4 * i] += dt * res[4 * i];
+        for (v = 1; v < 4; v++) cons[4 * i + v] += dt * (res[4 * i + v] + (force ? force[3 * i + v - 1] : 0.0));
         st[4 * i] = cons[4 * i];
         for (v = 1; v < 4; v++) st[4 * i + v] = cons[4 * i + v] / cons[4 * i];
     }
+    free(dtc);
 }
 
 /* One full fluid time step (Solver.cpp:266-439 with RK stages = 1). */

@@ -58,7 +58,15 @@ typedef struct {
     double gamma;    /* preconditioning gamma (1 = none) */
     double dt;       /* time-marching step */
     double channel_diameter; /* H of the parabolic (9) and shear (10) velocity BCs, domain.Y in the reference */
+    int time_stepping;       /* OR_TS_*: gpu_time_stepping of Solver_gpu.cu:767-790 */
+    double max_velocity;     /* U of the CFL estimate (global_variables) */
 } or_params;
+
+/* time stepping: 1 local, 2 constant dt, 3 global minimum of the local steps;
+ * with 1 and 3, dt is the CFL factor (time_marching_step) */
+#define OR_TS_LOCAL 1
+#define OR_TS_CONSTANT 2
+#define OR_TS_MIN 3
 
 /* boundary condition types (preprocessor.cpp:469-491); OR_BC_EMPTY marks the
  * faces of an "empty" patch, which carry no flux (Solver.cpp:308-310) */
@@ -87,6 +95,7 @@ void or_gradients(const or_mesh *m, const or_params *p, const double *st, double
 void or_face_flux(const or_mesh *m, const or_params *p, const double *st, const double *g,
                   double *flux /* [4 n_faces]: mass, mom x, y, z */);
 void or_residual(const or_mesh *m, const double *flux, double *res /* [4 n_cells] */);
+void or_local_dt(const or_mesh *m, const or_params *p, const double *st, double *dt_cell);
 void or_update(const or_mesh *m, const or_params *p, const double *res, const double *force,
                double *cons /* [4 n_cells] rho, rho u, rho v, rho w */, double *st);
 int or_fluid_step(const or_mesh *m, const or_params *p, double *st, double *cons, const double *force,

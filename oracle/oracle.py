@@ -31,7 +31,11 @@ class OrMesh(C.Structure):
 
 class OrParams(C.Structure):
     _fields_ = [("lattice", C.c_int), ("flags", C.c_int), ("visc", C.c_double), ("gamma", C.c_double),
-                ("dt", C.c_double), ("channel_diameter", C.c_double)]
+                ("dt", C.c_double), ("channel_diameter", C.c_double), ("time_stepping", C.c_int),
+                ("max_velocity", C.c_double)]
+
+
+OR_TS_LOCAL, OR_TS_CONSTANT, OR_TS_MIN = 1, 2, 3
 
 
 OR_BC_DIRICHLET, OR_BC_NEUMANN, OR_BC_PARABOLIC, OR_BC_SHEAR, OR_BC_EMPTY = 1, 2, 9, 10, 11
@@ -86,6 +90,8 @@ def load():
         lib.or_fluid_step.restype = C.c_int
         lib.or_fluid_step.argtypes = [om, op, _dp, _dp, _dp, _dp, _dp, _dp]
         lib.or_residual_rms.argtypes = [C.c_int, _dp, C.c_double, C.c_double, _dp]
+        lib.or_local_dt.restype = None
+        lib.or_local_dt.argtypes = [om, op, _dp, _dp]
         for name in ("or_bc_update", "or_gradients", "or_face_flux", "or_residual", "or_update", "or_residual_rms"):
             getattr(lib, name).restype = None
         og = C.POINTER(OrGrid)
@@ -139,7 +145,8 @@ class OracleFluid:
     """
 
     def __init__(self, mesh, lattice_kind, visc, dt, gamma=1.0, fneq_min=0.0, flags=0, grad_type=0,
-                 det_compat=False, visc_factor=None, closed=False, channel_diameter=0.0):
+                 det_compat=False, visc_factor=None, closed=False, channel_diameter=0.0,
+                 time_stepping=OR_TS_CONSTANT, max_velocity=0.01):
         lib = load()
         self.mesh = mesh
         nc, nf, ni = mesh.n_cells, mesh.n_faces, mesh.n_internal
@@ -182,7 +189,7 @@ class OracleFluid:
         rc = lib.or_ls_setup(C.byref(m), grad_type, 1 if det_compat else 0, ptr(self.lhs), ptr(self.rhs))
         if rc != 0:
             raise RuntimeError("or_ls_setup failed")
-        self.p = OrParams(lattice_kind, flags, visc, gamma, dt, channel_diameter)
+        self.p = OrParams(lattice_kind, flags, visc, gamma, dt, channel_diameter, time_stepping, max_velocity)
         self.st = np.zeros(4 * (nc + nb))
         self.cons = np.zeros(4 * nc)
         self.g = np.zeros(12 * (nc + nb))
@@ -203,6 +210,12 @@ class OracleFluid:
         nc = self.nc
         s = self.st[: 4 * nc]
         return s[0::4].copy(), s[1::4].copy(), s[2::4].copy(), s[3::4].copy()
+
+    def local_dt(self):
+        """Per-cell time steps of the current state (OR_TS_LOCAL / OR_TS_MIN)."""
+        out = np.zeros(self.nc)
+        load().or_local_dt(C.byref(self.m), C.byref(self.p), ptr(self.st), ptr(out))
+        return out
 
     def gradients(self):
         lib = load()

@@ -298,3 +298,51 @@ def test_extrapolated_face_values_for_profile_bcs():
     inlet = m.n_internal + np.nonzero(m.bc_patch[: m.n_bc] == 0)[0]
     # with the owner at rest, the extrapolated face velocity is the ghost-gradient step toward the target
     assert np.all(np.abs(fl[inlet, 0]) > 0)
+
+
+def test_local_time_step_closed_forms():
+    """get_cfl_device (LBFS.cu:2871-2936): on unit cubes A_x = A_y = A_z = 1,
+    and with gamma = 1 the effective sound speed is 1/sqrt(3), so
+    dt = CFL / (3 (U + 1/sqrt 3) + 24 nu / rho).  Scaling the mesh by 2
+    doubles the inviscid step and quadruples the viscous one; the "min" mode
+    gives every cell the smallest local step."""
+    U, cfl = 0.05, 0.4
+    m = abi.HexMesh(6, 5, 4, periodic=(1, 1, 1), jitter=0.0)
+    n = m.n_cells
+    rho = 1.0 + 0.01 * np.random.default_rng(3).standard_normal(n)
+    o = orc.OracleFluid(m, orc.OR_D3Q19, visc=0.02, dt=cfl, time_stepping=orc.OR_TS_LOCAL, max_velocity=U)
+    o.set_state(rho, np.zeros(n), np.zeros(n), np.zeros(n))
+    dt = o.local_dt()
+    np.testing.assert_allclose(dt, cfl / (3 * (U + 1 / np.sqrt(3)) + 24 * 0.02 / rho), rtol=1e-14)
+    m2 = abi.HexMesh(6, 5, 4, lengths=(12.0, 10.0, 8.0), periodic=(1, 1, 1), jitter=0.0)
+    for visc, ratio in ((0.0, 2.0), (1e6, 4.0)):
+        a = orc.OracleFluid(m, orc.OR_D3Q19, visc=visc, dt=cfl, time_stepping=orc.OR_TS_LOCAL, max_velocity=U)
+        b = orc.OracleFluid(m2, orc.OR_D3Q19, visc=visc, dt=cfl, time_stepping=orc.OR_TS_LOCAL, max_velocity=U)
+        for x in (a, b):
+            x.set_state(np.ones(n), np.zeros(n), np.zeros(n), np.zeros(n))
+        np.testing.assert_allclose(b.local_dt() / a.local_dt(), ratio, rtol=1e-6 if visc else 1e-14)
+    j = abi.HexMesh(6, 5, 4, periodic=(1, 0, 0), jitter=0.1, seed=3)
+    lo = orc.OracleFluid(j, orc.OR_D3Q19, visc=0.02, dt=cfl, time_stepping=orc.OR_TS_LOCAL, max_velocity=U)
+    mn = orc.OracleFluid(j, orc.OR_D3Q19, visc=0.02, dt=cfl, time_stepping=orc.OR_TS_MIN, max_velocity=U)
+    for x in (lo, mn):
+        x.set_state(rho, np.zeros(n), np.zeros(n), np.zeros(n))
+    dl, dm = lo.local_dt(), mn.local_dt()
+    assert np.ptp(dl) > 0 and np.all(dm == dl.min())
+
+
+def test_local_time_stepping_reaches_the_same_steady_channel():
+    """Local time steps change the path, not the steady state: a body-force
+    channel (closed faces) run to steady state with constant, local and min
+    stepping agrees to 1e-4 relative (the remaining difference is the slow
+    drift still present in the constant-step run)."""
+    m = abi.HexMesh(4, 8, 6, periodic=(1, 0, 0), jitter=0.1, seed=2)
+    n = m.n_cells
+    body = np.tile([1e-5, 0.0, 0.0], n)
+    out = []
+    for ts, dt in ((orc.OR_TS_CONSTANT, 0.25), (orc.OR_TS_LOCAL, 0.5), (orc.OR_TS_MIN, 0.5)):
+        o = orc.OracleFluid(m, orc.OR_D3Q19, visc=0.1, dt=dt, time_stepping=ts, max_velocity=0.01, closed=True)
+        o.set_state(np.ones(n), np.zeros(n), np.zeros(n), np.zeros(n))
+        o.step(6000, force=body)
+        out.append(o.state()[1])
+    assert np.all(out[0] > 0)
+    assert cases.rel_l2(out[1], out[0]) < 1e-4 and cases.rel_l2(out[2], out[0]) < 1e-4
