@@ -101,7 +101,19 @@ typedef struct {
     double rho0;        /* reference density; the device stores rho - rho0 */
     double body_force[3];
     double channel_diameter; /* H of LBFS_BC_PARABOLIC (domain.Y, Solver_gpu.cu:935); 0 if unused */
+    int time_stepping;  /* LBFS_TS_*; 0 = constant dt */
+    double max_velocity;/* U of the CFL estimate (LBFS_TS_LOCAL / LBFS_TS_MIN) */
 } lbfs_params;
+
+/* time stepping (gpu_time_stepping, Solver_gpu.cu:767-790; get_cfl_device
+ * LBFS.cu:2871-2936): with LOCAL and MIN, dt above is the CFL factor and each
+ * cell's step is dt V / lambda with lambda = (U + c)(A_x + A_y + A_z) / gamma
+ * + 8 nu |A|^2 / (gamma rho V), A_t = sum over the cell's faces of |n_t| A / 2,
+ * c = U sqrt(1 - gamma + (gamma / sqrt 3 / U)^2); MIN uses their minimum.
+ * Membranes need constant stepping. */
+#define LBFS_TS_LOCAL 1
+#define LBFS_TS_CONSTANT 2
+#define LBFS_TS_MIN 3
 
 typedef struct lbfs_ctx lbfs_ctx;
 
@@ -162,6 +174,8 @@ typedef struct {
     int mad_compat;                   /* 1: MAD over the first n_nodes edges (Solver_gpu.cu:2355) */
     double internal_viscosity_ratio;  /* interior / exterior viscosity; 1 leaves the field at 1
                                          (Solver_gpu.cu:884-889, lagrangian_object.h:85) */
+    int n_springs;                    /* 0: springs built from the triangles (lagrangian_object.cpp:322-490) */
+    const int *springs;               /* [4 n_springs] a, b, c, d as read from an MZ file (mesh_read) */
 } lbfs_membrane_desc;
 
 /* attach membranes (replaces any attached before); requires the mesh's logical
@@ -186,6 +200,18 @@ int lbfs_mesh_icosphere(int frequency, double *nodes, int *tri);
 int lbfs_mesh_evans_fung(int n_nodes, const double *unit_nodes, double d0, double *nodes);
 /* springs (a,b,c,d), edge a-c, of a closed triangulation (lagrangian_object.cpp:322-490) */
 int lbfs_mesh_springs(int n_nodes, int n_tri, const int *tri, int *springs, int *n_springs);
+
+/* membrane mesh files (lagrangian_object.cpp): Osay "N T" + xyz + 1-based
+ * triangles (osay_mesh_read 238-302); MZ "N S T" + xyz + triangles + 1-based
+ * springs a b c d + per-node neighbour lists (mesh_read 495-587).  Read
+ * positions are shifted by centre (the object's centre_x/y/z) as there. */
+#define LBFS_MESH_OSAY 0
+#define LBFS_MESH_MZ 1
+int lbfs_membrane_mesh_counts(const char *path, int format, int *n_nodes, int *n_tri, int *n_springs);
+int lbfs_membrane_mesh_read(const char *path, int format, const double centre[3], double *nodes, int *tri,
+                            int *springs /* [4 n_springs] or NULL */);
+int lbfs_membrane_mesh_write(const char *path, int format, int n_nodes, const double *nodes, int n_tri,
+                             const int *tri, int n_springs, const int *springs);
 
 /* ---- measurement (no counterpart in the reference; used by bench.py) ----
  * HIP events recorded on the context's stream.  lbfs_time_steps: wall time of

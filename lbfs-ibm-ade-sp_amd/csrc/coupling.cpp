@@ -46,6 +46,7 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     if (!ctx || !mem || count <= 0) return LBFS_E_INVALID_ARG;
     if (ibm_kernel != LBFS_IBM_LINEAR2 && ibm_kernel != LBFS_IBM_COS4)
         return ctx->fail(LBFS_E_INVALID_ARG, "unknown IBM kernel %d", ibm_kernel);
+    if (ctx->args.ts_mode != 0) return ctx->fail(LBFS_E_UNSUPPORTED, "membranes need constant time steps");
     if (ctx->mesh_nx <= 0 || ctx->mesh_ny <= 0 || ctx->mesh_nz <= 0 || !(ctx->mesh_spacing > 0.0) ||
         (long long)ctx->mesh_nx * ctx->mesh_ny * ctx->mesh_nz != ctx->lay.n_cells)
         return ctx->fail(LBFS_E_UNSUPPORTED, "membranes need a mesh with a logical i,j,k structure and spacing");

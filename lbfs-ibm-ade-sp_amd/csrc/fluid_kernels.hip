@@ -252,7 +252,13 @@ __global__ __launch_bounds__(256) void k_flux_update(FluidArgs a, const float4 *
         }
         r1 += fx; r2 += fy; r3 += fz;
         const float4 s0 = st[i];
-        const float dt = a.dt;
+        float dt = a.dt;
+        if (a.ts_mode == LBFS_TS_LOCAL) {
+            const float2 c = a.cfl_coef[i];
+            dt = a.dt / (c.x + c.y / (a.rho0 + s0.x));
+        } else if (a.ts_mode == LBFS_TS_MIN) {
+            dt = __uint_as_float(*a.dt_min);
+        }
         const float drho = s0.x + dt * r0;
         const float rho = a.rho0 + drho;
         const float inv_rho = 1.0f / rho;
@@ -272,6 +278,31 @@ __global__ __launch_bounds__(256) void k_face_flux_all(FluidArgs a, const float4
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f < n_tile_faces) out[f] = face_flux<L, BAND>(a, st, grad, f);
+}
+
+// ---------------------------------------------------------------------------
+// global minimum of the local time steps (min<128> + min_block,
+// Solver_gpu.cu:773-776): positive floats order like their bit patterns, so an
+// unsigned atomic minimum is exact and independent of the visiting order
+// ---------------------------------------------------------------------------
+__global__ void k_dt_reset(unsigned int *dt_min) { *dt_min = 0x7f800000u; }
+
+__global__ __launch_bounds__(256) void k_dt_min(FluidArgs a, const float4 *__restrict__ st)
+{
+    __shared__ float sh[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    float v = INFINITY;
+    if (i < a.n_cells) {
+        const float2 c = a.cfl_coef[i];
+        v = a.dt / (c.x + c.y / (a.rho0 + st[i].x));
+    }
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int stp = 128; stp > 0; stp >>= 1) {
+        if ((int)threadIdx.x < stp) sh[threadIdx.x] = fminf(sh[threadIdx.x], sh[threadIdx.x + stp]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMin(a.dt_min, __float_as_uint(sh[0]));
 }
 
 // ---------------------------------------------------------------------------
@@ -364,6 +395,13 @@ hipError_t launch_face_flux_all(const FluidArgs &a, const float4 *st, const floa
 {
     const int blocks = (n_tile_faces + 255) / 256;
     return dispatch<FaceFluxAll>(a, dim3(blocks), 0, s, st, grad, out, n_tile_faces);
+}
+
+hipError_t launch_dt_min(const FluidArgs &a, const float4 *st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(1), 0, s, a.dt_min);
+    hipLaunchKernelGGL(k_dt_min, dim3((a.n_cells + 255) / 256), dim3(256), 0, s, a, st);
+    return hipGetLastError();
 }
 
 hipError_t launch_diag_partials(int n, float rho0, const float4 *st, const float4 *res, const float *inv_vol, double *part,

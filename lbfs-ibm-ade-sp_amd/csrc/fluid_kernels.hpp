@@ -28,6 +28,11 @@ struct FluidArgs {
     // boundary conditions
     const int2 *bc_type;           // (rho type, vel type)
     const float4 *bc_val;          // (rho - rho0, u, v, w)
+    // time stepping: 0 constant dt; LBFS_TS_LOCAL: dt_i = dt / (c.x + c.y / rho_i);
+    // LBFS_TS_MIN: the minimum of those, reduced into *dt_min before the update
+    int ts_mode;
+    const float2 *cfl_coef;        // [n] or null
+    unsigned int *dt_min;          // float bits of the global step (LBFS_TS_MIN)
 };
 
 hipError_t launch_gradients(const FluidArgs &a, const float4 *st, float4 *grad, hipStream_t s);
@@ -35,6 +40,8 @@ hipError_t launch_flux_update(const FluidArgs &a, const float4 *st, const float4
                               float4 *res_out, int do_update, hipStream_t s);
 hipError_t launch_face_flux_all(const FluidArgs &a, const float4 *st, const float4 *grad, float4 *out,
                                 int n_tile_faces, hipStream_t s);
+// global minimum of the local steps into *a.dt_min (LBFS_TS_MIN)
+hipError_t launch_dt_min(const FluidArgs &a, const float4 *st, hipStream_t s);
 hipError_t launch_diag_partials(int n, float rho0, const float4 *st, const float4 *res, const float *inv_vol,
                                 double *part, hipStream_t s);
 

@@ -141,6 +141,37 @@ static int fluid_setup(lbfs_ctx *ctx, const lbfs_mesh_desc &m, const lbfs_params
     ctx->static_force = sf;
     ctx->cell_force = cf;
     a.cell_force = cf;
+    // local / minimum time steps: per-cell coefficients of dt V / lambda
+    if (p.time_stepping == LBFS_TS_LOCAL || p.time_stepping == LBFS_TS_MIN) {
+        if (!(p.max_velocity > 0.0)) return ctx->fail(LBFS_E_INVALID_ARG, "local time stepping needs max_velocity > 0");
+        const Geometry &g = ctx->geo;
+        std::vector<double> Ac(3 * (size_t)n, 0.0);
+        for (int f = 0; f < g.n_faces; f++)
+            for (int t = 0; t < 3; t++) {
+                const double v = 0.5 * std::fabs(g.normal[3 * (size_t)f + t] * g.area[f]);
+                Ac[3 * (size_t)g.owner[f] + t] += v;
+                if (g.nbr[f] < n) Ac[3 * (size_t)g.nbr[f] + t] += v;
+            }
+        const double U = p.max_velocity, gm = p.gamma;
+        const double c = U * std::sqrt(1.0 - gm + std::pow(gm / std::sqrt(3.0) / U, 2.0));
+        std::vector<float2> coef(n);
+        for (int i = 0; i < n; i++) {
+            const int cc = d.perm[i];
+            const double *A = &Ac[3 * (size_t)cc];
+            const double V = g.vol[cc];
+            coef[i] = make_float2((float)((U + c) * (A[0] + A[1] + A[2]) / (gm * V)),
+                                  (float)(8.0 * p.visc * (A[0] * A[0] + A[1] * A[1] + A[2] * A[2]) / (gm * V * V)));
+        }
+        float2 *dcoef;
+        unsigned int *dmin;
+        if ((rc = dev_upload(ctx, &dcoef, coef.data(), coef.size()))) return rc;
+        if ((rc = dev_upload<unsigned int>(ctx, &dmin, nullptr, 1))) return rc;
+        a.cfl_coef = dcoef;
+        a.dt_min = dmin;
+        a.ts_mode = p.time_stepping;
+    } else if (p.time_stepping != 0 && p.time_stepping != LBFS_TS_CONSTANT) {
+        return ctx->fail(LBFS_E_INVALID_ARG, "unknown time stepping %d", p.time_stepping);
+    }
     for (int t = 0; t < 3; t++) a.body_force[t] = 0.0f;
     // state buffers
     if ((rc = dev_upload<float4>(ctx, &ctx->st[0], nullptr, n))) return rc;

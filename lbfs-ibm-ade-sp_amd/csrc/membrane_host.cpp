@@ -4,7 +4,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <array>
 #include <map>
+#include <set>
+#include <string>
 #include <tuple>
 #include <unordered_set>
 
@@ -132,8 +135,30 @@ int membrane_setup(const lbfs_membrane_desc &m, MembraneSetup &s, std::string &e
     s.n_nodes = m.n_nodes;
     s.n_tri = m.n_tri;
     s.tri.assign(m.tri, m.tri + 3 * (size_t)m.n_tri);
-    int rc = build_springs(m.n_nodes, s.tri, s.springs, err);
-    if (rc) return rc;
+    int rc = LBFS_OK;
+    if (m.n_springs > 0 && m.springs) {
+        // springs given with the mesh (MZ files): each must be an edge a-c
+        // shared by the triangles (a, b, c) and (a, c, d)
+        if (2 * m.n_springs != 3 * m.n_tri) { err = "spring count must be 3/2 of the triangle count"; return LBFS_E_INVALID_ARG; }
+        std::set<std::array<int, 3>> tris;
+        for (int t = 0; t < m.n_tri; t++) {
+            const int *q = &s.tri[3 * (size_t)t];
+            for (int r = 0; r < 3; r++) tris.insert({q[r], q[(r + 1) % 3], q[(r + 2) % 3]});
+        }
+        s.springs.assign(m.springs, m.springs + 4 * (size_t)m.n_springs);
+        for (int e = 0; e < m.n_springs; e++) {
+            const int *q = &s.springs[4 * (size_t)e];
+            for (int r = 0; r < 4; r++)
+                if (q[r] < 0 || q[r] >= m.n_nodes) { err = "spring node out of range"; return LBFS_E_INVALID_ARG; }
+            if (!tris.count({q[0], q[1], q[2]}) || !tris.count({q[0], q[2], q[3]})) {
+                err = "spring " + std::to_string(e) + " is not an edge between triangles abc and acd";
+                return LBFS_E_INVALID_ARG;
+            }
+        }
+    } else {
+        rc = build_springs(m.n_nodes, s.tri, s.springs, err);
+        if (rc) return rc;
+    }
     const int ne = (int)s.springs.size() / 4;
     s.n_edges = ne;
     s.X.assign(m.nodes, m.nodes + 3 * (size_t)m.n_nodes);

@@ -73,6 +73,12 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
                      return LBFS_OK;
                  },
                  196.0 * n});
+    if (args.ts_mode == LBFS_TS_MIN)
+        s.push_back({"dt_min", [ctx]() -> int {
+                         HIPCHK(launch_dt_min(ctx->args, ctx->st[ctx->cur], ctx->stream));
+                         return LBFS_OK;
+                     },
+                     24.0 * n});
     // pass F: state 16 + gradients 48 + volume/centroid 16 + force 12 + state written 16 per cell,
     // plus owner/nb 8, normal 12, face centroid 12, area 4, dt 4 per face
     s.push_back({"flux_update", [ctx]() -> int {

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(_HERE, "liblbfs_amd.so")
 D3Q15, D3Q19, D2Q9 = 0, 1, 2
 GRAD_HYBRID, GRAD_LS, GRAD_GG = 0, 1, 2
 BC_DIRICHLET, BC_NEUMANN, BC_PARABOLIC, BC_SHEAR, BC_EMPTY = 1, 2, 9, 10, 11
+TS_LOCAL, TS_CONSTANT, TS_MIN = 1, 2, 3
+MESH_OSAY, MESH_MZ = 0, 1
 FLAG_UPWIND_BAND, FLAG_NEUMANN_SIGNED, FLAG_GHOST_DIR_GPU, FLAG_LS_DET_COMPAT, FLAG_CLOSED_FACES = 1, 2, 4, 16, 32
 IBM_LINEAR2, IBM_COS4 = 0, 1
 OPT_GRAPH = 1
@@ -38,6 +40,7 @@ class Params(C.Structure):
         ("lattice", C.c_int), ("gradient_type", C.c_int), ("flags", C.c_int),
         ("visc", C.c_double), ("gamma", C.c_double), ("fneq_min", C.c_double), ("dt", C.c_double),
         ("rho0", C.c_double), ("body_force", C.c_double * 3), ("channel_diameter", C.c_double),
+        ("time_stepping", C.c_int), ("max_velocity", C.c_double),
     ]
 
 
@@ -57,6 +60,7 @@ class MembraneDesc(C.Structure):
         ("global_bending_modulus", C.c_double), ("local_bending_modulus", C.c_double),
         ("viscosity", C.c_double), ("curvature_ratio", C.c_double), ("volume_ratio", C.c_double),
         ("sphere_reference", C.c_int), ("mad_compat", C.c_int), ("internal_viscosity_ratio", C.c_double),
+        ("n_springs", C.c_int), ("springs", _ip),
     ]
 
 
@@ -110,6 +114,9 @@ def load():
                                                  _dp, _ip, _ip, _ip, _dp, _ip]),
             "lbfs_mesh_geometry": (C.c_int, [C.POINTER(MeshDesc), C.c_double, C.c_double, C.c_double, C.c_int,
                                              _dp, _dp, _dp, _dp, _dp, _dp]),
+            "lbfs_membrane_mesh_counts": (C.c_int, [C.c_char_p, C.c_int, _ip, _ip, _ip]),
+            "lbfs_membrane_mesh_read": (C.c_int, [C.c_char_p, C.c_int, _dp, _dp, _ip, _ip]),
+            "lbfs_membrane_mesh_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip]),
             "lbfs_polymesh_counts": (C.c_int, [C.c_char_p, _ip, _ip, _ip, _ip, _ip]),
             "lbfs_polymesh_read": (C.c_int, [C.c_char_p, _dp, _ip, _ip, _ip, _ip, _dp]),
             "lbfs_polymesh_patch": (C.c_int, [C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int]),
@@ -325,12 +332,38 @@ def rbc(frequency=16, d0=20.0, center=(0.0, 0.0, 0.0), rot=None):
     return x + np.asarray(center, dtype=np.float64), tri
 
 
+def read_membrane_mesh(path, fmt=MESH_OSAY, centre=(0.0, 0.0, 0.0)):
+    """Osay or MZ membrane mesh file -> (nodes [N,3], tri [T,3], springs [S,4] or None)."""
+    lib = load()
+    n, t, sp = C.c_int(), C.c_int(), C.c_int()
+    p = os.fsencode(str(path))
+    check(lib.lbfs_membrane_mesh_counts(p, fmt, C.byref(n), C.byref(t), C.byref(sp)))
+    nodes = np.zeros(3 * n.value)
+    tri = np.zeros(3 * t.value, dtype=np.int32)
+    springs = np.zeros(4 * max(sp.value, 1), dtype=np.int32)
+    c = np.array(centre, dtype=np.float64)
+    check(lib.lbfs_membrane_mesh_read(p, fmt, ptr(c), ptr(nodes), ptr(tri, _ip), ptr(springs, _ip)))
+    return nodes.reshape(-1, 3), tri.reshape(-1, 3), (springs[: 4 * sp.value].reshape(-1, 4) if sp.value else None)
+
+
+def write_membrane_mesh(path, nodes, tri, fmt=MESH_OSAY, springs_=None):
+    lib = load()
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1)
+    tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1)
+    if fmt == MESH_MZ and springs_ is None:
+        springs_ = springs(len(nodes) // 3, tri)
+    sp = None if springs_ is None else np.ascontiguousarray(springs_, dtype=np.int32).reshape(-1)
+    check(lib.lbfs_membrane_mesh_write(os.fsencode(str(path)), fmt, len(nodes) // 3, ptr(nodes), len(tri) // 3,
+                                       ptr(tri, _ip), 0 if sp is None else len(sp) // 4, ptr(sp, _ip)))
+
+
 class Membrane:
     """Host-side description of one membrane (keeps its arrays alive)."""
 
-    def __init__(self, nodes, tri, nodes_stress_free=None, nodes_curvature=None, **params):
+    def __init__(self, nodes, tri, nodes_stress_free=None, nodes_curvature=None, springs=None, **params):
         self.nodes = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1)
         self.tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1)
+        self.springs = None if springs is None else np.ascontiguousarray(springs, dtype=np.int32).reshape(-1)
         self.sf = None if nodes_stress_free is None else np.ascontiguousarray(nodes_stress_free, dtype=np.float64).reshape(-1)
         self.cv = None if nodes_curvature is None else np.ascontiguousarray(nodes_curvature, dtype=np.float64).reshape(-1)
         p = dict(RBC_PARAMS)
@@ -342,17 +375,21 @@ class Membrane:
         d.n_nodes, d.n_tri = len(self.nodes) // 3, len(self.tri) // 3
         d.nodes, d.tri = ptr(self.nodes), ptr(self.tri, _ip)
         d.nodes_stress_free, d.nodes_curvature = ptr(self.sf), ptr(self.cv)
+        if self.springs is not None:
+            d.n_springs, d.springs = len(self.springs) // 4, ptr(self.springs, _ip)
         for k in MEMBRANE_PARAM_NAMES:
             setattr(d, k, self.params[k])
         return d
 
 
 def make_params(lattice=D3Q19, visc=1.0 / 6.0, dt=0.5, rho0=1.0, gamma=1.0, fneq_min=0.0, flags=0,
-                gradient_type=GRAD_HYBRID, body_force=(0.0, 0.0, 0.0), channel_diameter=0.0):
+                gradient_type=GRAD_HYBRID, body_force=(0.0, 0.0, 0.0), channel_diameter=0.0, time_stepping=0,
+                max_velocity=0.0):
     p = Params()
     p.lattice, p.gradient_type, p.flags = lattice, gradient_type, flags
     p.visc, p.gamma, p.fneq_min, p.dt, p.rho0 = visc, gamma, fneq_min, dt, rho0
     p.channel_diameter = channel_diameter
+    p.time_stepping, p.max_velocity = time_stepping, max_velocity
     for t in range(3):
         p.body_force[t] = body_force[t]
     return p

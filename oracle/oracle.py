@@ -261,14 +261,17 @@ def build_springs(n_nodes, tri):
 class OracleMembrane:
     """FP64 spring-network membrane (setup + per-step forces)."""
 
-    def __init__(self, nodes, tri, params, nodes_stress_free=None, nodes_curvature=None):
+    def __init__(self, nodes, tri, params, nodes_stress_free=None, nodes_curvature=None, springs=None):
         lib = load()
         self.visc_ratio = float(params.get("internal_viscosity_ratio", 1.0))
         self.X = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1).copy()
         self.tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1)
         n = len(self.X) // 3
         self.n = n
-        self.spring = build_springs(n, self.tri).reshape(-1)
+        if springs is None:
+            self.spring = build_springs(n, self.tri).reshape(-1)
+        else:
+            self.spring = np.ascontiguousarray(springs, dtype=np.int32).reshape(-1).copy()
         ne = len(self.spring) // 4
         self.ne = ne
         self.arr = {k: np.zeros(ne) for k in ("contour_length", "spring_constant", "curvature_0", "spring_angle_0",

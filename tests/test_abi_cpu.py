@@ -101,3 +101,33 @@ def test_polymesh_rejects_non_quad(tmp_path):
     (tmp_path / "faces").write_text(txt)
     with pytest.raises(abi.LbfsError):
         abi.HexMesh.read_polymesh(tmp_path)
+
+
+@pytest.mark.parametrize("fmt", [abi.MESH_OSAY, abi.MESH_MZ])
+def test_membrane_mesh_files_round_trip(tmp_path, fmt):
+    """Osay and MZ membrane files (lagrangian_object.cpp:238-302, 495-587):
+    written and read back with the object centre added to every node; MZ
+    carries the springs, which equal the ones built from the triangles."""
+    x, t = abi.rbc(3, 7.0)
+    path = tmp_path / "cell.dat"
+    abi.write_membrane_mesh(path, x, t, fmt=fmt)
+    c = (1.5, -2.0, 3.25)
+    xr, tr, sp = abi.read_membrane_mesh(path, fmt=fmt, centre=c)
+    np.testing.assert_array_equal(tr, t.reshape(-1, 3))
+    np.testing.assert_allclose(xr, x.reshape(-1, 3) + np.array(c), rtol=0, atol=1e-12)
+    if fmt == abi.MESH_MZ:
+        np.testing.assert_array_equal(sp, abi.springs(len(x), t))
+        # degree lines: every node lists its spring neighbours
+        lines = path.read_text().split("\n")
+        n, s_, tt = (int(v) for v in lines[0].split())
+        deg = [int(l.split()[0]) for l in lines[1 + n + tt + s_: 1 + n + tt + s_ + n]]
+        assert sum(deg) == 2 * s_
+    else:
+        assert sp is None
+
+
+def test_membrane_mesh_rejects_bad_ids(tmp_path):
+    path = tmp_path / "bad.dat"
+    path.write_text("4 2\n0 0 0\n1 0 0\n0 1 0\n0 0 1\n1 2 3\n1 3 9\n")
+    with pytest.raises(abi.LbfsError):
+        abi.read_membrane_mesh(path)

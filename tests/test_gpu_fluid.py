@@ -191,3 +191,24 @@ def test_unsupported_bc_types_rejected():
     m.set_patch_bc(2, abi.BC_NEUMANN, abi.BC_PARABOLIC, (1.0, 0.01, 0.0, 0.0))
     with pytest.raises(abi.LbfsError):
         abi.Solver(m, abi.make_params())          # parabolic without channel_diameter
+
+
+@pytest.mark.parametrize("ts", [abi.TS_LOCAL, abi.TS_MIN])
+def test_local_and_min_time_stepping_parity(ts):
+    """gpu_time_stepping 1 (local) and 3 (min): per-cell CFL steps
+    (LBFS.cu:2871-2936) and their minimum; 80 steps against FP64, relative L2
+    of velocity <= 1e-5."""
+    m = cases.channel_mesh(16, 12, 10)
+    body = (2e-5, 0.0, 0.0)
+    cfl, U = 0.5, 0.02
+    s = abi.Solver(m, abi.make_params(lattice=abi.D3Q19, visc=0.05, dt=cfl, body_force=body, time_stepping=ts,
+                                      max_velocity=U))
+    o = orc.OracleFluid(m, orc.OR_D3Q19, visc=0.05, dt=cfl, time_stepping=ts, max_velocity=U)
+    st = cases.smooth_state(m, seed=16)
+    s.set_state(*st)
+    o.set_state(*st)
+    s.step(80)
+    o.step(80, force=np.tile(np.array(body), m.n_cells))
+    _, us, vs, ws = s.get_state()
+    _, uo, vo, wo = o.state()
+    assert cases.rel_l2(np.concatenate([us, vs, ws]), np.concatenate([uo, vo, wo])) < 1e-5

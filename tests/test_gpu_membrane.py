@@ -151,3 +151,28 @@ def test_membrane_energies_match_oracle():
     eo = mo.energies()
     assert np.all(eo[[0, 1, 6]] > 0)
     np.testing.assert_allclose(e, eo, rtol=2e-3, atol=2e-4 * eo[6])
+
+
+def test_springs_from_mz_file_parity(tmp_path):
+    """Springs read from an MZ file (mesh_read, lagrangian_object.cpp:495-587)
+    in another valid orientation (c, d, a, b) drive the same coupled step as
+    the oracle given those springs."""
+    x0, tri = abi.rbc(4, 8.0, center=(12.0, 8.0, 8.0))
+    sp = abi.springs(len(x0), tri)[:, [2, 3, 0, 1]]
+    path = tmp_path / "cell.mz"
+    abi.write_membrane_mesh(path, x0, tri, fmt=abi.MESH_MZ, springs_=sp)
+    xr, tr, spr = abi.read_membrane_mesh(path, fmt=abi.MESH_MZ)
+    mesh = cases.channel_mesh(24, 16, 16, seed=8)
+    s = abi.Solver(mesh, abi.make_params(visc=VISC, dt=DT, body_force=(2e-5, 0.0, 0.0)))
+    st = cases.smooth_state(mesh, seed=8, amp=0.005)
+    s.set_state(*st)
+    s.add_membranes([abi.Membrane(xr, tr, springs=spr, **PARAMS)])
+    fl = orc.OracleFluid(mesh, orc.OR_D3Q19, visc=VISC, dt=DT)
+    fl.set_state(*st)
+    mo = orc.OracleMembrane(xr, tr, PARAMS, springs=spr)
+    cp = orc.OracleCoupled(fl, mesh, [mo], abi.IBM_COS4, body_force=(2e-5, 0.0, 0.0))
+    s.step(20)
+    cp.step(20)
+    X, _, F = s.membrane_nodes()
+    assert cases.rel_l2(X - xr, mo.X.reshape(-1, 3) - xr) < 1e-4
+    assert cases.rel_l2(F, mo.F.reshape(-1, 3)) < 1e-3
