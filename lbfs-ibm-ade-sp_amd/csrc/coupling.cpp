@@ -202,9 +202,10 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
         cen[i] = make_float4((float)ctx->geo.cc[3 * (size_t)c], (float)ctx->geo.cc[3 * (size_t)c + 1],
                              (float)ctx->geo.cc[3 * (size_t)c + 2], (float)ctx->geo.vol[c]);
     }
-    int *dinv, *bk, *br, *bc, *bs, *so, *tf;
+    int *dinv, *bk, *br, *bc, *bs, *so, *tf, *sp;
     float4 *dcen;
-    float *ws;
+    float *wn;
+    int4 *nbs;
     if ((rc = cp_alloc(ctx, &dinv, ctx->lay.inv.data(), nc))) return rc;
     if ((rc = cp_alloc(ctx, &dcen, cen.data(), nc))) return rc;
     if ((rc = cp_alloc<int>(ctx, &bk, nullptr, nn))) return rc;
@@ -213,9 +214,11 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     if ((rc = cp_alloc<int>(ctx, &bs, nullptr, g.n_bins + 1))) return rc;
     if ((rc = cp_alloc<int>(ctx, &so, nullptr, nn))) return rc;
     if ((rc = cp_alloc<int>(ctx, &tf, nullptr, g.n_bins))) return rc;
-    if ((rc = cp_alloc<float>(ctx, &ws, nullptr, nn))) return rc;
+    if ((rc = cp_alloc<int>(ctx, &sp, nullptr, (g.n_bins + 1023) / 1024))) return rc;
+    if ((rc = cp_alloc<float>(ctx, &wn, nullptr, 64 * (size_t)nn))) return rc;
+    if ((rc = cp_alloc<int4>(ctx, &nbs, nullptr, nn))) return rc;
     g.inv = dinv; g.cen = dcen; g.bin_key = bk; g.bin_rank = br; g.bin_count = bc; g.bin_start = bs;
-    g.sorted = so; g.tile_flag = tf; g.wsum_inv = ws;
+    g.sorted = so; g.tile_flag = tf; g.scan_part = sp; g.wnode = wn; g.nbase = nbs;
     g.static_force = ctx->static_force;
     g.cell_force = ctx->cell_force;
     g.n_cells = nc;
@@ -231,6 +234,9 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     }
     ctx->delta_h = (float)std::min(0.5 * ctx->mesh_spacing, 0.1);
     ctx->t_step = 0;
+    // stencil weights of the initial positions (the first interpolation uses them)
+    HIPCHK(launch_ibm_weights(m, g, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipDeviceSynchronize());
     ctx->have_membranes = true;
     return LBFS_OK;

@@ -47,7 +47,9 @@ struct IbmArgs {
     const int *inv;           // caller cell id (i + j nx + k nx ny) -> internal id
     const float4 *cen;        // internal order: centroid xyz, volume
     int *bin_key, *bin_rank, *bin_count, *bin_start, *sorted;
-    float *wsum_inv;          // per node: 1 / sum of cosine weights at the advected position
+    int *scan_part;           // per 1024 bins: count total
+    float *wnode;             // 64 per node: normalised stencil weights of its current position
+    int4 *nbase;              // per node: stencil base cell (logical) and bin
     int *tile_flag;           // per bin tile: holds IBM force this step
     const float *static_force;
     float *cell_force;
@@ -55,6 +57,8 @@ struct IbmArgs {
 
 hipError_t launch_ibm_interp(const MemArgs &m, const IbmArgs &g, const float4 *st, hipStream_t s);
 hipError_t launch_ibm_scan(const IbmArgs &g, hipStream_t s);
+// weights of the current node positions (after attaching membranes)
+hipError_t launch_ibm_weights(const MemArgs &m, const IbmArgs &g, hipStream_t s);
 hipError_t launch_ibm_place(const MemArgs &m, const IbmArgs &g, hipStream_t s);
 hipError_t launch_mem_globals(const MemArgs &m, hipStream_t s);
 hipError_t launch_mem_edges(const MemArgs &m, hipStream_t s);

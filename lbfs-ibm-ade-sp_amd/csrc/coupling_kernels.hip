@@ -100,86 +100,128 @@ __device__ __forceinline__ int bin_coord(const IbmArgs &g, int a, int base)
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// IBM: interpolation, advection, binning
+// IBM: interpolation, advection, weights, binning.  One wavefront per node,
+// one lane per stencil cell (64 for the 4-point kernel, 8 for the 2-point
+// one); sums are butterfly reductions, so their order is fixed.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void lane_offset(int S, int lane, int off[3])
+{
+    if (S == 4) { off[0] = lane & 3; off[1] = (lane >> 2) & 3; off[2] = lane >> 4; }
+    else { off[0] = lane & 1; off[1] = (lane >> 1) & 1; off[2] = lane >> 2; }
+}
+
+// INTERP: interpolate the cell velocity with the node's cached weights (those
+// of its current position), advect the node and count it into its bin.
+// Always: weights of the (new) position, normalised for the cosine kernel,
+// cached for spreading and the next interpolation.
+template <bool INTERP>
 __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const float4 *__restrict__ st)
 {
-    const int nd = blockIdx.x * blockDim.x + threadIdx.x;
-    if (nd >= m.n_nodes) return;
+    const int lane = threadIdx.x & 63;
+    const int nd = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (nd >= m.n_nodes) return;  // uniform over the wavefront
+    const int S = stencil_size(g);
+    const bool act = lane < S * S * S;
+    int off[3];
+    lane_offset(S, lane, off);
     const float4 o = m.origin[m.node_mem[nd]];
     float4 xl = m.X[nd];
-    float P[3] = {o.x + xl.x, o.y + xl.y, o.z + xl.z};
+    if (INTERP) {
+        const int4 nb = g.nbase[nd];
+        float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+        if (act) {
+            const int idx[3] = {nb.x + off[0], nb.y + off[1], nb.z + off[2]};
+            int w[3];
+            if (wrap_index(g, 0, idx[0], w[0]) && wrap_index(g, 1, idx[1], w[1]) && wrap_index(g, 2, idx[2], w[2])) {
+                const float wt = g.wnode[64 * (size_t)nd + lane];
+                const float4 s = st[cell_internal(g, w)];
+                ux = wt * s.y; uy = wt * s.z; uz = wt * s.w;
+            }
+        }
+        ux = wave_sum(ux); uy = wave_sum(uy); uz = wave_sum(uz);
+        xl.x += ux * g.dt; xl.y += uy * g.dt; xl.z += uz * g.dt;
+        if (lane == 0) {
+            m.U[nd] = make_float4(ux, uy, uz, 0.0f);
+            m.X[nd] = xl;
+        }
+    }
+    const float P[3] = {o.x + xl.x, o.y + xl.y, o.z + xl.z};
     float xr[3];
     int base[3];
     node_base(g, P, xr, base);
-    const int S = stencil_size(g);
-    float ux = 0.0f, uy = 0.0f, uz = 0.0f, ws = 0.0f;
-    for (int i = 0; i < S; i++)
-        for (int j = 0; j < S; j++)
-            for (int k = 0; k < S; k++) {
-                const int idx[3] = {base[0] + i, base[1] + j, base[2] + k};
-                int w[3];
-                if (!wrap_index(g, 0, idx[0], w[0]) || !wrap_index(g, 1, idx[1], w[1]) || !wrap_index(g, 2, idx[2], w[2]))
-                    continue;
-                const int ic = cell_internal(g, w);
-                const float wt = cell_weight(g, idx, w, ic, P, xr);
-                const float4 s = st[ic];
-                ux += wt * s.y; uy += wt * s.z; uz += wt * s.w;
-                ws += wt;
-            }
-    if (g.kernel == LBFS_IBM_COS4 && ws > 0.0f) {
-        const float iw = 1.0f / ws;
-        ux *= iw; uy *= iw; uz *= iw;
+    float wt = 0.0f;
+    if (act) {
+        const int idx[3] = {base[0] + off[0], base[1] + off[1], base[2] + off[2]};
+        int w[3];
+        if (wrap_index(g, 0, idx[0], w[0]) && wrap_index(g, 1, idx[1], w[1]) && wrap_index(g, 2, idx[2], w[2]))
+            wt = cell_weight(g, idx, w, cell_internal(g, w), P, xr);
     }
-    m.U[nd] = make_float4(ux, uy, uz, 0.0f);
-    xl.x += ux * g.dt; xl.y += uy * g.dt; xl.z += uz * g.dt;
-    m.X[nd] = xl;
-    // advected node: normalisation of its spreading weights and its bin
-    P[0] = o.x + xl.x; P[1] = o.y + xl.y; P[2] = o.z + xl.z;
-    node_base(g, P, xr, base);
-    if (g.kernel == LBFS_IBM_COS4) {
-        float s2 = 0.0f;
-        for (int i = 0; i < S; i++)
-            for (int j = 0; j < S; j++)
-                for (int k = 0; k < S; k++) {
-                    const int idx[3] = {base[0] + i, base[1] + j, base[2] + k};
-                    int w[3];
-                    if (!wrap_index(g, 0, idx[0], w[0]) || !wrap_index(g, 1, idx[1], w[1]) || !wrap_index(g, 2, idx[2], w[2]))
-                        continue;
-                    s2 += cell_weight(g, idx, w, cell_internal(g, w), P, xr);
-                }
-        g.wsum_inv[nd] = s2 > 0.0f ? 1.0f / s2 : 0.0f;
-    } else {
-        g.wsum_inv[nd] = 1.0f;
+    const float ws = wave_sum(wt);
+    const float inv = g.kernel == LBFS_IBM_COS4 ? (ws > 0.0f ? 1.0f / ws : 0.0f) : 1.0f;
+    g.wnode[64 * (size_t)nd + lane] = wt * inv;
+    if (lane == 0) {
+        const int key = bin_coord(g, 0, base[0]) + g.tn[0] * (bin_coord(g, 1, base[1]) + g.tn[1] * bin_coord(g, 2, base[2]));
+        g.nbase[nd] = make_int4(base[0], base[1], base[2], key);
+        if (INTERP) {
+            g.bin_key[nd] = key;
+            g.bin_rank[nd] = atomicAdd(&g.bin_count[key], 1);
+        }
     }
-    const int key = bin_coord(g, 0, base[0]) + g.tn[0] * (bin_coord(g, 1, base[1]) + g.tn[1] * bin_coord(g, 2, base[2]));
-    g.bin_key[nd] = key;
-    g.bin_rank[nd] = atomicAdd(&g.bin_count[key], 1);
 }
 
-__global__ __launch_bounds__(1024) void k_ibm_scan(IbmArgs g)
+// exclusive scan of the bin counts in chunks of 1024 (counts are cleared for
+// the next step), then the chunk offsets
+__global__ __launch_bounds__(1024) void k_ibm_scan1(IbmArgs g)
 {
-    __shared__ int part[1024];
-    const int nb = g.n_bins, per = (nb + 1023) / 1024;
-    const int b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
-    int s = 0;
-    for (int b = b0; b < b1; b++) s += g.bin_count[b];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    // Hillis-Steele inclusive scan of the 1024 partial sums
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
+    __shared__ int wsum[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int b = blockIdx.x * 1024 + threadIdx.x;
+    const int v = b < g.n_bins ? g.bin_count[b] : 0;
+    int x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
     }
-    int run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0;
-    for (int b = b0; b < b1; b++) {
-        g.bin_start[b] = run;
-        run += g.bin_count[b];
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    if (wave == 0) {
+        int t = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            const int y = __shfl_up(t, off, 64);
+            if (lane >= off) t += y;
+        }
+        if (lane < 16) wsum[lane] = t;
+    }
+    __syncthreads();
+    const int excl = x - v + (wave > 0 ? wsum[wave - 1] : 0);
+    if (b < g.n_bins) {
+        g.bin_start[b] = excl;
         g.bin_count[b] = 0;
     }
-    if (threadIdx.x == 1023) g.bin_start[nb] = part[1023];
+    if (threadIdx.x == 0) g.scan_part[blockIdx.x] = wsum[15];
+}
+
+__global__ __launch_bounds__(1024) void k_ibm_scan2(IbmArgs g)
+{
+    __shared__ int offset;
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int k = 0; k < (int)blockIdx.x; k++) s += g.scan_part[k];
+        offset = s;
+        if (blockIdx.x == gridDim.x - 1) g.bin_start[g.n_bins] = s + g.scan_part[blockIdx.x];
+    }
+    __syncthreads();
+    const int b = blockIdx.x * 1024 + threadIdx.x;
+    if (b < g.n_bins) g.bin_start[b] += offset;
 }
 
 __global__ __launch_bounds__(256) void k_ibm_place(MemArgs m, IbmArgs g)
@@ -190,14 +232,18 @@ __global__ __launch_bounds__(256) void k_ibm_place(MemArgs m, IbmArgs g)
 }
 
 // ---------------------------------------------------------------------------
-// IBM: spreading, one 64-thread workgroup per 4x4x4 tile of cells
+// IBM: spreading, one 64-thread workgroup per 4x4x4 tile of cells.  The nodes
+// binned in the 8 bins whose stencils can reach the tile are staged in LDS 64
+// at a time; each cell picks its cached weight of every node that reaches it.
+// Sums are 64-bit fixed point, so they do not depend on the visiting order.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_ibm_spread(MemArgs m, IbmArgs g)
 {
     const int t = blockIdx.x;
     const int tc[3] = {t % g.tn[0], (t / g.tn[0]) % g.tn[1], t / (g.tn[0] * g.tn[1])};
-    __shared__ int src_lo[8], src_hi[8];
-    __shared__ int total;
+    __shared__ int src_lo[8], src_cnt[8], src_pre[9];
+    __shared__ int4 cb[64];
+    __shared__ float4 cf[64];
     if (threadIdx.x < 8) {
         const int d[3] = {(int)(threadIdx.x & 1) - 1, (int)((threadIdx.x >> 1) & 1) - 1, (int)(threadIdx.x >> 2) - 1};
         int bc[3];
@@ -217,15 +263,16 @@ __global__ __launch_bounds__(64) void k_ibm_spread(MemArgs m, IbmArgs g)
             hi = g.bin_start[b + 1];
         }
         src_lo[threadIdx.x] = lo;
-        src_hi[threadIdx.x] = hi;
+        src_cnt[threadIdx.x] = hi - lo;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         int s = 0;
-        for (int q = 0; q < 8; q++) s += src_hi[q] - src_lo[q];
-        total = s;
+        for (int q = 0; q < 8; q++) { src_pre[q] = s; s += src_cnt[q]; }
+        src_pre[8] = s;
     }
     __syncthreads();
+    const int total = src_pre[8];
     const int l[3] = {(int)(threadIdx.x & 3), (int)((threadIdx.x >> 2) & 3), (int)(threadIdx.x >> 4)};
     const int w[3] = {4 * tc[0] + l[0], 4 * tc[1] + l[1], 4 * tc[2] + l[2]};
     const bool valid = w[0] < g.n[0] && w[1] < g.n[1] && w[2] < g.n[2];
@@ -240,33 +287,44 @@ __global__ __launch_bounds__(64) void k_ibm_spread(MemArgs m, IbmArgs g)
         }
         return;
     }
-    long long acc[3] = {0, 0, 0};
     const int S = stencil_size(g);
-    if (valid) {
-        for (int q = 0; q < 8; q++)
-            for (int p = src_lo[q]; p < src_hi[q]; p++) {
-                const int nd = g.sorted[p];
-                const float4 o = m.origin[m.node_mem[nd]];
-                const float4 xl = m.X[nd];
-                const float P[3] = {o.x + xl.x, o.y + xl.y, o.z + xl.z};
-                float xr[3];
-                int base[3];
-                node_base(g, P, xr, base);
-                int idx[3];
+    long long acc[3] = {0, 0, 0};
+    for (int c0 = 0; c0 < total; c0 += 64) {
+        const int q = c0 + (int)threadIdx.x;
+        if (q < total) {
+            int k = 0;
+            while (q >= src_pre[k + 1]) k++;
+            const int p = g.sorted[src_lo[k] + q - src_pre[k]];
+            const int4 nb = g.nbase[p];
+            cb[threadIdx.x] = make_int4(nb.x, nb.y, nb.z, p);
+            cf[threadIdx.x] = m.F[p];
+        }
+        __syncthreads();
+        const int cnt = min(64, total - c0);
+        if (valid)
+            for (int j = 0; j < cnt; j++) {
+                const int4 nb = cb[j];
+                const int bs[3] = {nb.x, nb.y, nb.z};
+                int slot = 0, mul = 1;
                 bool in = true;
+#pragma unroll
                 for (int a = 0; a < 3; a++) {
-                    int d = w[a] - base[a];
+                    int d = w[a] - bs[a];
                     if (g.per[a]) d = ((d % g.n[a]) + g.n[a]) % g.n[a];
-                    if (d < 0 || d >= S) { in = false; break; }
-                    idx[a] = base[a] + d;
+                    in = in && d >= 0 && d < S;
+                    slot += d * mul;
+                    mul *= S;
                 }
                 if (!in) continue;
-                const float wt = cell_weight(g, idx, w, ic, P, xr) * g.wsum_inv[nd];
-                const float4 F = m.F[nd];
+                const float wt = g.wnode[64 * (size_t)nb.w + slot];
+                const float4 F = cf[j];
                 acc[0] += llrint((double)(F.x * wt) * FIX_SCALE);
                 acc[1] += llrint((double)(F.y * wt) * FIX_SCALE);
                 acc[2] += llrint((double)(F.z * wt) * FIX_SCALE);
             }
+        __syncthreads();
+    }
+    if (valid) {
         const float sc = g.kernel == LBFS_IBM_LINEAR2 ? g.inv_h * g.inv_h * g.inv_h : 1.0f / g.cen[ic].w;
         for (int a = 0; a < 3; a++)
             g.cell_force[3 * (size_t)ic + a] = g.static_force[3 * (size_t)ic + a] + (float)((double)acc[a] / FIX_SCALE) * sc;
@@ -277,10 +335,10 @@ __global__ __launch_bounds__(64) void k_ibm_spread(MemArgs m, IbmArgs g)
 // ---------------------------------------------------------------------------
 // membrane
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_mem_globals(MemArgs m)
+__global__ __launch_bounds__(1024) void k_mem_globals(MemArgs m)
 {
     const int mem = blockIdx.x;
-    __shared__ double sh[3][256];
+    __shared__ double sh[3][1024];
     double V = 0.0, A = 0.0, M = 0.0;
     for (int t = m.mem_tri_start[mem] + threadIdx.x; t < m.mem_tri_start[mem + 1]; t += blockDim.x) {
         const int4 q = m.tri[t];
@@ -299,7 +357,7 @@ __global__ __launch_bounds__(256) void k_mem_globals(MemArgs m)
     sh[1][threadIdx.x] = A;
     sh[2][threadIdx.x] = M;
     __syncthreads();
-    for (int stp = 128; stp > 0; stp >>= 1) {
+    for (int stp = 512; stp > 0; stp >>= 1) {
         if ((int)threadIdx.x < stp)
             for (int q = 0; q < 3; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + stp];
         __syncthreads();
@@ -459,12 +517,19 @@ __global__ __launch_bounds__(256) void k_mem_nodes(MemArgs m)
 // ---------------------------------------------------------------------------
 hipError_t launch_ibm_interp(const MemArgs &m, const IbmArgs &g, const float4 *st, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ibm_interp, dim3((m.n_nodes + 255) / 256), dim3(256), 0, s, m, g, st);
+    hipLaunchKernelGGL(k_ibm_interp<true>, dim3((m.n_nodes + 3) / 4), dim3(256), 0, s, m, g, st);
+    return hipGetLastError();
+}
+hipError_t launch_ibm_weights(const MemArgs &m, const IbmArgs &g, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ibm_interp<false>, dim3((m.n_nodes + 3) / 4), dim3(256), 0, s, m, g, nullptr);
     return hipGetLastError();
 }
 hipError_t launch_ibm_scan(const IbmArgs &g, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ibm_scan, dim3(1), dim3(1024), 0, s, g);
+    const int blocks = (g.n_bins + 1023) / 1024;
+    hipLaunchKernelGGL(k_ibm_scan1, dim3(blocks), dim3(1024), 0, s, g);
+    hipLaunchKernelGGL(k_ibm_scan2, dim3(blocks), dim3(1024), 0, s, g);
     return hipGetLastError();
 }
 hipError_t launch_ibm_place(const MemArgs &m, const IbmArgs &g, hipStream_t s)
@@ -474,7 +539,7 @@ hipError_t launch_ibm_place(const MemArgs &m, const IbmArgs &g, hipStream_t s)
 }
 hipError_t launch_mem_globals(const MemArgs &m, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mem_globals, dim3(m.n_mem), dim3(256), 0, s, m);
+    hipLaunchKernelGGL(k_mem_globals, dim3(m.n_mem), dim3(1024), 0, s, m);
     return hipGetLastError();
 }
 hipError_t launch_mem_edges(const MemArgs &m, hipStream_t s)
