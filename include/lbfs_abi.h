@@ -140,6 +140,23 @@ int lbfs_stage_gradients(lbfs_ctx *ctx, double *grad /* [12 n_cells] or NULL */)
 int lbfs_stage_face_flux(lbfs_ctx *ctx, double *flux /* [4 n_faces] */);           /* LBFS.cu:2574-2810 */
 int lbfs_stage_residual(lbfs_ctx *ctx, double *res /* [4 n_cells] */);             /* LBFS.cu:2242-2281 */
 
+/* named stages of one step, in order (no counterpart in the reference as an
+ * API; the groups are the reference's kernel groups of Solver_gpu.cu:749-1246):
+ *   ibm_interp    interpolate_velocities_on_nodes + update_node_positions
+ *                 (immersed_boundary_method.cu:14-97, 234-264)
+ *   ibm_scan, ibm_place, ibm_spread   spread_forces_on_structured_grid (:947-1024)
+ *   mem_globals, mem_edges, mem_nodes run_spring_network_functions_atomics
+ *                 (Solver_gpu.cu:2311-2361)
+ *   internal_visc update_interior_viscosities_plucker (:389-527), when due
+ *   gradients     get_interior_gradients + get_bc_gradients (LBFS.cu:1933-2240)
+ *   dt_min        get_cfl_device + min (LBFS.cu:2871-2936), time_stepping 3 only
+ *   flux_update   calc_face_flux(_bc) + add_face_flux_to_cell + add + time_integration
+ *                 (LBFS.cu:2574-2810, 2242-2281, 2089-2172)
+ * Running them all in order, then lbfs_stage_end_step, equals lbfs_step(ctx, 1). */
+int lbfs_stage_names(lbfs_ctx *ctx, int max_count, const char **names, int *count);
+int lbfs_stage_run(lbfs_ctx *ctx, const char *name);
+int lbfs_stage_end_step(lbfs_ctx *ctx);
+
 /* cell force density added to the momentum residual (Solver_gpu.cu:834-846, 1008-1010) */
 int lbfs_get_cell_force(lbfs_ctx *ctx, double *f /* [3 n_cells] */);
 
@@ -212,6 +229,43 @@ int lbfs_membrane_mesh_read(const char *path, int format, const double centre[3]
                             int *springs /* [4 n_springs] or NULL */);
 int lbfs_membrane_mesh_write(const char *path, int format, int n_nodes, const double *nodes, int n_tri,
                              const int *tri, int n_springs, const int *springs);
+
+/* ---- case files in the reference's XML schema (preprocessor.cpp:87-857) ----
+ * The parts the time step reads.  Derived values follow global_variables.cpp:20-68
+ * (visc = mach_no * ref_length / reynolds_no; preconditioning "true" with
+ * gamma 1 gives gamma = 2 sqrt(3) mach_no) and preprocessor.cpp:419-530 (BC
+ * velocities scaled by mach_no / scale). */
+#define LBFS_CASE_MAX_PATCHES 32
+#define LBFS_CASE_MAX_OBJECTS 64
+typedef struct {
+    char name[64];
+    int rho_type, vel_type;    /* preprocessor.cpp numbering; "empty" -> LBFS_BC_EMPTY; unknown -> 0 */
+    double values[4];          /* rho, u, v, w */
+} lbfs_case_patch;
+typedef struct {
+    char name[64];
+    int type;                  /* 1 rigid_cylinder, 2 spring_network, 0 other */
+    int num_nodes;
+    double radius, centre[3];
+    char mesh_file[256], stress_free_mesh[256], curvature_mesh[256];
+    double shear_modulus, wlc_ratio, volume_modulus, local_area_modulus, global_area_modulus;
+    double global_bending_modulus, local_bending_modulus, viscosity, curvature_ratio, volume_ratio;
+    double internal_viscosity_ratio;
+    int sphere_reference, periodic, mz_importer;
+} lbfs_case_object;
+typedef struct {
+    char import_file[512], import_format[32];
+    int mesh_type, gpu_solver, gpu_device, output_step, time_stepping, gradient_type;
+    double max_velocity, reynolds_number, visc, gamma, fneq_min, time_marching_step, simulation_length, scale;
+    double domain[3], spacing[3], origin[3], streaming_dt, ref_length;
+    double average_rho, initial_velocity[3], body_force[3];
+    int n_patches;
+    lbfs_case_patch patches[LBFS_CASE_MAX_PATCHES];
+    int n_objects;
+    lbfs_case_object objects[LBFS_CASE_MAX_OBJECTS];
+    char error[256];
+} lbfs_case;
+int lbfs_case_read(const char *path, lbfs_case *out);
 
 /* ---- measurement (no counterpart in the reference; used by bench.py) ----
  * HIP events recorded on the context's stream.  lbfs_time_steps: wall time of

@@ -64,6 +64,36 @@ class MembraneDesc(C.Structure):
     ]
 
 
+class CasePatch(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("rho_type", C.c_int), ("vel_type", C.c_int), ("values", C.c_double * 4)]
+
+
+class CaseObject(C.Structure):
+    _fields_ = [
+        ("name", C.c_char * 64), ("type", C.c_int), ("num_nodes", C.c_int), ("radius", C.c_double),
+        ("centre", C.c_double * 3), ("mesh_file", C.c_char * 256), ("stress_free_mesh", C.c_char * 256),
+        ("curvature_mesh", C.c_char * 256), ("shear_modulus", C.c_double), ("wlc_ratio", C.c_double),
+        ("volume_modulus", C.c_double), ("local_area_modulus", C.c_double), ("global_area_modulus", C.c_double),
+        ("global_bending_modulus", C.c_double), ("local_bending_modulus", C.c_double), ("viscosity", C.c_double),
+        ("curvature_ratio", C.c_double), ("volume_ratio", C.c_double), ("internal_viscosity_ratio", C.c_double),
+        ("sphere_reference", C.c_int), ("periodic", C.c_int), ("mz_importer", C.c_int),
+    ]
+
+
+class Case(C.Structure):
+    _fields_ = [
+        ("import_file", C.c_char * 512), ("import_format", C.c_char * 32), ("mesh_type", C.c_int),
+        ("gpu_solver", C.c_int), ("gpu_device", C.c_int), ("output_step", C.c_int), ("time_stepping", C.c_int),
+        ("gradient_type", C.c_int), ("max_velocity", C.c_double), ("reynolds_number", C.c_double),
+        ("visc", C.c_double), ("gamma", C.c_double), ("fneq_min", C.c_double), ("time_marching_step", C.c_double),
+        ("simulation_length", C.c_double), ("scale", C.c_double), ("domain", C.c_double * 3),
+        ("spacing", C.c_double * 3), ("origin", C.c_double * 3), ("streaming_dt", C.c_double),
+        ("ref_length", C.c_double), ("average_rho", C.c_double), ("initial_velocity", C.c_double * 3),
+        ("body_force", C.c_double * 3), ("n_patches", C.c_int), ("patches", CasePatch * 32),
+        ("n_objects", C.c_int), ("objects", CaseObject * 64), ("error", C.c_char * 256),
+    ]
+
+
 MEMBRANE_PARAM_NAMES = ("shear_modulus", "wlc_ratio", "volume_modulus", "local_area_modulus", "global_area_modulus",
                         "global_bending_modulus", "local_bending_modulus", "viscosity", "curvature_ratio",
                         "volume_ratio", "sphere_reference", "mad_compat", "internal_viscosity_ratio")
@@ -115,6 +145,10 @@ def load():
             "lbfs_mesh_geometry": (C.c_int, [C.POINTER(MeshDesc), C.c_double, C.c_double, C.c_double, C.c_int,
                                              _dp, _dp, _dp, _dp, _dp, _dp]),
             "lbfs_membrane_mesh_counts": (C.c_int, [C.c_char_p, C.c_int, _ip, _ip, _ip]),
+            "lbfs_case_read": (C.c_int, [C.c_char_p, C.POINTER(Case)]),
+            "lbfs_stage_names": (C.c_int, [vp, C.c_int, C.POINTER(C.c_char_p), _ip]),
+            "lbfs_stage_run": (C.c_int, [vp, C.c_char_p]),
+            "lbfs_stage_end_step": (C.c_int, [vp]),
             "lbfs_membrane_mesh_read": (C.c_int, [C.c_char_p, C.c_int, _dp, _dp, _ip, _ip]),
             "lbfs_membrane_mesh_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip]),
             "lbfs_polymesh_counts": (C.c_int, [C.c_char_p, _ip, _ip, _ip, _ip, _ip]),
@@ -332,6 +366,32 @@ def rbc(frequency=16, d0=20.0, center=(0.0, 0.0, 0.0), rot=None):
     return x + np.asarray(center, dtype=np.float64), tri
 
 
+def read_case(path):
+    """Case file in the reference's XML schema -> Case (ctypes structure)."""
+    c = Case()
+    rc = load().lbfs_case_read(os.fsencode(str(path)), C.byref(c))
+    if rc != 0:
+        raise LbfsError("lbfs_case_read failed (%d): %s" % (rc, c.error.decode()))
+    return c
+
+
+def case_params(case, lattice=D3Q15, flags=0):
+    """Solver parameters of a case (the reference's GPU lattice by default)."""
+    return make_params(lattice=lattice, visc=case.visc, dt=case.time_marching_step, rho0=case.average_rho,
+                       gamma=case.gamma, fneq_min=case.fneq_min, flags=flags, gradient_type=case.gradient_type,
+                       body_force=tuple(case.body_force), channel_diameter=case.domain[1],
+                       time_stepping=case.time_stepping, max_velocity=case.max_velocity)
+
+
+def apply_case_bcs(mesh, case):
+    """Set the mesh's per-face BCs from the case's boundary conditions, matched by patch name."""
+    byname = {case.patches[k].name.decode(): case.patches[k] for k in range(case.n_patches)}
+    for k, name in enumerate(mesh.patch_names):
+        p = byname.get(name)
+        if p is not None:
+            mesh.set_patch_bc(k, p.rho_type, p.vel_type, tuple(p.values))
+
+
 def read_membrane_mesh(path, fmt=MESH_OSAY, centre=(0.0, 0.0, 0.0)):
     """Osay or MZ membrane mesh file -> (nodes [N,3], tri [T,3], springs [S,4] or None)."""
     lib = load()
@@ -468,6 +528,18 @@ class Solver:
         g = np.zeros(6 * len(self._membranes))
         check(load().lbfs_get_membrane_globals(self.ctx, ptr(g)), self.ctx)
         return g.reshape(-1, 6)
+
+    def stage_names(self):
+        names = (C.c_char_p * 32)()
+        n = C.c_int()
+        check(load().lbfs_stage_names(self.ctx, 32, names, C.byref(n)), self.ctx)
+        return [names[k].decode() for k in range(n.value)]
+
+    def stage_run(self, name):
+        check(load().lbfs_stage_run(self.ctx, name.encode()), self.ctx)
+
+    def stage_end_step(self):
+        check(load().lbfs_stage_end_step(self.ctx), self.ctx)
 
     def membrane_energies(self):
         """Per membrane: spring, local area, local bending, global area,

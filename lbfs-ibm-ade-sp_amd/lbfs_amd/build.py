@@ -51,8 +51,17 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-8000:]))
     os.replace(tmp, LIB)
+    # the C++ host mirror's check program (plain host C++ against the C ABI)
+    host = os.path.join(ROOT, "host")
+    exe = os.path.join(host, "gpu_solver_check")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + host, "-I" + os.path.join(REPO, "include"),
+           os.path.join(host, "gpu_solver_check.cpp"), "-L" + PKG_DIR, "-llbfs_amd",
+           "-Wl,-rpath,$ORIGIN/../lbfs_amd", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-8000:]))
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", exe)
     return LIB
 
 

@@ -131,3 +131,56 @@ def test_membrane_mesh_rejects_bad_ids(tmp_path):
     path.write_text("4 2\n0 0 0\n1 0 0\n0 1 0\n0 0 1\n1 2 3\n1 3 9\n")
     with pytest.raises(abi.LbfsError):
         abi.read_membrane_mesh(path)
+
+
+def test_case_xml_reader():
+    """The reference's XML schema (preprocessor.cpp:87-857) with its derived
+    values (global_variables.cpp:20-68): nu = mach_no ref_length / Re,
+    preconditioned gamma = 2 sqrt(3) mach_no, BC velocities scaled by
+    mach_no / scale, signed body force along pressure_direction, object
+    fields carried over from the previous object, sphere_reference written
+    into the periodic flag."""
+    c = abi.read_case(os.path.join(REPO, "tests", "golden", "channel_case.xml"))
+    assert abs(c.visc - 0.02 * 100 / 12) < 1e-15
+    assert abs(c.gamma - 2 * np.sqrt(3) * 0.02) < 1e-15
+    assert c.time_stepping == abi.TS_MIN and c.gradient_type == abi.GRAD_LS
+    assert c.import_file == b"polyMesh" and c.output_step == 50 and c.mesh_type == 3
+    assert list(c.body_force) == [2e-5, 0.0, 0.0]
+    assert c.n_patches == 4
+    p = {c.patches[k].name.decode(): c.patches[k] for k in range(4)}
+    assert (p["ymin"].rho_type, p["ymin"].vel_type) == (abi.BC_NEUMANN, abi.BC_DIRICHLET)
+    assert abs(p["ymax"].values[1] - 0.5 * 0.02) < 1e-17
+    assert p["zmin"].vel_type == abi.BC_PARABOLIC and abs(p["zmin"].values[1] - 0.02) < 1e-17
+    assert p["zmax"].rho_type == abi.BC_DIRICHLET and abs(p["zmax"].values[0] - 1.01) < 1e-15
+    assert c.n_objects == 2
+    a, b = c.objects[0], c.objects[1]
+    assert a.name == b"cell_a" and a.type == 2 and a.num_nodes == 642 and a.mesh_file == b"cell.dat"
+    assert a.sphere_reference == 1 and a.periodic == 0            # <sphere_reference>false</...> -> periodic
+    assert abs(a.volume_ratio - 0.95) < 1e-15 and abs(a.curvature_ratio - 1.5) < 1e-15
+    assert b.name == b"cell_b" and abs(b.shear_modulus - 0.006) < 1e-15
+    assert b.num_nodes == 642 and abs(b.volume_modulus - 0.1) < 1e-15 and b.mesh_file == b"cell.dat"
+    assert list(b.centre) == [4.0, 6.0, 5.0]
+    params = abi.case_params(c)
+    assert params.time_stepping == abi.TS_MIN and abs(params.channel_diameter - 12.0) < 1e-15
+    m = abi.HexMesh(16, 12, 10)
+    abi.apply_case_bcs(m, c)
+    sel = m.bc_patch[: m.n_bc] == 4
+    assert np.all(m.bc_vel_type[: m.n_bc][sel] == abi.BC_PARABOLIC)
+
+
+def test_case_xml_errors(tmp_path):
+    bad = tmp_path / "bad.xml"
+    bad.write_text("<case><global_variables><mach_no>0.1</mach_no></global_variables>")
+    with pytest.raises(abi.LbfsError):
+        abi.read_case(bad)
+
+
+def test_cpp_host_mirror_builds_and_fails_cleanly_without_device():
+    """The C++ host mirror (host/gpu_solver.hpp) compiles against the C ABI;
+    without a HIP device its check program reports LBFS_E_NO_DEVICE (exit 3)
+    instead of crashing."""
+    import subprocess
+    exe = os.path.join(REPO, "lbfs-ibm-ade-sp_amd", "host", "gpu_solver_check")
+    assert os.path.exists(exe)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode in (0, 3), r.stdout + r.stderr

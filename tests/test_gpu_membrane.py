@@ -176,3 +176,36 @@ def test_springs_from_mz_file_parity(tmp_path):
     X, _, F = s.membrane_nodes()
     assert cases.rel_l2(X - xr, mo.X.reshape(-1, 3) - xr) < 1e-4
     assert cases.rel_l2(F, mo.F.reshape(-1, 3)) < 1e-3
+
+
+def test_named_stages_equal_whole_steps():
+    """Running the named stages of each step in order (the reference's kernel
+    groups) gives bitwise the same state as lbfs_step; the C++ host mirror's
+    check program does the same through gpu_solver::General_Purpose_Solver_mk_i
+    and the reference-named group wrappers."""
+    import os
+    import subprocess
+
+    def build():
+        mesh = cases.channel_mesh(24, 16, 16, seed=9)
+        x0, tri = abi.rbc(4, 8.0, center=(12.0, 8.0, 8.0))
+        s = abi.Solver(mesh, abi.make_params(visc=VISC, dt=DT, body_force=(2e-5, 0.0, 0.0)))
+        s.set_state(*cases.smooth_state(mesh, seed=9, amp=0.005))
+        s.add_membranes([abi.Membrane(x0, tri, **PARAMS)])
+        return s
+
+    a, b = build(), build()
+    a.step(7)
+    names = b.stage_names()
+    assert names[0] == "ibm_interp" and names[-1] == "flux_update"
+    for _ in range(7):
+        for n in b.stage_names():
+            b.stage_run(n)
+        b.stage_end_step()
+    assert np.array_equal(np.concatenate(a.get_state()), np.concatenate(b.get_state()))
+    assert np.array_equal(a.membrane_nodes()[0], b.membrane_nodes()[0])
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lbfs-ibm-ade-sp_amd", "host",
+                       "gpu_solver_check")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "groups equal whole steps" in r.stdout
