@@ -63,6 +63,20 @@ __device__ __forceinline__ bool wrap_index(const IbmArgs &g, int a, int i, int &
 }
 
 // unnormalised weight of the cell with unwrapped index idx / wrapped index w
+// phi(r) = (1 + cos(pi r / 2)) / 4 for |r| < 2, written as (1 - sin^2(pi r / 4)) / 2
+// with sin from its Taylor series to x^11 (|x| <= pi/2: truncation below 6e-8)
+__device__ __forceinline__ float cos4_phi(float r)
+{
+    const float x = r * (PI_F * 0.25f), x2 = x * x;
+    float p = -2.5052108385e-8f;                 // -1/11!
+    p = fmaf(p, x2, 2.7557319224e-6f);           //  1/9!
+    p = fmaf(p, x2, -1.9841269841e-4f);          // -1/7!
+    p = fmaf(p, x2, 8.3333333333e-3f);           //  1/5!
+    p = fmaf(p, x2, -1.6666666667e-1f);          // -1/3!
+    const float sn = fmaf(p * x2, x, x);
+    return 0.5f * fmaf(-sn, sn, 1.0f);
+}
+
 __device__ __forceinline__ float cell_weight(const IbmArgs &g, const int idx[3], const int w[3], int internal,
                                              const float P[3], const float xr[3])
 {
@@ -77,7 +91,7 @@ __device__ __forceinline__ float cell_weight(const IbmArgs &g, const int idx[3],
         for (int a = 0; a < 3; a++) {
             const float xc = cc[a] + (float)(idx[a] - w[a]) * (g.L[a] / (float)g.n[a]);
             const float r = (xc - P[a]) * g.inv_h;
-            wt *= fabsf(r) < 2.0f ? 0.25f * (1.0f + cosf(PI_F * 0.5f * r)) : 0.0f;
+            wt *= fabsf(r) < 2.0f ? cos4_phi(r) : 0.0f;
         }
     }
     return wt;
@@ -232,27 +246,30 @@ __global__ __launch_bounds__(256) void k_ibm_place(MemArgs m, IbmArgs g)
 }
 
 // ---------------------------------------------------------------------------
-// IBM: spreading, one 64-thread workgroup per 4x4x4 tile of cells.  The nodes
-// binned in the 8 bins whose stencils can reach the tile are staged in LDS 64
-// at a time; each cell picks its cached weight of every node that reaches it.
-// Sums are 64-bit fixed point, so they do not depend on the visiting order.
+// IBM: spreading, one 256-thread workgroup per 4x4x4 tile of cells.  The
+// nodes binned in the 8 bins whose stencils can reach the tile are taken one
+// per wavefront at a time; each lane holds one stencil slot of that node
+// (its cached weight, one coalesced read per node) and adds the weighted force
+// into the tile's LDS accumulator when the slot's cell lies in the tile.  The
+// accumulators are 64-bit fixed point, so the sums do not depend on the order
+// the nodes are visited in.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_ibm_spread(MemArgs m, IbmArgs g)
+__global__ __launch_bounds__(256) void k_ibm_spread(MemArgs m, IbmArgs g)
 {
     const int t = blockIdx.x;
     const int tc[3] = {t % g.tn[0], (t / g.tn[0]) % g.tn[1], t / (g.tn[0] * g.tn[1])};
-    __shared__ int src_lo[8], src_cnt[8], src_pre[9];
-    __shared__ int4 cb[64];
-    __shared__ float4 cf[64];
-    if (threadIdx.x < 8) {
-        const int d[3] = {(int)(threadIdx.x & 1) - 1, (int)((threadIdx.x >> 1) & 1) - 1, (int)(threadIdx.x >> 2) - 1};
+    __shared__ int src_lo[8], src_pre[9];
+    __shared__ unsigned long long acc[3][64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid < 8) {
+        const int d[3] = {(tid & 1) - 1, ((tid >> 1) & 1) - 1, (tid >> 2) - 1};
         int bc[3];
         bool ok = true;
         for (int a = 0; a < 3; a++) {
             int v = tc[a] + d[a];
             if (v < 0 || v >= g.tn[a]) {
                 if (!g.per[a] || g.tn[a] == 1) { ok = false; break; }
-                v = (v + g.tn[a]) % g.tn[a];
+                v = v < 0 ? v + g.tn[a] : v - g.tn[a];
             }
             bc[a] = v;
         }
@@ -262,74 +279,74 @@ __global__ __launch_bounds__(64) void k_ibm_spread(MemArgs m, IbmArgs g)
             lo = g.bin_start[b];
             hi = g.bin_start[b + 1];
         }
-        src_lo[threadIdx.x] = lo;
-        src_cnt[threadIdx.x] = hi - lo;
+        src_lo[tid] = lo;
+        src_pre[tid + 1] = hi - lo;
     }
+    if (tid < 192) acc[tid >> 6][tid & 63] = 0ull;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int s = 0;
-        for (int q = 0; q < 8; q++) { src_pre[q] = s; s += src_cnt[q]; }
-        src_pre[8] = s;
+    if (tid == 0) {
+        src_pre[0] = 0;
+        for (int q = 0; q < 8; q++) src_pre[q + 1] += src_pre[q];
     }
     __syncthreads();
     const int total = src_pre[8];
-    const int l[3] = {(int)(threadIdx.x & 3), (int)((threadIdx.x >> 2) & 3), (int)(threadIdx.x >> 4)};
+    // cell of this thread (first 64 threads): restore the static force in
+    // tiles that held IBM force last step and hold none now
+    const int l[3] = {lane & 3, (lane >> 2) & 3, lane >> 4};
     const int w[3] = {4 * tc[0] + l[0], 4 * tc[1] + l[1], 4 * tc[2] + l[2]};
     const bool valid = w[0] < g.n[0] && w[1] < g.n[1] && w[2] < g.n[2];
-    const int ic = valid ? cell_internal(g, w) : 0;
     const int flag = g.tile_flag[t];
     if (total == 0) {
         if (flag) {
-            if (valid)
+            if (wave == 0 && valid) {
+                const int ic = cell_internal(g, w);
                 for (int a = 0; a < 3; a++) g.cell_force[3 * (size_t)ic + a] = g.static_force[3 * (size_t)ic + a];
+            }
             __syncthreads();
-            if (threadIdx.x == 0) g.tile_flag[t] = 0;
+            if (tid == 0) g.tile_flag[t] = 0;
         }
         return;
     }
     const int S = stencil_size(g);
-    long long acc[3] = {0, 0, 0};
-    for (int c0 = 0; c0 < total; c0 += 64) {
-        const int q = c0 + (int)threadIdx.x;
-        if (q < total) {
-            int k = 0;
-            while (q >= src_pre[k + 1]) k++;
-            const int p = g.sorted[src_lo[k] + q - src_pre[k]];
-            const int4 nb = g.nbase[p];
-            cb[threadIdx.x] = make_int4(nb.x, nb.y, nb.z, p);
-            cf[threadIdx.x] = m.F[p];
-        }
-        __syncthreads();
-        const int cnt = min(64, total - c0);
-        if (valid)
-            for (int j = 0; j < cnt; j++) {
-                const int4 nb = cb[j];
-                const int bs[3] = {nb.x, nb.y, nb.z};
-                int slot = 0, mul = 1;
-                bool in = true;
+    int off[3];
+    lane_offset(S, lane, off);
+    const bool act = lane < S * S * S;
+    const int t0[3] = {4 * tc[0], 4 * tc[1], 4 * tc[2]};
+    for (int q = wave; q < total; q += 4) {
+        int k = 0;
+        while (q >= src_pre[k + 1]) k++;
+        const int p = g.sorted[src_lo[k] + q - src_pre[k]];
+        const int4 nb = g.nbase[p];
+        const float4 F = m.F[p];
+        const float wt = g.wnode[64 * (size_t)p + lane];
+        if (!act || wt == 0.0f) continue;
+        const int c[3] = {nb.x + off[0], nb.y + off[1], nb.z + off[2]};
+        int loc = 0, mul = 1;
+        bool in = true;
 #pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    int d = w[a] - bs[a];
-                    if (g.per[a]) d = ((d % g.n[a]) + g.n[a]) % g.n[a];
-                    in = in && d >= 0 && d < S;
-                    slot += d * mul;
-                    mul *= S;
-                }
-                if (!in) continue;
-                const float wt = g.wnode[64 * (size_t)nb.w + slot];
-                const float4 F = cf[j];
-                acc[0] += llrint((double)(F.x * wt) * FIX_SCALE);
-                acc[1] += llrint((double)(F.y * wt) * FIX_SCALE);
-                acc[2] += llrint((double)(F.z * wt) * FIX_SCALE);
-            }
-        __syncthreads();
+        for (int a = 0; a < 3; a++) {
+            int v = c[a];
+            if (g.per[a]) v = v < 0 ? v + g.n[a] : (v >= g.n[a] ? v - g.n[a] : v);
+            const int d = v - t0[a];
+            in = in && d >= 0 && d < 4;
+            loc += d * mul;
+            mul *= 4;
+        }
+        if (!in) continue;
+        atomicAdd(&acc[0][loc], (unsigned long long)llrint((double)(F.x * wt) * FIX_SCALE));
+        atomicAdd(&acc[1][loc], (unsigned long long)llrint((double)(F.y * wt) * FIX_SCALE));
+        atomicAdd(&acc[2][loc], (unsigned long long)llrint((double)(F.z * wt) * FIX_SCALE));
     }
-    if (valid) {
+    __syncthreads();
+    if (wave == 0 && valid) {
+        const int ic = cell_internal(g, w);
         const float sc = g.kernel == LBFS_IBM_LINEAR2 ? g.inv_h * g.inv_h * g.inv_h : 1.0f / g.cen[ic].w;
-        for (int a = 0; a < 3; a++)
-            g.cell_force[3 * (size_t)ic + a] = g.static_force[3 * (size_t)ic + a] + (float)((double)acc[a] / FIX_SCALE) * sc;
+        for (int a = 0; a < 3; a++) {
+            const double v = (double)(long long)acc[a][lane] / FIX_SCALE;
+            g.cell_force[3 * (size_t)ic + a] = g.static_force[3 * (size_t)ic + a] + (float)v * sc;
+        }
     }
-    if (threadIdx.x == 0 && !flag) g.tile_flag[t] = 1;
+    if (tid == 0 && !flag) g.tile_flag[t] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -500,16 +517,26 @@ __global__ __launch_bounds__(128) void k_mem_edges(MemArgs m)
     for (int j = 0; j < 4; j++) m.edge_force[4 * (size_t)e + j] = make_float4(Fo[j].x, Fo[j].y, Fo[j].z, 0.0f);
 }
 
+// eight lanes per node, each summing every eighth incidence; the eight
+// partial sums are combined by a butterfly, so the order is fixed
 __global__ __launch_bounds__(256) void k_mem_nodes(MemArgs m)
 {
-    const int nd = blockIdx.x * blockDim.x + threadIdx.x;
-    if (nd >= m.n_nodes) return;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nd = gid >> 3, sub = gid & 7;
+    const bool live = nd < m.n_nodes;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    for (int j = m.inc_start[nd]; j < m.inc_start[nd + 1]; j++) {
-        const float4 f = m.edge_force[m.inc[j]];
-        fx += f.x; fy += f.y; fz += f.z;
+    if (live)
+        for (int j = m.inc_start[nd] + sub; j < m.inc_start[nd + 1]; j += 8) {
+            const float4 f = m.edge_force[m.inc[j]];
+            fx += f.x; fy += f.y; fz += f.z;
+        }
+#pragma unroll
+    for (int off = 4; off > 0; off >>= 1) {
+        fx += __shfl_xor(fx, off, 64);
+        fy += __shfl_xor(fy, off, 64);
+        fz += __shfl_xor(fz, off, 64);
     }
-    m.F[nd] = make_float4(fx, fy, fz, 0.0f);
+    if (live && sub == 0) m.F[nd] = make_float4(fx, fy, fz, 0.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -549,12 +576,12 @@ hipError_t launch_mem_edges(const MemArgs &m, hipStream_t s)
 }
 hipError_t launch_mem_nodes(const MemArgs &m, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mem_nodes, dim3((m.n_nodes + 255) / 256), dim3(256), 0, s, m);
+    hipLaunchKernelGGL(k_mem_nodes, dim3((8 * m.n_nodes + 255) / 256), dim3(256), 0, s, m);
     return hipGetLastError();
 }
 hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ibm_spread, dim3(g.n_bins), dim3(64), 0, s, m, g);
+    hipLaunchKernelGGL(k_ibm_spread, dim3(g.n_bins), dim3(256), 0, s, m, g);
     return hipGetLastError();
 }
 
