@@ -103,6 +103,9 @@ typedef struct {
     double channel_diameter; /* H of LBFS_BC_PARABOLIC (domain.Y, Solver_gpu.cu:935); 0 if unused */
     int time_stepping;  /* LBFS_TS_*; 0 = constant dt */
     double max_velocity;/* U of the CFL estimate (LBFS_TS_LOCAL / LBFS_TS_MIN) */
+    int tile[3];        /* cells per tile along x, y, z of the logical grid (no counterpart in the
+                           reference); 0 0 0 = 8 8 4.  Results do not depend on it. */
+    int block_threads;  /* workgroup size of the flux/update kernel: 0 (= 256), 256 or 512 */
 } lbfs_params;
 
 /* time stepping (gpu_time_stepping, Solver_gpu.cu:767-790; get_cfl_device

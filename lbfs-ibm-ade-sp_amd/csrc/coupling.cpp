@@ -215,6 +215,12 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     if ((rc = cp_alloc<int>(ctx, &so, nullptr, nn))) return rc;
     if ((rc = cp_alloc<int>(ctx, &tf, nullptr, g.n_bins))) return rc;
     if ((rc = cp_alloc<int>(ctx, &sp, nullptr, (g.n_bins + 1023) / 1024))) return rc;
+    int *tt;
+    unsigned long long *dacc;
+    if ((rc = cp_alloc<int>(ctx, &tt, nullptr, g.n_bins))) return rc;
+    if ((rc = cp_alloc<unsigned long long>(ctx, &dacc, nullptr, 3 * (size_t)nc))) return rc;
+    g.tile_touched = tt;
+    g.acc = dacc;
     if ((rc = cp_alloc<float>(ctx, &wn, nullptr, 64 * (size_t)nn))) return rc;
     if ((rc = cp_alloc<int4>(ctx, &nbs, nullptr, nn))) return rc;
     g.inv = dinv; g.cen = dcen; g.bin_key = bk; g.bin_rank = br; g.bin_count = bc; g.bin_start = bs;

@@ -51,6 +51,8 @@ struct IbmArgs {
     float *wnode;             // 64 per node: normalised stencil weights of its current position
     int4 *nbase;              // per node: stencil base cell (logical) and bin
     int *tile_flag;           // per bin tile: holds IBM force this step
+    int *tile_touched;        // per bin tile: received spread force in this step
+    unsigned long long *acc;  // 3 per cell (internal order): fixed-point spread force
     const float *static_force;
     float *cell_force;
 };

@@ -246,107 +246,129 @@ __global__ __launch_bounds__(256) void k_ibm_place(MemArgs m, IbmArgs g)
 }
 
 // ---------------------------------------------------------------------------
-// IBM: spreading, one 256-thread workgroup per 4x4x4 tile of cells.  The
-// nodes binned in the 8 bins whose stencils can reach the tile are taken one
-// per wavefront at a time; each lane holds one stencil slot of that node
-// (its cached weight, one coalesced read per node) and adds the weighted force
-// into the tile's LDS accumulator when the slot's cell lies in the tile.  The
-// accumulators are 64-bit fixed point, so the sums do not depend on the order
-// the nodes are visited in.
+// IBM: spreading.  One workgroup per node bin (the 4x4x4 block of cells that
+// holds the nodes' stencil base): its nodes' stencils stay inside a 7x7x7
+// window, where the weighted forces are accumulated in LDS (one wavefront per
+// node, one lane per stencil slot), then the window is added to the per-cell
+// accumulators in global memory.  All sums are 64-bit fixed point integer
+// additions, so the result does not depend on the order of the nodes, lanes
+// or workgroups.  k_ibm_finalize then turns the accumulators of the touched
+// tiles into force densities and clears them.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_ibm_spread(MemArgs m, IbmArgs g)
 {
-    const int t = blockIdx.x;
-    const int tc[3] = {t % g.tn[0], (t / g.tn[0]) % g.tn[1], t / (g.tn[0] * g.tn[1])};
-    __shared__ int src_lo[8], src_pre[9];
-    __shared__ unsigned long long acc[3][64];
+    const int b = blockIdx.x;
+    const int lo = g.bin_start[b], hi = g.bin_start[b + 1];
+    if (lo == hi) return;  // uniform over the workgroup
+    const int bo[3] = {4 * (b % g.tn[0]), 4 * ((b / g.tn[0]) % g.tn[1]), 4 * (b / (g.tn[0] * g.tn[1]))};
+    __shared__ unsigned long long racc[3][343];
+    __shared__ int cnd[64];
+    __shared__ int4 cbase[64];
+    __shared__ float4 cF[64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid < 8) {
-        const int d[3] = {(tid & 1) - 1, ((tid >> 1) & 1) - 1, (tid >> 2) - 1};
-        int bc[3];
-        bool ok = true;
-        for (int a = 0; a < 3; a++) {
-            int v = tc[a] + d[a];
-            if (v < 0 || v >= g.tn[a]) {
-                if (!g.per[a] || g.tn[a] == 1) { ok = false; break; }
-                v = v < 0 ? v + g.tn[a] : v - g.tn[a];
-            }
-            bc[a] = v;
-        }
-        int lo = 0, hi = 0;
-        if (ok) {
-            const int b = bc[0] + g.tn[0] * (bc[1] + g.tn[1] * bc[2]);
-            lo = g.bin_start[b];
-            hi = g.bin_start[b + 1];
-        }
-        src_lo[tid] = lo;
-        src_pre[tid + 1] = hi - lo;
-    }
-    if (tid < 192) acc[tid >> 6][tid & 63] = 0ull;
-    __syncthreads();
-    if (tid == 0) {
-        src_pre[0] = 0;
-        for (int q = 0; q < 8; q++) src_pre[q + 1] += src_pre[q];
-    }
-    __syncthreads();
-    const int total = src_pre[8];
-    // cell of this thread (first 64 threads): restore the static force in
-    // tiles that held IBM force last step and hold none now
-    const int l[3] = {lane & 3, (lane >> 2) & 3, lane >> 4};
-    const int w[3] = {4 * tc[0] + l[0], 4 * tc[1] + l[1], 4 * tc[2] + l[2]};
-    const bool valid = w[0] < g.n[0] && w[1] < g.n[1] && w[2] < g.n[2];
-    const int flag = g.tile_flag[t];
-    if (total == 0) {
-        if (flag) {
-            if (wave == 0 && valid) {
-                const int ic = cell_internal(g, w);
-                for (int a = 0; a < 3; a++) g.cell_force[3 * (size_t)ic + a] = g.static_force[3 * (size_t)ic + a];
-            }
-            __syncthreads();
-            if (tid == 0) g.tile_flag[t] = 0;
-        }
-        return;
-    }
+    for (int q = tid; q < 3 * 343; q += 256) racc[q / 343][q % 343] = 0ull;
     const int S = stencil_size(g);
     int off[3];
     lane_offset(S, lane, off);
     const bool act = lane < S * S * S;
-    const int t0[3] = {4 * tc[0], 4 * tc[1], 4 * tc[2]};
-    for (int q = wave; q < total; q += 4) {
-        int k = 0;
-        while (q >= src_pre[k + 1]) k++;
-        const int p = g.sorted[src_lo[k] + q - src_pre[k]];
-        const int4 nb = g.nbase[p];
-        const float4 F = m.F[p];
-        const float wt = g.wnode[64 * (size_t)p + lane];
-        if (!act || wt == 0.0f) continue;
-        const int c[3] = {nb.x + off[0], nb.y + off[1], nb.z + off[2]};
-        int loc = 0, mul = 1;
-        bool in = true;
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            int v = c[a];
-            if (g.per[a]) v = v < 0 ? v + g.n[a] : (v >= g.n[a] ? v - g.n[a] : v);
-            const int d = v - t0[a];
-            in = in && d >= 0 && d < 4;
-            loc += d * mul;
-            mul *= 4;
+    for (int c0 = lo; c0 < hi; c0 += 64) {
+        const int cnt = min(64, hi - c0);
+        __syncthreads();
+        if (tid < cnt) {
+            const int p = g.sorted[c0 + tid];
+            cnd[tid] = p;
+            cbase[tid] = g.nbase[p];
+            cF[tid] = m.F[p];
         }
-        if (!in) continue;
-        atomicAdd(&acc[0][loc], (unsigned long long)llrint((double)(F.x * wt) * FIX_SCALE));
-        atomicAdd(&acc[1][loc], (unsigned long long)llrint((double)(F.y * wt) * FIX_SCALE));
-        atomicAdd(&acc[2][loc], (unsigned long long)llrint((double)(F.z * wt) * FIX_SCALE));
+        __syncthreads();
+        for (int jn = wave; jn < cnt; jn += 4) {
+            const int p = cnd[jn];
+            const float wt = g.wnode[64 * (size_t)p + lane];
+            if (!act || wt == 0.0f) continue;
+            const int4 nb = cbase[jn];
+            const int bs[3] = {nb.x, nb.y, nb.z};
+            int loc = 0, mul = 1;
+            bool in = true;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                int c = bs[a] + off[a];
+                int l;
+                if (g.per[a]) {
+                    // window coordinate of the wrapped base, then the slot offset
+                    // (nodes advect freely across periodic boundaries, so
+                    // the base may lie several periods away)
+                    const int wb = ((bs[a] % g.n[a]) + g.n[a]) % g.n[a];
+                    l = wb - bo[a] + off[a];
+                } else {
+                    in = in && c >= 0 && c < g.n[a];
+                    l = c - bo[a];
+                }
+                in = in && l >= 0 && l < 7;
+                loc += l * mul;
+                mul *= 7;
+            }
+            if (!in) continue;
+            const float4 F = cF[jn];
+            atomicAdd(&racc[0][loc], (unsigned long long)llrint((double)(F.x * wt) * FIX_SCALE));
+            atomicAdd(&racc[1][loc], (unsigned long long)llrint((double)(F.y * wt) * FIX_SCALE));
+            atomicAdd(&racc[2][loc], (unsigned long long)llrint((double)(F.z * wt) * FIX_SCALE));
+        }
     }
     __syncthreads();
-    if (wave == 0 && valid) {
-        const int ic = cell_internal(g, w);
-        const float sc = g.kernel == LBFS_IBM_LINEAR2 ? g.inv_h * g.inv_h * g.inv_h : 1.0f / g.cen[ic].w;
+    for (int q = tid; q < 343; q += 256) {
+        const unsigned long long v0 = racc[0][q], v1 = racc[1][q], v2 = racc[2][q];
+        if ((v0 | v1 | v2) == 0ull) continue;
+        const int l[3] = {q % 7, (q / 7) % 7, q / 49};
+        int w[3];
+        bool ok = true;
+#pragma unroll
         for (int a = 0; a < 3; a++) {
-            const double v = (double)(long long)acc[a][lane] / FIX_SCALE;
-            g.cell_force[3 * (size_t)ic + a] = g.static_force[3 * (size_t)ic + a] + (float)v * sc;
+            int c = bo[a] + l[a];
+            if (c >= g.n[a]) {
+                if (!g.per[a]) { ok = false; break; }
+                c -= g.n[a];
+                if (c >= g.n[a]) c %= g.n[a];
+            }
+            w[a] = c;
+        }
+        if (!ok) continue;
+        const int ic = cell_internal(g, w);
+        unsigned long long *acc = g.acc + 3 * (size_t)ic;
+        if (v0) atomicAdd(acc, v0);
+        if (v1) atomicAdd(acc + 1, v1);
+        if (v2) atomicAdd(acc + 2, v2);
+        g.tile_touched[(w[0] >> 2) + g.tn[0] * ((w[1] >> 2) + g.tn[1] * (w[2] >> 2))] = 1;
+    }
+}
+
+// per 4x4x4 tile: force density of the touched cells from their accumulators
+// (which are cleared), static force back where IBM force was held last step
+__global__ __launch_bounds__(64) void k_ibm_finalize(IbmArgs g)
+{
+    const int t = blockIdx.x;
+    const int touched = g.tile_touched[t], flag = g.tile_flag[t];
+    if (!touched && !flag) return;
+    const int tc[3] = {t % g.tn[0], (t / g.tn[0]) % g.tn[1], t / (g.tn[0] * g.tn[1])};
+    const int lane = threadIdx.x;
+    const int w[3] = {4 * tc[0] + (lane & 3), 4 * tc[1] + ((lane >> 2) & 3), 4 * tc[2] + (lane >> 4)};
+    if (w[0] < g.n[0] && w[1] < g.n[1] && w[2] < g.n[2]) {
+        const int ic = cell_internal(g, w);
+        if (touched) {
+            const float sc = g.kernel == LBFS_IBM_LINEAR2 ? g.inv_h * g.inv_h * g.inv_h : 1.0f / g.cen[ic].w;
+            unsigned long long *acc = g.acc + 3 * (size_t)ic;
+            for (int a = 0; a < 3; a++) {
+                const double v = (double)(long long)acc[a] / FIX_SCALE;
+                g.cell_force[3 * (size_t)ic + a] = g.static_force[3 * (size_t)ic + a] + (float)v * sc;
+                acc[a] = 0ull;
+            }
+        } else {
+            for (int a = 0; a < 3; a++) g.cell_force[3 * (size_t)ic + a] = g.static_force[3 * (size_t)ic + a];
         }
     }
-    if (tid == 0 && !flag) g.tile_flag[t] = 1;
+    if (lane == 0) {
+        g.tile_flag[t] = touched;
+        g.tile_touched[t] = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -582,6 +604,7 @@ hipError_t launch_mem_nodes(const MemArgs &m, hipStream_t s)
 hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s)
 {
     hipLaunchKernelGGL(k_ibm_spread, dim3(g.n_bins), dim3(256), 0, s, m, g);
+    hipLaunchKernelGGL(k_ibm_finalize, dim3(g.n_bins), dim3(64), 0, s, g);
     return hipGetLastError();
 }
 

@@ -16,6 +16,8 @@
 // discrete sums of the reference reproduce exactly on these lattices.
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "fluid_kernels.hpp"
 #include "lattice.hpp"
 
@@ -97,6 +99,82 @@ constexpr bool opposite_pairs()
     return (L::Q % 2) == 1;
 }
 
+// Sums over the components of a lattice direction with the zero components
+// dropped at compile time: 0 * x cannot be folded away at run time (x may be
+// infinite or NaN), so these are written out per direction.  -0.0f is the
+// exact identity of addition, so the first term folds to itself.
+template <int EX, int EY, int EZ>
+__device__ __forceinline__ float edot(float a, float b, float c)
+{
+    float s = -0.0f;
+    if constexpr (EX == 1) s += a; else if constexpr (EX == -1) s -= a;
+    if constexpr (EY == 1) s += b; else if constexpr (EY == -1) s -= b;
+    if constexpr (EZ == 1) s += c; else if constexpr (EZ == -1) s -= c;
+    return s;
+}
+
+template <int E>
+__device__ __forceinline__ void eacc(float &s, float v)
+{
+    if constexpr (E == 1) s += v; else if constexpr (E == -1) s -= v;
+}
+
+// equilibrium deviation g = w (drho + rho P) of one lattice point with state
+// ph = (drho, u, v, w): P = 3 e.u + 4.5 (e.u)^2 / gamma - 1.5 u.u / gamma
+template <int EX, int EY, int EZ>
+__device__ __forceinline__ float eq_point(const float ph[4], float w, float rho0, float ig)
+{
+    const float eu = edot<EX, EY, EZ>(ph[1], ph[2], ph[3]);
+    const float uu = ph[1] * ph[1] + ph[2] * ph[2] + ph[3] * ph[3];
+    const float P = eu * (3.0f + 4.5f * ig * eu) - 1.5f * ig * uu;
+    return w * (ph[0] + (rho0 + ph[0]) * P);
+}
+
+struct FaceSums {
+    float S0, S1x, S1y, S1z, Tx, Ty, Tz;
+};
+
+// one pair of opposite directions (e, -e): one e.n and one e.grad per side
+template <class L, bool BAND, int K>
+__device__ __forceinline__ void pair_step(FaceSums &f, const float po[4], const float pn[4], const float p0[4],
+                                          const float go[12], const float gn[12], float nx, float ny, float nz,
+                                          float bw, bool ghost, float rho0, float ig)
+{
+    constexpr int ex = L::ex[K], ey = L::ey[K], ez = L::ez[K];
+    const float en = edot<ex, ey, ez>(nx, ny, nz);
+    float pk[4], pb[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const float ego = edot<ex, ey, ez>(go[3 * v], go[3 * v + 1], go[3 * v + 2]);
+        const float egn = edot<ex, ey, ez>(gn[3 * v], gn[3 * v + 1], gn[3 * v + 2]);
+        if (!BAND) {
+            // point x_f - e dt sits on the neighbour side when e.n < 0 (ties to the owner)
+            pk[v] = en < 0.0f ? pn[v] - egn : po[v] - ego;
+            pb[v] = en > 0.0f ? pn[v] + egn : po[v] + ego;
+        } else {
+            const float avg = ghost ? egn : 0.5f * (ego + egn);
+            pk[v] = en > bw ? po[v] - ego : (en < -bw ? pn[v] - egn : p0[v] - avg);
+            pb[v] = en < -bw ? po[v] + ego : (en > bw ? pn[v] + egn : p0[v] + avg);
+        }
+    }
+    const float gk = eq_point<ex, ey, ez>(pk, L::w(K), rho0, ig);
+    const float gb = eq_point<-ex, -ey, -ez>(pb, L::w(K + 1), rho0, ig);
+    const float gs = gk + gb, gd = gk - gb;
+    f.S0 += gs;
+    eacc<ex>(f.S1x, gd); eacc<ey>(f.S1y, gd); eacc<ez>(f.S1z, gd);
+    const float eng = en * gs;
+    eacc<ex>(f.Tx, eng); eacc<ey>(f.Ty, eng); eacc<ez>(f.Tz, eng);
+}
+
+template <class L, bool BAND, int... P>
+__device__ __forceinline__ void all_pairs(std::integer_sequence<int, P...>, FaceSums &f, const float po[4],
+                                          const float pn[4], const float p0[4], const float go[12],
+                                          const float gn[12], float nx, float ny, float nz, float bw, bool ghost,
+                                          float rho0, float ig)
+{
+    (pair_step<L, BAND, 2 * P + 1>(f, po, pn, p0, go, gn, nx, ny, nz, bw, ghost, rho0, ig), ...);
+}
+
 template <class L, bool BAND>
 __device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__restrict__ st,
                                             const float4 *__restrict__ grad, int tf)
@@ -162,47 +240,16 @@ __device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__
     for (int t = 0; t < 12; t++) { go[t] *= dt; gn[t] *= dt; }
     const float rho0 = a.rho0, ig = a.inv_gamma;
     const float bw = 0.001f * dt;
-    float S0, S1x = 0.0f, S1y = 0.0f, S1z = 0.0f, Tx = 0.0f, Ty = 0.0f, Tz = 0.0f;
+    FaceSums f;
+    f.S1x = f.S1y = f.S1z = f.Tx = f.Ty = f.Tz = 0.0f;
     {
         const float uu = p0[1] * p0[1] + p0[2] * p0[2] + p0[3] * p0[3];
         const float P = -1.5f * uu * ig;
-        S0 = L::w(0) * (p0[0] + (rho0 + p0[0]) * P);
+        f.S0 = L::w(0) * (p0[0] + (rho0 + p0[0]) * P);
     }
-    // equilibrium deviation g = w (drho + rho P) of one lattice point
-    auto point = [&](const float ph[4], int ex, int ey, int ez, float w) -> float {
-        const float eu = ex * ph[1] + ey * ph[2] + ez * ph[3];
-        const float uu = ph[1] * ph[1] + ph[2] * ph[2] + ph[3] * ph[3];
-        const float P = eu * (3.0f + 4.5f * ig * eu) - 1.5f * ig * uu;
-        return w * (ph[0] + (rho0 + ph[0]) * P);
-    };
-    // directions come in pairs (e, -e): one e.n and one e.grad per pair
-#pragma unroll
-    for (int k = 1; k < L::Q; k += 2) {
-        const int ex = L::ex[k], ey = L::ey[k], ez = L::ez[k];
-        const float en = ex * nx + ey * ny + ez * nz;
-        float pk[4], pb[4];
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const float ego = ex * go[3 * v] + ey * go[3 * v + 1] + ez * go[3 * v + 2];
-            const float egn = ex * gn[3 * v] + ey * gn[3 * v + 1] + ez * gn[3 * v + 2];
-            if (!BAND) {
-                // point x_f - e dt sits on the neighbour side when e.n < 0 (ties to the owner)
-                pk[v] = en < 0.0f ? pn[v] - egn : po[v] - ego;
-                pb[v] = en > 0.0f ? pn[v] + egn : po[v] + ego;
-            } else {
-                const float avg = ghost ? egn : 0.5f * (ego + egn);
-                pk[v] = en > bw ? po[v] - ego : (en < -bw ? pn[v] - egn : p0[v] - avg);
-                pb[v] = en < -bw ? po[v] + ego : (en > bw ? pn[v] + egn : p0[v] + avg);
-            }
-        }
-        const float gk = point(pk, ex, ey, ez, L::w(k));
-        const float gb = point(pb, -ex, -ey, -ez, L::w(k + 1));
-        const float gs = gk + gb, gd = gk - gb;
-        S0 += gs;
-        S1x += ex * gd; S1y += ey * gd; S1z += ez * gd;
-        const float eng = en * gs;
-        Tx += ex * eng; Ty += ey * eng; Tz += ez * eng;
-    }
+    all_pairs<L, BAND>(std::make_integer_sequence<int, (L::Q - 1) / 2>{}, f, po, pn, p0, go, gn, nx, ny, nz, bw, ghost,
+                       rho0, ig);
+    const float S0 = f.S0, S1x = f.S1x, S1y = f.S1y, S1z = f.S1z, Tx = f.Tx, Ty = f.Ty, Tz = f.Tz;
     const float rho_f = rho0 + S0;
     const float mn = S1x * nx + S1y * ny + S1z * nz;
     const float vf = a.visc_factor ? a.visc_factor[o] : 1.0f;
@@ -219,8 +266,8 @@ __device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__
     return r;
 }
 
-template <class L, bool BAND>
-__global__ __launch_bounds__(256) void k_flux_update(FluidArgs a, const float4 *__restrict__ st,
+template <class L, bool BAND, int NT>
+__global__ __launch_bounds__(NT) void k_flux_update(FluidArgs a, const float4 *__restrict__ st,
                                                      const float4 *__restrict__ grad, float4 *__restrict__ st_out,
                                                      float4 *__restrict__ res_out, int do_update)
 {
@@ -370,7 +417,10 @@ struct FluxUpdate {
     static void launch(dim3 g, size_t lds, hipStream_t s, const FluidArgs &a, const float4 *st, const float4 *grad,
                        float4 *st_out, float4 *res_out, int do_update)
     {
-        hipLaunchKernelGGL((k_flux_update<L, BAND>), g, dim3(256), lds, s, a, st, grad, st_out, res_out, do_update);
+        if (a.block_threads == 512)
+            hipLaunchKernelGGL((k_flux_update<L, BAND, 512>), g, dim3(512), lds, s, a, st, grad, st_out, res_out, do_update);
+        else
+            hipLaunchKernelGGL((k_flux_update<L, BAND, 256>), g, dim3(256), lds, s, a, st, grad, st_out, res_out, do_update);
     }
 };
 

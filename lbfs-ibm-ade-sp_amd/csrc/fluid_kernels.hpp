@@ -10,6 +10,7 @@ namespace lbfs {
 
 struct FluidArgs {
     int n_cells, n_bc, n_tiles, max_tile_faces;
+    int block_threads;             // workgroup size of the flux/update kernel (256 or 512)
     int lattice, flags;
     float rho0, inv_gamma, visc3_over_gamma, dt;
     float body_force[3];

@@ -60,7 +60,16 @@ static int fluid_setup(lbfs_ctx *ctx, const lbfs_mesh_desc &m, const lbfs_params
     std::vector<double> lhs, rhs;
     rc = compute_ls(ctx->geo, p.gradient_type, (p.flags & LBFS_FLAG_LS_DET_COMPAT) != 0, lhs, rhs, err);
     if (rc) return ctx->fail(rc, "least squares: %s", err.c_str());
-    rc = build_layout(m, ctx->geo, lhs, rhs, p.rho0, p.body_force, p.lattice == LBFS_D2Q9 ? 2 : 3, ctx->lay, err);
+    int tile[3] = {8, 8, 4};
+    if (p.tile[0] || p.tile[1] || p.tile[2]) {
+        if (p.tile[0] < 1 || p.tile[1] < 1 || p.tile[2] < 1 || p.tile[0] * p.tile[1] * p.tile[2] > 2048)
+            return ctx->fail(LBFS_E_INVALID_ARG, "tile %dx%dx%d: each side >= 1, at most 2048 cells", p.tile[0],
+                             p.tile[1], p.tile[2]);
+        for (int t = 0; t < 3; t++) tile[t] = p.tile[t];
+    }
+    if (p.block_threads != 0 && p.block_threads != 256 && p.block_threads != 512)
+        return ctx->fail(LBFS_E_INVALID_ARG, "block_threads must be 256 or 512");
+    rc = build_layout(m, ctx->geo, lhs, rhs, p.rho0, p.body_force, p.lattice == LBFS_D2Q9 ? 2 : 3, tile, ctx->lay, err);
     if (rc) return ctx->fail(rc, "layout: %s", err.c_str());
     const DeviceLayout &d = ctx->lay;
     const int n = d.n_cells, nbc = d.n_bc;
@@ -70,6 +79,9 @@ static int fluid_setup(lbfs_ctx *ctx, const lbfs_mesh_desc &m, const lbfs_params
     a.n_bc = nbc;
     a.n_tiles = d.n_tiles;
     a.max_tile_faces = d.max_tile_faces;
+    a.block_threads = p.block_threads == 512 ? 512 : 256;
+    if ((size_t)d.max_tile_faces * 16 > 96 * 1024)
+        return ctx->fail(LBFS_E_INVALID_ARG, "tile too large: %d faces need more than 96 KiB of LDS", d.max_tile_faces);
     a.lattice = p.lattice;
     a.flags = p.flags;
     a.rho0 = (float)p.rho0;

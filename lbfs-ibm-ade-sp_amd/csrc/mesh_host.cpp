@@ -290,19 +290,19 @@ int compute_ls(const Geometry &g, int grad_type, bool det_compat, std::vector<do
 // tile-blocked device layout
 // ---------------------------------------------------------------------------
 int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<double> &lhs,
-                 const std::vector<double> &rhs, double rho0, const double body_force[3], int dims, DeviceLayout &d,
-                 std::string &err)
+                 const std::vector<double> &rhs, double rho0, const double body_force[3], int dims, const int tile[3],
+                 DeviceLayout &d, std::string &err)
 {
     const int nc = g.n_cells;
     d.n_cells = nc;
     d.n_bc = g.n_bc;
     d.perm.clear();
     d.tile_cell_start.assign(1, 0);
-    // cell order: 8x8x4 blocks of the logical grid when the mesh has one,
-    // else consecutive chunks of 256 caller ids
+    // cell order: TX x TY x TZ blocks of the logical grid when the mesh has
+    // one, else consecutive chunks of TX TY TZ caller ids
     const bool logical = m.nx > 0 && m.ny > 0 && m.nz > 0 && (long long)m.nx * m.ny * m.nz == nc;
+    const int TX = tile[0], TY = tile[1], TZ = tile[2], TN = TX * TY * TZ;
     if (logical) {
-        const int TX = 8, TY = 8, TZ = 4;
         for (int tz = 0; tz < m.nz; tz += TZ)
             for (int ty = 0; ty < m.ny; ty += TY)
                 for (int tx = 0; tx < m.nx; tx += TX) {
@@ -314,7 +314,7 @@ int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<d
     } else {
         for (int i = 0; i < nc; i++) {
             d.perm.push_back(i);
-            if ((i + 1) % 256 == 0 || i + 1 == nc) d.tile_cell_start.push_back(i + 1);
+            if ((i + 1) % TN == 0 || i + 1 == nc) d.tile_cell_start.push_back(i + 1);
         }
     }
     d.n_tiles = (int)d.tile_cell_start.size() - 1;

@@ -74,7 +74,7 @@ struct DeviceLayout {
     std::vector<int> bc_owner;    // internal owner id per bc
 };
 int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<double> &lhs,
-                 const std::vector<double> &rhs, double rho0, const double body_force[3], int dims, DeviceLayout &d,
-                 std::string &err);
+                 const std::vector<double> &rhs, double rho0, const double body_force[3], int dims, const int tile[3],
+                 DeviceLayout &d, std::string &err);
 
 }  // namespace lbfs

@@ -40,7 +40,7 @@ class Params(C.Structure):
         ("lattice", C.c_int), ("gradient_type", C.c_int), ("flags", C.c_int),
         ("visc", C.c_double), ("gamma", C.c_double), ("fneq_min", C.c_double), ("dt", C.c_double),
         ("rho0", C.c_double), ("body_force", C.c_double * 3), ("channel_diameter", C.c_double),
-        ("time_stepping", C.c_int), ("max_velocity", C.c_double),
+        ("time_stepping", C.c_int), ("max_velocity", C.c_double), ("tile", C.c_int * 3), ("block_threads", C.c_int),
     ]
 
 
@@ -444,12 +444,14 @@ class Membrane:
 
 def make_params(lattice=D3Q19, visc=1.0 / 6.0, dt=0.5, rho0=1.0, gamma=1.0, fneq_min=0.0, flags=0,
                 gradient_type=GRAD_HYBRID, body_force=(0.0, 0.0, 0.0), channel_diameter=0.0, time_stepping=0,
-                max_velocity=0.0):
+                max_velocity=0.0, tile=(0, 0, 0), block_threads=0):
     p = Params()
     p.lattice, p.gradient_type, p.flags = lattice, gradient_type, flags
     p.visc, p.gamma, p.fneq_min, p.dt, p.rho0 = visc, gamma, fneq_min, dt, rho0
     p.channel_diameter = channel_diameter
     p.time_stepping, p.max_velocity = time_stepping, max_velocity
+    p.tile[0], p.tile[1], p.tile[2] = tile
+    p.block_threads = block_threads
     for t in range(3):
         p.body_force[t] = body_force[t]
     return p

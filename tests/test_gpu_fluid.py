@@ -212,3 +212,32 @@ def test_local_and_min_time_stepping_parity(ts):
     _, us, vs, ws = s.get_state()
     _, uo, vo, wo = o.state()
     assert cases.rel_l2(np.concatenate([us, vs, ws]), np.concatenate([uo, vo, wo])) < 1e-5
+
+
+def test_tile_shape_and_workgroup_size_do_not_change_results():
+    """The cell tiling and the flux/update workgroup size (the autotuner's
+    knobs) change neither arithmetic nor summation order: 30 steps are
+    bitwise equal across shapes, including ragged tiles."""
+    m = cases.channel_mesh(20, 12, 10)
+    body = (2e-5, 0.0, 0.0)
+    st = cases.smooth_state(m, seed=17)
+    out = []
+    for tile, nt in (((8, 8, 4), 256), ((8, 8, 8), 512), ((16, 4, 4), 256), ((3, 5, 7), 512), ((1, 1, 1), 256)):
+        s = abi.Solver(m, abi.make_params(lattice=abi.D3Q19, visc=VISC, dt=DT, body_force=body, tile=tile,
+                                          block_threads=nt))
+        s.set_state(*st)
+        s.step(30)
+        out.append(np.concatenate(s.get_state()))
+        s.close()
+    for o in out[1:]:
+        assert np.array_equal(o, out[0])
+
+
+def test_autotuner_picks_a_valid_candidate():
+    from lbfs_amd import autotune
+    m = cases.channel_mesh(32, 16, 16)
+    p = abi.make_params(lattice=abi.D3Q19, visc=VISC, dt=DT)
+    best, table = autotune.tune(m, p, steps=4, warmup=2)
+    assert len(table) >= 3
+    assert tuple(best.tile) in [t for t, _, _ in table]
+    assert best.block_threads in (256, 512)
