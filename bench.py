@@ -52,6 +52,19 @@ def cpu_model():
     return "unknown"
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE per dispatch, each
+    scaled by the same counter's reading on a copy of known size), or None."""
+    path = os.path.join(REPO, "profiles", "traffic_channel_128x64x64.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else rec.get("bytes_per_launch")
+
+
 def timed(solver, steps, warmup):
     """Wall time per step of `steps` whole steps after `warmup`, synchronised
     on both sides (inputs resident in HBM)."""
@@ -71,6 +84,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU design is documented, not built); "
@@ -84,6 +98,14 @@ def main():
     log("building the 128x64x64 channel")
     mesh, params, state = workloads.channel()
     n = mesh.n_cells
+    tuning = None
+    if not args.no_autotune:
+        # tile shape and workgroup size of the flux/update kernel (results do
+        # not depend on them); tuned on the plain step, outside the timed runs
+        from lbfs_amd import autotune
+        params, table = autotune.tune(mesh, params, state=state, steps=20, warmup=5)
+        tuning = [{"tile": list(t), "threads": nt, "ms_per_step": ms} for t, nt, ms in table]
+        log("autotuned tile %s, %d threads" % (list(params.tile), params.block_threads))
     plain = abi.Solver(mesh, params)
     plain.set_state(*state)
     ms_plain = timed(plain, args.steps, args.warmup)
@@ -113,6 +135,8 @@ def main():
     if not args.no_batched:
         log("building the 256-cell suspension")
         m3, p3, s3, mems, d0 = workloads.suspension()
+        p3.tile[0], p3.tile[1], p3.tile[2] = params.tile[0], params.tile[1], params.tile[2]
+        p3.block_threads = params.block_threads
         b_plain = abi.Solver(m3, p3)
         b_plain.set_state(*s3)
         ms3_plain = timed(b_plain, max(args.steps // 4, 10), 5)
@@ -153,13 +177,16 @@ def main():
         "data": "synthetic (seeded jittered hex mesh, rest state + body force, seeded RBC shapes)",
         "config": {"workload": "channel 128x64x64 jittered hex (+-10%, seed 2), x-periodic, walls y/z, D3Q19, "
                                "body-force Poiseuille, one Evans-Fung RBC 2562 nodes / 5120 triangles, 4-point IBM",
-                   "cells": n, "faces": mesh.n_faces, "lattice": "D3Q19"},
+                   "cells": n, "faces": mesh.n_faces, "lattice": "D3Q19",
+                   "tile": [params.tile[0], params.tile[1], params.tile[2]],
+                   "block_threads": params.block_threads or 256},
+        "autotune": tuning,
         "achieved_hbm_gbps_model": step_gbps,
         "plain_ms_per_step": ms_plain,
         "plain_mlups": n / (ms_plain * 1e-3) / 1e6,
         "rbc_step_vs_plain": ms_step / ms_plain,
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(names[dom]),
                      "model_bytes_per_launch": mbytes[dom], "avg_launch_ms": kms[names[dom]]},
         "kernels_ms": kms,
         "event_ms_per_step": ev_ms,

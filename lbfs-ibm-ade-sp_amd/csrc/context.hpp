@@ -16,6 +16,8 @@ struct lbfs_ctx {
     std::string err;
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;       // second stream: the gradient pass beside the coupling chain
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     lbfs_params params{};
     int mesh_nx = 0, mesh_ny = 0, mesh_nz = 0;
     double mesh_origin[3] = {0, 0, 0};

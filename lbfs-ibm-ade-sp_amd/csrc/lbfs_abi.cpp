@@ -224,6 +224,13 @@ extern "C" int lbfs_create(const lbfs_mesh_desc *mesh, const lbfs_params *params
         delete ctx;
         return LBFS_E_HIP;
     }
+    if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
+        g_create_error = "could not create a HIP stream or event";
+        lbfs_destroy(ctx);
+        return LBFS_E_HIP;
+    }
     int rc = fluid_setup(ctx, *mesh, *params);
     if (rc) {
         g_create_error = ctx->err;
@@ -247,7 +254,11 @@ extern "C" int lbfs_destroy(lbfs_ctx *ctx)
     ctx->release_extensions();
     for (void *p : ctx->allocs) hipFree(p);
     ctx->allocs.clear();
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return LBFS_OK;
 }

@@ -93,8 +93,28 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
 
 int lbfs_ctx::run_stages(bool with_visc)
 {
-    for (auto &st : step_stages(with_visc)) {
-        int rc = st.run();
+    lbfs_ctx *ctx = this;
+    if (!have_membranes || !side) {
+        for (auto &st : step_stages(with_visc)) {
+            int rc = st.run();
+            if (rc) return rc;
+        }
+        return LBFS_OK;
+    }
+    // With membranes the gradient pass, which reads only the state at the
+    // start of the step, runs on the second stream beside the coupling chain
+    // (a latency-bound sequence of small kernels); the flux/update waits for
+    // both.  Same kernels, same inputs, so the results are those of the
+    // sequential order.
+    HIPCHK(hipEventRecord(ev_fork, stream));
+    HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
+    HIPCHK(launch_gradients(args, st[cur], grad, side));
+    HIPCHK(hipEventRecord(ev_join, side));
+    for (auto &s : step_stages(with_visc)) {
+        if (std::strcmp(s.name, "gradients") == 0) continue;
+        if (std::strcmp(s.name, "flux_update") == 0 || std::strcmp(s.name, "dt_min") == 0)
+            HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
+        int rc = s.run();
         if (rc) return rc;
     }
     return LBFS_OK;
