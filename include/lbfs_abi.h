@@ -64,6 +64,8 @@ extern "C" {
 /* ---- IBM kernels ---- */
 #define LBFS_IBM_LINEAR2 0 /* 2-point trilinear on the logical grid, immersed_boundary_method.cu:14-97, 947-1024 */
 #define LBFS_IBM_COS4 1    /* 4-point cosine on physical distances, renormalised */
+#define LBFS_IBM_LAGRANGE4 2 /* 4-point Lagrange per axis for tensor-product (stretched) grids,
+                                immersed_boundary_method.cu:588-872, 919-941 */
 
 /*
  * Hexahedral mesh in OpenFOAM polyMesh form (unstructured_mesh.cpp:670-941):
@@ -88,6 +90,14 @@ typedef struct {
     int nx, ny, nz;
     double origin[3];
     double spacing;              /* nominal cell size (IBM stencil unit) */
+    /* ensemble of independent boxes stepped together (no counterpart in the
+     * reference): cells numbered box after box, each box logical on its own
+     * (i + j nx + k nx ny after the box's first cell); nx, ny, nz, origin and
+     * spacing above are then unused.  0 or 1 = the single box above. */
+    int n_blocks;
+    const int *block_dims;       /* [3 n_blocks] */
+    const double *block_origin;  /* [3 n_blocks] */
+    const double *block_spacing; /* [n_blocks] */
 } lbfs_mesh_desc;
 
 typedef struct {
@@ -196,6 +206,7 @@ typedef struct {
                                          (Solver_gpu.cu:884-889, lagrangian_object.h:85) */
     int n_springs;                    /* 0: springs built from the triangles (lagrangian_object.cpp:322-490) */
     const int *springs;               /* [4 n_springs] a, b, c, d as read from an MZ file (mesh_read) */
+    int block;                        /* box of the ensemble the membrane lives in (0 for one box) */
 } lbfs_membrane_desc;
 
 /* attach membranes (replaces any attached before); requires the mesh's logical

@@ -19,9 +19,7 @@ struct lbfs_ctx {
     hipStream_t side = nullptr;       // second stream: the gradient pass beside the coupling chain
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     lbfs_params params{};
-    int mesh_nx = 0, mesh_ny = 0, mesh_nz = 0;
-    double mesh_origin[3] = {0, 0, 0};
-    double mesh_spacing = 0.0;
+    std::vector<lbfs::HostBlock> blocks;   // logical boxes of the mesh (empty: none)
 
     lbfs::Geometry geo;
     lbfs::DeviceLayout lay;

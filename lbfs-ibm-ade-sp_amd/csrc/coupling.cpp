@@ -44,12 +44,14 @@ void lbfs_ctx::release_extensions()
 extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, int count, int ibm_kernel)
 {
     if (!ctx || !mem || count <= 0) return LBFS_E_INVALID_ARG;
-    if (ibm_kernel != LBFS_IBM_LINEAR2 && ibm_kernel != LBFS_IBM_COS4)
+    if (ibm_kernel != LBFS_IBM_LINEAR2 && ibm_kernel != LBFS_IBM_COS4 && ibm_kernel != LBFS_IBM_LAGRANGE4)
         return ctx->fail(LBFS_E_INVALID_ARG, "unknown IBM kernel %d", ibm_kernel);
     if (ctx->args.ts_mode != 0) return ctx->fail(LBFS_E_UNSUPPORTED, "membranes need constant time steps");
-    if (ctx->mesh_nx <= 0 || ctx->mesh_ny <= 0 || ctx->mesh_nz <= 0 || !(ctx->mesh_spacing > 0.0) ||
-        (long long)ctx->mesh_nx * ctx->mesh_ny * ctx->mesh_nz != ctx->lay.n_cells)
+    if (ctx->blocks.empty() || !(ctx->blocks[0].spacing > 0.0))
         return ctx->fail(LBFS_E_UNSUPPORTED, "membranes need a mesh with a logical i,j,k structure and spacing");
+    for (int i = 0; i < count; i++)
+        if (mem[i].block < 0 || mem[i].block >= (int)ctx->blocks.size())
+            return ctx->fail(LBFS_E_INVALID_ARG, "membrane %d: block %d does not exist", i, mem[i].block);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->drop_graph();
     ctx->release_extensions();
@@ -173,28 +175,97 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
         ctx->mem_origin[3 * (size_t)i + 1] = origin[i].y;
         ctx->mem_origin[3 * (size_t)i + 2] = origin[i].z;
     }
-    // IBM grid
+    // IBM grids: one per logical box, bins (= spreading tiles) numbered box after box
     IbmArgs &g = ctx->ibm;
     memset(&g, 0, sizeof(g));
     g.kernel = ibm_kernel;
-    const int nl[3] = {ctx->mesh_nx, ctx->mesh_ny, ctx->mesh_nz};
-    for (int a = 0; a < 3; a++) {
-        g.n[a] = nl[a];
-        g.origin[a] = (float)ctx->mesh_origin[a];
-        g.L[a] = (float)(nl[a] * ctx->mesh_spacing);
-        g.per[a] = 0;
-        g.tn[a] = (nl[a] + 3) / 4;
+    const int nblk = (int)ctx->blocks.size();
+    std::vector<IbmBlock> blk(nblk);
+    std::vector<int> bin_block;
+    std::vector<int> cell_blk_caller(ctx->lay.n_cells, 0);
+    for (int b = 0; b < nblk; b++) {
+        const HostBlock &hb = ctx->blocks[b];
+        IbmBlock &B = blk[b];
+        for (int a = 0; a < 3; a++) {
+            B.n[a] = hb.n[a];
+            B.origin[a] = (float)hb.origin[a];
+            B.L[a] = (float)(hb.n[a] * hb.spacing);
+            B.per[a] = 0;
+            B.tn[a] = (hb.n[a] + 3) / 4;
+        }
+        B.h = (float)hb.spacing;
+        B.inv_h = (float)(1.0 / hb.spacing);
+        B.cell0 = hb.cell0;
+        B.bin0 = (int)bin_block.size();
+        bin_block.insert(bin_block.end(), (size_t)B.tn[0] * B.tn[1] * B.tn[2], b);
+        const int nb_cells = hb.n[0] * hb.n[1] * hb.n[2];
+        for (int c = 0; c < nb_cells; c++) cell_blk_caller[hb.cell0 + c] = b;
     }
+    // periodic axes of each box from the translations of its wrap faces
     for (int f = 0; f < ctx->geo.n_faces; f++)
         for (int a = 0; a < 3; a++)
             if (ctx->geo.shift[3 * (size_t)f + a] != 0.0) {
-                g.per[a] = 1;
-                g.L[a] = (float)std::fabs(ctx->geo.shift[3 * (size_t)f + a]);
+                IbmBlock &B = blk[cell_blk_caller[ctx->geo.owner[f]]];
+                B.per[a] = 1;
+                B.L[a] = (float)std::fabs(ctx->geo.shift[3 * (size_t)f + a]);
             }
-    g.h = (float)ctx->mesh_spacing;
-    g.inv_h = (float)(1.0 / ctx->mesh_spacing);
-    g.n_bins = g.tn[0] * g.tn[1] * g.tn[2];
+    // 4-point Lagrange kernel: per box and axis the vertex planes along the
+    // first grid line (face centroids of its cells, the reference's plateau
+    // arrays) and the cell-line centroids
+    std::vector<double> axis;
+    for (int b = 0; b < nblk; b++) {
+        const HostBlock &hb = ctx->blocks[b];
+        const Geometry &gm = ctx->geo;
+        const int stride[3] = {1, hb.n[0], hb.n[0] * hb.n[1]};
+        for (int a = 0; a < 3; a++) {
+            blk[b].ax_off[a] = (int)axis.size();
+            const int n = hb.n[a];
+            std::vector<double> plane(n + 1), line(n);
+            for (int i = 0; i < n; i++) {
+                const int c = hb.cell0 + i * stride[a];
+                line[i] = gm.cc[3 * (size_t)c + a];
+                double lo_n = 2.0, hi_n = -2.0;
+                for (int k = 0; k < 6; k++) {
+                    const int f = gm.gface[6 * (size_t)c + k];
+                    const bool nb = gm.grole[6 * (size_t)c + k] == 1;
+                    const double na = (nb ? -1.0 : 1.0) * gm.normal[3 * (size_t)f + a];
+                    const double x = gm.fc[3 * (size_t)f + a] - (nb ? gm.shift[3 * (size_t)f + a] : 0.0);
+                    if (na < lo_n) { lo_n = na; plane[i] = x; }
+                    if (i == n - 1 && na > hi_n) { hi_n = na; plane[n] = x; }
+                }
+            }
+            axis.insert(axis.end(), plane.begin(), plane.end());
+            axis.insert(axis.end(), line.begin(), line.end());
+        }
+    }
+    double *dax;
+    if ((rc = cp_alloc(ctx, &dax, axis.data(), axis.size()))) return rc;
+    g.axis = dax;
+    g.n_blocks = nblk;
+    g.n_bins = (int)bin_block.size();
     g.dt = (float)ctx->params.dt;
+    std::vector<int> node_block(nn), mem_block(count);
+    for (int i = 0; i < count; i++) {
+        mem_block[i] = mem[i].block;
+        for (int k = mem_node_start[i]; k < mem_node_start[i + 1]; k++) node_block[k] = mem[i].block;
+    }
+    IbmBlock *dblk;
+    int *dbb2, *dnb, *dmb;
+    if ((rc = cp_alloc(ctx, &dblk, blk.data(), blk.size()))) return rc;
+    if ((rc = cp_alloc(ctx, &dbb2, bin_block.data(), bin_block.size()))) return rc;
+    if ((rc = cp_alloc(ctx, &dnb, node_block.data(), node_block.size()))) return rc;
+    if ((rc = cp_alloc(ctx, &dmb, mem_block.data(), mem_block.size()))) return rc;
+    g.blk = dblk;
+    g.bin_block = dbb2;
+    g.node_block = dnb;
+    ctx->mem.mem_block = dmb;
+    if (nblk > 1) {
+        std::vector<int> cb(ctx->lay.n_cells);
+        for (int i = 0; i < ctx->lay.n_cells; i++) cb[i] = cell_blk_caller[ctx->lay.perm[i]];
+        int *dcb;
+        if ((rc = cp_alloc(ctx, &dcb, cb.data(), cb.size()))) return rc;
+        g.cell_block = dcb;
+    }
     const int nc = ctx->lay.n_cells;
     std::vector<float4> cen(nc);
     for (int i = 0; i < nc; i++) {
@@ -238,7 +309,9 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
         ctx->visc_factor = vf;
         ctx->args.visc_factor = vf;
     }
-    ctx->delta_h = (float)std::min(0.5 * ctx->mesh_spacing, 0.1);
+    double hmin = ctx->blocks[0].spacing;
+    for (const HostBlock &hb : ctx->blocks) hmin = std::min(hmin, hb.spacing);
+    ctx->delta_h = (float)std::min(0.5 * hmin, 0.1);
     ctx->t_step = 0;
     // stencil weights of the initial positions (the first interpolation uses them)
     HIPCHK(launch_ibm_weights(m, g, ctx->stream));

@@ -35,16 +35,30 @@ struct MemArgs {
     const MemParamDev *params;
     double *glob;             // 8 per membrane: V, A, MAD, V0, A0, MAD0, A_prev, 0
     float4 *bbox;             // 2 per membrane: lower and upper corner of the nodes
+    const int *mem_block;     // box of each membrane
+};
+
+// one logical box of cells (an ensemble member, or the whole mesh)
+struct IbmBlock {
+    int n[3], per[3];
+    int tn[3];                // 4x4x4 bins (= spreading tiles) per axis
+    int cell0;                // first caller cell id of the box (cells i + j nx + k nx ny after it)
+    int bin0;                 // first bin id of the box
+    int ax_off[3];            // 4-point Lagrange kernel: offset of the axis data in IbmArgs::axis
+    float origin[3], L[3], h, inv_h;
 };
 
 struct IbmArgs {
     int kernel;               // LBFS_IBM_*
     int n_cells;
-    int n[3], per[3];
-    float origin[3], L[3], h, inv_h;
-    int tn[3], n_bins;        // 4x4x4 bins of the logical grid
+    int n_blocks, n_bins;
+    const IbmBlock *blk;      // [n_blocks]
+    const int *bin_block;     // [n_bins] box of each bin
+    const int *node_block;    // [n_nodes] box of each node
+    const int *cell_block;    // [n_cells] box of each cell (internal order), null for one box
+    const double *axis;       // per box and axis: n + 1 vertex plane coordinates, then n cell-line centroids
     float dt;
-    const int *inv;           // caller cell id (i + j nx + k nx ny) -> internal id
+    const int *inv;           // caller cell id -> internal id
     const float4 *cen;        // internal order: centroid xyz, volume
     int *bin_key, *bin_rank, *bin_count, *bin_start, *sorted;
     int *scan_part;           // per 1024 bins: count total

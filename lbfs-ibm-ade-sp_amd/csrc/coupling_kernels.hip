@@ -43,21 +43,66 @@ __device__ __forceinline__ float cmp(f3 a, int k) { return k == 0 ? a.x : (k == 
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int stencil_size(const IbmArgs &g) { return g.kernel == LBFS_IBM_LINEAR2 ? 2 : 4; }
 
-__device__ __forceinline__ void node_base(const IbmArgs &g, const float P[3], float xr[3], int base[3])
+// nearest vertex plane, ties to the larger index (find_index_closest,
+// immersed_boundary_method.cu:588-636), by bisection
+__device__ __forceinline__ int closest_vertex(const double *v, int n, double t)
+{
+    if (t <= v[0]) return 0;
+    if (t >= v[n - 1]) return n - 1;
+    int lo = 0, hi = n - 1;  // v[lo] < t <= v[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (v[mid] < t) lo = mid; else hi = mid;
+    }
+    return (t - v[lo] >= v[hi] - t) ? hi : lo;
+}
+
+// cell-line coordinate at unwrapped index c (periodic image or wall continuation)
+__device__ __forceinline__ double line_coord(const IbmArgs &g, const IbmBlock &B, int a, int c)
+{
+    const int n = B.n[a];
+    const double *l = g.axis + B.ax_off[a] + n + 1;
+    if (c >= 0 && c < n) return l[c];
+    if (B.per[a]) {
+        const int k = c >= 0 ? c / n : -((-c + n - 1) / n);
+        return l[c - k * n] + (double)k * (double)B.L[a];
+    }
+    const double d0 = n > 1 ? l[1] - l[0] : (double)B.h, d1 = n > 1 ? l[n - 1] - l[n - 2] : (double)B.h;
+    return c < 0 ? l[0] + c * d0 : l[n - 1] + (c - (n - 1)) * d1;
+}
+
+// cubic Lagrange basis i of the 4 line points from `base` at x (the first
+// column of the inverse Vandermonde matrix, vandemonde_inverse :919-941)
+__device__ __forceinline__ double lagrange4(const IbmArgs &g, const IbmBlock &B, int a, int base, int i, double x)
+{
+    double p[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) p[k] = line_coord(g, B, a, base + k);
+    double v = 1.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (k != i) v *= (x - p[k]) / (p[i] - p[k]);
+    return v;
+}
+
+__device__ __forceinline__ void node_base(const IbmArgs &g, const IbmBlock &B, const float P[3], float xr[3], int base[3])
 {
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        xr[a] = (P[a] - g.origin[a]) * g.inv_h;
-        base[a] = g.kernel == LBFS_IBM_LINEAR2 ? (int)roundf(xr[a]) - 1 : (int)floorf(xr[a] - 1.5f);
+        xr[a] = (P[a] - B.origin[a]) * B.inv_h;
+        if (g.kernel == LBFS_IBM_LAGRANGE4)
+            base[a] = closest_vertex(g.axis + B.ax_off[a], B.n[a] + 1, (double)P[a]) - 2;
+        else
+            base[a] = g.kernel == LBFS_IBM_LINEAR2 ? (int)roundf(xr[a]) - 1 : (int)floorf(xr[a] - 1.5f);
     }
 }
 
 // wrap a logical index into the grid; false if it lies outside a wall
-__device__ __forceinline__ bool wrap_index(const IbmArgs &g, int a, int i, int &w)
+__device__ __forceinline__ bool wrap_index(const IbmBlock &B, int a, int i, int &w)
 {
-    const int n = g.n[a];
+    const int n = B.n[a];
     if (i >= 0 && i < n) { w = i; return true; }
-    if (!g.per[a]) return false;
+    if (!B.per[a]) return false;
     w = ((i % n) + n) % n;
     return true;
 }
@@ -77,8 +122,8 @@ __device__ __forceinline__ float cos4_phi(float r)
     return 0.5f * fmaf(-sn, sn, 1.0f);
 }
 
-__device__ __forceinline__ float cell_weight(const IbmArgs &g, const int idx[3], const int w[3], int internal,
-                                             const float P[3], const float xr[3])
+__device__ __forceinline__ float cell_weight(const IbmArgs &g, const IbmBlock &B, const int idx[3], const int w[3],
+                                             int internal, const float P[3], const float xr[3])
 {
     float wt = 1.0f;
     if (g.kernel == LBFS_IBM_LINEAR2) {
@@ -89,24 +134,24 @@ __device__ __forceinline__ float cell_weight(const IbmArgs &g, const int idx[3],
         const float cc[3] = {c.x, c.y, c.z};
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            const float xc = cc[a] + (float)(idx[a] - w[a]) * (g.L[a] / (float)g.n[a]);
-            const float r = (xc - P[a]) * g.inv_h;
+            const float xc = cc[a] + (float)(idx[a] - w[a]) * (B.L[a] / (float)B.n[a]);
+            const float r = (xc - P[a]) * B.inv_h;
             wt *= fabsf(r) < 2.0f ? cos4_phi(r) : 0.0f;
         }
     }
     return wt;
 }
 
-__device__ __forceinline__ int cell_internal(const IbmArgs &g, const int w[3])
+__device__ __forceinline__ int cell_internal(const IbmArgs &g, const IbmBlock &B, const int w[3])
 {
-    return g.inv[w[0] + g.n[0] * (w[1] + g.n[1] * w[2])];
+    return g.inv[B.cell0 + w[0] + B.n[0] * (w[1] + B.n[1] * w[2])];
 }
 
-__device__ __forceinline__ int bin_coord(const IbmArgs &g, int a, int base)
+__device__ __forceinline__ int bin_coord(const IbmBlock &B, int a, int base)
 {
-    const int n = g.n[a];
+    const int n = B.n[a];
     int b = base;
-    if (g.per[a]) b = ((b % n) + n) % n;
+    if (B.per[a]) b = ((b % n) + n) % n;
     else b = b < 0 ? 0 : (b >= n ? n - 1 : b);
     return b >> 2;
 }
@@ -145,6 +190,7 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
     const bool act = lane < S * S * S;
     int off[3];
     lane_offset(S, lane, off);
+    const IbmBlock &B = g.blk[g.node_block[nd]];
     const float4 o = m.origin[m.node_mem[nd]];
     float4 xl = m.X[nd];
     if (INTERP) {
@@ -153,9 +199,9 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
         if (act) {
             const int idx[3] = {nb.x + off[0], nb.y + off[1], nb.z + off[2]};
             int w[3];
-            if (wrap_index(g, 0, idx[0], w[0]) && wrap_index(g, 1, idx[1], w[1]) && wrap_index(g, 2, idx[2], w[2])) {
+            if (wrap_index(B, 0, idx[0], w[0]) && wrap_index(B, 1, idx[1], w[1]) && wrap_index(B, 2, idx[2], w[2])) {
                 const float wt = g.wnode[64 * (size_t)nd + lane];
-                const float4 s = st[cell_internal(g, w)];
+                const float4 s = st[cell_internal(g, B, w)];
                 ux = wt * s.y; uy = wt * s.z; uz = wt * s.w;
             }
         }
@@ -169,19 +215,26 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
     const float P[3] = {o.x + xl.x, o.y + xl.y, o.z + xl.z};
     float xr[3];
     int base[3];
-    node_base(g, P, xr, base);
+    node_base(g, B, P, xr, base);
     float wt = 0.0f;
     if (act) {
         const int idx[3] = {base[0] + off[0], base[1] + off[1], base[2] + off[2]};
         int w[3];
-        if (wrap_index(g, 0, idx[0], w[0]) && wrap_index(g, 1, idx[1], w[1]) && wrap_index(g, 2, idx[2], w[2]))
-            wt = cell_weight(g, idx, w, cell_internal(g, w), P, xr);
+        if (wrap_index(B, 0, idx[0], w[0]) && wrap_index(B, 1, idx[1], w[1]) && wrap_index(B, 2, idx[2], w[2])) {
+            if (g.kernel == LBFS_IBM_LAGRANGE4)
+                wt = (float)(lagrange4(g, B, 0, base[0], off[0], (double)P[0]) *
+                             lagrange4(g, B, 1, base[1], off[1], (double)P[1]) *
+                             lagrange4(g, B, 2, base[2], off[2], (double)P[2]));
+            else
+                wt = cell_weight(g, B, idx, w, cell_internal(g, B, w), P, xr);
+        }
     }
     const float ws = wave_sum(wt);
     const float inv = g.kernel == LBFS_IBM_COS4 ? (ws > 0.0f ? 1.0f / ws : 0.0f) : 1.0f;
     g.wnode[64 * (size_t)nd + lane] = wt * inv;
     if (lane == 0) {
-        const int key = bin_coord(g, 0, base[0]) + g.tn[0] * (bin_coord(g, 1, base[1]) + g.tn[1] * bin_coord(g, 2, base[2]));
+        const int key = B.bin0 + bin_coord(B, 0, base[0]) +
+                        B.tn[0] * (bin_coord(B, 1, base[1]) + B.tn[1] * bin_coord(B, 2, base[2]));
         g.nbase[nd] = make_int4(base[0], base[1], base[2], key);
         if (INTERP) {
             g.bin_key[nd] = key;
@@ -260,7 +313,9 @@ __global__ __launch_bounds__(256) void k_ibm_spread(MemArgs m, IbmArgs g)
     const int b = blockIdx.x;
     const int lo = g.bin_start[b], hi = g.bin_start[b + 1];
     if (lo == hi) return;  // uniform over the workgroup
-    const int bo[3] = {4 * (b % g.tn[0]), 4 * ((b / g.tn[0]) % g.tn[1]), 4 * (b / (g.tn[0] * g.tn[1]))};
+    const IbmBlock &B = g.blk[g.bin_block[b]];
+    const int lb = b - B.bin0;
+    const int bo[3] = {4 * (lb % B.tn[0]), 4 * ((lb / B.tn[0]) % B.tn[1]), 4 * (lb / (B.tn[0] * B.tn[1]))};
     __shared__ unsigned long long racc[3][343];
     __shared__ int cnd[64];
     __shared__ int4 cbase[64];
@@ -293,14 +348,14 @@ __global__ __launch_bounds__(256) void k_ibm_spread(MemArgs m, IbmArgs g)
             for (int a = 0; a < 3; a++) {
                 int c = bs[a] + off[a];
                 int l;
-                if (g.per[a]) {
+                if (B.per[a]) {
                     // window coordinate of the wrapped base, then the slot offset
                     // (nodes advect freely across periodic boundaries, so
                     // the base may lie several periods away)
-                    const int wb = ((bs[a] % g.n[a]) + g.n[a]) % g.n[a];
+                    const int wb = ((bs[a] % B.n[a]) + B.n[a]) % B.n[a];
                     l = wb - bo[a] + off[a];
                 } else {
-                    in = in && c >= 0 && c < g.n[a];
+                    in = in && c >= 0 && c < B.n[a];
                     l = c - bo[a];
                 }
                 in = in && l >= 0 && l < 7;
@@ -324,20 +379,20 @@ __global__ __launch_bounds__(256) void k_ibm_spread(MemArgs m, IbmArgs g)
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             int c = bo[a] + l[a];
-            if (c >= g.n[a]) {
-                if (!g.per[a]) { ok = false; break; }
-                c -= g.n[a];
-                if (c >= g.n[a]) c %= g.n[a];
+            if (c >= B.n[a]) {
+                if (!B.per[a]) { ok = false; break; }
+                c -= B.n[a];
+                if (c >= B.n[a]) c %= B.n[a];
             }
             w[a] = c;
         }
         if (!ok) continue;
-        const int ic = cell_internal(g, w);
+        const int ic = cell_internal(g, B, w);
         unsigned long long *acc = g.acc + 3 * (size_t)ic;
         if (v0) atomicAdd(acc, v0);
         if (v1) atomicAdd(acc + 1, v1);
         if (v2) atomicAdd(acc + 2, v2);
-        g.tile_touched[(w[0] >> 2) + g.tn[0] * ((w[1] >> 2) + g.tn[1] * (w[2] >> 2))] = 1;
+        g.tile_touched[B.bin0 + (w[0] >> 2) + B.tn[0] * ((w[1] >> 2) + B.tn[1] * (w[2] >> 2))] = 1;
     }
 }
 
@@ -348,13 +403,15 @@ __global__ __launch_bounds__(64) void k_ibm_finalize(IbmArgs g)
     const int t = blockIdx.x;
     const int touched = g.tile_touched[t], flag = g.tile_flag[t];
     if (!touched && !flag) return;
-    const int tc[3] = {t % g.tn[0], (t / g.tn[0]) % g.tn[1], t / (g.tn[0] * g.tn[1])};
+    const IbmBlock &B = g.blk[g.bin_block[t]];
+    const int lt = t - B.bin0;
+    const int tc[3] = {lt % B.tn[0], (lt / B.tn[0]) % B.tn[1], lt / (B.tn[0] * B.tn[1])};
     const int lane = threadIdx.x;
     const int w[3] = {4 * tc[0] + (lane & 3), 4 * tc[1] + ((lane >> 2) & 3), 4 * tc[2] + (lane >> 4)};
-    if (w[0] < g.n[0] && w[1] < g.n[1] && w[2] < g.n[2]) {
-        const int ic = cell_internal(g, w);
+    if (w[0] < B.n[0] && w[1] < B.n[1] && w[2] < B.n[2]) {
+        const int ic = cell_internal(g, B, w);
         if (touched) {
-            const float sc = g.kernel == LBFS_IBM_LINEAR2 ? g.inv_h * g.inv_h * g.inv_h : 1.0f / g.cen[ic].w;
+            const float sc = g.kernel == LBFS_IBM_LINEAR2 ? B.inv_h * B.inv_h * B.inv_h : 1.0f / g.cen[ic].w;
             unsigned long long *acc = g.acc + 3 * (size_t)ic;
             for (int a = 0; a < 3; a++) {
                 const double v = (double)(long long)acc[a] / FIX_SCALE;
@@ -648,11 +705,12 @@ __global__ __launch_bounds__(256) void k_internal_visc(MemArgs m, IbmArgs g, flo
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.n_cells) return;
     const float4 c = g.cen[i];
+    const int cb = g.cell_block ? g.cell_block[i] : 0;
     float out = 1.0f;
     bool any = false;
     for (int mem = 0; mem < m.n_mem; mem++) {
         const float r = m.params[mem].visc_ratio;
-        if (r == 1.0f) continue;
+        if (r == 1.0f || m.mem_block[mem] != cb) continue;
         const float4 lo = m.bbox[2 * mem], hi = m.bbox[2 * mem + 1];
         const float xl = lo.x - 2.0f * delta_h, xh = hi.x + 2.0f * delta_h;
         if (c.x < xl || c.x > xh || c.y < lo.y - 2.0f * delta_h || c.y > hi.y + 2.0f * delta_h ||

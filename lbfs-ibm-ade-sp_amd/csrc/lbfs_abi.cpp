@@ -69,7 +69,10 @@ static int fluid_setup(lbfs_ctx *ctx, const lbfs_mesh_desc &m, const lbfs_params
     }
     if (p.block_threads != 0 && p.block_threads != 256 && p.block_threads != 512)
         return ctx->fail(LBFS_E_INVALID_ARG, "block_threads must be 256 or 512");
-    rc = build_layout(m, ctx->geo, lhs, rhs, p.rho0, p.body_force, p.lattice == LBFS_D2Q9 ? 2 : 3, tile, ctx->lay, err);
+    rc = mesh_blocks(m, ctx->blocks, err);
+    if (rc) return ctx->fail(rc, "mesh blocks: %s", err.c_str());
+    rc = build_layout(m, ctx->geo, lhs, rhs, p.rho0, p.body_force, p.lattice == LBFS_D2Q9 ? 2 : 3, tile, ctx->blocks,
+                      ctx->lay, err);
     if (rc) return ctx->fail(rc, "layout: %s", err.c_str());
     const DeviceLayout &d = ctx->lay;
     const int n = d.n_cells, nbc = d.n_bc;
@@ -216,9 +219,6 @@ extern "C" int lbfs_create(const lbfs_mesh_desc *mesh, const lbfs_params *params
     }
     lbfs_ctx *ctx = new lbfs_ctx();
     ctx->params = *params;
-    ctx->mesh_nx = mesh->nx; ctx->mesh_ny = mesh->ny; ctx->mesh_nz = mesh->nz;
-    for (int t = 0; t < 3; t++) ctx->mesh_origin[t] = mesh->origin[t];
-    ctx->mesh_spacing = mesh->spacing;
     if (hipGetDevice(&ctx->device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         g_create_error = "could not create a HIP stream";
         delete ctx;

@@ -289,9 +289,44 @@ int compute_ls(const Geometry &g, int grad_type, bool det_compat, std::vector<do
 // ---------------------------------------------------------------------------
 // tile-blocked device layout
 // ---------------------------------------------------------------------------
+int mesh_blocks(const lbfs_mesh_desc &m, std::vector<HostBlock> &blocks, std::string &err)
+{
+    blocks.clear();
+    if (m.n_blocks > 1) {
+        if (!m.block_dims || !m.block_origin || !m.block_spacing) { err = "block arrays missing"; return LBFS_E_INVALID_ARG; }
+        long long c0 = 0;
+        for (int b = 0; b < m.n_blocks; b++) {
+            HostBlock h;
+            for (int t = 0; t < 3; t++) {
+                h.n[t] = m.block_dims[3 * b + t];
+                h.origin[t] = m.block_origin[3 * b + t];
+            }
+            h.spacing = m.block_spacing[b];
+            if (h.n[0] < 1 || h.n[1] < 1 || h.n[2] < 1 || !(h.spacing > 0.0)) {
+                err = "block " + std::to_string(b) + ": dims and spacing must be positive";
+                return LBFS_E_INVALID_ARG;
+            }
+            h.cell0 = (int)c0;
+            c0 += (long long)h.n[0] * h.n[1] * h.n[2];
+            blocks.push_back(h);
+        }
+        if (c0 != m.n_cells) { err = "block cells do not add up to n_cells"; return LBFS_E_INVALID_ARG; }
+        return LBFS_OK;
+    }
+    if (m.nx > 0 && m.ny > 0 && m.nz > 0 && (long long)m.nx * m.ny * m.nz == m.n_cells) {
+        HostBlock h;
+        h.n[0] = m.nx; h.n[1] = m.ny; h.n[2] = m.nz;
+        for (int t = 0; t < 3; t++) h.origin[t] = m.origin[t];
+        h.spacing = m.spacing;
+        h.cell0 = 0;
+        blocks.push_back(h);
+    }
+    return LBFS_OK;
+}
+
 int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<double> &lhs,
                  const std::vector<double> &rhs, double rho0, const double body_force[3], int dims, const int tile[3],
-                 DeviceLayout &d, std::string &err)
+                 const std::vector<HostBlock> &blocks, DeviceLayout &d, std::string &err)
 {
     const int nc = g.n_cells;
     d.n_cells = nc;
@@ -300,17 +335,20 @@ int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<d
     d.tile_cell_start.assign(1, 0);
     // cell order: TX x TY x TZ blocks of the logical grid when the mesh has
     // one, else consecutive chunks of TX TY TZ caller ids
-    const bool logical = m.nx > 0 && m.ny > 0 && m.nz > 0 && (long long)m.nx * m.ny * m.nz == nc;
     const int TX = tile[0], TY = tile[1], TZ = tile[2], TN = TX * TY * TZ;
-    if (logical) {
-        for (int tz = 0; tz < m.nz; tz += TZ)
-            for (int ty = 0; ty < m.ny; ty += TY)
-                for (int tx = 0; tx < m.nx; tx += TX) {
-                    for (int k = tz; k < std::min(tz + TZ, m.nz); k++)
-                        for (int j = ty; j < std::min(ty + TY, m.ny); j++)
-                            for (int i = tx; i < std::min(tx + TX, m.nx); i++) d.perm.push_back(i + j * m.nx + k * m.nx * m.ny);
-                    d.tile_cell_start.push_back((int)d.perm.size());
-                }
+    if (!blocks.empty()) {
+        for (const HostBlock &b : blocks) {
+            const int nx = b.n[0], ny = b.n[1], nz = b.n[2];
+            for (int tz = 0; tz < nz; tz += TZ)
+                for (int ty = 0; ty < ny; ty += TY)
+                    for (int tx = 0; tx < nx; tx += TX) {
+                        for (int k = tz; k < std::min(tz + TZ, nz); k++)
+                            for (int j = ty; j < std::min(ty + TY, ny); j++)
+                                for (int i = tx; i < std::min(tx + TX, nx); i++)
+                                    d.perm.push_back(b.cell0 + i + j * nx + k * nx * ny);
+                        d.tile_cell_start.push_back((int)d.perm.size());
+                    }
+        }
     } else {
         for (int i = 0; i < nc; i++) {
             d.perm.push_back(i);

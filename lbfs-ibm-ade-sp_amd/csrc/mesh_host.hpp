@@ -50,6 +50,16 @@ int compute_geometry(const lbfs_mesh_desc &m, double visc, double gamma, double 
 int compute_ls(const Geometry &g, int grad_type, bool det_compat, std::vector<double> &lhs,
                std::vector<double> &rhs, std::string &err);
 
+// Logical boxes of the mesh: one for a generated box, several for an
+// ensemble; empty when the mesh has no logical numbering.
+struct HostBlock {
+    int n[3];
+    double origin[3];
+    double spacing;
+    int cell0;
+};
+int mesh_blocks(const lbfs_mesh_desc &m, std::vector<HostBlock> &blocks, std::string &err);
+
 // Tile-blocked device layout (all indices are internal, tile-ordered ids).
 struct DeviceLayout {
     int n_cells = 0, n_bc = 0, n_tiles = 0, max_tile_faces = 0;
@@ -75,6 +85,6 @@ struct DeviceLayout {
 };
 int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<double> &lhs,
                  const std::vector<double> &rhs, double rho0, const double body_force[3], int dims, const int tile[3],
-                 DeviceLayout &d, std::string &err);
+                 const std::vector<HostBlock> &blocks, DeviceLayout &d, std::string &err);
 
 }  // namespace lbfs

@@ -17,7 +17,7 @@ BC_DIRICHLET, BC_NEUMANN, BC_PARABOLIC, BC_SHEAR, BC_EMPTY = 1, 2, 9, 10, 11
 TS_LOCAL, TS_CONSTANT, TS_MIN = 1, 2, 3
 MESH_OSAY, MESH_MZ = 0, 1
 FLAG_UPWIND_BAND, FLAG_NEUMANN_SIGNED, FLAG_GHOST_DIR_GPU, FLAG_LS_DET_COMPAT, FLAG_CLOSED_FACES = 1, 2, 4, 16, 32
-IBM_LINEAR2, IBM_COS4 = 0, 1
+IBM_LINEAR2, IBM_COS4, IBM_LAGRANGE4 = 0, 1, 2
 OPT_GRAPH = 1
 
 _dp = C.POINTER(C.c_double)
@@ -32,6 +32,7 @@ class MeshDesc(C.Structure):
         ("bc_rho_type", _ip), ("bc_vel_type", _ip), ("bc_values", _dp),
         ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
         ("origin", C.c_double * 3), ("spacing", C.c_double),
+        ("n_blocks", C.c_int), ("block_dims", _ip), ("block_origin", _dp), ("block_spacing", _dp),
     ]
 
 
@@ -60,7 +61,7 @@ class MembraneDesc(C.Structure):
         ("global_bending_modulus", C.c_double), ("local_bending_modulus", C.c_double),
         ("viscosity", C.c_double), ("curvature_ratio", C.c_double), ("volume_ratio", C.c_double),
         ("sphere_reference", C.c_int), ("mad_compat", C.c_int), ("internal_viscosity_ratio", C.c_double),
-        ("n_springs", C.c_int), ("springs", _ip),
+        ("n_springs", C.c_int), ("springs", _ip), ("block", C.c_int),
     ]
 
 
@@ -326,6 +327,77 @@ class HexMesh:
         return out
 
 
+class Ensemble:
+    """Independent boxes stepped together as one mesh (cells numbered box
+    after box, all internal faces first, then all boundary faces)."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        pts, fv, own, nb, sh, patch = [], [], [], [], [], []
+        bfv, bown, bsh, bpatch, brt, bvt, bval = [], [], [], [], [], [], []
+        p0 = c0 = 0
+        dims, origin, spacing = [], [], []
+        for k, m in enumerate(self.members):
+            ni = m.n_internal
+            f = m.face_vertices.reshape(-1, 4) + p0
+            pts.append(m.points.reshape(-1, 3))
+            fv.append(f[:ni]); bfv.append(f[ni:])
+            own.append(m.owner[:ni] + c0); bown.append(m.owner[ni:] + c0)
+            nb.append(m.neighbour[:ni] + c0)
+            s = m.face_shift.reshape(-1, 3)
+            sh.append(s[:ni]); bsh.append(s[ni:])
+            patch.append(m.patch[:ni]); bpatch.append(m.patch[ni:])
+            brt.append(m.bc_rho_type[: m.n_bc]); bvt.append(m.bc_vel_type[: m.n_bc])
+            bval.append(m.bc_values[: 4 * m.n_bc])
+            dims.append((m.nx, m.ny, m.nz)); origin.append(tuple(m.origin)); spacing.append(m.spacing)
+            p0 += m.n_points
+            c0 += m.n_cells
+        self.points = np.ascontiguousarray(np.concatenate(pts).reshape(-1))
+        self.face_vertices = np.ascontiguousarray(np.concatenate(fv + bfv).reshape(-1), dtype=np.int32)
+        self.owner = np.ascontiguousarray(np.concatenate(own + bown), dtype=np.int32)
+        self.neighbour = np.ascontiguousarray(np.concatenate(nb), dtype=np.int32)
+        self.face_shift = np.ascontiguousarray(np.concatenate(sh + bsh).reshape(-1))
+        self.patch = np.ascontiguousarray(np.concatenate(patch + bpatch), dtype=np.int32)
+        self.n_points = p0
+        self.n_cells = c0
+        self.n_internal = len(self.neighbour)
+        self.n_faces = len(self.owner)
+        self.n_bc = self.n_faces - self.n_internal
+        self.bc_patch = self.patch[self.n_internal:].copy()
+        self.bc_rho_type = np.ascontiguousarray(np.concatenate(brt), dtype=np.int32)
+        self.bc_vel_type = np.ascontiguousarray(np.concatenate(bvt), dtype=np.int32)
+        self.bc_values = np.ascontiguousarray(np.concatenate(bval))
+        self.block_dims = np.array(dims, dtype=np.int32).reshape(-1)
+        self.block_origin = np.array(origin, dtype=np.float64).reshape(-1)
+        self.block_spacing = np.array(spacing, dtype=np.float64)
+        self.cell_start = np.cumsum([0] + [m.n_cells for m in self.members])
+        self.nx = self.ny = self.nz = 0
+        self.origin = np.zeros(3)
+        self.spacing = float(spacing[0])
+
+    def desc(self):
+        d = MeshDesc()
+        d.n_points = self.n_points
+        d.points = ptr(self.points)
+        d.n_cells, d.n_faces, d.n_internal = self.n_cells, self.n_faces, self.n_internal
+        d.face_vertices = ptr(self.face_vertices, _ip)
+        d.owner = ptr(self.owner, _ip)
+        d.neighbour = ptr(self.neighbour, _ip)
+        d.face_shift = ptr(self.face_shift)
+        d.bc_rho_type = ptr(self.bc_rho_type, _ip)
+        d.bc_vel_type = ptr(self.bc_vel_type, _ip)
+        d.bc_values = ptr(self.bc_values)
+        d.n_blocks = len(self.members)
+        d.block_dims = ptr(self.block_dims, _ip)
+        d.block_origin = ptr(self.block_origin)
+        d.block_spacing = ptr(self.block_spacing)
+        return d
+
+    def split(self, arr):
+        """Per-member slices of a per-cell array."""
+        return [arr[self.cell_start[k]: self.cell_start[k + 1]] for k in range(len(self.members))]
+
+
 def icosphere(frequency):
     """Frequency-n geodesic unit sphere: (nodes [N,3], triangles [T,3])."""
     n, t = 10 * frequency * frequency + 2, 20 * frequency * frequency
@@ -420,10 +492,11 @@ def write_membrane_mesh(path, nodes, tri, fmt=MESH_OSAY, springs_=None):
 class Membrane:
     """Host-side description of one membrane (keeps its arrays alive)."""
 
-    def __init__(self, nodes, tri, nodes_stress_free=None, nodes_curvature=None, springs=None, **params):
+    def __init__(self, nodes, tri, nodes_stress_free=None, nodes_curvature=None, springs=None, block=0, **params):
         self.nodes = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1)
         self.tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1)
         self.springs = None if springs is None else np.ascontiguousarray(springs, dtype=np.int32).reshape(-1)
+        self.block = int(block)
         self.sf = None if nodes_stress_free is None else np.ascontiguousarray(nodes_stress_free, dtype=np.float64).reshape(-1)
         self.cv = None if nodes_curvature is None else np.ascontiguousarray(nodes_curvature, dtype=np.float64).reshape(-1)
         p = dict(RBC_PARAMS)
@@ -437,6 +510,7 @@ class Membrane:
         d.nodes_stress_free, d.nodes_curvature = ptr(self.sf), ptr(self.cv)
         if self.springs is not None:
             d.n_springs, d.springs = len(self.springs) // 4, ptr(self.springs, _ip)
+        d.block = self.block
         for k in MEMBRANE_PARAM_NAMES:
             setattr(d, k, self.params[k])
         return d

@@ -98,3 +98,38 @@ def tgv_2d(n=64, jitter=0.05, seed=1, u0=0.01, re=100.0, lattice=abi.D2Q9, dt=0.
     v = u0 * np.sin(k * x) * np.cos(k * y)
     rho = 1.0 - 0.75 * u0 * u0 * (np.cos(2 * k * x) + np.cos(2 * k * y))
     return mesh, params, (rho, u, v, np.zeros_like(u))
+
+
+ENSEMBLE_EDGES = (32, 48, 64, 96)
+
+
+def ensemble(n_instances=64, seed=5000, edges=ENSEMBLE_EDGES, max_rbcs=8, frequency=8, jitter=0.1,
+             lattice=abi.D3Q19, u_max=0.01, params=None):
+    """Ensemble of independent channel boxes stepped as one mesh: instance i
+    has edges drawn from `edges` per axis and 1..max_rbcs cells of 10 f^2 + 2
+    nodes (642 for f = 8), all seeded with seed + i.  Cells sit on slots along
+    the box's x axis, diameter 0.45 of the smaller cross-section edge."""
+    boxes, mems = [], []
+    forces = []
+    for i in range(n_instances):
+        rng = np.random.default_rng(seed + i)
+        nx, ny, nz = (int(e) for e in rng.choice(edges, 3))
+        box = abi.HexMesh(nx, ny, nz, periodic=(1, 0, 0), jitter=jitter, seed=seed + i)
+        boxes.append(box)
+        forces.append(duct_body_force(u_max, ny, nz, CHANNEL_VISC))
+        k = int(rng.integers(1, max_rbcs + 1))
+        d0 = 0.45 * min(ny, nz)
+        k = max(1, min(k, int(nx // (0.45 * d0 + 3))))
+        for c in range(k):
+            rot = abi.rotation((0, 0, 1), rng.uniform(0, 2 * np.pi)) @ abi.rotation((1, 0, 0), rng.uniform(-0.2, 0.2))
+            x, tri = abi.rbc(frequency, d0, rot=rot)
+            centre = np.array([(c + 0.5) * nx / k, 0.5 * ny, 0.5 * nz])
+            mems.append(abi.Membrane(x + centre, tri, block=i, **(params or {})))
+    mesh = abi.Ensemble(boxes)
+    # one body force per box would need a per-cell force; the boxes share the
+    # smallest one (every box is driven, none beyond its duct estimate)
+    f = min(forces)
+    pr = abi.make_params(lattice=lattice, visc=CHANNEL_VISC, dt=CHANNEL_DT, rho0=1.0, body_force=(f, 0.0, 0.0))
+    n = mesh.n_cells
+    state = (np.ones(n), np.zeros(n), np.zeros(n), np.zeros(n))
+    return mesh, pr, state, mems

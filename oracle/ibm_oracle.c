@@ -16,6 +16,18 @@
  * over the 4x4x4 logical stencil and renormalised to sum 1; spreading uses
  * the same weights, F w / V_c, so sum_c f_c V_c = F and interpolation and
  * spreading are adjoint.
+ *
+ * OR_IBM_LAGRANGE4 restates the non-uniform-grid kernels
+ * interpolate_velocities_on_nodes_non_uniform_grid and
+ * spread_forces_on_non_uniform_grid (immersed_boundary_method.cu:640-872):
+ * per axis the vertex nearest the node (find_index_closest :588-636, ties to
+ * the larger index) less 2 gives the first of 4 cells; the weights are the
+ * first column of the inverse Vandermonde matrix of the 4 cell-line centroid
+ * offsets (vandemonde_inverse :919-941), i.e. the cubic Lagrange basis at the
+ * node; spreading F w / V_c.  Extensions: periodic axes wrap the cell index
+ * and shift its line coordinate by the period; on walls the line coordinates
+ * continue with the end spacing and cells outside the grid are dropped (the
+ * reference reads outside its arrays there).
  */
 #include "lbfs_oracle.h"
 
@@ -32,9 +44,68 @@ static int wrap(int i, int n, int periodic, int *ok)
     return i;
 }
 
+/* nearest vertex, ties to the larger index (find_index_closest) */
+static int closest_vertex(const double *v, int n, double t)
+{
+    int u = 0;
+    if (t <= v[0]) return 0;
+    if (t >= v[n - 1]) return n - 1;
+    while (u < n && v[u] < t) u++;
+    return (t - v[u - 1] >= v[u] - t) ? u : u - 1;
+}
+
+/* coordinate of the cell line at (unwrapped) index c along axis a */
+static double line_coord(const or_grid *g, int a, int c, int n, int periodic)
+{
+    const double *l = g->line[a];
+    if (c >= 0 && c < n) return l[c];
+    if (periodic) {
+        int k = c >= 0 ? c / n : -((-c + n - 1) / n);
+        return l[c - k * n] + k * g->L[a];
+    }
+    if (c < 0) return l[0] + c * (n > 1 ? l[1] - l[0] : g->h);
+    return l[n - 1] + (c - (n - 1)) * (n > 1 ? l[n - 1] - l[n - 2] : g->h);
+}
+
+/* Lagrange weights of 4 line points p at x (first column of the inverse
+ * Vandermonde matrix of the offsets p - x) */
+static void lagrange4(const double p[4], double x, double w[4])
+{
+    int i, j;
+    for (i = 0; i < 4; i++) {
+        double v = 1.0;
+        for (j = 0; j < 4; j++)
+            if (j != i) v *= (x - p[j]) / (p[i] - p[j]);
+        w[i] = v;
+    }
+}
+
 /* stencil of node X: up to 64 (cell id, weight) pairs; returns count */
 static int stencil(int kernel, const or_grid *g, const double *X, int *cells, double *wts)
 {
+    if (kernel == OR_IBM_LAGRANGE4) {
+        const int n3[3] = {g->nx, g->ny, g->nz};
+        int base[3], a, i, j, k, cnt = 0;
+        double wa[3][4];
+        for (a = 0; a < 3; a++) {
+            double p[4];
+            base[a] = closest_vertex(g->plateau[a], n3[a] + 1, X[a]) - 2;
+            for (i = 0; i < 4; i++) p[i] = line_coord(g, a, base[a] + i, n3[a], g->periodic[a]);
+            lagrange4(p, X[a], wa[a]);
+        }
+        for (i = 0; i < 4; i++)
+            for (j = 0; j < 4; j++)
+                for (k = 0; k < 4; k++) {
+                    int idx[3] = {base[0] + i, base[1] + j, base[2] + k}, w3[3], ok = 1;
+                    for (a = 0; a < 3; a++) w3[a] = wrap(idx[a], n3[a], g->periodic[a], &ok);
+                    if (!ok) continue;
+                    cells[cnt] = w3[0] + w3[1] * g->nx + w3[2] * g->nx * g->ny;
+                    wts[cnt] = wa[0][i] * wa[1][j] * wa[2][k];
+                    cnt++;
+                }
+        return cnt;
+    }
+    {
     const int n3[3] = {g->nx, g->ny, g->nz};
     int base[3], S = kernel == OR_IBM_LINEAR2 ? 2 : 4, a, i, j, k, cnt = 0;
     double xr[3], wsum = 0.0;
@@ -71,6 +142,7 @@ static int stencil(int kernel, const or_grid *g, const double *X, int *cells, do
     if (kernel == OR_IBM_COS4 && wsum > 0.0)
         for (i = 0; i < cnt; i++) wts[i] /= wsum;
     return cnt;
+    }
 }
 
 int or_ibm_interpolate(int kernel, const or_grid *g, int n_nodes, const double *X, const double *st, double *U)

@@ -105,6 +105,7 @@ void or_residual_rms(int n_cells, const double *res, double ref_rho, double ref_
 /* ---- immersed boundary (immersed_boundary_method.cu) ---- */
 #define OR_IBM_LINEAR2 0  /* 2-point trilinear on the logical grid (immersed_boundary_method.cu:14-97, 947-1024) */
 #define OR_IBM_COS4 1     /* 4-point cosine on physical centroid distances, renormalised */
+#define OR_IBM_LAGRANGE4 2 /* 4-point Lagrange per axis on tensor-product grids (immersed_boundary_method.cu:588-872) */
 /* logical grid: cell id i + j nx + k nx ny, nominal spacing h from `origin`;
  * periodic axes wrap (lengths L); cc / st / vol are in cell-id order */
 typedef struct {
@@ -113,6 +114,11 @@ typedef struct {
     double origin[3], L[3], h;
     const double *cc;   /* [3 n] centroids */
     const double *vol;  /* [n] */
+    /* OR_IBM_LAGRANGE4 only: per axis the vertex coordinates (n_a + 1, the
+     * reference's plateau arrays) and the centroid coordinates of the cells
+     * along the first grid line of that axis (n_a) */
+    const double *plateau[3];
+    const double *line[3];
 } or_grid;
 int or_ibm_interpolate(int kernel, const or_grid *g, int n_nodes, const double *X, const double *st,
                        double *U);

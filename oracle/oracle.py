@@ -43,7 +43,8 @@ OR_BC_DIRICHLET, OR_BC_NEUMANN, OR_BC_PARABOLIC, OR_BC_SHEAR, OR_BC_EMPTY = 1, 2
 
 class OrGrid(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("periodic", C.c_int * 3),
-                ("origin", C.c_double * 3), ("L", C.c_double * 3), ("h", C.c_double), ("cc", _dp), ("vol", _dp)]
+                ("origin", C.c_double * 3), ("L", C.c_double * 3), ("h", C.c_double), ("cc", _dp), ("vol", _dp),
+                ("plateau", _dp * 3), ("line", _dp * 3)]
 
 
 class OrMemParams(C.Structure):
@@ -318,7 +319,7 @@ class OracleCoupled:
     the start-of-step velocity to the nodes and advect them, membrane forces
     at the advected nodes, spread onto the body force, fluid step."""
 
-    def __init__(self, fluid, mesh, membranes, ibm_kernel, body_force=(0.0, 0.0, 0.0)):
+    def __init__(self, fluid, mesh, membranes, ibm_kernel, body_force=(0.0, 0.0, 0.0), planes=None, lines=None):
         self.fluid = fluid
         self.mems = membranes
         self.kernel = ibm_kernel
@@ -332,6 +333,13 @@ class OracleCoupled:
         self.cc = fluid.cc[: 3 * fluid.nc].copy()
         self.vol = fluid.vol.copy()
         g.cc, g.vol = ptr(self.cc), ptr(self.vol)
+        if planes is not None:
+            # vertex planes and cell lines per axis (4-point Lagrange kernel)
+            self.planes = [np.ascontiguousarray(p, dtype=np.float64) for p in planes]
+            self.lines = [np.ascontiguousarray(l, dtype=np.float64) for l in lines]
+            for a in range(3):
+                g.plateau[a] = ptr(self.planes[a])
+                g.line[a] = ptr(self.lines[a])
         self.g = g
         self.body = np.asarray(body_force, dtype=np.float64)
         self.last_force = None

@@ -209,3 +209,69 @@ def test_named_stages_equal_whole_steps():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "groups equal whole steps" in r.stdout
+
+
+def test_ensemble_equals_members_run_alone():
+    """Independent boxes stepped together as one mesh (one launch per stage
+    over all boxes, membranes in their own box) give bit for bit what each box
+    gives run alone."""
+    members = [abi.HexMesh(16, 10, 10, periodic=(1, 0, 0), jitter=0.1, seed=31),
+               abi.HexMesh(20, 12, 10, periodic=(1, 0, 0), jitter=0.1, seed=32),
+               abi.HexMesh(12, 10, 12, periodic=(0, 0, 0), jitter=0.1, seed=33)]
+    cells = {0: [(8.0, 5.0, 5.0)], 1: [(6.0, 6.0, 5.0), (15.0, 6.0, 5.0)], 2: []}
+    body = (2e-5, 0.0, 0.0)
+    states = [cases.smooth_state(m, seed=50 + k, amp=0.004) for k, m in enumerate(members)]
+
+    def mems(k, block):
+        out = []
+        for c in cells[k]:
+            x, t = abi.rbc(3, 5.0, center=c)
+            out.append(abi.Membrane(x, t, block=block, **PARAMS))
+        return out
+
+    ens = abi.Ensemble(members)
+    s = abi.Solver(ens, abi.make_params(visc=VISC, dt=DT, body_force=body))
+    s.set_state(*[np.concatenate([st[v] for st in states]) for v in range(4)])
+    s.add_membranes([mm for k in range(3) for mm in mems(k, k)])
+    s.step(12)
+    ens_state = s.get_state()
+    ens_nodes = s.membrane_nodes()[0]
+    node0 = 0
+    for k, m in enumerate(members):
+        r = abi.Solver(m, abi.make_params(visc=VISC, dt=DT, body_force=body))
+        r.set_state(*states[k])
+        if cells[k]:
+            r.add_membranes(mems(k, 0))
+        r.step(12)
+        for v in range(4):
+            assert np.array_equal(ens.split(ens_state[v])[k], r.get_state()[v]), (k, v)
+        if cells[k]:
+            X = r.membrane_nodes()[0]
+            assert np.array_equal(ens_nodes[node0: node0 + len(X)], X), k
+            node0 += len(X)
+        r.close()
+
+
+def test_lagrange4_coupled_parity():
+    """4-point Lagrange IBM kernel on a stretched tensor-product channel
+    (mesh_type 4 of the reference): 30 coupled steps against the oracle."""
+    from test_oracle_membrane import _stretched_mesh, lagrange_grid_arrays
+    m = _stretched_mesh(24, 16, 16)
+    planes, lines = lagrange_grid_arrays(m)
+    x0, tri = abi.rbc(4, 7.0, center=(12.0, 8.0, 8.0))
+    body = (2e-5, 0.0, 0.0)
+    s = abi.Solver(m, abi.make_params(visc=VISC, dt=DT, body_force=body))
+    st = cases.smooth_state(m, seed=61, amp=0.004)
+    s.set_state(*st)
+    s.add_membranes([abi.Membrane(x0, tri, **PARAMS)], ibm_kernel=abi.IBM_LAGRANGE4)
+    fl = orc.OracleFluid(m, orc.OR_D3Q19, visc=VISC, dt=DT)
+    fl.set_state(*st)
+    mo = orc.OracleMembrane(x0, tri, PARAMS)
+    cp = orc.OracleCoupled(fl, m, [mo], abi.IBM_LAGRANGE4, body_force=body, planes=planes, lines=lines)
+    s.step(30)
+    cp.step(30)
+    X, U, F = s.membrane_nodes()
+    assert cases.rel_l2(X - x0, mo.X.reshape(-1, 3) - x0) < 1e-4
+    _, us, vs, ws = s.get_state()
+    _, uo, vo, wo = cp.fluid.state()
+    assert cases.rel_l2(np.concatenate([us, vs, ws]), np.concatenate([uo, vo, wo])) < 1e-5

@@ -346,3 +346,32 @@ def test_local_time_stepping_reaches_the_same_steady_channel():
         out.append(o.state()[1])
     assert np.all(out[0] > 0)
     assert cases.rel_l2(out[1], out[0]) < 1e-4 and cases.rel_l2(out[2], out[0]) < 1e-4
+
+
+def _members():
+    return [abi.HexMesh(12, 8, 8, periodic=(1, 0, 0), jitter=0.1, seed=31),
+            abi.HexMesh(16, 10, 8, periodic=(1, 0, 0), jitter=0.1, seed=32),
+            abi.HexMesh(10, 8, 12, periodic=(0, 0, 0), jitter=0.1, seed=33)]
+
+
+def test_ensemble_mesh_is_the_disjoint_union():
+    """An ensemble (boxes numbered box after box, internal faces first) is
+    the disjoint union of its members: the FP64 oracle stepping the ensemble
+    gives each member's own trajectory bit for bit."""
+    members = _members()
+    ens = abi.Ensemble(members)
+    assert ens.n_cells == sum(m.n_cells for m in members)
+    assert ens.n_internal == sum(m.n_internal for m in members)
+    assert np.all(ens.owner[: ens.n_internal] < np.repeat(ens.cell_start[1:], [m.n_internal for m in members]))
+    body = np.array([1e-5, 0.0, 0.0])
+    states = [cases.smooth_state(m, seed=40 + k) for k, m in enumerate(members)]
+    o = orc.OracleFluid(ens, orc.OR_D3Q19, visc=1 / 6, dt=0.25)
+    o.set_state(*[np.concatenate([s[v] for s in states]) for v in range(4)])
+    o.step(15, force=np.tile(body, ens.n_cells))
+    out = o.state()
+    for k, m in enumerate(members):
+        om = orc.OracleFluid(m, orc.OR_D3Q19, visc=1 / 6, dt=0.25)
+        om.set_state(*states[k])
+        om.step(15, force=np.tile(body, m.n_cells))
+        for v in range(4):
+            assert np.array_equal(ens.split(out[v])[k], om.state()[v])

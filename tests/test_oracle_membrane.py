@@ -7,6 +7,7 @@ state, forces equal to minus the gradient of the energies they derive from
 bending term, and partition of unity / conservation / adjointness of the IBM
 weights.
 """
+import ctypes as C
 import numpy as np
 import pytest
 
@@ -279,3 +280,63 @@ def test_energies_vanish_at_rest_and_spring_energy_matches_wlc_force():
     exp_la = params["global_area_modulus"] * 0.5 * np.sum((sc * sc - 1) ** 2 * n0)
     assert abs(e[1] - exp_la) < 1e-12 * exp_la
     assert abs(e[6] - e[:6].sum()) < 1e-15
+
+
+def _stretched_mesh(nx=20, ny=16, nz=16, amp=0.25, periodic=(1, 0, 0), seed=1):
+    """Tensor-product stretched box: vertex coordinates x -> x + amp L/(2 pi) sin(2 pi x / L)
+    per axis (no jitter), so the vertex planes and cell lines are exact."""
+    m = abi.HexMesh(nx, ny, nz, periodic=periodic, jitter=0.0, seed=seed)
+    P = m.points.reshape(-1, 3)
+    for a, n in enumerate((nx, ny, nz)):
+        L = float(n)
+        P[:, a] = P[:, a] + amp * L / (2 * np.pi) * np.sin(2 * np.pi * P[:, a] / L)
+    return m
+
+
+def lagrange_grid_arrays(m):
+    """Vertex planes and first-line cell centroids per axis from the points and the oracle geometry."""
+    P = m.points.reshape(-1, 3)
+    planes = [np.unique(np.round(P[:, a], 12)) for a in range(3)]
+    cc = orc.OracleFluid(m, orc.OR_D3Q19, visc=1 / 6, dt=0.25).cc.reshape(-1, 3)
+    nx, ny = m.nx, m.ny
+    lines = [cc[np.arange(m.nx), 0], cc[np.arange(m.ny) * nx, 1], cc[np.arange(m.nz) * nx * ny, 2]]
+    return planes, lines
+
+
+def test_lagrange4_reproduces_cubics_on_stretched_grid():
+    """OR_IBM_LAGRANGE4 (immersed_boundary_method.cu:588-872): on a
+    tensor-product stretched grid the weights are the cubic Lagrange basis of
+    the 4 nearest cell lines per axis, so any field cubic in each coordinate
+    is interpolated exactly, the weights sum to 1 and reproduce the node
+    position; spreading conserves momentum (sum f V = F)."""
+    m = _stretched_mesh()
+    planes, lines = lagrange_grid_arrays(m)
+    assert [len(p) for p in planes] == [m.nx + 1, m.ny + 1, m.nz + 1]
+    fl = orc.OracleFluid(m, orc.OR_D3Q19, visc=1 / 6, dt=0.25)
+    cc = fl.cc.reshape(-1, 3)[: m.n_cells]
+    cp = orc.OracleCoupled(fl, m, [], abi.IBM_LAGRANGE4, planes=planes, lines=lines)
+    rng = np.random.default_rng(4)
+    X = np.column_stack([rng.uniform(3, m.nx - 3, 40), rng.uniform(3, m.ny - 3, 40), rng.uniform(3, m.nz - 3, 40)])
+    def field(p):
+        x, y, z = p[:, 0], p[:, 1], p[:, 2]
+        return 0.01 + 1e-4 * x ** 3 - 2e-4 * x * y ** 2 + 3e-5 * z ** 3 * y - 1e-3 * z
+    n = m.n_cells
+    st = np.zeros(4 * n)
+    st[1::4] = field(cc)
+    U = np.zeros(3 * len(X))
+    orc.load().or_ibm_interpolate(abi.IBM_LAGRANGE4, C.byref(cp.g), len(X), orc.ptr(np.ascontiguousarray(X.reshape(-1))),
+                                  orc.ptr(st), orc.ptr(U))
+    np.testing.assert_allclose(U[0::3], field(X), rtol=0, atol=1e-12)
+    for comp, vals in ((2, np.ones(n)), (3, cc[:, 0])):
+        st2 = np.zeros(4 * n)
+        st2[comp::4] = vals
+        U2 = np.zeros(3 * len(X))
+        orc.load().or_ibm_interpolate(abi.IBM_LAGRANGE4, C.byref(cp.g), len(X),
+                                      orc.ptr(np.ascontiguousarray(X.reshape(-1))), orc.ptr(st2), orc.ptr(U2))
+        expect = np.ones(len(X)) if comp == 2 else X[:, 0]
+        np.testing.assert_allclose(U2[comp - 1::3], expect, rtol=0, atol=1e-12)
+    F = rng.standard_normal(X.shape)
+    f = np.zeros(3 * n)
+    orc.load().or_ibm_spread(abi.IBM_LAGRANGE4, C.byref(cp.g), len(X), orc.ptr(np.ascontiguousarray(X.reshape(-1))),
+                             orc.ptr(np.ascontiguousarray(F.reshape(-1))), orc.ptr(f))
+    np.testing.assert_allclose((f.reshape(-1, 3) * fl.vol[:, None]).sum(0), F.sum(0), atol=1e-12)
