@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="also time the 64-instance ensemble (edges from {32,48,64,96}, 1-8 cells of 642 nodes each)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU design is documented, not built); "
@@ -154,6 +156,28 @@ def main():
                    "mlups": m3.n_cells / (ms3 * 1e-3) / 1e6, "kernels_ms": k3}
         log("suspension step %.4f ms (plain %.4f ms)" % (ms3, ms3_plain))
 
+    ensemble = None
+    if args.ensemble:
+        log("building the 64-instance ensemble")
+        m5, p5, s5, mems5 = workloads.ensemble()
+        p5.tile[0], p5.tile[1], p5.tile[2] = params.tile[0], params.tile[1], params.tile[2]
+        p5.block_threads = params.block_threads
+        e_plain = abi.Solver(m5, p5)
+        e_plain.set_state(*s5)
+        ms5_plain = timed(e_plain, max(args.steps // 4, 10), 5)
+        e_plain.close()
+        e = abi.Solver(m5, p5)
+        e.set_state(*s5)
+        e.add_membranes(mems5, ibm_kernel=abi.IBM_COS4)
+        ms5 = timed(e, max(args.steps // 4, 10), 5)
+        k5 = e.time_kernels(10)
+        e.close()
+        ensemble = {"workload": "64 independent jittered channel boxes (edges from {32,48,64,96}, seeds 5000+i), "
+                                "%d cells of 642 nodes, D3Q19, one mesh" % len(mems5),
+                    "cells": m5.n_cells, "ms_per_step": ms5, "plain_ms_per_step": ms5_plain,
+                    "ratio_vs_plain": ms5 / ms5_plain, "mlups": m5.n_cells / (ms5 * 1e-3) / 1e6, "kernels_ms": k5}
+        log("ensemble step %.4f ms (plain %.4f ms)" % (ms5, ms5_plain))
+
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (FP64 oracle, %d steps)" % args.cpu_steps)
@@ -193,6 +217,7 @@ def main():
         "max_speed": diag["max_speed"],
         "membrane_volume_rel_change": mglob[0] / mglob[3] - 1.0,
         "batched": batched,
+        "ensemble": ensemble,
         "adjoint": None,
         "cpu_baseline": cpu,
     }
