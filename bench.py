@@ -85,6 +85,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tile", type=int, nargs=3, default=None,
+                    help="fixed tile shape of the flux/update kernel (skips the autotuner)")
+    ap.add_argument("--threads", type=int, default=0, help="workgroup size with --tile (256 or 512)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three 1 GiB device-to-device copies (the reference for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
     ap.add_argument("--ensemble", action="store_true",
                     help="also time the 64-instance ensemble (edges from {32,48,64,96}, 1-8 cells of 642 nodes each)")
     args = ap.parse_args()
@@ -92,6 +98,16 @@ def main():
         print("bench.py: this build runs on one GPU only (the multi-GPU design is documented, not built); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 1 << 30, "copies": 3}))
+        return
 
     from lbfs_amd import abi, workloads
     log = lambda m: print("bench: " + m, file=sys.stderr, flush=True)  # noqa: E731
@@ -101,7 +117,10 @@ def main():
     mesh, params, state = workloads.channel()
     n = mesh.n_cells
     tuning = None
-    if not args.no_autotune:
+    if args.tile:
+        from lbfs_amd import autotune
+        params = autotune.with_tile(params, tuple(args.tile), args.threads or 256)
+    elif not args.no_autotune:
         # tile shape and workgroup size of the flux/update kernel (results do
         # not depend on them); tuned on the plain step, outside the timed runs
         from lbfs_amd import autotune

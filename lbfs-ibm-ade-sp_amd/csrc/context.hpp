@@ -17,7 +17,8 @@ struct lbfs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;       // second stream: the gradient pass beside the coupling chain
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t side2 = nullptr;      // third stream: membrane forces beside the node binning
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr;
     lbfs_params params{};
     std::vector<lbfs::HostBlock> blocks;   // logical boxes of the mesh (empty: none)
 

@@ -320,6 +320,7 @@ __global__ __launch_bounds__(256) void k_ibm_spread(MemArgs m, IbmArgs g)
     __shared__ int cnd[64];
     __shared__ int4 cbase[64];
     __shared__ float4 cF[64];
+    __shared__ float cw[64][64];   // the chunk's cached stencil weights, one row per node
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int q = tid; q < 3 * 343; q += 256) racc[q / 343][q % 343] = 0ull;
     const int S = stencil_size(g);
@@ -336,9 +337,11 @@ __global__ __launch_bounds__(256) void k_ibm_spread(MemArgs m, IbmArgs g)
             cF[tid] = m.F[p];
         }
         __syncthreads();
+        // all weight rows of the chunk in flight at once (one coalesced row per node)
+        for (int q = tid; q < 64 * cnt; q += 256) cw[q >> 6][q & 63] = g.wnode[64 * (size_t)cnd[q >> 6] + (q & 63)];
+        __syncthreads();
         for (int jn = wave; jn < cnt; jn += 4) {
-            const int p = cnd[jn];
-            const float wt = g.wnode[64 * (size_t)p + lane];
+            const float wt = cw[jn][lane];
             if (!act || wt == 0.0f) continue;
             const int4 nb = cbase[jn];
             const int bs[3] = {nb.x, nb.y, nb.z};

@@ -225,8 +225,11 @@ extern "C" int lbfs_create(const lbfs_mesh_desc *mesh, const lbfs_params *params
         return LBFS_E_HIP;
     }
     if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->side2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork2, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join2, hipEventDisableTiming) != hipSuccess) {
         g_create_error = "could not create a HIP stream or event";
         lbfs_destroy(ctx);
         return LBFS_E_HIP;
@@ -255,9 +258,13 @@ extern "C" int lbfs_destroy(lbfs_ctx *ctx)
     for (void *p : ctx->allocs) hipFree(p);
     ctx->allocs.clear();
     if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+    if (ctx->side2) (void)hipStreamSynchronize(ctx->side2);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->ev_fork2) (void)hipEventDestroy(ctx->ev_fork2);
+    if (ctx->ev_join2) (void)hipEventDestroy(ctx->ev_join2);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
+    if (ctx->side2) (void)hipStreamDestroy(ctx->side2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return LBFS_OK;

@@ -106,16 +106,31 @@ int lbfs_ctx::run_stages(bool with_visc)
     // (a latency-bound sequence of small kernels); the flux/update waits for
     // both.  Same kernels, same inputs, so the results are those of the
     // sequential order.
+    // After interpolation the membrane forces (globals, edges, nodes) and the
+    // node binning (scan, place) are independent until spreading, so the
+    // membrane chain runs on a third stream.
     HIPCHK(hipEventRecord(ev_fork, stream));
     HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
     HIPCHK(launch_gradients(args, st[cur], grad, side));
     HIPCHK(hipEventRecord(ev_join, side));
     for (auto &s : step_stages(with_visc)) {
-        if (std::strcmp(s.name, "gradients") == 0) continue;
-        if (std::strcmp(s.name, "flux_update") == 0 || std::strcmp(s.name, "dt_min") == 0)
+        const char *nm = s.name;
+        if (std::strcmp(nm, "gradients") == 0 || std::strcmp(nm, "mem_globals") == 0 ||
+            std::strcmp(nm, "mem_edges") == 0 || std::strcmp(nm, "mem_nodes") == 0)
+            continue;
+        if (std::strcmp(nm, "ibm_spread") == 0) HIPCHK(hipStreamWaitEvent(stream, ev_join2, 0));
+        if (std::strcmp(nm, "flux_update") == 0 || std::strcmp(nm, "dt_min") == 0)
             HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
         int rc = s.run();
         if (rc) return rc;
+        if (std::strcmp(nm, "ibm_interp") == 0) {
+            HIPCHK(hipEventRecord(ev_fork2, stream));
+            HIPCHK(hipStreamWaitEvent(side2, ev_fork2, 0));
+            HIPCHK(launch_mem_globals(mem, side2));
+            HIPCHK(launch_mem_edges(mem, side2));
+            HIPCHK(launch_mem_nodes(mem, side2));
+            HIPCHK(hipEventRecord(ev_join2, side2));
+        }
     }
     return LBFS_OK;
 }
