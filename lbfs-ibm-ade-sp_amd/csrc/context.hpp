@@ -41,6 +41,7 @@ struct lbfs_ctx {
     lbfs::IbmArgs ibm{};
     std::vector<int> mem_node_start;
     double *mem_energy = nullptr;   // [7 n_mem] output of lbfs_get_membrane_energies
+    int ibm_binned_option = -1;     // LBFS_OPT_IBM_BINNED: -1 by node count, 0 per node, 1 per bin
     std::vector<double> mem_origin;
     std::vector<void *> coupling_allocs;
     float *visc_factor = nullptr;   // [n] internal viscosity field, null while every ratio is 1

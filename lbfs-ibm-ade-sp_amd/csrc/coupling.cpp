@@ -162,6 +162,32 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     if ((rc = cp_alloc(ctx, &dns, mem_node_start.data(), mem_node_start.size()))) return rc;
     if ((rc = cp_alloc<float4>(ctx, &dbb, nullptr, 2 * (size_t)count))) return rc;
     if ((rc = cp_alloc<double>(ctx, &ctx->mem_energy, nullptr, 7 * (size_t)count))) return rc;
+    // chunks of about 256 triangles for the two-level volume/area/MAD sums
+    {
+        std::vector<int> chunk, mcs(1, 0);
+        for (int i = 0; i < count; i++) {
+            const int t0 = mem_tri_start[i], nt = mem_tri_start[i + 1] - t0;
+            const int e0 = mem_edge_start[i], nm = params[i].n_mad;
+            const int nch = std::max(1, (nt + 255) / 256);
+            for (int c = 0; c < nch; c++) {
+                chunk.push_back(i);
+                chunk.push_back(t0 + (int)((long long)nt * c / nch));
+                chunk.push_back(t0 + (int)((long long)nt * (c + 1) / nch));
+                chunk.push_back(e0 + (int)((long long)nm * c / nch));
+                chunk.push_back(e0 + (int)((long long)nm * (c + 1) / nch));
+            }
+            mcs.push_back(mcs.back() + nch);
+        }
+        int *dch, *dmcs;
+        double *dpart;
+        if ((rc = cp_alloc(ctx, &dch, chunk.data(), chunk.size()))) return rc;
+        if ((rc = cp_alloc(ctx, &dmcs, mcs.data(), mcs.size()))) return rc;
+        if ((rc = cp_alloc<double>(ctx, &dpart, nullptr, 3 * (size_t)mcs.back()))) return rc;
+        m.n_chunks = mcs.back();
+        m.chunk = dch;
+        m.mem_chunk_start = dmcs;
+        m.part = dpart;
+    }
     m.mem_node_start = dns;
     m.bbox = dbb;
     m.X = dX; m.U = dU; m.F = dF; m.node_mem = dnm; m.origin = dorig; m.tri = dtri;
@@ -243,6 +269,10 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     g.axis = dax;
     g.n_blocks = nblk;
     g.n_bins = (int)bin_block.size();
+    // few nodes: spread per node straight into the cell accumulators (fewer,
+    // shorter launches); many nodes: bin them and spread per bin through LDS,
+    // which keeps the global atomic traffic bounded
+    g.binned = ctx->ibm_binned_option >= 0 ? ctx->ibm_binned_option : (nn > 65536 ? 1 : 0);
     g.dt = (float)ctx->params.dt;
     std::vector<int> node_block(nn), mem_block(count);
     for (int i = 0; i < count; i++) {

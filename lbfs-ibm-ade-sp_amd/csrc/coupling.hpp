@@ -36,6 +36,10 @@ struct MemArgs {
     double *glob;             // 8 per membrane: V, A, MAD, V0, A0, MAD0, A_prev, 0
     float4 *bbox;             // 2 per membrane: lower and upper corner of the nodes
     const int *mem_block;     // box of each membrane
+    int n_chunks;             // partial-sum chunks of the volume/area/MAD sums
+    const int *chunk;         // 5 per chunk: membrane, triangle range, MAD edge range
+    const int *mem_chunk_start;
+    double *part;             // 3 per chunk: V, A, MAD partial sums
 };
 
 // one logical box of cells (an ensemble member, or the whole mesh)
@@ -52,6 +56,7 @@ struct IbmArgs {
     int kernel;               // LBFS_IBM_*
     int n_cells;
     int n_blocks, n_bins;
+    int binned;               // 1: nodes binned and spread per bin; 0: spread per node with global atomics
     const IbmBlock *blk;      // [n_blocks]
     const int *bin_block;     // [n_bins] box of each bin
     const int *node_block;    // [n_nodes] box of each node

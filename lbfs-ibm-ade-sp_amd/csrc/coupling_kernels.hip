@@ -190,7 +190,7 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
     const bool act = lane < S * S * S;
     int off[3];
     lane_offset(S, lane, off);
-    const IbmBlock &B = g.blk[g.node_block[nd]];
+    const IbmBlock B = g.blk[g.node_block[nd]];
     const float4 o = m.origin[m.node_mem[nd]];
     float4 xl = m.X[nd];
     if (INTERP) {
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
         const int key = B.bin0 + bin_coord(B, 0, base[0]) +
                         B.tn[0] * (bin_coord(B, 1, base[1]) + B.tn[1] * bin_coord(B, 2, base[2]));
         g.nbase[nd] = make_int4(base[0], base[1], base[2], key);
-        if (INTERP) {
+        if (INTERP && g.binned) {
             g.bin_key[nd] = key;
             g.bin_rank[nd] = atomicAdd(&g.bin_count[key], 1);
         }
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(256) void k_ibm_spread(MemArgs m, IbmArgs g)
     const int b = blockIdx.x;
     const int lo = g.bin_start[b], hi = g.bin_start[b + 1];
     if (lo == hi) return;  // uniform over the workgroup
-    const IbmBlock &B = g.blk[g.bin_block[b]];
+    const IbmBlock B = g.blk[g.bin_block[b]];
     const int lb = b - B.bin0;
     const int bo[3] = {4 * (lb % B.tn[0]), 4 * ((lb / B.tn[0]) % B.tn[1]), 4 * (lb / (B.tn[0] * B.tn[1]))};
     __shared__ unsigned long long racc[3][343];
@@ -399,6 +399,35 @@ __global__ __launch_bounds__(256) void k_ibm_spread(MemArgs m, IbmArgs g)
     }
 }
 
+// Spreading for small node counts: one wavefront per node, one lane per
+// stencil slot, adding its weighted force straight into the cell's 64-bit
+// fixed-point accumulators (integer additions: order independent).  No node
+// binning is needed.
+__global__ __launch_bounds__(256) void k_ibm_spread_nodes(MemArgs m, IbmArgs g)
+{
+    const int lane = threadIdx.x & 63;
+    const int nd = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (nd >= m.n_nodes) return;
+    const int S = stencil_size(g);
+    if (lane >= S * S * S) return;
+    const float wt = g.wnode[64 * (size_t)nd + lane];
+    if (wt == 0.0f) return;
+    const IbmBlock B = g.blk[g.node_block[nd]];
+    const int4 nb = g.nbase[nd];
+    int off[3];
+    lane_offset(S, lane, off);
+    const int idx[3] = {nb.x + off[0], nb.y + off[1], nb.z + off[2]};
+    int w[3];
+    if (!(wrap_index(B, 0, idx[0], w[0]) && wrap_index(B, 1, idx[1], w[1]) && wrap_index(B, 2, idx[2], w[2]))) return;
+    const float4 F = m.F[nd];
+    const int ic = cell_internal(g, B, w);
+    unsigned long long *acc = g.acc + 3 * (size_t)ic;
+    atomicAdd(acc, (unsigned long long)llrint((double)(F.x * wt) * FIX_SCALE));
+    atomicAdd(acc + 1, (unsigned long long)llrint((double)(F.y * wt) * FIX_SCALE));
+    atomicAdd(acc + 2, (unsigned long long)llrint((double)(F.z * wt) * FIX_SCALE));
+    g.tile_touched[B.bin0 + (w[0] >> 2) + B.tn[0] * ((w[1] >> 2) + B.tn[1] * (w[2] >> 2))] = 1;
+}
+
 // per 4x4x4 tile: force density of the touched cells from their accumulators
 // (which are cleared), static force back where IBM force was held last step
 __global__ __launch_bounds__(64) void k_ibm_finalize(IbmArgs g)
@@ -406,7 +435,7 @@ __global__ __launch_bounds__(64) void k_ibm_finalize(IbmArgs g)
     const int t = blockIdx.x;
     const int touched = g.tile_touched[t], flag = g.tile_flag[t];
     if (!touched && !flag) return;
-    const IbmBlock &B = g.blk[g.bin_block[t]];
+    const IbmBlock B = g.blk[g.bin_block[t]];
     const int lt = t - B.bin0;
     const int tc[3] = {lt % B.tn[0], (lt / B.tn[0]) % B.tn[1], lt / (B.tn[0] * B.tn[1])};
     const int lane = threadIdx.x;
@@ -434,21 +463,23 @@ __global__ __launch_bounds__(64) void k_ibm_finalize(IbmArgs g)
 // ---------------------------------------------------------------------------
 // membrane
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_mem_globals(MemArgs m)
+// volume, area and MAD sums in two levels: partial sums over chunks of about
+// 256 triangles (and the matching share of the MAD edges), then one
+// wavefront per membrane adds its chunks' partials; both in a fixed order
+__global__ __launch_bounds__(256) void k_mem_partials(MemArgs m)
 {
-    const int mem = blockIdx.x;
-    __shared__ double sh[3][1024];
+    const int *q = m.chunk + 5 * (size_t)blockIdx.x;
+    const int t0 = q[1], t1 = q[2], e0 = q[3], e1 = q[4];
+    __shared__ double sh[3][256];
     double V = 0.0, A = 0.0, M = 0.0;
-    for (int t = m.mem_tri_start[mem] + threadIdx.x; t < m.mem_tri_start[mem + 1]; t += blockDim.x) {
-        const int4 q = m.tri[t];
-        const f3 a = ld3(m.X[q.x]), b = ld3(m.X[q.y]), c = ld3(m.X[q.z]);
+    for (int t = t0 + threadIdx.x; t < t1; t += 256) {
+        const int4 qq = m.tri[t];
+        const f3 a = ld3(m.X[qq.x]), b = ld3(m.X[qq.y]), c = ld3(m.X[qq.z]);
         const f3 N = cross(b - a, c - a);
         A += 0.5 * (double)sqrtf(dot(N, N));
         V += (double)dot(N, (a + b + c) * (1.0f / 3.0f)) / 6.0;
     }
-    const int e0 = m.mem_edge_start[mem];
-    const int e1 = e0 + m.params[mem].n_mad;
-    for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    for (int e = e0 + threadIdx.x; e < e1; e += 256) {
         const float4 s = m.edge_state[e];
         M += (double)s.x * ((double)s.y + (double)s.z) / 3.0;
     }
@@ -456,17 +487,35 @@ __global__ __launch_bounds__(1024) void k_mem_globals(MemArgs m)
     sh[1][threadIdx.x] = A;
     sh[2][threadIdx.x] = M;
     __syncthreads();
-    for (int stp = 512; stp > 0; stp >>= 1) {
+    for (int stp = 128; stp > 0; stp >>= 1) {
         if ((int)threadIdx.x < stp)
-            for (int q = 0; q < 3; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + stp];
+            for (int k = 0; k < 3; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + stp];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 3) m.part[3 * (size_t)blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(64) void k_mem_globals(MemArgs m)
+{
+    const int mem = blockIdx.x, lane = threadIdx.x;
+    double V = 0.0, A = 0.0, M = 0.0;
+    for (int c = m.mem_chunk_start[mem] + lane; c < m.mem_chunk_start[mem + 1]; c += 64) {
+        V += m.part[3 * (size_t)c];
+        A += m.part[3 * (size_t)c + 1];
+        M += m.part[3 * (size_t)c + 2];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        V += __shfl_xor(V, off, 64);
+        A += __shfl_xor(A, off, 64);
+        M += __shfl_xor(M, off, 64);
+    }
+    if (lane == 0) {
         double *gl = m.glob + 8 * (size_t)mem;
-        gl[2] = sh[2][0] / gl[6];  // MAD of the previous call, over the area it was computed with
-        gl[0] = sh[0][0];
-        gl[1] = sh[1][0];
-        gl[6] = sh[1][0];
+        gl[2] = M / gl[6];  // MAD of the previous call, over the area it was computed with
+        gl[0] = V;
+        gl[1] = A;
+        gl[6] = A;
     }
 }
 
@@ -648,7 +697,8 @@ hipError_t launch_ibm_place(const MemArgs &m, const IbmArgs &g, hipStream_t s)
 }
 hipError_t launch_mem_globals(const MemArgs &m, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mem_globals, dim3(m.n_mem), dim3(1024), 0, s, m);
+    hipLaunchKernelGGL(k_mem_partials, dim3(m.n_chunks), dim3(256), 0, s, m);
+    hipLaunchKernelGGL(k_mem_globals, dim3(m.n_mem), dim3(64), 0, s, m);
     return hipGetLastError();
 }
 hipError_t launch_mem_edges(const MemArgs &m, hipStream_t s)
@@ -663,7 +713,10 @@ hipError_t launch_mem_nodes(const MemArgs &m, hipStream_t s)
 }
 hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ibm_spread, dim3(g.n_bins), dim3(256), 0, s, m, g);
+    if (g.binned)
+        hipLaunchKernelGGL(k_ibm_spread, dim3(g.n_bins), dim3(256), 0, s, m, g);
+    else
+        hipLaunchKernelGGL(k_ibm_spread_nodes, dim3((m.n_nodes + 3) / 4), dim3(256), 0, s, m, g);
     hipLaunchKernelGGL(k_ibm_finalize, dim3(g.n_bins), dim3(64), 0, s, g);
     return hipGetLastError();
 }

@@ -30,11 +30,13 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
                          return LBFS_OK;
                      },
                      60.0 * nn});
+        if (ibm.binned)
         s.push_back({"ibm_scan", [ctx]() -> int {
                          HIPCHK(launch_ibm_scan(ctx->ibm, ctx->stream));
                          return LBFS_OK;
                      },
                      8.0 * ibm.n_bins});
+        if (ibm.binned)
         s.push_back({"ibm_place", [ctx]() -> int {
                          HIPCHK(launch_ibm_place(ctx->mem, ctx->ibm, ctx->stream));
                          return LBFS_OK;

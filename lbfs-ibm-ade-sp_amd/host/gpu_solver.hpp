@@ -86,8 +86,8 @@ public:
     // spread_forces_on_structured_grid
     void spread_forces_on_structured_grid()
     {
-        stage("ibm_scan");
-        stage("ibm_place");
+        stage_if_present("ibm_scan");   // node binning, used for large node counts
+        stage_if_present("ibm_place");
         stage("ibm_spread");
     }
     // get_interior_gradients + get_bc_gradients

@@ -18,7 +18,7 @@ TS_LOCAL, TS_CONSTANT, TS_MIN = 1, 2, 3
 MESH_OSAY, MESH_MZ = 0, 1
 FLAG_UPWIND_BAND, FLAG_NEUMANN_SIGNED, FLAG_GHOST_DIR_GPU, FLAG_LS_DET_COMPAT, FLAG_CLOSED_FACES = 1, 2, 4, 16, 32
 IBM_LINEAR2, IBM_COS4, IBM_LAGRANGE4 = 0, 1, 2
-OPT_GRAPH = 1
+OPT_GRAPH, OPT_IBM_BINNED = 1, 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -604,6 +604,10 @@ class Solver:
         g = np.zeros(6 * len(self._membranes))
         check(load().lbfs_get_membrane_globals(self.ctx, ptr(g)), self.ctx)
         return g.reshape(-1, 6)
+
+    def set_ibm_binned(self, mode):
+        """Spreading path of the next add_membranes: -1 by node count, 0 per node, 1 per bin."""
+        check(load().lbfs_set_option(self.ctx, OPT_IBM_BINNED, int(mode)), self.ctx)
 
     def stage_names(self):
         names = (C.c_char_p * 32)()
