@@ -121,6 +121,13 @@ __device__ __forceinline__ void eacc(float &s, float v)
 
 // equilibrium deviation g = w (drho + rho P) of one lattice point with state
 // ph = (drho, u, v, w): P = 3 e.u + 4.5 (e.u)^2 / gamma - 1.5 u.u / gamma
+__device__ __forceinline__ float eq_point_eu(const float ph[4], float eu, float w, float rho0, float ig)
+{
+    const float uu = ph[1] * ph[1] + ph[2] * ph[2] + ph[3] * ph[3];
+    const float P = eu * (3.0f + 4.5f * ig * eu) - 1.5f * ig * uu;
+    return w * (ph[0] + (rho0 + ph[0]) * P);
+}
+
 template <int EX, int EY, int EZ>
 __device__ __forceinline__ float eq_point(const float ph[4], float w, float rho0, float ig)
 {
@@ -128,6 +135,27 @@ __device__ __forceinline__ float eq_point(const float ph[4], float w, float rho0
     const float uu = ph[1] * ph[1] + ph[2] * ph[2] + ph[3] * ph[3];
     const float P = eu * (3.0f + 4.5f * ig * eu) - 1.5f * ig * uu;
     return w * (ph[0] + (rho0 + ph[0]) * P);
+}
+
+// Strict upwinding with the selects (ties, e.n = 0, go to the owner):
+// outlined so that the common tie-free path carries none of its work.
+template <int EX, int EY, int EZ>
+__device__ __attribute__((noinline)) float2 pair_ties(float en, float po0, float po1, float po2, float po3, float pn0,
+                                                      float pn1, float pn2, float pn3, float eo0, float eo1, float eo2,
+                                                      float eo3, float en0, float en1, float en2, float en3, float w,
+                                                      float rho0, float ig)
+{
+    const float po[4] = {po0, po1, po2, po3}, pn[4] = {pn0, pn1, pn2, pn3};
+    const float ego[4] = {eo0, eo1, eo2, eo3}, egn[4] = {en0, en1, en2, en3};
+    float pk[4], pb[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        pk[v] = en < 0.0f ? pn[v] - egn[v] : po[v] - ego[v];
+        pb[v] = en > 0.0f ? pn[v] + egn[v] : po[v] + ego[v];
+    }
+    const float gk = eq_point<EX, EY, EZ>(pk, w, rho0, ig);
+    const float gb = eq_point<-EX, -EY, -EZ>(pb, w, rho0, ig);
+    return make_float2(gk + gb, gk - gb);
 }
 
 struct FaceSums {
@@ -141,7 +169,44 @@ __device__ __forceinline__ void pair_step(FaceSums &f, const float po[4], const 
                                           float bw, bool ghost, float rho0, float ig)
 {
     constexpr int ex = L::ex[K], ey = L::ey[K], ez = L::ez[K];
+    static_assert(L::w(K) == L::w(K + 1), "opposite directions carry equal weights");
     const float en = edot<ex, ey, ez>(nx, ny, nz);
+    if constexpr (!BAND) {
+        float ego[4], egn[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            ego[v] = edot<ex, ey, ez>(go[3 * v], go[3 * v + 1], go[3 * v + 2]);
+            egn[v] = edot<ex, ey, ez>(gn[3 * v], gn[3 * v + 1], gn[3 * v + 2]);
+        }
+        float gs, gd;
+        if (__builtin_expect(__ballot(en == 0.0f) == 0, 1)) {
+            // No tie in the wavefront: the owner side feeds the point that
+            // moves along s e and the neighbour side the one along -s e,
+            // s = sign(e.n): the values of the strict rule without selects
+            const float sg = en > 0.0f ? 1.0f : -1.0f;
+            float qo[4], qn[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                qo[v] = fmaf(-sg, ego[v], po[v]);
+                qn[v] = fmaf(sg, egn[v], pn[v]);
+            }
+            const float g_o = eq_point_eu(qo, sg * edot<ex, ey, ez>(qo[1], qo[2], qo[3]), L::w(K), rho0, ig);
+            const float g_n = eq_point_eu(qn, -sg * edot<ex, ey, ez>(qn[1], qn[2], qn[3]), L::w(K), rho0, ig);
+            gs = g_o + g_n;
+            gd = sg * (g_o - g_n);
+        } else {
+            const float2 r = pair_ties<ex, ey, ez>(en, po[0], po[1], po[2], po[3], pn[0], pn[1], pn[2], pn[3], ego[0],
+                                                   ego[1], ego[2], ego[3], egn[0], egn[1], egn[2], egn[3], L::w(K),
+                                                   rho0, ig);
+            gs = r.x;
+            gd = r.y;
+        }
+        f.S0 += gs;
+        eacc<ex>(f.S1x, gd); eacc<ey>(f.S1y, gd); eacc<ez>(f.S1z, gd);
+        const float eng = en * gs;
+        eacc<ex>(f.Tx, eng); eacc<ey>(f.Ty, eng); eacc<ez>(f.Tz, eng);
+        return;
+    }
     float pk[4], pb[4];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
@@ -175,7 +240,10 @@ __device__ __forceinline__ void all_pairs(std::integer_sequence<int, P...>, Face
     (pair_step<L, BAND, 2 * P + 1>(f, po, pn, p0, go, gn, nx, ny, nz, bw, ghost, rho0, ig), ...);
 }
 
-template <class L, bool BAND>
+// GHOST: compile-time face kind (interior faces come before boundary faces in
+// every tile's list, so the flux loop is split and neither variant carries
+// the other's selects)
+template <class L, bool BAND, bool GHOST>
 __device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__restrict__ st,
                                             const float4 *__restrict__ grad, int tf)
 {
@@ -183,14 +251,14 @@ __device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__
     const float4 fa = a.fA[tf], fb = a.fB[tf], fcv = a.fC[tf];
     const int2 on = a.fI[tf];
     const int o = on.x, nb = on.y;
-    const bool ghost = nb >= a.n_cells;
+    constexpr bool ghost = GHOST;
     const float nx = fa.x, ny = fa.y, nz = fa.z, A = fa.w, dt = fb.w;
     const float4 so = st[o];
     float go[12], gn[12];
     load_grad(grad, o, go);
     const float sov[4] = {so.x, so.y, so.z, so.w};
     float po[4], pn[4], p0[4];
-    if (!ghost) {
+    if constexpr (!GHOST) {
         const float4 sn = st[nb];
         load_grad(grad, nb, gn);
         const float snv[4] = {sn.x, sn.y, sn.z, sn.w};
@@ -274,7 +342,9 @@ __global__ __launch_bounds__(NT) void k_flux_update(FluidArgs a, const float4 *_
     extern __shared__ float4 lds_flux[];
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int f0 = a.tile_face_start[t], f1 = a.tile_face_start[t + 1];
-    for (int f = f0 + threadIdx.x; f < f1; f += blockDim.x) lds_flux[f - f0] = face_flux<L, BAND>(a, st, grad, f);
+    const int fb = a.tile_bface_start[t];
+    for (int f = f0 + threadIdx.x; f < fb; f += blockDim.x) lds_flux[f - f0] = face_flux<L, BAND, false>(a, st, grad, f);
+    for (int f = fb + threadIdx.x; f < f1; f += blockDim.x) lds_flux[f - f0] = face_flux<L, BAND, true>(a, st, grad, f);
     __syncthreads();
     const int c0 = a.tile_cell_start[t], c1 = a.tile_cell_start[t + 1];
     for (int i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
@@ -324,7 +394,8 @@ __global__ __launch_bounds__(256) void k_face_flux_all(FluidArgs a, const float4
                                                        int n_tile_faces)
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f < n_tile_faces) out[f] = face_flux<L, BAND>(a, st, grad, f);
+    if (f < n_tile_faces)
+        out[f] = a.fI[f].y >= a.n_cells ? face_flux<L, BAND, true>(a, st, grad, f) : face_flux<L, BAND, false>(a, st, grad, f);
 }
 
 // ---------------------------------------------------------------------------

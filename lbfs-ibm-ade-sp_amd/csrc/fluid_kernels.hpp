@@ -23,6 +23,7 @@ struct FluidArgs {
     const float *cell_force;       // [3 n] or null (IBM force density)
     // tiles
     const int *tile_cell_start, *tile_face_start;
+    const int *tile_bface_start;   // first boundary face of each tile's list
     // tile faces
     const float4 *fA, *fB, *fC;
     const int2 *fI;

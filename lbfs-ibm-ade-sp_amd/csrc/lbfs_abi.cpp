@@ -100,6 +100,9 @@ static int fluid_setup(lbfs_ctx *ctx, const lbfs_mesh_desc &m, const lbfs_params
     if ((rc = dev_upload(ctx, &slots, d.cell_slots.data(), d.cell_slots.size()))) return rc;
     if ((rc = dev_upload(ctx, &tcs, d.tile_cell_start.data(), d.tile_cell_start.size()))) return rc;
     if ((rc = dev_upload(ctx, &tfs, d.tile_face_start.data(), d.tile_face_start.size()))) return rc;
+    int *tbs;
+    if ((rc = dev_upload(ctx, &tbs, d.tile_bface_start.data(), d.tile_bface_start.size()))) return rc;
+    a.tile_bface_start = tbs;
     const size_t ntf = d.face_ref.size();
     if ((rc = dev_upload(ctx, &fA, (const float4 *)d.fA.data(), ntf))) return rc;
     if ((rc = dev_upload(ctx, &fB, (const float4 *)d.fB.data(), ntf))) return rc;

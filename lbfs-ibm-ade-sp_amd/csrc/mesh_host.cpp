@@ -377,6 +377,7 @@ int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<d
     }
     // tile face lists: every face touching a cell of the tile, in caller face order
     d.tile_face_start.assign(1, 0);
+    d.tile_bface_start.clear();
     d.fA.clear(); d.fB.clear(); d.fC.clear(); d.fI.clear(); d.face_ref.clear();
     d.face_first.assign(g.n_faces, -1);
     d.max_tile_faces = 0;
@@ -389,6 +390,8 @@ int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<d
         faces.erase(std::unique(faces.begin(), faces.end()), faces.end());
         const int base = (int)d.face_ref.size();
         if ((int)faces.size() > 32767) { err = "tile has too many faces"; return LBFS_E_UNSUPPORTED; }
+        // caller order puts internal faces (ids < n_int) before boundary faces
+        d.tile_bface_start.push_back(base + (int)(std::lower_bound(faces.begin(), faces.end(), g.n_int) - faces.begin()));
         d.max_tile_faces = std::max(d.max_tile_faces, (int)faces.size());
         for (int f : faces) {
             const int o = g.owner[f], n = g.nbr[f];

@@ -66,6 +66,7 @@ struct DeviceLayout {
     std::vector<int> perm;        // internal id -> caller id
     std::vector<int> inv;         // caller id -> internal id
     std::vector<int> tile_cell_start, tile_face_start;
+    std::vector<int> tile_bface_start;  // per tile: first boundary face (interior faces come first)
     // per cell (internal order)
     std::vector<int> stencil;     // 6 * n, SoA [k*n + i]; ghost entries encoded -1 - bc
     std::vector<float> gw;        // 18 * n, SoA [(k*3+c)*n + i], W_k = LHS^-1 RHS_k
