@@ -146,7 +146,8 @@ int lbfs_step(lbfs_ctx *ctx, int nsteps);
 
 /* options (no counterpart in the reference) */
 #define LBFS_OPT_GRAPH 1 /* 1 (default): replay the step from a captured hipGraph, 0: launch kernels one by one */
-#define LBFS_OPT_IBM_BINNED 2 /* spreading for the next lbfs_add_membranes: -1 (default) by node count, 0 one
+#define LBFS_OPT_IBM_BINNED 2 /* spreading for the next lbfs_add_membranes: -1 (default) by node count
+                                 (binned from 2048 nodes), 0 one
                                  wavefront per node into integer cell accumulators, 1 nodes binned and spread per
                                  bin through LDS; the result is the same bit for bit */
 int lbfs_set_option(lbfs_ctx *ctx, int option, int value);

@@ -269,10 +269,11 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     g.axis = dax;
     g.n_blocks = nblk;
     g.n_bins = (int)bin_block.size();
-    // few nodes: spread per node straight into the cell accumulators (fewer,
-    // shorter launches); many nodes: bin them and spread per bin through LDS,
-    // which keeps the global atomic traffic bounded
-    g.binned = ctx->ibm_binned_option >= 0 ? ctx->ibm_binned_option : (nn > 65536 ? 1 : 0);
+    // few nodes: spread per node straight into the cell accumulators (fewer
+    // launches); from a couple of thousand nodes on, binning them and
+    // spreading per bin through LDS is faster (one 2562-node cell in a
+    // 128x64x64 channel: 28 us against 39 us for the spreading)
+    g.binned = ctx->ibm_binned_option >= 0 ? ctx->ibm_binned_option : (nn >= 2048 ? 1 : 0);
     g.dt = (float)ctx->params.dt;
     std::vector<int> node_block(nn), mem_block(count);
     for (int i = 0; i < count; i++) {

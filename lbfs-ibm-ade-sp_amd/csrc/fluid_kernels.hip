@@ -137,27 +137,6 @@ __device__ __forceinline__ float eq_point(const float ph[4], float w, float rho0
     return w * (ph[0] + (rho0 + ph[0]) * P);
 }
 
-// Strict upwinding with the selects (ties, e.n = 0, go to the owner):
-// outlined so that the common tie-free path carries none of its work.
-template <int EX, int EY, int EZ>
-__device__ __attribute__((noinline)) float2 pair_ties(float en, float po0, float po1, float po2, float po3, float pn0,
-                                                      float pn1, float pn2, float pn3, float eo0, float eo1, float eo2,
-                                                      float eo3, float en0, float en1, float en2, float en3, float w,
-                                                      float rho0, float ig)
-{
-    const float po[4] = {po0, po1, po2, po3}, pn[4] = {pn0, pn1, pn2, pn3};
-    const float ego[4] = {eo0, eo1, eo2, eo3}, egn[4] = {en0, en1, en2, en3};
-    float pk[4], pb[4];
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-        pk[v] = en < 0.0f ? pn[v] - egn[v] : po[v] - ego[v];
-        pb[v] = en > 0.0f ? pn[v] + egn[v] : po[v] + ego[v];
-    }
-    const float gk = eq_point<EX, EY, EZ>(pk, w, rho0, ig);
-    const float gb = eq_point<-EX, -EY, -EZ>(pb, w, rho0, ig);
-    return make_float2(gk + gb, gk - gb);
-}
-
 struct FaceSums {
     float S0, S1x, S1y, S1z, Tx, Ty, Tz;
 };
@@ -178,29 +157,35 @@ __device__ __forceinline__ void pair_step(FaceSums &f, const float po[4], const 
             ego[v] = edot<ex, ey, ez>(go[3 * v], go[3 * v + 1], go[3 * v + 2]);
             egn[v] = edot<ex, ey, ez>(gn[3 * v], gn[3 * v + 1], gn[3 * v + 2]);
         }
-        float gs, gd;
-        if (__builtin_expect(__ballot(en == 0.0f) == 0, 1)) {
-            // No tie in the wavefront: the owner side feeds the point that
-            // moves along s e and the neighbour side the one along -s e,
-            // s = sign(e.n): the values of the strict rule without selects
-            const float sg = en > 0.0f ? 1.0f : -1.0f;
-            float qo[4], qn[4];
+        // The owner side feeds the point that moves along s e and the
+        // neighbour side the one along -s e, s = sign(e.n): the strict rule
+        // without selects.  A tie (e.n = 0) takes s = -1 and feeds both
+        // points from the owner: only the inputs of the second point change,
+        // so every face runs the same arithmetic whatever the other lanes of
+        // its wavefront hold.
+        const float sg = en > 0.0f ? 1.0f : -1.0f;
+        float an[4], bn[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            an[v] = pn[v];
+            bn[v] = egn[v];
+        }
+        if (__builtin_expect(__ballot(en == 0.0f) != 0, 0)) {
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                qo[v] = fmaf(-sg, ego[v], po[v]);
-                qn[v] = fmaf(sg, egn[v], pn[v]);
+                an[v] = en == 0.0f ? po[v] : pn[v];
+                bn[v] = en == 0.0f ? ego[v] : egn[v];
             }
-            const float g_o = eq_point_eu(qo, sg * edot<ex, ey, ez>(qo[1], qo[2], qo[3]), L::w(K), rho0, ig);
-            const float g_n = eq_point_eu(qn, -sg * edot<ex, ey, ez>(qn[1], qn[2], qn[3]), L::w(K), rho0, ig);
-            gs = g_o + g_n;
-            gd = sg * (g_o - g_n);
-        } else {
-            const float2 r = pair_ties<ex, ey, ez>(en, po[0], po[1], po[2], po[3], pn[0], pn[1], pn[2], pn[3], ego[0],
-                                                   ego[1], ego[2], ego[3], egn[0], egn[1], egn[2], egn[3], L::w(K),
-                                                   rho0, ig);
-            gs = r.x;
-            gd = r.y;
         }
+        float qo[4], qn[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            qo[v] = fmaf(-sg, ego[v], po[v]);
+            qn[v] = fmaf(sg, bn[v], an[v]);
+        }
+        const float g_o = eq_point_eu(qo, sg * edot<ex, ey, ez>(qo[1], qo[2], qo[3]), L::w(K), rho0, ig);
+        const float g_n = eq_point_eu(qn, -sg * edot<ex, ey, ez>(qn[1], qn[2], qn[3]), L::w(K), rho0, ig);
+        const float gs = g_o + g_n, gd = sg * (g_o - g_n);
         f.S0 += gs;
         eacc<ex>(f.S1x, gd); eacc<ey>(f.S1y, gd); eacc<ez>(f.S1z, gd);
         const float eng = en * gs;
