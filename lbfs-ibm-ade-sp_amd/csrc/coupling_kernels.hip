@@ -18,164 +18,15 @@
 #include <hip/hip_runtime.h>
 
 #include "coupling.hpp"
+#include "ibm_device.hpp"
+#include "membrane_force.hpp"
 
 namespace lbfs {
 
-namespace {
-
-constexpr double FIX_SCALE = 1099511627776.0;  // 2^40
-constexpr float PI_F = 3.14159265358979f;
-
-struct f3 {
-    float x, y, z;
-};
-__device__ __forceinline__ f3 mk(float x, float y, float z) { return {x, y, z}; }
-__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
-__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ f3 cross(f3 a, f3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-__device__ __forceinline__ f3 ld3(float4 v) { return {v.x, v.y, v.z}; }
-__device__ __forceinline__ float cmp(f3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
-
-// ---------------------------------------------------------------------------
-// IBM stencil helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ int stencil_size(const IbmArgs &g) { return g.kernel == LBFS_IBM_LINEAR2 ? 2 : 4; }
-
-// nearest vertex plane, ties to the larger index (find_index_closest,
-// immersed_boundary_method.cu:588-636), by bisection
-__device__ __forceinline__ int closest_vertex(const double *v, int n, double t)
-{
-    if (t <= v[0]) return 0;
-    if (t >= v[n - 1]) return n - 1;
-    int lo = 0, hi = n - 1;  // v[lo] < t <= v[hi]
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (v[mid] < t) lo = mid; else hi = mid;
-    }
-    return (t - v[lo] >= v[hi] - t) ? hi : lo;
-}
-
-// cell-line coordinate at unwrapped index c (periodic image or wall continuation)
-__device__ __forceinline__ double line_coord(const IbmArgs &g, const IbmBlock &B, int a, int c)
-{
-    const int n = B.n[a];
-    const double *l = g.axis + B.ax_off[a] + n + 1;
-    if (c >= 0 && c < n) return l[c];
-    if (B.per[a]) {
-        const int k = c >= 0 ? c / n : -((-c + n - 1) / n);
-        return l[c - k * n] + (double)k * (double)B.L[a];
-    }
-    const double d0 = n > 1 ? l[1] - l[0] : (double)B.h, d1 = n > 1 ? l[n - 1] - l[n - 2] : (double)B.h;
-    return c < 0 ? l[0] + c * d0 : l[n - 1] + (c - (n - 1)) * d1;
-}
-
-// cubic Lagrange basis i of the 4 line points from `base` at x (the first
-// column of the inverse Vandermonde matrix, vandemonde_inverse :919-941)
-__device__ __forceinline__ double lagrange4(const IbmArgs &g, const IbmBlock &B, int a, int base, int i, double x)
-{
-    double p[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) p[k] = line_coord(g, B, a, base + k);
-    double v = 1.0;
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-        if (k != i) v *= (x - p[k]) / (p[i] - p[k]);
-    return v;
-}
-
-__device__ __forceinline__ void node_base(const IbmArgs &g, const IbmBlock &B, const float P[3], float xr[3], int base[3])
-{
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        xr[a] = (P[a] - B.origin[a]) * B.inv_h;
-        if (g.kernel == LBFS_IBM_LAGRANGE4)
-            base[a] = closest_vertex(g.axis + B.ax_off[a], B.n[a] + 1, (double)P[a]) - 2;
-        else
-            base[a] = g.kernel == LBFS_IBM_LINEAR2 ? (int)roundf(xr[a]) - 1 : (int)floorf(xr[a] - 1.5f);
-    }
-}
-
-// wrap a logical index into the grid; false if it lies outside a wall
-__device__ __forceinline__ bool wrap_index(const IbmBlock &B, int a, int i, int &w)
-{
-    const int n = B.n[a];
-    if (i >= 0 && i < n) { w = i; return true; }
-    if (!B.per[a]) return false;
-    w = ((i % n) + n) % n;
-    return true;
-}
-
-// unnormalised weight of the cell with unwrapped index idx / wrapped index w
-// phi(r) = (1 + cos(pi r / 2)) / 4 for |r| < 2, written as (1 - sin^2(pi r / 4)) / 2
-// with sin from its Taylor series to x^11 (|x| <= pi/2: truncation below 6e-8)
-__device__ __forceinline__ float cos4_phi(float r)
-{
-    const float x = r * (PI_F * 0.25f), x2 = x * x;
-    float p = -2.5052108385e-8f;                 // -1/11!
-    p = fmaf(p, x2, 2.7557319224e-6f);           //  1/9!
-    p = fmaf(p, x2, -1.9841269841e-4f);          // -1/7!
-    p = fmaf(p, x2, 8.3333333333e-3f);           //  1/5!
-    p = fmaf(p, x2, -1.6666666667e-1f);          // -1/3!
-    const float sn = fmaf(p * x2, x, x);
-    return 0.5f * fmaf(-sn, sn, 1.0f);
-}
-
-__device__ __forceinline__ float cell_weight(const IbmArgs &g, const IbmBlock &B, const int idx[3], const int w[3],
-                                             int internal, const float P[3], const float xr[3])
-{
-    float wt = 1.0f;
-    if (g.kernel == LBFS_IBM_LINEAR2) {
-#pragma unroll
-        for (int a = 0; a < 3; a++) wt *= 1.0f - fabsf((float)idx[a] + 0.5f - xr[a]);
-    } else {
-        const float4 c = g.cen[internal];
-        const float cc[3] = {c.x, c.y, c.z};
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            const float xc = cc[a] + (float)(idx[a] - w[a]) * (B.L[a] / (float)B.n[a]);
-            const float r = (xc - P[a]) * B.inv_h;
-            wt *= fabsf(r) < 2.0f ? cos4_phi(r) : 0.0f;
-        }
-    }
-    return wt;
-}
-
-__device__ __forceinline__ int cell_internal(const IbmArgs &g, const IbmBlock &B, const int w[3])
-{
-    return g.inv[B.cell0 + w[0] + B.n[0] * (w[1] + B.n[1] * w[2])];
-}
-
-__device__ __forceinline__ int bin_coord(const IbmBlock &B, int a, int base)
-{
-    const int n = B.n[a];
-    int b = base;
-    if (B.per[a]) b = ((b % n) + n) % n;
-    else b = b < 0 ? 0 : (b >= n ? n - 1 : b);
-    return b >> 2;
-}
-
-}  // namespace
-
 // ---------------------------------------------------------------------------
 // IBM: interpolation, advection, weights, binning.  One wavefront per node,
-// one lane per stencil cell (64 for the 4-point kernel, 8 for the 2-point
-// one); sums are butterfly reductions, so their order is fixed.
+// one lane per stencil cell.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float wave_sum(float v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-__device__ __forceinline__ void lane_offset(int S, int lane, int off[3])
-{
-    if (S == 4) { off[0] = lane & 3; off[1] = (lane >> 2) & 3; off[2] = lane >> 4; }
-    else { off[0] = lane & 1; off[1] = (lane >> 1) & 1; off[2] = lane >> 2; }
-}
-
 // INTERP: interpolate the cell velocity with the node's cached weights (those
 // of its current position), advect the node and count it into its bin.
 // Always: weights of the (new) position, normalised for the cosine kernel,
@@ -519,25 +370,22 @@ __global__ __launch_bounds__(64) void k_mem_globals(MemArgs m)
     }
 }
 
-namespace {
-// derivatives of N = (q - p) x (r - p) and of N . centroid for the nine
-// coordinates p.xyz q.xyz r.xyz: dN = e_k x (next - prev) (spring_network.cu:1348-1412)
-__device__ __forceinline__ void tri_derivs(f3 p, f3 q, f3 r, f3 N, f3 dN[9], float dV[9])
+// per-membrane global coefficients in FP64 (volume and area constraints,
+// MAD), rounded once to the working precision
+__device__ __forceinline__ void mem_coefs(const MemParamDev &p, const double *gl, float &vol_coef, float &area_coef,
+                                          float &mad_coef)
 {
-    const f3 v[3] = {p, q, r};
-    const f3 cen = (p + q + r) * (1.0f / 3.0f);
-#pragma unroll
-    for (int vt = 0; vt < 3; vt++) {
-        const f3 dd = v[(vt + 1) % 3] - v[(vt + 2) % 3];
-        const f3 dk[3] = {mk(0.0f, -dd.z, dd.y), mk(dd.z, 0.0f, -dd.x), mk(-dd.y, dd.x, 0.0f)};
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            dN[3 * vt + k] = dk[k];
-            dV[3 * vt + k] = cmp(N, k) * (1.0f / 3.0f) + dot(dk[k], cen);
-        }
-    }
+    vol_coef = (float)(-(double)p.volume_modulus * (gl[0] / gl[3] - 1.0) / 6.0);
+    area_coef = (float)(-(double)p.global_area_modulus * (gl[1] / gl[4] - 1.0));
+    mad_coef = (float)(-2.0 * (double)p.global_bending * (gl[2] - gl[5]) / gl[4]);
 }
-}  // namespace
+
+__device__ __forceinline__ memf::EdgeConst edge_const(const MemArgs &m, const MemParamDev &p, int e)
+{
+    const float4 ec = m.edge_const[e];
+    const float2 a0 = m.edge_area0[e];
+    return {ec.x, ec.z, a0.x, a0.y, p.local_area_modulus, p.viscosity, p.spring_c};
+}
 
 __global__ __launch_bounds__(128) void k_mem_edges(MemArgs m)
 {
@@ -545,107 +393,24 @@ __global__ __launch_bounds__(128) void k_mem_edges(MemArgs m)
     if (e >= m.n_edges) return;
     const int mem = m.edge_mem[e];
     const MemParamDev p = m.params[mem];
-    const double *gl = m.glob + 8 * (size_t)mem;
-    // global coefficients in FP64 (volume and area constraints, MAD)
-    const float vol_coef = (float)(-(double)p.volume_modulus * (gl[0] / gl[3] - 1.0) / 6.0);
-    const float area_coef = (float)(-(double)p.global_area_modulus * (gl[1] / gl[4] - 1.0));
-    const float mad_coef = (float)(-2.0 * (double)p.global_bending * (gl[2] - gl[5]) / gl[4]);
+    memf::EdgeIn<float> in;
+    mem_coefs(p, m.glob + 8 * (size_t)mem, in.vol_coef, in.area_coef, in.mad_coef);
     const int4 q = m.edges[e];
-    const f3 A = ld3(m.X[q.x]), B = ld3(m.X[q.y]), C = ld3(m.X[q.z]), D = ld3(m.X[q.w]);
-    const f3 l = C - A;
-    const float length = sqrtf(dot(l, l));
-    const f3 N1 = cross(B - A, C - A), N2 = cross(C - A, D - A);
-    const float n1 = sqrtf(dot(N1, N1)), n2 = sqrtf(dot(N2, N2));
-    f3 dN1[9], dN2[9];
-    float dV1[9], dV2[9];
-    tri_derivs(A, B, C, N1, dN1, dV1);
-    tri_derivs(C, D, A, N2, dN2, dV2);
-    const float in1 = 1.0f / n1, in2 = 1.0f / n2;
-    float dn1[9], dn2[9];
+    const float4 xa = m.X[q.x], xb = m.X[q.y], xc = m.X[q.z], xd = m.X[q.w], va = m.U[q.x], vc = m.U[q.z];
+    in.A = {xa.x, xa.y, xa.z};
+    in.B = {xb.x, xb.y, xb.z};
+    in.C = {xc.x, xc.y, xc.z};
+    in.D = {xd.x, xd.y, xd.z};
+    in.Ua = {va.x, va.y, va.z};
+    in.Uc = {vc.x, vc.y, vc.z};
+    in.ks = m.edge_const[e].y;
+    in.lb = p.local_bending;
+    in.gb = p.global_bending;
+    memf::EdgeOut<float> out;
+    memf::edge_force(in, edge_const(m, p, e), out);
+    m.edge_state[e] = make_float4(out.kappa, out.n1, out.n2, 0.0f);
 #pragma unroll
-    for (int j = 0; j < 9; j++) {
-        dn1[j] = dot(dN1[j], N1) * in1;
-        dn2[j] = dot(dN2[j], N2) * in2;
-    }
-    // slots A(0-2) B(3-5) C(6-8) D(9-11); triangle 2 is (C, D, A)
-    float d_area[12], d_vol[12], d_ang[12];
-#pragma unroll
-    for (int j = 0; j < 9; j++) { d_area[j] = dn1[j]; d_vol[j] = dV1[j]; }
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-        d_area[9 + j] = dn2[3 + j];
-        d_vol[9 + j] = dV2[3 + j];
-        d_area[6 + j] += dn2[j];
-        d_vol[6 + j] += dV2[j];
-        d_area[j] += dn2[6 + j];
-        d_vol[j] += dV2[6 + j];
-    }
-#pragma unroll
-    for (int j = 0; j < 12; j++) { d_area[j] *= (1.0f / 6.0f); d_vol[j] *= (1.0f / 18.0f); }
-    float cs = dot(N1, N2) * in1 * in2;
-    cs = fminf(fmaxf(cs, -1.0f), 1.0f);
-    const f3 c1 = (A + B + C) * (1.0f / 3.0f), c2 = (A + C + D) * (1.0f / 3.0f);
-    const float dir = dot(N1, c2 - c1);
-    const float angle = acosf(cs) * (float)((dir < 0.0f) - (dir > 0.0f));
-    const float il = 1.0f / length;
-    const f3 dL = l * il;
-    {
-        const f3 h1 = cross(N1 * in1, dL), h2 = cross(dL, N2 * in2);
-        float a2[9];
-#pragma unroll
-        for (int j = 0; j < 9; j++) {
-            d_ang[j] = -in1 * dot(h1, dN1[j]);
-            a2[j] = -in2 * dot(h2, dN2[j]);
-        }
-        d_ang[9] = a2[3]; d_ang[10] = a2[4]; d_ang[11] = a2[5];
-        d_ang[6] += a2[0]; d_ang[7] += a2[1]; d_ang[8] += a2[2];
-        d_ang[0] += a2[6]; d_ang[1] += a2[7]; d_ang[2] += a2[8];
-    }
-    float d_curve[12];
-#pragma unroll
-    for (int j = 0; j < 12; j++) d_curve[j] = d_ang[j] * length;
-    d_curve[0] -= dL.x * angle; d_curve[1] -= dL.y * angle; d_curve[2] -= dL.z * angle;
-    d_curve[6] += dL.x * angle; d_curve[7] += dL.y * angle; d_curve[8] += dL.z * angle;
-    f3 Fo[4] = {mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
-    // local and global area: one opposite node per triangle
-    const float2 a0 = m.edge_area0[e];
-    const float ca = -p.local_area_modulus * (n1 / a0.x - 1.0f) + area_coef;
-    const float cb = -p.local_area_modulus * (n2 / a0.y - 1.0f) + area_coef;
-    Fo[1] = Fo[1] + mk(dn1[3], dn1[4], dn1[5]) * ca;
-    Fo[3] = Fo[3] + mk(dn2[3], dn2[4], dn2[5]) * cb;
-    // volume
-#pragma unroll
-    for (int j = 0; j < 4; j++) Fo[j] = Fo[j] + mk(d_vol[3 * j], d_vol[3 * j + 1], d_vol[3 * j + 2]) * vol_coef;
-    // membrane viscosity
-    {
-        const float4 va = m.U[q.x], vc = m.U[q.z];
-        const f3 vca = mk(vc.x - va.x, vc.y - va.y, vc.z - va.z);
-        const f3 vf = dL * (-dot(dL, vca) * 4.0f / 1.7320508075688772f * p.viscosity);
-        Fo[2] = Fo[2] + vf;
-        Fo[0] = Fo[0] - vf;
-    }
-    // bending
-    const float4 ec = m.edge_const[e];
-    const float kappa = angle * length / ((n1 + n2) * (1.0f / 6.0f));
-    const float dk0 = ec.z - kappa;
-    const float lbc = -0.5f * p.local_bending * dk0 * dk0;
-    const float ccf = -p.global_bending * (kappa - ec.z) + mad_coef;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        Fo[j] = Fo[j] + mk(d_area[3 * j], d_area[3 * j + 1], d_area[3 * j + 2]) * lbc;
-        Fo[j] = Fo[j] + mk(d_curve[3 * j], d_curve[3 * j + 1], d_curve[3 * j + 2]) * ccf;
-    }
-    // worm-like chain spring
-    {
-        const float x = length / ec.x;
-        const float om = 1.0f - x;
-        const float fc = -ec.y * (0.25f / (om * om) - 0.25f + x - p.spring_c);
-        Fo[2] = Fo[2] + dL * fc;
-        Fo[0] = Fo[0] - dL * fc;
-    }
-    m.edge_state[e] = make_float4(kappa, n1, n2, 0.0f);
-#pragma unroll
-    for (int j = 0; j < 4; j++) m.edge_force[4 * (size_t)e + j] = make_float4(Fo[j].x, Fo[j].y, Fo[j].z, 0.0f);
+    for (int j = 0; j < 4; j++) m.edge_force[4 * (size_t)e + j] = make_float4(out.F[j].x, out.F[j].y, out.F[j].z, 0.0f);
 }
 
 // eight lanes per node, each summing every eighth incidence; the eight

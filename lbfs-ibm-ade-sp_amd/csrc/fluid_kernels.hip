@@ -18,20 +18,11 @@
 
 #include <utility>
 
+#include "fluid_device.hpp"
 #include "fluid_kernels.hpp"
 #include "lattice.hpp"
 
 namespace lbfs {
-
-__device__ __forceinline__ int xcd_remap(int bid, int nblocks)
-{
-    // consecutive tile ids land on the same XCD (8 XCDs, round-robin dispatch)
-    const int per = (nblocks + 7) >> 3;
-    const int xcd = bid & 7, idx = bid >> 3;
-    const int full = nblocks - (per - 1) * 8;  // XCDs that get `per` blocks
-    int base = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
-    return base + idx;
-}
 
 // ---------------------------------------------------------------------------
 // gradients
@@ -83,152 +74,7 @@ __global__ __launch_bounds__(256) void k_gradients(FluidArgs a, const float4 *__
 // ---------------------------------------------------------------------------
 // face reconstruction and flux
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void load_grad(const float4 *__restrict__ grad, int i, float g[12])
-{
-    const float4 a = grad[3 * (size_t)i], b = grad[3 * (size_t)i + 1], c = grad[3 * (size_t)i + 2];
-    g[0] = a.x; g[1] = a.y; g[2] = a.z; g[3] = a.w;
-    g[4] = b.x; g[5] = b.y; g[6] = b.z; g[7] = b.w;
-    g[8] = c.x; g[9] = c.y; g[10] = c.z; g[11] = c.w;
-}
-
-template <class L>
-constexpr bool opposite_pairs()
-{
-    for (int k = 1; k + 1 < L::Q; k += 2)
-        if (L::ex[k] != -L::ex[k + 1] || L::ey[k] != -L::ey[k + 1] || L::ez[k] != -L::ez[k + 1]) return false;
-    return (L::Q % 2) == 1;
-}
-
-// Sums over the components of a lattice direction with the zero components
-// dropped at compile time: 0 * x cannot be folded away at run time (x may be
-// infinite or NaN), so these are written out per direction.  -0.0f is the
-// exact identity of addition, so the first term folds to itself.
-template <int EX, int EY, int EZ>
-__device__ __forceinline__ float edot(float a, float b, float c)
-{
-    float s = -0.0f;
-    if constexpr (EX == 1) s += a; else if constexpr (EX == -1) s -= a;
-    if constexpr (EY == 1) s += b; else if constexpr (EY == -1) s -= b;
-    if constexpr (EZ == 1) s += c; else if constexpr (EZ == -1) s -= c;
-    return s;
-}
-
-template <int E>
-__device__ __forceinline__ void eacc(float &s, float v)
-{
-    if constexpr (E == 1) s += v; else if constexpr (E == -1) s -= v;
-}
-
-// equilibrium deviation g = w (drho + rho P) of one lattice point with state
-// ph = (drho, u, v, w): P = 3 e.u + 4.5 (e.u)^2 / gamma - 1.5 u.u / gamma
-__device__ __forceinline__ float eq_point_eu(const float ph[4], float eu, float w, float rho0, float ig)
-{
-    const float uu = ph[1] * ph[1] + ph[2] * ph[2] + ph[3] * ph[3];
-    const float P = eu * (3.0f + 4.5f * ig * eu) - 1.5f * ig * uu;
-    return w * (ph[0] + (rho0 + ph[0]) * P);
-}
-
-template <int EX, int EY, int EZ>
-__device__ __forceinline__ float eq_point(const float ph[4], float w, float rho0, float ig)
-{
-    const float eu = edot<EX, EY, EZ>(ph[1], ph[2], ph[3]);
-    const float uu = ph[1] * ph[1] + ph[2] * ph[2] + ph[3] * ph[3];
-    const float P = eu * (3.0f + 4.5f * ig * eu) - 1.5f * ig * uu;
-    return w * (ph[0] + (rho0 + ph[0]) * P);
-}
-
-struct FaceSums {
-    float S0, S1x, S1y, S1z, Tx, Ty, Tz;
-};
-
-// one pair of opposite directions (e, -e): one e.n and one e.grad per side
-template <class L, bool BAND, int K>
-__device__ __forceinline__ void pair_step(FaceSums &f, const float po[4], const float pn[4], const float p0[4],
-                                          const float go[12], const float gn[12], float nx, float ny, float nz,
-                                          float bw, bool ghost, float rho0, float ig)
-{
-    constexpr int ex = L::ex[K], ey = L::ey[K], ez = L::ez[K];
-    static_assert(L::w(K) == L::w(K + 1), "opposite directions carry equal weights");
-    const float en = edot<ex, ey, ez>(nx, ny, nz);
-    if constexpr (!BAND) {
-        float ego[4], egn[4];
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            ego[v] = edot<ex, ey, ez>(go[3 * v], go[3 * v + 1], go[3 * v + 2]);
-            egn[v] = edot<ex, ey, ez>(gn[3 * v], gn[3 * v + 1], gn[3 * v + 2]);
-        }
-        // The owner side feeds the point that moves along s e and the
-        // neighbour side the one along -s e, s = sign(e.n): the strict rule
-        // without selects.  A tie (e.n = 0) takes s = -1 and feeds both
-        // points from the owner: only the inputs of the second point change,
-        // so every face runs the same arithmetic whatever the other lanes of
-        // its wavefront hold.
-        const float sg = en > 0.0f ? 1.0f : -1.0f;
-        float an[4], bn[4];
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            an[v] = pn[v];
-            bn[v] = egn[v];
-        }
-        if (__builtin_expect(__ballot(en == 0.0f) != 0, 0)) {
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                an[v] = en == 0.0f ? po[v] : pn[v];
-                bn[v] = en == 0.0f ? ego[v] : egn[v];
-            }
-        }
-        float qo[4], qn[4];
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            qo[v] = fmaf(-sg, ego[v], po[v]);
-            qn[v] = fmaf(sg, bn[v], an[v]);
-        }
-        const float g_o = eq_point_eu(qo, sg * edot<ex, ey, ez>(qo[1], qo[2], qo[3]), L::w(K), rho0, ig);
-        const float g_n = eq_point_eu(qn, -sg * edot<ex, ey, ez>(qn[1], qn[2], qn[3]), L::w(K), rho0, ig);
-        const float gs = g_o + g_n, gd = sg * (g_o - g_n);
-        f.S0 += gs;
-        eacc<ex>(f.S1x, gd); eacc<ey>(f.S1y, gd); eacc<ez>(f.S1z, gd);
-        const float eng = en * gs;
-        eacc<ex>(f.Tx, eng); eacc<ey>(f.Ty, eng); eacc<ez>(f.Tz, eng);
-        return;
-    }
-    float pk[4], pb[4];
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const float ego = edot<ex, ey, ez>(go[3 * v], go[3 * v + 1], go[3 * v + 2]);
-        const float egn = edot<ex, ey, ez>(gn[3 * v], gn[3 * v + 1], gn[3 * v + 2]);
-        if (!BAND) {
-            // point x_f - e dt sits on the neighbour side when e.n < 0 (ties to the owner)
-            pk[v] = en < 0.0f ? pn[v] - egn : po[v] - ego;
-            pb[v] = en > 0.0f ? pn[v] + egn : po[v] + ego;
-        } else {
-            const float avg = ghost ? egn : 0.5f * (ego + egn);
-            pk[v] = en > bw ? po[v] - ego : (en < -bw ? pn[v] - egn : p0[v] - avg);
-            pb[v] = en < -bw ? po[v] + ego : (en > bw ? pn[v] + egn : p0[v] + avg);
-        }
-    }
-    const float gk = eq_point<ex, ey, ez>(pk, L::w(K), rho0, ig);
-    const float gb = eq_point<-ex, -ey, -ez>(pb, L::w(K + 1), rho0, ig);
-    const float gs = gk + gb, gd = gk - gb;
-    f.S0 += gs;
-    eacc<ex>(f.S1x, gd); eacc<ey>(f.S1y, gd); eacc<ez>(f.S1z, gd);
-    const float eng = en * gs;
-    eacc<ex>(f.Tx, eng); eacc<ey>(f.Ty, eng); eacc<ez>(f.Tz, eng);
-}
-
-template <class L, bool BAND, int... P>
-__device__ __forceinline__ void all_pairs(std::integer_sequence<int, P...>, FaceSums &f, const float po[4],
-                                          const float pn[4], const float p0[4], const float go[12],
-                                          const float gn[12], float nx, float ny, float nz, float bw, bool ghost,
-                                          float rho0, float ig)
-{
-    (pair_step<L, BAND, 2 * P + 1>(f, po, pn, p0, go, gn, nx, ny, nz, bw, ghost, rho0, ig), ...);
-}
-
-// GHOST: compile-time face kind (interior faces come before boundary faces in
-// every tile's list, so the flux loop is split and neither variant carries
-// the other's selects)
-template <class L, bool BAND, bool GHOST>
+template <class L, bool BAND>
 __device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__restrict__ st,
                                             const float4 *__restrict__ grad, int tf)
 {
@@ -236,14 +82,14 @@ __device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__
     const float4 fa = a.fA[tf], fb = a.fB[tf], fcv = a.fC[tf];
     const int2 on = a.fI[tf];
     const int o = on.x, nb = on.y;
-    constexpr bool ghost = GHOST;
+    const bool ghost = nb >= a.n_cells;
     const float nx = fa.x, ny = fa.y, nz = fa.z, A = fa.w, dt = fb.w;
     const float4 so = st[o];
     float go[12], gn[12];
     load_grad(grad, o, go);
     const float sov[4] = {so.x, so.y, so.z, so.w};
     float po[4], pn[4], p0[4];
-    if constexpr (!GHOST) {
+    if (!ghost) {
         const float4 sn = st[nb];
         load_grad(grad, nb, gn);
         const float snv[4] = {sn.x, sn.y, sn.z, sn.w};
@@ -327,9 +173,8 @@ __global__ __launch_bounds__(NT) void k_flux_update(FluidArgs a, const float4 *_
     extern __shared__ float4 lds_flux[];
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int f0 = a.tile_face_start[t], f1 = a.tile_face_start[t + 1];
-    const int fb = a.tile_bface_start[t];
-    for (int f = f0 + threadIdx.x; f < fb; f += blockDim.x) lds_flux[f - f0] = face_flux<L, BAND, false>(a, st, grad, f);
-    for (int f = fb + threadIdx.x; f < f1; f += blockDim.x) lds_flux[f - f0] = face_flux<L, BAND, true>(a, st, grad, f);
+    for (int f = f0 + threadIdx.x; f < f1; f += blockDim.x)
+        lds_flux[f - f0] = face_flux<L, BAND>(a, st, grad, f);
     __syncthreads();
     const int c0 = a.tile_cell_start[t], c1 = a.tile_cell_start[t + 1];
     for (int i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
@@ -380,7 +225,7 @@ __global__ __launch_bounds__(256) void k_face_flux_all(FluidArgs a, const float4
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f < n_tile_faces)
-        out[f] = a.fI[f].y >= a.n_cells ? face_flux<L, BAND, true>(a, st, grad, f) : face_flux<L, BAND, false>(a, st, grad, f);
+        out[f] = face_flux<L, BAND>(a, st, grad, f);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,165 @@
+// Device helpers shared by the IBM/membrane kernels and their adjoints:
+// small 3-vectors, the IBM stencil (base cell, periodic wrap, weights of the
+// 2-point, 4-point cosine and 4-point Lagrange kernels) and wavefront sums.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "coupling.hpp"
+
+namespace lbfs {
+namespace {
+
+constexpr double FIX_SCALE = 1099511627776.0;  // 2^40
+constexpr float PI_F = 3.14159265358979f;
+
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ __forceinline__ f3 ld3(float4 v) { return {v.x, v.y, v.z}; }
+__device__ __forceinline__ float cmp(f3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+// ---------------------------------------------------------------------------
+// IBM stencil helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int stencil_size(const IbmArgs &g) { return g.kernel == LBFS_IBM_LINEAR2 ? 2 : 4; }
+
+// nearest vertex plane, ties to the larger index (find_index_closest,
+// immersed_boundary_method.cu:588-636), by bisection
+__device__ __forceinline__ int closest_vertex(const double *v, int n, double t)
+{
+    if (t <= v[0]) return 0;
+    if (t >= v[n - 1]) return n - 1;
+    int lo = 0, hi = n - 1;  // v[lo] < t <= v[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (v[mid] < t) lo = mid; else hi = mid;
+    }
+    return (t - v[lo] >= v[hi] - t) ? hi : lo;
+}
+
+// cell-line coordinate at unwrapped index c (periodic image or wall continuation)
+__device__ __forceinline__ double line_coord(const IbmArgs &g, const IbmBlock &B, int a, int c)
+{
+    const int n = B.n[a];
+    const double *l = g.axis + B.ax_off[a] + n + 1;
+    if (c >= 0 && c < n) return l[c];
+    if (B.per[a]) {
+        const int k = c >= 0 ? c / n : -((-c + n - 1) / n);
+        return l[c - k * n] + (double)k * (double)B.L[a];
+    }
+    const double d0 = n > 1 ? l[1] - l[0] : (double)B.h, d1 = n > 1 ? l[n - 1] - l[n - 2] : (double)B.h;
+    return c < 0 ? l[0] + c * d0 : l[n - 1] + (c - (n - 1)) * d1;
+}
+
+// cubic Lagrange basis i of the 4 line points from `base` at x (the first
+// column of the inverse Vandermonde matrix, vandemonde_inverse :919-941)
+__device__ __forceinline__ double lagrange4(const IbmArgs &g, const IbmBlock &B, int a, int base, int i, double x)
+{
+    double p[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) p[k] = line_coord(g, B, a, base + k);
+    double v = 1.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (k != i) v *= (x - p[k]) / (p[i] - p[k]);
+    return v;
+}
+
+__device__ __forceinline__ void node_base(const IbmArgs &g, const IbmBlock &B, const float P[3], float xr[3], int base[3])
+{
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        xr[a] = (P[a] - B.origin[a]) * B.inv_h;
+        if (g.kernel == LBFS_IBM_LAGRANGE4)
+            base[a] = closest_vertex(g.axis + B.ax_off[a], B.n[a] + 1, (double)P[a]) - 2;
+        else
+            base[a] = g.kernel == LBFS_IBM_LINEAR2 ? (int)roundf(xr[a]) - 1 : (int)floorf(xr[a] - 1.5f);
+    }
+}
+
+// wrap a logical index into the grid; false if it lies outside a wall
+__device__ __forceinline__ bool wrap_index(const IbmBlock &B, int a, int i, int &w)
+{
+    const int n = B.n[a];
+    if (i >= 0 && i < n) { w = i; return true; }
+    if (!B.per[a]) return false;
+    w = ((i % n) + n) % n;
+    return true;
+}
+
+// unnormalised weight of the cell with unwrapped index idx / wrapped index w
+// phi(r) = (1 + cos(pi r / 2)) / 4 for |r| < 2, written as (1 - sin^2(pi r / 4)) / 2
+// with sin from its Taylor series to x^11 (|x| <= pi/2: truncation below 6e-8)
+__device__ __forceinline__ float cos4_phi(float r)
+{
+    const float x = r * (PI_F * 0.25f), x2 = x * x;
+    float p = -2.5052108385e-8f;                 // -1/11!
+    p = fmaf(p, x2, 2.7557319224e-6f);           //  1/9!
+    p = fmaf(p, x2, -1.9841269841e-4f);          // -1/7!
+    p = fmaf(p, x2, 8.3333333333e-3f);           //  1/5!
+    p = fmaf(p, x2, -1.6666666667e-1f);          // -1/3!
+    const float sn = fmaf(p * x2, x, x);
+    return 0.5f * fmaf(-sn, sn, 1.0f);
+}
+
+__device__ __forceinline__ float cell_weight(const IbmArgs &g, const IbmBlock &B, const int idx[3], const int w[3],
+                                             int internal, const float P[3], const float xr[3])
+{
+    float wt = 1.0f;
+    if (g.kernel == LBFS_IBM_LINEAR2) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) wt *= 1.0f - fabsf((float)idx[a] + 0.5f - xr[a]);
+    } else {
+        const float4 c = g.cen[internal];
+        const float cc[3] = {c.x, c.y, c.z};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float xc = cc[a] + (float)(idx[a] - w[a]) * (B.L[a] / (float)B.n[a]);
+            const float r = (xc - P[a]) * B.inv_h;
+            wt *= fabsf(r) < 2.0f ? cos4_phi(r) : 0.0f;
+        }
+    }
+    return wt;
+}
+
+__device__ __forceinline__ int cell_internal(const IbmArgs &g, const IbmBlock &B, const int w[3])
+{
+    return g.inv[B.cell0 + w[0] + B.n[0] * (w[1] + B.n[1] * w[2])];
+}
+
+__device__ __forceinline__ int bin_coord(const IbmBlock &B, int a, int base)
+{
+    const int n = B.n[a];
+    int b = base;
+    if (B.per[a]) b = ((b % n) + n) % n;
+    else b = b < 0 ? 0 : (b >= n ? n - 1 : b);
+    return b >> 2;
+}
+
+// ---------------------------------------------------------------------------
+// wavefront helpers: one wavefront per node, one lane per stencil cell (64
+// for the 4-point kernels, 8 for the 2-point one); sums are butterfly
+// reductions, so their order is fixed
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void lane_offset(int S, int lane, int off[3])
+{
+    if (S == 4) { off[0] = lane & 3; off[1] = (lane >> 2) & 3; off[2] = lane >> 4; }
+    else { off[0] = lane & 1; off[1] = (lane >> 1) & 1; off[2] = lane >> 2; }
+}
+
+}  // namespace
+}  // namespace lbfs
