@@ -91,6 +91,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 1 GiB device-to-device copies (the reference for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--adjoint-every", type=int, default=100)
     ap.add_argument("--ensemble", action="store_true",
                     help="also time the 64-instance ensemble (edges from {32,48,64,96}, 1-8 cells of 642 nodes each)")
     args = ap.parse_args()
@@ -175,6 +178,44 @@ def main():
                    "mlups": m3.n_cells / (ms3 * 1e-3) / 1e6, "kernels_ms": k3}
         log("suspension step %.4f ms (plain %.4f ms)" % (ms3, ms3_plain))
 
+    # --- config 4: adjoint of the config-2 step, node loss against a run at
+    # moduli x 1.1, checkpoints every K steps ---
+    adjoint = None
+    if not args.no_adjoint:
+        log("adjoint: %d steps, checkpoints every %d" % (args.adjoint_steps, args.adjoint_every))
+        mems = workloads.single_rbc(mesh)
+        mp = mems[0].params
+        mu, kl, kg = mp["shear_modulus"], mp["local_bending_modulus"], mp["global_bending_modulus"]
+        a = abi.Solver(mesh, params)
+        a.set_state(*state)
+        a.add_membranes(mems, ibm_kernel=abi.IBM_COS4)
+        a.set_membrane_moduli(0, 1.1 * mu, 1.1 * kl, 1.1 * kg)
+        a.adjoint_forward(args.adjoint_steps, args.adjoint_every)
+        target, _, _ = a.membrane_nodes()
+        a.adjoint_rewind()
+        a.set_membrane_moduli(0, mu, kl, kg)
+        a.sync()
+        t0 = time.perf_counter()
+        a.adjoint_forward(args.adjoint_steps, args.adjoint_every)
+        t1 = time.perf_counter()
+        X, _, _ = a.membrane_nodes()
+        g = a.adjoint_backward(lam_nodes=2.0 * (X - target))
+        t2 = time.perf_counter()
+        a.close()
+        fwd = (t1 - t0) * 1e3 / args.adjoint_steps
+        bwd = (t2 - t1) * 1e3 / args.adjoint_steps
+        adjoint = {"workload": "config-2 channel + RBC, %d steps, checkpoints every %d, L = sum |X - X(moduli x 1.1)|^2"
+                               % (args.adjoint_steps, args.adjoint_every),
+                   "loss": float(np.sum((X - target) ** 2)),
+                   "grad": {"shear_modulus": float(g["moduli"][0, 0]),
+                            "local_bending_modulus": float(g["moduli"][0, 1]),
+                            "global_bending_modulus": float(g["moduli"][0, 2]),
+                            "body_force_x": float(g["body_force"][0])},
+                   "forward_ms_per_step": fwd, "backward_ms_per_step": bwd,
+                   "backward_vs_forward": bwd / fwd, "backward_vs_graph_step": bwd / ms_step,
+                   "note": "backward = segment replay + per-step recompute + reverse kernels"}
+        log("adjoint: forward %.4f ms/step, backward %.4f ms/step" % (fwd, bwd))
+
     ensemble = None
     if args.ensemble:
         log("building the 64-instance ensemble")
@@ -237,7 +278,7 @@ def main():
         "membrane_volume_rel_change": mglob[0] / mglob[3] - 1.0,
         "batched": batched,
         "ensemble": ensemble,
-        "adjoint": None,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

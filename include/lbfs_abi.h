@@ -229,6 +229,39 @@ int lbfs_get_membrane_globals(lbfs_ctx *ctx, double *out6);
  * spring_network.cu:1687-1768); evaluated from the last force call's state */
 int lbfs_get_membrane_energies(lbfs_ctx *ctx, double *out /* [7 n_membranes] */);
 
+/* ---- parameters the adjoint differentiates ----
+ * Uniform body force (the reference's pressure gradient along its direction,
+ * Solver_gpu.cu:834-839, preprocessor.cpp:722-736); replaces lbfs_params.body_force. */
+int lbfs_set_body_force(lbfs_ctx *ctx, const double *f /* [3] */);
+/* shear modulus (spring constants k_s are proportional to it, spring_network.cu:1600-1615),
+ * local and global bending moduli of membrane `index` (lagrangian_object.h) */
+int lbfs_set_membrane_moduli(lbfs_ctx *ctx, int index, double shear_modulus, double local_bending_modulus,
+                             double global_bending_modulus);
+
+/* ---- discrete adjoint of the coupled step (no counterpart in the reference:
+ * gradients of a loss on the state after n steps, for callers that fit
+ * membrane moduli or forcing) ----
+ * lbfs_adjoint_forward runs n_steps steps from the current state, keeping the
+ * state every checkpoint_every steps.  lbfs_adjoint_backward takes the
+ * loss's adjoints of the final node positions (3 per node, absolute) and of
+ * the final state (4 per cell: rho, u, v, w; caller order), either may be
+ * NULL for zero, and returns dL/d(body force) [3], per membrane dL/d(shear
+ * modulus, local bending, global bending) [3 n_membranes], and the adjoints
+ * of the initial state [4 n_cells] and node positions [3 n_nodes] (with the
+ * membranes' reference areas, volume and MAD, set up from the initial shape,
+ * held fixed); any output may be NULL.  Each segment of checkpoint_every steps is replayed from its
+ * checkpoint; the context is left in the state after the forward run.
+ * Needs constant time steps, strict upwinding, the 4-point cosine IBM kernel
+ * and internal viscosity ratios of 1.  lbfs_adjoint_rewind restores the
+ * state the last forward run started from.  lbfs_adjoint_run: forward +
+ * backward for L = sum over nodes |X - target|^2 (target 3 per node). */
+int lbfs_adjoint_forward(lbfs_ctx *ctx, int n_steps, int checkpoint_every);
+int lbfs_adjoint_backward(lbfs_ctx *ctx, const double *lam_nodes, const double *lam_state, double *grad_body_force,
+                          double *grad_moduli, double *lam_state0, double *lam_nodes0);
+int lbfs_adjoint_rewind(lbfs_ctx *ctx);
+int lbfs_adjoint_run(lbfs_ctx *ctx, int n_steps, int checkpoint_every, const double *target_nodes, double *loss,
+                     double *grad_body_force, double *grad_moduli);
+
 /* host-side membrane meshes: frequency-n geodesic sphere (10 n^2 + 2 nodes) */
 int lbfs_mesh_icosphere(int frequency, double *nodes, int *tri);
 /* Evans-Fung biconcave disc of diameter d0 from a unit sphere (symmetry axis z) */

@@ -1,0 +1,459 @@
+// Discrete adjoint of the coupled time step behind lbfs_adjoint_*: a forward
+// run that keeps the state every K steps, then a reverse sweep over segments
+// of K steps.  Each segment is replayed from its checkpoint keeping every
+// step's start state; each step is then recomputed from its start state (the
+// forward intermediates: interpolated velocities, advected nodes, membrane
+// forces, cell forces, gradients, residual) and reversed by the adjoint
+// kernels.  The replays run the very kernels of lbfs_step, so the recomputed
+// trajectory is the recorded one bit for bit.
+//
+// Also the two setters the parameter gradients refer to: the uniform body
+// force and the membrane moduli.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "adjoint.hpp"
+#include "context.hpp"
+
+using namespace lbfs;
+
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return ctx->fail(LBFS_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+namespace lbfs {
+
+// the device state a step starts from
+struct Snapshot {
+    float4 *st = nullptr, *X = nullptr, *edge_state = nullptr;
+    float *wnode = nullptr;
+    int4 *nbase = nullptr;
+    double *glob = nullptr;
+    long long t = 0;
+};
+
+struct AdjointState {
+    std::vector<void *> allocs;
+    int n_steps = 0, every = 0;
+    bool recorded = false;
+    std::vector<Snapshot> ckpt;   // start states of steps 0, K, 2K, ...
+    std::vector<Snapshot> seg;    // start states of the steps of one segment (slot 0 unused)
+    Snapshot start, end;          // before and after the recorded run
+    // fluid adjoint
+    float4 *lam_st[2] = {nullptr, nullptr};
+    float4 *mu = nullptr, *lcf = nullptr, *ls = nullptr, *ls_acc = nullptr, *lam_g = nullptr;
+    double *fpart = nullptr, *gforce = nullptr;
+    int n_fpart = 0;
+    int *rev_start = nullptr, *rev_list = nullptr;
+    // coupling adjoint
+    float4 *lam_X = nullptr, *lam_U = nullptr, *lam_F = nullptr, *grec = nullptr;
+    float *erec = nullptr, *mrec = nullptr;
+    double *mglob = nullptr, *gmem = nullptr;
+    unsigned int *umax = nullptr;
+};
+
+void adjoint_release(lbfs_ctx *ctx)
+{
+    if (!ctx->adj) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (void *p : ctx->adj->allocs) (void)hipFree(p);
+    delete ctx->adj;
+    ctx->adj = nullptr;
+}
+
+}  // namespace lbfs
+
+namespace {
+
+template <class T>
+int adj_alloc(lbfs_ctx *ctx, T **dst, size_t n)
+{
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, (n > 0 ? n : 1) * sizeof(T)));
+    ctx->adj->allocs.push_back(p);
+    HIPCHK(hipMemset(p, 0, (n > 0 ? n : 1) * sizeof(T)));
+    *dst = (T *)p;
+    return LBFS_OK;
+}
+
+int snap_alloc(lbfs_ctx *ctx, Snapshot &s)
+{
+    int rc;
+    if ((rc = adj_alloc(ctx, &s.st, (size_t)ctx->lay.n_cells))) return rc;
+    if (!ctx->have_membranes) return LBFS_OK;
+    const MemArgs &m = ctx->mem;
+    if ((rc = adj_alloc(ctx, &s.X, (size_t)m.n_nodes))) return rc;
+    if ((rc = adj_alloc(ctx, &s.edge_state, (size_t)m.n_edges))) return rc;
+    if ((rc = adj_alloc(ctx, &s.wnode, 64 * (size_t)m.n_nodes))) return rc;
+    if ((rc = adj_alloc(ctx, &s.nbase, (size_t)m.n_nodes))) return rc;
+    if ((rc = adj_alloc(ctx, &s.glob, 8 * (size_t)m.n_mem))) return rc;
+    return LBFS_OK;
+}
+
+// save: live state -> s; otherwise s -> live state (state into buffer 0)
+int snap_copy(lbfs_ctx *ctx, Snapshot &s, bool save)
+{
+    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    const size_t n = (size_t)ctx->lay.n_cells;
+    if (save) {
+        HIPCHK(hipMemcpyAsync(s.st, ctx->st[ctx->cur], sizeof(float4) * n, k, ctx->stream));
+        s.t = ctx->t_step;
+    } else {
+        HIPCHK(hipMemcpyAsync(ctx->st[0], s.st, sizeof(float4) * n, k, ctx->stream));
+        ctx->cur = 0;
+        ctx->t_step = s.t;
+    }
+    if (!ctx->have_membranes) return LBFS_OK;
+    MemArgs &m = ctx->mem;
+    IbmArgs &g = ctx->ibm;
+    const size_t nn = (size_t)m.n_nodes, ne = (size_t)m.n_edges, nm = (size_t)m.n_mem;
+    if (save) {
+        HIPCHK(hipMemcpyAsync(s.X, m.X, sizeof(float4) * nn, k, ctx->stream));
+        HIPCHK(hipMemcpyAsync(s.edge_state, m.edge_state, sizeof(float4) * ne, k, ctx->stream));
+        HIPCHK(hipMemcpyAsync(s.wnode, g.wnode, sizeof(float) * 64 * nn, k, ctx->stream));
+        HIPCHK(hipMemcpyAsync(s.nbase, g.nbase, sizeof(int4) * nn, k, ctx->stream));
+        HIPCHK(hipMemcpyAsync(s.glob, m.glob, sizeof(double) * 8 * nm, k, ctx->stream));
+    } else {
+        HIPCHK(hipMemcpyAsync(m.X, s.X, sizeof(float4) * nn, k, ctx->stream));
+        HIPCHK(hipMemcpyAsync(m.edge_state, s.edge_state, sizeof(float4) * ne, k, ctx->stream));
+        HIPCHK(hipMemcpyAsync(g.wnode, s.wnode, sizeof(float) * 64 * nn, k, ctx->stream));
+        HIPCHK(hipMemcpyAsync(g.nbase, s.nbase, sizeof(int4) * nn, k, ctx->stream));
+        HIPCHK(hipMemcpyAsync(m.glob, s.glob, sizeof(double) * 8 * nm, k, ctx->stream));
+    }
+    return LBFS_OK;
+}
+
+int save(lbfs_ctx *ctx, Snapshot &s) { return snap_copy(ctx, s, true); }
+int restore(lbfs_ctx *ctx, Snapshot &s) { return snap_copy(ctx, s, false); }
+
+// the conditions under which the adjoint is the derivative of the step
+int adjoint_supported(lbfs_ctx *ctx)
+{
+    if (ctx->args.ts_mode != 0) return ctx->fail(LBFS_E_UNSUPPORTED, "the adjoint needs constant time steps");
+    if (ctx->args.flags & LBFS_FLAG_UPWIND_BAND)
+        return ctx->fail(LBFS_E_UNSUPPORTED, "the adjoint follows strict upwinding (no tie band)");
+    for (int t = 0; t < ctx->lay.n_tiles; t++)
+        if (ctx->lay.tile_cell_start[t + 1] - ctx->lay.tile_cell_start[t] > 1024)
+            return ctx->fail(LBFS_E_UNSUPPORTED, "the adjoint takes tiles of at most 1024 cells");
+    if (ctx->have_membranes) {
+        if (ctx->ibm.kernel != LBFS_IBM_COS4)
+            return ctx->fail(LBFS_E_UNSUPPORTED, "the adjoint supports the 4-point cosine IBM kernel");
+        if (ctx->visc_factor)
+            return ctx->fail(LBFS_E_UNSUPPORTED, "the adjoint needs internal viscosity ratios of 1");
+    }
+    return LBFS_OK;
+}
+
+int adjoint_setup(lbfs_ctx *ctx, int n_steps, int every)
+{
+    if (ctx->adj && ctx->adj->n_steps == n_steps && ctx->adj->every == every) {
+        ctx->adj->recorded = false;  // same shape: the buffers are reused
+        return LBFS_OK;
+    }
+    adjoint_release(ctx);
+    ctx->adj = new AdjointState();
+    AdjointState &A = *ctx->adj;
+    A.n_steps = n_steps;
+    A.every = every;
+    const int n = ctx->lay.n_cells;
+    int rc;
+    const int n_ckpt = (n_steps + every - 1) / every;
+    A.ckpt.resize(n_ckpt);
+    for (auto &s : A.ckpt)
+        if ((rc = snap_alloc(ctx, s))) return rc;
+    A.seg.resize(every);
+    for (int k = 1; k < every && k < n_steps; k++)
+        if ((rc = snap_alloc(ctx, A.seg[k]))) return rc;
+    if ((rc = snap_alloc(ctx, A.end))) return rc;
+    for (auto &b : A.lam_st)
+        if ((rc = adj_alloc(ctx, &b, (size_t)n))) return rc;
+    if ((rc = adj_alloc(ctx, &A.mu, (size_t)n))) return rc;
+    if ((rc = adj_alloc(ctx, &A.lcf, (size_t)n))) return rc;
+    if ((rc = adj_alloc(ctx, &A.ls, (size_t)n))) return rc;
+    if ((rc = adj_alloc(ctx, &A.ls_acc, (size_t)n))) return rc;
+    if ((rc = adj_alloc(ctx, &A.lam_g, 3 * (size_t)n))) return rc;
+    A.n_fpart = (n + 255) / 256;
+    if ((rc = adj_alloc(ctx, &A.fpart, 3 * (size_t)A.n_fpart))) return rc;
+    if ((rc = adj_alloc(ctx, &A.gforce, 3))) return rc;
+    // reverse stencil: for every cell the (cell, slot) pairs whose stencil holds it
+    {
+        const std::vector<int> &sten = ctx->lay.stencil;
+        std::vector<int> start(n + 1, 0);
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < 6; k++) {
+                const int j = sten[(size_t)k * n + i];
+                if (j >= 0) start[j + 1]++;
+            }
+        for (int i = 0; i < n; i++) start[i + 1] += start[i];
+        std::vector<int> list(start[n]), fill(start.begin(), start.end() - 1);
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < 6; k++) {
+                const int j = sten[(size_t)k * n + i];
+                if (j >= 0) list[fill[j]++] = (i << 3) | k;
+            }
+        if ((rc = adj_alloc(ctx, &A.rev_start, start.size()))) return rc;
+        if ((rc = adj_alloc(ctx, &A.rev_list, list.size()))) return rc;
+        HIPCHK(hipMemcpy(A.rev_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(A.rev_list, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice));
+    }
+    if (ctx->have_membranes) {
+        const MemArgs &m = ctx->mem;
+        const size_t nn = m.n_nodes, ne = m.n_edges, nm = m.n_mem;
+        if ((rc = adj_alloc(ctx, &A.lam_X, nn))) return rc;
+        if ((rc = adj_alloc(ctx, &A.lam_U, nn))) return rc;
+        if ((rc = adj_alloc(ctx, &A.lam_F, nn))) return rc;
+        if ((rc = adj_alloc(ctx, &A.grec, 4 * ne))) return rc;
+        if ((rc = adj_alloc(ctx, &A.erec, 24 * ne))) return rc;
+        if ((rc = adj_alloc(ctx, &A.mrec, 12 * ne))) return rc;
+        if ((rc = adj_alloc(ctx, &A.mglob, 8 * nm))) return rc;
+        if ((rc = adj_alloc(ctx, &A.gmem, 4 * nm))) return rc;
+        if ((rc = adj_alloc(ctx, &A.umax, 1))) return rc;
+    }
+    return LBFS_OK;
+}
+
+// the kernels of step n from its start state, keeping the intermediates
+// (residual into ctx->res; the updated state goes to the other buffer)
+int recompute_step(lbfs_ctx *ctx)
+{
+    if (ctx->have_membranes) {
+        HIPCHK(launch_ibm_interp(ctx->mem, ctx->ibm, ctx->st[0], ctx->stream));
+        if (ctx->ibm.binned) {
+            HIPCHK(launch_ibm_scan(ctx->ibm, ctx->stream));
+            HIPCHK(launch_ibm_place(ctx->mem, ctx->ibm, ctx->stream));
+        }
+        HIPCHK(launch_mem_globals(ctx->mem, ctx->stream));
+        HIPCHK(launch_mem_edges(ctx->mem, ctx->stream));
+        HIPCHK(launch_mem_nodes(ctx->mem, ctx->stream));
+        HIPCHK(launch_ibm_spread(ctx->mem, ctx->ibm, ctx->stream));
+    }
+    HIPCHK(launch_gradients(ctx->args, ctx->st[0], ctx->grad, ctx->stream));
+    HIPCHK(launch_flux_update(ctx->args, ctx->st[0], ctx->grad, ctx->st[1], ctx->res, 1, ctx->stream));
+    return LBFS_OK;
+}
+
+int reverse_step(lbfs_ctx *ctx, const Snapshot &S, bool first, int &cur_l)
+{
+    AdjointState &A = *ctx->adj;
+    AdjFluidArgs j{};
+    j.lam_in = A.lam_st[cur_l];
+    j.lam_out = A.lam_st[cur_l ^ 1];
+    j.mu = A.mu; j.lcf = A.lcf; j.ls = A.ls; j.ls_acc = A.ls_acc; j.lam_g = A.lam_g;
+    j.fpart = A.fpart; j.gforce = A.gforce; j.n_fpart = A.n_fpart;
+    j.rev_start = A.rev_start; j.rev_list = A.rev_list;
+    j.acc = ctx->have_membranes ? ctx->ibm.acc : nullptr;
+    j.acc_rw = ctx->have_membranes ? ctx->ibm.acc : nullptr;
+    j.umax = A.umax;
+    HIPCHK(launch_adj_update(ctx->args, j, ctx->st[0], ctx->res, ctx->stream));
+    if (ctx->have_membranes) {
+        AdjMemArgs jm{};
+        jm.Xn = S.X; jm.glob_prev = S.glob;
+        jm.lam_X = A.lam_X; jm.lam_U = A.lam_U; jm.lam_F = A.lam_F;
+        jm.erec = A.erec; jm.grec = A.grec; jm.mrec = A.mrec; jm.mglob = A.mglob; jm.gmem = A.gmem;
+        jm.umax = A.umax; jm.lcf = A.lcf; jm.st_n = ctx->st[0]; jm.acc = ctx->ibm.acc;
+        jm.mad_lag = first ? 0 : 1;
+        HIPCHK(hipMemsetAsync(A.umax, 0, sizeof(unsigned int), ctx->stream));
+        HIPCHK(launch_adj_coupling(ctx->mem, ctx->ibm, jm, ctx->stream));
+    }
+    HIPCHK(launch_adj_flux(ctx->args, j, ctx->st[0], ctx->grad, ctx->stream));
+    HIPCHK(launch_adj_grad(ctx->args, j, ctx->stream));
+    cur_l ^= 1;
+    return LBFS_OK;
+}
+
+}  // namespace
+
+extern "C" int lbfs_adjoint_forward(lbfs_ctx *ctx, int n_steps, int checkpoint_every)
+{
+    if (!ctx || n_steps <= 0 || checkpoint_every <= 0) return LBFS_E_INVALID_ARG;
+    int rc;
+    if ((rc = adjoint_supported(ctx))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int every = std::min(checkpoint_every, n_steps);
+    if ((rc = adjoint_setup(ctx, n_steps, every))) return rc;
+    AdjointState &A = *ctx->adj;
+    A.start = A.ckpt[0];
+    if ((rc = save(ctx, A.ckpt[0]))) return rc;
+    for (int t = 0; t < n_steps; t++) {
+        if ((rc = ctx->enqueue_step())) return rc;
+        if ((t + 1) % every == 0 && t + 1 < n_steps)
+            if ((rc = save(ctx, A.ckpt[(t + 1) / every]))) return rc;
+    }
+    if ((rc = save(ctx, A.end))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    A.recorded = true;
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_adjoint_rewind(lbfs_ctx *ctx)
+{
+    if (!ctx) return LBFS_E_INVALID_ARG;
+    if (!ctx->adj || !ctx->adj->recorded) return ctx->fail(LBFS_E_INVALID_ARG, "no recorded forward run");
+    int rc;
+    if ((rc = restore(ctx, ctx->adj->start))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_adjoint_backward(lbfs_ctx *ctx, const double *lam_nodes, const double *lam_state,
+                                     double *grad_body_force, double *grad_moduli, double *lam_state0,
+                                     double *lam_nodes0)
+{
+    if (!ctx) return LBFS_E_INVALID_ARG;
+    if (!ctx->adj || !ctx->adj->recorded) return ctx->fail(LBFS_E_INVALID_ARG, "no recorded forward run");
+    if ((lam_nodes || lam_nodes0) && !ctx->have_membranes)
+        return ctx->fail(LBFS_E_INVALID_ARG, "node adjoints given without membranes");
+    AdjointState &A = *ctx->adj;
+    const int n = ctx->lay.n_cells;
+    int rc;
+    // seeds: adjoints of the final state and node positions
+    {
+        std::vector<float4> h(n, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        if (lam_state)
+            for (int i = 0; i < n; i++) {
+                const int c = ctx->lay.perm[i];
+                h[i] = make_float4((float)lam_state[4 * (size_t)c], (float)lam_state[4 * (size_t)c + 1],
+                                   (float)lam_state[4 * (size_t)c + 2], (float)lam_state[4 * (size_t)c + 3]);
+            }
+        HIPCHK(hipMemcpy(A.lam_st[0], h.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(A.gforce, 0, 3 * sizeof(double)));
+    }
+    if (ctx->have_membranes) {
+        const int nn = ctx->mem.n_nodes;
+        std::vector<float4> h(nn, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        if (lam_nodes)
+            for (int i = 0; i < nn; i++)
+                h[i] = make_float4((float)lam_nodes[3 * (size_t)i], (float)lam_nodes[3 * (size_t)i + 1],
+                                   (float)lam_nodes[3 * (size_t)i + 2], 0.0f);
+        HIPCHK(hipMemcpy(A.lam_X, h.data(), sizeof(float4) * nn, hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(A.gmem, 0, 4 * sizeof(double) * ctx->mem.n_mem));
+    }
+    int cur_l = 0;
+    const int K = A.every, N = A.n_steps;
+    for (int sg = (N + K - 1) / K - 1; sg >= 0; sg--) {
+        const int s0 = sg * K, s1 = std::min(s0 + K, N);
+        // replay the segment keeping the start state of each of its steps
+        if ((rc = restore(ctx, A.ckpt[sg]))) return rc;
+        for (int t = s0; t + 1 < s1; t++) {
+            if ((rc = ctx->enqueue_step())) return rc;
+            if ((rc = save(ctx, A.seg[t + 1 - s0]))) return rc;
+        }
+        for (int t = s1 - 1; t >= s0; t--) {
+            Snapshot &S = t == s0 ? A.ckpt[sg] : A.seg[t - s0];
+            if ((rc = restore(ctx, S))) return rc;
+            if ((rc = recompute_step(ctx))) return rc;
+            if ((rc = reverse_step(ctx, S, t == 0, cur_l))) return rc;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (grad_body_force) HIPCHK(hipMemcpy(grad_body_force, A.gforce, 3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (grad_moduli && ctx->have_membranes) {
+        const int nm = ctx->mem.n_mem;
+        std::vector<double> g(4 * (size_t)nm);
+        HIPCHK(hipMemcpy(g.data(), A.gmem, sizeof(double) * g.size(), hipMemcpyDeviceToHost));
+        for (int i = 0; i < nm; i++) {
+            // the spring constants are proportional to the shear modulus
+            grad_moduli[3 * (size_t)i] = ctx->mem_shear[i] != 0.0 ? g[4 * (size_t)i] / ctx->mem_shear[i] : 0.0;
+            grad_moduli[3 * (size_t)i + 1] = g[4 * (size_t)i + 1];
+            grad_moduli[3 * (size_t)i + 2] = g[4 * (size_t)i + 2];
+        }
+    }
+    if (lam_state0) {
+        std::vector<float4> h(n);
+        HIPCHK(hipMemcpy(h.data(), A.lam_st[cur_l], sizeof(float4) * n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; i++) {
+            const int c = ctx->lay.perm[i];
+            lam_state0[4 * (size_t)c] = h[i].x;
+            lam_state0[4 * (size_t)c + 1] = h[i].y;
+            lam_state0[4 * (size_t)c + 2] = h[i].z;
+            lam_state0[4 * (size_t)c + 3] = h[i].w;
+        }
+    }
+    if (lam_nodes0) {
+        const int nn = ctx->mem.n_nodes;
+        std::vector<float4> h(nn);
+        HIPCHK(hipMemcpy(h.data(), A.lam_X, sizeof(float4) * nn, hipMemcpyDeviceToHost));
+        for (int i = 0; i < nn; i++) {
+            lam_nodes0[3 * (size_t)i] = h[i].x;
+            lam_nodes0[3 * (size_t)i + 1] = h[i].y;
+            lam_nodes0[3 * (size_t)i + 2] = h[i].z;
+        }
+    }
+    // leave the context as the forward run left it
+    if ((rc = restore(ctx, A.end))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_adjoint_run(lbfs_ctx *ctx, int n_steps, int checkpoint_every, const double *target_nodes,
+                                double *loss, double *grad_body_force, double *grad_moduli)
+{
+    if (!ctx || !target_nodes || !loss) return LBFS_E_INVALID_ARG;
+    if (!ctx->have_membranes) return ctx->fail(LBFS_E_INVALID_ARG, "the node loss needs membranes");
+    int rc;
+    if ((rc = lbfs_adjoint_forward(ctx, n_steps, checkpoint_every))) return rc;
+    const int nn = ctx->mem.n_nodes;
+    std::vector<double> X(3 * (size_t)nn), lam(3 * (size_t)nn);
+    if ((rc = lbfs_get_membrane_nodes(ctx, X.data(), nullptr, nullptr))) return rc;
+    double L = 0.0;
+    for (size_t k = 0; k < X.size(); k++) {
+        const double d = X[k] - target_nodes[k];
+        L += d * d;
+        lam[k] = 2.0 * d;
+    }
+    *loss = L;
+    return lbfs_adjoint_backward(ctx, lam.data(), nullptr, grad_body_force, grad_moduli, nullptr, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// parameters
+// ---------------------------------------------------------------------------
+extern "C" int lbfs_set_body_force(lbfs_ctx *ctx, const double *f)
+{
+    if (!ctx || !f) return LBFS_E_INVALID_ARG;
+    for (int t = 0; t < 3; t++)
+        if (!std::isfinite(f[t])) return ctx->fail(LBFS_E_INVALID_ARG, "non-finite body force");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int n = ctx->lay.n_cells;
+    const int dims = ctx->params.lattice == LBFS_D2Q9 ? 2 : 3;
+    const double rho0 = ctx->params.rho0;
+    std::vector<float> sf(3 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        const int c = ctx->lay.perm[i];
+        for (int t = 0; t < 3; t++)
+            sf[3 * (size_t)i + t] =
+                (float)(f[t] - (t < dims ? rho0 / 3.0 * ctx->lay.closure[3 * (size_t)c + t] / ctx->geo.vol[c] : 0.0));
+    }
+    HIPCHK(hipMemcpy(ctx->static_force, sf.data(), sizeof(float) * sf.size(), hipMemcpyHostToDevice));
+    // every cell holds the static force until the next spreading overwrites it
+    HIPCHK(hipMemcpy(ctx->cell_force, sf.data(), sizeof(float) * sf.size(), hipMemcpyHostToDevice));
+    for (int t = 0; t < 3; t++) ctx->params.body_force[t] = f[t];
+    return LBFS_OK;
+}
+
+extern "C" int lbfs_set_membrane_moduli(lbfs_ctx *ctx, int index, double shear_modulus, double local_bending_modulus,
+                                        double global_bending_modulus)
+{
+    if (!ctx) return LBFS_E_INVALID_ARG;
+    if (!ctx->have_membranes || index < 0 || index >= ctx->mem.n_mem)
+        return ctx->fail(LBFS_E_INVALID_ARG, "no membrane %d", index);
+    if (!std::isfinite(shear_modulus) || !std::isfinite(local_bending_modulus) || !std::isfinite(global_bending_modulus))
+        return ctx->fail(LBFS_E_INVALID_ARG, "non-finite modulus");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    MemParamDev &p = ctx->mem_params[index];
+    p.local_bending = (float)local_bending_modulus;
+    p.global_bending = (float)global_bending_modulus;
+    ctx->mem_shear[index] = shear_modulus;
+    HIPCHK(hipMemcpy((void *)(ctx->mem.params + index), &p, sizeof(MemParamDev), hipMemcpyHostToDevice));
+    const int e0 = ctx->mem_edge_start[index], e1 = ctx->mem_edge_start[index + 1];
+    for (int e = e0; e < e1; e++) ctx->mem_edge_const[e].y = (float)(ctx->mem_ks_unit[e] * shear_modulus);
+    HIPCHK(hipMemcpy((void *)(ctx->mem.edge_const + e0), ctx->mem_edge_const.data() + e0, sizeof(float4) * (e1 - e0),
+                     hipMemcpyHostToDevice));
+    return LBFS_OK;
+}

@@ -12,6 +12,11 @@
 #include "lbfs_abi.h"
 #include "mesh_host.hpp"
 
+namespace lbfs {
+struct AdjointState;  // adjoint.cpp
+void adjoint_release(lbfs_ctx *ctx);
+}  // namespace lbfs
+
 struct lbfs_ctx {
     std::string err;
     int device = 0;
@@ -43,10 +48,20 @@ struct lbfs_ctx {
     double *mem_energy = nullptr;   // [7 n_mem] output of lbfs_get_membrane_energies
     int ibm_binned_option = -1;     // LBFS_OPT_IBM_BINNED: -1 by node count, 0 per node, 1 per bin
     std::vector<double> mem_origin;
+    // host copies behind lbfs_set_membrane_moduli: per-membrane parameters,
+    // per-edge spring constant per unit shear modulus (FP64), edge ranges
+    std::vector<lbfs::MemParamDev> mem_params;
+    std::vector<float4> mem_edge_const;
+    std::vector<double> mem_ks_unit;
+    std::vector<int> mem_edge_start;
+    std::vector<double> mem_shear;
     std::vector<void *> coupling_allocs;
     float *visc_factor = nullptr;   // [n] internal viscosity field, null while every ratio is 1
     float delta_h = 0.1f;           // smoothing width of the field (unstructured_mesh.cpp:263-264)
     long long t_step = 0;           // steps since the membranes were attached
+
+    // checkpoints and buffers of the discrete adjoint (lbfs_adjoint_*)
+    lbfs::AdjointState *adj = nullptr;
 
     // hipGraph of two consecutive steps starting from cur == 0
     bool use_graph = true;

@@ -33,6 +33,7 @@ static int cp_alloc(lbfs_ctx *ctx, T **dst, const T *src, size_t n)
 
 void lbfs_ctx::release_extensions()
 {
+    adjoint_release(this);
     for (void *p : coupling_allocs) (void)hipFree(p);
     coupling_allocs.clear();
     have_membranes = false;
@@ -195,6 +196,17 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     m.edge_const = dec; m.edge_area0 = dea; m.edge_state = des; m.edge_force = def;
     m.inc_start = dis; m.inc = dinc; m.params = dpar; m.glob = dgl;
     ctx->mem_node_start = mem_node_start;
+    ctx->mem_params = params;
+    ctx->mem_edge_const = edge_const;
+    ctx->mem_edge_start = mem_edge_start;
+    ctx->mem_shear.assign(count, 0.0);
+    ctx->mem_ks_unit.assign(ne, 0.0);
+    for (int i = 0; i < count; i++) {
+        ctx->mem_shear[i] = mem[i].shear_modulus;
+        for (int e = 0; e < ms[i].n_edges; e++)
+            ctx->mem_ks_unit[mem_edge_start[i] + e] =
+                mem[i].shear_modulus != 0.0 ? ms[i].spring_constant[e] / mem[i].shear_modulus : 0.0;
+    }
     ctx->mem_origin.assign(3 * (size_t)count, 0.0);
     for (int i = 0; i < count; i++) {
         ctx->mem_origin[3 * (size_t)i] = origin[i].x;

@@ -1,0 +1,367 @@
+// Adjoint of the IBM coupling and of the spring-network membrane for gfx950
+// (4-point cosine kernel).  In reverse order of the forward step:
+//
+//   k_adj_spread      per node (one wavefront, one lane per stencil cell) at
+//                     X^n+1: adjoint of the node force through spreading
+//                     (immersed_boundary_method.cu:947-1024), and of X^n+1
+//                     through the normalised weights
+//   k_adj_mem_edges   per edge: the edge force (spring_network.cu:613-971) in
+//                     forward-mode, one input direction at a time, contracted
+//                     with the adjoint of the four node forces
+//   k_adj_mem_reduce  per membrane (one wavefront): adjoints of the global
+//                     volume/area/MAD coefficients and the modulus gradients
+//   k_adj_mem_geo     per edge: volume and area constraint terms at X^n+1
+//   k_adj_mem_mad     per edge: the MAD term, a function of X^n (one step lag)
+//   k_adj_mem_nodes   per node: fixed-order sum of its edge slots; adds the
+//                     advection X^n+1 = X^n + dt U^n in reverse
+//   k_adj_interp      per node at X^n: adjoint of X^n through the weights of
+//                     the interpolation (immersed_boundary_method.cu:14-97),
+//                     and the node velocity adjoint spread onto the cells in
+//                     64-bit fixed point (order independent)
+#include <hip/hip_runtime.h>
+
+#include "adjoint.hpp"
+#include "coupling.hpp"
+#include "ibm_device.hpp"
+#include "membrane_force.hpp"
+
+namespace lbfs {
+
+namespace {
+
+// stencil cell of lane `lane` for a node at P: weight, its gradient and the
+// internal cell id (-1 outside a wall)
+__device__ __forceinline__ int stencil_cell(const IbmArgs &g, const IbmBlock &B, const float P[3], int lane, float &wt,
+                                            float dw[3])
+{
+    float xr[3];
+    int base[3], off[3];
+    node_base(g, B, P, xr, base);
+    lane_offset(4, lane, off);
+    const int idx[3] = {base[0] + off[0], base[1] + off[1], base[2] + off[2]};
+    int w[3];
+    wt = 0.0f;
+    dw[0] = dw[1] = dw[2] = 0.0f;
+    if (!(wrap_index(B, 0, idx[0], w[0]) && wrap_index(B, 1, idx[1], w[1]) && wrap_index(B, 2, idx[2], w[2])))
+        return -1;
+    const int ic = cell_internal(g, B, w);
+    wt = cos4_weight_grad(B, g.cen[ic], idx, w, P, dw);
+    return ic;
+}
+
+__device__ __forceinline__ float3 f3of(float4 v) { return make_float3(v.x, v.y, v.z); }
+
+// Gradient of sum_c w_c b_c for normalised weights w_c = wh_c / sum wh:
+// (sum_c dwh_c b_c - (sum_c dwh_c) (sum_c w_c b_c)) / sum wh
+__device__ __forceinline__ void weight_term(float wh, const float dw[3], float inv, float b, float out[3])
+{
+    const float sb = wave_sum(wh * inv * b);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float s1 = wave_sum(dw[a] * b), s2 = wave_sum(dw[a]);
+        out[a] = (s1 - s2 * sb) * inv;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// spreading: f_c = static_c + (1 / V_c) sum_n w_nc F_n
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_adj_spread(MemArgs m, IbmArgs g, AdjMemArgs j)
+{
+    const int lane = threadIdx.x & 63;
+    const int nd = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (nd >= m.n_nodes) return;
+    const IbmBlock B = g.blk[g.node_block[nd]];
+    const float4 o = m.origin[m.node_mem[nd]], xl = m.X[nd];
+    const float P[3] = {o.x + xl.x, o.y + xl.y, o.z + xl.z};
+    float wh, dw[3];
+    const int ic = stencil_cell(g, B, P, lane, wh, dw);
+    float3 a = make_float3(0.0f, 0.0f, 0.0f);
+    if (ic >= 0) {
+        const float4 l = j.lcf[ic];
+        const float iv = 1.0f / g.cen[ic].w;
+        a = make_float3(l.x * iv, l.y * iv, l.z * iv);
+    }
+    const float ws = wave_sum(wh);
+    const float inv = ws > 0.0f ? 1.0f / ws : 0.0f;
+    const float w = wh * inv;
+    const float lfx = wave_sum(w * a.x), lfy = wave_sum(w * a.y), lfz = wave_sum(w * a.z);
+    const float4 F = m.F[nd];
+    const float b = F.x * a.x + F.y * a.y + F.z * a.z;
+    float lx[3];
+    weight_term(wh, dw, inv, b, lx);
+    if (lane == 0) {
+        j.lam_F[nd] = make_float4(lfx, lfy, lfz, 0.0f);
+        const float4 l = j.lam_X[nd];
+        j.lam_X[nd] = make_float4(l.x + lx[0], l.y + lx[1], l.z + lx[2], 0.0f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// membrane
+// ---------------------------------------------------------------------------
+namespace {
+
+// the 24 differentiated inputs of an edge, in the order of the adjoint record:
+// A B C D (12), Ua Uc (6), vol/area/MAD coefficients (3), ks, local and
+// global bending moduli (3)
+__device__ __forceinline__ memf::EdgeIn<memf::Dual> seeded(const float x[24], int d)
+{
+    using memf::Dual;
+    Dual y[24];
+#pragma unroll
+    for (int k = 0; k < 24; k++) y[k] = Dual(x[k], k == d ? 1.0f : 0.0f);
+    memf::EdgeIn<Dual> in;
+    in.A = {y[0], y[1], y[2]};
+    in.B = {y[3], y[4], y[5]};
+    in.C = {y[6], y[7], y[8]};
+    in.D = {y[9], y[10], y[11]};
+    in.Ua = {y[12], y[13], y[14]};
+    in.Uc = {y[15], y[16], y[17]};
+    in.vol_coef = y[18];
+    in.area_coef = y[19];
+    in.mad_coef = y[20];
+    in.ks = y[21];
+    in.lb = y[22];
+    in.gb = y[23];
+    return in;
+}
+
+__device__ __forceinline__ void edge_inputs(const MemArgs &m, const float4 *X, int e, const MemParamDev &p,
+                                            float x[24])
+{
+    const int4 q = m.edges[e];
+    const float4 xa = X[q.x], xb = X[q.y], xc = X[q.z], xd = X[q.w], va = m.U[q.x], vc = m.U[q.z];
+    const float v[18] = {xa.x, xa.y, xa.z, xb.x, xb.y, xb.z, xc.x, xc.y, xc.z,
+                         xd.x, xd.y, xd.z, va.x, va.y, va.z, vc.x, vc.y, vc.z};
+#pragma unroll
+    for (int k = 0; k < 18; k++) x[k] = v[k];
+    const double *gl = m.glob + 8 * (size_t)m.edge_mem[e];
+    x[18] = (float)(-(double)p.volume_modulus * (gl[0] / gl[3] - 1.0) / 6.0);
+    x[19] = (float)(-(double)p.global_area_modulus * (gl[1] / gl[4] - 1.0));
+    x[20] = (float)(-2.0 * (double)p.global_bending * (gl[2] - gl[5]) / gl[4]);
+    x[21] = m.edge_const[e].y;
+    x[22] = p.local_bending;
+    x[23] = p.global_bending;
+}
+
+__device__ __forceinline__ memf::EdgeConst edge_consts(const MemArgs &m, const MemParamDev &p, int e)
+{
+    const float4 ec = m.edge_const[e];
+    const float2 a0 = m.edge_area0[e];
+    return {ec.x, ec.z, a0.x, a0.y, p.local_area_modulus, p.viscosity, p.spring_c};
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(64) void k_adj_mem_edges(MemArgs m, AdjMemArgs j)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.n_edges) return;
+    const MemParamDev p = m.params[m.edge_mem[e]];
+    float x[24];
+    edge_inputs(m, m.X, e, p, x);
+    const int4 q = m.edges[e];
+    const float4 l0 = j.lam_F[q.x], l1 = j.lam_F[q.y], l2 = j.lam_F[q.z], l3 = j.lam_F[q.w];
+    const memf::EdgeConst kc = edge_consts(m, p, e);
+#pragma unroll 1
+    for (int d = 0; d < 24; d++) {
+        memf::EdgeOut<memf::Dual> out;
+        memf::edge_force(seeded(x, d), kc, out);
+        const float s = l0.x * out.F[0].x.d + l0.y * out.F[0].y.d + l0.z * out.F[0].z.d +
+                        l1.x * out.F[1].x.d + l1.y * out.F[1].y.d + l1.z * out.F[1].z.d +
+                        l2.x * out.F[2].x.d + l2.y * out.F[2].y.d + l2.z * out.F[2].z.d +
+                        l3.x * out.F[3].x.d + l3.y * out.F[3].y.d + l3.z * out.F[3].z.d;
+        j.erec[24 * (size_t)e + d] = s;
+    }
+}
+
+// Per membrane: sums of the coefficient and modulus adjoints over its edges
+// (FP64, fixed order), chained through the FP64 coefficient formulas:
+//   vol  = -k_v (V / V0 - 1) / 6     -> lambda_V   = l_vol (-k_v / (6 V0))
+//   area = -k_A (A / A0 - 1)          -> lambda_A   = l_area (-k_A / A0)
+//   mad  = -2 k_gb (MAD - MAD0) / A0  -> lambda_MAD = l_mad (-2 k_gb / A0),
+//                                        d/dk_gb   = l_mad (-2 (MAD - MAD0) / A0)
+__global__ __launch_bounds__(64) void k_adj_mem_reduce(MemArgs m, AdjMemArgs j)
+{
+    const int mem = blockIdx.x, lane = threadIdx.x;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int e = m.mem_edge_start[mem] + lane; e < m.mem_edge_start[mem + 1]; e += 64) {
+        const float *r = j.erec + 24 * (size_t)e + 18;
+        s[0] += r[0];
+        s[1] += r[1];
+        s[2] += r[2];
+        s[3] += (double)r[3] * (double)m.edge_const[e].y;
+        s[4] += r[4];
+        s[5] += r[5];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < 6; k++) s[k] += __shfl_xor(s[k], off, 64);
+    if (lane == 0) {
+        const MemParamDev p = m.params[mem];
+        const double *gl = m.glob + 8 * (size_t)mem;
+        const double *gp = j.glob_prev + 8 * (size_t)mem;
+        double *mg = j.mglob + 8 * (size_t)mem;
+        const double lmad = s[2] * (-2.0 * (double)p.global_bending / gl[4]);
+        mg[0] = s[0] * (-(double)p.volume_modulus / (6.0 * gl[3]));
+        mg[1] = s[1] * (-(double)p.global_area_modulus / gl[4]);
+        mg[2] = lmad;
+        // MAD = M / A(X^n), M = sum over the MAD edges of kappa (|N1| + |N2|) / 3
+        mg[3] = lmad / gp[6];
+        mg[4] = lmad * gl[2] / gp[6];
+        double *gm = j.gmem + 4 * (size_t)mem;
+        gm[0] += s[3];
+        gm[1] += s[4];
+        gm[2] += s[5] + s[2] * (-2.0 * (gl[2] - gl[5]) / gl[4]);
+    }
+}
+
+// lambda_V dV/dX + lambda_A dA/dX at X^n+1, per edge slot
+__global__ __launch_bounds__(128) void k_adj_mem_geo(MemArgs m, AdjMemArgs j)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.n_edges) return;
+    const double *mg = j.mglob + 8 * (size_t)m.edge_mem[e];
+    const float lv = (float)mg[0], la = (float)mg[1];
+    const int4 q = m.edges[e];
+    const float4 xa = m.X[q.x], xb = m.X[q.y], xc = m.X[q.z], xd = m.X[q.w];
+    memf::EdgeGeometry<float> g;
+    memf::edge_geometry<float>({xa.x, xa.y, xa.z}, {xb.x, xb.y, xb.z}, {xc.x, xc.y, xc.z}, {xd.x, xd.y, xd.z}, g);
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+        j.grec[4 * (size_t)e + s] = make_float4(lv * g.d_vol[3 * s] + la * g.d_area[3 * s],
+                                                lv * g.d_vol[3 * s + 1] + la * g.d_area[3 * s + 1],
+                                                lv * g.d_vol[3 * s + 2] + la * g.d_area[3 * s + 2], 0.0f);
+}
+
+// gradient at X^n of c1 [MAD edge] kappa (|N1| + |N2|) / 3 - c2 (|N1| + |N2|) / 6,
+// c1 = lambda_MAD / A(X^n), c2 = lambda_MAD MAD / A(X^n); the sum of the second
+// term over the edges is c2 A.  The first step's MAD comes from the set-up
+// (a constant of the run, like the reference areas and volume)
+__global__ __launch_bounds__(64) void k_adj_mem_mad(MemArgs m, AdjMemArgs j)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.n_edges) return;
+    const int mem = m.edge_mem[e];
+    const double *mg = j.mglob + 8 * (size_t)mem;
+    float *rec = j.mrec + 12 * (size_t)e;
+    if (!j.mad_lag || mg[2] == 0.0) {
+#pragma unroll
+        for (int d = 0; d < 12; d++) rec[d] = 0.0f;
+        return;
+    }
+    const MemParamDev p = m.params[mem];
+    const bool in_mad = e - m.mem_edge_start[mem] < p.n_mad;
+    const float c1 = in_mad ? (float)mg[3] : 0.0f, c2 = (float)mg[4];
+    float x[24];
+    edge_inputs(m, j.Xn, e, p, x);
+    const memf::EdgeConst kc = edge_consts(m, p, e);
+#pragma unroll 1
+    for (int d = 0; d < 12; d++) {
+        memf::EdgeOut<memf::Dual> out;
+        memf::edge_force(seeded(x, d), kc, out);
+        const memf::Dual nn = out.n1 + out.n2;
+        rec[d] = c1 * (out.kappa.d * nn.v + out.kappa.v * nn.d) * (1.0f / 3.0f) - c2 * nn.d * (1.0f / 6.0f);
+    }
+}
+
+// Per node (eight lanes, fixed order): lambda_X^n+1 is complete once the
+// edge, constraint and spreading shares are in; then the advection
+// X^n+1 = X^n + dt U^n in reverse: lambda_U^n += dt lambda_X^n+1,
+// lambda_X^n = lambda_X^n+1 + the MAD share (the interpolation weights add theirs next)
+__global__ __launch_bounds__(256) void k_adj_mem_nodes(MemArgs m, IbmArgs g, AdjMemArgs j)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nd = gid >> 3, sub = gid & 7;
+    const bool live = nd < m.n_nodes;
+    float x[3] = {0, 0, 0}, u[3] = {0, 0, 0}, xm[3] = {0, 0, 0};
+    if (live)
+        for (int k = m.inc_start[nd] + sub; k < m.inc_start[nd + 1]; k += 8) {
+            const int r = m.inc[k];
+            const int e = r >> 2, s = r & 3;
+            const float *er = j.erec + 24 * (size_t)e;
+            const float4 gr = j.grec[r];
+            const float *mr = j.mrec + 12 * (size_t)e + 3 * s;
+            x[0] += er[3 * s] + gr.x;
+            x[1] += er[3 * s + 1] + gr.y;
+            x[2] += er[3 * s + 2] + gr.z;
+            xm[0] += mr[0]; xm[1] += mr[1]; xm[2] += mr[2];
+            if (s == 0 || s == 2) {
+                const int b = s == 0 ? 12 : 15;
+                u[0] += er[b]; u[1] += er[b + 1]; u[2] += er[b + 2];
+            }
+        }
+#pragma unroll
+    for (int off = 4; off > 0; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            x[c] += __shfl_xor(x[c], off, 64);
+            u[c] += __shfl_xor(u[c], off, 64);
+            xm[c] += __shfl_xor(xm[c], off, 64);
+        }
+    if (live && sub == 0) {
+        const float4 l = j.lam_X[nd];
+        const float lx[3] = {l.x + x[0], l.y + x[1], l.z + x[2]};
+        const float lu[3] = {u[0] + g.dt * lx[0], u[1] + g.dt * lx[1], u[2] + g.dt * lx[2]};
+        j.lam_U[nd] = make_float4(lu[0], lu[1], lu[2], 0.0f);
+        j.lam_X[nd] = make_float4(lx[0] + xm[0], lx[1] + xm[1], lx[2] + xm[2], 0.0f);
+        const float mx = fmaxf(fabsf(lu[0]), fmaxf(fabsf(lu[1]), fabsf(lu[2])));
+        atomicMax(j.umax, __float_as_uint(mx));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// interpolation: U_n = sum_c w_nc(X^n) u_c
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_adj_interp(MemArgs m, IbmArgs g, AdjMemArgs j)
+{
+    const int lane = threadIdx.x & 63;
+    const int nd = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (nd >= m.n_nodes) return;
+    const IbmBlock B = g.blk[g.node_block[nd]];
+    const float4 o = m.origin[m.node_mem[nd]], xl = j.Xn[nd];
+    const float P[3] = {o.x + xl.x, o.y + xl.y, o.z + xl.z};
+    float wh, dw[3];
+    const int ic = stencil_cell(g, B, P, lane, wh, dw);
+    const float4 lu = j.lam_U[nd];
+    float b = 0.0f;
+    if (ic >= 0) {
+        const float4 s = j.st_n[ic];
+        b = lu.x * s.y + lu.y * s.z + lu.z * s.w;
+    }
+    const float ws = wave_sum(wh);
+    const float inv = ws > 0.0f ? 1.0f / ws : 0.0f;
+    float lx[3];
+    weight_term(wh, dw, inv, b, lx);
+    if (lane == 0) {
+        const float4 l = j.lam_X[nd];
+        j.lam_X[nd] = make_float4(l.x + lx[0], l.y + lx[1], l.z + lx[2], 0.0f);
+    }
+    const double scale = adj_fix_scale(*j.umax);
+    const float w = wh * inv;
+    if (ic >= 0 && w != 0.0f && scale > 0.0) {
+        unsigned long long *acc = j.acc + 3 * (size_t)ic;
+        atomicAdd(acc, (unsigned long long)llrint((double)(w * lu.x) * scale));
+        atomicAdd(acc + 1, (unsigned long long)llrint((double)(w * lu.y) * scale));
+        atomicAdd(acc + 2, (unsigned long long)llrint((double)(w * lu.z) * scale));
+    }
+}
+
+hipError_t launch_adj_coupling(const MemArgs &m, const IbmArgs &g, const AdjMemArgs &j, hipStream_t s)
+{
+    const int nb4 = (m.n_nodes + 3) / 4;
+    hipLaunchKernelGGL(k_adj_spread, dim3(nb4), dim3(256), 0, s, m, g, j);
+    hipLaunchKernelGGL(k_adj_mem_edges, dim3((m.n_edges + 63) / 64), dim3(64), 0, s, m, j);
+    hipLaunchKernelGGL(k_adj_mem_reduce, dim3(m.n_mem), dim3(64), 0, s, m, j);
+    hipLaunchKernelGGL(k_adj_mem_geo, dim3((m.n_edges + 127) / 128), dim3(128), 0, s, m, j);
+    hipLaunchKernelGGL(k_adj_mem_mad, dim3((m.n_edges + 63) / 64), dim3(64), 0, s, m, j);
+    hipLaunchKernelGGL(k_adj_mem_nodes, dim3((8 * m.n_nodes + 255) / 256), dim3(256), 0, s, m, g, j);
+    hipLaunchKernelGGL(k_adj_interp, dim3(nb4), dim3(256), 0, s, m, g, j);
+    return hipGetLastError();
+}
+
+}  // namespace lbfs

@@ -109,6 +109,26 @@ __device__ __forceinline__ float cos4_phi(float r)
     return 0.5f * fmaf(-sn, sn, 1.0f);
 }
 
+// d phi / d r of the same polynomial: phi = (1 - sn^2) / 2, sn(x) as above,
+// so phi' = -sn sn'(x) pi / 4 with sn' the term-by-term derivative
+__device__ __forceinline__ float cos4_dphi(float r)
+{
+    const float x = r * (PI_F * 0.25f), x2 = x * x;
+    float p = -2.5052108385e-8f;
+    p = fmaf(p, x2, 2.7557319224e-6f);
+    p = fmaf(p, x2, -1.9841269841e-4f);
+    p = fmaf(p, x2, 8.3333333333e-3f);
+    p = fmaf(p, x2, -1.6666666667e-1f);
+    const float sn = fmaf(p * x2, x, x);
+    float d = 11.0f * -2.5052108385e-8f;
+    d = fmaf(d, x2, 9.0f * 2.7557319224e-6f);
+    d = fmaf(d, x2, 7.0f * -1.9841269841e-4f);
+    d = fmaf(d, x2, 5.0f * 8.3333333333e-3f);
+    d = fmaf(d, x2, 3.0f * -1.6666666667e-1f);
+    const float dsn = fmaf(d, x2, 1.0f);
+    return -sn * dsn * (PI_F * 0.25f);
+}
+
 __device__ __forceinline__ float cell_weight(const IbmArgs &g, const IbmBlock &B, const int idx[3], const int w[3],
                                              int internal, const float P[3], const float xr[3])
 {
@@ -126,6 +146,30 @@ __device__ __forceinline__ float cell_weight(const IbmArgs &g, const IbmBlock &B
             wt *= fabsf(r) < 2.0f ? cos4_phi(r) : 0.0f;
         }
     }
+    return wt;
+}
+
+// unnormalised 4-point cosine weight of a cell and its gradient with respect
+// to the node position P (r = (x_c - P) / h, so dr/dP = -1/h)
+__device__ __forceinline__ float cos4_weight_grad(const IbmBlock &B, const float4 c, const int idx[3], const int w[3],
+                                                  const float P[3], float dw[3])
+{
+    const float cc[3] = {c.x, c.y, c.z};
+    float ph[3], dph[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float xc = cc[a] + (float)(idx[a] - w[a]) * (B.L[a] / (float)B.n[a]);
+        const float r = (xc - P[a]) * B.inv_h;
+        const bool in = fabsf(r) < 2.0f;
+        ph[a] = in ? cos4_phi(r) : 0.0f;
+        dph[a] = in ? -cos4_dphi(r) * B.inv_h : 0.0f;
+    }
+    dw[0] = dph[0] * ph[1] * ph[2];
+    dw[1] = ph[0] * dph[1] * ph[2];
+    dw[2] = ph[0] * ph[1] * dph[2];
+    float wt = 1.0f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) wt *= ph[a];
     return wt;
 }
 

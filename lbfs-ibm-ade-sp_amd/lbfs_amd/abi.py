@@ -152,6 +152,12 @@ def load():
             "lbfs_stage_end_step": (C.c_int, [vp]),
             "lbfs_membrane_mesh_read": (C.c_int, [C.c_char_p, C.c_int, _dp, _dp, _ip, _ip]),
             "lbfs_membrane_mesh_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip]),
+            "lbfs_set_body_force": (C.c_int, [vp, _dp]),
+            "lbfs_set_membrane_moduli": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double]),
+            "lbfs_adjoint_forward": (C.c_int, [vp, C.c_int, C.c_int]),
+            "lbfs_adjoint_backward": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp, _dp]),
+            "lbfs_adjoint_rewind": (C.c_int, [vp]),
+            "lbfs_adjoint_run": (C.c_int, [vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
             "lbfs_polymesh_counts": (C.c_int, [C.c_char_p, _ip, _ip, _ip, _ip, _ip]),
             "lbfs_polymesh_read": (C.c_int, [C.c_char_p, _dp, _ip, _ip, _ip, _ip, _dp]),
             "lbfs_polymesh_patch": (C.c_int, [C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int]),
@@ -652,6 +658,48 @@ class Solver:
         cnt = C.c_int()
         check(load().lbfs_model_bytes(self.ctx, max_count, ptr(b), C.byref(cnt)), self.ctx)
         return b[: cnt.value].copy()
+
+    def set_body_force(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        check(load().lbfs_set_body_force(self.ctx, ptr(f)), self.ctx)
+
+    def set_membrane_moduli(self, index, shear, local_bending, global_bending):
+        check(load().lbfs_set_membrane_moduli(self.ctx, int(index), float(shear), float(local_bending),
+                                              float(global_bending)), self.ctx)
+
+    def adjoint_forward(self, n_steps, checkpoint_every):
+        """Run n_steps steps keeping checkpoints every checkpoint_every steps."""
+        check(load().lbfs_adjoint_forward(self.ctx, int(n_steps), int(checkpoint_every)), self.ctx)
+
+    def adjoint_rewind(self):
+        check(load().lbfs_adjoint_rewind(self.ctx), self.ctx)
+
+    def adjoint_backward(self, lam_nodes=None, lam_state=None):
+        """Reverse sweep of the last adjoint_forward for the loss adjoints of
+        the final node positions ([n_nodes, 3]) and state ([n_cells, 4]:
+        rho, u, v, w).  Returns body_force [3], moduli [n_mem, 3] (shear,
+        local bending, global bending), state0 [n_cells, 4], nodes0 [n_nodes, 3]."""
+        n_mem = len(getattr(self, "_membranes", []))
+        ln = None if lam_nodes is None else np.ascontiguousarray(lam_nodes, dtype=np.float64).reshape(-1)
+        ls = None if lam_state is None else np.ascontiguousarray(lam_state, dtype=np.float64).reshape(-1)
+        gb = np.zeros(3)
+        gm = np.zeros(3 * max(n_mem, 1))
+        s0 = np.zeros(4 * self.n)
+        x0 = np.zeros(3 * max(getattr(self, "n_mem_nodes", 0), 1))
+        check(load().lbfs_adjoint_backward(self.ctx, ptr(ln), ptr(ls), ptr(gb), ptr(gm) if n_mem else None,
+                                           ptr(s0), ptr(x0) if n_mem else None), self.ctx)
+        return {"body_force": gb, "moduli": gm[: 3 * n_mem].reshape(-1, 3), "state0": s0.reshape(-1, 4),
+                "nodes0": x0[: 3 * getattr(self, "n_mem_nodes", 0)].reshape(-1, 3)}
+
+    def adjoint_run(self, n_steps, checkpoint_every, target_nodes):
+        """Loss sum |X - target|^2 after n_steps and its gradients."""
+        t = np.ascontiguousarray(target_nodes, dtype=np.float64).reshape(-1)
+        loss = C.c_double()
+        gb = np.zeros(3)
+        gm = np.zeros(3 * len(self._membranes))
+        check(load().lbfs_adjoint_run(self.ctx, int(n_steps), int(checkpoint_every), ptr(t), C.byref(loss), ptr(gb),
+                                      ptr(gm)), self.ctx)
+        return loss.value, gb, gm.reshape(-1, 3)
 
     def diagnostics(self, ref_rho=1.0, ref_u=0.01):
         d = Diag()
