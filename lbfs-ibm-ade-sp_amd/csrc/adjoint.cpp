@@ -1,11 +1,12 @@
 // Discrete adjoint of the coupled time step behind lbfs_adjoint_*: a forward
 // run that keeps the state every K steps, then a reverse sweep over segments
-// of K steps.  Each segment is replayed from its checkpoint keeping every
-// step's start state; each step is then recomputed from its start state (the
-// forward intermediates: interpolated velocities, advected nodes, membrane
-// forces, cell forces, gradients, residual) and reversed by the adjoint
-// kernels.  The replays run the very kernels of lbfs_step, so the recomputed
-// trajectory is the recorded one bit for bit.
+// of K steps.  Each segment is replayed from its checkpoint, keeping what the
+// reverse step needs of every step (start state, gradients, residual, cell
+// force, nodes before and after advection, node velocities and forces,
+// membrane globals), written straight into per-step records; then the
+// segment's steps are reversed by the adjoint kernels, last first.  The
+// replays run the very kernels of lbfs_step on the same inputs, so the
+// replayed trajectory is the recorded one bit for bit.
 //
 // Also the two setters the parameter gradients refer to: the uniform body
 // force and the membrane moduli.
@@ -38,23 +39,38 @@ struct Snapshot {
     long long t = 0;
 };
 
+// what the reverse of one step reads: the forward quantities of that step
+struct StepRec {
+    float4 *st = nullptr;       // state at the start (aliases the checkpoint for a segment's first step)
+    float4 *grad = nullptr;     // gradients of st
+    float4 *res = nullptr;      // flux residual (before forces)
+    float4 *mom = nullptr;      // face moments S0, S1 per tile face
+    float *cf = nullptr;        // cell force of the step [3 n]
+    float4 *X = nullptr, *Xn1 = nullptr, *U = nullptr, *F = nullptr;  // X^n, X^n+1, U^n, F^n
+    double *glob_prev = nullptr, *glob = nullptr;  // globals at the start, after the step's sums
+    // X and glob_prev alias the previous record's Xn1 and glob (the checkpoint's for the first)
+};
+
 struct AdjointState {
     std::vector<void *> allocs;
     int n_steps = 0, every = 0;
+    int max_tile_cells = 0;
     bool recorded = false;
     std::vector<Snapshot> ckpt;   // start states of steps 0, K, 2K, ...
-    std::vector<Snapshot> seg;    // start states of the steps of one segment (slot 0 unused)
+    std::vector<StepRec> rec;     // the steps of one segment
     Snapshot start, end;          // before and after the recorded run
     // fluid adjoint
     float4 *lam_st[2] = {nullptr, nullptr};
     float4 *mu = nullptr, *lcf = nullptr, *ls = nullptr, *ls_acc = nullptr, *lam_g = nullptr;
     double *fpart = nullptr, *gforce = nullptr;
     int n_fpart = 0;
-    int *rev_start = nullptr, *rev_list = nullptr;
+    float4 *rev_w = nullptr;
     // coupling adjoint
     float4 *lam_X = nullptr, *lam_U = nullptr, *lam_F = nullptr, *grec = nullptr;
     float *erec = nullptr, *mrec = nullptr;
-    double *mglob = nullptr, *gmem = nullptr;
+    double *mglob = nullptr, *gmem = nullptr, *epart = nullptr;
+    int *echunk = nullptr, *echunk_start = nullptr;
+    int n_echunks = 0;
     unsigned int *umax = nullptr;
 };
 
@@ -161,15 +177,30 @@ int adjoint_setup(lbfs_ctx *ctx, int n_steps, int every)
     AdjointState &A = *ctx->adj;
     A.n_steps = n_steps;
     A.every = every;
+    for (int t = 0; t < ctx->lay.n_tiles; t++)
+        A.max_tile_cells = std::max(A.max_tile_cells, ctx->lay.tile_cell_start[t + 1] - ctx->lay.tile_cell_start[t]);
     const int n = ctx->lay.n_cells;
     int rc;
     const int n_ckpt = (n_steps + every - 1) / every;
     A.ckpt.resize(n_ckpt);
     for (auto &s : A.ckpt)
         if ((rc = snap_alloc(ctx, s))) return rc;
-    A.seg.resize(every);
-    for (int k = 1; k < every && k < n_steps; k++)
-        if ((rc = snap_alloc(ctx, A.seg[k]))) return rc;
+    A.rec.resize(every);
+    for (int k = 0; k < every; k++) {
+        StepRec &r = A.rec[k];
+        if (k > 0 && (rc = adj_alloc(ctx, &r.st, (size_t)n))) return rc;
+        if ((rc = adj_alloc(ctx, &r.grad, 3 * (size_t)n))) return rc;
+        if ((rc = adj_alloc(ctx, &r.res, (size_t)n))) return rc;
+        if ((rc = adj_alloc(ctx, &r.mom, ctx->lay.fA.size() / 4))) return rc;
+        if (ctx->have_membranes && (rc = adj_alloc(ctx, &r.cf, 3 * (size_t)n))) return rc;
+        if (ctx->have_membranes) {
+            const size_t nn = ctx->mem.n_nodes, nm = ctx->mem.n_mem;
+            if ((rc = adj_alloc(ctx, &r.Xn1, nn))) return rc;
+            if ((rc = adj_alloc(ctx, &r.U, nn))) return rc;
+            if ((rc = adj_alloc(ctx, &r.F, nn))) return rc;
+            if ((rc = adj_alloc(ctx, &r.glob, 8 * nm))) return rc;
+        }
+    }
     if ((rc = snap_alloc(ctx, A.end))) return rc;
     for (auto &b : A.lam_st)
         if ((rc = adj_alloc(ctx, &b, (size_t)n))) return rc;
@@ -181,26 +212,35 @@ int adjoint_setup(lbfs_ctx *ctx, int n_steps, int every)
     A.n_fpart = (n + 255) / 256;
     if ((rc = adj_alloc(ctx, &A.fpart, 3 * (size_t)A.n_fpart))) return rc;
     if ((rc = adj_alloc(ctx, &A.gforce, 3))) return rc;
-    // reverse stencil: for every cell the (cell, slot) pairs whose stencil holds it
+    // reverse stencil rows: for slot k of cell i with neighbour c, the sum of
+    // the weight vectors of c's slots that point back to i; every such slot
+    // of c is claimed by exactly one slot of i (the stencil is symmetric on
+    // face-neighbour meshes; anything else is refused)
     {
         const std::vector<int> &sten = ctx->lay.stencil;
-        std::vector<int> start(n + 1, 0);
+        const std::vector<float> &gw = ctx->lay.gw;
+        std::vector<float4> rw(6 * (size_t)n, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        std::vector<unsigned char> used(6 * (size_t)n, 0);
         for (int i = 0; i < n; i++)
             for (int k = 0; k < 6; k++) {
-                const int j = sten[(size_t)k * n + i];
-                if (j >= 0) start[j + 1]++;
+                const int c = sten[(size_t)k * n + i];
+                if (c < 0) continue;
+                for (int kb = 0; kb < 6; kb++) {
+                    if (used[(size_t)kb * n + c] || sten[(size_t)kb * n + c] != i) continue;
+                    used[(size_t)kb * n + c] = 1;
+                    float4 &w = rw[(size_t)k * n + i];
+                    w.x += gw[(size_t)(3 * kb) * n + c];
+                    w.y += gw[(size_t)(3 * kb + 1) * n + c];
+                    w.z += gw[(size_t)(3 * kb + 2) * n + c];
+                    break;
+                }
             }
-        for (int i = 0; i < n; i++) start[i + 1] += start[i];
-        std::vector<int> list(start[n]), fill(start.begin(), start.end() - 1);
         for (int i = 0; i < n; i++)
-            for (int k = 0; k < 6; k++) {
-                const int j = sten[(size_t)k * n + i];
-                if (j >= 0) list[fill[j]++] = (i << 3) | k;
-            }
-        if ((rc = adj_alloc(ctx, &A.rev_start, start.size()))) return rc;
-        if ((rc = adj_alloc(ctx, &A.rev_list, list.size()))) return rc;
-        HIPCHK(hipMemcpy(A.rev_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(A.rev_list, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice));
+            for (int k = 0; k < 6; k++)
+                if (sten[(size_t)k * n + i] >= 0 && !used[(size_t)k * n + i])
+                    return ctx->fail(LBFS_E_UNSUPPORTED, "the adjoint needs a symmetric gradient stencil");
+        if ((rc = adj_alloc(ctx, &A.rev_w, rw.size()))) return rc;
+        HIPCHK(hipMemcpy(A.rev_w, rw.data(), sizeof(float4) * rw.size(), hipMemcpyHostToDevice));
     }
     if (ctx->have_membranes) {
         const MemArgs &m = ctx->mem;
@@ -214,55 +254,117 @@ int adjoint_setup(lbfs_ctx *ctx, int n_steps, int every)
         if ((rc = adj_alloc(ctx, &A.mglob, 8 * nm))) return rc;
         if ((rc = adj_alloc(ctx, &A.gmem, 4 * nm))) return rc;
         if ((rc = adj_alloc(ctx, &A.umax, 1))) return rc;
-    }
-    return LBFS_OK;
-}
-
-// the kernels of step n from its start state, keeping the intermediates
-// (residual into ctx->res; the updated state goes to the other buffer)
-int recompute_step(lbfs_ctx *ctx)
-{
-    if (ctx->have_membranes) {
-        HIPCHK(launch_ibm_interp(ctx->mem, ctx->ibm, ctx->st[0], ctx->stream));
-        if (ctx->ibm.binned) {
-            HIPCHK(launch_ibm_scan(ctx->ibm, ctx->stream));
-            HIPCHK(launch_ibm_place(ctx->mem, ctx->ibm, ctx->stream));
+        // chunks of at most 256 edges per membrane for the two-level sums
+        std::vector<int> ch, cs(1, 0);
+        for (int i = 0; i < (int)nm; i++) {
+            for (int e = ctx->mem_edge_start[i]; e < ctx->mem_edge_start[i + 1]; e += 256) {
+                ch.push_back(i);
+                ch.push_back(e);
+                ch.push_back(std::min(e + 256, ctx->mem_edge_start[i + 1]));
+            }
+            cs.push_back((int)ch.size() / 3);
         }
-        HIPCHK(launch_mem_globals(ctx->mem, ctx->stream));
-        HIPCHK(launch_mem_edges(ctx->mem, ctx->stream));
-        HIPCHK(launch_mem_nodes(ctx->mem, ctx->stream));
-        HIPCHK(launch_ibm_spread(ctx->mem, ctx->ibm, ctx->stream));
+        A.n_echunks = cs.back();
+        if ((rc = adj_alloc(ctx, &A.echunk, ch.size()))) return rc;
+        if ((rc = adj_alloc(ctx, &A.echunk_start, cs.size()))) return rc;
+        if ((rc = adj_alloc(ctx, &A.epart, 6 * (size_t)A.n_echunks))) return rc;
+        HIPCHK(hipMemcpy(A.echunk, ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(A.echunk_start, cs.data(), sizeof(int) * cs.size(), hipMemcpyHostToDevice));
     }
-    HIPCHK(launch_gradients(ctx->args, ctx->st[0], ctx->grad, ctx->stream));
-    HIPCHK(launch_flux_update(ctx->args, ctx->st[0], ctx->grad, ctx->st[1], ctx->res, 1, ctx->stream));
     return LBFS_OK;
 }
 
-int reverse_step(lbfs_ctx *ctx, const Snapshot &S, bool first, int &cur_l)
+// One step of a segment replay, from the live state with its fluid state in
+// `st` (the record's start state), keeping what the reverse step reads; the
+// new state goes to `st_next`.  The kernels and their order are those of
+// lbfs_step.
+int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next)
+{
+    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    const size_t n = (size_t)ctx->lay.n_cells;
+    FluidArgs a = ctx->args;
+    a.face_moments = r.mom;
+    if (!ctx->have_membranes) {
+        HIPCHK(launch_gradients(a, st, r.grad, ctx->stream));
+        HIPCHK(launch_flux_update(a, st, r.grad, st_next, r.res, 1, ctx->stream));
+        return LBFS_OK;
+    }
+    // the gradients beside the coupling chain, the membrane forces beside the
+    // node binning, as in the time step
+    HIPCHK(hipEventRecord(ctx->ev_fork, ctx->stream));
+    HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+    HIPCHK(launch_gradients(a, st, r.grad, ctx->side));
+    HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
+    // the node kernels write into the record: X^n is advected in r.Xn1,
+    // U^n, F^n and the globals of the step land in r.U, r.F, r.glob
+    MemArgs m = ctx->mem;
+    const size_t nn = m.n_nodes, nm = m.n_mem;
+    HIPCHK(hipMemcpyAsync(r.Xn1, r.X, sizeof(float4) * nn, k, ctx->stream));
+    HIPCHK(hipMemcpyAsync(r.glob, r.glob_prev, sizeof(double) * 8 * nm, k, ctx->stream));
+    m.X = r.Xn1;
+    m.U = r.U;
+    m.F = r.F;
+    m.glob = r.glob;
+    HIPCHK(launch_ibm_interp(m, ctx->ibm, st, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev_fork2, ctx->stream));
+    HIPCHK(hipStreamWaitEvent(ctx->side2, ctx->ev_fork2, 0));
+    HIPCHK(launch_mem_globals(m, ctx->side2));
+    HIPCHK(launch_mem_edges(m, ctx->side2));
+    HIPCHK(launch_mem_nodes(m, ctx->side2));
+    HIPCHK(hipEventRecord(ctx->ev_join2, ctx->side2));
+    if (ctx->ibm.binned) {
+        HIPCHK(launch_ibm_scan(ctx->ibm, ctx->stream));
+        HIPCHK(launch_ibm_place(m, ctx->ibm, ctx->stream));
+    }
+    HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join2, 0));
+    HIPCHK(launch_ibm_spread(m, ctx->ibm, ctx->stream));
+    HIPCHK(hipMemcpyAsync(r.cf, ctx->cell_force, sizeof(float) * 3 * n, k, ctx->stream));
+    HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+    HIPCHK(launch_flux_update(a, st, r.grad, st_next, r.res, 1, ctx->stream));
+    return LBFS_OK;
+}
+
+int reverse_step(lbfs_ctx *ctx, const StepRec &r, bool first, int &cur_l)
 {
     AdjointState &A = *ctx->adj;
+    FluidArgs a = ctx->args;
+    if (ctx->have_membranes) a.cell_force = r.cf;
+    a.face_moments = r.mom;
     AdjFluidArgs j{};
     j.lam_in = A.lam_st[cur_l];
     j.lam_out = A.lam_st[cur_l ^ 1];
     j.mu = A.mu; j.lcf = A.lcf; j.ls = A.ls; j.ls_acc = A.ls_acc; j.lam_g = A.lam_g;
     j.fpart = A.fpart; j.gforce = A.gforce; j.n_fpart = A.n_fpart;
-    j.rev_start = A.rev_start; j.rev_list = A.rev_list;
+    j.rev_w = A.rev_w;
     j.acc = ctx->have_membranes ? ctx->ibm.acc : nullptr;
     j.acc_rw = ctx->have_membranes ? ctx->ibm.acc : nullptr;
     j.umax = A.umax;
-    HIPCHK(launch_adj_update(ctx->args, j, ctx->st[0], ctx->res, ctx->stream));
-    if (ctx->have_membranes) {
-        AdjMemArgs jm{};
-        jm.Xn = S.X; jm.glob_prev = S.glob;
-        jm.lam_X = A.lam_X; jm.lam_U = A.lam_U; jm.lam_F = A.lam_F;
-        jm.erec = A.erec; jm.grec = A.grec; jm.mrec = A.mrec; jm.mglob = A.mglob; jm.gmem = A.gmem;
-        jm.umax = A.umax; jm.lcf = A.lcf; jm.st_n = ctx->st[0]; jm.acc = ctx->ibm.acc;
-        jm.mad_lag = first ? 0 : 1;
-        HIPCHK(hipMemsetAsync(A.umax, 0, sizeof(unsigned int), ctx->stream));
-        HIPCHK(launch_adj_coupling(ctx->mem, ctx->ibm, jm, ctx->stream));
+    HIPCHK(launch_adj_update(a, j, r.st, r.res, ctx->stream));
+    if (!ctx->have_membranes) {
+        HIPCHK(launch_adj_flux(a, j, r.st, r.grad, A.max_tile_cells, ctx->stream));
+        HIPCHK(launch_adj_grad(a, j, ctx->stream));
+        cur_l ^= 1;
+        return LBFS_OK;
     }
-    HIPCHK(launch_adj_flux(ctx->args, j, ctx->st[0], ctx->grad, ctx->stream));
-    HIPCHK(launch_adj_grad(ctx->args, j, ctx->stream));
+    // the face adjoint (side stream) beside the coupling adjoint chain; both
+    // read only the update's adjoint, and the gradient transpose waits for both
+    HIPCHK(hipEventRecord(ctx->ev_fork, ctx->stream));
+    HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+    HIPCHK(launch_adj_flux(a, j, r.st, r.grad, A.max_tile_cells, ctx->side));
+    HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
+    MemArgs m = ctx->mem;
+    m.X = r.Xn1; m.U = r.U; m.F = r.F; m.glob = r.glob;
+    AdjMemArgs jm{};
+    jm.Xn = r.X; jm.glob_prev = r.glob_prev;
+    jm.lam_X = A.lam_X; jm.lam_U = A.lam_U; jm.lam_F = A.lam_F;
+    jm.erec = A.erec; jm.grec = A.grec; jm.mrec = A.mrec; jm.mglob = A.mglob; jm.gmem = A.gmem;
+    jm.umax = A.umax; jm.lcf = A.lcf; jm.st_n = r.st; jm.acc = ctx->ibm.acc;
+    jm.mad_lag = first ? 0 : 1;
+    jm.echunk = A.echunk; jm.echunk_start = A.echunk_start; jm.epart = A.epart; jm.n_echunks = A.n_echunks;
+    HIPCHK(hipMemsetAsync(A.umax, 0, sizeof(unsigned int), ctx->stream));
+    HIPCHK(launch_adj_coupling(m, ctx->ibm, jm, ctx->stream));
+    HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+    HIPCHK(launch_adj_grad(a, j, ctx->stream));
     cur_l ^= 1;
     return LBFS_OK;
 }
@@ -312,6 +414,7 @@ extern "C" int lbfs_adjoint_backward(lbfs_ctx *ctx, const double *lam_nodes, con
     AdjointState &A = *ctx->adj;
     const int n = ctx->lay.n_cells;
     int rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));  // the context's streams do not wait for the null stream
     // seeds: adjoints of the final state and node positions
     {
         std::vector<float4> h(n, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
@@ -323,6 +426,7 @@ extern "C" int lbfs_adjoint_backward(lbfs_ctx *ctx, const double *lam_nodes, con
             }
         HIPCHK(hipMemcpy(A.lam_st[0], h.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
         HIPCHK(hipMemset(A.gforce, 0, 3 * sizeof(double)));
+        HIPCHK(hipMemset(A.fpart, 0, 3 * sizeof(double) * A.n_fpart));
     }
     if (ctx->have_membranes) {
         const int nn = ctx->mem.n_nodes;
@@ -338,18 +442,24 @@ extern "C" int lbfs_adjoint_backward(lbfs_ctx *ctx, const double *lam_nodes, con
     const int K = A.every, N = A.n_steps;
     for (int sg = (N + K - 1) / K - 1; sg >= 0; sg--) {
         const int s0 = sg * K, s1 = std::min(s0 + K, N);
-        // replay the segment keeping the start state of each of its steps
+        // replay the segment into its step records, then reverse it
         if ((rc = restore(ctx, A.ckpt[sg]))) return rc;
-        for (int t = s0; t + 1 < s1; t++) {
-            if ((rc = ctx->enqueue_step())) return rc;
-            if ((rc = save(ctx, A.seg[t + 1 - s0]))) return rc;
+        for (int t = s0; t < s1; t++) {
+            StepRec &r = A.rec[t - s0];
+            const StepRec *p = t > s0 ? &A.rec[t - 1 - s0] : nullptr;
+            if (!p) r.st = A.ckpt[sg].st;
+            r.X = p ? p->Xn1 : A.ckpt[sg].X;
+            r.glob_prev = p ? p->glob : A.ckpt[sg].glob;
+            float4 *next = t + 1 < s1 ? A.rec[t + 1 - s0].st : ctx->st[1];
+            if ((rc = replay_step(ctx, r, t == s0 ? ctx->st[0] : r.st, next))) return rc;
         }
-        for (int t = s1 - 1; t >= s0; t--) {
-            Snapshot &S = t == s0 ? A.ckpt[sg] : A.seg[t - s0];
-            if ((rc = restore(ctx, S))) return rc;
-            if ((rc = recompute_step(ctx))) return rc;
-            if ((rc = reverse_step(ctx, S, t == 0, cur_l))) return rc;
-        }
+        for (int t = s1 - 1; t >= s0; t--)
+            if ((rc = reverse_step(ctx, A.rec[t - s0], t == 0, cur_l))) return rc;
+    }
+    {
+        AdjFluidArgs j{};
+        j.fpart = A.fpart; j.gforce = A.gforce; j.n_fpart = A.n_fpart;
+        HIPCHK(launch_adj_force_sum(j, ctx->stream));
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (grad_body_force) HIPCHK(hipMemcpy(grad_body_force, A.gforce, 3 * sizeof(double), hipMemcpyDeviceToHost));

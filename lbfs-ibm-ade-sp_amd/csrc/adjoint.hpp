@@ -25,10 +25,10 @@ struct AdjFluidArgs {
     float4 *ls;                // [n] adjoint of st^n through the update
     float4 *ls_acc;            // [n] ls plus the face contributions
     float4 *lam_g;             // [3 n] adjoint of the gradients, in the gradient layout
-    double *fpart;             // [3 blocks] partial sums of the cell-force adjoint
-    double *gforce;            // [3] running gradient of the uniform body force
-    const int *rev_start;      // [n + 1] cells whose stencil holds cell i
-    const int *rev_list;       // (cell << 3) | stencil slot
+    double *fpart;             // [3 blocks] running per-block sums of the cell-force adjoint
+    double *gforce;            // [3] gradient of the uniform body force (sum of fpart)
+    const float4 *rev_w;       // [6 n] SoA: for stencil slot k of cell i (neighbour c), the weight
+                               // vector of c's stencil rows that point back to i (x, y, z)
     const unsigned long long *acc;  // [3 n] fixed-point adjoint of the interpolated velocity, or null
     unsigned long long *acc_rw;
     const unsigned int *umax;  // float bits of the largest |adjoint node velocity| (fixed-point scale)
@@ -51,6 +51,10 @@ struct AdjMemArgs {
     const float4 *st_n;        // [n cells] state at the start of the step
     unsigned long long *acc;   // [3 n cells] fixed-point accumulators
     int mad_lag;               // 0 on the first step: its MAD is a set-up constant
+    const int *echunk;         // 3 per chunk: membrane, first edge, end edge (at most 256 edges)
+    const int *echunk_start;   // [mem + 1] first chunk of each membrane
+    double *epart;             // [6 chunks] partial sums
+    int n_echunks;
 };
 
 // Fixed-point scale of the interpolation adjoint: values are held as
@@ -75,9 +79,12 @@ __device__ __forceinline__ double adj_fix_inv_scale(unsigned int m)
 
 hipError_t launch_adj_update(const FluidArgs &a, const AdjFluidArgs &j, const float4 *st, const float4 *res,
                              hipStream_t s);
+// a.face_moments holds the forward moments of the step; tiles of at most 1024 cells
 hipError_t launch_adj_flux(const FluidArgs &a, const AdjFluidArgs &j, const float4 *st, const float4 *grad,
-                           hipStream_t s);
+                           int max_tile_cells, hipStream_t s);
 hipError_t launch_adj_grad(const FluidArgs &a, const AdjFluidArgs &j, hipStream_t s);
+// gforce = sum of the per-block partials, in a fixed order
+hipError_t launch_adj_force_sum(const AdjFluidArgs &j, hipStream_t s);
 
 hipError_t launch_adj_coupling(const MemArgs &m, const IbmArgs &g, const AdjMemArgs &j, hipStream_t s);
 

@@ -8,8 +8,10 @@
 //   k_adj_mem_edges   per edge: the edge force (spring_network.cu:613-971) in
 //                     forward-mode, one input direction at a time, contracted
 //                     with the adjoint of the four node forces
-//   k_adj_mem_reduce  per membrane (one wavefront): adjoints of the global
-//                     volume/area/MAD coefficients and the modulus gradients
+//   k_adj_mem_partials, k_adj_mem_reduce
+//                     per chunk of edges, then per membrane (fixed order):
+//                     adjoints of the global volume/area/MAD coefficients and
+//                     the modulus gradients
 //   k_adj_mem_geo     per edge: volume and area constraint terms at X^n+1
 //   k_adj_mem_mad     per edge: the MAD term, a function of X^n (one step lag)
 //   k_adj_mem_nodes   per node: fixed-order sum of its edge slots; adds the
@@ -156,26 +158,23 @@ __device__ __forceinline__ memf::EdgeConst edge_consts(const MemArgs &m, const M
 
 }  // namespace
 
-__global__ __launch_bounds__(64) void k_adj_mem_edges(MemArgs m, AdjMemArgs j)
+// one thread per (edge, input direction)
+__global__ __launch_bounds__(256) void k_adj_mem_edges(MemArgs m, AdjMemArgs j)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = gid / 24, d = gid - 24 * e;
     if (e >= m.n_edges) return;
     const MemParamDev p = m.params[m.edge_mem[e]];
     float x[24];
     edge_inputs(m, m.X, e, p, x);
     const int4 q = m.edges[e];
     const float4 l0 = j.lam_F[q.x], l1 = j.lam_F[q.y], l2 = j.lam_F[q.z], l3 = j.lam_F[q.w];
-    const memf::EdgeConst kc = edge_consts(m, p, e);
-#pragma unroll 1
-    for (int d = 0; d < 24; d++) {
-        memf::EdgeOut<memf::Dual> out;
-        memf::edge_force(seeded(x, d), kc, out);
-        const float s = l0.x * out.F[0].x.d + l0.y * out.F[0].y.d + l0.z * out.F[0].z.d +
-                        l1.x * out.F[1].x.d + l1.y * out.F[1].y.d + l1.z * out.F[1].z.d +
-                        l2.x * out.F[2].x.d + l2.y * out.F[2].y.d + l2.z * out.F[2].z.d +
-                        l3.x * out.F[3].x.d + l3.y * out.F[3].y.d + l3.z * out.F[3].z.d;
-        j.erec[24 * (size_t)e + d] = s;
-    }
+    memf::EdgeOut<memf::Dual> out;
+    memf::edge_force(seeded(x, d), edge_consts(m, p, e), out);
+    j.erec[24 * (size_t)e + d] = l0.x * out.F[0].x.d + l0.y * out.F[0].y.d + l0.z * out.F[0].z.d +
+                                 l1.x * out.F[1].x.d + l1.y * out.F[1].y.d + l1.z * out.F[1].z.d +
+                                 l2.x * out.F[2].x.d + l2.y * out.F[2].y.d + l2.z * out.F[2].z.d +
+                                 l3.x * out.F[3].x.d + l3.y * out.F[3].y.d + l3.z * out.F[3].z.d;
 }
 
 // Per membrane: sums of the coefficient and modulus adjoints over its edges
@@ -184,19 +183,41 @@ __global__ __launch_bounds__(64) void k_adj_mem_edges(MemArgs m, AdjMemArgs j)
 //   area = -k_A (A / A0 - 1)          -> lambda_A   = l_area (-k_A / A0)
 //   mad  = -2 k_gb (MAD - MAD0) / A0  -> lambda_MAD = l_mad (-2 k_gb / A0),
 //                                        d/dk_gb   = l_mad (-2 (MAD - MAD0) / A0)
+// first level: per chunk of at most 256 edges of one membrane
+__global__ __launch_bounds__(256) void k_adj_mem_partials(MemArgs m, AdjMemArgs j)
+{
+    __shared__ double sh[6][256];
+    const int *q = j.echunk + 3 * (size_t)blockIdx.x;
+    const int tid = threadIdx.x;
+    const int e = q[1] + tid;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    if (e < q[2]) {
+        const float *r = j.erec + 24 * (size_t)e + 18;
+        s[0] = r[0];
+        s[1] = r[1];
+        s[2] = r[2];
+        s[3] = (double)r[3] * (double)m.edge_const[e].y;
+        s[4] = r[4];
+        s[5] = r[5];
+    }
+    for (int k = 0; k < 6; k++) sh[k][tid] = s[k];
+    __syncthreads();
+    for (int stp = 128; stp > 0; stp >>= 1) {
+        if (tid < stp)
+            for (int k = 0; k < 6; k++) sh[k][tid] += sh[k][tid + stp];
+        __syncthreads();
+    }
+    if (tid < 6) j.epart[6 * (size_t)blockIdx.x + tid] = sh[tid][0];
+}
+
+// second level: one wavefront per membrane over its chunks, then the chain
+// rule through the coefficient formulas
 __global__ __launch_bounds__(64) void k_adj_mem_reduce(MemArgs m, AdjMemArgs j)
 {
     const int mem = blockIdx.x, lane = threadIdx.x;
     double s[6] = {0, 0, 0, 0, 0, 0};
-    for (int e = m.mem_edge_start[mem] + lane; e < m.mem_edge_start[mem + 1]; e += 64) {
-        const float *r = j.erec + 24 * (size_t)e + 18;
-        s[0] += r[0];
-        s[1] += r[1];
-        s[2] += r[2];
-        s[3] += (double)r[3] * (double)m.edge_const[e].y;
-        s[4] += r[4];
-        s[5] += r[5];
-    }
+    for (int c = j.echunk_start[mem] + lane; c < j.echunk_start[mem + 1]; c += 64)
+        for (int k = 0; k < 6; k++) s[k] += j.epart[6 * (size_t)c + k];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
@@ -242,16 +263,16 @@ __global__ __launch_bounds__(128) void k_adj_mem_geo(MemArgs m, AdjMemArgs j)
 // c1 = lambda_MAD / A(X^n), c2 = lambda_MAD MAD / A(X^n); the sum of the second
 // term over the edges is c2 A.  The first step's MAD comes from the set-up
 // (a constant of the run, like the reference areas and volume)
-__global__ __launch_bounds__(64) void k_adj_mem_mad(MemArgs m, AdjMemArgs j)
+__global__ __launch_bounds__(256) void k_adj_mem_mad(MemArgs m, AdjMemArgs j)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = gid / 12, d = gid - 12 * e;
     if (e >= m.n_edges) return;
     const int mem = m.edge_mem[e];
     const double *mg = j.mglob + 8 * (size_t)mem;
-    float *rec = j.mrec + 12 * (size_t)e;
+    float *rec = j.mrec + 12 * (size_t)e + d;
     if (!j.mad_lag || mg[2] == 0.0) {
-#pragma unroll
-        for (int d = 0; d < 12; d++) rec[d] = 0.0f;
+        *rec = 0.0f;
         return;
     }
     const MemParamDev p = m.params[mem];
@@ -259,14 +280,10 @@ __global__ __launch_bounds__(64) void k_adj_mem_mad(MemArgs m, AdjMemArgs j)
     const float c1 = in_mad ? (float)mg[3] : 0.0f, c2 = (float)mg[4];
     float x[24];
     edge_inputs(m, j.Xn, e, p, x);
-    const memf::EdgeConst kc = edge_consts(m, p, e);
-#pragma unroll 1
-    for (int d = 0; d < 12; d++) {
-        memf::EdgeOut<memf::Dual> out;
-        memf::edge_force(seeded(x, d), kc, out);
-        const memf::Dual nn = out.n1 + out.n2;
-        rec[d] = c1 * (out.kappa.d * nn.v + out.kappa.v * nn.d) * (1.0f / 3.0f) - c2 * nn.d * (1.0f / 6.0f);
-    }
+    memf::EdgeOut<memf::Dual> out;
+    memf::edge_force(seeded(x, d), edge_consts(m, p, e), out);
+    const memf::Dual nn = out.n1 + out.n2;
+    *rec = c1 * (out.kappa.d * nn.v + out.kappa.v * nn.d) * (1.0f / 3.0f) - c2 * nn.d * (1.0f / 6.0f);
 }
 
 // Per node (eight lanes, fixed order): lambda_X^n+1 is complete once the
@@ -355,10 +372,11 @@ hipError_t launch_adj_coupling(const MemArgs &m, const IbmArgs &g, const AdjMemA
 {
     const int nb4 = (m.n_nodes + 3) / 4;
     hipLaunchKernelGGL(k_adj_spread, dim3(nb4), dim3(256), 0, s, m, g, j);
-    hipLaunchKernelGGL(k_adj_mem_edges, dim3((m.n_edges + 63) / 64), dim3(64), 0, s, m, j);
+    hipLaunchKernelGGL(k_adj_mem_edges, dim3((24 * m.n_edges + 255) / 256), dim3(256), 0, s, m, j);
+    hipLaunchKernelGGL(k_adj_mem_partials, dim3(j.n_echunks), dim3(256), 0, s, m, j);
     hipLaunchKernelGGL(k_adj_mem_reduce, dim3(m.n_mem), dim3(64), 0, s, m, j);
     hipLaunchKernelGGL(k_adj_mem_geo, dim3((m.n_edges + 127) / 128), dim3(128), 0, s, m, j);
-    hipLaunchKernelGGL(k_adj_mem_mad, dim3((m.n_edges + 63) / 64), dim3(64), 0, s, m, j);
+    hipLaunchKernelGGL(k_adj_mem_mad, dim3((12 * m.n_edges + 255) / 256), dim3(256), 0, s, m, j);
     hipLaunchKernelGGL(k_adj_mem_nodes, dim3((8 * m.n_nodes + 255) / 256), dim3(256), 0, s, m, g, j);
     hipLaunchKernelGGL(k_adj_interp, dim3(nb4), dim3(256), 0, s, m, g, j);
     return hipGetLastError();

@@ -9,9 +9,10 @@
 //                 staged in LDS, then every cell gathers its own side of its
 //                 six faces (state and gradient adjoints) in a fixed order
 //   k_adj_grad    per cell: transpose of the least-squares gradients
-//                 (LBFS.cu:2177-2240): its own stencil rows plus the rows of
-//                 the cells whose stencil holds it (a reverse-stencil list),
-//                 plus the interpolation adjoint held in fixed point
+//                 (LBFS.cu:2177-2240): its own stencil rows plus, per
+//                 neighbour, the neighbour's row that points back to it
+//                 (weights gathered once on the host), plus the
+//                 interpolation adjoint held in fixed point
 //
 // Strict upwinding (ties to the owner) is piecewise smooth; its derivative is
 // taken on the side the forward step took.  No floating-point atomics:
@@ -65,10 +66,11 @@ __global__ __launch_bounds__(256) void k_adj_update(FluidArgs a, AdjFluidArgs j,
             for (int t = 0; t < 3; t++) sh[t][threadIdx.x] += sh[t][threadIdx.x + stp];
         __syncthreads();
     }
-    if (threadIdx.x < 3) j.fpart[3 * (size_t)blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+    // every block owns its partials: the running sums need no atomics
+    if (threadIdx.x < 3) j.fpart[3 * (size_t)blockIdx.x + threadIdx.x] += sh[threadIdx.x][0];
 }
 
-// fixed-order sum of the partials into the running body-force gradient
+// fixed-order sum of the partials into the body-force gradient
 __global__ __launch_bounds__(256) void k_adj_force_sum(AdjFluidArgs j)
 {
     __shared__ double sh[3][256];
@@ -82,7 +84,7 @@ __global__ __launch_bounds__(256) void k_adj_force_sum(AdjFluidArgs j)
             for (int t = 0; t < 3; t++) sh[t][threadIdx.x] += sh[t][threadIdx.x + stp];
         __syncthreads();
     }
-    if (threadIdx.x < 3) j.gforce[threadIdx.x] += sh[threadIdx.x][0];
+    if (threadIdx.x < 3) j.gforce[threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // ---------------------------------------------------------------------------
@@ -168,10 +170,11 @@ __device__ __forceinline__ void all_pair_vjps(std::integer_sequence<int, P...>, 
 }
 
 // Adjoint of one tile-face's flux (the forward of face_flux in
-// fluid_kernels.hip, strict upwinding).  out[0] state and out[1..3] gradient
+// fluid_kernels.hip, strict upwinding), from the face's moments S0, S1 as the
+// forward pass computed them.  out[0] state and out[1..3] gradient
 // adjoint of the owner, out[4..7] the same for the neighbour (zero for a
 // boundary face, whose ghost is a function of the owner).
-template <class L>
+template <class L, bool GHOST>
 __device__ __forceinline__ void face_vjp(const FluidArgs &a, const float4 *__restrict__ st,
                                          const float4 *__restrict__ grad, const float4 *__restrict__ mu, int tf,
                                          float4 out[8])
@@ -179,7 +182,7 @@ __device__ __forceinline__ void face_vjp(const FluidArgs &a, const float4 *__res
     const float4 fa = a.fA[tf], fb = a.fB[tf], fcv = a.fC[tf];
     const int2 on = a.fI[tf];
     const int o = on.x, nb = on.y;
-    const bool ghost = nb >= a.n_cells;
+    constexpr bool ghost = GHOST;  // boundary faces follow the interior faces in every tile list
     const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
     for (int k = 0; k < 8; k++) out[k] = zero;
@@ -239,17 +242,14 @@ __device__ __forceinline__ void face_vjp(const FluidArgs &a, const float4 *__res
 #pragma unroll
     for (int t = 0; t < 12; t++) { Go[t] = go[t] * dt; Gn[t] = gn[t] * dt; }
     const float rho0 = a.rho0, ig = a.inv_gamma;
-    // forward moments
+    // forward moments S0, S1 of this face, kept by the forward pass
+    const float4 fm = a.face_moments[tf];
     FaceSums f;
-    f.S1x = f.S1y = f.S1z = f.Tx = f.Ty = f.Tz = 0.0f;
-    float P0;
-    {
-        const float uu = p0[1] * p0[1] + p0[2] * p0[2] + p0[3] * p0[3];
-        P0 = -1.5f * uu * ig;
-        f.S0 = L::w(0) * (p0[0] + (rho0 + p0[0]) * P0);
-    }
-    all_pairs<L, false>(std::make_integer_sequence<int, (L::Q - 1) / 2>{}, f, po, pn, p0, Go, Gn, nx, ny, nz, 0.0f,
-                        ghost, rho0, ig);
+    f.S0 = fm.x;
+    f.S1x = fm.y;
+    f.S1y = fm.z;
+    f.S1z = fm.w;
+    const float P0 = -1.5f * (p0[1] * p0[1] + p0[2] * p0[2] + p0[3] * p0[3]) * ig;
     const float rho_f = rho0 + f.S0;
     const float mnrm = f.S1x * nx + f.S1y * ny + f.S1z * nz;
     const float vf = a.visc_factor ? a.visc_factor[o] : 1.0f;
@@ -340,14 +340,14 @@ __device__ __forceinline__ void face_vjp(const FluidArgs &a, const float4 *__res
 
 constexpr int ADJ_THREADS = 512;
 constexpr int ADJ_CHUNK = 512;       // faces per LDS chunk (8 float4 each: 64 KiB)
-constexpr int ADJ_CELLS = 2;         // cells per thread (tiles of up to 1024 cells)
 
 __device__ __forceinline__ void add4(float *acc, float4 v)
 {
     acc[0] += v.x; acc[1] += v.y; acc[2] += v.z; acc[3] += v.w;
 }
 
-template <class L>
+// ADJ_CELLS: cells per thread (1: tiles of up to 512 cells, 2: up to 1024)
+template <class L, int ADJ_CELLS>
 __global__ __launch_bounds__(ADJ_THREADS) void k_adj_flux(FluidArgs a, AdjFluidArgs j, const float4 *__restrict__ st,
                                                           const float4 *__restrict__ grad)
 {
@@ -355,6 +355,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_flux(FluidArgs a, AdjFluidA
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int f0 = a.tile_face_start[t], f1 = a.tile_face_start[t + 1];
     const int c0 = a.tile_cell_start[t], c1 = a.tile_cell_start[t + 1];
+    const int fb = a.tile_bface_start[t];
     float acc[ADJ_CELLS][16];
     uint32_t slot[ADJ_CELLS][6];
 #pragma unroll
@@ -377,7 +378,8 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_flux(FluidArgs a, AdjFluidA
         const int ce = cs + ADJ_CHUNK < f1 ? cs + ADJ_CHUNK : f1;
         for (int f = cs + threadIdx.x; f < ce; f += ADJ_THREADS) {
             float4 o[8];
-            face_vjp<L>(a, st, grad, j.mu, f, o);
+            if (f < fb) face_vjp<L, false>(a, st, grad, j.mu, f, o);
+            else face_vjp<L, true>(a, st, grad, j.mu, f, o);
 #pragma unroll
             for (int k = 0; k < 8; k++) lds_adj[8 * (f - cs) + k] = o[k];
         }
@@ -447,17 +449,14 @@ __global__ __launch_bounds__(256) void k_adj_grad(FluidArgs a, AdjFluidArgs j)
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) acc[v] -= c[v] * (wx * g[3 * v] + wy * g[3 * v + 1] + wz * g[3 * v + 2]);
-    }
-    for (int r = j.rev_start[i]; r < j.rev_start[i + 1]; r++) {
-        const int e = j.rev_list[r];
-        const int c = e >> 3, k = e & 7;
-        const float wx = a.gw[(size_t)(3 * k) * n + c];
-        const float wy = a.gw[(size_t)(3 * k + 1) * n + c];
-        const float wz = a.gw[(size_t)(3 * k + 2) * n + c];
-        float gc[12];
-        lam_grad(j.lam_g, c, gc);
+        if (jn >= 0) {
+            // the neighbour's stencil row that points back here
+            const float4 r = j.rev_w[(size_t)k * n + i];
+            float gc[12];
+            lam_grad(j.lam_g, jn, gc);
 #pragma unroll
-        for (int v = 0; v < 4; v++) acc[v] += wx * gc[3 * v] + wy * gc[3 * v + 1] + wz * gc[3 * v + 2];
+            for (int v = 0; v < 4; v++) acc[v] += r.x * gc[3 * v] + r.y * gc[3 * v + 1] + r.z * gc[3 * v + 2];
+        }
     }
     if (j.acc_rw) {
         // the interpolation adjoint is held as v 2^(40 - e), e the binary
@@ -484,18 +483,34 @@ hipError_t launch_adj_update(const FluidArgs &a, const AdjFluidArgs &j, const fl
 {
     const int blocks = (a.n_cells + 255) / 256;
     hipLaunchKernelGGL(k_adj_update, dim3(blocks), dim3(256), 0, s, a, j, st, res);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_force_sum(const AdjFluidArgs &j, hipStream_t s)
+{
     hipLaunchKernelGGL(k_adj_force_sum, dim3(1), dim3(256), 0, s, j);
     return hipGetLastError();
 }
 
+template <class L>
+static void launch_adj_flux_l(const FluidArgs &a, const AdjFluidArgs &j, const float4 *st, const float4 *grad,
+                              int cells_per_thread, size_t lds, hipStream_t s)
+{
+    if (cells_per_thread == 1)
+        hipLaunchKernelGGL((k_adj_flux<L, 1>), dim3(a.n_tiles), dim3(ADJ_THREADS), lds, s, a, j, st, grad);
+    else
+        hipLaunchKernelGGL((k_adj_flux<L, 2>), dim3(a.n_tiles), dim3(ADJ_THREADS), lds, s, a, j, st, grad);
+}
+
 hipError_t launch_adj_flux(const FluidArgs &a, const AdjFluidArgs &j, const float4 *st, const float4 *grad,
-                           hipStream_t s)
+                           int max_tile_cells, hipStream_t s)
 {
     const size_t lds = sizeof(float4) * 8 * (size_t)ADJ_CHUNK;
+    const int cpt = max_tile_cells <= ADJ_THREADS ? 1 : 2;
     switch (a.lattice) {
-    case LBFS_D3Q15: hipLaunchKernelGGL(k_adj_flux<D3Q15>, dim3(a.n_tiles), dim3(ADJ_THREADS), lds, s, a, j, st, grad); break;
-    case LBFS_D3Q19: hipLaunchKernelGGL(k_adj_flux<D3Q19>, dim3(a.n_tiles), dim3(ADJ_THREADS), lds, s, a, j, st, grad); break;
-    case LBFS_D2Q9: hipLaunchKernelGGL(k_adj_flux<D2Q9>, dim3(a.n_tiles), dim3(ADJ_THREADS), lds, s, a, j, st, grad); break;
+    case LBFS_D3Q15: launch_adj_flux_l<D3Q15>(a, j, st, grad, cpt, lds, s); break;
+    case LBFS_D3Q19: launch_adj_flux_l<D3Q19>(a, j, st, grad, cpt, lds, s); break;
+    case LBFS_D2Q9: launch_adj_flux_l<D2Q9>(a, j, st, grad, cpt, lds, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

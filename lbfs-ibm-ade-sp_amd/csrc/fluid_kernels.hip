@@ -149,6 +149,7 @@ __device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__
     all_pairs<L, BAND>(std::make_integer_sequence<int, (L::Q - 1) / 2>{}, f, po, pn, p0, go, gn, nx, ny, nz, bw, ghost,
                        rho0, ig);
     const float S0 = f.S0, S1x = f.S1x, S1y = f.S1y, S1z = f.S1z, Tx = f.Tx, Ty = f.Ty, Tz = f.Tz;
+    if (a.face_moments) a.face_moments[tf] = make_float4(S0, S1x, S1y, S1z);
     const float rho_f = rho0 + S0;
     const float mn = S1x * nx + S1y * ny + S1z * nz;
     const float vf = a.visc_factor ? a.visc_factor[o] : 1.0f;

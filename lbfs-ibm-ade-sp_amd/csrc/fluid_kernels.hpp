@@ -21,6 +21,7 @@ struct FluidArgs {
     const int16_t *cell_slots;     // [6 n]
     const float *visc_factor;      // [n] or null
     const float *cell_force;       // [3 n] or null (IBM force density)
+    float4 *face_moments;          // [tile faces] or null: S0, S1 of each face (kept for the adjoint)
     // tiles
     const int *tile_cell_start, *tile_face_start;
     const int *tile_bface_start;   // first boundary face of each tile's list
