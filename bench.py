@@ -92,6 +92,8 @@ def main():
                     help="only run three 1 GiB device-to-device copies (the reference for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--only-rbc", action="store_true",
+                    help="time the channel + RBC step alone (no plain step, suspension or adjoint): profiling runs")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-every", type=int, default=100)
     ap.add_argument("--ensemble", action="store_true",
@@ -130,11 +132,15 @@ def main():
         params, table = autotune.tune(mesh, params, state=state, steps=20, warmup=5)
         tuning = [{"tile": list(t), "threads": nt, "ms_per_step": ms} for t, nt, ms in table]
         log("autotuned tile %s, %d threads" % (list(params.tile), params.block_threads))
-    plain = abi.Solver(mesh, params)
-    plain.set_state(*state)
-    ms_plain = timed(plain, args.steps, args.warmup)
-    plain.close()
-    log("plain fluid step %.4f ms" % ms_plain)
+    if args.only_rbc:
+        args.no_batched = args.no_adjoint = True
+        ms_plain = float("nan")
+    else:
+        plain = abi.Solver(mesh, params)
+        plain.set_state(*state)
+        ms_plain = timed(plain, args.steps, args.warmup)
+        plain.close()
+        log("plain fluid step %.4f ms" % ms_plain)
 
     solver = abi.Solver(mesh, params)
     solver.set_state(*state)

@@ -289,11 +289,12 @@ int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next)
         HIPCHK(launch_flux_update(a, st, r.grad, st_next, r.res, 1, ctx->stream));
         return LBFS_OK;
     }
-    // the gradients beside the coupling chain, the membrane forces beside the
-    // node binning, as in the time step
+    // as in the time step: gradients and the residual pass beside the
+    // coupling chain, the membrane forces beside the node binning
     HIPCHK(hipEventRecord(ctx->ev_fork, ctx->stream));
     HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
     HIPCHK(launch_gradients(a, st, r.grad, ctx->side));
+    HIPCHK(launch_flux_update(a, st, r.grad, nullptr, r.res, 0, ctx->side));
     HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
     // the node kernels write into the record: X^n is advected in r.Xn1,
     // U^n, F^n and the globals of the step land in r.U, r.F, r.glob
@@ -320,7 +321,7 @@ int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next)
     HIPCHK(launch_ibm_spread(m, ctx->ibm, ctx->stream));
     HIPCHK(hipMemcpyAsync(r.cf, ctx->cell_force, sizeof(float) * 3 * n, k, ctx->stream));
     HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
-    HIPCHK(launch_flux_update(a, st, r.grad, st_next, r.res, 1, ctx->stream));
+    HIPCHK(launch_update(a, st, r.res, st_next, ctx->stream));
     return LBFS_OK;
 }
 

@@ -41,6 +41,8 @@ struct FluidArgs {
 hipError_t launch_gradients(const FluidArgs &a, const float4 *st, float4 *grad, hipStream_t s);
 hipError_t launch_flux_update(const FluidArgs &a, const float4 *st, const float4 *grad, float4 *st_out,
                               float4 *res_out, int do_update, hipStream_t s);
+// cell update from the residual res (the flux pass run with do_update = 0)
+hipError_t launch_update(const FluidArgs &a, const float4 *st, const float4 *res, float4 *st_out, hipStream_t s);
 hipError_t launch_face_flux_all(const FluidArgs &a, const float4 *st, const float4 *grad, float4 *out,
                                 int n_tile_faces, hipStream_t s);
 // global minimum of the local steps into *a.dt_min (LBFS_TS_MIN)

@@ -81,6 +81,26 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
                          return LBFS_OK;
                      },
                      24.0 * n});
+    if (have_membranes) {
+        // the face pass does not need the cell force: it writes the residual
+        // and runs beside the coupling chain; the update follows spreading.
+        // Face pass: state 16 + gradients 48 + volume/centroid 16 + residual
+        // written 16 per cell, plus 40 per face; update: state 16 + residual
+        // 16 + force 12 + state written 16 per cell
+        s.push_back({"flux_residual", [ctx]() -> int {
+                         HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, nullptr, ctx->res, 0,
+                                                   ctx->stream));
+                         return LBFS_OK;
+                     },
+                     96.0 * n + 40.0 * nf});
+        s.push_back({"update", [ctx]() -> int {
+                         HIPCHK(launch_update(ctx->args, ctx->st[ctx->cur], ctx->res, ctx->st[ctx->cur ^ 1], ctx->stream));
+                         ctx->cur ^= 1;
+                         return LBFS_OK;
+                     },
+                     60.0 * n});
+        return s;
+    }
     // pass F: state 16 + gradients 48 + volume/centroid 16 + force 12 + state written 16 per cell,
     // plus owner/nb 8, normal 12, face centroid 12, area 4, dt 4 per face
     s.push_back({"flux_update", [ctx]() -> int {
@@ -96,33 +116,35 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
 int lbfs_ctx::run_stages(bool with_visc)
 {
     lbfs_ctx *ctx = this;
-    if (!have_membranes || !side) {
+    if (!have_membranes || !side || with_visc) {
+        // (the internal viscosity field feeds the face pass: in the steps
+        // that rebuild it, everything runs in order on one stream)
         for (auto &st : step_stages(with_visc)) {
             int rc = st.run();
             if (rc) return rc;
         }
         return LBFS_OK;
     }
-    // With membranes the gradient pass, which reads only the state at the
-    // start of the step, runs on the second stream beside the coupling chain
-    // (a latency-bound sequence of small kernels); the flux/update waits for
-    // both.  Same kernels, same inputs, so the results are those of the
+    // With membranes the gradients and the face pass, which read only the
+    // state at the start of the step, run on the second stream beside the
+    // coupling chain (a latency-bound sequence of small kernels); after
+    // interpolation the membrane forces (globals, edges, nodes) run on a third
+    // stream beside the node binning.  The update waits for the face pass and
+    // for spreading.  Same kernels, same inputs: the results are those of the
     // sequential order.
-    // After interpolation the membrane forces (globals, edges, nodes) and the
-    // node binning (scan, place) are independent until spreading, so the
-    // membrane chain runs on a third stream.
     HIPCHK(hipEventRecord(ev_fork, stream));
     HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
     HIPCHK(launch_gradients(args, st[cur], grad, side));
+    HIPCHK(launch_flux_update(args, st[cur], grad, nullptr, res, 0, side));
     HIPCHK(hipEventRecord(ev_join, side));
-    for (auto &s : step_stages(with_visc)) {
+    for (auto &s : step_stages(false)) {
         const char *nm = s.name;
-        if (std::strcmp(nm, "gradients") == 0 || std::strcmp(nm, "mem_globals") == 0 ||
-            std::strcmp(nm, "mem_edges") == 0 || std::strcmp(nm, "mem_nodes") == 0)
+        if (std::strcmp(nm, "gradients") == 0 || std::strcmp(nm, "flux_residual") == 0 ||
+            std::strcmp(nm, "mem_globals") == 0 || std::strcmp(nm, "mem_edges") == 0 ||
+            std::strcmp(nm, "mem_nodes") == 0)
             continue;
         if (std::strcmp(nm, "ibm_spread") == 0) HIPCHK(hipStreamWaitEvent(stream, ev_join2, 0));
-        if (std::strcmp(nm, "flux_update") == 0 || std::strcmp(nm, "dt_min") == 0)
-            HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
+        if (std::strcmp(nm, "update") == 0) HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
         int rc = s.run();
         if (rc) return rc;
         if (std::strcmp(nm, "ibm_interp") == 0) {

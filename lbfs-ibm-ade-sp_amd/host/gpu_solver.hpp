@@ -95,7 +95,13 @@ public:
     // get_cfl_device + min (time_stepping 3)
     void get_cfl() { stage_if_present("dt_min"); }
     // calc_face_flux + calc_face_flux_bc + add_face_flux_to_cell + add + time_integration
-    void time_integration() { stage("flux_update"); }
+    // (with membranes the residual pass and the update are separate launches)
+    void time_integration()
+    {
+        stage_if_present("flux_update");
+        stage_if_present("flux_residual");
+        stage_if_present("update");
+    }
     void end_step() { check(lbfs_stage_end_step(ctx_), "lbfs_stage_end_step"); }
 
     // one step group by group; equals General_Purpose_Solver_mk_i(1)
