@@ -272,9 +272,9 @@ def main():
                    "block_threads": params.block_threads or 256},
         "autotune": tuning,
         "achieved_hbm_gbps_model": step_gbps,
-        "plain_ms_per_step": ms_plain,
-        "plain_mlups": n / (ms_plain * 1e-3) / 1e6,
-        "rbc_step_vs_plain": ms_step / ms_plain,
+        "plain_ms_per_step": None if args.only_rbc else ms_plain,
+        "plain_mlups": None if args.only_rbc else n / (ms_plain * 1e-3) / 1e6,
+        "rbc_step_vs_plain": None if args.only_rbc else ms_step / ms_plain,
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(names[dom]),
                      "model_bytes_per_launch": mbytes[dom], "avg_launch_ms": kms[names[dom]]},
