@@ -289,15 +289,10 @@ int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next)
         HIPCHK(launch_flux_update(a, st, r.grad, st_next, r.res, 1, ctx->stream));
         return LBFS_OK;
     }
-    // as in the time step: gradients and the residual pass beside the
-    // coupling chain, the membrane forces beside the node binning
-    HIPCHK(hipEventRecord(ctx->ev_fork, ctx->stream));
-    HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-    HIPCHK(launch_gradients(a, st, r.grad, ctx->side));
-    HIPCHK(launch_flux_update(a, st, r.grad, nullptr, r.res, 0, ctx->side));
-    HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
-    // the node kernels write into the record: X^n is advected in r.Xn1,
-    // U^n, F^n and the globals of the step land in r.U, r.F, r.glob
+    // as in the time step: after the interpolation, the gradients and the
+    // residual pass beside the coupling chain, the membrane forces beside the
+    // node binning; the node kernels write into the record (X^n is advected
+    // in r.Xn1; U^n, F^n and the step's globals land in r.U, r.F, r.glob)
     MemArgs m = ctx->mem;
     const size_t nn = m.n_nodes, nm = m.n_mem;
     HIPCHK(hipMemcpyAsync(r.Xn1, r.X, sizeof(float4) * nn, k, ctx->stream));
@@ -307,16 +302,16 @@ int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next)
     m.F = r.F;
     m.glob = r.glob;
     HIPCHK(launch_ibm_interp(m, ctx->ibm, st, ctx->stream));
-    HIPCHK(hipEventRecord(ctx->ev_fork2, ctx->stream));
-    HIPCHK(hipStreamWaitEvent(ctx->side2, ctx->ev_fork2, 0));
+    HIPCHK(hipEventRecord(ctx->ev_fork, ctx->stream));
+    HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+    HIPCHK(launch_gradients(a, st, r.grad, ctx->side));
+    HIPCHK(launch_flux_update(a, st, r.grad, nullptr, r.res, 0, ctx->side));
+    HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
+    HIPCHK(hipStreamWaitEvent(ctx->side2, ctx->ev_fork, 0));
     HIPCHK(launch_mem_globals(m, ctx->side2));
     HIPCHK(launch_mem_edges(m, ctx->side2));
-    HIPCHK(launch_mem_nodes(m, ctx->side2));
+    if (!ctx->ibm.binned) HIPCHK(launch_mem_nodes(m, ctx->side2));
     HIPCHK(hipEventRecord(ctx->ev_join2, ctx->side2));
-    if (ctx->ibm.binned) {
-        HIPCHK(launch_ibm_scan(ctx->ibm, ctx->stream));
-        HIPCHK(launch_ibm_place(m, ctx->ibm, ctx->stream));
-    }
     HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join2, 0));
     HIPCHK(launch_ibm_spread(m, ctx->ibm, ctx->stream));
     HIPCHK(hipMemcpyAsync(r.cf, ctx->cell_force, sizeof(float) * 3 * n, k, ctx->stream));

@@ -316,19 +316,17 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
         cen[i] = make_float4((float)ctx->geo.cc[3 * (size_t)c], (float)ctx->geo.cc[3 * (size_t)c + 1],
                              (float)ctx->geo.cc[3 * (size_t)c + 2], (float)ctx->geo.vol[c]);
     }
-    int *dinv, *bk, *br, *bc, *bs, *so, *tf, *sp;
+    int *dinv, *bc, *slots, *ov, *nov, *tf;
     float4 *dcen;
     float *wn;
     int4 *nbs;
     if ((rc = cp_alloc(ctx, &dinv, ctx->lay.inv.data(), nc))) return rc;
     if ((rc = cp_alloc(ctx, &dcen, cen.data(), nc))) return rc;
-    if ((rc = cp_alloc<int>(ctx, &bk, nullptr, nn))) return rc;
-    if ((rc = cp_alloc<int>(ctx, &br, nullptr, nn))) return rc;
     if ((rc = cp_alloc<int>(ctx, &bc, nullptr, g.n_bins))) return rc;
-    if ((rc = cp_alloc<int>(ctx, &bs, nullptr, g.n_bins + 1))) return rc;
-    if ((rc = cp_alloc<int>(ctx, &so, nullptr, nn))) return rc;
+    if ((rc = cp_alloc<int>(ctx, &slots, nullptr, (size_t)IBM_BIN_CAP * g.n_bins))) return rc;
+    if ((rc = cp_alloc<int>(ctx, &ov, nullptr, nn))) return rc;
+    if ((rc = cp_alloc<int>(ctx, &nov, nullptr, 1))) return rc;
     if ((rc = cp_alloc<int>(ctx, &tf, nullptr, g.n_bins))) return rc;
-    if ((rc = cp_alloc<int>(ctx, &sp, nullptr, (g.n_bins + 1023) / 1024))) return rc;
     int *tt;
     unsigned long long *dacc;
     if ((rc = cp_alloc<int>(ctx, &tt, nullptr, g.n_bins))) return rc;
@@ -337,8 +335,8 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     g.acc = dacc;
     if ((rc = cp_alloc<float>(ctx, &wn, nullptr, 64 * (size_t)nn))) return rc;
     if ((rc = cp_alloc<int4>(ctx, &nbs, nullptr, nn))) return rc;
-    g.inv = dinv; g.cen = dcen; g.bin_key = bk; g.bin_rank = br; g.bin_count = bc; g.bin_start = bs;
-    g.sorted = so; g.tile_flag = tf; g.scan_part = sp; g.wnode = wn; g.nbase = nbs;
+    g.inv = dinv; g.cen = dcen; g.bin_count = bc; g.bin_slots = slots; g.overflow = ov; g.n_overflow = nov;
+    g.tile_flag = tf; g.wnode = wn; g.nbase = nbs;
     g.static_force = ctx->static_force;
     g.cell_force = ctx->cell_force;
     g.n_cells = nc;

@@ -7,6 +7,8 @@
 
 namespace lbfs {
 
+constexpr int IBM_BIN_CAP = 128;  // node slots per 4x4x4 bin (fuller bins overflow to a list)
+
 struct MemParamDev {
     float spring_c;           // WLC constant making F(L0) = 0
     float volume_modulus, local_area_modulus, global_area_modulus;
@@ -65,8 +67,9 @@ struct IbmArgs {
     float dt;
     const int *inv;           // caller cell id -> internal id
     const float4 *cen;        // internal order: centroid xyz, volume
-    int *bin_key, *bin_rank, *bin_count, *bin_start, *sorted;
-    int *scan_part;           // per 1024 bins: count total
+    int *bin_count;           // nodes put in each bin this step (cleared by the finalize)
+    int *bin_slots;           // IBM_BIN_CAP node ids per bin
+    int *overflow, *n_overflow;  // nodes of full bins
     float *wnode;             // 64 per node: normalised stencil weights of its current position
     int4 *nbase;              // per node: stencil base cell (logical) and bin
     int *tile_flag;           // per bin tile: holds IBM force this step
@@ -77,10 +80,8 @@ struct IbmArgs {
 };
 
 hipError_t launch_ibm_interp(const MemArgs &m, const IbmArgs &g, const float4 *st, hipStream_t s);
-hipError_t launch_ibm_scan(const IbmArgs &g, hipStream_t s);
 // weights of the current node positions (after attaching membranes)
 hipError_t launch_ibm_weights(const MemArgs &m, const IbmArgs &g, hipStream_t s);
-hipError_t launch_ibm_place(const MemArgs &m, const IbmArgs &g, hipStream_t s);
 hipError_t launch_mem_globals(const MemArgs &m, hipStream_t s);
 hipError_t launch_mem_edges(const MemArgs &m, hipStream_t s);
 hipError_t launch_mem_nodes(const MemArgs &m, hipStream_t s);

@@ -2,9 +2,8 @@
 //
 //   k_ibm_interp   per node: fluid velocity at the node (immersed_boundary_method.cu:14-97
 //                  or the 4-point cosine kernel), explicit advection X += U dt
-//                  (immersed_boundary_method.cu:234-264), bin key of the advected node
-//   k_ibm_scan     exclusive scan of the bin counts (one workgroup), counts cleared
-//   k_ibm_place    nodes placed in bin order
+//                  (immersed_boundary_method.cu:234-264), the advected node put in
+//                  its bin
 //   k_mem_globals  per membrane (one workgroup): volume, area (FP64) and the MAD of
 //                  the previous force call (spring_network.cu:1186-1238,
 //                  lagrangian_object.cpp:601-650)
@@ -88,82 +87,92 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
                         B.tn[0] * (bin_coord(B, 1, base[1]) + B.tn[1] * bin_coord(B, 2, base[2]));
         g.nbase[nd] = make_int4(base[0], base[1], base[2], key);
         if (INTERP && g.binned) {
-            g.bin_key[nd] = key;
-            g.bin_rank[nd] = atomicAdd(&g.bin_count[key], 1);
+            // straight into the bin's slots; a full bin sends the node to the
+            // overflow list (spread node by node)
+            const int r = atomicAdd(&g.bin_count[key], 1);
+            if (r < IBM_BIN_CAP) {
+                g.bin_slots[(size_t)key * IBM_BIN_CAP + r] = nd;
+            } else {
+                const int o = atomicAdd(g.n_overflow, 1);
+                if (o < m.n_nodes) g.overflow[o] = nd;
+            }
         }
     }
-}
-
-// exclusive scan of the bin counts in chunks of 1024 (counts are cleared for
-// the next step), then the chunk offsets
-__global__ __launch_bounds__(1024) void k_ibm_scan1(IbmArgs g)
-{
-    __shared__ int wsum[16];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int b = blockIdx.x * 1024 + threadIdx.x;
-    const int v = b < g.n_bins ? g.bin_count[b] : 0;
-    int x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
-    if (lane == 63) wsum[wave] = x;
-    __syncthreads();
-    if (wave == 0) {
-        int t = lane < 16 ? wsum[lane] : 0;
-#pragma unroll
-        for (int off = 1; off < 16; off <<= 1) {
-            const int y = __shfl_up(t, off, 64);
-            if (lane >= off) t += y;
-        }
-        if (lane < 16) wsum[lane] = t;
-    }
-    __syncthreads();
-    const int excl = x - v + (wave > 0 ? wsum[wave - 1] : 0);
-    if (b < g.n_bins) {
-        g.bin_start[b] = excl;
-        g.bin_count[b] = 0;
-    }
-    if (threadIdx.x == 0) g.scan_part[blockIdx.x] = wsum[15];
-}
-
-__global__ __launch_bounds__(1024) void k_ibm_scan2(IbmArgs g)
-{
-    __shared__ int offset;
-    if (threadIdx.x == 0) {
-        int s = 0;
-        for (int k = 0; k < (int)blockIdx.x; k++) s += g.scan_part[k];
-        offset = s;
-        if (blockIdx.x == gridDim.x - 1) g.bin_start[g.n_bins] = s + g.scan_part[blockIdx.x];
-    }
-    __syncthreads();
-    const int b = blockIdx.x * 1024 + threadIdx.x;
-    if (b < g.n_bins) g.bin_start[b] += offset;
-}
-
-__global__ __launch_bounds__(256) void k_ibm_place(MemArgs m, IbmArgs g)
-{
-    const int nd = blockIdx.x * blockDim.x + threadIdx.x;
-    if (nd >= m.n_nodes) return;
-    g.sorted[g.bin_start[g.bin_key[nd]] + g.bin_rank[nd]] = nd;
 }
 
 // ---------------------------------------------------------------------------
 // IBM: spreading.  One workgroup per node bin (the 4x4x4 block of cells that
-// holds the nodes' stencil base): its nodes' stencils stay inside a 7x7x7
-// window, where the weighted forces are accumulated in LDS (one wavefront per
-// node, one lane per stencil slot), then the window is added to the per-cell
-// accumulators in global memory.  All sums are 64-bit fixed point integer
-// additions, so the result does not depend on the order of the nodes, lanes
-// or workgroups.  k_ibm_finalize then turns the accumulators of the touched
-// tiles into force densities and clears them.
+// holds the nodes' stencil base; its nodes were put in the bin's slots by the
+// interpolation): the nodes' stencils stay inside a 7x7x7 window, where the
+// weighted forces are accumulated in LDS (one wavefront per node, one lane per
+// stencil slot), then the window is added to the per-cell accumulators in
+// global memory.  The node forces are gathered from their edge records here
+// (eight lanes per node, the order of k_mem_nodes) and stored.  Nodes of full
+// bins are spread one wavefront each by the workgroups after the bins.  All
+// sums are 64-bit fixed point integer additions, so the result does not
+// depend on the order of the nodes, lanes or workgroups.  k_ibm_finalize then
+// turns the accumulators of the touched tiles into force densities and
+// clears them.
 // ---------------------------------------------------------------------------
+// sum of a node's edge slots by eight lanes (sub = lane & 7), fixed order
+__device__ __forceinline__ float4 node_force8(const MemArgs &m, int nd, int sub)
+{
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    for (int j = m.inc_start[nd] + sub; j < m.inc_start[nd + 1]; j += 8) {
+        const float4 f = m.edge_force[m.inc[j]];
+        fx += f.x; fy += f.y; fz += f.z;
+    }
+#pragma unroll
+    for (int off = 4; off > 0; off >>= 1) {
+        fx += __shfl_xor(fx, off, 64);
+        fy += __shfl_xor(fy, off, 64);
+        fz += __shfl_xor(fz, off, 64);
+    }
+    return make_float4(fx, fy, fz, 0.0f);
+}
+
+// one node by one wavefront straight into the global accumulators
+__device__ __forceinline__ void spread_node_global(const MemArgs &m, const IbmArgs &g, int nd, float4 F, int lane)
+{
+    const int S = stencil_size(g);
+    if (lane >= S * S * S) return;
+    const float wt = g.wnode[64 * (size_t)nd + lane];
+    if (wt == 0.0f) return;
+    const IbmBlock B = g.blk[g.node_block[nd]];
+    const int4 nb = g.nbase[nd];
+    int off[3];
+    lane_offset(S, lane, off);
+    const int idx[3] = {nb.x + off[0], nb.y + off[1], nb.z + off[2]};
+    int w[3];
+    if (!(wrap_index(B, 0, idx[0], w[0]) && wrap_index(B, 1, idx[1], w[1]) && wrap_index(B, 2, idx[2], w[2]))) return;
+    const int ic = cell_internal(g, B, w);
+    unsigned long long *acc = g.acc + 3 * (size_t)ic;
+    atomicAdd(acc, (unsigned long long)llrint((double)(F.x * wt) * FIX_SCALE));
+    atomicAdd(acc + 1, (unsigned long long)llrint((double)(F.y * wt) * FIX_SCALE));
+    atomicAdd(acc + 2, (unsigned long long)llrint((double)(F.z * wt) * FIX_SCALE));
+    g.tile_touched[B.bin0 + (w[0] >> 2) + B.tn[0] * ((w[1] >> 2) + B.tn[1] * (w[2] >> 2))] = 1;
+}
+
 __global__ __launch_bounds__(256) void k_ibm_spread(MemArgs m, IbmArgs g)
 {
     const int b = blockIdx.x;
-    const int lo = g.bin_start[b], hi = g.bin_start[b + 1];
-    if (lo == hi) return;  // uniform over the workgroup
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (b >= g.n_bins) {
+        // nodes of full bins, one wavefront each
+        const int nov = min(*g.n_overflow, m.n_nodes);
+        for (int k = (b - g.n_bins) * 4 + wave; k < nov; k += (gridDim.x - g.n_bins) * 4) {
+            const int nd = g.overflow[k];
+            float4 F = node_force8(m, nd, lane & 7);
+            F.x = __shfl(F.x, 0, 64);
+            F.y = __shfl(F.y, 0, 64);
+            F.z = __shfl(F.z, 0, 64);
+            if (lane == 0) m.F[nd] = F;
+            spread_node_global(m, g, nd, F, lane);
+        }
+        return;
+    }
+    const int cnt_all = min(g.bin_count[b], IBM_BIN_CAP);
+    if (cnt_all == 0) return;  // uniform over the workgroup
     const IbmBlock B = g.blk[g.bin_block[b]];
     const int lb = b - B.bin0;
     const int bo[3] = {4 * (lb % B.tn[0]), 4 * ((lb / B.tn[0]) % B.tn[1]), 4 * (lb / (B.tn[0] * B.tn[1]))};
@@ -172,22 +181,30 @@ __global__ __launch_bounds__(256) void k_ibm_spread(MemArgs m, IbmArgs g)
     __shared__ int4 cbase[64];
     __shared__ float4 cF[64];
     __shared__ float cw[64][64];   // the chunk's cached stencil weights, one row per node
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int q = tid; q < 3 * 343; q += 256) racc[q / 343][q % 343] = 0ull;
     const int S = stencil_size(g);
     int off[3];
     lane_offset(S, lane, off);
     const bool act = lane < S * S * S;
-    for (int c0 = lo; c0 < hi; c0 += 64) {
-        const int cnt = min(64, hi - c0);
+    const int *slots = g.bin_slots + (size_t)b * IBM_BIN_CAP;
+    for (int c0 = 0; c0 < cnt_all; c0 += 64) {
+        const int cnt = min(64, cnt_all - c0);
         __syncthreads();
         if (tid < cnt) {
-            const int p = g.sorted[c0 + tid];
+            const int p = slots[c0 + tid];
             cnd[tid] = p;
             cbase[tid] = g.nbase[p];
-            cF[tid] = m.F[p];
         }
         __syncthreads();
+        // node forces from the edge records: eight lanes per node
+        for (int jn = tid >> 3; jn < cnt; jn += 32) {
+            const int p = cnd[jn];
+            const float4 F = node_force8(m, p, tid & 7);
+            if ((tid & 7) == 0) {
+                cF[jn] = F;
+                m.F[p] = F;
+            }
+        }
         // all weight rows of the chunk in flight at once (one coalesced row per node)
         for (int q = tid; q < 64 * cnt; q += 256) cw[q >> 6][q & 63] = g.wnode[64 * (size_t)cnd[q >> 6] + (q & 63)];
         __syncthreads();
@@ -259,24 +276,7 @@ __global__ __launch_bounds__(256) void k_ibm_spread_nodes(MemArgs m, IbmArgs g)
     const int lane = threadIdx.x & 63;
     const int nd = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (nd >= m.n_nodes) return;
-    const int S = stencil_size(g);
-    if (lane >= S * S * S) return;
-    const float wt = g.wnode[64 * (size_t)nd + lane];
-    if (wt == 0.0f) return;
-    const IbmBlock B = g.blk[g.node_block[nd]];
-    const int4 nb = g.nbase[nd];
-    int off[3];
-    lane_offset(S, lane, off);
-    const int idx[3] = {nb.x + off[0], nb.y + off[1], nb.z + off[2]};
-    int w[3];
-    if (!(wrap_index(B, 0, idx[0], w[0]) && wrap_index(B, 1, idx[1], w[1]) && wrap_index(B, 2, idx[2], w[2]))) return;
-    const float4 F = m.F[nd];
-    const int ic = cell_internal(g, B, w);
-    unsigned long long *acc = g.acc + 3 * (size_t)ic;
-    atomicAdd(acc, (unsigned long long)llrint((double)(F.x * wt) * FIX_SCALE));
-    atomicAdd(acc + 1, (unsigned long long)llrint((double)(F.y * wt) * FIX_SCALE));
-    atomicAdd(acc + 2, (unsigned long long)llrint((double)(F.z * wt) * FIX_SCALE));
-    g.tile_touched[B.bin0 + (w[0] >> 2) + B.tn[0] * ((w[1] >> 2) + B.tn[1] * (w[2] >> 2))] = 1;
+    spread_node_global(m, g, nd, m.F[nd], lane);
 }
 
 // per 4x4x4 tile: force density of the touched cells from their accumulators
@@ -284,6 +284,10 @@ __global__ __launch_bounds__(256) void k_ibm_spread_nodes(MemArgs m, IbmArgs g)
 __global__ __launch_bounds__(64) void k_ibm_finalize(IbmArgs g)
 {
     const int t = blockIdx.x;
+    if (threadIdx.x == 0 && g.binned) {
+        g.bin_count[t] = 0;  // bins are the tiles; cleared for the next step
+        if (t == 0) *g.n_overflow = 0;
+    }
     const int touched = g.tile_touched[t], flag = g.tile_flag[t];
     if (!touched && !flag) return;
     const IbmBlock B = g.blk[g.bin_block[t]];
@@ -448,18 +452,6 @@ hipError_t launch_ibm_weights(const MemArgs &m, const IbmArgs &g, hipStream_t s)
     hipLaunchKernelGGL(k_ibm_interp<false>, dim3((m.n_nodes + 3) / 4), dim3(256), 0, s, m, g, nullptr);
     return hipGetLastError();
 }
-hipError_t launch_ibm_scan(const IbmArgs &g, hipStream_t s)
-{
-    const int blocks = (g.n_bins + 1023) / 1024;
-    hipLaunchKernelGGL(k_ibm_scan1, dim3(blocks), dim3(1024), 0, s, g);
-    hipLaunchKernelGGL(k_ibm_scan2, dim3(blocks), dim3(1024), 0, s, g);
-    return hipGetLastError();
-}
-hipError_t launch_ibm_place(const MemArgs &m, const IbmArgs &g, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_ibm_place, dim3((m.n_nodes + 255) / 256), dim3(256), 0, s, m, g);
-    return hipGetLastError();
-}
 hipError_t launch_mem_globals(const MemArgs &m, hipStream_t s)
 {
     hipLaunchKernelGGL(k_mem_partials, dim3(m.n_chunks), dim3(256), 0, s, m);
@@ -478,8 +470,8 @@ hipError_t launch_mem_nodes(const MemArgs &m, hipStream_t s)
 }
 hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s)
 {
-    if (g.binned)
-        hipLaunchKernelGGL(k_ibm_spread, dim3(g.n_bins), dim3(256), 0, s, m, g);
+    if (g.binned)  // the bins, then 64 workgroups for the nodes of full bins
+        hipLaunchKernelGGL(k_ibm_spread, dim3(g.n_bins + 64), dim3(256), 0, s, m, g);
     else
         hipLaunchKernelGGL(k_ibm_spread_nodes, dim3((m.n_nodes + 3) / 4), dim3(256), 0, s, m, g);
     hipLaunchKernelGGL(k_ibm_finalize, dim3(g.n_bins), dim3(64), 0, s, g);

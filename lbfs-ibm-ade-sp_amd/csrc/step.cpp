@@ -30,18 +30,6 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
                          return LBFS_OK;
                      },
                      60.0 * nn});
-        if (ibm.binned)
-        s.push_back({"ibm_scan", [ctx]() -> int {
-                         HIPCHK(launch_ibm_scan(ctx->ibm, ctx->stream));
-                         return LBFS_OK;
-                     },
-                     8.0 * ibm.n_bins});
-        if (ibm.binned)
-        s.push_back({"ibm_place", [ctx]() -> int {
-                         HIPCHK(launch_ibm_place(ctx->mem, ctx->ibm, ctx->stream));
-                         return LBFS_OK;
-                     },
-                     16.0 * nn});
         s.push_back({"mem_globals", [ctx]() -> int {
                          HIPCHK(launch_mem_globals(ctx->mem, ctx->stream));
                          return LBFS_OK;
@@ -52,11 +40,12 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
                          return LBFS_OK;
                      },
                      52.0 * ne});
-        s.push_back({"mem_nodes", [ctx]() -> int {
-                         HIPCHK(launch_mem_nodes(ctx->mem, ctx->stream));
-                         return LBFS_OK;
-                     },
-                     36.0 * nn});
+        if (!ibm.binned)  // the binned spreading gathers the node forces itself
+            s.push_back({"mem_nodes", [ctx]() -> int {
+                             HIPCHK(launch_mem_nodes(ctx->mem, ctx->stream));
+                             return LBFS_OK;
+                         },
+                         36.0 * nn});
         if (with_visc)
             s.push_back({"internal_visc", [ctx]() -> int {
                              HIPCHK(launch_internal_visc(ctx->mem, ctx->ibm, ctx->delta_h, ctx->visc_factor, ctx->stream));
@@ -125,18 +114,27 @@ int lbfs_ctx::run_stages(bool with_visc)
         }
         return LBFS_OK;
     }
-    // With membranes the gradients and the face pass, which read only the
-    // state at the start of the step, run on the second stream beside the
-    // coupling chain (a latency-bound sequence of small kernels); after
-    // interpolation the membrane forces (globals, edges, nodes) run on a third
-    // stream beside the node binning.  The update waits for the face pass and
-    // for spreading.  Same kernels, same inputs: the results are those of the
-    // sequential order.
-    HIPCHK(hipEventRecord(ev_fork, stream));
-    HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
-    HIPCHK(launch_gradients(args, st[cur], grad, side));
-    HIPCHK(launch_flux_update(args, st[cur], grad, nullptr, res, 0, side));
-    HIPCHK(hipEventRecord(ev_join, side));
+    // With membranes the gradients and the face pass (which read only the
+    // state at the start of the step) run on the second stream beside the
+    // coupling chain (a latency-bound sequence of small kernels), and the
+    // membrane forces (globals, edges) on a third stream after the
+    // interpolation.  The update waits for the face pass and for spreading.
+    // Same kernels, same inputs: the results are those of the sequential order.
+    // few nodes: the interpolation (latency-bound) first and alone, since
+    // the big fluid launches would slow it; many nodes: it is a big kernel
+    // itself and overlaps the fluid pass from the start
+    const bool interp_first = mem.n_nodes <= 65536;
+    auto fork_fluid = [this]() -> int {
+        lbfs_ctx *ctx = this;
+        HIPCHK(hipEventRecord(ev_fork, stream));
+        HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
+        HIPCHK(launch_gradients(args, st[cur], grad, side));
+        HIPCHK(launch_flux_update(args, st[cur], grad, nullptr, res, 0, side));
+        HIPCHK(hipEventRecord(ev_join, side));
+        return LBFS_OK;
+    };
+    int rc;
+    if (!interp_first && (rc = fork_fluid())) return rc;
     for (auto &s : step_stages(false)) {
         const char *nm = s.name;
         if (std::strcmp(nm, "gradients") == 0 || std::strcmp(nm, "flux_residual") == 0 ||
@@ -145,14 +143,14 @@ int lbfs_ctx::run_stages(bool with_visc)
             continue;
         if (std::strcmp(nm, "ibm_spread") == 0) HIPCHK(hipStreamWaitEvent(stream, ev_join2, 0));
         if (std::strcmp(nm, "update") == 0) HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
-        int rc = s.run();
-        if (rc) return rc;
+        if ((rc = s.run())) return rc;
         if (std::strcmp(nm, "ibm_interp") == 0) {
+            if (interp_first && (rc = fork_fluid())) return rc;
             HIPCHK(hipEventRecord(ev_fork2, stream));
             HIPCHK(hipStreamWaitEvent(side2, ev_fork2, 0));
             HIPCHK(launch_mem_globals(mem, side2));
             HIPCHK(launch_mem_edges(mem, side2));
-            HIPCHK(launch_mem_nodes(mem, side2));
+            if (!ibm.binned) HIPCHK(launch_mem_nodes(mem, side2));
             HIPCHK(hipEventRecord(ev_join2, side2));
         }
     }

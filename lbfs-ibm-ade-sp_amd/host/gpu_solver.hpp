@@ -79,15 +79,13 @@ public:
     {
         stage("mem_globals");
         stage("mem_edges");
-        stage("mem_nodes");
+        stage_if_present("mem_nodes");  // the binned spreading gathers the node forces itself
     }
     // update_interior_viscosities_plucker, when the step is due (t = 1 mod 10000)
     void update_interior_viscosities() { stage_if_present("internal_visc"); }
     // spread_forces_on_structured_grid
     void spread_forces_on_structured_grid()
     {
-        stage_if_present("ibm_scan");   // node binning, used for large node counts
-        stage_if_present("ibm_place");
         stage("ibm_spread");
     }
     // get_interior_gradients + get_bc_gradients

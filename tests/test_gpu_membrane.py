@@ -292,9 +292,30 @@ def test_spreading_paths_are_bitwise_equal(kernel):
         xb, tb = abi.rbc(3, 6.0, center=(18.0, 8.0, 8.0))
         s.add_membranes([abi.Membrane(xa, ta, **PARAMS), abi.Membrane(xb, tb, **PARAMS)], ibm_kernel=kernel)
         names = s.stage_names()
-        assert ("ibm_scan" in names) == (mode == 1)
+        assert ("mem_nodes" in names) == (mode == 0)  # the binned path gathers node forces itself
         s.step(9)
-        outs.append((np.concatenate(s.get_state()), s.cell_force(), s.membrane_nodes()[0]))
+        X, U, F = s.membrane_nodes()
+        outs.append((np.concatenate(s.get_state()), s.cell_force(), X, F))
+        s.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
+def test_full_bins_overflow_to_the_node_list():
+    """A 2562-node cell 6 cells across puts hundreds of nodes in some 4x4x4
+    bins (more than the 128 slots of a bin): those nodes take the per-node
+    path, and the step still equals the per-node spreading bit for bit."""
+    outs = []
+    for mode in (0, 1):
+        mesh = cases.channel_mesh(24, 16, 16, seed=13)
+        s = abi.Solver(mesh, abi.make_params(visc=VISC, dt=DT, body_force=(2e-5, 0.0, 0.0)))
+        s.set_state(*cases.smooth_state(mesh, seed=13, amp=0.005))
+        s.set_ibm_binned(mode)
+        x, t = abi.rbc(16, 6.0, center=(12.0, 8.0, 8.0))
+        s.add_membranes([abi.Membrane(x, t, **PARAMS)], ibm_kernel=abi.IBM_COS4)
+        s.step(3)
+        X, U, F = s.membrane_nodes()
+        outs.append((np.concatenate(s.get_state()), s.cell_force(), X, F))
         s.close()
     for a, b in zip(outs[0], outs[1]):
         assert np.array_equal(a, b)
