@@ -314,9 +314,11 @@ int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next)
     HIPCHK(hipEventRecord(ctx->ev_join2, ctx->side2));
     HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join2, 0));
     HIPCHK(launch_ibm_spread(m, ctx->ibm, ctx->stream));
-    HIPCHK(hipMemcpyAsync(r.cf, ctx->cell_force, sizeof(float) * 3 * n, k, ctx->stream));
     HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
-    HIPCHK(launch_update(a, st, r.res, st_next, ctx->stream));
+    // the step's cell force goes straight into the record
+    const UpdateForce uf{ctx->ibm.acc, ctx->static_force, ctx->ibm.spread_sc, r.cf,
+                         ctx->ibm.binned ? ctx->ibm.n_overflow : nullptr};
+    HIPCHK(launch_update(a, st, r.res, st_next, uf, ctx->stream));
     return LBFS_OK;
 }
 

@@ -163,32 +163,6 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     if ((rc = cp_alloc(ctx, &dns, mem_node_start.data(), mem_node_start.size()))) return rc;
     if ((rc = cp_alloc<float4>(ctx, &dbb, nullptr, 2 * (size_t)count))) return rc;
     if ((rc = cp_alloc<double>(ctx, &ctx->mem_energy, nullptr, 7 * (size_t)count))) return rc;
-    // chunks of about 256 triangles for the two-level volume/area/MAD sums
-    {
-        std::vector<int> chunk, mcs(1, 0);
-        for (int i = 0; i < count; i++) {
-            const int t0 = mem_tri_start[i], nt = mem_tri_start[i + 1] - t0;
-            const int e0 = mem_edge_start[i], nm = params[i].n_mad;
-            const int nch = std::max(1, (nt + 255) / 256);
-            for (int c = 0; c < nch; c++) {
-                chunk.push_back(i);
-                chunk.push_back(t0 + (int)((long long)nt * c / nch));
-                chunk.push_back(t0 + (int)((long long)nt * (c + 1) / nch));
-                chunk.push_back(e0 + (int)((long long)nm * c / nch));
-                chunk.push_back(e0 + (int)((long long)nm * (c + 1) / nch));
-            }
-            mcs.push_back(mcs.back() + nch);
-        }
-        int *dch, *dmcs;
-        double *dpart;
-        if ((rc = cp_alloc(ctx, &dch, chunk.data(), chunk.size()))) return rc;
-        if ((rc = cp_alloc(ctx, &dmcs, mcs.data(), mcs.size()))) return rc;
-        if ((rc = cp_alloc<double>(ctx, &dpart, nullptr, 3 * (size_t)mcs.back()))) return rc;
-        m.n_chunks = mcs.back();
-        m.chunk = dch;
-        m.mem_chunk_start = dmcs;
-        m.part = dpart;
-    }
     m.mem_node_start = dns;
     m.bbox = dbb;
     m.X = dX; m.U = dU; m.F = dF; m.node_mem = dnm; m.origin = dorig; m.tri = dtri;
@@ -316,7 +290,21 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
         cen[i] = make_float4((float)ctx->geo.cc[3 * (size_t)c], (float)ctx->geo.cc[3 * (size_t)c + 1],
                              (float)ctx->geo.cc[3 * (size_t)c + 2], (float)ctx->geo.vol[c]);
     }
-    int *dinv, *bc, *slots, *ov, *nov, *tf;
+    // force density scale of a cell's spread sum: 1 / V (the physical-distance
+    // kernels), 1 / h^3 of its box (the 2-point kernel); float as the kernels
+    std::vector<float> ssc(nc);
+    for (int i = 0; i < nc; i++) {
+        if (ibm_kernel == LBFS_IBM_LINEAR2) {
+            const IbmBlock &B = blk[nblk > 1 ? cell_blk_caller[ctx->lay.perm[i]] : 0];
+            ssc[i] = B.inv_h * B.inv_h * B.inv_h;
+        } else {
+            ssc[i] = 1.0f / cen[i].w;
+        }
+    }
+    float *dssc;
+    if ((rc = cp_alloc(ctx, &dssc, ssc.data(), ssc.size()))) return rc;
+    g.spread_sc = dssc;
+    int *dinv, *bc, *slots, *ov, *nov;
     float4 *dcen;
     float *wn;
     int4 *nbs;
@@ -326,17 +314,13 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     if ((rc = cp_alloc<int>(ctx, &slots, nullptr, (size_t)IBM_BIN_CAP * g.n_bins))) return rc;
     if ((rc = cp_alloc<int>(ctx, &ov, nullptr, nn))) return rc;
     if ((rc = cp_alloc<int>(ctx, &nov, nullptr, 1))) return rc;
-    if ((rc = cp_alloc<int>(ctx, &tf, nullptr, g.n_bins))) return rc;
-    int *tt;
     unsigned long long *dacc;
-    if ((rc = cp_alloc<int>(ctx, &tt, nullptr, g.n_bins))) return rc;
     if ((rc = cp_alloc<unsigned long long>(ctx, &dacc, nullptr, 3 * (size_t)nc))) return rc;
-    g.tile_touched = tt;
     g.acc = dacc;
     if ((rc = cp_alloc<float>(ctx, &wn, nullptr, 64 * (size_t)nn))) return rc;
     if ((rc = cp_alloc<int4>(ctx, &nbs, nullptr, nn))) return rc;
     g.inv = dinv; g.cen = dcen; g.bin_count = bc; g.bin_slots = slots; g.overflow = ov; g.n_overflow = nov;
-    g.tile_flag = tf; g.wnode = wn; g.nbase = nbs;
+    g.wnode = wn; g.nbase = nbs;
     g.static_force = ctx->static_force;
     g.cell_force = ctx->cell_force;
     g.n_cells = nc;

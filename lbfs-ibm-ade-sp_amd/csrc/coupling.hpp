@@ -38,10 +38,6 @@ struct MemArgs {
     double *glob;             // 8 per membrane: V, A, MAD, V0, A0, MAD0, A_prev, 0
     float4 *bbox;             // 2 per membrane: lower and upper corner of the nodes
     const int *mem_block;     // box of each membrane
-    int n_chunks;             // partial-sum chunks of the volume/area/MAD sums
-    const int *chunk;         // 5 per chunk: membrane, triangle range, MAD edge range
-    const int *mem_chunk_start;
-    double *part;             // 3 per chunk: V, A, MAD partial sums
 };
 
 // one logical box of cells (an ensemble member, or the whole mesh)
@@ -67,13 +63,12 @@ struct IbmArgs {
     float dt;
     const int *inv;           // caller cell id -> internal id
     const float4 *cen;        // internal order: centroid xyz, volume
-    int *bin_count;           // nodes put in each bin this step (cleared by the finalize)
+    int *bin_count;           // nodes put in each bin this step (cleared by the spreading)
     int *bin_slots;           // IBM_BIN_CAP node ids per bin
-    int *overflow, *n_overflow;  // nodes of full bins
+    int *overflow, *n_overflow;  // nodes of full bins (the count is cleared by the cell update)
     float *wnode;             // 64 per node: normalised stencil weights of its current position
     int4 *nbase;              // per node: stencil base cell (logical) and bin
-    int *tile_flag;           // per bin tile: holds IBM force this step
-    int *tile_touched;        // per bin tile: received spread force in this step
+    const float *spread_sc;   // [n_cells] force density per unit spread sum: 1/V, or 1/h^3 (2-point)
     unsigned long long *acc;  // 3 per cell (internal order): fixed-point spread force
     const float *static_force;
     float *cell_force;
@@ -85,6 +80,8 @@ hipError_t launch_ibm_weights(const MemArgs &m, const IbmArgs &g, hipStream_t s)
 hipError_t launch_mem_globals(const MemArgs &m, hipStream_t s);
 hipError_t launch_mem_edges(const MemArgs &m, hipStream_t s);
 hipError_t launch_mem_nodes(const MemArgs &m, hipStream_t s);
+// spreading into the fixed-point cell accumulators; the cell update turns them
+// into force densities (launch_update with its UpdateForce)
 hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s);
 // internal viscosity field (immersed_boundary_method.cu:389-527)
 hipError_t launch_internal_visc(const MemArgs &m, const IbmArgs &g, float delta_h, float *visc_factor, hipStream_t s);

@@ -110,9 +110,8 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
 // (eight lanes per node, the order of k_mem_nodes) and stored.  Nodes of full
 // bins are spread one wavefront each by the workgroups after the bins.  All
 // sums are 64-bit fixed point integer additions, so the result does not
-// depend on the order of the nodes, lanes or workgroups.  k_ibm_finalize then
-// turns the accumulators of the touched tiles into force densities and
-// clears them.
+// depend on the order of the nodes, lanes or workgroups.  The cell update
+// turns the accumulators into force densities and clears them.
 // ---------------------------------------------------------------------------
 // sum of a node's edge slots by eight lanes (sub = lane & 7), fixed order
 __device__ __forceinline__ float4 node_force8(const MemArgs &m, int nd, int sub)
@@ -150,7 +149,6 @@ __device__ __forceinline__ void spread_node_global(const MemArgs &m, const IbmAr
     atomicAdd(acc, (unsigned long long)llrint((double)(F.x * wt) * FIX_SCALE));
     atomicAdd(acc + 1, (unsigned long long)llrint((double)(F.y * wt) * FIX_SCALE));
     atomicAdd(acc + 2, (unsigned long long)llrint((double)(F.z * wt) * FIX_SCALE));
-    g.tile_touched[B.bin0 + (w[0] >> 2) + B.tn[0] * ((w[1] >> 2) + B.tn[1] * (w[2] >> 2))] = 1;
 }
 
 __global__ __launch_bounds__(256) void k_ibm_spread(MemArgs m, IbmArgs g)
@@ -263,8 +261,8 @@ __global__ __launch_bounds__(256) void k_ibm_spread(MemArgs m, IbmArgs g)
         if (v0) atomicAdd(acc, v0);
         if (v1) atomicAdd(acc + 1, v1);
         if (v2) atomicAdd(acc + 2, v2);
-        g.tile_touched[B.bin0 + (w[0] >> 2) + B.tn[0] * ((w[1] >> 2) + B.tn[1] * (w[2] >> 2))] = 1;
     }
+    if (tid == 0) g.bin_count[b] = 0;  // read by this workgroup only; cleared for the next step
 }
 
 // Spreading for small node counts: one wavefront per node, one lane per
@@ -279,98 +277,43 @@ __global__ __launch_bounds__(256) void k_ibm_spread_nodes(MemArgs m, IbmArgs g)
     spread_node_global(m, g, nd, m.F[nd], lane);
 }
 
-// per 4x4x4 tile: force density of the touched cells from their accumulators
-// (which are cleared), static force back where IBM force was held last step
-__global__ __launch_bounds__(64) void k_ibm_finalize(IbmArgs g)
-{
-    const int t = blockIdx.x;
-    if (threadIdx.x == 0 && g.binned) {
-        g.bin_count[t] = 0;  // bins are the tiles; cleared for the next step
-        if (t == 0) *g.n_overflow = 0;
-    }
-    const int touched = g.tile_touched[t], flag = g.tile_flag[t];
-    if (!touched && !flag) return;
-    const IbmBlock B = g.blk[g.bin_block[t]];
-    const int lt = t - B.bin0;
-    const int tc[3] = {lt % B.tn[0], (lt / B.tn[0]) % B.tn[1], lt / (B.tn[0] * B.tn[1])};
-    const int lane = threadIdx.x;
-    const int w[3] = {4 * tc[0] + (lane & 3), 4 * tc[1] + ((lane >> 2) & 3), 4 * tc[2] + (lane >> 4)};
-    if (w[0] < B.n[0] && w[1] < B.n[1] && w[2] < B.n[2]) {
-        const int ic = cell_internal(g, B, w);
-        if (touched) {
-            const float sc = g.kernel == LBFS_IBM_LINEAR2 ? B.inv_h * B.inv_h * B.inv_h : 1.0f / g.cen[ic].w;
-            unsigned long long *acc = g.acc + 3 * (size_t)ic;
-            for (int a = 0; a < 3; a++) {
-                const double v = (double)(long long)acc[a] / FIX_SCALE;
-                g.cell_force[3 * (size_t)ic + a] = g.static_force[3 * (size_t)ic + a] + (float)v * sc;
-                acc[a] = 0ull;
-            }
-        } else {
-            for (int a = 0; a < 3; a++) g.cell_force[3 * (size_t)ic + a] = g.static_force[3 * (size_t)ic + a];
-        }
-    }
-    if (lane == 0) {
-        g.tile_flag[t] = touched;
-        g.tile_touched[t] = 0;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // membrane
 // ---------------------------------------------------------------------------
-// volume, area and MAD sums in two levels: partial sums over chunks of about
-// 256 triangles (and the matching share of the MAD edges), then one
-// wavefront per membrane adds its chunks' partials; both in a fixed order
-__global__ __launch_bounds__(256) void k_mem_partials(MemArgs m)
+// volume, area and MAD sums: one workgroup per membrane, FP64, fixed-order
+// tree (lagrangian_object.cpp:601-650, spring_network.cu:1186-1238)
+__global__ __launch_bounds__(1024) void k_mem_globals(MemArgs m)
 {
-    const int *q = m.chunk + 5 * (size_t)blockIdx.x;
-    const int t0 = q[1], t1 = q[2], e0 = q[3], e1 = q[4];
-    __shared__ double sh[3][256];
+    const int mem = blockIdx.x, tid = threadIdx.x;
+    __shared__ double sh[3][1024];
     double V = 0.0, A = 0.0, M = 0.0;
-    for (int t = t0 + threadIdx.x; t < t1; t += 256) {
+    for (int t = m.mem_tri_start[mem] + tid; t < m.mem_tri_start[mem + 1]; t += 1024) {
         const int4 qq = m.tri[t];
         const f3 a = ld3(m.X[qq.x]), b = ld3(m.X[qq.y]), c = ld3(m.X[qq.z]);
         const f3 N = cross(b - a, c - a);
         A += 0.5 * (double)sqrtf(dot(N, N));
         V += (double)dot(N, (a + b + c) * (1.0f / 3.0f)) / 6.0;
     }
-    for (int e = e0 + threadIdx.x; e < e1; e += 256) {
+    const int e0 = m.mem_edge_start[mem], e1 = e0 + m.params[mem].n_mad;
+    for (int e = e0 + tid; e < e1; e += 1024) {
         const float4 s = m.edge_state[e];
         M += (double)s.x * ((double)s.y + (double)s.z) / 3.0;
     }
-    sh[0][threadIdx.x] = V;
-    sh[1][threadIdx.x] = A;
-    sh[2][threadIdx.x] = M;
+    sh[0][tid] = V;
+    sh[1][tid] = A;
+    sh[2][tid] = M;
     __syncthreads();
-    for (int stp = 128; stp > 0; stp >>= 1) {
-        if ((int)threadIdx.x < stp)
-            for (int k = 0; k < 3; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + stp];
+    for (int stp = 512; stp > 0; stp >>= 1) {
+        if (tid < stp)
+            for (int k = 0; k < 3; k++) sh[k][tid] += sh[k][tid + stp];
         __syncthreads();
     }
-    if (threadIdx.x < 3) m.part[3 * (size_t)blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
-}
-
-__global__ __launch_bounds__(64) void k_mem_globals(MemArgs m)
-{
-    const int mem = blockIdx.x, lane = threadIdx.x;
-    double V = 0.0, A = 0.0, M = 0.0;
-    for (int c = m.mem_chunk_start[mem] + lane; c < m.mem_chunk_start[mem + 1]; c += 64) {
-        V += m.part[3 * (size_t)c];
-        A += m.part[3 * (size_t)c + 1];
-        M += m.part[3 * (size_t)c + 2];
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        V += __shfl_xor(V, off, 64);
-        A += __shfl_xor(A, off, 64);
-        M += __shfl_xor(M, off, 64);
-    }
-    if (lane == 0) {
+    if (tid == 0) {
         double *gl = m.glob + 8 * (size_t)mem;
-        gl[2] = M / gl[6];  // MAD of the previous call, over the area it was computed with
-        gl[0] = V;
-        gl[1] = A;
-        gl[6] = A;
+        gl[2] = sh[2][0] / gl[6];  // MAD of the previous call, over the area it was computed with
+        gl[0] = sh[0][0];
+        gl[1] = sh[1][0];
+        gl[6] = sh[1][0];
     }
 }
 
@@ -454,8 +397,7 @@ hipError_t launch_ibm_weights(const MemArgs &m, const IbmArgs &g, hipStream_t s)
 }
 hipError_t launch_mem_globals(const MemArgs &m, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mem_partials, dim3(m.n_chunks), dim3(256), 0, s, m);
-    hipLaunchKernelGGL(k_mem_globals, dim3(m.n_mem), dim3(64), 0, s, m);
+    hipLaunchKernelGGL(k_mem_globals, dim3(m.n_mem), dim3(1024), 0, s, m);
     return hipGetLastError();
 }
 hipError_t launch_mem_edges(const MemArgs &m, hipStream_t s)
@@ -474,7 +416,6 @@ hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s)
         hipLaunchKernelGGL(k_ibm_spread, dim3(g.n_bins + 64), dim3(256), 0, s, m, g);
     else
         hipLaunchKernelGGL(k_ibm_spread_nodes, dim3((m.n_nodes + 3) / 4), dim3(256), 0, s, m, g);
-    hipLaunchKernelGGL(k_ibm_finalize, dim3(g.n_bins), dim3(64), 0, s, g);
     return hipGetLastError();
 }
 

@@ -3,7 +3,8 @@
 //
 //   k_gradients      per-cell least-squares gradients of (rho, u, v, w)
 //                    (LBFS.cu:2177-2240, ghost rows of LBFS.cu:1933-2048 folded in)
-//   k_update         the cell update alone, from a residual of k_flux_update
+//   k_update         the cell update alone, from a residual of k_flux_update,
+//                    with the spread IBM force turned into a force density
 //   k_flux_update    one workgroup per tile of cells: every face touching the
 //                    tile is reconstructed once into LDS (LBFS.cu:2574-2810),
 //                    then each cell gathers its six face fluxes in a fixed
@@ -224,14 +225,28 @@ __global__ __launch_bounds__(NT) void k_flux_update(FluidArgs a, const float4 *_
 // lets the face pass run before the cell force is known (the coupled step
 // spreads the membrane forces meanwhile).  Same arithmetic as the fused form.
 __global__ __launch_bounds__(256) void k_update(FluidArgs a, const float4 *__restrict__ st,
-                                                const float4 *__restrict__ res, float4 *__restrict__ st_out)
+                                                const float4 *__restrict__ res, float4 *__restrict__ st_out,
+                                                UpdateForce uf)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
+    if (uf.n_overflow && i == 0) *uf.n_overflow = 0;
     if (i >= a.n_cells) return;
     const float4 r = res[i];
     float r0 = r.x, r1 = r.y, r2 = r.z, r3 = r.w;
     float fx = a.body_force[0], fy = a.body_force[1], fz = a.body_force[2];
-    if (a.cell_force) {
+    if (uf.acc) {
+        unsigned long long *p = uf.acc + 3 * (size_t)i;
+        const float sc = uf.spread_sc[i];
+        float f[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long long v = (long long)p[c];
+            f[c] = uf.static_force[3 * (size_t)i + c] + (float)((double)v / 1099511627776.0) * sc;
+            if (v != 0) p[c] = 0ull;
+            uf.cell_force_out[3 * (size_t)i + c] = f[c];
+        }
+        fx += f[0]; fy += f[1]; fz += f[2];
+    } else if (a.cell_force) {
         const float *cf = a.cell_force + 3 * (size_t)i;
         fx += cf[0]; fy += cf[1]; fz += cf[2];
     }
@@ -385,9 +400,10 @@ hipError_t launch_face_flux_all(const FluidArgs &a, const float4 *st, const floa
     return dispatch<FaceFluxAll>(a, dim3(blocks), 0, s, st, grad, out, n_tile_faces);
 }
 
-hipError_t launch_update(const FluidArgs &a, const float4 *st, const float4 *res, float4 *st_out, hipStream_t s)
+hipError_t launch_update(const FluidArgs &a, const float4 *st, const float4 *res, float4 *st_out,
+                         const UpdateForce &uf, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_update, dim3((a.n_cells + 255) / 256), dim3(256), 0, s, a, st, res, st_out);
+    hipLaunchKernelGGL(k_update, dim3((a.n_cells + 255) / 256), dim3(256), 0, s, a, st, res, st_out, uf);
     return hipGetLastError();
 }
 

@@ -75,7 +75,8 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
         // and runs beside the coupling chain; the update follows spreading.
         // Face pass: state 16 + gradients 48 + volume/centroid 16 + residual
         // written 16 per cell, plus 40 per face; update: state 16 + residual
-        // 16 + force 12 + state written 16 per cell
+        // 16 + static force 12 + spread sums 24 + scale 4 + force and state
+        // written 28 per cell
         s.push_back({"flux_residual", [ctx]() -> int {
                          HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, nullptr, ctx->res, 0,
                                                    ctx->stream));
@@ -83,11 +84,14 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
                      },
                      96.0 * n + 40.0 * nf});
         s.push_back({"update", [ctx]() -> int {
-                         HIPCHK(launch_update(ctx->args, ctx->st[ctx->cur], ctx->res, ctx->st[ctx->cur ^ 1], ctx->stream));
+                         const UpdateForce uf{ctx->ibm.acc, ctx->static_force, ctx->ibm.spread_sc, ctx->cell_force,
+                                              ctx->ibm.binned ? ctx->ibm.n_overflow : nullptr};
+                         HIPCHK(launch_update(ctx->args, ctx->st[ctx->cur], ctx->res, ctx->st[ctx->cur ^ 1], uf,
+                                              ctx->stream));
                          ctx->cur ^= 1;
                          return LBFS_OK;
                      },
-                     60.0 * n});
+                     100.0 * n});
         return s;
     }
     // pass F: state 16 + gradients 48 + volume/centroid 16 + force 12 + state written 16 per cell,
