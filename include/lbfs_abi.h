@@ -150,6 +150,9 @@ int lbfs_step(lbfs_ctx *ctx, int nsteps);
                                  (binned from 2048 nodes), 0 one
                                  wavefront per node into integer cell accumulators, 1 nodes binned and spread per
                                  bin through LDS; the result is the same bit for bit */
+#define LBFS_OPT_INTERP_FIRST 3 /* coupled step schedule: 0 or -1 (default) the fluid pass from the
+                                   start of the step beside the interpolation, 1 the interpolation
+                                   alone before the fluid pass; the result is the same bit for bit */
 int lbfs_set_option(lbfs_ctx *ctx, int option, int value);
 
 /* per-stage entry points (parity tests) */
@@ -161,14 +164,18 @@ int lbfs_stage_residual(lbfs_ctx *ctx, double *res /* [4 n_cells] */);          
  * API; the groups are the reference's kernel groups of Solver_gpu.cu:749-1246):
  *   ibm_interp    interpolate_velocities_on_nodes + update_node_positions
  *                 (immersed_boundary_method.cu:14-97, 234-264)
- *   ibm_scan, ibm_place, ibm_spread   spread_forces_on_structured_grid (:947-1024)
- *   mem_globals, mem_edges, mem_nodes run_spring_network_functions_atomics
- *                 (Solver_gpu.cu:2311-2361)
+ *   mem_volume, mem_edges, mem_nodes  run_spring_network_functions_atomics
+ *                 (Solver_gpu.cu:2311-2361) with the volume, area and MAD sums
+ *                 (lagrangian_object.cpp:601-650); mem_nodes only without
+ *                 binned spreading
  *   internal_visc update_interior_viscosities_plucker (:389-527), when due
+ *   ibm_spread    spread_forces_on_structured_grid (immersed_boundary_method.cu:947-1024)
  *   gradients     get_interior_gradients + get_bc_gradients (LBFS.cu:1933-2240)
  *   dt_min        get_cfl_device + min (LBFS.cu:2871-2936), time_stepping 3 only
  *   flux_update   calc_face_flux(_bc) + add_face_flux_to_cell + add + time_integration
- *                 (LBFS.cu:2574-2810, 2242-2281, 2089-2172)
+ *                 (LBFS.cu:2574-2810, 2242-2281, 2089-2172), without membranes
+ *   flux_residual, update   the same split in two with membranes: the face pass
+ *                 writes the residual, the update adds the spread force
  * Running them all in order, then lbfs_stage_end_step, equals lbfs_step(ctx, 1). */
 int lbfs_stage_names(lbfs_ctx *ctx, int max_count, const char **names, int *count);
 int lbfs_stage_run(lbfs_ctx *ctx, const char *name);

@@ -308,7 +308,7 @@ int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next)
     HIPCHK(launch_flux_update(a, st, r.grad, nullptr, r.res, 0, ctx->side));
     HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
     HIPCHK(hipStreamWaitEvent(ctx->side2, ctx->ev_fork, 0));
-    HIPCHK(launch_mem_globals(m, ctx->side2));
+    HIPCHK(launch_mem_volume(m, ctx->side2));
     HIPCHK(launch_mem_edges(m, ctx->side2));
     if (!ctx->ibm.binned) HIPCHK(launch_mem_nodes(m, ctx->side2));
     HIPCHK(hipEventRecord(ctx->ev_join2, ctx->side2));
@@ -317,7 +317,7 @@ int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next)
     HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
     // the step's cell force goes straight into the record
     const UpdateForce uf{ctx->ibm.acc, ctx->static_force, ctx->ibm.spread_sc, r.cf,
-                         ctx->ibm.binned ? ctx->ibm.n_overflow : nullptr};
+                         ctx->ibm.binned ? ctx->ibm.n_overflow : nullptr, m.mem_acc, m.glob, m.n_mem};
     HIPCHK(launch_update(a, st, r.res, st_next, uf, ctx->stream));
     return LBFS_OK;
 }

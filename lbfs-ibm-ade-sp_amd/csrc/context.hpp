@@ -64,6 +64,7 @@ struct lbfs_ctx {
     lbfs::AdjointState *adj = nullptr;
 
     // hipGraph of two consecutive steps starting from cur == 0
+    int interp_first_option = -1;   // LBFS_OPT_INTERP_FIRST
     bool use_graph = true;
     hipGraphExec_t graph_exec = nullptr;
     void drop_graph();

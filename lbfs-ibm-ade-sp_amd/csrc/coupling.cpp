@@ -114,6 +114,7 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
         p.visc_ratio = mem[i].internal_viscosity_ratio > 0.0 ? (float)mem[i].internal_viscosity_ratio : 1.0f;
         double *gl = &glob[8 * (size_t)i];
         gl[0] = s.V; gl[1] = s.A; gl[2] = s.MAD; gl[3] = s.V0; gl[4] = s.A0; gl[5] = s.MAD0; gl[6] = s.A;
+        gl[7] = s.MAD;  // the first force call uses the MAD of the initial shape
     }
     const int nn = (int)X.size(), ne = (int)edges.size();
     // node -> edge-slot incidences in (edge, slot) order
@@ -158,6 +159,8 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     if ((rc = cp_alloc(ctx, &dinc, inc.data(), inc.size()))) return rc;
     if ((rc = cp_alloc(ctx, &dpar, params.data(), count))) return rc;
     if ((rc = cp_alloc(ctx, &dgl, glob.data(), glob.size()))) return rc;
+    unsigned long long *dmacc;
+    if ((rc = cp_alloc<unsigned long long>(ctx, &dmacc, nullptr, 4 * (size_t)count))) return rc;
     int *dns;
     float4 *dbb;
     if ((rc = cp_alloc(ctx, &dns, mem_node_start.data(), mem_node_start.size()))) return rc;
@@ -168,7 +171,7 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     m.X = dX; m.U = dU; m.F = dF; m.node_mem = dnm; m.origin = dorig; m.tri = dtri;
     m.mem_tri_start = dts; m.mem_edge_start = des0; m.edges = dedg; m.edge_mem = dem;
     m.edge_const = dec; m.edge_area0 = dea; m.edge_state = des; m.edge_force = def;
-    m.inc_start = dis; m.inc = dinc; m.params = dpar; m.glob = dgl;
+    m.inc_start = dis; m.inc = dinc; m.params = dpar; m.glob = dgl; m.mem_acc = dmacc;
     ctx->mem_node_start = mem_node_start;
     ctx->mem_params = params;
     ctx->mem_edge_const = edge_const;

@@ -35,7 +35,10 @@ struct MemArgs {
     float4 *edge_force;       // 4 per edge: forces on a, b, c, d
     const int *inc_start, *inc;  // node -> (edge * 4 + slot) incidences
     const MemParamDev *params;
-    double *glob;             // 8 per membrane: V, A, MAD, V0, A0, MAD0, A_prev, 0
+    double *glob;             // 8 per membrane: V, A, MAD (of the last force call), V0, A0, MAD0,
+                              // A_prev, MAD of the next force call
+    unsigned long long *mem_acc;  // 4 per membrane: running sums of V, A and the MAD numerator
+                              // (fixed point, MEM_SUM_SCALE), closed out by the cell update
     float4 *bbox;             // 2 per membrane: lower and upper corner of the nodes
     const int *mem_block;     // box of each membrane
 };
@@ -77,7 +80,7 @@ struct IbmArgs {
 hipError_t launch_ibm_interp(const MemArgs &m, const IbmArgs &g, const float4 *st, hipStream_t s);
 // weights of the current node positions (after attaching membranes)
 hipError_t launch_ibm_weights(const MemArgs &m, const IbmArgs &g, hipStream_t s);
-hipError_t launch_mem_globals(const MemArgs &m, hipStream_t s);
+hipError_t launch_mem_volume(const MemArgs &m, hipStream_t s);
 hipError_t launch_mem_edges(const MemArgs &m, hipStream_t s);
 hipError_t launch_mem_nodes(const MemArgs &m, hipStream_t s);
 // spreading into the fixed-point cell accumulators; the cell update turns them

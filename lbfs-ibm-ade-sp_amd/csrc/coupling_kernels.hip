@@ -4,12 +4,13 @@
 //                  or the 4-point cosine kernel), explicit advection X += U dt
 //                  (immersed_boundary_method.cu:234-264), the advected node put in
 //                  its bin
-//   k_mem_globals  per membrane (one workgroup): volume, area (FP64) and the MAD of
-//                  the previous force call (spring_network.cu:1186-1238,
-//                  lagrangian_object.cpp:601-650)
+//   k_mem_volume   per triangle: volume and area sums (lagrangian_object.cpp:601-650)
+//                  in 64-bit fixed point, wavefront-reduced
 //   k_mem_edges    per spring: all membrane force terms (spring_network.cu:613-971),
-//                  written per edge and slot (no atomics)
-//   k_mem_nodes    per node: sum of its edge slots in a fixed order
+//                  written per edge and slot (no atomics); the MAD sum of the next
+//                  force call (spring_network.cu:1186-1238)
+//   k_mem_nodes    per node: sum of its edge slots in a fixed order, combined
+//                  with the membrane's global coefficients
 //   k_ibm_spread   per 4x4x4 bin tile: each cell gathers the forces of the nodes
 //                  binned around it (immersed_boundary_method.cu:947-1024); the sum
 //                  is accumulated in 64-bit fixed point so its value does not depend
@@ -17,10 +18,65 @@
 #include <hip/hip_runtime.h>
 
 #include "coupling.hpp"
+#include "fluid_kernels.hpp"
 #include "ibm_device.hpp"
 #include "membrane_force.hpp"
 
 namespace lbfs {
+
+// Add one value per lane to the membrane sums (64-bit fixed point, so the
+// result does not depend on the order): one atomic per wavefront when its
+// lanes belong to one membrane (mem < 0: no value), lane by lane otherwise.
+__device__ __forceinline__ void mem_sum_add(unsigned long long *acc, int mem, const long long v[2], int nv)
+{
+    int m0 = mem;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m0 = max(m0, __shfl_xor(m0, off, 64));
+    if (m0 < 0) return;
+    if (__all(mem < 0 || mem == m0)) {
+        long long s[2] = {v[0], v[1]};
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int k = 0; k < 2; k++) s[k] += __shfl_xor(s[k], off, 64);
+        if ((threadIdx.x & 63) == 0)
+            for (int k = 0; k < nv; k++) atomicAdd(acc + 4 * (size_t)m0 + k, (unsigned long long)s[k]);
+    } else if (mem >= 0) {
+        for (int k = 0; k < nv; k++) atomicAdd(acc + 4 * (size_t)mem + k, (unsigned long long)v[k]);
+    }
+}
+
+__device__ __forceinline__ long long mem_fx(double x) { return __double2ll_rn(x * MEM_SUM_SCALE); }
+
+// volume and area of triangle t (thread t) added to its membrane's sums
+__device__ __forceinline__ void triangle_sums(const MemArgs &m, int t)
+{
+    int tm = -1;
+    long long v[2] = {0, 0};
+    if (t < m.n_tri) {
+        const int4 qq = m.tri[t];
+        const f3 a = ld3(m.X[qq.x]), b = ld3(m.X[qq.y]), c = ld3(m.X[qq.z]);
+        const f3 N = cross(b - a, c - a);
+        v[0] = mem_fx((double)dot(N, (a + b + c) * (1.0f / 3.0f)) / 6.0);
+        v[1] = mem_fx(0.5 * (double)sqrtf(dot(N, N)));
+        tm = qq.w;
+    }
+    mem_sum_add(m.mem_acc, tm, v, 2);
+}
+
+// the membrane's global coefficients in FP64 from the running volume and area
+// sums and the MAD of this force call, rounded once to the working precision
+__device__ __forceinline__ void mem_coefs(const MemArgs &m, int mem, float &vol, float &area, float &mad)
+{
+    const double *gl = m.glob + 8 * (size_t)mem;
+    const unsigned long long *acc = m.mem_acc + 4 * (size_t)mem;
+    const double V = (double)(long long)acc[0] / MEM_SUM_SCALE, A = (double)(long long)acc[1] / MEM_SUM_SCALE;
+    const MemParamDev &p = m.params[mem];
+    vol = (float)(-(double)p.volume_modulus * (V / gl[3] - 1.0) / 6.0);
+    area = (float)(-(double)p.global_area_modulus * (A / gl[4] - 1.0));
+    mad = (float)(-2.0 * (double)p.global_bending * (gl[7] - gl[5]) / gl[4]);
+}
+
 
 // ---------------------------------------------------------------------------
 // IBM: interpolation, advection, weights, binning.  One wavefront per node,
@@ -113,14 +169,16 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
 // depend on the order of the nodes, lanes or workgroups.  The cell update
 // turns the accumulators into force densities and clears them.
 // ---------------------------------------------------------------------------
-// sum of a node's edge slots by eight lanes (sub = lane & 7), fixed order
+// sum of a node's edge slots by eight lanes (sub = lane & 7; 8 or more: no
+// slots), fixed order
 __device__ __forceinline__ float4 node_force8(const MemArgs &m, int nd, int sub)
 {
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    for (int j = m.inc_start[nd] + sub; j < m.inc_start[nd + 1]; j += 8) {
-        const float4 f = m.edge_force[m.inc[j]];
-        fx += f.x; fy += f.y; fz += f.z;
-    }
+    if (sub < 8)
+        for (int j = m.inc_start[nd] + sub; j < m.inc_start[nd + 1]; j += 8) {
+            const float4 f = m.edge_force[m.inc[j]];
+            fx += f.x; fy += f.y; fz += f.z;
+        }
 #pragma unroll
     for (int off = 4; off > 0; off >>= 1) {
         fx += __shfl_xor(fx, off, 64);
@@ -280,53 +338,8 @@ __global__ __launch_bounds__(256) void k_ibm_spread_nodes(MemArgs m, IbmArgs g)
 // ---------------------------------------------------------------------------
 // membrane
 // ---------------------------------------------------------------------------
-// volume, area and MAD sums: one workgroup per membrane, FP64, fixed-order
-// tree (lagrangian_object.cpp:601-650, spring_network.cu:1186-1238)
-__global__ __launch_bounds__(1024) void k_mem_globals(MemArgs m)
-{
-    const int mem = blockIdx.x, tid = threadIdx.x;
-    __shared__ double sh[3][1024];
-    double V = 0.0, A = 0.0, M = 0.0;
-    for (int t = m.mem_tri_start[mem] + tid; t < m.mem_tri_start[mem + 1]; t += 1024) {
-        const int4 qq = m.tri[t];
-        const f3 a = ld3(m.X[qq.x]), b = ld3(m.X[qq.y]), c = ld3(m.X[qq.z]);
-        const f3 N = cross(b - a, c - a);
-        A += 0.5 * (double)sqrtf(dot(N, N));
-        V += (double)dot(N, (a + b + c) * (1.0f / 3.0f)) / 6.0;
-    }
-    const int e0 = m.mem_edge_start[mem], e1 = e0 + m.params[mem].n_mad;
-    for (int e = e0 + tid; e < e1; e += 1024) {
-        const float4 s = m.edge_state[e];
-        M += (double)s.x * ((double)s.y + (double)s.z) / 3.0;
-    }
-    sh[0][tid] = V;
-    sh[1][tid] = A;
-    sh[2][tid] = M;
-    __syncthreads();
-    for (int stp = 512; stp > 0; stp >>= 1) {
-        if (tid < stp)
-            for (int k = 0; k < 3; k++) sh[k][tid] += sh[k][tid + stp];
-        __syncthreads();
-    }
-    if (tid == 0) {
-        double *gl = m.glob + 8 * (size_t)mem;
-        gl[2] = sh[2][0] / gl[6];  // MAD of the previous call, over the area it was computed with
-        gl[0] = sh[0][0];
-        gl[1] = sh[1][0];
-        gl[6] = sh[1][0];
-    }
-}
 
-// per-membrane global coefficients in FP64 (volume and area constraints,
-// MAD), rounded once to the working precision
-__device__ __forceinline__ void mem_coefs(const MemParamDev &p, const double *gl, float &vol_coef, float &area_coef,
-                                          float &mad_coef)
-{
-    vol_coef = (float)(-(double)p.volume_modulus * (gl[0] / gl[3] - 1.0) / 6.0);
-    area_coef = (float)(-(double)p.global_area_modulus * (gl[1] / gl[4] - 1.0));
-    mad_coef = (float)(-2.0 * (double)p.global_bending * (gl[2] - gl[5]) / gl[4]);
-}
-
+// constants of one edge
 __device__ __forceinline__ memf::EdgeConst edge_const(const MemArgs &m, const MemParamDev &p, int e)
 {
     const float4 ec = m.edge_const[e];
@@ -334,52 +347,61 @@ __device__ __forceinline__ memf::EdgeConst edge_const(const MemArgs &m, const Me
     return {ec.x, ec.z, a0.x, a0.y, p.local_area_modulus, p.viscosity, p.spring_c};
 }
 
+
+// per triangle: volume and area sums
+__global__ __launch_bounds__(256) void k_mem_volume(MemArgs m)
+{
+    triangle_sums(m, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Per spring: the force on its four nodes with the membrane's global
+// coefficients (from the volume and area sums of k_mem_volume and the MAD of
+// this call), one record per edge slot; its MAD contribution (the MAD of the
+// next force call) added to the membrane's sum.
 __global__ __launch_bounds__(128) void k_mem_edges(MemArgs m)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= m.n_edges) return;
-    const int mem = m.edge_mem[e];
-    const MemParamDev p = m.params[mem];
-    memf::EdgeIn<float> in;
-    mem_coefs(p, m.glob + 8 * (size_t)mem, in.vol_coef, in.area_coef, in.mad_coef);
-    const int4 q = m.edges[e];
-    const float4 xa = m.X[q.x], xb = m.X[q.y], xc = m.X[q.z], xd = m.X[q.w], va = m.U[q.x], vc = m.U[q.z];
-    in.A = {xa.x, xa.y, xa.z};
-    in.B = {xb.x, xb.y, xb.z};
-    in.C = {xc.x, xc.y, xc.z};
-    in.D = {xd.x, xd.y, xd.z};
-    in.Ua = {va.x, va.y, va.z};
-    in.Uc = {vc.x, vc.y, vc.z};
-    in.ks = m.edge_const[e].y;
-    in.lb = p.local_bending;
-    in.gb = p.global_bending;
-    memf::EdgeOut<float> out;
-    memf::edge_force(in, edge_const(m, p, e), out);
-    m.edge_state[e] = make_float4(out.kappa, out.n1, out.n2, 0.0f);
+    int em = -1;
+    long long mv[2] = {0, 0};
+    if (e < m.n_edges) {
+        const int mem = m.edge_mem[e];
+        const MemParamDev p = m.params[mem];
+        memf::EdgeIn<float> in;
+        mem_coefs(m, mem, in.vol_coef, in.area_coef, in.mad_coef);
+        const int4 q = m.edges[e];
+        const float4 xa = m.X[q.x], xb = m.X[q.y], xc = m.X[q.z], xd = m.X[q.w], va = m.U[q.x], vc = m.U[q.z];
+        in.A = {xa.x, xa.y, xa.z};
+        in.B = {xb.x, xb.y, xb.z};
+        in.C = {xc.x, xc.y, xc.z};
+        in.D = {xd.x, xd.y, xd.z};
+        in.Ua = {va.x, va.y, va.z};
+        in.Uc = {vc.x, vc.y, vc.z};
+        in.ks = m.edge_const[e].y;
+        in.lb = p.local_bending;
+        in.gb = p.global_bending;
+        memf::EdgeOut<float> out;
+        memf::edge_force(in, edge_const(m, p, e), out);
+        m.edge_state[e] = make_float4(out.kappa, out.n1, out.n2, 0.0f);
 #pragma unroll
-    for (int j = 0; j < 4; j++) m.edge_force[4 * (size_t)e + j] = make_float4(out.F[j].x, out.F[j].y, out.F[j].z, 0.0f);
+        for (int j = 0; j < 4; j++)
+            m.edge_force[4 * (size_t)e + j] = make_float4(out.F[j].x, out.F[j].y, out.F[j].z, 0.0f);
+        if (e - m.mem_edge_start[mem] < p.n_mad) {
+            em = mem;
+            mv[0] = mem_fx((double)out.kappa * ((double)out.n1 + (double)out.n2) / 3.0);
+        }
+    }
+    // the MAD sum goes to the third slot
+    mem_sum_add(m.mem_acc + 2, em, mv, 1);
 }
 
-// eight lanes per node, each summing every eighth incidence; the eight
-// partial sums are combined by a butterfly, so the order is fixed
+// eight lanes per node, each summing every eighth incidence
 __global__ __launch_bounds__(256) void k_mem_nodes(MemArgs m)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int nd = gid >> 3, sub = gid & 7;
     const bool live = nd < m.n_nodes;
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    if (live)
-        for (int j = m.inc_start[nd] + sub; j < m.inc_start[nd + 1]; j += 8) {
-            const float4 f = m.edge_force[m.inc[j]];
-            fx += f.x; fy += f.y; fz += f.z;
-        }
-#pragma unroll
-    for (int off = 4; off > 0; off >>= 1) {
-        fx += __shfl_xor(fx, off, 64);
-        fy += __shfl_xor(fy, off, 64);
-        fz += __shfl_xor(fz, off, 64);
-    }
-    if (live && sub == 0) m.F[nd] = make_float4(fx, fy, fz, 0.0f);
+    const float4 F = node_force8(m, live ? nd : 0, live ? sub : 8);
+    if (live && sub == 0) m.F[nd] = F;
 }
 
 // ---------------------------------------------------------------------------
@@ -395,14 +417,14 @@ hipError_t launch_ibm_weights(const MemArgs &m, const IbmArgs &g, hipStream_t s)
     hipLaunchKernelGGL(k_ibm_interp<false>, dim3((m.n_nodes + 3) / 4), dim3(256), 0, s, m, g, nullptr);
     return hipGetLastError();
 }
-hipError_t launch_mem_globals(const MemArgs &m, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_mem_globals, dim3(m.n_mem), dim3(1024), 0, s, m);
-    return hipGetLastError();
-}
 hipError_t launch_mem_edges(const MemArgs &m, hipStream_t s)
 {
     hipLaunchKernelGGL(k_mem_edges, dim3((m.n_edges + 127) / 128), dim3(128), 0, s, m);
+    return hipGetLastError();
+}
+hipError_t launch_mem_volume(const MemArgs &m, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mem_volume, dim3((m.n_tri + 255) / 256), dim3(256), 0, s, m);
     return hipGetLastError();
 }
 hipError_t launch_mem_nodes(const MemArgs &m, hipStream_t s)

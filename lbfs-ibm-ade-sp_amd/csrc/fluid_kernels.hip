@@ -230,6 +230,19 @@ __global__ __launch_bounds__(256) void k_update(FluidArgs a, const float4 *__res
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (uf.n_overflow && i == 0) *uf.n_overflow = 0;
+    if (uf.mem_acc)
+        for (int k = i; k < uf.n_mem; k += gridDim.x * 256) {
+            unsigned long long *s = uf.mem_acc + 4 * (size_t)k;
+            double *gl = uf.glob + 8 * (size_t)k;
+            const double V = (double)(long long)s[0] / MEM_SUM_SCALE, A = (double)(long long)s[1] / MEM_SUM_SCALE;
+            const double M = (double)(long long)s[2] / MEM_SUM_SCALE;
+            gl[0] = V;
+            gl[1] = A;
+            gl[2] = gl[7];
+            gl[6] = A;
+            gl[7] = M / A;
+            s[0] = s[1] = s[2] = 0ull;
+        }
     if (i >= a.n_cells) return;
     const float4 r = res[i];
     float r0 = r.x, r1 = r.y, r2 = r.z, r3 = r.w;

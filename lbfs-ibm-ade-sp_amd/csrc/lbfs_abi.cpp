@@ -400,6 +400,12 @@ extern "C" int lbfs_set_option(lbfs_ctx *ctx, int option, int value)
         ctx->ibm_binned_option = value;  // applies to the next lbfs_add_membranes
         return LBFS_OK;
     }
+    if (option == LBFS_OPT_INTERP_FIRST) {
+        if (value < -1 || value > 1) return ctx->fail(LBFS_E_INVALID_ARG, "LBFS_OPT_INTERP_FIRST takes -1, 0 or 1");
+        ctx->interp_first_option = value;
+        ctx->drop_graph();
+        return LBFS_OK;
+    }
     return ctx->fail(LBFS_E_INVALID_ARG, "unknown option %d", option);
 }
 

@@ -29,17 +29,20 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
                          HIPCHK(launch_ibm_interp(ctx->mem, ctx->ibm, ctx->st[ctx->cur], ctx->stream));
                          return LBFS_OK;
                      },
-                     60.0 * nn});
-        s.push_back({"mem_globals", [ctx]() -> int {
-                         HIPCHK(launch_mem_globals(ctx->mem, ctx->stream));
+                     60.0 * nn + 16.0 * ne});
+        // volume and area sums: triangle 16 + nodes 48 per triangle; edge
+        // forces: nodes 96 + indices 24 + constants 24 + state 16 + forces
+        // written 64 per edge
+        s.push_back({"mem_volume", [ctx]() -> int {
+                         HIPCHK(launch_mem_volume(ctx->mem, ctx->stream));
                          return LBFS_OK;
                      },
-                     32.0 * nt + 16.0 * ne});
+                     64.0 * nt});
         s.push_back({"mem_edges", [ctx]() -> int {
                          HIPCHK(launch_mem_edges(ctx->mem, ctx->stream));
                          return LBFS_OK;
                      },
-                     52.0 * ne});
+                     224.0 * ne});
         if (!ibm.binned)  // the binned spreading gathers the node forces itself
             s.push_back({"mem_nodes", [ctx]() -> int {
                              HIPCHK(launch_mem_nodes(ctx->mem, ctx->stream));
@@ -85,7 +88,8 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
                      96.0 * n + 40.0 * nf});
         s.push_back({"update", [ctx]() -> int {
                          const UpdateForce uf{ctx->ibm.acc, ctx->static_force, ctx->ibm.spread_sc, ctx->cell_force,
-                                              ctx->ibm.binned ? ctx->ibm.n_overflow : nullptr};
+                                              ctx->ibm.binned ? ctx->ibm.n_overflow : nullptr, ctx->mem.mem_acc,
+                                              ctx->mem.glob, ctx->mem.n_mem};
                          HIPCHK(launch_update(ctx->args, ctx->st[ctx->cur], ctx->res, ctx->st[ctx->cur ^ 1], uf,
                                               ctx->stream));
                          ctx->cur ^= 1;
@@ -121,13 +125,12 @@ int lbfs_ctx::run_stages(bool with_visc)
     // With membranes the gradients and the face pass (which read only the
     // state at the start of the step) run on the second stream beside the
     // coupling chain (a latency-bound sequence of small kernels), and the
-    // membrane forces (globals, edges) on a third stream after the
-    // interpolation.  The update waits for the face pass and for spreading.
+    // membrane forces on a third stream after the interpolation.  The update
+    // waits for the face pass and for spreading.
     // Same kernels, same inputs: the results are those of the sequential order.
-    // few nodes: the interpolation (latency-bound) first and alone, since
-    // the big fluid launches would slow it; many nodes: it is a big kernel
-    // itself and overlaps the fluid pass from the start
-    const bool interp_first = mem.n_nodes <= 65536;
+    // The fluid pass starts with the step, beside the interpolation
+    // (LBFS_OPT_INTERP_FIRST = 1: after it)
+    const bool interp_first = interp_first_option == 1;
     auto fork_fluid = [this]() -> int {
         lbfs_ctx *ctx = this;
         HIPCHK(hipEventRecord(ev_fork, stream));
@@ -142,7 +145,7 @@ int lbfs_ctx::run_stages(bool with_visc)
     for (auto &s : step_stages(false)) {
         const char *nm = s.name;
         if (std::strcmp(nm, "gradients") == 0 || std::strcmp(nm, "flux_residual") == 0 ||
-            std::strcmp(nm, "mem_globals") == 0 || std::strcmp(nm, "mem_edges") == 0 ||
+            std::strcmp(nm, "mem_volume") == 0 || std::strcmp(nm, "mem_edges") == 0 ||
             std::strcmp(nm, "mem_nodes") == 0)
             continue;
         if (std::strcmp(nm, "ibm_spread") == 0) HIPCHK(hipStreamWaitEvent(stream, ev_join2, 0));
@@ -152,7 +155,7 @@ int lbfs_ctx::run_stages(bool with_visc)
             if (interp_first && (rc = fork_fluid())) return rc;
             HIPCHK(hipEventRecord(ev_fork2, stream));
             HIPCHK(hipStreamWaitEvent(side2, ev_fork2, 0));
-            HIPCHK(launch_mem_globals(mem, side2));
+            HIPCHK(launch_mem_volume(mem, side2));
             HIPCHK(launch_mem_edges(mem, side2));
             if (!ibm.binned) HIPCHK(launch_mem_nodes(mem, side2));
             HIPCHK(hipEventRecord(ev_join2, side2));

@@ -77,7 +77,7 @@ public:
     // run_spring_network_functions_atomics
     void run_spring_network_functions_atomics()
     {
-        stage("mem_globals");
+        stage("mem_volume");
         stage("mem_edges");
         stage_if_present("mem_nodes");  // the binned spreading gathers the node forces itself
     }

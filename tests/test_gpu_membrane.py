@@ -319,3 +319,23 @@ def test_full_bins_overflow_to_the_node_list():
         s.close()
     for a, b in zip(outs[0], outs[1]):
         assert np.array_equal(a, b)
+
+
+def test_step_schedules_are_bitwise_equal():
+    """The two step schedules (interpolation before or beside the fluid pass)
+    give the same step bit for bit."""
+    outs = []
+    for first in (0, 1):
+        mesh = cases.channel_mesh(24, 16, 16, seed=14)
+        s = abi.Solver(mesh, abi.make_params(visc=VISC, dt=DT, body_force=(2e-5, 0.0, 0.0)))
+        s.set_state(*cases.smooth_state(mesh, seed=14, amp=0.005))
+        s.set_interp_first(first)
+        xa, ta = abi.rbc(4, 8.0, center=(8.0, 8.0, 8.0))
+        xb, tb = abi.rbc(3, 6.0, center=(18.0, 8.0, 8.0))
+        s.add_membranes([abi.Membrane(xa, ta, **PARAMS), abi.Membrane(xb, tb, **PARAMS)], ibm_kernel=abi.IBM_COS4)
+        s.step(12)
+        X, U, F = s.membrane_nodes()
+        outs.append((np.concatenate(s.get_state()), X, F, s.membrane_globals()))
+        s.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
