@@ -179,14 +179,14 @@ __device__ __forceinline__ void face_vjp(const FluidArgs &a, const float4 *__res
                                          const float4 *__restrict__ grad, const float4 *__restrict__ mu, int tf,
                                          float4 out[8])
 {
-    const float4 fa = a.fA[tf], fb = a.fB[tf], fcv = a.fC[tf];
-    const int2 on = a.fI[tf];
-    const int o = on.x, nb = on.y;
+    const FaceRec fr = load_face(a, tf);
+    const float4 fb = fr.d1, fcv = fr.d2;
+    const int o = fr.o, nb = fr.nb;
     constexpr bool ghost = GHOST;  // boundary faces follow the interior faces in every tile list
     const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
     for (int k = 0; k < 8; k++) out[k] = zero;
-    const float nx = fa.x, ny = fa.y, nz = fa.z, A = fa.w, dt = fb.w;
+    const float nx = fr.nx, ny = fr.ny, nz = fr.nz, A = fr.A, dt = fr.dt;
     const float4 so = st[o];
     const float4 mo = mu[o];
     float go[12], gn[12];

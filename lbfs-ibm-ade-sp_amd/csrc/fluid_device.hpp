@@ -30,6 +30,29 @@ __device__ __forceinline__ void load_grad(const float4 *__restrict__ grad, int i
     g[8] = c.x; g[9] = c.y; g[10] = c.z; g[11] = c.w;
 }
 
+// one tile-face record: unit normal and area from the area-weighted normal
+struct FaceRec {
+    float nx, ny, nz, A, dt;
+    float4 d1, d2;  // owner and neighbour image centroid - face centroid (w: the ids' bits)
+    int o, nb;
+};
+__device__ __forceinline__ FaceRec load_face(const FluidArgs &a, int tf)
+{
+    const float4 fa = a.fA[tf];
+    FaceRec r;
+    r.d1 = a.fB[tf];
+    r.d2 = a.fC[tf];
+    r.A = sqrtf(fa.x * fa.x + fa.y * fa.y + fa.z * fa.z);
+    const float iA = r.A > 0.0f ? 1.0f / r.A : 0.0f;
+    r.nx = fa.x * iA;
+    r.ny = fa.y * iA;
+    r.nz = fa.z * iA;
+    r.dt = fa.w;
+    r.o = __float_as_int(r.d1.w);
+    r.nb = __float_as_int(r.d2.w);
+    return r;
+}
+
 template <class L>
 constexpr bool opposite_pairs()
 {

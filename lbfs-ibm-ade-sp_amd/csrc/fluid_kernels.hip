@@ -81,11 +81,11 @@ __device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__
                                             const float4 *__restrict__ grad, int tf)
 {
     static_assert(opposite_pairs<L>(), "lattice directions must come in opposite pairs after the rest point");
-    const float4 fa = a.fA[tf], fb = a.fB[tf], fcv = a.fC[tf];
-    const int2 on = a.fI[tf];
-    const int o = on.x, nb = on.y;
+    const FaceRec fr = load_face(a, tf);
+    const float4 fb = fr.d1, fcv = fr.d2;
+    const int o = fr.o, nb = fr.nb;
     const bool ghost = nb >= a.n_cells;
-    const float nx = fa.x, ny = fa.y, nz = fa.z, A = fa.w, dt = fb.w;
+    const float nx = fr.nx, ny = fr.ny, nz = fr.nz, A = fr.A, dt = fr.dt;
     const float4 so = st[o];
     float go[12], gn[12];
     load_grad(grad, o, go);

@@ -26,8 +26,7 @@ struct FluidArgs {
     const int *tile_cell_start, *tile_face_start;
     const int *tile_bface_start;   // first boundary face of each tile's list
     // tile faces
-    const float4 *fA, *fB, *fC;
-    const int2 *fI;
+    const float4 *fA, *fB, *fC;    // normal * area, dt | owner offset, owner | neighbour offset, neighbour
     // boundary conditions
     const int2 *bc_type;           // (rho type, vel type)
     const float4 *bc_val;          // (rho - rho0, u, v, w)

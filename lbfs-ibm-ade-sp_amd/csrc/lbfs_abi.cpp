@@ -93,7 +93,7 @@ static int fluid_setup(lbfs_ctx *ctx, const lbfs_mesh_desc &m, const lbfs_params
     a.dt = (float)p.dt;
     for (int t = 0; t < 3; t++) a.body_force[t] = (float)p.body_force[t];
     int *stencil; float *gw, *inv_vol; int16_t *slots; int *tcs, *tfs;
-    float4 *fA, *fB, *fC; int2 *fI, *bct; float4 *bcv;
+    float4 *fA, *fB, *fC; int2 *bct; float4 *bcv;
     if ((rc = dev_upload(ctx, &stencil, d.stencil.data(), d.stencil.size()))) return rc;
     if ((rc = dev_upload(ctx, &gw, d.gw.data(), d.gw.size()))) return rc;
     if ((rc = dev_upload(ctx, &inv_vol, d.inv_vol.data(), d.inv_vol.size()))) return rc;
@@ -107,7 +107,6 @@ static int fluid_setup(lbfs_ctx *ctx, const lbfs_mesh_desc &m, const lbfs_params
     if ((rc = dev_upload(ctx, &fA, (const float4 *)d.fA.data(), ntf))) return rc;
     if ((rc = dev_upload(ctx, &fB, (const float4 *)d.fB.data(), ntf))) return rc;
     if ((rc = dev_upload(ctx, &fC, (const float4 *)d.fC.data(), ntf))) return rc;
-    if ((rc = dev_upload(ctx, &fI, (const int2 *)d.fI.data(), ntf))) return rc;
     std::vector<int2> bt(nbc > 0 ? nbc : 1);
     ctx->bc_vel_type.assign(nbc, LBFS_BC_NEUMANN);
     std::vector<float4> bvals(nbc > 0 ? nbc : 1);
@@ -149,7 +148,7 @@ static int fluid_setup(lbfs_ctx *ctx, const lbfs_mesh_desc &m, const lbfs_params
     if ((rc = dev_upload(ctx, &bcv, bvals.data(), bvals.size()))) return rc;
     a.stencil = stencil; a.gw = gw; a.inv_vol = inv_vol; a.cell_slots = slots;
     a.tile_cell_start = tcs; a.tile_face_start = tfs;
-    a.fA = fA; a.fB = fB; a.fC = fC; a.fI = fI;
+    a.fA = fA; a.fB = fB; a.fC = fC;
     a.bc_type = bct; a.bc_val = bcv;
     a.visc_factor = nullptr;
     // cell force density: body force + closure term, IBM forces are added on top

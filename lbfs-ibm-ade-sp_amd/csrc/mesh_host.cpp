@@ -378,7 +378,7 @@ int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<d
     // tile face lists: every face touching a cell of the tile, in caller face order
     d.tile_face_start.assign(1, 0);
     d.tile_bface_start.clear();
-    d.fA.clear(); d.fB.clear(); d.fC.clear(); d.fI.clear(); d.face_ref.clear();
+    d.fA.clear(); d.fB.clear(); d.fC.clear(); d.face_ref.clear();
     d.face_first.assign(g.n_faces, -1);
     d.max_tile_faces = 0;
     std::vector<int> faces;
@@ -398,12 +398,15 @@ int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<d
             const V3 xf = load3(&g.fc[3 * (size_t)f]);
             const V3 d1 = load3(&g.cc[3 * (size_t)o]) - xf;
             const V3 d2 = load3(&g.cc[3 * (size_t)n]) + load3(&g.shift[3 * (size_t)f]) - xf;
-            d.fA.insert(d.fA.end(), {(float)g.normal[3 * (size_t)f], (float)g.normal[3 * (size_t)f + 1],
-                                     (float)g.normal[3 * (size_t)f + 2], (float)g.area[f]});
-            d.fB.insert(d.fB.end(), {(float)d1.x, (float)d1.y, (float)d1.z, (float)g.fdt[f]});
-            d.fC.insert(d.fC.end(), {(float)d2.x, (float)d2.y, (float)d2.z, 0.0f});
-            d.fI.push_back(d.inv[o]);
-            d.fI.push_back(n < nc ? d.inv[n] : n);
+            const double A = g.area[f];
+            const int on[2] = {d.inv[o], n < nc ? d.inv[n] : n};
+            float ob, nbb;
+            std::memcpy(&ob, &on[0], 4);
+            std::memcpy(&nbb, &on[1], 4);
+            d.fA.insert(d.fA.end(), {(float)(g.normal[3 * (size_t)f] * A), (float)(g.normal[3 * (size_t)f + 1] * A),
+                                     (float)(g.normal[3 * (size_t)f + 2] * A), (float)g.fdt[f]});
+            d.fB.insert(d.fB.end(), {(float)d1.x, (float)d1.y, (float)d1.z, ob});
+            d.fC.insert(d.fC.end(), {(float)d2.x, (float)d2.y, (float)d2.z, nbb});
             if (d.face_first[f] < 0) d.face_first[f] = (int)d.face_ref.size();
             d.face_ref.push_back(f);
         }

@@ -75,10 +75,12 @@ struct DeviceLayout {
     std::vector<double> closure;  // 3 * n, caller order: sum of n A over the faces that carry flux
     std::vector<int16_t> cell_slots; // 6 * n, AoS [i*6+k]: tile-local face slot, sign in bit 15 (1 = owner)
     // per tile-face (duplicated across tiles)
-    std::vector<float> fA;        // 4 per face: normal xyz, area
-    std::vector<float> fB;        // 4 per face: owner centroid - face centroid, face dt
-    std::vector<float> fC;        // 4 per face: neighbour image centroid - face centroid, 0
-    std::vector<int> fI;          // 2 per face: owner, neighbour (ghost: n_cells + bc)
+    // 48 B per face: the area-weighted normal (area = its length), the
+    // centroid offsets, and the owner and neighbour ids as int bits
+    std::vector<float> fA;        // 4 per face: normal * area xyz, face dt
+    std::vector<float> fB;        // 4 per face: owner centroid - face centroid, owner id
+    std::vector<float> fC;        // 4 per face: neighbour image centroid - face centroid, neighbour id
+                                  // (ghost: n_cells + bc)
     std::vector<int> face_ref;    // caller face id of each tile-face
     std::vector<int> face_first;  // caller face id -> one tile-face holding it
     // boundary faces
