@@ -173,9 +173,9 @@ int lbfs_stage_residual(lbfs_ctx *ctx, double *res /* [4 n_cells] */);          
  *   gradients     get_interior_gradients + get_bc_gradients (LBFS.cu:1933-2240)
  *   dt_min        get_cfl_device + min (LBFS.cu:2871-2936), time_stepping 3 only
  *   flux_update   calc_face_flux(_bc) + add_face_flux_to_cell + add + time_integration
- *                 (LBFS.cu:2574-2810, 2242-2281, 2089-2172), without membranes
- *   flux_residual, update   the same split in two with membranes: the face pass
- *                 writes the residual, the update adds the spread force
+ *                 (LBFS.cu:2574-2810, 2242-2281, 2089-2172) with the static force
+ *   ibm_apply     with membranes: the spread force density added to the update
+ *                 (common_kernels.cu:220-233) in the tiles the spreading reached
  * Running them all in order, then lbfs_stage_end_step, equals lbfs_step(ctx, 1). */
 int lbfs_stage_names(lbfs_ctx *ctx, int max_count, const char **names, int *count);
 int lbfs_stage_run(lbfs_ctx *ctx, const char *name);

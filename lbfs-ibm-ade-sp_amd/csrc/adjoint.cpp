@@ -305,7 +305,7 @@ int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next)
     HIPCHK(hipEventRecord(ctx->ev_fork, ctx->stream));
     HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
     HIPCHK(launch_gradients(a, st, r.grad, ctx->side));
-    HIPCHK(launch_flux_update(a, st, r.grad, nullptr, r.res, 0, ctx->side));
+    HIPCHK(launch_flux_update(a, st, r.grad, st_next, r.res, 1, ctx->side));
     HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
     HIPCHK(hipStreamWaitEvent(ctx->side2, ctx->ev_fork, 0));
     HIPCHK(launch_mem_volume(m, ctx->side2));
@@ -315,10 +315,10 @@ int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next)
     HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join2, 0));
     HIPCHK(launch_ibm_spread(m, ctx->ibm, ctx->stream));
     HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
-    // the step's cell force goes straight into the record
-    const UpdateForce uf{ctx->ibm.acc, ctx->static_force, ctx->ibm.spread_sc, r.cf,
-                         ctx->ibm.binned ? ctx->ibm.n_overflow : nullptr, m.mem_acc, m.glob, m.n_mem};
-    HIPCHK(launch_update(a, st, r.res, st_next, uf, ctx->stream));
+    // the step's cell force goes straight into the record (every cell)
+    lbfs::ApplyArgs p = ctx->apply_args(st_next, r.cf, 1);
+    p.glob = m.glob;
+    HIPCHK(launch_ibm_apply(ctx->ibm, p, ctx->stream));
     return LBFS_OK;
 }
 

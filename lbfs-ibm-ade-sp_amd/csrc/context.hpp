@@ -68,6 +68,22 @@ struct lbfs_ctx {
     bool use_graph = true;
     hipGraphExec_t graph_exec = nullptr;
     void drop_graph();
+    // arguments of the apply pass that ends a coupled step
+    lbfs::ApplyArgs apply_args(float4 *st_new, float *cell_force_out, int all) const
+    {
+        lbfs::ApplyArgs p{};
+        p.tile_cell_start = args.tile_cell_start;
+        p.n_tiles = lay.n_tiles;
+        p.dt = args.dt;
+        p.rho0 = args.rho0;
+        p.st = st_new;
+        p.cell_force_out = cell_force_out;
+        p.all = all;
+        p.mem_acc = mem.mem_acc;
+        p.glob = mem.glob;
+        p.n_mem = mem.n_mem;
+        return p;
+    }
 
     int fail(int code, const char *fmt, ...);
     int enqueue_step();

@@ -327,6 +327,20 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     g.static_force = ctx->static_force;
     g.cell_force = ctx->cell_force;
     g.n_cells = nc;
+    // fluid tile of each cell, and the per-tile flags of the apply pass
+    {
+        const int nt = ctx->lay.n_tiles;
+        std::vector<int> ct(nc);
+        for (int t = 0; t < nt; t++)
+            for (int i = ctx->lay.tile_cell_start[t]; i < ctx->lay.tile_cell_start[t + 1]; i++) ct[i] = t;
+        int *dct, *dnow, *dprev;
+        if ((rc = cp_alloc(ctx, &dct, ct.data(), ct.size()))) return rc;
+        if ((rc = cp_alloc<int>(ctx, &dnow, nullptr, nt))) return rc;
+        if ((rc = cp_alloc<int>(ctx, &dprev, nullptr, nt))) return rc;
+        g.cell_tile = dct;
+        g.tile_now = dnow;
+        g.tile_prev = dprev;
+    }
     // internal viscosity field, only when some membrane has a ratio other than 1
     bool any_ratio = false;
     for (int i = 0; i < count; i++) any_ratio |= params[i].visc_ratio != 1.0f;

@@ -8,6 +8,8 @@
 namespace lbfs {
 
 constexpr int IBM_BIN_CAP = 128;  // node slots per 4x4x4 bin (fuller bins overflow to a list)
+// scale of the membranes' 64-bit fixed-point sums (volume, area, MAD numerator)
+constexpr double MEM_SUM_SCALE = 1073741824.0;  // 2^30
 
 struct MemParamDev {
     float spring_c;           // WLC constant making F(L0) = 0
@@ -75,6 +77,28 @@ struct IbmArgs {
     unsigned long long *acc;  // 3 per cell (internal order): fixed-point spread force
     const float *static_force;
     float *cell_force;
+    const int *cell_tile;     // [n_cells] fluid tile of each cell
+    int *tile_now;            // [n_tiles] 1: the spreading wrote into the tile this step
+    int *tile_prev;           // [n_tiles] 1: the cell force output holds spread force there
+};
+
+// The end of the coupled step: adds the spread force density to the updated
+// state (the face pass updated it with the static force), writes the cell
+// force output and clears the accumulators, tile by tile: only the tiles the
+// spreading wrote into (all = 0; the tiles of the previous step get their
+// output reset to the static force), or every tile (all = 1, for per-step
+// force records).  Also closes out the membrane sums of the step and clears
+// the node-binning overflow count.
+struct ApplyArgs {
+    const int *tile_cell_start;
+    int n_tiles;
+    float dt, rho0;
+    float4 *st;               // the updated state
+    float *cell_force_out;    // [3 n]
+    int all;
+    unsigned long long *mem_acc;  // [4 n_mem] membrane sums
+    double *glob;             // [8 n_mem]
+    int n_mem;
 };
 
 hipError_t launch_ibm_interp(const MemArgs &m, const IbmArgs &g, const float4 *st, hipStream_t s);
@@ -83,9 +107,10 @@ hipError_t launch_ibm_weights(const MemArgs &m, const IbmArgs &g, hipStream_t s)
 hipError_t launch_mem_volume(const MemArgs &m, hipStream_t s);
 hipError_t launch_mem_edges(const MemArgs &m, hipStream_t s);
 hipError_t launch_mem_nodes(const MemArgs &m, hipStream_t s);
-// spreading into the fixed-point cell accumulators; the cell update turns them
-// into force densities (launch_update with its UpdateForce)
+// spreading into the fixed-point cell accumulators; launch_ibm_apply turns
+// them into force densities
 hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s);
+hipError_t launch_ibm_apply(const IbmArgs &g, const ApplyArgs &p, hipStream_t s);
 // internal viscosity field (immersed_boundary_method.cu:389-527)
 hipError_t launch_internal_visc(const MemArgs &m, const IbmArgs &g, float delta_h, float *visc_factor, hipStream_t s);
 // output-step energies, 7 doubles per membrane (Solver_gpu.cu:1060-1139)

@@ -18,7 +18,6 @@
 #include <hip/hip_runtime.h>
 
 #include "coupling.hpp"
-#include "fluid_kernels.hpp"
 #include "ibm_device.hpp"
 #include "membrane_force.hpp"
 
@@ -166,8 +165,9 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
 // (eight lanes per node, the order of k_mem_nodes) and stored.  Nodes of full
 // bins are spread one wavefront each by the workgroups after the bins.  All
 // sums are 64-bit fixed point integer additions, so the result does not
-// depend on the order of the nodes, lanes or workgroups.  The cell update
-// turns the accumulators into force densities and clears them.
+// depend on the order of the nodes, lanes or workgroups.  Each write flags
+// the cell's fluid tile; k_ibm_apply turns the accumulators of the flagged
+// tiles into force densities and clears them.
 // ---------------------------------------------------------------------------
 // sum of a node's edge slots by eight lanes (sub = lane & 7; 8 or more: no
 // slots), fixed order
@@ -207,6 +207,7 @@ __device__ __forceinline__ void spread_node_global(const MemArgs &m, const IbmAr
     atomicAdd(acc, (unsigned long long)llrint((double)(F.x * wt) * FIX_SCALE));
     atomicAdd(acc + 1, (unsigned long long)llrint((double)(F.y * wt) * FIX_SCALE));
     atomicAdd(acc + 2, (unsigned long long)llrint((double)(F.z * wt) * FIX_SCALE));
+    g.tile_now[g.cell_tile[ic]] = 1;
 }
 
 __global__ __launch_bounds__(256) void k_ibm_spread(MemArgs m, IbmArgs g)
@@ -319,6 +320,7 @@ __global__ __launch_bounds__(256) void k_ibm_spread(MemArgs m, IbmArgs g)
         if (v0) atomicAdd(acc, v0);
         if (v1) atomicAdd(acc + 1, v1);
         if (v2) atomicAdd(acc + 2, v2);
+        g.tile_now[g.cell_tile[ic]] = 1;
     }
     if (tid == 0) g.bin_count[b] = 0;  // read by this workgroup only; cleared for the next step
 }
@@ -333,6 +335,65 @@ __global__ __launch_bounds__(256) void k_ibm_spread_nodes(MemArgs m, IbmArgs g)
     const int nd = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (nd >= m.n_nodes) return;
     spread_node_global(m, g, nd, m.F[nd], lane);
+}
+
+// One workgroup per fluid tile: the spread sums of the tile's cells turned
+// into force densities (static force + sum * scale), added to the state the
+// face pass updated with the static force (rho u += dt f, the update's own
+// form: u_new = u* + dt f / rho_new, rho_new unchanged by the force), written
+// to the cell force output and cleared.
+__global__ __launch_bounds__(256) void k_ibm_apply(IbmArgs g, ApplyArgs p)
+{
+    const int t = blockIdx.x, tid = threadIdx.x;
+    if (t == 0) {
+        if (tid == 0 && g.binned) *g.n_overflow = 0;
+        // the step's membrane sums: V, A, the MAD the step used, A as the next
+        // A_prev and the MAD of the next force call (numerator / A)
+        for (int k = tid; k < p.n_mem; k += blockDim.x) {
+            unsigned long long *s = p.mem_acc + 4 * (size_t)k;
+            double *gl = p.glob + 8 * (size_t)k;
+            const double V = (double)(long long)s[0] / MEM_SUM_SCALE, A = (double)(long long)s[1] / MEM_SUM_SCALE;
+            const double M = (double)(long long)s[2] / MEM_SUM_SCALE;
+            gl[0] = V;
+            gl[1] = A;
+            gl[2] = gl[7];
+            gl[6] = A;
+            gl[7] = M / A;
+            s[0] = s[1] = s[2] = 0ull;
+        }
+    }
+    const int now = g.tile_now[t], prev = g.tile_prev[t];
+    if (!(now | prev | p.all)) return;  // uniform over the workgroup
+    for (int i = p.tile_cell_start[t] + tid; i < p.tile_cell_start[t + 1]; i += blockDim.x) {
+        float f[3] = {0.0f, 0.0f, 0.0f};
+        if (now) {
+            unsigned long long *a = g.acc + 3 * (size_t)i;
+            const float sc = g.spread_sc[i];
+            bool any = false;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const long long v = (long long)a[c];
+                f[c] = (float)((double)v / FIX_SCALE) * sc;
+                any |= v != 0;
+            }
+            if (any) {
+                a[0] = a[1] = a[2] = 0ull;
+                float4 s = p.st[i];
+                const float q = p.dt / (p.rho0 + s.x);
+                s.y += q * f[0];
+                s.z += q * f[1];
+                s.w += q * f[2];
+                p.st[i] = s;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) p.cell_force_out[3 * (size_t)i + c] = g.static_force[3 * (size_t)i + c] + f[c];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        if (now) g.tile_now[t] = 0;
+        if (!p.all) g.tile_prev[t] = now;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -420,6 +481,11 @@ hipError_t launch_ibm_weights(const MemArgs &m, const IbmArgs &g, hipStream_t s)
 hipError_t launch_mem_edges(const MemArgs &m, hipStream_t s)
 {
     hipLaunchKernelGGL(k_mem_edges, dim3((m.n_edges + 127) / 128), dim3(128), 0, s, m);
+    return hipGetLastError();
+}
+hipError_t launch_ibm_apply(const IbmArgs &g, const ApplyArgs &p, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ibm_apply, dim3(p.n_tiles), dim3(256), 0, s, g, p);
     return hipGetLastError();
 }
 hipError_t launch_mem_volume(const MemArgs &m, hipStream_t s)

@@ -3,8 +3,6 @@
 //
 //   k_gradients      per-cell least-squares gradients of (rho, u, v, w)
 //                    (LBFS.cu:2177-2240, ghost rows of LBFS.cu:1933-2048 folded in)
-//   k_update         the cell update alone, from a residual of k_flux_update,
-//                    with the spread IBM force turned into a force density
 //   k_flux_update    one workgroup per tile of cells: every face touching the
 //                    tile is reconstructed once into LDS (LBFS.cu:2574-2810),
 //                    then each cell gathers its six face fluxes in a fixed
@@ -221,68 +219,6 @@ __global__ __launch_bounds__(NT) void k_flux_update(FluidArgs a, const float4 *_
     }
 }
 
-// The cell update alone, from the residual of k_flux_update (do_update = 0):
-// lets the face pass run before the cell force is known (the coupled step
-// spreads the membrane forces meanwhile).  Same arithmetic as the fused form.
-__global__ __launch_bounds__(256) void k_update(FluidArgs a, const float4 *__restrict__ st,
-                                                const float4 *__restrict__ res, float4 *__restrict__ st_out,
-                                                UpdateForce uf)
-{
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (uf.n_overflow && i == 0) *uf.n_overflow = 0;
-    if (uf.mem_acc)
-        for (int k = i; k < uf.n_mem; k += gridDim.x * 256) {
-            unsigned long long *s = uf.mem_acc + 4 * (size_t)k;
-            double *gl = uf.glob + 8 * (size_t)k;
-            const double V = (double)(long long)s[0] / MEM_SUM_SCALE, A = (double)(long long)s[1] / MEM_SUM_SCALE;
-            const double M = (double)(long long)s[2] / MEM_SUM_SCALE;
-            gl[0] = V;
-            gl[1] = A;
-            gl[2] = gl[7];
-            gl[6] = A;
-            gl[7] = M / A;
-            s[0] = s[1] = s[2] = 0ull;
-        }
-    if (i >= a.n_cells) return;
-    const float4 r = res[i];
-    float r0 = r.x, r1 = r.y, r2 = r.z, r3 = r.w;
-    float fx = a.body_force[0], fy = a.body_force[1], fz = a.body_force[2];
-    if (uf.acc) {
-        unsigned long long *p = uf.acc + 3 * (size_t)i;
-        const float sc = uf.spread_sc[i];
-        float f[3];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const long long v = (long long)p[c];
-            f[c] = uf.static_force[3 * (size_t)i + c] + (float)((double)v / 1099511627776.0) * sc;
-            if (v != 0) p[c] = 0ull;
-            uf.cell_force_out[3 * (size_t)i + c] = f[c];
-        }
-        fx += f[0]; fy += f[1]; fz += f[2];
-    } else if (a.cell_force) {
-        const float *cf = a.cell_force + 3 * (size_t)i;
-        fx += cf[0]; fy += cf[1]; fz += cf[2];
-    }
-    r1 += fx; r2 += fy; r3 += fz;
-    const float4 s0 = st[i];
-    float dt = a.dt;
-    if (a.ts_mode == LBFS_TS_LOCAL) {
-        const float2 c = a.cfl_coef[i];
-        dt = a.dt / (c.x + c.y / (a.rho0 + s0.x));
-    } else if (a.ts_mode == LBFS_TS_MIN) {
-        dt = __uint_as_float(*a.dt_min);
-    }
-    const float drho = s0.x + dt * r0;
-    const float rho = a.rho0 + drho;
-    const float inv_rho = 1.0f / rho;
-    float4 sn;
-    sn.x = drho;
-    sn.y = s0.y + dt * (r1 - s0.y * r0) * inv_rho;
-    sn.z = s0.z + dt * (r2 - s0.z * r0) * inv_rho;
-    sn.w = s0.w + dt * (r3 - s0.w * r0) * inv_rho;
-    st_out[i] = sn;
-}
-
 template <class L, bool BAND>
 __global__ __launch_bounds__(256) void k_face_flux_all(FluidArgs a, const float4 *__restrict__ st,
                                                        const float4 *__restrict__ grad, float4 *__restrict__ out,
@@ -411,13 +347,6 @@ hipError_t launch_face_flux_all(const FluidArgs &a, const float4 *st, const floa
 {
     const int blocks = (n_tile_faces + 255) / 256;
     return dispatch<FaceFluxAll>(a, dim3(blocks), 0, s, st, grad, out, n_tile_faces);
-}
-
-hipError_t launch_update(const FluidArgs &a, const float4 *st, const float4 *res, float4 *st_out,
-                         const UpdateForce &uf, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_update, dim3((a.n_cells + 255) / 256), dim3(256), 0, s, a, st, res, st_out, uf);
-    return hipGetLastError();
 }
 
 hipError_t launch_dt_min(const FluidArgs &a, const float4 *st, hipStream_t s)

@@ -40,30 +40,6 @@ struct FluidArgs {
 hipError_t launch_gradients(const FluidArgs &a, const float4 *st, float4 *grad, hipStream_t s);
 hipError_t launch_flux_update(const FluidArgs &a, const float4 *st, const float4 *grad, float4 *st_out,
                               float4 *res_out, int do_update, hipStream_t s);
-// scale of the membranes' 64-bit fixed-point sums (volume, area, MAD numerator)
-constexpr double MEM_SUM_SCALE = 1073741824.0;  // 2^30
-
-// The IBM force of the coupled step, turned into a force density by the cell
-// update: static force + (64-bit fixed-point spread sum, scale 2^40) * sc,
-// the accumulators cleared for the next step; the force is also written out
-// (the step's cell force).  acc == null: the update uses FluidArgs::cell_force.
-// The update also closes out the step's membrane sums: glob (8 per membrane)
-// gets V, A, the MAD the step used, A as the next A_prev and the MAD of the
-// next force call (MAD numerator / A); the sums are cleared.
-struct UpdateForce {
-    unsigned long long *acc;   // [3 n] spread sums
-    const float *static_force; // [3 n] body force + closure term
-    const float *spread_sc;    // [n] force density per unit spread sum
-    float *cell_force_out;     // [3 n]
-    int *n_overflow;           // cleared here (node-binning overflow count) or null
-    unsigned long long *mem_acc;  // [4 n_mem] membrane sums, or null
-    double *glob;              // [8 n_mem]
-    int n_mem;
-};
-
-// cell update from the residual res (the flux pass run with do_update = 0)
-hipError_t launch_update(const FluidArgs &a, const float4 *st, const float4 *res, float4 *st_out,
-                         const UpdateForce &uf, hipStream_t s);
 hipError_t launch_face_flux_all(const FluidArgs &a, const float4 *st, const float4 *grad, float4 *out,
                                 int n_tile_faces, hipStream_t s);
 // global minimum of the local steps into *a.dt_min (LBFS_TS_MIN)

@@ -156,8 +156,8 @@ static int fluid_setup(lbfs_ctx *ctx, const lbfs_mesh_desc &m, const lbfs_params
     if ((rc = dev_upload(ctx, &sf, d.static_force.data(), d.static_force.size()))) return rc;
     if ((rc = dev_upload(ctx, &cf, d.static_force.data(), d.static_force.size()))) return rc;
     ctx->static_force = sf;
-    ctx->cell_force = cf;
-    a.cell_force = cf;
+    ctx->cell_force = cf;  // output: static + spread force of the last coupled step
+    a.cell_force = sf;
     // local / minimum time steps: per-cell coefficients of dt V / lambda
     if (p.time_stepping == LBFS_TS_LOCAL || p.time_stepping == LBFS_TS_MIN) {
         if (!(p.max_velocity > 0.0)) return ctx->fail(LBFS_E_INVALID_ARG, "local time stepping needs max_velocity > 0");

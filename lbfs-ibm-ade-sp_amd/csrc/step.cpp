@@ -3,6 +3,7 @@
 // event-based measurement entry points.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 
 #include "context.hpp"
@@ -74,28 +75,26 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
                      },
                      24.0 * n});
     if (have_membranes) {
-        // the face pass does not need the cell force: it writes the residual
-        // and runs beside the coupling chain; the update follows spreading.
-        // Face pass: state 16 + gradients 48 + volume/centroid 16 + residual
-        // written 16 per cell, plus 40 per face; update: state 16 + residual
-        // 16 + static force 12 + spread sums 24 + scale 4 + force and state
-        // written 28 per cell
-        s.push_back({"flux_residual", [ctx]() -> int {
-                         HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, nullptr, ctx->res, 0,
-                                                   ctx->stream));
+        // The face pass updates the cell with the static force (it does not
+        // need the spread force, so it runs beside the coupling chain); the
+        // apply pass then adds the spread force density in the tiles the
+        // spreading wrote into.  Face pass as below; apply: spread sums 24 +
+        // scale 4 + static force 12 + state read and written 32 + force
+        // written 12 per touched cell (about 343 cells per 16 nodes)
+        const double touched = std::min(n, 343.0 * mem.n_nodes / 16.0);
+        s.push_back({"flux_update", [ctx]() -> int {
+                         HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->st[ctx->cur ^ 1],
+                                                   nullptr, 1, ctx->stream));
                          return LBFS_OK;
                      },
-                     96.0 * n + 40.0 * nf});
-        s.push_back({"update", [ctx]() -> int {
-                         const UpdateForce uf{ctx->ibm.acc, ctx->static_force, ctx->ibm.spread_sc, ctx->cell_force,
-                                              ctx->ibm.binned ? ctx->ibm.n_overflow : nullptr, ctx->mem.mem_acc,
-                                              ctx->mem.glob, ctx->mem.n_mem};
-                         HIPCHK(launch_update(ctx->args, ctx->st[ctx->cur], ctx->res, ctx->st[ctx->cur ^ 1], uf,
-                                              ctx->stream));
+                     108.0 * n + 40.0 * nf});
+        s.push_back({"ibm_apply", [ctx]() -> int {
+                         HIPCHK(launch_ibm_apply(ctx->ibm, ctx->apply_args(ctx->st[ctx->cur ^ 1], ctx->cell_force, 0),
+                                                 ctx->stream));
                          ctx->cur ^= 1;
                          return LBFS_OK;
                      },
-                     100.0 * n});
+                     84.0 * touched});
         return s;
     }
     // pass F: state 16 + gradients 48 + volume/centroid 16 + force 12 + state written 16 per cell,
@@ -125,8 +124,8 @@ int lbfs_ctx::run_stages(bool with_visc)
     // With membranes the gradients and the face pass (which read only the
     // state at the start of the step) run on the second stream beside the
     // coupling chain (a latency-bound sequence of small kernels), and the
-    // membrane forces on a third stream after the interpolation.  The update
-    // waits for the face pass and for spreading.
+    // membrane forces on a third stream after the interpolation.  The apply
+    // pass waits for the face pass and for spreading.
     // Same kernels, same inputs: the results are those of the sequential order.
     // The fluid pass starts with the step, beside the interpolation
     // (LBFS_OPT_INTERP_FIRST = 1: after it)
@@ -136,7 +135,7 @@ int lbfs_ctx::run_stages(bool with_visc)
         HIPCHK(hipEventRecord(ev_fork, stream));
         HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
         HIPCHK(launch_gradients(args, st[cur], grad, side));
-        HIPCHK(launch_flux_update(args, st[cur], grad, nullptr, res, 0, side));
+        HIPCHK(launch_flux_update(args, st[cur], grad, st[cur ^ 1], nullptr, 1, side));
         HIPCHK(hipEventRecord(ev_join, side));
         return LBFS_OK;
     };
@@ -144,12 +143,12 @@ int lbfs_ctx::run_stages(bool with_visc)
     if (!interp_first && (rc = fork_fluid())) return rc;
     for (auto &s : step_stages(false)) {
         const char *nm = s.name;
-        if (std::strcmp(nm, "gradients") == 0 || std::strcmp(nm, "flux_residual") == 0 ||
+        if (std::strcmp(nm, "gradients") == 0 || std::strcmp(nm, "flux_update") == 0 ||
             std::strcmp(nm, "mem_volume") == 0 || std::strcmp(nm, "mem_edges") == 0 ||
             std::strcmp(nm, "mem_nodes") == 0)
             continue;
         if (std::strcmp(nm, "ibm_spread") == 0) HIPCHK(hipStreamWaitEvent(stream, ev_join2, 0));
-        if (std::strcmp(nm, "update") == 0) HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
+        if (std::strcmp(nm, "ibm_apply") == 0) HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
         if ((rc = s.run())) return rc;
         if (std::strcmp(nm, "ibm_interp") == 0) {
             if (interp_first && (rc = fork_fluid())) return rc;

@@ -93,12 +93,11 @@ public:
     // get_cfl_device + min (time_stepping 3)
     void get_cfl() { stage_if_present("dt_min"); }
     // calc_face_flux + calc_face_flux_bc + add_face_flux_to_cell + add + time_integration
-    // (with membranes the residual pass and the update are separate launches)
+    // (with membranes the spread force is added by a second, tile-sparse pass)
     void time_integration()
     {
-        stage_if_present("flux_update");
-        stage_if_present("flux_residual");
-        stage_if_present("update");
+        stage("flux_update");
+        stage_if_present("ibm_apply");  // with membranes: the spread force added
     }
     void end_step() { check(lbfs_stage_end_step(ctx_), "lbfs_stage_end_step"); }
 

@@ -197,7 +197,7 @@ def test_named_stages_equal_whole_steps():
     a, b = build(), build()
     a.step(7)
     names = b.stage_names()
-    assert names[0] == "ibm_interp" and names[-2:] == ["flux_residual", "update"]
+    assert names[0] == "ibm_interp" and names[-2:] == ["flux_update", "ibm_apply"]
     for _ in range(7):
         for n in b.stage_names():
             b.stage_run(n)
