@@ -147,12 +147,16 @@ int lbfs_step(lbfs_ctx *ctx, int nsteps);
 /* options (no counterpart in the reference) */
 #define LBFS_OPT_GRAPH 1 /* 1 (default): replay the step from a captured hipGraph, 0: launch kernels one by one */
 #define LBFS_OPT_IBM_BINNED 2 /* spreading for the next lbfs_add_membranes: -1 (default) by node count
-                                 (binned from 2048 nodes), 0 one
-                                 wavefront per node into integer cell accumulators, 1 nodes binned and spread per
-                                 bin through LDS; the result is the same bit for bit */
-#define LBFS_OPT_INTERP_FIRST 3 /* coupled step schedule: 0 or -1 (default) the fluid pass from the
-                                   start of the step beside the interpolation, 1 the interpolation
-                                   alone before the fluid pass; the result is the same bit for bit */
+                                 (binned from 2048 nodes), 0 one wavefront per node into integer cell
+                                 accumulators, 1 nodes binned and spread per bin through LDS (the occupied
+                                 bins listed up to 65536 nodes, every bin of the grid above), 2 binned
+                                 with every bin of the grid; the result is the same bit for bit */
+#define LBFS_OPT_STEP_SCHEDULE 3 /* coupled step schedule (the result is the same bit for bit):
+                                    0 (default) the fluid passes on the main stream, the coupling
+                                    chain in order on a second; 1 the fluid passes on a second
+                                    stream from the start of the step, the membrane forces on a
+                                    third after the interpolation; 2 as 1 with the interpolation
+                                    alone first */
 int lbfs_set_option(lbfs_ctx *ctx, int option, int value);
 
 /* per-stage entry points (parity tests) */

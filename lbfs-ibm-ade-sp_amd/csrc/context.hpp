@@ -64,7 +64,7 @@ struct lbfs_ctx {
     lbfs::AdjointState *adj = nullptr;
 
     // hipGraph of two consecutive steps starting from cur == 0
-    int interp_first_option = -1;   // LBFS_OPT_INTERP_FIRST
+    int schedule_option = 0;        // LBFS_OPT_STEP_SCHEDULE
     bool use_graph = true;
     hipGraphExec_t graph_exec = nullptr;
     void drop_graph();

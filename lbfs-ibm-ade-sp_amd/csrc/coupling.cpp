@@ -262,7 +262,7 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     // launches); from a couple of thousand nodes on, binning them and
     // spreading per bin through LDS is faster (one 2562-node cell in a
     // 128x64x64 channel: 28 us against 39 us for the spreading)
-    g.binned = ctx->ibm_binned_option >= 0 ? ctx->ibm_binned_option : (nn >= 2048 ? 1 : 0);
+    g.binned = ctx->ibm_binned_option >= 0 ? (ctx->ibm_binned_option >= 1 ? 1 : 0) : (nn >= 2048 ? 1 : 0);
     g.dt = (float)ctx->params.dt;
     std::vector<int> node_block(nn), mem_block(count);
     for (int i = 0; i < count; i++) {
@@ -317,6 +317,14 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     if ((rc = cp_alloc<int>(ctx, &slots, nullptr, (size_t)IBM_BIN_CAP * g.n_bins))) return rc;
     if ((rc = cp_alloc<int>(ctx, &ov, nullptr, nn))) return rc;
     if ((rc = cp_alloc<int>(ctx, &nov, nullptr, 1))) return rc;
+    int *blist, *nblist;
+    if ((rc = cp_alloc<int>(ctx, &blist, nullptr, g.n_bins))) return rc;
+    if ((rc = cp_alloc<int>(ctx, &nblist, nullptr, 1))) return rc;
+    g.bin_list = blist;
+    g.n_bin_list = nblist;
+    // few nodes occupy few bins: list them (one atomic per occupied bin) and
+    // loop over the list; many nodes: one workgroup per bin of the grid
+    g.use_list = ctx->ibm_binned_option == 2 ? 0 : (nn <= 65536 ? 1 : 0);
     unsigned long long *dacc;
     if ((rc = cp_alloc<unsigned long long>(ctx, &dacc, nullptr, 3 * (size_t)nc))) return rc;
     g.acc = dacc;

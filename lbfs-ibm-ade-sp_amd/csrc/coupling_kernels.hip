@@ -17,6 +17,8 @@
 //                  on the order the nodes are visited in
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "coupling.hpp"
 #include "ibm_device.hpp"
 #include "membrane_force.hpp"
@@ -145,6 +147,10 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
             // straight into the bin's slots; a full bin sends the node to the
             // overflow list (spread node by node)
             const int r = atomicAdd(&g.bin_count[key], 1);
+            if (r == 0 && g.use_list) {  // once per bin and step
+                const int li = atomicAdd(g.n_bin_list, 1);
+                if (li < g.n_bins) g.bin_list[li] = key;
+            }
             if (r < IBM_BIN_CAP) {
                 g.bin_slots[(size_t)key * IBM_BIN_CAP + r] = nd;
             } else {
@@ -156,16 +162,17 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
 }
 
 // ---------------------------------------------------------------------------
-// IBM: spreading.  One workgroup per node bin (the 4x4x4 block of cells that
-// holds the nodes' stencil base; its nodes were put in the bin's slots by the
-// interpolation): the nodes' stencils stay inside a 7x7x7 window, where the
-// weighted forces are accumulated in LDS (one wavefront per node, one lane per
-// stencil slot), then the window is added to the per-cell accumulators in
-// global memory.  The node forces are gathered from their edge records here
-// (eight lanes per node, the order of k_mem_nodes) and stored.  Nodes of full
-// bins are spread one wavefront each by the workgroups after the bins.  All
-// sums are 64-bit fixed point integer additions, so the result does not
-// depend on the order of the nodes, lanes or workgroups.  Each write flags
+// IBM: spreading.  Per node bin (the 4x4x4 block of cells that holds the
+// nodes' stencil base; its nodes were put in the bin's slots, and the bin in
+// the list of occupied bins, by the interpolation), one workgroup at a time
+// (the workgroups loop over the list): the nodes' stencils stay inside a
+// 7x7x7 window, where the weighted forces are accumulated in LDS (one
+// wavefront per node, one lane per stencil slot), then the window is added to
+// the per-cell accumulators in global memory.  The node forces are gathered
+// from their edge records here (eight lanes per node, the order of
+// k_mem_nodes) and stored.  Nodes of full bins are spread one wavefront each
+// after the bins.  All sums are 64-bit fixed point integer additions, so the
+// result does not depend on the order of the nodes, lanes or workgroups.  Each write flags
 // the cell's fluid tile; k_ibm_apply turns the accumulators of the flagged
 // tiles into force densities and clears them.
 // ---------------------------------------------------------------------------
@@ -210,119 +217,126 @@ __device__ __forceinline__ void spread_node_global(const MemArgs &m, const IbmAr
     g.tile_now[g.cell_tile[ic]] = 1;
 }
 
-__global__ __launch_bounds__(256) void k_ibm_spread(MemArgs m, IbmArgs g)
+// SPREAD_THREADS: 512 when few bins are occupied (the workgroups loop over
+// the list; wider workgroups finish a bin sooner), 256 for one workgroup per
+// bin of the grid (many nodes: more bins in flight)
+template <int SPREAD_THREADS>
+__global__ __launch_bounds__(SPREAD_THREADS) void k_ibm_spread(MemArgs m, IbmArgs g)
 {
-    const int b = blockIdx.x;
+    constexpr int SPREAD_WAVES = SPREAD_THREADS / 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (b >= g.n_bins) {
-        // nodes of full bins, one wavefront each
-        const int nov = min(*g.n_overflow, m.n_nodes);
-        for (int k = (b - g.n_bins) * 4 + wave; k < nov; k += (gridDim.x - g.n_bins) * 4) {
-            const int nd = g.overflow[k];
-            float4 F = node_force8(m, nd, lane & 7);
-            F.x = __shfl(F.x, 0, 64);
-            F.y = __shfl(F.y, 0, 64);
-            F.z = __shfl(F.z, 0, 64);
-            if (lane == 0) m.F[nd] = F;
-            spread_node_global(m, g, nd, F, lane);
-        }
-        return;
-    }
-    const int cnt_all = min(g.bin_count[b], IBM_BIN_CAP);
-    if (cnt_all == 0) return;  // uniform over the workgroup
-    const IbmBlock B = g.blk[g.bin_block[b]];
-    const int lb = b - B.bin0;
-    const int bo[3] = {4 * (lb % B.tn[0]), 4 * ((lb / B.tn[0]) % B.tn[1]), 4 * (lb / (B.tn[0] * B.tn[1]))};
     __shared__ unsigned long long racc[3][343];
     __shared__ int cnd[64];
     __shared__ int4 cbase[64];
     __shared__ float4 cF[64];
     __shared__ float cw[64][64];   // the chunk's cached stencil weights, one row per node
-    for (int q = tid; q < 3 * 343; q += 256) racc[q / 343][q % 343] = 0ull;
     const int S = stencil_size(g);
     int off[3];
     lane_offset(S, lane, off);
     const bool act = lane < S * S * S;
-    const int *slots = g.bin_slots + (size_t)b * IBM_BIN_CAP;
-    for (int c0 = 0; c0 < cnt_all; c0 += 64) {
-        const int cnt = min(64, cnt_all - c0);
-        __syncthreads();
-        if (tid < cnt) {
-            const int p = slots[c0 + tid];
-            cnd[tid] = p;
-            cbase[tid] = g.nbase[p];
-        }
-        __syncthreads();
-        // node forces from the edge records: eight lanes per node
-        for (int jn = tid >> 3; jn < cnt; jn += 32) {
-            const int p = cnd[jn];
-            const float4 F = node_force8(m, p, tid & 7);
-            if ((tid & 7) == 0) {
-                cF[jn] = F;
-                m.F[p] = F;
+    const int n_list = g.use_list ? min(*g.n_bin_list, g.n_bins) : g.n_bins;  // uniform over the workgroup
+    for (int e = blockIdx.x; e < n_list; e += gridDim.x) {
+        const int b = g.use_list ? g.bin_list[e] : e;
+        const int cnt_all = min(g.bin_count[b], IBM_BIN_CAP);
+        if (cnt_all == 0) continue;  // (the grid form) uniform over the workgroup
+        const IbmBlock B = g.blk[g.bin_block[b]];
+        const int lb = b - B.bin0;
+        const int bo[3] = {4 * (lb % B.tn[0]), 4 * ((lb / B.tn[0]) % B.tn[1]), 4 * (lb / (B.tn[0] * B.tn[1]))};
+        __syncthreads();  // the previous bin's window is flushed
+        for (int q = tid; q < 3 * 343; q += SPREAD_THREADS) racc[q / 343][q % 343] = 0ull;
+        const int *slots = g.bin_slots + (size_t)b * IBM_BIN_CAP;
+        for (int c0 = 0; c0 < cnt_all; c0 += 64) {
+            const int cnt = min(64, cnt_all - c0);
+            __syncthreads();
+            if (tid < cnt) {
+                const int p = slots[c0 + tid];
+                cnd[tid] = p;
+                cbase[tid] = g.nbase[p];
+            }
+            __syncthreads();
+            // node forces from the edge records: eight lanes per node
+            for (int jn = tid >> 3; jn < cnt; jn += SPREAD_THREADS / 8) {
+                const int p = cnd[jn];
+                const float4 F = node_force8(m, p, tid & 7);
+                if ((tid & 7) == 0) {
+                    cF[jn] = F;
+                    m.F[p] = F;
+                }
+            }
+            // all weight rows of the chunk in flight at once (one coalesced row per node)
+            for (int q = tid; q < 64 * cnt; q += SPREAD_THREADS)
+                cw[q >> 6][q & 63] = g.wnode[64 * (size_t)cnd[q >> 6] + (q & 63)];
+            __syncthreads();
+            for (int jn = wave; jn < cnt; jn += SPREAD_WAVES) {
+                const float wt = cw[jn][lane];
+                if (!act || wt == 0.0f) continue;
+                const int4 nb = cbase[jn];
+                const int bs[3] = {nb.x, nb.y, nb.z};
+                int loc = 0, mul = 1;
+                bool in = true;
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    int c = bs[a] + off[a];
+                    int l;
+                    if (B.per[a]) {
+                        // window coordinate of the wrapped base, then the slot offset
+                        // (nodes advect freely across periodic boundaries, so
+                        // the base may lie several periods away)
+                        const int wb = ((bs[a] % B.n[a]) + B.n[a]) % B.n[a];
+                        l = wb - bo[a] + off[a];
+                    } else {
+                        in = in && c >= 0 && c < B.n[a];
+                        l = c - bo[a];
+                    }
+                    in = in && l >= 0 && l < 7;
+                    loc += l * mul;
+                    mul *= 7;
+                }
+                if (!in) continue;
+                const float4 F = cF[jn];
+                atomicAdd(&racc[0][loc], (unsigned long long)llrint((double)(F.x * wt) * FIX_SCALE));
+                atomicAdd(&racc[1][loc], (unsigned long long)llrint((double)(F.y * wt) * FIX_SCALE));
+                atomicAdd(&racc[2][loc], (unsigned long long)llrint((double)(F.z * wt) * FIX_SCALE));
             }
         }
-        // all weight rows of the chunk in flight at once (one coalesced row per node)
-        for (int q = tid; q < 64 * cnt; q += 256) cw[q >> 6][q & 63] = g.wnode[64 * (size_t)cnd[q >> 6] + (q & 63)];
         __syncthreads();
-        for (int jn = wave; jn < cnt; jn += 4) {
-            const float wt = cw[jn][lane];
-            if (!act || wt == 0.0f) continue;
-            const int4 nb = cbase[jn];
-            const int bs[3] = {nb.x, nb.y, nb.z};
-            int loc = 0, mul = 1;
-            bool in = true;
+        for (int q = tid; q < 343; q += SPREAD_THREADS) {
+            const unsigned long long v0 = racc[0][q], v1 = racc[1][q], v2 = racc[2][q];
+            if ((v0 | v1 | v2) == 0ull) continue;
+            const int l[3] = {q % 7, (q / 7) % 7, q / 49};
+            int w[3];
+            bool ok = true;
 #pragma unroll
             for (int a = 0; a < 3; a++) {
-                int c = bs[a] + off[a];
-                int l;
-                if (B.per[a]) {
-                    // window coordinate of the wrapped base, then the slot offset
-                    // (nodes advect freely across periodic boundaries, so
-                    // the base may lie several periods away)
-                    const int wb = ((bs[a] % B.n[a]) + B.n[a]) % B.n[a];
-                    l = wb - bo[a] + off[a];
-                } else {
-                    in = in && c >= 0 && c < B.n[a];
-                    l = c - bo[a];
+                int c = bo[a] + l[a];
+                if (c >= B.n[a]) {
+                    if (!B.per[a]) { ok = false; break; }
+                    c -= B.n[a];
+                    if (c >= B.n[a]) c %= B.n[a];
                 }
-                in = in && l >= 0 && l < 7;
-                loc += l * mul;
-                mul *= 7;
+                w[a] = c;
             }
-            if (!in) continue;
-            const float4 F = cF[jn];
-            atomicAdd(&racc[0][loc], (unsigned long long)llrint((double)(F.x * wt) * FIX_SCALE));
-            atomicAdd(&racc[1][loc], (unsigned long long)llrint((double)(F.y * wt) * FIX_SCALE));
-            atomicAdd(&racc[2][loc], (unsigned long long)llrint((double)(F.z * wt) * FIX_SCALE));
+            if (!ok) continue;
+            const int ic = cell_internal(g, B, w);
+            unsigned long long *acc = g.acc + 3 * (size_t)ic;
+            if (v0) atomicAdd(acc, v0);
+            if (v1) atomicAdd(acc + 1, v1);
+            if (v2) atomicAdd(acc + 2, v2);
+            g.tile_now[g.cell_tile[ic]] = 1;
         }
+        if (tid == 0) g.bin_count[b] = 0;  // read by this workgroup only; cleared for the next step
     }
-    __syncthreads();
-    for (int q = tid; q < 343; q += 256) {
-        const unsigned long long v0 = racc[0][q], v1 = racc[1][q], v2 = racc[2][q];
-        if ((v0 | v1 | v2) == 0ull) continue;
-        const int l[3] = {q % 7, (q / 7) % 7, q / 49};
-        int w[3];
-        bool ok = true;
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            int c = bo[a] + l[a];
-            if (c >= B.n[a]) {
-                if (!B.per[a]) { ok = false; break; }
-                c -= B.n[a];
-                if (c >= B.n[a]) c %= B.n[a];
-            }
-            w[a] = c;
-        }
-        if (!ok) continue;
-        const int ic = cell_internal(g, B, w);
-        unsigned long long *acc = g.acc + 3 * (size_t)ic;
-        if (v0) atomicAdd(acc, v0);
-        if (v1) atomicAdd(acc + 1, v1);
-        if (v2) atomicAdd(acc + 2, v2);
-        g.tile_now[g.cell_tile[ic]] = 1;
+    // nodes of full bins, one wavefront each
+    const int nov = min(*g.n_overflow, m.n_nodes);
+    for (int k = blockIdx.x * SPREAD_WAVES + wave; k < nov; k += gridDim.x * SPREAD_WAVES) {
+        const int nd = g.overflow[k];
+        float4 F = node_force8(m, nd, lane & 7);
+        F.x = __shfl(F.x, 0, 64);
+        F.y = __shfl(F.y, 0, 64);
+        F.z = __shfl(F.z, 0, 64);
+        if (lane == 0) m.F[nd] = F;
+        spread_node_global(m, g, nd, F, lane);
     }
-    if (tid == 0) g.bin_count[b] = 0;  // read by this workgroup only; cleared for the next step
 }
 
 // Spreading for small node counts: one wavefront per node, one lane per
@@ -346,7 +360,10 @@ __global__ __launch_bounds__(256) void k_ibm_apply(IbmArgs g, ApplyArgs p)
 {
     const int t = blockIdx.x, tid = threadIdx.x;
     if (t == 0) {
-        if (tid == 0 && g.binned) *g.n_overflow = 0;
+        if (tid == 0 && g.binned) {
+            *g.n_overflow = 0;
+            *g.n_bin_list = 0;
+        }
         // the step's membrane sums: V, A, the MAD the step used, A as the next
         // A_prev and the MAD of the next force call (numerator / A)
         for (int k = tid; k < p.n_mem; k += blockDim.x) {
@@ -500,8 +517,12 @@ hipError_t launch_mem_nodes(const MemArgs &m, hipStream_t s)
 }
 hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s)
 {
-    if (g.binned)  // the bins, then 64 workgroups for the nodes of full bins
-        hipLaunchKernelGGL(k_ibm_spread, dim3(g.n_bins + 64), dim3(256), 0, s, m, g);
+    // the workgroups loop over the listed bins (or take one bin of the grid
+    // each), then over the nodes of full bins
+    if (g.binned && g.use_list)
+        hipLaunchKernelGGL(k_ibm_spread<512>, dim3(std::min(g.n_bins, 1024)), dim3(512), 0, s, m, g);
+    else if (g.binned)
+        hipLaunchKernelGGL(k_ibm_spread<256>, dim3(g.n_bins), dim3(256), 0, s, m, g);
     else
         hipLaunchKernelGGL(k_ibm_spread_nodes, dim3((m.n_nodes + 3) / 4), dim3(256), 0, s, m, g);
     return hipGetLastError();

@@ -395,13 +395,13 @@ extern "C" int lbfs_set_option(lbfs_ctx *ctx, int option, int value)
         return LBFS_OK;
     }
     if (option == LBFS_OPT_IBM_BINNED) {
-        if (value < -1 || value > 1) return ctx->fail(LBFS_E_INVALID_ARG, "LBFS_OPT_IBM_BINNED takes -1, 0 or 1");
+        if (value < -1 || value > 2) return ctx->fail(LBFS_E_INVALID_ARG, "LBFS_OPT_IBM_BINNED takes -1, 0, 1 or 2");
         ctx->ibm_binned_option = value;  // applies to the next lbfs_add_membranes
         return LBFS_OK;
     }
-    if (option == LBFS_OPT_INTERP_FIRST) {
-        if (value < -1 || value > 1) return ctx->fail(LBFS_E_INVALID_ARG, "LBFS_OPT_INTERP_FIRST takes -1, 0 or 1");
-        ctx->interp_first_option = value;
+    if (option == LBFS_OPT_STEP_SCHEDULE) {
+        if (value < 0 || value > 2) return ctx->fail(LBFS_E_INVALID_ARG, "LBFS_OPT_STEP_SCHEDULE takes 0, 1 or 2");
+        ctx->schedule_option = value;
         ctx->drop_graph();
         return LBFS_OK;
     }

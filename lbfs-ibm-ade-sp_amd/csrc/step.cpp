@@ -122,14 +122,32 @@ int lbfs_ctx::run_stages(bool with_visc)
         return LBFS_OK;
     }
     // With membranes the gradients and the face pass (which read only the
-    // state at the start of the step) run on the second stream beside the
-    // coupling chain (a latency-bound sequence of small kernels), and the
-    // membrane forces on a third stream after the interpolation.  The apply
-    // pass waits for the face pass and for spreading.
-    // Same kernels, same inputs: the results are those of the sequential order.
-    // The fluid pass starts with the step, beside the interpolation
-    // (LBFS_OPT_INTERP_FIRST = 1: after it)
-    const bool interp_first = interp_first_option == 1;
+    // state at the start of the step) run beside the coupling chain (a
+    // latency-bound sequence of small kernels); the apply pass waits for the
+    // face pass and for spreading.  Same kernels, same inputs: the results are
+    // those of the sequential order.
+    // Schedule 0 (the default) keeps the fluid passes on the main stream and
+    // runs the whole coupling chain in order on the second stream (fewest
+    // cross-stream waits); schedule 1 starts the fluid passes on the second
+    // stream beside the interpolation, schedule 2 after it.
+    int rc;
+    if (schedule_option == 0) {
+        HIPCHK(hipEventRecord(ev_fork, stream));
+        HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
+        HIPCHK(launch_ibm_interp(mem, ibm, st[cur], side));
+        HIPCHK(launch_mem_volume(mem, side));
+        HIPCHK(launch_mem_edges(mem, side));
+        if (!ibm.binned) HIPCHK(launch_mem_nodes(mem, side));
+        HIPCHK(launch_ibm_spread(mem, ibm, side));
+        HIPCHK(hipEventRecord(ev_join, side));
+        HIPCHK(launch_gradients(args, st[cur], grad, stream));
+        HIPCHK(launch_flux_update(args, st[cur], grad, st[cur ^ 1], nullptr, 1, stream));
+        HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
+        HIPCHK(launch_ibm_apply(ibm, apply_args(st[cur ^ 1], cell_force, 0), stream));
+        cur ^= 1;
+        return LBFS_OK;
+    }
+    const bool interp_first = schedule_option == 2;
     auto fork_fluid = [this]() -> int {
         lbfs_ctx *ctx = this;
         HIPCHK(hipEventRecord(ev_fork, stream));
@@ -139,7 +157,6 @@ int lbfs_ctx::run_stages(bool with_visc)
         HIPCHK(hipEventRecord(ev_join, side));
         return LBFS_OK;
     };
-    int rc;
     if (!interp_first && (rc = fork_fluid())) return rc;
     for (auto &s : step_stages(false)) {
         const char *nm = s.name;

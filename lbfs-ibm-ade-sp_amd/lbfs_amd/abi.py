@@ -18,7 +18,7 @@ TS_LOCAL, TS_CONSTANT, TS_MIN = 1, 2, 3
 MESH_OSAY, MESH_MZ = 0, 1
 FLAG_UPWIND_BAND, FLAG_NEUMANN_SIGNED, FLAG_GHOST_DIR_GPU, FLAG_LS_DET_COMPAT, FLAG_CLOSED_FACES = 1, 2, 4, 16, 32
 IBM_LINEAR2, IBM_COS4, IBM_LAGRANGE4 = 0, 1, 2
-OPT_GRAPH, OPT_IBM_BINNED, OPT_INTERP_FIRST = 1, 2, 3
+OPT_GRAPH, OPT_IBM_BINNED, OPT_STEP_SCHEDULE = 1, 2, 3
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -612,12 +612,14 @@ class Solver:
         return g.reshape(-1, 6)
 
     def set_ibm_binned(self, mode):
-        """Spreading path of the next add_membranes: -1 by node count, 0 per node, 1 per bin."""
+        """Spreading path of the next add_membranes: -1 by node count, 0 per node, 1 per bin
+        (occupied bins listed for up to 65536 nodes), 2 per bin over the whole bin grid."""
         check(load().lbfs_set_option(self.ctx, OPT_IBM_BINNED, int(mode)), self.ctx)
 
-    def set_interp_first(self, mode):
-        """Coupled step schedule: 1 interpolation before the fluid pass, 0 or -1 (default) beside it."""
-        check(load().lbfs_set_option(self.ctx, OPT_INTERP_FIRST, int(mode)), self.ctx)
+    def set_step_schedule(self, mode):
+        """Coupled step schedule: 0 (default) fluid on the main stream, the coupling chain on one
+        side stream; 1 fluid on a side stream from the start; 2 as 1 after the interpolation."""
+        check(load().lbfs_set_option(self.ctx, OPT_STEP_SCHEDULE, int(mode)), self.ctx)
 
     def stage_names(self):
         names = (C.c_char_p * 32)()

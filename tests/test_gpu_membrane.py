@@ -280,10 +280,11 @@ def test_lagrange4_coupled_parity():
 @pytest.mark.parametrize("kernel", [abi.IBM_COS4, abi.IBM_LINEAR2])
 def test_spreading_paths_are_bitwise_equal(kernel):
     """Per-node spreading into integer cell accumulators and binned per-bin
-    spreading through LDS windows add the same 64-bit fixed-point terms, so
-    whole coupled steps agree bit for bit."""
+    spreading through LDS windows (over the list of occupied bins, or over the
+    whole bin grid) add the same 64-bit fixed-point terms, so whole coupled
+    steps agree bit for bit."""
     outs = []
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         mesh = cases.channel_mesh(24, 16, 16, seed=12)
         s = abi.Solver(mesh, abi.make_params(visc=VISC, dt=DT, body_force=(2e-5, 0.0, 0.0)))
         s.set_state(*cases.smooth_state(mesh, seed=12, amp=0.005))
@@ -297,8 +298,9 @@ def test_spreading_paths_are_bitwise_equal(kernel):
         X, U, F = s.membrane_nodes()
         outs.append((np.concatenate(s.get_state()), s.cell_force(), X, F))
         s.close()
-    for a, b in zip(outs[0], outs[1]):
-        assert np.array_equal(a, b)
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
 
 
 def test_full_bins_overflow_to_the_node_list():
@@ -306,7 +308,7 @@ def test_full_bins_overflow_to_the_node_list():
     bins (more than the 128 slots of a bin): those nodes take the per-node
     path, and the step still equals the per-node spreading bit for bit."""
     outs = []
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         mesh = cases.channel_mesh(24, 16, 16, seed=13)
         s = abi.Solver(mesh, abi.make_params(visc=VISC, dt=DT, body_force=(2e-5, 0.0, 0.0)))
         s.set_state(*cases.smooth_state(mesh, seed=13, amp=0.005))
@@ -317,19 +319,20 @@ def test_full_bins_overflow_to_the_node_list():
         X, U, F = s.membrane_nodes()
         outs.append((np.concatenate(s.get_state()), s.cell_force(), X, F))
         s.close()
-    for a, b in zip(outs[0], outs[1]):
-        assert np.array_equal(a, b)
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
 
 
 def test_step_schedules_are_bitwise_equal():
-    """The two step schedules (interpolation before or beside the fluid pass)
-    give the same step bit for bit."""
+    """The step schedules (stream assignments of the same kernels) give the
+    same step bit for bit."""
     outs = []
-    for first in (0, 1):
+    for first in (0, 1, 2):
         mesh = cases.channel_mesh(24, 16, 16, seed=14)
         s = abi.Solver(mesh, abi.make_params(visc=VISC, dt=DT, body_force=(2e-5, 0.0, 0.0)))
         s.set_state(*cases.smooth_state(mesh, seed=14, amp=0.005))
-        s.set_interp_first(first)
+        s.set_step_schedule(first)
         xa, ta = abi.rbc(4, 8.0, center=(8.0, 8.0, 8.0))
         xb, tb = abi.rbc(3, 6.0, center=(18.0, 8.0, 8.0))
         s.add_membranes([abi.Membrane(xa, ta, **PARAMS), abi.Membrane(xb, tb, **PARAMS)], ibm_kernel=abi.IBM_COS4)
@@ -337,5 +340,6 @@ def test_step_schedules_are_bitwise_equal():
         X, U, F = s.membrane_nodes()
         outs.append((np.concatenate(s.get_state()), X, F, s.membrane_globals()))
         s.close()
-    for a, b in zip(outs[0], outs[1]):
-        assert np.array_equal(a, b)
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
