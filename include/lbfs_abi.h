@@ -335,6 +335,9 @@ int lbfs_case_read(const char *path, lbfs_case *out);
  * kernel of the step, names[i] / ms[i] for i < *count (at most max_count). */
 int lbfs_time_steps(lbfs_ctx *ctx, int nsteps, double *ms_total);
 int lbfs_time_kernels(lbfs_ctx *ctx, int nsteps, int max_count, const char **names, double *ms, int *count);
+/* average duration of one fluid pass ("gradients" or "flux_update") over n
+ * back-to-back launches on the current state, which they leave unchanged */
+int lbfs_time_stage(lbfs_ctx *ctx, const char *name, int n, double *ms_avg);
 /* device bytes touched per launch by the model of each kernel (same order as lbfs_time_kernels) */
 int lbfs_model_bytes(lbfs_ctx *ctx, int max_count, double *bytes, int *count);
 

@@ -238,6 +238,38 @@ extern "C" int lbfs_time_kernels(lbfs_ctx *ctx, int nsteps, int max_count, const
     return LBFS_OK;
 }
 
+extern "C" int lbfs_time_stage(lbfs_ctx *ctx, const char *name, int n, double *ms_avg)
+{
+    if (!ctx || !name || n <= 0 || !ms_avg) return LBFS_E_INVALID_ARG;
+    // the fluid passes read the state and write scratch or the other state
+    // buffer, so back-to-back launches leave the state as it was
+    if (std::strcmp(name, "gradients") != 0 && std::strcmp(name, "flux_update") != 0)
+        return ctx->fail(LBFS_E_INVALID_ARG, "lbfs_time_stage times the fluid passes only (gradients, flux_update)");
+    std::vector<lbfs_ctx::Stage> stages = ctx->step_stages();
+    const lbfs_ctx::Stage *st = nullptr;
+    for (auto &s : stages)
+        if (std::strcmp(s.name, name) == 0) st = &s;
+    if (!st) return ctx->fail(LBFS_E_INVALID_ARG, "no stage %s", name);
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    const int cur = ctx->cur;
+    HIPCHK(hipEventRecord(a, ctx->stream));
+    for (int k = 0; k < n; k++) {
+        int rc = st->run();
+        ctx->cur = cur;
+        if (rc) return rc;
+    }
+    HIPCHK(hipEventRecord(b, ctx->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    *ms_avg = ms / n;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return LBFS_OK;
+}
+
 extern "C" int lbfs_model_bytes(lbfs_ctx *ctx, int max_count, double *bytes, int *count)
 {
     if (!ctx || !bytes || !count || max_count <= 0) return LBFS_E_INVALID_ARG;

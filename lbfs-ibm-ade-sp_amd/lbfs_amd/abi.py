@@ -139,6 +139,7 @@ def load():
             "lbfs_mesh_springs": (C.c_int, [C.c_int, C.c_int, _ip, _ip, _ip]),
             "lbfs_time_steps": (C.c_int, [vp, C.c_int, _dp]),
             "lbfs_time_kernels": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_char_p), _dp, _ip]),
+            "lbfs_time_stage": (C.c_int, [vp, C.c_char_p, C.c_int, _dp]),
             "lbfs_model_bytes": (C.c_int, [vp, C.c_int, _dp, _ip]),
             "lbfs_mesh_hex_counts": (C.c_int, [C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip]),
             "lbfs_mesh_hex_generate": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp, _ip, C.c_double, C.c_uint64,
@@ -658,6 +659,12 @@ class Solver:
         cnt = C.c_int()
         check(load().lbfs_time_kernels(self.ctx, int(n), max_count, names, ptr(ms), C.byref(cnt)), self.ctx)
         return {names[i].decode(): ms[i] for i in range(cnt.value)}
+
+    def time_stage(self, name, n):
+        """Average ms of one fluid pass ("gradients", "flux_update") over n back-to-back launches."""
+        ms = C.c_double()
+        check(load().lbfs_time_stage(self.ctx, name.encode(), int(n), C.byref(ms)), self.ctx)
+        return ms.value
 
     def model_bytes(self, max_count=16):
         b = np.zeros(max_count)
