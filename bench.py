@@ -117,7 +117,7 @@ def main():
     from lbfs_amd import abi, workloads
     log = lambda m: print("bench: " + m, file=sys.stderr, flush=True)  # noqa: E731
 
-    # --- config 2: 128x64x64 channel, one 2562-node RBC (the headline step) ---
+    # --- 128x64x64 channel with one 2562-node RBC (the headline step) ---
     log("building the 128x64x64 channel")
     mesh, params, state = workloads.channel()
     n = mesh.n_cells
@@ -163,7 +163,7 @@ def main():
     mglob = solver.membrane_globals()[0]
     solver.close()
 
-    # --- config 3: 256x128x128 suspension of 256 RBCs of mixed resolution ---
+    # --- 256x128x128 suspension of 256 RBCs of mixed resolution ---
     batched = None
     if not args.no_batched:
         log("building the 256-cell suspension")
@@ -187,43 +187,49 @@ def main():
                    "mlups": m3.n_cells / (ms3 * 1e-3) / 1e6, "kernels_ms": k3}
         log("suspension step %.4f ms (plain %.4f ms)" % (ms3, ms3_plain))
 
-    # --- config 4: adjoint of the config-2 step, node loss against a run at
-    # moduli x 1.1, checkpoints every K steps ---
-    adjoint = None
+    # --- adjoint of the channel + RBC step, node loss against a run at
+    # moduli x 1.1: every step recorded in HBM (no replay), and checkpoints
+    # every K steps with segment replays (little memory) ---
+    adjoint = adjoint_ckpt = None
     if not args.no_adjoint:
-        log("adjoint: %d steps, checkpoints every %d" % (args.adjoint_steps, args.adjoint_every))
-        mems = workloads.single_rbc(mesh)
-        mp = mems[0].params
-        mu, kl, kg = mp["shear_modulus"], mp["local_bending_modulus"], mp["global_bending_modulus"]
-        a = abi.Solver(mesh, params)
-        a.set_state(*state)
-        a.add_membranes(mems, ibm_kernel=abi.IBM_COS4)
-        a.set_membrane_moduli(0, 1.1 * mu, 1.1 * kl, 1.1 * kg)
-        a.adjoint_forward(args.adjoint_steps, args.adjoint_every)
-        target, _, _ = a.membrane_nodes()
-        a.adjoint_rewind()
-        a.set_membrane_moduli(0, mu, kl, kg)
-        a.sync()
-        t0 = time.perf_counter()
-        a.adjoint_forward(args.adjoint_steps, args.adjoint_every)
-        t1 = time.perf_counter()
-        X, _, _ = a.membrane_nodes()
-        g = a.adjoint_backward(lam_nodes=2.0 * (X - target))
-        t2 = time.perf_counter()
-        a.close()
-        fwd = (t1 - t0) * 1e3 / args.adjoint_steps
-        bwd = (t2 - t1) * 1e3 / args.adjoint_steps
-        adjoint = {"workload": "config-2 channel + RBC, %d steps, checkpoints every %d, L = sum |X - X(moduli x 1.1)|^2"
-                               % (args.adjoint_steps, args.adjoint_every),
-                   "loss": float(np.sum((X - target) ** 2)),
-                   "grad": {"shear_modulus": float(g["moduli"][0, 0]),
-                            "local_bending_modulus": float(g["moduli"][0, 1]),
-                            "global_bending_modulus": float(g["moduli"][0, 2]),
-                            "body_force_x": float(g["body_force"][0])},
-                   "forward_ms_per_step": fwd, "backward_ms_per_step": bwd,
-                   "backward_vs_forward": bwd / fwd, "backward_vs_graph_step": bwd / ms_step,
-                   "note": "backward = segment replay + per-step recompute + reverse kernels"}
-        log("adjoint: forward %.4f ms/step, backward %.4f ms/step" % (fwd, bwd))
+        def run_adjoint(every):
+            log("adjoint: %d steps, checkpoints every %d" % (args.adjoint_steps, every))
+            mems = workloads.single_rbc(mesh)
+            mp = mems[0].params
+            mu, kl, kg = mp["shear_modulus"], mp["local_bending_modulus"], mp["global_bending_modulus"]
+            a = abi.Solver(mesh, params)
+            a.set_state(*state)
+            a.add_membranes(mems, ibm_kernel=abi.IBM_COS4)
+            a.set_membrane_moduli(0, 1.1 * mu, 1.1 * kl, 1.1 * kg)
+            a.adjoint_forward(args.adjoint_steps, every)
+            target, _, _ = a.membrane_nodes()
+            a.adjoint_rewind()
+            a.set_membrane_moduli(0, mu, kl, kg)
+            a.sync()
+            t0 = time.perf_counter()
+            a.adjoint_forward(args.adjoint_steps, every)
+            t1 = time.perf_counter()
+            X, _, _ = a.membrane_nodes()
+            g = a.adjoint_backward(lam_nodes=2.0 * (X - target))
+            t2 = time.perf_counter()
+            a.close()
+            fwd = (t1 - t0) * 1e3 / args.adjoint_steps
+            bwd = (t2 - t1) * 1e3 / args.adjoint_steps
+            log("adjoint: forward %.4f ms/step, backward %.4f ms/step" % (fwd, bwd))
+            return {"workload": "128x64x64 channel + RBC, %d steps, checkpoints every %d, "
+                                "L = sum |X - X(moduli x 1.1)|^2" % (args.adjoint_steps, every),
+                    "loss": float(np.sum((X - target) ** 2)),
+                    "grad": {"shear_modulus": float(g["moduli"][0, 0]),
+                             "local_bending_modulus": float(g["moduli"][0, 1]),
+                             "global_bending_modulus": float(g["moduli"][0, 2]),
+                             "body_force_x": float(g["body_force"][0])},
+                    "forward_ms_per_step": fwd, "backward_ms_per_step": bwd,
+                    "backward_vs_forward": bwd / fwd, "backward_vs_graph_step": bwd / ms_step,
+                    "note": ("forward records every step; backward = reverse kernels"
+                             if every >= args.adjoint_steps else
+                             "forward records the last segment; backward = segment replays + reverse kernels")}
+        adjoint = run_adjoint(args.adjoint_steps)
+        adjoint_ckpt = run_adjoint(args.adjoint_every)
 
     ensemble = None
     if args.ensemble:
@@ -288,6 +294,7 @@ def main():
         "batched": batched,
         "ensemble": ensemble,
         "adjoint": adjoint,
+        "adjoint_checkpointed": adjoint_ckpt,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

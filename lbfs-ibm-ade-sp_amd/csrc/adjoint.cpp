@@ -56,6 +56,7 @@ struct AdjointState {
     int n_steps = 0, every = 0;
     int max_tile_cells = 0;
     bool recorded = false;
+    bool last_in_rec = false;     // the records hold the last segment (written by the forward pass)
     std::vector<Snapshot> ckpt;   // start states of steps 0, K, 2K, ...
     std::vector<StepRec> rec;     // the steps of one segment
     Snapshot start, end;          // before and after the recorded run
@@ -93,7 +94,7 @@ int adj_alloc(lbfs_ctx *ctx, T **dst, size_t n)
     void *p = nullptr;
     HIPCHK(hipMalloc(&p, (n > 0 ? n : 1) * sizeof(T)));
     ctx->adj->allocs.push_back(p);
-    HIPCHK(hipMemset(p, 0, (n > 0 ? n : 1) * sizeof(T)));
+    HIPCHK(hipMemsetAsync(p, 0, (n > 0 ? n : 1) * sizeof(T), ctx->stream));  // ordered before the kernels
     *dst = (T *)p;
     return LBFS_OK;
 }
@@ -240,7 +241,8 @@ int adjoint_setup(lbfs_ctx *ctx, int n_steps, int every)
                 if (sten[(size_t)k * n + i] >= 0 && !used[(size_t)k * n + i])
                     return ctx->fail(LBFS_E_UNSUPPORTED, "the adjoint needs a symmetric gradient stencil");
         if ((rc = adj_alloc(ctx, &A.rev_w, rw.size()))) return rc;
-        HIPCHK(hipMemcpy(A.rev_w, rw.data(), sizeof(float4) * rw.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(A.rev_w, rw.data(), sizeof(float4) * rw.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));  // (rw is freed at the end of the block)
     }
     if (ctx->have_membranes) {
         const MemArgs &m = ctx->mem;
@@ -268,9 +270,13 @@ int adjoint_setup(lbfs_ctx *ctx, int n_steps, int every)
         if ((rc = adj_alloc(ctx, &A.echunk, ch.size()))) return rc;
         if ((rc = adj_alloc(ctx, &A.echunk_start, cs.size()))) return rc;
         if ((rc = adj_alloc(ctx, &A.epart, 6 * (size_t)A.n_echunks))) return rc;
-        HIPCHK(hipMemcpy(A.echunk, ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(A.echunk_start, cs.data(), sizeof(int) * cs.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(A.echunk, ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(A.echunk_start, cs.data(), sizeof(int) * cs.size(), hipMemcpyHostToDevice,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
     }
+    // the buffers were cleared on the context's stream
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return LBFS_OK;
 }
 
@@ -367,6 +373,27 @@ int reverse_step(lbfs_ctx *ctx, const StepRec &r, bool first, int &cur_l)
     return LBFS_OK;
 }
 
+// Replays segment sg from its checkpoint into the step records; the state
+// after the segment is left in ctx->st[1].
+int record_segment(lbfs_ctx *ctx, int sg)
+{
+    AdjointState &A = *ctx->adj;
+    const int K = A.every, N = A.n_steps;
+    const int s0 = sg * K, s1 = std::min(s0 + K, N);
+    int rc;
+    if ((rc = restore(ctx, A.ckpt[sg]))) return rc;
+    for (int t = s0; t < s1; t++) {
+        StepRec &r = A.rec[t - s0];
+        const StepRec *p = t > s0 ? &A.rec[t - 1 - s0] : nullptr;
+        if (!p) r.st = A.ckpt[sg].st;
+        r.X = p ? p->Xn1 : A.ckpt[sg].X;
+        r.glob_prev = p ? p->glob : A.ckpt[sg].glob;
+        float4 *next = t + 1 < s1 ? A.rec[t + 1 - s0].st : ctx->st[1];
+        if ((rc = replay_step(ctx, r, t == s0 ? ctx->st[0] : r.st, next))) return rc;
+    }
+    return LBFS_OK;
+}
+
 }  // namespace
 
 extern "C" int lbfs_adjoint_forward(lbfs_ctx *ctx, int n_steps, int checkpoint_every)
@@ -380,11 +407,37 @@ extern "C" int lbfs_adjoint_forward(lbfs_ctx *ctx, int n_steps, int checkpoint_e
     AdjointState &A = *ctx->adj;
     A.start = A.ckpt[0];
     if ((rc = save(ctx, A.ckpt[0]))) return rc;
-    for (int t = 0; t < n_steps; t++) {
+    // the steps before the last segment, checkpointed every K steps
+    const int n_seg = (n_steps + every - 1) / every, s_last = (n_seg - 1) * every;
+    for (int t = 0; t < s_last; t++) {
         if ((rc = ctx->enqueue_step())) return rc;
-        if ((t + 1) % every == 0 && t + 1 < n_steps)
+        if ((t + 1) % every == 0)
             if ((rc = save(ctx, A.ckpt[(t + 1) / every]))) return rc;
     }
+    // the last segment straight into the step records (the same kernels on
+    // the same inputs), so the backward pass starts without replaying it;
+    // then the solver takes the end state from the records
+    if ((rc = record_segment(ctx, n_seg - 1))) return rc;
+    {
+        const StepRec &r = A.rec[n_steps - 1 - s_last];
+        const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+        ctx->cur = 1;
+        ctx->t_step += n_steps - s_last;
+        if (ctx->have_membranes) {
+            MemArgs &m = ctx->mem;
+            const size_t nn = m.n_nodes, nm = m.n_mem;
+            HIPCHK(hipMemcpyAsync(m.X, r.Xn1, sizeof(float4) * nn, k, ctx->stream));
+            HIPCHK(hipMemcpyAsync(m.U, r.U, sizeof(float4) * nn, k, ctx->stream));
+            HIPCHK(hipMemcpyAsync(m.F, r.F, sizeof(float4) * nn, k, ctx->stream));
+            HIPCHK(hipMemcpyAsync(m.glob, r.glob, sizeof(double) * 8 * nm, k, ctx->stream));
+            HIPCHK(hipMemcpyAsync(ctx->cell_force, r.cf, sizeof(float) * 3 * (size_t)ctx->lay.n_cells, k,
+                                  ctx->stream));
+            // the cell force output now holds spread force anywhere: the
+            // next apply pass resets every tile it does not write
+            HIPCHK(hipMemsetD32Async((hipDeviceptr_t)ctx->ibm.tile_prev, 1, ctx->lay.n_tiles, ctx->stream));
+        }
+    }
+    A.last_in_rec = true;
     if ((rc = save(ctx, A.end))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     A.recorded = true;
@@ -412,8 +465,9 @@ extern "C" int lbfs_adjoint_backward(lbfs_ctx *ctx, const double *lam_nodes, con
     AdjointState &A = *ctx->adj;
     const int n = ctx->lay.n_cells;
     int rc;
-    HIPCHK(hipStreamSynchronize(ctx->stream));  // the context's streams do not wait for the null stream
-    // seeds: adjoints of the final state and node positions
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // seeds: adjoints of the final state and node positions, cleared sums;
+    // all on the context's stream (it does not wait for the null stream)
     {
         std::vector<float4> h(n, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
         if (lam_state)
@@ -422,9 +476,10 @@ extern "C" int lbfs_adjoint_backward(lbfs_ctx *ctx, const double *lam_nodes, con
                 h[i] = make_float4((float)lam_state[4 * (size_t)c], (float)lam_state[4 * (size_t)c + 1],
                                    (float)lam_state[4 * (size_t)c + 2], (float)lam_state[4 * (size_t)c + 3]);
             }
-        HIPCHK(hipMemcpy(A.lam_st[0], h.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
-        HIPCHK(hipMemset(A.gforce, 0, 3 * sizeof(double)));
-        HIPCHK(hipMemset(A.fpart, 0, 3 * sizeof(double) * A.n_fpart));
+        HIPCHK(hipMemcpyAsync(A.lam_st[0], h.data(), sizeof(float4) * n, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemsetAsync(A.gforce, 0, 3 * sizeof(double), ctx->stream));
+        HIPCHK(hipMemsetAsync(A.fpart, 0, 3 * sizeof(double) * A.n_fpart, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
     }
     if (ctx->have_membranes) {
         const int nn = ctx->mem.n_nodes;
@@ -433,27 +488,25 @@ extern "C" int lbfs_adjoint_backward(lbfs_ctx *ctx, const double *lam_nodes, con
             for (int i = 0; i < nn; i++)
                 h[i] = make_float4((float)lam_nodes[3 * (size_t)i], (float)lam_nodes[3 * (size_t)i + 1],
                                    (float)lam_nodes[3 * (size_t)i + 2], 0.0f);
-        HIPCHK(hipMemcpy(A.lam_X, h.data(), sizeof(float4) * nn, hipMemcpyHostToDevice));
-        HIPCHK(hipMemset(A.gmem, 0, 4 * sizeof(double) * ctx->mem.n_mem));
+        HIPCHK(hipMemcpyAsync(A.lam_X, h.data(), sizeof(float4) * nn, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemsetAsync(A.gmem, 0, 4 * sizeof(double) * ctx->mem.n_mem, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
     }
     int cur_l = 0;
     const int K = A.every, N = A.n_steps;
-    for (int sg = (N + K - 1) / K - 1; sg >= 0; sg--) {
+    const int n_seg = (N + K - 1) / K;
+    for (int sg = n_seg - 1; sg >= 0; sg--) {
         const int s0 = sg * K, s1 = std::min(s0 + K, N);
-        // replay the segment into its step records, then reverse it
-        if ((rc = restore(ctx, A.ckpt[sg]))) return rc;
-        for (int t = s0; t < s1; t++) {
-            StepRec &r = A.rec[t - s0];
-            const StepRec *p = t > s0 ? &A.rec[t - 1 - s0] : nullptr;
-            if (!p) r.st = A.ckpt[sg].st;
-            r.X = p ? p->Xn1 : A.ckpt[sg].X;
-            r.glob_prev = p ? p->glob : A.ckpt[sg].glob;
-            float4 *next = t + 1 < s1 ? A.rec[t + 1 - s0].st : ctx->st[1];
-            if ((rc = replay_step(ctx, r, t == s0 ? ctx->st[0] : r.st, next))) return rc;
+        // replay the segment into its step records (the forward pass left
+        // the last one there), then reverse it
+        if (!(sg == n_seg - 1 && A.last_in_rec)) {
+            if ((rc = record_segment(ctx, sg))) return rc;
+            A.last_in_rec = false;
         }
         for (int t = s1 - 1; t >= s0; t--)
             if ((rc = reverse_step(ctx, A.rec[t - s0], t == 0, cur_l))) return rc;
     }
+    if (n_seg == 1) A.last_in_rec = true;  // the records still hold the only segment
     {
         AdjFluidArgs j{};
         j.fpart = A.fpart; j.gforce = A.gforce; j.n_fpart = A.n_fpart;
@@ -538,9 +591,12 @@ extern "C" int lbfs_set_body_force(lbfs_ctx *ctx, const double *f)
             sf[3 * (size_t)i + t] =
                 (float)(f[t] - (t < dims ? rho0 / 3.0 * ctx->lay.closure[3 * (size_t)c + t] / ctx->geo.vol[c] : 0.0));
     }
-    HIPCHK(hipMemcpy(ctx->static_force, sf.data(), sizeof(float) * sf.size(), hipMemcpyHostToDevice));
+    // (copies on the context's stream: it does not wait for the null stream)
+    HIPCHK(hipMemcpyAsync(ctx->static_force, sf.data(), sizeof(float) * sf.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
     // every cell holds the static force until the next spreading overwrites it
-    HIPCHK(hipMemcpy(ctx->cell_force, sf.data(), sizeof(float) * sf.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(ctx->cell_force, sf.data(), sizeof(float) * sf.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     for (int t = 0; t < 3; t++) ctx->params.body_force[t] = f[t];
     return LBFS_OK;
 }
@@ -558,10 +614,12 @@ extern "C" int lbfs_set_membrane_moduli(lbfs_ctx *ctx, int index, double shear_m
     p.local_bending = (float)local_bending_modulus;
     p.global_bending = (float)global_bending_modulus;
     ctx->mem_shear[index] = shear_modulus;
-    HIPCHK(hipMemcpy((void *)(ctx->mem.params + index), &p, sizeof(MemParamDev), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync((void *)(ctx->mem.params + index), &p, sizeof(MemParamDev), hipMemcpyHostToDevice,
+                          ctx->stream));
     const int e0 = ctx->mem_edge_start[index], e1 = ctx->mem_edge_start[index + 1];
     for (int e = e0; e < e1; e++) ctx->mem_edge_const[e].y = (float)(ctx->mem_ks_unit[e] * shear_modulus);
-    HIPCHK(hipMemcpy((void *)(ctx->mem.edge_const + e0), ctx->mem_edge_const.data() + e0, sizeof(float4) * (e1 - e0),
-                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync((void *)(ctx->mem.edge_const + e0), ctx->mem_edge_const.data() + e0,
+                          sizeof(float4) * (e1 - e0), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return LBFS_OK;
 }

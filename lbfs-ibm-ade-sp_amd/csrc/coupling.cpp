@@ -26,7 +26,7 @@ static int cp_alloc(lbfs_ctx *ctx, T **dst, const T *src, size_t n)
     HIPCHK(hipMalloc(&p, (n > 0 ? n : 1) * sizeof(T) + 16));
     ctx->coupling_allocs.push_back(p);
     if (src && n) HIPCHK(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
-    else HIPCHK(hipMemset(p, 0, (n > 0 ? n : 1) * sizeof(T)));
+    else HIPCHK(hipMemsetAsync(p, 0, (n > 0 ? n : 1) * sizeof(T), ctx->stream));  // ordered before the kernels
     *dst = (T *)p;
     return LBFS_OK;
 }
@@ -57,8 +57,8 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     ctx->drop_graph();
     ctx->release_extensions();
     // restore the force field without IBM contributions
-    HIPCHK(hipMemcpy(ctx->cell_force, ctx->static_force, sizeof(float) * 3 * (size_t)ctx->lay.n_cells,
-                     hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpyAsync(ctx->cell_force, ctx->static_force, sizeof(float) * 3 * (size_t)ctx->lay.n_cells,
+                          hipMemcpyDeviceToDevice, ctx->stream));
     std::vector<MembraneSetup> ms(count);
     std::string err;
     for (int i = 0; i < count; i++) {
