@@ -152,11 +152,12 @@ int lbfs_step(lbfs_ctx *ctx, int nsteps);
                                  bins listed up to 65536 nodes, every bin of the grid above), 2 binned
                                  with every bin of the grid; the result is the same bit for bit */
 #define LBFS_OPT_STEP_SCHEDULE 3 /* coupled step schedule (the result is the same bit for bit):
-                                    0 (default) the fluid passes on the main stream, the coupling
-                                    chain in order on a second; 1 the fluid passes on a second
-                                    stream from the start of the step, the membrane forces on a
-                                    third after the interpolation; 2 as 1 with the interpolation
-                                    alone first */
+                                    0 (default) the fluid passes on the main stream from the start
+                                    of the step, the whole coupling chain on a second; 1 the
+                                    interpolation alone, then the fluid passes on the main stream
+                                    and the membrane forces and spreading on a second; 2 the fluid
+                                    passes on a second stream from the start, the membrane forces
+                                    on a third after the interpolation */
 int lbfs_set_option(lbfs_ctx *ctx, int option, int value);
 
 /* per-stage entry points (parity tests) */

@@ -68,10 +68,11 @@ struct IbmArgs {
     float dt;
     const int *inv;           // caller cell id -> internal id
     const float4 *cen;        // internal order: centroid xyz, volume
-    int *bin_count;           // nodes put in each bin this step (cleared by the spreading)
+    int *bin_count;           // nodes put in each bin this step (cleared by the spreading, or by the
+                              // apply pass for listed bins)
     int *bin_slots;           // IBM_BIN_CAP node ids per bin
     int use_list;             // 1: the interpolation lists the occupied bins and the spreading loops
-                              // over the list (few nodes); 0: one spreading workgroup per bin
+                              // over chunks of their nodes (few nodes); 0: one spreading workgroup per bin
     int *bin_list, *n_bin_list;  // the bins holding nodes this step (the count is cleared by the apply pass)
     int *overflow, *n_overflow;  // nodes of full bins (the count is cleared by the cell update)
     float *wnode;             // 64 per node: normalised stencil weights of its current position

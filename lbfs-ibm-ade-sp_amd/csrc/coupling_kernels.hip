@@ -164,17 +164,17 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
 // ---------------------------------------------------------------------------
 // IBM: spreading.  Per node bin (the 4x4x4 block of cells that holds the
 // nodes' stencil base; its nodes were put in the bin's slots, and the bin in
-// the list of occupied bins, by the interpolation), one workgroup at a time
-// (the workgroups loop over the list): the nodes' stencils stay inside a
-// 7x7x7 window, where the weighted forces are accumulated in LDS (one
-// wavefront per node, one lane per stencil slot), then the window is added to
-// the per-cell accumulators in global memory.  The node forces are gathered
-// from their edge records here (eight lanes per node, the order of
-// k_mem_nodes) and stored.  Nodes of full bins are spread one wavefront each
-// after the bins.  All sums are 64-bit fixed point integer additions, so the
-// result does not depend on the order of the nodes, lanes or workgroups.  Each write flags
-// the cell's fluid tile; k_ibm_apply turns the accumulators of the flagged
-// tiles into force densities and clears them.
+// the list of occupied bins, by the interpolation), or per chunk of a bin's
+// nodes: the nodes' stencils stay inside a 7x7x7 window, where the weighted
+// forces are accumulated in LDS (one wavefront per node, one lane per
+// stencil slot), then the window is added to the per-cell accumulators in
+// global memory.  The node forces are gathered from their edge records here
+// (eight lanes per node, the order of k_mem_nodes) and stored.  Nodes of full
+// bins are spread one wavefront each after the bins.  All sums are 64-bit
+// fixed point integer additions, so the result does not depend on the order
+// of the nodes, lanes or workgroups.  Each write flags the cell's fluid tile;
+// k_ibm_apply turns the accumulators of the flagged tiles into force
+// densities and clears them.
 // ---------------------------------------------------------------------------
 // sum of a node's edge slots by eight lanes (sub = lane & 7; 8 or more: no
 // slots), fixed order
@@ -217,36 +217,41 @@ __device__ __forceinline__ void spread_node_global(const MemArgs &m, const IbmAr
     g.tile_now[g.cell_tile[ic]] = 1;
 }
 
-// SPREAD_THREADS: 512 when few bins are occupied (the workgroups loop over
-// the list; wider workgroups finish a bin sooner), 256 for one workgroup per
-// bin of the grid (many nodes: more bins in flight)
-template <int SPREAD_THREADS>
-__global__ __launch_bounds__(SPREAD_THREADS) void k_ibm_spread(MemArgs m, IbmArgs g)
+// LIST (few nodes): the workgroups loop over work items of CH nodes of the
+// listed bins (IBM_BIN_CAP / CH items per bin, empty ones skipped), so a full
+// bin is shared by several workgroups; the apply pass clears the bin counts.
+// Otherwise one workgroup per bin of the grid, its nodes in chunks of CH.
+template <int NT, int CH, bool LIST>
+__global__ __launch_bounds__(NT) void k_ibm_spread(MemArgs m, IbmArgs g)
 {
-    constexpr int SPREAD_WAVES = SPREAD_THREADS / 64;
+    constexpr int NW = NT / 64;
+    constexpr int NCH = LIST ? IBM_BIN_CAP / CH : 1;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     __shared__ unsigned long long racc[3][343];
-    __shared__ int cnd[64];
-    __shared__ int4 cbase[64];
-    __shared__ float4 cF[64];
-    __shared__ float cw[64][64];   // the chunk's cached stencil weights, one row per node
+    __shared__ int cnd[CH];
+    __shared__ int4 cbase[CH];
+    __shared__ float4 cF[CH];
+    __shared__ float cw[CH][64];   // the chunk's cached stencil weights, one row per node
     const int S = stencil_size(g);
     int off[3];
     lane_offset(S, lane, off);
     const bool act = lane < S * S * S;
-    const int n_list = g.use_list ? min(*g.n_bin_list, g.n_bins) : g.n_bins;  // uniform over the workgroup
-    for (int e = blockIdx.x; e < n_list; e += gridDim.x) {
-        const int b = g.use_list ? g.bin_list[e] : e;
+    // uniform over the workgroup
+    const int n_items = LIST ? min(*g.n_bin_list, g.n_bins) * NCH : g.n_bins;
+    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const int b = LIST ? g.bin_list[it / NCH] : it;
         const int cnt_all = min(g.bin_count[b], IBM_BIN_CAP);
-        if (cnt_all == 0) continue;  // (the grid form) uniform over the workgroup
+        const int c_begin = LIST ? (it % NCH) * CH : 0;
+        const int c_end = LIST ? min(cnt_all, c_begin + CH) : cnt_all;
+        if (c_begin >= c_end) continue;  // uniform over the workgroup
         const IbmBlock B = g.blk[g.bin_block[b]];
         const int lb = b - B.bin0;
         const int bo[3] = {4 * (lb % B.tn[0]), 4 * ((lb / B.tn[0]) % B.tn[1]), 4 * (lb / (B.tn[0] * B.tn[1]))};
-        __syncthreads();  // the previous bin's window is flushed
-        for (int q = tid; q < 3 * 343; q += SPREAD_THREADS) racc[q / 343][q % 343] = 0ull;
+        __syncthreads();  // the previous item's window is flushed
+        for (int q = tid; q < 3 * 343; q += NT) racc[q / 343][q % 343] = 0ull;
         const int *slots = g.bin_slots + (size_t)b * IBM_BIN_CAP;
-        for (int c0 = 0; c0 < cnt_all; c0 += 64) {
-            const int cnt = min(64, cnt_all - c0);
+        for (int c0 = c_begin; c0 < c_end; c0 += CH) {
+            const int cnt = min(CH, c_end - c0);
             __syncthreads();
             if (tid < cnt) {
                 const int p = slots[c0 + tid];
@@ -255,7 +260,7 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_ibm_spread(MemArgs m, IbmArg
             }
             __syncthreads();
             // node forces from the edge records: eight lanes per node
-            for (int jn = tid >> 3; jn < cnt; jn += SPREAD_THREADS / 8) {
+            for (int jn = tid >> 3; jn < cnt; jn += NT / 8) {
                 const int p = cnd[jn];
                 const float4 F = node_force8(m, p, tid & 7);
                 if ((tid & 7) == 0) {
@@ -264,10 +269,9 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_ibm_spread(MemArgs m, IbmArg
                 }
             }
             // all weight rows of the chunk in flight at once (one coalesced row per node)
-            for (int q = tid; q < 64 * cnt; q += SPREAD_THREADS)
-                cw[q >> 6][q & 63] = g.wnode[64 * (size_t)cnd[q >> 6] + (q & 63)];
+            for (int q = tid; q < 64 * cnt; q += NT) cw[q >> 6][q & 63] = g.wnode[64 * (size_t)cnd[q >> 6] + (q & 63)];
             __syncthreads();
-            for (int jn = wave; jn < cnt; jn += SPREAD_WAVES) {
+            for (int jn = wave; jn < cnt; jn += NW) {
                 const float wt = cw[jn][lane];
                 if (!act || wt == 0.0f) continue;
                 const int4 nb = cbase[jn];
@@ -300,7 +304,7 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_ibm_spread(MemArgs m, IbmArg
             }
         }
         __syncthreads();
-        for (int q = tid; q < 343; q += SPREAD_THREADS) {
+        for (int q = tid; q < 343; q += NT) {
             const unsigned long long v0 = racc[0][q], v1 = racc[1][q], v2 = racc[2][q];
             if ((v0 | v1 | v2) == 0ull) continue;
             const int l[3] = {q % 7, (q / 7) % 7, q / 49};
@@ -324,11 +328,12 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_ibm_spread(MemArgs m, IbmArg
             if (v2) atomicAdd(acc + 2, v2);
             g.tile_now[g.cell_tile[ic]] = 1;
         }
-        if (tid == 0) g.bin_count[b] = 0;  // read by this workgroup only; cleared for the next step
+        // (the grid form) read by this workgroup only; cleared for the next step
+        if (!LIST && tid == 0) g.bin_count[b] = 0;
     }
     // nodes of full bins, one wavefront each
     const int nov = min(*g.n_overflow, m.n_nodes);
-    for (int k = blockIdx.x * SPREAD_WAVES + wave; k < nov; k += gridDim.x * SPREAD_WAVES) {
+    for (int k = blockIdx.x * NW + wave; k < nov; k += gridDim.x * NW) {
         const int nd = g.overflow[k];
         float4 F = node_force8(m, nd, lane & 7);
         F.x = __shfl(F.x, 0, 64);
@@ -360,6 +365,12 @@ __global__ __launch_bounds__(256) void k_ibm_apply(IbmArgs g, ApplyArgs p)
 {
     const int t = blockIdx.x, tid = threadIdx.x;
     if (t == 0) {
+        // the listed bins' counts (the spreading shares a bin between workgroups)
+        if (g.binned && g.use_list) {
+            const int nl = min(*g.n_bin_list, g.n_bins);
+            for (int e = tid; e < nl; e += blockDim.x) g.bin_count[g.bin_list[e]] = 0;
+        }
+        __syncthreads();
         if (tid == 0 && g.binned) {
             *g.n_overflow = 0;
             *g.n_bin_list = 0;
@@ -520,9 +531,10 @@ hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s)
     // the workgroups loop over the listed bins (or take one bin of the grid
     // each), then over the nodes of full bins
     if (g.binned && g.use_list)
-        hipLaunchKernelGGL(k_ibm_spread<512>, dim3(std::min(g.n_bins, 1024)), dim3(512), 0, s, m, g);
+        hipLaunchKernelGGL((k_ibm_spread<512, 32, true>), dim3(std::min(g.n_bins * (IBM_BIN_CAP / 32), 1024)), dim3(512),
+                           0, s, m, g);
     else if (g.binned)
-        hipLaunchKernelGGL(k_ibm_spread<256>, dim3(g.n_bins), dim3(256), 0, s, m, g);
+        hipLaunchKernelGGL((k_ibm_spread<256, 64, false>), dim3(g.n_bins), dim3(256), 0, s, m, g);
     else
         hipLaunchKernelGGL(k_ibm_spread_nodes, dim3((m.n_nodes + 3) / 4), dim3(256), 0, s, m, g);
     return hipGetLastError();

@@ -126,57 +126,51 @@ int lbfs_ctx::run_stages(bool with_visc)
     // latency-bound sequence of small kernels); the apply pass waits for the
     // face pass and for spreading.  Same kernels, same inputs: the results are
     // those of the sequential order.
-    // Schedule 0 (the default) keeps the fluid passes on the main stream and
-    // runs the whole coupling chain in order on the second stream (fewest
-    // cross-stream waits); schedule 1 starts the fluid passes on the second
-    // stream beside the interpolation, schedule 2 after it.
-    int rc;
-    if (schedule_option == 0) {
+    // Schedule 0 (the default): the fluid passes on the main stream from the
+    // start of the step, the whole coupling chain on a second.  Schedule 1:
+    // the interpolation first and alone on the main stream (beside the fluid
+    // passes it runs several times slower), then the fluid passes on the main
+    // stream and the membrane forces and spreading on a second.  Schedule 2:
+    // the fluid passes on a second stream from the start, the membrane forces
+    // on a third after the interpolation.  (The fluid passes are enqueued
+    // first: a captured graph launches its branches in that order.)  In a
+    // graph each wait across streams costs 10-20 us; schedules 0 and 2 measure
+    // the same, schedule 1 some 2 us more per step.
+    if (schedule_option == 0 || schedule_option == 1) {
+        const bool interp_first = schedule_option == 1;
+        if (interp_first) HIPCHK(launch_ibm_interp(mem, ibm, st[cur], stream));
         HIPCHK(hipEventRecord(ev_fork, stream));
+        HIPCHK(launch_gradients(args, st[cur], grad, stream));
+        HIPCHK(launch_flux_update(args, st[cur], grad, st[cur ^ 1], nullptr, 1, stream));
         HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
-        HIPCHK(launch_ibm_interp(mem, ibm, st[cur], side));
+        if (!interp_first) HIPCHK(launch_ibm_interp(mem, ibm, st[cur], side));
         HIPCHK(launch_mem_volume(mem, side));
         HIPCHK(launch_mem_edges(mem, side));
         if (!ibm.binned) HIPCHK(launch_mem_nodes(mem, side));
         HIPCHK(launch_ibm_spread(mem, ibm, side));
         HIPCHK(hipEventRecord(ev_join, side));
-        HIPCHK(launch_gradients(args, st[cur], grad, stream));
-        HIPCHK(launch_flux_update(args, st[cur], grad, st[cur ^ 1], nullptr, 1, stream));
         HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
         HIPCHK(launch_ibm_apply(ibm, apply_args(st[cur ^ 1], cell_force, 0), stream));
         cur ^= 1;
         return LBFS_OK;
     }
-    const bool interp_first = schedule_option == 2;
-    auto fork_fluid = [this]() -> int {
-        lbfs_ctx *ctx = this;
-        HIPCHK(hipEventRecord(ev_fork, stream));
-        HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
-        HIPCHK(launch_gradients(args, st[cur], grad, side));
-        HIPCHK(launch_flux_update(args, st[cur], grad, st[cur ^ 1], nullptr, 1, side));
-        HIPCHK(hipEventRecord(ev_join, side));
-        return LBFS_OK;
-    };
-    if (!interp_first && (rc = fork_fluid())) return rc;
-    for (auto &s : step_stages(false)) {
-        const char *nm = s.name;
-        if (std::strcmp(nm, "gradients") == 0 || std::strcmp(nm, "flux_update") == 0 ||
-            std::strcmp(nm, "mem_volume") == 0 || std::strcmp(nm, "mem_edges") == 0 ||
-            std::strcmp(nm, "mem_nodes") == 0)
-            continue;
-        if (std::strcmp(nm, "ibm_spread") == 0) HIPCHK(hipStreamWaitEvent(stream, ev_join2, 0));
-        if (std::strcmp(nm, "ibm_apply") == 0) HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
-        if ((rc = s.run())) return rc;
-        if (std::strcmp(nm, "ibm_interp") == 0) {
-            if (interp_first && (rc = fork_fluid())) return rc;
-            HIPCHK(hipEventRecord(ev_fork2, stream));
-            HIPCHK(hipStreamWaitEvent(side2, ev_fork2, 0));
-            HIPCHK(launch_mem_volume(mem, side2));
-            HIPCHK(launch_mem_edges(mem, side2));
-            if (!ibm.binned) HIPCHK(launch_mem_nodes(mem, side2));
-            HIPCHK(hipEventRecord(ev_join2, side2));
-        }
-    }
+    HIPCHK(hipEventRecord(ev_fork, stream));
+    HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
+    HIPCHK(launch_gradients(args, st[cur], grad, side));
+    HIPCHK(launch_flux_update(args, st[cur], grad, st[cur ^ 1], nullptr, 1, side));
+    HIPCHK(hipEventRecord(ev_join, side));
+    HIPCHK(launch_ibm_interp(mem, ibm, st[cur], stream));
+    HIPCHK(hipEventRecord(ev_fork2, stream));
+    HIPCHK(hipStreamWaitEvent(side2, ev_fork2, 0));
+    HIPCHK(launch_mem_volume(mem, side2));
+    HIPCHK(launch_mem_edges(mem, side2));
+    if (!ibm.binned) HIPCHK(launch_mem_nodes(mem, side2));
+    HIPCHK(hipEventRecord(ev_join2, side2));
+    HIPCHK(hipStreamWaitEvent(stream, ev_join2, 0));
+    HIPCHK(launch_ibm_spread(mem, ibm, stream));
+    HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
+    HIPCHK(launch_ibm_apply(ibm, apply_args(st[cur ^ 1], cell_force, 0), stream));
+    cur ^= 1;
     return LBFS_OK;
 }
 

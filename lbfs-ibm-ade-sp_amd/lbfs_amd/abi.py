@@ -618,8 +618,9 @@ class Solver:
         check(load().lbfs_set_option(self.ctx, OPT_IBM_BINNED, int(mode)), self.ctx)
 
     def set_step_schedule(self, mode):
-        """Coupled step schedule: 0 (default) fluid on the main stream, the coupling chain on one
-        side stream; 1 fluid on a side stream from the start; 2 as 1 after the interpolation."""
+        """Coupled step schedule: 0 (default) fluid on the main stream from the start, the whole
+        coupling chain on a side stream; 1 interpolation alone, then fluid on the main stream and
+        membrane forces + spreading on a side stream; 2 fluid on a side stream, forces on a third."""
         check(load().lbfs_set_option(self.ctx, OPT_STEP_SCHEDULE, int(mode)), self.ctx)
 
     def stage_names(self):
