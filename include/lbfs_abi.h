@@ -363,6 +363,23 @@ int lbfs_polymesh_write(const char *dir, int n_points, const double *points, int
                         const int *face_vertices, const int *owner, const int *neighbour, int n_patches,
                         const char *const *patch_names, const char *const *patch_types, const int *patch_start,
                         const int *patch_size, const double *face_shift /* or NULL */);
+/* ---- restart and output files (host only, no TECIO) ----
+ * State files <dir>/rho.txt, u.txt, v.txt, w.txt, one line "i ,value" per cell
+ * in caller order: Solution::output (Solution.cpp:221-251) and
+ * Solution::import (Solution.cpp:987-1064).  precision <= 0: the reference's
+ * default 6 significant digits; 17 round-trips a double. */
+int lbfs_state_txt_write(const char *dir, int n_cells, const double *rho, const double *u, const double *v,
+                         const double *w, int precision);
+int lbfs_state_txt_read(const char *dir, int n_cells, double *rho, double *u, double *v, double *w);
+/* membrane zone file of tecplot_output::spring_network_output_gpu
+ * (tecplot_output.cpp:844-915): header, per node x y z, force, velocity,
+ * nodal area (a third of the incident triangles, spring_network.cu:1486),
+ * index, curvature (0.5 theta L over the node's springs, convex positive,
+ * spring_network.cu:1532), then the 1-based triangle vertex ids one per line.
+ * nodal_area / curvature [n_nodes] receive the written columns (or NULL). */
+int lbfs_membrane_dat_write(const char *path, int n_nodes, const double *X, const double *F, const double *U, int n_tri,
+                            const int *tri, int time, int precision, double *nodal_area, double *curvature);
+
 /* FP64 geometry of a mesh (unstructured_mesh.cpp:1592-1692) */
 int lbfs_mesh_geometry(const lbfs_mesh_desc *mesh, double visc, double gamma, double fneq_min, int flags,
                        double *area, double *normal, double *face_centroid, double *face_dt,

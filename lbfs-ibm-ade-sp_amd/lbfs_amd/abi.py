@@ -159,6 +159,10 @@ def load():
             "lbfs_adjoint_backward": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp, _dp]),
             "lbfs_adjoint_rewind": (C.c_int, [vp]),
             "lbfs_adjoint_run": (C.c_int, [vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
+            "lbfs_state_txt_write": (C.c_int, [C.c_char_p, C.c_int, _dp, _dp, _dp, _dp, C.c_int]),
+            "lbfs_state_txt_read": (C.c_int, [C.c_char_p, C.c_int, _dp, _dp, _dp, _dp]),
+            "lbfs_membrane_dat_write": (C.c_int, [C.c_char_p, C.c_int, _dp, _dp, _dp, C.c_int, _ip, C.c_int, C.c_int,
+                                                  _dp, _dp]),
             "lbfs_polymesh_counts": (C.c_int, [C.c_char_p, _ip, _ip, _ip, _ip, _ip]),
             "lbfs_polymesh_read": (C.c_int, [C.c_char_p, _dp, _ip, _ip, _ip, _ip, _dp]),
             "lbfs_polymesh_patch": (C.c_int, [C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int]),
@@ -496,6 +500,33 @@ def write_membrane_mesh(path, nodes, tri, fmt=MESH_OSAY, springs_=None):
                                        ptr(tri, _ip), 0 if sp is None else len(sp) // 4, ptr(sp, _ip)))
 
 
+def write_state_txt(directory, rho, u, v, w, precision=17):
+    """Restart files rho.txt, u.txt, v.txt, w.txt ("i ,value" per cell); precision 0: 6 digits."""
+    lib = load()
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, u, v, w)]
+    check(lib.lbfs_state_txt_write(os.fsencode(str(directory)), len(arrs[0]), *[ptr(a) for a in arrs], int(precision)))
+
+
+def read_state_txt(directory, n_cells):
+    lib = load()
+    arrs = [np.zeros(n_cells) for _ in range(4)]
+    check(lib.lbfs_state_txt_read(os.fsencode(str(directory)), int(n_cells), *[ptr(a) for a in arrs]))
+    return tuple(arrs)
+
+
+def write_membrane_dat(path, X, F, U, tri, time=0, precision=0):
+    """Membrane zone file (positions, forces, velocities, nodal area, index, curvature, triangles);
+    returns the nodal areas and curvatures written."""
+    lib = load()
+    X, F, U = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (X, F, U)]
+    tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1)
+    n = len(X) // 3
+    area, curv = np.zeros(n), np.zeros(n)
+    check(lib.lbfs_membrane_dat_write(os.fsencode(str(path)), n, ptr(X), ptr(F), ptr(U), len(tri) // 3, ptr(tri, _ip),
+                                      int(time), int(precision), ptr(area), ptr(curv)))
+    return area, curv
+
+
 class Membrane:
     """Host-side description of one membrane (keeps its arrays alive)."""
 
@@ -606,6 +637,22 @@ class Solver:
         X, U, F = (np.zeros(3 * self.n_mem_nodes) for _ in range(3))
         check(load().lbfs_get_membrane_nodes(self.ctx, ptr(X), ptr(U), ptr(F)), self.ctx)
         return X.reshape(-1, 3), U.reshape(-1, 3), F.reshape(-1, 3)
+
+    def write_restart(self, directory, precision=17):
+        """The state as the reference's restart files (rho.txt, u.txt, v.txt, w.txt)."""
+        write_state_txt(directory, *self.get_state(), precision=precision)
+
+    def read_restart(self, directory):
+        """Set the state from the reference's restart files."""
+        self.set_state(*read_state_txt(directory, self.mesh.n_cells))
+
+    def write_membrane_dat(self, path, index=0, time=0, precision=0):
+        """Membrane `index` as the reference's zone file; returns nodal areas and curvatures."""
+        X, U, F = self.membrane_nodes()
+        n0 = sum(len(m.nodes) // 3 for m in self._membranes[:index])
+        m = self._membranes[index]
+        n = len(m.nodes) // 3
+        return write_membrane_dat(path, X[n0:n0 + n], F[n0:n0 + n], U[n0:n0 + n], m.tri, time, precision)
 
     def membrane_globals(self):
         g = np.zeros(6 * len(self._membranes))

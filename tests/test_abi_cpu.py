@@ -184,3 +184,64 @@ def test_cpp_host_mirror_builds_and_fails_cleanly_without_device():
     assert os.path.exists(exe)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode in (0, 3), r.stdout + r.stderr
+
+
+def test_state_files_in_the_reference_format(tmp_path):
+    """Restart files: one "i ,value" line per cell in caller order
+    (Solution::output); 17 digits round-trip exactly, the default 6 digits
+    equal the stream format the reference writes; files written in the
+    reference's own layout read back."""
+    rng = np.random.default_rng(4)
+    n = 37
+    st = [1.0 + 1e-3 * rng.standard_normal(n)] + [1e-2 * rng.standard_normal(n) for _ in range(3)]
+    abi.write_state_txt(tmp_path, *st)
+    back = abi.read_state_txt(tmp_path, n)
+    for a, b in zip(st, back):
+        assert np.array_equal(a, b)
+    lines = (tmp_path / "u.txt").read_text().splitlines()
+    assert len(lines) == n and lines[5].startswith("5 ,")
+    abi.write_state_txt(tmp_path, *st, precision=0)
+    lines = (tmp_path / "rho.txt").read_text().splitlines()
+    assert lines[3] == "3 ,%g" % st[0][3]
+    back6 = abi.read_state_txt(tmp_path, n)
+    np.testing.assert_allclose(back6[0], st[0], rtol=1e-5)
+    # a file as the reference writes it (stream default format, index, " ,", value)
+    d = tmp_path / "ref"
+    d.mkdir()
+    for name, vals in zip(("rho", "u", "v", "w"), ([1.00001, 0.99998], [0.01, -2e-05], [0, 1e-07], [3.5, -1])):
+        (d / (name + ".txt")).write_text("".join("%d ,%s\n" % (i, repr(x) if isinstance(x, float) else x)
+                                                 for i, x in enumerate(vals)))
+    r, u, v, w = abi.read_state_txt(d, 2)
+    assert list(r) == [1.00001, 0.99998] and list(u) == [0.01, -2e-05] and list(v) == [0.0, 1e-07] and list(w) == [3.5, -1.0]
+    with pytest.raises(abi.LbfsError):
+        abi.read_state_txt(d, 3)  # too few lines
+
+
+def test_membrane_dat_file(tmp_path):
+    """Membrane zone file: the reference's header and columns, 1-based
+    triangle ids one per line; nodal areas sum to the surface area, and the
+    curvature sum 0.5 theta L over all springs of a sphere approaches the
+    integral of twice the mean curvature, 8 pi R (convex positive)."""
+    R = 5.0
+    x, tri = abi.icosphere(12)
+    x = R * x.reshape(-1, 3)
+    n, t = len(x), len(tri)
+    F = np.tile([1.0, 2.0, 3.0], (n, 1))
+    U = np.tile([0.1, 0.2, 0.3], (n, 1))
+    area, curv = abi.write_membrane_dat(tmp_path / "cell.dat", x, F, U, tri, time=40)
+    text = (tmp_path / "cell.dat").read_text().split("\n")
+    assert text[0].startswith('VARIABLES = "nodx" "nody" "nodz" "i_force"') and text[0].endswith('"curvature"')
+    assert text[1] == "ZONE Nodes= %d Elements= %d F=FEPOINT ET=TRIANGLE " % (n, t // 3 if tri.ndim == 1 else t)
+    assert text[2] == "SOLUTIONTIME = 40 "
+    row = text[4].split("\t")
+    assert len(row) == 13 and row[-1] == "" and row[10] == "0"
+    assert float(row[3]) == 1.0 and float(row[8]) == 0.3
+    tri_flat = np.asarray(tri).reshape(-1)
+    ids = [int(s.strip()) for s in text[4 + n: 4 + n + len(tri_flat)]]
+    assert ids == list(tri_flat + 1)
+    a_tri = 0.0
+    for a, b, c in tri_flat.reshape(-1, 3):
+        a_tri += 0.5 * np.linalg.norm(np.cross(x[b] - x[a], x[c] - x[a]))
+    assert abs(area.sum() - a_tri) < 1e-9 * a_tri
+    assert abs(curv.sum() - 8 * np.pi * R) < 0.01 * 8 * np.pi * R
+    assert np.all(curv > 0)

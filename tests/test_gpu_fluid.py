@@ -241,3 +241,22 @@ def test_autotuner_picks_a_valid_candidate():
     assert len(table) >= 3
     assert tuple(best.tile) in [t for t, _, _ in table]
     assert best.block_threads in (256, 512)
+
+
+def test_restart_files_continue_the_run(tmp_path):
+    """A run written to the reference's restart files (17 digits) and read
+    into a new solver continues bit for bit like the uninterrupted run."""
+    mesh = cases.channel_mesh(16, 12, 12, seed=8)
+    params = abi.make_params(lattice=abi.D3Q19, visc=VISC, dt=DT, body_force=(2e-5, 0.0, 0.0))
+    a = abi.Solver(mesh, params)
+    a.set_state(*cases.smooth_state(mesh, seed=8, amp=0.005))
+    a.step(10)
+    a.write_restart(tmp_path)
+    b = abi.Solver(mesh, params)
+    b.read_restart(tmp_path)
+    assert np.array_equal(np.concatenate(a.get_state()), np.concatenate(b.get_state()))
+    a.step(10)
+    b.step(10)
+    assert np.array_equal(np.concatenate(a.get_state()), np.concatenate(b.get_state()))
+    a.close()
+    b.close()

@@ -343,3 +343,17 @@ def test_step_schedules_are_bitwise_equal():
     for o in outs[1:]:
         for a, b in zip(outs[0], o):
             assert np.array_equal(a, b)
+
+
+def test_membrane_zone_file_from_the_solver(tmp_path):
+    """The zone file of a membrane after some coupled steps holds the solver's
+    node positions, forces and velocities in the reference's columns."""
+    s, cp, mo, x0 = coupled_pair(abi.IBM_COS4, 5)
+    X, U, F = s.membrane_nodes()
+    area, curv = s.write_membrane_dat(tmp_path / "cell.dat", time=5, precision=17)
+    rows = (tmp_path / "cell.dat").read_text().split("\n")[4:4 + len(X)]
+    vals = np.array([[float(v) for v in r.split("\t")[:12]] for r in rows])
+    assert np.array_equal(vals[:, 0:3], X) and np.array_equal(vals[:, 3:6], F) and np.array_equal(vals[:, 6:9], U)
+    assert np.array_equal(vals[:, 10], np.arange(len(X)))
+    np.testing.assert_allclose(area.sum(), s.membrane_globals()[0][1], rtol=1e-6)
+    s.close()
