@@ -357,3 +357,24 @@ def test_membrane_zone_file_from_the_solver(tmp_path):
     assert np.array_equal(vals[:, 10], np.arange(len(X)))
     np.testing.assert_allclose(area.sum(), s.membrane_globals()[0][1], rtol=1e-6)
     s.close()
+
+
+def test_binned_spreading_across_a_ragged_periodic_axis():
+    """A cell straddling the periodic boundary of an axis whose length is not a
+    multiple of the 4-cell bins (22 cells): spreading over every bin of the
+    grid reaches the wrapped stencils and equals the per-node spreading bit
+    for bit."""
+    outs = []
+    for mode in (0, 2):
+        mesh = cases.channel_mesh(22, 16, 16, seed=16)
+        s = abi.Solver(mesh, abi.make_params(visc=VISC, dt=DT, body_force=(2e-5, 0.0, 0.0)))
+        s.set_state(*cases.smooth_state(mesh, seed=16, amp=0.005))
+        s.set_ibm_binned(mode)
+        x, t = abi.rbc(4, 8.0, center=(1.0, 8.0, 8.0))
+        s.add_membranes([abi.Membrane(x, t, **PARAMS)], ibm_kernel=abi.IBM_COS4)
+        s.step(6)
+        X, U, F = s.membrane_nodes()
+        outs.append((np.concatenate(s.get_state()), s.cell_force(), X, F))
+        s.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
