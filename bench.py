@@ -92,6 +92,8 @@ def main():
                     help="only run three 1 GiB device-to-device copies (the reference for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--long-steps", type=int, default=5000, help="steps of the long channel + RBC run (0: none)")
+    ap.add_argument("--tgv-steps", type=int, default=1000, help="steps of the Taylor-Green check (0: none)")
     ap.add_argument("--only-rbc", action="store_true",
                     help="time the channel + RBC step alone (no plain step, suspension or adjoint): profiling runs")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
@@ -134,6 +136,7 @@ def main():
         log("autotuned tile %s, %d threads" % (list(params.tile), params.block_threads))
     if args.only_rbc:
         args.no_batched = args.no_adjoint = True
+        args.long_steps = args.tgv_steps = 0
         ms_plain = float("nan")
     else:
         plain = abi.Solver(mesh, params)
@@ -162,6 +165,50 @@ def main():
     diag = solver.diagnostics(ref_rho=1.0, ref_u=0.01)
     mglob = solver.membrane_globals()[0]
     solver.close()
+
+    # --- the same channel + RBC over 5000 steps from rest: stability and
+    # conservation of the long run (not timed against the others) ---
+    long_run = None
+    if args.long_steps > 0:
+        log("long run: %d steps" % args.long_steps)
+        lr = abi.Solver(mesh, params)
+        lr.set_state(*state)
+        lr.add_membranes(workloads.single_rbc(mesh), ibm_kernel=abi.IBM_COS4)
+        d0 = lr.diagnostics(ref_rho=1.0, ref_u=0.01)
+        g0 = lr.membrane_globals()[0]
+        t0 = time.perf_counter()
+        lr.step(args.long_steps)
+        lr.sync()
+        t1 = time.perf_counter()
+        d1 = lr.diagnostics(ref_rho=1.0, ref_u=0.01)
+        g1 = lr.membrane_globals()[0]
+        X, _, _ = lr.membrane_nodes()
+        lr.close()
+        long_run = {"steps": args.long_steps, "ms_per_step": (t1 - t0) * 1e3 / args.long_steps,
+                    "mass_rel_change": d1["total_mass"] / d0["total_mass"] - 1.0, "max_speed": d1["max_speed"],
+                    "membrane_volume_rel_change": g1[0] / g0[0] - 1.0, "membrane_area_rel_change": g1[1] / g0[1] - 1.0,
+                    "membrane_centroid": [float(v) for v in X.mean(axis=0)], "finite": bool(np.isfinite(X).all())}
+        log("long run: %.4f ms/step, volume change %.2e" % (long_run["ms_per_step"], long_run["membrane_volume_rel_change"]))
+
+    # --- Taylor-Green vortex, 64x64 D2Q9 on a jittered periodic mesh (the
+    # reference's 2D verification case): amplitude against exp(-2 nu k^2 t) ---
+    taylor_green = None
+    if args.tgv_steps > 0:
+        mt, pt, stt = workloads.tgv_2d()
+        tg = abi.Solver(mt, pt)
+        tg.set_state(*stt)
+        tg.step(args.tgv_steps)
+        _, u1, v1, _ = tg.get_state()
+        tg.close()
+        k = 2 * np.pi / 64
+        tt = args.tgv_steps * pt.dt
+        decay = float(np.exp(-2 * pt.visc * k * k * tt))
+        amp = float(np.dot(np.concatenate([u1, v1]), np.concatenate([stt[1], stt[2]])) /
+                    np.dot(np.concatenate([stt[1], stt[2]]), np.concatenate([stt[1], stt[2]])))
+        taylor_green = {"workload": "64x64 D2Q9, jitter 0.05, periodic, Re 100, U0 0.01, dt %g, %d steps"
+                                    % (pt.dt, args.tgv_steps),
+                        "amplitude": amp, "analytic": decay, "rel_error": amp / decay - 1.0}
+        log("Taylor-Green: amplitude %.6f, analytic %.6f" % (amp, decay))
 
     # --- 256x128x128 suspension of 256 RBCs of mixed resolution ---
     batched = None
@@ -291,6 +338,8 @@ def main():
         "event_ms_per_step": ev_ms,
         "max_speed": diag["max_speed"],
         "membrane_volume_rel_change": mglob[0] / mglob[3] - 1.0,
+        "long_run": long_run,
+        "taylor_green": taylor_green,
         "batched": batched,
         "ensemble": ensemble,
         "adjoint": adjoint,

@@ -92,7 +92,8 @@ def test_channel_step_parity(lat_a, lat_o, flags):
 
 
 def test_taylor_green_2d_parity_and_decay():
-    """Config-1 style: D2Q9 on a jittered 2D periodic mesh, 400 steps."""
+    """Taylor-Green vortex: D2Q9 on a jittered 2D periodic mesh, 400 steps,
+    against the FP64 oracle and the analytic decay."""
     N = 32
     m = abi.HexMesh(N, N, 1, periodic=(1, 1, 1), jitter=0.05, seed=1)
     nu = 0.01 * N / 100.0

@@ -193,7 +193,7 @@ def _tgv_runs(flags, N=24, steps=160, dt=0.25, empty=False, jitter=0.05):
 
 
 def test_taylor_green_d3q15_projects_exactly_with_tie_band():
-    """Config-1 style TGV on a jittered 2D mesh (one z layer, z-periodic).
+    """Taylor-Green vortex on a jittered 2D mesh (one z layer, z-periodic).
     With the tie band (LBFS.cu:349-352) the +-z points of D3Q15 take the face
     average, and D3Q15 then reproduces D2Q9 to round-off (projection identity)."""
     _, _, runs = _tgv_runs(orc.FLAG_UPWIND_BAND)
