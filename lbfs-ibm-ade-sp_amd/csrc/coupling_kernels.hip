@@ -220,7 +220,8 @@ __device__ __forceinline__ void spread_node_global(const MemArgs &m, const IbmAr
 // LIST (few nodes): the workgroups loop over work items of CH nodes of the
 // listed bins (IBM_BIN_CAP / CH items per bin, empty ones skipped), so a full
 // bin is shared by several workgroups; the apply pass clears the bin counts.
-// Otherwise one workgroup per bin of the grid, its nodes in chunks of CH.
+// Otherwise one workgroup per bin of the grid, its nodes in chunks of CH
+// (small workgroups measured fastest there: 128 threads, 16-node chunks).
 template <int NT, int CH, bool LIST>
 __global__ __launch_bounds__(NT) void k_ibm_spread(MemArgs m, IbmArgs g)
 {
@@ -534,7 +535,7 @@ hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s)
         hipLaunchKernelGGL((k_ibm_spread<512, 32, true>), dim3(std::min(g.n_bins * (IBM_BIN_CAP / 32), 1024)), dim3(512),
                            0, s, m, g);
     else if (g.binned)
-        hipLaunchKernelGGL((k_ibm_spread<256, 64, false>), dim3(g.n_bins), dim3(256), 0, s, m, g);
+        hipLaunchKernelGGL((k_ibm_spread<128, 16, false>), dim3(g.n_bins), dim3(128), 0, s, m, g);
     else
         hipLaunchKernelGGL(k_ibm_spread_nodes, dim3((m.n_nodes + 3) / 4), dim3(256), 0, s, m, g);
     return hipGetLastError();
