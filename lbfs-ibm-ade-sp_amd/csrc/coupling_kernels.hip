@@ -87,8 +87,10 @@ __device__ __forceinline__ void mem_coefs(const MemArgs &m, int mem, float &vol,
 // of its current position), advect the node and count it into its bin.
 // Always: weights of the (new) position, normalised for the cosine kernel,
 // cached for spreading and the next interpolation.
+// (held to 64 registers: 8 wavefronts per SIMD keep more nodes' loads in flight)
 template <bool INTERP>
-__global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const float4 *__restrict__ st)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_ibm_interp(MemArgs m, IbmArgs g,
+                                                                                         const float4 *__restrict__ st)
 {
     const int lane = threadIdx.x & 63;
     const int nd = blockIdx.x * 4 + (threadIdx.x >> 6);
