@@ -21,8 +21,8 @@ struct lbfs_ctx {
     std::string err;
     int device = 0;
     hipStream_t stream = nullptr;
-    hipStream_t side = nullptr;       // second stream: the gradient pass beside the coupling chain
-    hipStream_t side2 = nullptr;      // third stream: membrane forces beside the node binning
+    hipStream_t side = nullptr;       // second stream: the coupling chain beside the fluid passes (or the reverse)
+    hipStream_t side2 = nullptr;      // third stream: membrane forces (step schedule 2)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr;
     lbfs_params params{};
     std::vector<lbfs::HostBlock> blocks;   // logical boxes of the mesh (empty: none)
