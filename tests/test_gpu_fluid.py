@@ -112,6 +112,33 @@ def test_taylor_green_2d_parity_and_decay():
     assert abs(np.dot(us, u0) / np.dot(u0, u0) / decay - 1.0) < 0.02
 
 
+def test_taylor_green_64_for_1000_steps():
+    """The Taylor-Green verification case at its full size: 64x64 D2Q9 on a
+    jittered periodic mesh (jitter 0.05, seed 1), Re 100, U0 0.01, 1000 steps:
+    velocity within 1e-5 relative L2 of the FP64 oracle and the amplitude
+    within 0.2 % of exp(-2 nu k^2 t)."""
+    N = 64
+    m = abi.HexMesh(N, N, 1, periodic=(1, 1, 1), jitter=0.05, seed=1)
+    nu = 0.01 * N / 100.0
+    s, o = pair(m, abi.D2Q9, orc.OR_D2Q9, visc=nu)
+    st = cases.tgv_state(m, 0.01)
+    s.set_state(*st)
+    o.set_state(*st)
+    s.step(1000)
+    o.step(1000)
+    _, us, vs, _ = s.get_state()
+    _, uo, vo, _ = o.state()
+    err = cases.rel_l2(np.concatenate([us, vs]), np.concatenate([uo, vo]))
+    k = 2 * np.pi / N
+    decay = np.exp(-2 * nu * k * k * DT * 1000)
+    q0 = np.concatenate([st[1], st[2]])
+    amp = np.dot(np.concatenate([us, vs]), q0) / np.dot(q0, q0)
+    print("Taylor-Green 64x64, 1000 steps: rel L2 vs oracle %.2e, amplitude %.6f vs analytic %.6f" % (err, amp, decay))
+    assert err < 1e-5, err
+    assert abs(amp / decay - 1.0) < 2e-3
+    s.close()
+
+
 def test_bitwise_repeatable():
     m = cases.channel_mesh(16, 12, 10)
     outs = []
