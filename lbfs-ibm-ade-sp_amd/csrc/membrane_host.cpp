@@ -116,11 +116,9 @@ int build_springs(int n_nodes, const std::vector<int> &tri, std::vector<int> &sp
         }
         for (int j = 0; j < d; j++) {
             int q[4] = {i, ring[j], ring[(j + 1) % d], ring[(j + 2) % d]};
-            int s[4] = {q[0], q[1], q[2], q[3]};
-            std::sort(s, s + 4);
-            uint64_t key = 0;
-            for (int a = 0; a < 4; a++) key = key * (uint64_t)(n_nodes + 1) + (uint64_t)s[a];
-            if (!seen.insert(key).second) continue;
+            // a spring is its edge a-c (each edge of a closed surface once)
+            const uint64_t lo = (uint64_t)std::min(q[0], q[2]), hi = (uint64_t)std::max(q[0], q[2]);
+            if (!seen.insert(lo * (uint64_t)n_nodes + hi).second) continue;
             springs.insert(springs.end(), q, q + 4);
         }
     }
