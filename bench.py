@@ -245,21 +245,23 @@ def main():
             mp = mems[0].params
             mu, kl, kg = mp["shear_modulus"], mp["local_bending_modulus"], mp["global_bending_modulus"]
             a = abi.Solver(mesh, params)
-            a.set_state(*state)
-            a.add_membranes(mems, ibm_kernel=abi.IBM_COS4)
-            a.set_membrane_moduli(0, 1.1 * mu, 1.1 * kl, 1.1 * kg)
-            a.adjoint_forward(args.adjoint_steps, every)
-            target, _, _ = a.membrane_nodes()
-            a.adjoint_rewind()
-            a.set_membrane_moduli(0, mu, kl, kg)
-            a.sync()
-            t0 = time.perf_counter()
-            a.adjoint_forward(args.adjoint_steps, every)
-            t1 = time.perf_counter()
-            X, _, _ = a.membrane_nodes()
-            g = a.adjoint_backward(lam_nodes=2.0 * (X - target))
-            t2 = time.perf_counter()
-            a.close()
+            try:
+                a.set_state(*state)
+                a.add_membranes(mems, ibm_kernel=abi.IBM_COS4)
+                a.set_membrane_moduli(0, 1.1 * mu, 1.1 * kl, 1.1 * kg)
+                a.adjoint_forward(args.adjoint_steps, every)
+                target, _, _ = a.membrane_nodes()
+                a.adjoint_rewind()
+                a.set_membrane_moduli(0, mu, kl, kg)
+                a.sync()
+                t0 = time.perf_counter()
+                a.adjoint_forward(args.adjoint_steps, every)
+                t1 = time.perf_counter()
+                X, _, _ = a.membrane_nodes()
+                g = a.adjoint_backward(lam_nodes=2.0 * (X - target))
+                t2 = time.perf_counter()
+            finally:
+                a.close()
             fwd = (t1 - t0) * 1e3 / args.adjoint_steps
             bwd = (t2 - t1) * 1e3 / args.adjoint_steps
             log("adjoint: forward %.4f ms/step, backward %.4f ms/step" % (fwd, bwd))
@@ -275,7 +277,15 @@ def main():
                     "note": ("forward records every step; backward = reverse kernels"
                              if every >= args.adjoint_steps else
                              "forward records the last segment; backward = segment replays + reverse kernels")}
-        adjoint = run_adjoint(args.adjoint_steps)
+        # every step recorded (about 77 MB per step on this mesh): if the device
+        # cannot hold the whole run, record the last quarter or less instead
+        adjoint = None
+        for every in (args.adjoint_steps, args.adjoint_steps // 4, args.adjoint_every):
+            try:
+                adjoint = run_adjoint(every)
+                break
+            except abi.LbfsError as e:
+                log("adjoint with %d recorded steps failed (%s); fewer records" % (every, e))
         adjoint_ckpt = run_adjoint(args.adjoint_every)
 
     ensemble = None
