@@ -4,7 +4,10 @@ The reference has no tests for spring_network.cu or the IBM kernels; the
 oracle is pinned by translation/rotation invariance, the zero-force reference
 state, forces equal to minus the gradient of the energies they derive from
 (finite differences), an independent dihedral-angle evaluation for the
-bending term, and partition of unity / conservation / adjointness of the IBM
+bending terms, the closed form of the area difference, frozen-coefficient
+gradients for the terms that are not energy gradients (curvature, local
+bending, area difference), a dissipation function for the membrane
+viscosity, and partition of unity / conservation / adjointness of the IBM
 weights.
 """
 import ctypes as C
@@ -144,6 +147,142 @@ def test_bending_force_is_gradient_of_theta_length_with_frozen_coefficients():
         xm[j] -= h
         g = (G(xp) - G(xm)) / (2 * h)
         assert abs(-g - F[j]) < 1e-9 + 1e-5 * abs(F[j]), (j, -g, F[j])
+
+
+def _theta_length(X, sp):
+    L = np.linalg.norm(X[sp[:, 2]] - X[sp[:, 0]], axis=1)
+    return _dihedral(X, sp) * L
+
+
+def _spring_norms(X, sp):
+    """|N1|, |N2|: twice the areas of the spring's triangles (a, b, c) and (a, c, d)."""
+    A, B, Cn, D = (X[sp[:, k]] for k in range(4))
+    return (np.linalg.norm(np.cross(B - A, Cn - A), axis=1), np.linalg.norm(np.cross(Cn - A, D - A), axis=1))
+
+
+def _surface_area(X, t):
+    tt = t.reshape(-1, 3)
+    return 0.5 * np.sum(np.linalg.norm(np.cross(X[tt[:, 1]] - X[tt[:, 0]], X[tt[:, 2]] - X[tt[:, 0]]), axis=1))
+
+
+def test_mad_term_uses_the_previous_call_and_the_closed_form_mad():
+    """The area-difference term of spring_network.cu:645 and :899-916: a second
+    force call adds mad_coef d(sum theta L)/dX with mad_coef = -2 k (MAD - MAD0) / A0
+    taken from the previous call's MAD, and MAD = sum kappa (|N1| + |N2|) / 3 / A
+    reduces to 2 sum theta L / A (kappa = theta L / ((|N1| + |N2|) / 6))."""
+    x0, t = small_rbc(freq=3, d0=6.0)
+    k = 2e-3
+    params = dict(NO_BEND, global_bending_modulus=k, shear_modulus=0.0, volume_modulus=0.0,
+                  local_area_modulus=0.0, global_area_modulus=0.0)
+    m = orc.OracleMembrane(x0, t, params)
+    sp = m.spring.reshape(-1, 4)
+    mad = lambda X: 2.0 * np.sum(_theta_length(X, sp)) / _surface_area(X, t)
+    assert abs(m.globals()[5] - mad(x0)) < 1e-12 * abs(mad(x0))
+    rng = np.random.default_rng(6)
+    X1 = 1.3 * x0 + 0.05 * rng.standard_normal(x0.shape)  # MAD scales as 1/size
+    m.forces(X1)
+    assert abs(m.globals()[2] - mad(X1)) < 1e-12 * abs(mad(X1))
+    X = x0 + 0.02 * rng.standard_normal(x0.shape)
+    F = m.forces(X).reshape(-1)
+    mad_coef = -2.0 * k * (mad(X1) - mad(x0)) / _surface_area(x0, t)
+    assert abs(mad_coef) > 0.1 * k * abs(mad(x0)) / _surface_area(x0, t)
+    c = k * (m.arr["curvature"] - m.arr["curvature_0"]) - mad_coef
+    G = lambda xf: np.sum(c * _theta_length(xf.reshape(-1, 3), sp))
+    Xf = X.reshape(-1).copy()
+    h = 1e-6
+    for j in np.random.default_rng(7).choice(len(Xf), 25, replace=False):
+        xp, xm = Xf.copy(), Xf.copy()
+        xp[j] += h
+        xm[j] -= h
+        g = (G(xp) - G(xm)) / (2 * h)
+        assert abs(-g - F[j]) < 1e-9 + 1e-5 * abs(F[j]), (j, -g, F[j])
+    # the term alone: a fresh membrane's first call at X has mad_coef = 0 and
+    # the same curvature coefficients, so the difference is
+    # mad_coef d(sum theta L)/dX (this pins the factor 2, A0 rather than A and
+    # the previous call's MAD rather than the current one)
+    m2 = orc.OracleMembrane(x0, t, params)
+    dF = F - m2.forces(X).reshape(-1)
+    S = lambda xf: np.sum(_theta_length(xf.reshape(-1, 3), sp))
+    for j in np.random.default_rng(12).choice(len(Xf), 25, replace=False):
+        xp, xm = Xf.copy(), Xf.copy()
+        xp[j] += h
+        xm[j] -= h
+        g = mad_coef * (S(xp) - S(xm)) / (2 * h)
+        assert abs(g - dF[j]) < 1e-6 * np.max(np.abs(dF)), (j, g, dF[j])
+
+
+def test_local_bending_force_is_gradient_of_spring_area_with_frozen_curvature():
+    """The local bending term of spring_network.cu:870-896:
+    F = -d/dX sum_e 0.5 k_b (kappa0_e - kappa_e)^2 (|N1| + |N2|) / 6 with kappa
+    held at the evaluation point."""
+    x0, t = small_rbc(freq=3, d0=6.0)
+    kb = 5e-3
+    params = dict(NO_BEND, local_bending_modulus=kb, shear_modulus=0.0, volume_modulus=0.0,
+                  local_area_modulus=0.0, global_area_modulus=0.0)
+    m = orc.OracleMembrane(x0, t, params)
+    sp = m.spring.reshape(-1, 4)
+    X = x0 + 0.02 * np.random.default_rng(8).standard_normal(x0.shape)
+    F = m.forces(X).reshape(-1)
+    assert np.max(np.abs(F)) > 0
+    w = 0.5 * kb * (m.arr["curvature_0"] - m.arr["curvature"]) ** 2
+
+    def G(xf):
+        n1, n2 = _spring_norms(xf.reshape(-1, 3), sp)
+        return np.sum(w * (n1 + n2) / 6.0)
+
+    Xf = X.reshape(-1).copy()
+    h = 1e-6
+    for j in np.random.default_rng(9).choice(len(Xf), 25, replace=False):
+        xp, xm = Xf.copy(), Xf.copy()
+        xp[j] += h
+        xm[j] -= h
+        g = (G(xp) - G(xm)) / (2 * h)
+        assert abs(-g - F[j]) < 1e-12 + 1e-5 * abs(F[j]), (j, -g, F[j])
+    # the curvature the term uses is theta L / ((|N1| + |N2|) / 6)
+    n1, n2 = _spring_norms(X, sp)
+    kappa = _theta_length(X, sp) / ((n1 + n2) / 6.0)
+    np.testing.assert_allclose(m.arr["curvature"], kappa, rtol=1e-10, atol=1e-12)
+
+
+def test_osay_viscous_force_is_minus_the_gradient_of_a_dissipation_function():
+    """The viscous term of spring_network.cu:851-866 is linear in the node
+    velocities and equals -dR/dV with R = (2 / sqrt 3) eta sum_e (e_hat . (v_c - v_a))^2;
+    it conserves momentum, vanishes for rigid translation and rotation and
+    dissipates (V . F <= 0)."""
+    x0, t = small_rbc(freq=3, d0=6.0)
+    eta = 0.7
+    params = dict(abi.RBC_PARAMS, viscosity=eta)
+    m = orc.OracleMembrane(x0, t, params)
+    sp = m.spring.reshape(-1, 4)
+    rng = np.random.default_rng(10)
+    X = x0 + 0.02 * rng.standard_normal(x0.shape)
+    m.forces(X)  # the area-difference term uses the previous call's MAD: make it MAD(X)
+    F0 = m.forces(X)
+    visc = lambda V: (m.forces(X, V) - F0).reshape(-1)
+
+    d = X[sp[:, 2]] - X[sp[:, 0]]
+    e = d / np.linalg.norm(d, axis=1)[:, None]
+
+    def R(vf):
+        V = vf.reshape(-1, 3)
+        return 2.0 / np.sqrt(3.0) * eta * np.sum(np.einsum("ij,ij->i", e, V[sp[:, 2]] - V[sp[:, 0]]) ** 2)
+
+    V = rng.standard_normal(X.shape)
+    Fv = visc(V)
+    Vf = V.reshape(-1).copy()
+    h = 1e-3  # R is quadratic: central differences are exact up to round-off
+    for j in np.random.default_rng(11).choice(len(Vf), 30, replace=False):
+        vp, vm = Vf.copy(), Vf.copy()
+        vp[j] += h
+        vm[j] -= h
+        g = (R(vp) - R(vm)) / (2 * h)
+        assert abs(-g - Fv[j]) < 1e-10 + 1e-8 * abs(Fv[j]), (j, -g, Fv[j])
+    W = rng.standard_normal(X.shape)
+    np.testing.assert_allclose(visc(2.0 * V - 3.0 * W), 2.0 * Fv - 3.0 * visc(W), rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(Fv.reshape(-1, 3).sum(axis=0), 0.0, atol=1e-10 * np.abs(Fv).max())
+    assert np.dot(Vf, Fv) < 0
+    rigid = np.array([0.3, -0.2, 0.1]) + np.cross(np.array([0.05, 0.02, -0.04]), X - X.mean(axis=0))
+    assert np.max(np.abs(visc(rigid))) < 1e-12 * np.max(np.abs(Fv))
 
 
 def test_spring_constant_formula():
