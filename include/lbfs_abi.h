@@ -60,6 +60,12 @@ extern "C" {
 #define LBFS_FLAG_LS_DET_COMPAT 16   /* LS determinant as written in gradients.cpp:282-284 */
 #define LBFS_FLAG_CLOSED_FACES 32    /* face area |sum of triangle normals| / 2 instead of the sum of
                                         triangle areas (unstructured_mesh.cpp:1193-1196): cells close exactly */
+#define LBFS_FLAG_GHOST_FACE_GRAD 64 /* on a boundary face every lattice point takes the ghost gradient, as the
+                                        GPU reconstruction sets grad_*_1 = grad_*_2 there (LBFS.cu:391-394);
+                                        without it the owner-side points take the owner's own gradient
+                                        (Solver.cpp:918-1000) */
+/* the reference GPU solver's semantics (calc_face_flux / calc_face_flux_bc, LBFS.cu:328-936, 1933-2048) */
+#define LBFS_FLAGS_REFERENCE_GPU (LBFS_FLAG_UPWIND_BAND | LBFS_FLAG_GHOST_DIR_GPU | LBFS_FLAG_GHOST_FACE_GRAD)
 
 /* ---- IBM kernels ---- */
 #define LBFS_IBM_LINEAR2 0 /* 2-point trilinear on the logical grid, immersed_boundary_method.cu:14-97, 947-1024 */

@@ -238,6 +238,14 @@ __device__ __forceinline__ void face_vjp(const FluidArgs &a, const float4 *__res
             pn[v] = p0[v];
         }
     }
+    // LBFS_FLAG_GHOST_FACE_GRAD: the owner-side points of a boundary face take
+    // the ghost gradient (LBFS.cu:391-394); the reverse below routes their
+    // gradient adjoint into the ghost gradient's
+    const bool ghost_grad_all = ghost && (a.flags & LBFS_FLAG_GHOST_FACE_GRAD);
+    if (ghost_grad_all) {
+#pragma unroll
+        for (int t = 0; t < 12; t++) go[t] = gn[t];
+    }
     float Go[12], Gn[12];
 #pragma unroll
     for (int t = 0; t < 12; t++) { Go[t] = go[t] * dt; Gn[t] = gn[t] * dt; }
@@ -313,8 +321,13 @@ __device__ __forceinline__ void face_vjp(const FluidArgs &a, const float4 *__res
             float lgv[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                lgo_[3 * v + c] = dt * lGo[3 * v + c];
-                lgv[c] = dt * lGn[3 * v + c];
+                if (ghost_grad_all) {
+                    lgo_[3 * v + c] = 0.0f;
+                    lgv[c] = dt * (lGn[3 * v + c] + lGo[3 * v + c]);
+                } else {
+                    lgo_[3 * v + c] = dt * lGo[3 * v + c];
+                    lgv[c] = dt * lGn[3 * v + c];
+                }
             }
             lso[v] = 0.0f;
             if (ty[v] == 1 || ty[v] == 3) {

@@ -134,6 +134,12 @@ __device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__
             po[v] = p0[v];
             pn[v] = p0[v];
         }
+        // the GPU reconstruction hands the ghost gradient to the owner-side
+        // points too (grad_*_1 = grad_*_2, LBFS.cu:391-394)
+        if (a.flags & LBFS_FLAG_GHOST_FACE_GRAD) {
+#pragma unroll
+            for (int t = 0; t < 12; t++) go[t] = gn[t];
+        }
     }
 #pragma unroll
     for (int t = 0; t < 12; t++) { go[t] *= dt; gn[t] *= dt; }

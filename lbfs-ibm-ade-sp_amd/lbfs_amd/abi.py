@@ -17,6 +17,9 @@ BC_DIRICHLET, BC_NEUMANN, BC_PARABOLIC, BC_SHEAR, BC_EMPTY = 1, 2, 9, 10, 11
 TS_LOCAL, TS_CONSTANT, TS_MIN = 1, 2, 3
 MESH_OSAY, MESH_MZ = 0, 1
 FLAG_UPWIND_BAND, FLAG_NEUMANN_SIGNED, FLAG_GHOST_DIR_GPU, FLAG_LS_DET_COMPAT, FLAG_CLOSED_FACES = 1, 2, 4, 16, 32
+FLAG_GHOST_FACE_GRAD = 64
+# the reference GPU solver's face semantics (tie band, ghost direction, ghost gradient at every point)
+FLAGS_REFERENCE_GPU = FLAG_UPWIND_BAND | FLAG_GHOST_DIR_GPU | FLAG_GHOST_FACE_GRAD
 IBM_LINEAR2, IBM_COS4, IBM_LAGRANGE4 = 0, 1, 2
 OPT_GRAPH, OPT_IBM_BINNED, OPT_STEP_SCHEDULE = 1, 2, 3
 

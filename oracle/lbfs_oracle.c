@@ -403,7 +403,9 @@ void or_gradients(const or_mesh *m, const or_params *p, const double *st, double
  * the rest point the average.  Boundary faces (ghost neighbour) use the face
  * value from the BC and the ghost gradient on the outer side
  * (Solver.cpp:855-900).  OR_FLAG_UPWIND_BAND switches to the GPU tie band
- * |e.n| <= 0.001 dt with the averaged gradient (LBFS.cu:349-352, 456-932).
+ * |e.n| <= 0.001 dt with the averaged gradient (LBFS.cu:349-352, 456-932),
+ * OR_FLAG_GHOST_FACE_GRAD to the GPU's ghost gradient at every lattice point
+ * of a boundary face (LBFS.cu:391-394).
  * The relaxation factor carries the owner's viscosity ratio as on the GPU
  * (LBFS.cu:2676, 1488).
  */
@@ -422,6 +424,9 @@ void or_face_flux(const or_mesh *m, const or_params *p, const double *st, const 
         double dt = m->fdt[f], A = m->area[f];
         double t1[3], t2[3], phi_i[4], phi_nb[4], phi0[4];
         const double *gi = g + 12 * i, *gn = g + 12 * nb;
+        /* GPU form: the owner-side points of a boundary face use the ghost
+         * gradient too (LBFS.cu:391-394); Solver.cpp:918-1000 keeps the owner's */
+        if (ghost && (p->flags & OR_FLAG_GHOST_FACE_GRAD)) gi = gn;
         double lat[27][4], feq[27], rho_f, u_f[3], tau, fneq_fac;
         for (t = 0; t < 3; t++) {
             t1[t] = m->cc[3 * i + t] - m->fc[3 * f + t];

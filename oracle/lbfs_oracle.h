@@ -28,6 +28,8 @@ extern "C" {
 #define OR_FLAG_NEUMANN_SIGNED 2 /* CPU Neumann ghost gradient (1 - n) (gradients.cpp:412-414) instead of (1 - |n|) */
 #define OR_FLAG_GHOST_DIR_GPU 4  /* Dirichlet ghost gradient along owner->ghost (LBFS.cu:1965-1985) instead of face normal */
 #define OR_FLAG_LITERAL_D3Q15 8  /* use the hand-expanded D3Q15 formulas instead of the table-driven lattice */
+#define OR_FLAG_GHOST_FACE_GRAD 64 /* boundary faces: every lattice point takes the ghost gradient, as the GPU
+                                      reconstruction sets grad_*_1 = grad_*_2 there (LBFS.cu:391-394) */
 
 typedef struct {
     int n_cells, n_faces, n_int, n_bc;

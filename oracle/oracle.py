@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 OR_D3Q15, OR_D3Q19, OR_D2Q9 = 0, 1, 2
 FLAG_UPWIND_BAND, FLAG_NEUMANN_SIGNED, FLAG_GHOST_DIR_GPU, FLAG_LITERAL_D3Q15 = 1, 2, 4, 8
+FLAG_GHOST_FACE_GRAD = 64
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

@@ -176,13 +176,14 @@ def test_rest_state_drifts_with_reference_face_area():
         assert (np.abs(u).max() > 1e-6) == moves
 
 
-def _tgv_runs(flags, N=24, steps=160, dt=0.25, empty=False, jitter=0.05):
+def _tgv_runs(flags, N=24, steps=160, dt=0.25, empty=False, jitter=0.05, nu=None):
     m = abi.HexMesh(N, N, 1, periodic=(1, 1, 0 if empty else 1), jitter=jitter, seed=1)
     if empty:
         for patch in (4, 5):
             m.set_patch_bc(patch, abi.BC_EMPTY, abi.BC_EMPTY, (1.0, 0.0, 0.0, 0.0))
     U0, Re = 0.01, 100.0
-    nu = U0 * N / Re
+    if nu is None:
+        nu = U0 * N / Re
     runs = {}
     for kind in (orc.OR_D3Q15, orc.OR_D2Q9):
         o = orc.OracleFluid(m, kind, visc=nu, dt=dt, flags=flags)
@@ -202,16 +203,23 @@ def test_taylor_green_d3q15_projects_exactly_with_tie_band():
 
 
 def test_taylor_green_decays_at_viscous_rate():
-    """Strict upwinding (Solver.cpp): both lattices decay as exp(-2 nu k^2 t)."""
-    m, nu, runs = _tgv_runs(0)
-    N, dt, steps = 24, 0.25, 160
+    """Strict upwinding (Solver.cpp): both lattices decay as exp(-2 nu k^2 t).
+    nu = 0.05 over 320 steps decays the vortex to 0.58 of its amplitude, and
+    the decay exponent is asserted within 5 %: the scheme's own numerical
+    dissipation adds about 3 % (an effective 0.0015 on top of nu, measured),
+    while a dropped or halved viscous flux would miss by 50 % or more."""
+    N, dt, steps, nu = 24, 0.25, 320, 0.05
+    m, _, runs = _tgv_runs(0, N=N, steps=steps, dt=dt, nu=nu)
     k = 2 * np.pi / N
     decay = np.exp(-2 * nu * k * k * dt * steps)
-    u0 = cases.tgv_state(m, 0.01)[1]
+    assert decay < 0.6
+    st = cases.tgv_state(m, 0.01)
+    q0 = np.concatenate([st[1], st[2]])
     for kind in (orc.OR_D3Q15, orc.OR_D2Q9):
-        _, u, _, _ = runs[kind]
-        amp = np.dot(u, u0) / np.dot(u0, u0)
-        assert abs(amp / decay - 1.0) < 0.02, (kind, amp, decay)
+        _, u, v, _ = runs[kind]
+        amp = np.dot(np.concatenate([u, v]), q0) / np.dot(q0, q0)
+        ratio = np.log(amp) / np.log(decay)
+        assert 1.0 < ratio < 1.05, (kind, amp, decay, ratio)
 
 
 def test_empty_patches_carry_no_flux_and_project_onto_d2q9():
