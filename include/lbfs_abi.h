@@ -195,14 +195,32 @@ int lbfs_stage_end_step(lbfs_ctx *ctx);
 /* cell force density added to the momentum residual (Solver_gpu.cu:834-846, 1008-1010) */
 int lbfs_get_cell_force(lbfs_ctx *ctx, double *f /* [3 n_cells] */);
 
-/* diagnostics: residual RMS (residuals.cpp:114-132) of the last step and FP64 mass */
+/* diagnostics: residual RMS (square + total<128> + residuals::l2_norm_rms_moukallad,
+ * Solver_gpu.cu:1032-1046, residuals.cpp:114-132: sqrt(sum r^2 / n_cells) / ref_rho for
+ * rho, / (ref_rho ref_u) for the momentum rows, whose residual includes the force density
+ * as after add(), Solver_gpu.cu:1008-1010), FP64 mass and the largest speed; finite = 0
+ * and LBFS_E_NONFINITE when the state or a fixed-point sum of the coupling is not finite */
 typedef struct {
     double rho_rms, u_rms, v_rms, w_rms;
     double total_mass;
     double max_speed;
     int finite;
 } lbfs_diag;
+/* residual of the step that starts from the current state with the static force
+ * (body force); the state is unchanged (equals the next step's RMS without membranes) */
 int lbfs_reduce_diagnostics(lbfs_ctx *ctx, double ref_rho, double ref_u, lbfs_diag *out);
+/* one step (eager launches) and the residual RMS of that step as the reference
+ * computes it: flux residual of the state at its start plus the force density the
+ * step applied (static force and, with membranes, the force it spread) */
+int lbfs_step_residual(lbfs_ctx *ctx, double ref_rho, double ref_u, lbfs_diag *out);
+/* the time loop's exit test (Solver_gpu.cu:1206: max_error() < tolerance or time up;
+ * residuals::max_error residuals.cpp:158-166 = max of the u, v, w RMS): steps until
+ * a checked step's RMS is below tolerance or max_steps are done; every check_every-th
+ * step is a lbfs_step_residual step (1 = every step, the reference's test), the others
+ * run as lbfs_step.  *steps_done counts the steps taken, *converged is 1 when the
+ * tolerance was met, *out holds the last checked step's diagnostics. */
+int lbfs_run_to_tolerance(lbfs_ctx *ctx, int max_steps, double tolerance, int check_every, double ref_rho,
+                          double ref_u, int *steps_done, int *converged, lbfs_diag *out);
 
 /* ---- spring-network membranes and immersed-boundary coupling ----
  * One descriptor per membrane (lagrangian_object, lagrangian_object.h:17-141;

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "coupling.hpp"
+#include "fixed_point.hpp"
 #include "fluid_kernels.hpp"
 #include "lbfs_abi.h"
 #include "mesh_host.hpp"
@@ -36,6 +37,8 @@ struct lbfs_ctx {
     int cur = 0;
     float4 *grad = nullptr, *res = nullptr, *face_scratch = nullptr;
     double *diag_part = nullptr;
+    int *dev_err = nullptr;         // device error word: ERR_FIX_* bits (fixed_point.hpp), sticky until
+                                    // the next lbfs_add_membranes
     float *static_force = nullptr;  // [3 n] body force + closure term
     float *cell_force = nullptr;    // [3 n] static force + spread IBM force
     std::vector<void *> allocs;
@@ -85,7 +88,14 @@ struct lbfs_ctx {
         return p;
     }
 
+    // lbfs_step_residual: the face pass of an eager step also writes the flux
+    // residual (never set while a graph is captured)
+    bool res_track = false;
+    float4 *res_out() const { return res_track ? res : nullptr; }
+
     int fail(int code, const char *fmt, ...);
+    // waits for the stream and turns a set device error word into LBFS_E_NONFINITE
+    int check_device_error();
     int enqueue_step();
     int enqueue_steps(int nsteps);
     void release_extensions();

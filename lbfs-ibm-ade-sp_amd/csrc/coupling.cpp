@@ -193,6 +193,10 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     // IBM grids: one per logical box, bins (= spreading tiles) numbered box after box
     IbmArgs &g = ctx->ibm;
     memset(&g, 0, sizeof(g));
+    // the device error word (fixed-point range of the spread and membrane sums)
+    g.err = ctx->dev_err;
+    m.err = ctx->dev_err;
+    HIPCHK(hipMemsetAsync(ctx->dev_err, 0, sizeof(int), ctx->stream));
     g.kernel = ibm_kernel;
     const int nblk = (int)ctx->blocks.size();
     std::vector<IbmBlock> blk(nblk);
@@ -375,6 +379,8 @@ extern "C" int lbfs_get_membrane_nodes(lbfs_ctx *ctx, double *X, double *U, doub
 {
     if (!ctx) return LBFS_E_INVALID_ARG;
     if (!ctx->have_membranes) return ctx->fail(LBFS_E_STATE, "no membranes attached");
+    int rc = ctx->check_device_error();
+    if (rc) return rc;
     const int nn = ctx->mem.n_nodes;
     std::vector<float4> hx(nn), hu(nn), hf(nn);
     std::vector<int> nm(nn);

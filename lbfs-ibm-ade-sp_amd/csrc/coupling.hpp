@@ -43,6 +43,7 @@ struct MemArgs {
                               // (fixed point, MEM_SUM_SCALE), closed out by the cell update
     float4 *bbox;             // 2 per membrane: lower and upper corner of the nodes
     const int *mem_block;     // box of each membrane
+    int *err;                 // the context's device error word (ERR_FIX_* bits, fixed_point.hpp)
 };
 
 // one logical box of cells (an ensemble member, or the whole mesh)
@@ -84,6 +85,7 @@ struct IbmArgs {
     const int *cell_tile;     // [n_cells] fluid tile of each cell
     int *tile_now;            // [n_tiles] 1: the spreading wrote into the tile this step
     int *tile_prev;           // [n_tiles] 1: the cell force output holds spread force there
+    int *err;                 // the context's device error word (ERR_FIX_* bits, fixed_point.hpp)
 };
 
 // The end of the coupled step: adds the spread force density to the updated

@@ -362,9 +362,9 @@ __global__ __launch_bounds__(256) void k_adj_interp(MemArgs m, IbmArgs g, AdjMem
     const float w = wh * inv;
     if (ic >= 0 && w != 0.0f && scale > 0.0) {
         unsigned long long *acc = j.acc + 3 * (size_t)ic;
-        atomicAdd(acc, (unsigned long long)llrint((double)(w * lu.x) * scale));
-        atomicAdd(acc + 1, (unsigned long long)llrint((double)(w * lu.y) * scale));
-        atomicAdd(acc + 2, (unsigned long long)llrint((double)(w * lu.z) * scale));
+        atomicAdd(acc, fix_term((double)(w * lu.x), scale, g.err, ERR_FIX_ADJ));
+        atomicAdd(acc + 1, fix_term((double)(w * lu.y), scale, g.err, ERR_FIX_ADJ));
+        atomicAdd(acc + 2, fix_term((double)(w * lu.z), scale, g.err, ERR_FIX_ADJ));
     }
 }
 

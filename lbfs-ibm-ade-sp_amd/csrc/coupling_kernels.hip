@@ -47,7 +47,10 @@ __device__ __forceinline__ void mem_sum_add(unsigned long long *acc, int mem, co
     }
 }
 
-__device__ __forceinline__ long long mem_fx(double x) { return __double2ll_rn(x * MEM_SUM_SCALE); }
+__device__ __forceinline__ long long mem_fx(double x, int *err)
+{
+    return (long long)fix_term(x, MEM_SUM_SCALE, err, ERR_FIX_MEM);
+}
 
 // volume and area of triangle t (thread t) added to its membrane's sums
 __device__ __forceinline__ void triangle_sums(const MemArgs &m, int t)
@@ -58,8 +61,8 @@ __device__ __forceinline__ void triangle_sums(const MemArgs &m, int t)
         const int4 qq = m.tri[t];
         const f3 a = ld3(m.X[qq.x]), b = ld3(m.X[qq.y]), c = ld3(m.X[qq.z]);
         const f3 N = cross(b - a, c - a);
-        v[0] = mem_fx((double)dot(N, (a + b + c) * (1.0f / 3.0f)) / 6.0);
-        v[1] = mem_fx(0.5 * (double)sqrtf(dot(N, N)));
+        v[0] = mem_fx((double)dot(N, (a + b + c) * (1.0f / 3.0f)) / 6.0, m.err);
+        v[1] = mem_fx(0.5 * (double)sqrtf(dot(N, N)), m.err);
         tm = qq.w;
     }
     mem_sum_add(m.mem_acc, tm, v, 2);
@@ -213,9 +216,9 @@ __device__ __forceinline__ void spread_node_global(const MemArgs &m, const IbmAr
     if (!(wrap_index(B, 0, idx[0], w[0]) && wrap_index(B, 1, idx[1], w[1]) && wrap_index(B, 2, idx[2], w[2]))) return;
     const int ic = cell_internal(g, B, w);
     unsigned long long *acc = g.acc + 3 * (size_t)ic;
-    atomicAdd(acc, (unsigned long long)llrint((double)(F.x * wt) * FIX_SCALE));
-    atomicAdd(acc + 1, (unsigned long long)llrint((double)(F.y * wt) * FIX_SCALE));
-    atomicAdd(acc + 2, (unsigned long long)llrint((double)(F.z * wt) * FIX_SCALE));
+    atomicAdd(acc, fix_term((double)(F.x * wt), FIX_SCALE, g.err, ERR_FIX_IBM));
+    atomicAdd(acc + 1, fix_term((double)(F.y * wt), FIX_SCALE, g.err, ERR_FIX_IBM));
+    atomicAdd(acc + 2, fix_term((double)(F.z * wt), FIX_SCALE, g.err, ERR_FIX_IBM));
     g.tile_now[g.cell_tile[ic]] = 1;
 }
 
@@ -301,9 +304,9 @@ __global__ __launch_bounds__(NT) void k_ibm_spread(MemArgs m, IbmArgs g)
                 }
                 if (!in) continue;
                 const float4 F = cF[jn];
-                atomicAdd(&racc[0][loc], (unsigned long long)llrint((double)(F.x * wt) * FIX_SCALE));
-                atomicAdd(&racc[1][loc], (unsigned long long)llrint((double)(F.y * wt) * FIX_SCALE));
-                atomicAdd(&racc[2][loc], (unsigned long long)llrint((double)(F.z * wt) * FIX_SCALE));
+                atomicAdd(&racc[0][loc], fix_term((double)(F.x * wt), FIX_SCALE, g.err, ERR_FIX_IBM));
+                atomicAdd(&racc[1][loc], fix_term((double)(F.y * wt), FIX_SCALE, g.err, ERR_FIX_IBM));
+                atomicAdd(&racc[2][loc], fix_term((double)(F.z * wt), FIX_SCALE, g.err, ERR_FIX_IBM));
             }
         }
         __syncthreads();
@@ -479,7 +482,7 @@ __global__ __launch_bounds__(128) void k_mem_edges(MemArgs m)
             m.edge_force[4 * (size_t)e + j] = make_float4(out.F[j].x, out.F[j].y, out.F[j].z, 0.0f);
         if (e - m.mem_edge_start[mem] < p.n_mad) {
             em = mem;
-            mv[0] = mem_fx((double)out.kappa * ((double)out.n1 + (double)out.n2) / 3.0);
+            mv[0] = mem_fx((double)out.kappa * ((double)out.n1 + (double)out.n2) / 3.0, m.err);
         }
     }
     // the MAD sum goes to the third slot

@@ -263,9 +263,11 @@ __global__ __launch_bounds__(256) void k_dt_min(FluidArgs a, const float4 *__res
 // ---------------------------------------------------------------------------
 // FP64 diagnostics: per-block partial sums (fixed order), summed on the host
 // ---------------------------------------------------------------------------
+// (force: the momentum force density added to the residual, 3 per cell, as
+// the reference adds force_x/y/z before squaring, Solver_gpu.cu:1008-1046)
 __global__ __launch_bounds__(256) void k_diag_partials(int n, float rho0, const float4 *__restrict__ st,
-                                                       const float4 *__restrict__ res, const float *__restrict__ inv_vol,
-                                                       double *__restrict__ part)
+                                                       const float4 *__restrict__ res, const float *__restrict__ force,
+                                                       const float *__restrict__ inv_vol, double *__restrict__ part)
 {
     __shared__ double sh[6][256];
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -273,10 +275,16 @@ __global__ __launch_bounds__(256) void k_diag_partials(int n, float rho0, const 
     if (i < n) {
         const float4 s = st[i];
         const float4 r = res[i];
+        double r1 = r.y, r2 = r.z, r3 = r.w;
+        if (force) {
+            r1 += (double)force[3 * (size_t)i];
+            r2 += (double)force[3 * (size_t)i + 1];
+            r3 += (double)force[3 * (size_t)i + 2];
+        }
         v[0] = (double)r.x * r.x;
-        v[1] = (double)r.y * r.y;
-        v[2] = (double)r.z * r.z;
-        v[3] = (double)r.w * r.w;
+        v[1] = r1 * r1;
+        v[2] = r2 * r2;
+        v[3] = r3 * r3;
         v[4] = ((double)rho0 + (double)s.x) / (double)inv_vol[i];
         const double sp = sqrt((double)s.y * s.y + (double)s.z * s.z + (double)s.w * s.w);
         v[5] = (isfinite(s.x) && isfinite(s.y) && isfinite(s.z) && isfinite(s.w)) ? sp : INFINITY;
@@ -362,11 +370,12 @@ hipError_t launch_dt_min(const FluidArgs &a, const float4 *st, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_diag_partials(int n, float rho0, const float4 *st, const float4 *res, const float *inv_vol, double *part,
+hipError_t launch_diag_partials(int n, float rho0, const float4 *st, const float4 *res, const float *force,
+                                const float *inv_vol, double *part,
                                 hipStream_t s)
 {
     const int blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(k_diag_partials, dim3(blocks), dim3(256), 0, s, n, rho0, st, res, inv_vol, part);
+    hipLaunchKernelGGL(k_diag_partials, dim3(blocks), dim3(256), 0, s, n, rho0, st, res, force, inv_vol, part);
     return hipGetLastError();
 }
 

@@ -44,7 +44,7 @@ hipError_t launch_face_flux_all(const FluidArgs &a, const float4 *st, const floa
                                 int n_tile_faces, hipStream_t s);
 // global minimum of the local steps into *a.dt_min (LBFS_TS_MIN)
 hipError_t launch_dt_min(const FluidArgs &a, const float4 *st, hipStream_t s);
-hipError_t launch_diag_partials(int n, float rho0, const float4 *st, const float4 *res, const float *inv_vol,
-                                double *part, hipStream_t s);
+hipError_t launch_diag_partials(int n, float rho0, const float4 *st, const float4 *res, const float *force,
+                                const float *inv_vol, double *part, hipStream_t s);
 
 }  // namespace lbfs

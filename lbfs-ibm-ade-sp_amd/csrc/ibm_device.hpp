@@ -6,12 +6,25 @@
 #include <hip/hip_runtime.h>
 
 #include "coupling.hpp"
+#include "fixed_point.hpp"
 
 namespace lbfs {
 namespace {
 
 constexpr double FIX_SCALE = 1099511627776.0;  // 2^40
 constexpr float PI_F = 3.14159265358979f;
+
+// 64-bit fixed-point term of v at `scale`, rounded to nearest; a term that is
+// not finite or beyond the range fix_in_range allows is replaced by 0 and
+// flagged in the error word, so an unstable run reports an error instead of
+// wrapping the integer sums
+__device__ __forceinline__ unsigned long long fix_term(double v, double scale, int *err, int bit)
+{
+    const double s = v * scale;
+    if (fix_in_range(s)) return (unsigned long long)__double2ll_rn(s);
+    atomicOr(err, bit);
+    return 0ull;
+}
 
 struct f3 {
     float x, y, z;

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -197,6 +198,8 @@ static int fluid_setup(lbfs_ctx *ctx, const lbfs_mesh_desc &m, const lbfs_params
     if ((rc = dev_upload<float4>(ctx, &ctx->res, nullptr, n))) return rc;
     if ((rc = dev_upload<float4>(ctx, &ctx->face_scratch, nullptr, ntf))) return rc;
     if ((rc = dev_upload<double>(ctx, &ctx->diag_part, nullptr, 6 * (size_t)((n + 255) / 256)))) return rc;
+    if ((rc = dev_upload<int>(ctx, &ctx->dev_err, nullptr, 1))) return rc;
+    HIPCHK(hipMemsetAsync(ctx->dev_err, 0, sizeof(int), ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->st[0], 0, sizeof(float4) * n, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->st[1], 0, sizeof(float4) * n, ctx->stream));
     ctx->cur = 0;
@@ -272,11 +275,27 @@ extern "C" int lbfs_destroy(lbfs_ctx *ctx)
     return LBFS_OK;
 }
 
+int lbfs_ctx::check_device_error()
+{
+    lbfs_ctx *ctx = this;
+    int h = 0;
+    HIPCHK(hipMemcpyAsync(&h, dev_err, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    if (h & ERR_FIX_IBM)
+        return fail(LBFS_E_NONFINITE, "IBM spreading: a node force term is not finite or out of the fixed-point "
+                                      "range (|F w| >= 4096)");
+    if (h & ERR_FIX_MEM)
+        return fail(LBFS_E_NONFINITE, "membrane sums: a volume, area or curvature term is not finite or out of "
+                                      "the fixed-point range");
+    if (h & ERR_FIX_ADJ) return fail(LBFS_E_NONFINITE, "adjoint interpolation: a term is not finite");
+    return LBFS_OK;
+}
+
 extern "C" int lbfs_sync(lbfs_ctx *ctx)
 {
     if (!ctx) return LBFS_E_INVALID_ARG;
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    return LBFS_OK;
+    return ctx->check_device_error();
 }
 
 extern "C" int lbfs_set_state(lbfs_ctx *ctx, const double *rho, const double *u, const double *v, const double *w)
@@ -491,13 +510,14 @@ extern "C" int lbfs_get_cell_force(lbfs_ctx *ctx, double *f_out)
     return LBFS_OK;
 }
 
-extern "C" int lbfs_reduce_diagnostics(lbfs_ctx *ctx, double ref_rho, double ref_u, lbfs_diag *out)
+// FP64 partial sums of k_diag_partials over the state `st`, the flux residual
+// in ctx->res and the momentum force density `force`, closed out on the host
+// in block order (residuals::l2_norm_rms_moukallad, residuals.cpp:114-132)
+static int finish_diagnostics(lbfs_ctx *ctx, const float4 *st, const float *force, double ref_rho, double ref_u,
+                              lbfs_diag *out)
 {
-    if (!ctx || !out || !(ref_rho > 0.0) || !(ref_u > 0.0)) return LBFS_E_INVALID_ARG;
     const int n = ctx->lay.n_cells, nb = (n + 255) / 256;
-    HIPCHK(launch_gradients(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->stream));
-    HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, nullptr, ctx->res, 0, ctx->stream));
-    HIPCHK(launch_diag_partials(n, ctx->args.rho0, ctx->st[ctx->cur], ctx->res, ctx->args.inv_vol, ctx->diag_part,
+    HIPCHK(launch_diag_partials(n, ctx->args.rho0, st, ctx->res, force, ctx->args.inv_vol, ctx->diag_part,
                                 ctx->stream));
     std::vector<double> h(6 * (size_t)nb);
     HIPCHK(hipMemcpyAsync(h.data(), ctx->diag_part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
@@ -514,7 +534,65 @@ extern "C" int lbfs_reduce_diagnostics(lbfs_ctx *ctx, double ref_rho, double ref
     out->total_mass = s[4];
     out->max_speed = mx;
     out->finite = std::isfinite(mx) && std::isfinite(s[0]) && std::isfinite(s[1]) ? 1 : 0;
+    int rc = ctx->check_device_error();
+    if (rc) {
+        out->finite = 0;
+        return rc;
+    }
     return out->finite ? LBFS_OK : ctx->fail(LBFS_E_NONFINITE, "non-finite fluid state");
+}
+
+extern "C" int lbfs_reduce_diagnostics(lbfs_ctx *ctx, double ref_rho, double ref_u, lbfs_diag *out)
+{
+    if (!ctx || !out || !(ref_rho > 0.0) || !(ref_u > 0.0)) return LBFS_E_INVALID_ARG;
+    // residual of the step that starts from the current state, with the
+    // static force (body force) added to the momentum rows
+    HIPCHK(launch_gradients(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->stream));
+    HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, nullptr, ctx->res, 0, ctx->stream));
+    return finish_diagnostics(ctx, ctx->st[ctx->cur], ctx->static_force, ref_rho, ref_u, out);
+}
+
+extern "C" int lbfs_step_residual(lbfs_ctx *ctx, double ref_rho, double ref_u, lbfs_diag *out)
+{
+    if (!ctx || !out || !(ref_rho > 0.0) || !(ref_u > 0.0)) return LBFS_E_INVALID_ARG;
+    // one eager step whose face pass also writes the flux residual; the
+    // step's force density is the static force plus, with membranes, the
+    // force the step spread (the cell force output of the apply pass)
+    ctx->res_track = true;
+    int rc = ctx->enqueue_step();
+    ctx->res_track = false;
+    if (rc) return rc;
+    const float *force = ctx->have_membranes ? ctx->cell_force : ctx->static_force;
+    return finish_diagnostics(ctx, ctx->st[ctx->cur], force, ref_rho, ref_u, out);
+}
+
+extern "C" int lbfs_run_to_tolerance(lbfs_ctx *ctx, int max_steps, double tolerance, int check_every, double ref_rho,
+                                     double ref_u, int *steps_done, int *converged, lbfs_diag *out)
+{
+    if (!ctx || max_steps < 0 || check_every < 1 || !(ref_rho > 0.0) || !(ref_u > 0.0) || !steps_done ||
+        !converged || !out)
+        return LBFS_E_INVALID_ARG;
+    *steps_done = 0;
+    *converged = 0;
+    memset(out, 0, sizeof(*out));
+    int done = 0;
+    while (done < max_steps) {
+        const int chunk = std::min(check_every, max_steps - done);
+        int rc = chunk > 1 ? ctx->enqueue_steps(chunk - 1) : LBFS_OK;
+        if (!rc) rc = lbfs_step_residual(ctx, ref_rho, ref_u, out);
+        if (rc) {
+            *steps_done = done;
+            return rc;
+        }
+        done += chunk;
+        // residuals::max_error (residuals.cpp:158-166): the momentum rows only
+        if (std::fmax(std::fmax(out->w_rms, out->u_rms), out->v_rms) < tolerance) {
+            *converged = 1;
+            break;
+        }
+    }
+    *steps_done = done;
+    return LBFS_OK;
 }
 
 // ---------------------------------------------------------------------------

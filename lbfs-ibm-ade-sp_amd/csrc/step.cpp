@@ -84,7 +84,7 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
         const double touched = std::min(n, 343.0 * mem.n_nodes / 16.0);
         s.push_back({"flux_update", [ctx]() -> int {
                          HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->st[ctx->cur ^ 1],
-                                                   nullptr, 1, ctx->stream));
+                                                   ctx->res_out(), 1, ctx->stream));
                          return LBFS_OK;
                      },
                      108.0 * n + 40.0 * nf});
@@ -100,8 +100,8 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
     // pass F: state 16 + gradients 48 + volume/centroid 16 + force 12 + state written 16 per cell,
     // plus owner/nb 8, normal 12, face centroid 12, area 4, dt 4 per face
     s.push_back({"flux_update", [ctx]() -> int {
-                     HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->st[ctx->cur ^ 1], nullptr,
-                                               1, ctx->stream));
+                     HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->st[ctx->cur ^ 1],
+                                               ctx->res_out(), 1, ctx->stream));
                      ctx->cur ^= 1;
                      return LBFS_OK;
                  },
@@ -141,7 +141,7 @@ int lbfs_ctx::run_stages(bool with_visc)
         if (interp_first) HIPCHK(launch_ibm_interp(mem, ibm, st[cur], stream));
         HIPCHK(hipEventRecord(ev_fork, stream));
         HIPCHK(launch_gradients(args, st[cur], grad, stream));
-        HIPCHK(launch_flux_update(args, st[cur], grad, st[cur ^ 1], nullptr, 1, stream));
+        HIPCHK(launch_flux_update(args, st[cur], grad, st[cur ^ 1], res_out(), 1, stream));
         HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
         if (!interp_first) HIPCHK(launch_ibm_interp(mem, ibm, st[cur], side));
         HIPCHK(launch_mem_volume(mem, side));
@@ -157,7 +157,7 @@ int lbfs_ctx::run_stages(bool with_visc)
     HIPCHK(hipEventRecord(ev_fork, stream));
     HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
     HIPCHK(launch_gradients(args, st[cur], grad, side));
-    HIPCHK(launch_flux_update(args, st[cur], grad, st[cur ^ 1], nullptr, 1, side));
+    HIPCHK(launch_flux_update(args, st[cur], grad, st[cur ^ 1], res_out(), 1, side));
     HIPCHK(hipEventRecord(ev_join, side));
     HIPCHK(launch_ibm_interp(mem, ibm, st[cur], stream));
     HIPCHK(hipEventRecord(ev_fork2, stream));
@@ -181,49 +181,71 @@ int lbfs_ctx::enqueue_step()
     return rc;
 }
 
+// HIP events destroyed on every exit path
+struct EventSet {
+    std::vector<hipEvent_t> ev;
+    ~EventSet()
+    {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    hipError_t create(size_t n)
+    {
+        ev.assign(n, nullptr);
+        for (hipEvent_t &e : ev) {
+            hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+        }
+        return hipSuccess;
+    }
+};
+
 extern "C" int lbfs_time_steps(lbfs_ctx *ctx, int nsteps, double *ms_total)
 {
     if (!ctx || nsteps <= 0 || !ms_total) return LBFS_E_INVALID_ARG;
-    hipEvent_t a, b;
-    HIPCHK(hipEventCreate(&a));
-    HIPCHK(hipEventCreate(&b));
-    HIPCHK(hipEventRecord(a, ctx->stream));
+    EventSet E;
+    HIPCHK(E.create(2));
+    HIPCHK(hipEventRecord(E.ev[0], ctx->stream));
     int rc = ctx->enqueue_steps(nsteps);
     if (rc) return rc;
-    HIPCHK(hipEventRecord(b, ctx->stream));
-    HIPCHK(hipEventSynchronize(b));
+    HIPCHK(hipEventRecord(E.ev[1], ctx->stream));
+    HIPCHK(hipEventSynchronize(E.ev[1]));
     float ms = 0.0f;
-    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    HIPCHK(hipEventElapsedTime(&ms, E.ev[0], E.ev[1]));
     *ms_total = ms;
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
     return LBFS_OK;
 }
 
 extern "C" int lbfs_time_kernels(lbfs_ctx *ctx, int nsteps, int max_count, const char **names, double *ms, int *count)
 {
     if (!ctx || nsteps <= 0 || !ms || !count || max_count <= 0) return LBFS_E_INVALID_ARG;
+    // whole steps, one stage at a time with an event between stages (the
+    // stage list of a step that does not rebuild the internal viscosity
+    // field; such steps are refused so the schedule stays that of lbfs_step)
+    for (int s = 0; s < nsteps; s++)
+        if (ctx->visc_due(ctx->t_step + s))
+            return ctx->fail(LBFS_E_STATE, "a timed step would rebuild the internal viscosity field");
     std::vector<lbfs_ctx::Stage> stages = ctx->step_stages();
     const int ns = (int)stages.size();
-    std::vector<hipEvent_t> ev((size_t)nsteps * (ns + 1));
-    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    EventSet E;
+    HIPCHK(E.create((size_t)nsteps * (ns + 1)));
     for (int s = 0; s < nsteps; s++) {
         for (int k = 0; k < ns; k++) {
-            HIPCHK(hipEventRecord(ev[(size_t)s * (ns + 1) + k], ctx->stream));
+            HIPCHK(hipEventRecord(E.ev[(size_t)s * (ns + 1) + k], ctx->stream));
             int rc = stages[k].run();
             if (rc) return rc;
         }
-        HIPCHK(hipEventRecord(ev[(size_t)s * (ns + 1) + ns], ctx->stream));
+        HIPCHK(hipEventRecord(E.ev[(size_t)s * (ns + 1) + ns], ctx->stream));
+        ctx->t_step++;  // a whole step was taken
     }
-    HIPCHK(hipEventSynchronize(ev.back()));
+    HIPCHK(hipEventSynchronize(E.ev.back()));
     std::vector<double> acc(ns, 0.0);
     for (int s = 0; s < nsteps; s++)
         for (int k = 0; k < ns; k++) {
             float t = 0.0f;
-            HIPCHK(hipEventElapsedTime(&t, ev[(size_t)s * (ns + 1) + k], ev[(size_t)s * (ns + 1) + k + 1]));
+            HIPCHK(hipEventElapsedTime(&t, E.ev[(size_t)s * (ns + 1) + k], E.ev[(size_t)s * (ns + 1) + k + 1]));
             acc[k] += t;
         }
-    for (auto &e : ev) (void)hipEventDestroy(e);
     *count = ns < max_count ? ns : max_count;
     for (int k = 0; k < *count; k++) {
         ms[k] = acc[k] / nsteps;
@@ -244,23 +266,20 @@ extern "C" int lbfs_time_stage(lbfs_ctx *ctx, const char *name, int n, double *m
     for (auto &s : stages)
         if (std::strcmp(s.name, name) == 0) st = &s;
     if (!st) return ctx->fail(LBFS_E_INVALID_ARG, "no stage %s", name);
-    hipEvent_t a, b;
-    HIPCHK(hipEventCreate(&a));
-    HIPCHK(hipEventCreate(&b));
+    EventSet E;
+    HIPCHK(E.create(2));
     const int cur = ctx->cur;
-    HIPCHK(hipEventRecord(a, ctx->stream));
+    HIPCHK(hipEventRecord(E.ev[0], ctx->stream));
     for (int k = 0; k < n; k++) {
         int rc = st->run();
         ctx->cur = cur;
         if (rc) return rc;
     }
-    HIPCHK(hipEventRecord(b, ctx->stream));
-    HIPCHK(hipEventSynchronize(b));
+    HIPCHK(hipEventRecord(E.ev[1], ctx->stream));
+    HIPCHK(hipEventSynchronize(E.ev[1]));
     float ms = 0.0f;
-    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    HIPCHK(hipEventElapsedTime(&ms, E.ev[0], E.ev[1]));
     *ms_avg = ms / n;
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
     return LBFS_OK;
 }
 

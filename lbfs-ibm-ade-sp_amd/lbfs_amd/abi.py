@@ -12,6 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblbfs_amd.so")
 
 D3Q15, D3Q19, D2Q9 = 0, 1, 2
+E_INVALID_ARG, E_HIP, E_NONFINITE, E_UNSUPPORTED, E_NO_DEVICE, E_STATE = -1, -2, -3, -4, -5, -6
 GRAD_HYBRID, GRAD_LS, GRAD_GG = 0, 1, 2
 BC_DIRICHLET, BC_NEUMANN, BC_PARABOLIC, BC_SHEAR, BC_EMPTY = 1, 2, 9, 10, 11
 TS_LOCAL, TS_CONSTANT, TS_MIN = 1, 2, 3
@@ -131,6 +132,9 @@ def load():
             "lbfs_stage_face_flux": (C.c_int, [vp, _dp]),
             "lbfs_stage_residual": (C.c_int, [vp, _dp]),
             "lbfs_reduce_diagnostics": (C.c_int, [vp, C.c_double, C.c_double, C.POINTER(Diag)]),
+            "lbfs_step_residual": (C.c_int, [vp, C.c_double, C.c_double, C.POINTER(Diag)]),
+            "lbfs_run_to_tolerance": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, _ip, _ip,
+                                                C.POINTER(Diag)]),
             "lbfs_get_cell_force": (C.c_int, [vp, _dp]),
             "lbfs_set_option": (C.c_int, [vp, C.c_int, C.c_int]),
             "lbfs_add_membranes": (C.c_int, [vp, C.POINTER(MembraneDesc), C.c_int, C.c_int]),
@@ -187,13 +191,17 @@ def ptr(a, kind=_dp):
 
 
 class LbfsError(RuntimeError):
-    pass
+    """A failed ABI call; `code` is its LBFS_E_* status."""
+
+    def __init__(self, msg, code=0):
+        super().__init__(msg)
+        self.code = code
 
 
 def check(rc, ctx=None):
     if rc != 0:
         msg = load().lbfs_last_error(ctx)
-        raise LbfsError("lbfs call failed (%d): %s" % (rc, msg.decode() if msg else ""))
+        raise LbfsError("lbfs call failed (%d): %s" % (rc, msg.decode() if msg else ""), rc)
 
 
 PATCH_NAMES = ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")
@@ -766,6 +774,23 @@ class Solver:
         return loss.value, gb, gm.reshape(-1, 3)
 
     def diagnostics(self, ref_rho=1.0, ref_u=0.01):
+        """Residual RMS of the step that would start from the current state
+        (static force only), FP64 mass, largest speed."""
         d = Diag()
         check(load().lbfs_reduce_diagnostics(self.ctx, ref_rho, ref_u, C.byref(d)), self.ctx)
         return {k: getattr(d, k) for k, _ in Diag._fields_}
+
+    def step_residual(self, ref_rho=1.0, ref_u=0.01):
+        """One step; the residual RMS of that step (residuals.cpp:114-132)."""
+        d = Diag()
+        check(load().lbfs_step_residual(self.ctx, ref_rho, ref_u, C.byref(d)), self.ctx)
+        return {k: getattr(d, k) for k, _ in Diag._fields_}
+
+    def run_to_tolerance(self, max_steps, tolerance, check_every=1, ref_rho=1.0, ref_u=0.01):
+        """Steps until a checked step's max(u, v, w) residual RMS is below
+        tolerance (Solver_gpu.cu:1206); returns (steps, converged, diagnostics)."""
+        d = Diag()
+        done, conv = C.c_int(), C.c_int()
+        check(load().lbfs_run_to_tolerance(self.ctx, int(max_steps), float(tolerance), int(check_every), ref_rho,
+                                           ref_u, C.byref(done), C.byref(conv), C.byref(d)), self.ctx)
+        return done.value, bool(conv.value), {k: getattr(d, k) for k, _ in Diag._fields_}

@@ -235,6 +235,12 @@ class OracleFluid:
         load().or_residual(C.byref(self.m), ptr(self.flux), ptr(self.res))
         return self.res.copy()
 
+    def residual_rms(self, ref_rho=1.0, ref_u=0.01):
+        """or_residual_rms of the residual of the last step (force included)."""
+        out = np.zeros(4)
+        load().or_residual_rms(self.nc, ptr(self.res), ref_rho, ref_u, ptr(out))
+        return out
+
     def step(self, n=1, force=None):
         lib = load()
         fp = None
