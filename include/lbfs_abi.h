@@ -264,6 +264,16 @@ int lbfs_get_membrane_globals(lbfs_ctx *ctx, double *out6);
  * local bending, global area, volume, ADE, total (Solver_gpu.cu:1060-1139,
  * spring_network.cu:1687-1768); evaluated from the last force call's state */
 int lbfs_get_membrane_energies(lbfs_ctx *ctx, double *out /* [7 n_membranes] */);
+/* The same energies, bit for bit, staged in a fixed device array instead of
+ * memory the context allocates: this call allocates nothing (no hipMalloc, no
+ * hipFree, no allocating library routine).  No counterpart in the reference;
+ * it is for callers that cannot allocate (serving loops, code that must not
+ * call hipMalloc).  At most LBFS_NOALLOC_MAX_MEMBRANES membranes (else
+ * LBFS_E_INVALID_ARG); the staging is 7 * 8192 doubles (448 KiB) per device,
+ * shared by every context on that device and held until the process ends, so
+ * calls on one device from different threads or overlapping streams race. */
+#define LBFS_NOALLOC_MAX_MEMBRANES 8192
+int lbfs_get_membrane_energies_noalloc(lbfs_ctx *ctx, double *out /* [7 n_membranes] */);
 
 /* ---- parameters the adjoint differentiates ----
  * Uniform body force (the reference's pressure gradient along its direction,

@@ -423,6 +423,25 @@ extern "C" int lbfs_get_membrane_energies(lbfs_ctx *ctx, double *out7)
     return LBFS_OK;
 }
 
+static_assert(NOALLOC_MAX_MEMBRANES == LBFS_NOALLOC_MAX_MEMBRANES, "staging size of the ABI header");
+
+extern "C" int lbfs_get_membrane_energies_noalloc(lbfs_ctx *ctx, double *out7)
+{
+    if (!ctx || !out7) return LBFS_E_INVALID_ARG;
+    if (!ctx->have_membranes) return ctx->fail(LBFS_E_STATE, "no membranes attached");
+    const int nm = ctx->mem.n_mem;
+    if (nm > NOALLOC_MAX_MEMBRANES)
+        return ctx->fail(LBFS_E_INVALID_ARG, "lbfs_get_membrane_energies_noalloc: %d membranes, at most %d", nm,
+                         NOALLOC_MAX_MEMBRANES);
+    if (!ctx->energy_scratch) return ctx->fail(LBFS_E_STATE, "no energy staging address on this device");
+    // every entry read back is written by this launch first
+    HIPCHK(launch_mem_energy(ctx->mem, ctx->energy_scratch, ctx->stream));
+    HIPCHK(hipMemcpyAsync(out7, ctx->energy_scratch, sizeof(double) * 7 * (size_t)nm, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return LBFS_OK;
+}
+
 extern "C" int lbfs_mesh_icosphere(int frequency, double *nodes, int *tri)
 {
     if (frequency < 1 || frequency > 64 || !nodes || !tri) return LBFS_E_INVALID_ARG;

@@ -121,5 +121,9 @@ hipError_t launch_ibm_apply(const IbmArgs &g, const ApplyArgs &p, hipStream_t s)
 hipError_t launch_internal_visc(const MemArgs &m, const IbmArgs &g, float delta_h, float *visc_factor, hipStream_t s);
 // output-step energies, 7 doubles per membrane (Solver_gpu.cu:1060-1139)
 hipError_t launch_mem_energy(const MemArgs &m, double *out, hipStream_t s);
+// device address of the static energy staging of the current device (taken
+// once per device with that device current); NOALLOC_MAX_MEMBRANES membranes
+constexpr int NOALLOC_MAX_MEMBRANES = 8192;
+hipError_t energy_scratch_setup(int device, double **out);
 
 }  // namespace lbfs

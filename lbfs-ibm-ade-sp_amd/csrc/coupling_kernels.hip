@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "coupling.hpp"
 #include "ibm_device.hpp"
@@ -691,6 +692,32 @@ hipError_t launch_mem_energy(const MemArgs &m, double *out, hipStream_t s)
 {
     hipLaunchKernelGGL(k_mem_energy, dim3(m.n_mem), dim3(256), 0, s, m, out);
     return hipGetLastError();
+}
+
+// Staging of lbfs_get_membrane_energies_noalloc: 7 FP64 energies per
+// membrane, at most NOALLOC_MAX_MEMBRANES membranes (448 KiB per device, held
+// until the process ends).  Host code never passes `scratch` itself (its host
+// name is not a device address): energy_scratch_setup takes each device's
+// address once, with that device current, and the kernels get that pointer.
+static __device__ __align__(256) double scratch[7 * NOALLOC_MAX_MEMBRANES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+static double *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+hipError_t energy_scratch_setup(int device, double **out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
 }
 
 hipError_t launch_internal_visc(const MemArgs &m, const IbmArgs &g, float delta_h, float *visc_factor, hipStream_t s)

@@ -245,6 +245,13 @@ extern "C" int lbfs_create(const lbfs_mesh_desc *mesh, const lbfs_params *params
         lbfs_destroy(ctx);
         return rc;
     }
+    // the static staging of lbfs_get_membrane_energies_noalloc on this device
+    const hipError_t se = energy_scratch_setup(ctx->device, &ctx->energy_scratch);
+    if (se != hipSuccess) {
+        g_create_error = std::string("hipGetSymbolAddress of the energy staging: ") + hipGetErrorString(se);
+        lbfs_destroy(ctx);
+        return LBFS_E_HIP;
+    }
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
         g_create_error = "stream synchronisation failed during set-up";
         lbfs_destroy(ctx);

@@ -141,6 +141,7 @@ def load():
             "lbfs_get_membrane_nodes": (C.c_int, [vp, _dp, _dp, _dp]),
             "lbfs_get_membrane_globals": (C.c_int, [vp, _dp]),
             "lbfs_get_membrane_energies": (C.c_int, [vp, _dp]),
+            "lbfs_get_membrane_energies_noalloc": (C.c_int, [vp, _dp]),
             "lbfs_mesh_icosphere": (C.c_int, [C.c_int, _dp, _ip]),
             "lbfs_mesh_evans_fung": (C.c_int, [C.c_int, _dp, C.c_double, _dp]),
             "lbfs_mesh_springs": (C.c_int, [C.c_int, C.c_int, _ip, _ip, _ip]),
@@ -698,6 +699,13 @@ class Solver:
         volume, ADE, total."""
         out = np.zeros(7 * len(self._membranes))
         check(load().lbfs_get_membrane_energies(self.ctx, ptr(out)), self.ctx)
+        return out.reshape(-1, 7)
+
+    def membrane_energies_noalloc(self):
+        """The same through lbfs_get_membrane_energies_noalloc (static device
+        staging; allocates nothing)."""
+        out = np.zeros(7 * len(self._membranes))
+        check(load().lbfs_get_membrane_energies_noalloc(self.ctx, ptr(out)), self.ctx)
         return out.reshape(-1, 7)
 
     def cell_force(self):
