@@ -479,3 +479,60 @@ def test_lagrange4_reproduces_cubics_on_stretched_grid():
     orc.load().or_ibm_spread(abi.IBM_LAGRANGE4, C.byref(cp.g), len(X), orc.ptr(np.ascontiguousarray(X.reshape(-1))),
                              orc.ptr(np.ascontiguousarray(F.reshape(-1))), orc.ptr(f))
     np.testing.assert_allclose((f.reshape(-1, 3) * fl.vol[:, None]).sum(0), F.sum(0), atol=1e-12)
+
+
+def test_mad_compat_sums_the_first_n_nodes_springs():
+    """mad_compat (Solver_gpu.cu:2355: the MAD block sum runs over the first
+    n_nodes entries of the per-spring array): MAD = 2 sum_{e < n_nodes}
+    theta_e L_e / A, against the full sum 2 sum_e theta_e L_e / A without it."""
+    x0, t = small_rbc(freq=3, d0=6.0)
+    X = x0 + 0.03 * np.random.default_rng(13).standard_normal(x0.shape)
+    params = dict(NO_BEND, global_bending_modulus=1e-3)
+    full = orc.OracleMembrane(X, t, params)
+    part = orc.OracleMembrane(X, t, dict(params, mad_compat=1))
+    sp = full.spring.reshape(-1, 4)
+    n = full.n
+    assert n < len(sp)
+    tl = _theta_length(X, sp)
+    A = _surface_area(X, t)
+    assert abs(full.globals()[5] - 2.0 * np.sum(tl) / A) < 1e-12 * abs(full.globals()[5])
+    assert abs(part.globals()[5] - 2.0 * np.sum(tl[:n]) / A) < 1e-12 * abs(part.globals()[5])
+    assert abs(part.globals()[5] - full.globals()[5]) > 1e-3 * abs(full.globals()[5])
+
+
+def test_sphere_reference_bending_uses_the_spherical_curvature():
+    """sphere_reference (spring_network.cu:493-495): every spring's
+    spontaneous curvature is curvature_ratio / 3.265483301, so the curvature
+    and local bending terms are the frozen-coefficient gradients with that
+    constant kappa0 (kappa = theta L / ((|N1| + |N2|) / 6) computed here)."""
+    x0, t = small_rbc(freq=3, d0=6.0)
+    k, kb, ratio = 2e-3, 3e-3, 1.7
+    params = dict(NO_BEND, global_bending_modulus=k, local_bending_modulus=kb, shear_modulus=0.0,
+                  volume_modulus=0.0, local_area_modulus=0.0, global_area_modulus=0.0,
+                  sphere_reference=1, curvature_ratio=ratio)
+    m = orc.OracleMembrane(x0, t, params)
+    kappa0 = ratio / 3.265483301
+    sp = m.spring.reshape(-1, 4)
+    X = x0 + 0.02 * np.random.default_rng(14).standard_normal(x0.shape)
+    F = m.forces(X).reshape(-1)
+    n1, n2 = _spring_norms(X, sp)
+    kappa = _theta_length(X, sp) / ((n1 + n2) / 6.0)
+    c = k * (kappa - kappa0)
+    w = 0.5 * kb * (kappa0 - kappa) ** 2
+
+    def G(xf):
+        Xr = xf.reshape(-1, 3)
+        a1, a2 = _spring_norms(Xr, sp)
+        return np.sum(c * _theta_length(Xr, sp)) + np.sum(w * (a1 + a2) / 6.0)
+
+    Xf = X.reshape(-1).copy()
+    h = 1e-6
+    for j in np.random.default_rng(15).choice(len(Xf), 25, replace=False):
+        xp, xm = Xf.copy(), Xf.copy()
+        xp[j] += h
+        xm[j] -= h
+        g = (G(xp) - G(xm)) / (2 * h)
+        assert abs(-g - F[j]) < 1e-10 + 1e-5 * abs(F[j]), (j, -g, F[j])
+    # without it the spontaneous curvature is zero and the force differs
+    m0 = orc.OracleMembrane(x0, t, dict(params, sphere_reference=0))
+    assert np.max(np.abs(m0.forces(X).reshape(-1) - F)) > 1e-3 * np.max(np.abs(F))
