@@ -76,6 +76,32 @@ def timed(solver, steps, warmup):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
+def run_suspension(args, params, log):
+    """The 256-cell suspension (256x128x128, three resolutions, 20 %
+    hematocrit): its step and its plain step on the same mesh, per kernel."""
+    from lbfs_amd import abi, workloads
+    log("building the 256-cell suspension")
+    m3, p3, s3, mems, d0 = workloads.suspension()
+    p3.tile[0], p3.tile[1], p3.tile[2] = params.tile[0], params.tile[1], params.tile[2]
+    p3.block_threads = params.block_threads
+    b_plain = abi.Solver(m3, p3)
+    b_plain.set_state(*s3)
+    ms3_plain = timed(b_plain, max(args.steps // 4, 10), 5)
+    b_plain.close()
+    b = abi.Solver(m3, p3)
+    b.set_state(*s3)
+    b.add_membranes(mems, ibm_kernel=abi.IBM_COS4)
+    ms3 = timed(b, max(args.steps // 4, 10), 5)
+    k3 = b.time_kernels(10)
+    b.close()
+    nodes = sum(len(mm.nodes) // 3 for mm in mems)
+    log("suspension step %.4f ms (plain %.4f ms)" % (ms3, ms3_plain))
+    return {"workload": "256x128x128 jittered hex, 256 Evans-Fung cells (d0 %.1f, %d nodes), 20%% hematocrit, "
+                        "D3Q19" % (d0, nodes),
+            "ms_per_step": ms3, "plain_ms_per_step": ms3_plain, "ratio_vs_plain": ms3 / ms3_plain,
+            "mlups": m3.n_cells / (ms3 * 1e-3) / 1e6, "kernels_ms": k3}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +124,8 @@ def main():
                     help="time the channel + RBC step alone (no plain step, suspension or adjoint): profiling runs")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-every", type=int, default=100)
+    ap.add_argument("--only-batched", action="store_true",
+                    help="time the 256-cell suspension alone (no channel, adjoint or CPU baseline): profiling runs")
     ap.add_argument("--ensemble", action="store_true",
                     help="also time the 64-instance ensemble (edges from {32,48,64,96}, 1-8 cells of 642 nodes each)")
     args = ap.parse_args()
@@ -118,6 +146,14 @@ def main():
 
     from lbfs_amd import abi, workloads
     log = lambda m: print("bench: " + m, file=sys.stderr, flush=True)  # noqa: E731
+
+    if args.only_batched:
+        p0 = abi.make_params()
+        if args.tile:
+            from lbfs_amd import autotune
+            p0 = autotune.with_tile(p0, tuple(args.tile), args.threads or 256)
+        print(json.dumps({"batched": run_suspension(args, p0, log)}))
+        return
 
     # --- 128x64x64 channel with one 2562-node RBC (the headline step) ---
     log("building the 128x64x64 channel")
@@ -213,26 +249,7 @@ def main():
     # --- 256x128x128 suspension of 256 RBCs of mixed resolution ---
     batched = None
     if not args.no_batched:
-        log("building the 256-cell suspension")
-        m3, p3, s3, mems, d0 = workloads.suspension()
-        p3.tile[0], p3.tile[1], p3.tile[2] = params.tile[0], params.tile[1], params.tile[2]
-        p3.block_threads = params.block_threads
-        b_plain = abi.Solver(m3, p3)
-        b_plain.set_state(*s3)
-        ms3_plain = timed(b_plain, max(args.steps // 4, 10), 5)
-        b_plain.close()
-        b = abi.Solver(m3, p3)
-        b.set_state(*s3)
-        b.add_membranes(mems, ibm_kernel=abi.IBM_COS4)
-        ms3 = timed(b, max(args.steps // 4, 10), 5)
-        k3 = b.time_kernels(10)
-        b.close()
-        nodes = sum(len(mm.nodes) // 3 for mm in mems)
-        batched = {"workload": "256x128x128 jittered hex, 256 Evans-Fung cells (d0 %.1f, %d nodes), 20%% hematocrit, "
-                               "D3Q19" % (d0, nodes),
-                   "ms_per_step": ms3, "plain_ms_per_step": ms3_plain, "ratio_vs_plain": ms3 / ms3_plain,
-                   "mlups": m3.n_cells / (ms3 * 1e-3) / 1e6, "kernels_ms": k3}
-        log("suspension step %.4f ms (plain %.4f ms)" % (ms3, ms3_plain))
+        batched = run_suspension(args, params, log)
 
     # --- adjoint of the channel + RBC step, node loss against a run at
     # moduli x 1.1: every step recorded in HBM (no replay), and checkpoints

@@ -359,14 +359,16 @@ def test_membrane_zone_file_from_the_solver(tmp_path):
     s.close()
 
 
-def test_binned_spreading_across_a_ragged_periodic_axis():
+@pytest.mark.parametrize("nx", [22, 9])
+def test_binned_spreading_across_a_ragged_periodic_axis(nx):
     """A cell straddling the periodic boundary of an axis whose length is not a
-    multiple of the 4-cell bins (22 cells): spreading over every bin of the
-    grid reaches the wrapped stencils and equals the per-node spreading bit
-    for bit."""
+    multiple of the bins (22 cells; 9 cells, where an 11-cell window of the
+    8-cell bins covers cells twice): spreading over every bin of the grid
+    reaches the wrapped stencils and equals the per-node spreading bit for
+    bit."""
     outs = []
     for mode in (0, 2):
-        mesh = cases.channel_mesh(22, 16, 16, seed=16)
+        mesh = cases.channel_mesh(nx, 16, 16, seed=16)
         s = abi.Solver(mesh, abi.make_params(visc=VISC, dt=DT, body_force=(2e-5, 0.0, 0.0)))
         s.set_state(*cases.smooth_state(mesh, seed=16, amp=0.005))
         s.set_ibm_binned(mode)
