@@ -76,6 +76,15 @@ def timed(solver, steps, warmup):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
+def new_solver(mesh, params, fused=None):
+    """A solver with the given fluid pass (None: the library's default)."""
+    from lbfs_amd import abi
+    s = abi.Solver(mesh, params)
+    if fused is not None:
+        s.set_fused(fused)
+    return s
+
+
 def run_suspension(args, params, log):
     """The 256-cell suspension (256x128x128, three resolutions, 20 %
     hematocrit): its step and its plain step on the same mesh, per kernel."""
@@ -84,11 +93,19 @@ def run_suspension(args, params, log):
     m3, p3, s3, mems, d0 = workloads.suspension()
     p3.tile[0], p3.tile[1], p3.tile[2] = params.tile[0], params.tile[1], params.tile[2]
     p3.block_threads = params.block_threads
-    b_plain = abi.Solver(m3, p3)
-    b_plain.set_state(*s3)
-    ms3_plain = timed(b_plain, max(args.steps // 4, 10), 5)
-    b_plain.close()
-    b = abi.Solver(m3, p3)
+    # the fluid pass (fused or two passes, the same bits) by the plain step of this mesh
+    times = {}
+    for fz in (True, False):
+        try:
+            b_plain = new_solver(m3, p3, fz)
+        except abi.LbfsError:
+            continue
+        b_plain.set_state(*s3)
+        times[fz] = timed(b_plain, max(args.steps // 4, 10), 5)
+        b_plain.close()
+    fused = min(times, key=times.get)
+    ms3_plain = times[fused]
+    b = new_solver(m3, p3, fused)
     b.set_state(*s3)
     b.add_membranes(mems, ibm_kernel=abi.IBM_COS4)
     ms3 = timed(b, max(args.steps // 4, 10), 5)
@@ -98,6 +115,7 @@ def run_suspension(args, params, log):
     log("suspension step %.4f ms (plain %.4f ms)" % (ms3, ms3_plain))
     return {"workload": "256x128x128 jittered hex, 256 Evans-Fung cells (d0 %.1f, %d nodes), 20%% hematocrit, "
                         "D3Q19" % (d0, nodes),
+            "fluid_pass": "fused" if fused else "two-pass",
             "ms_per_step": ms3, "plain_ms_per_step": ms3_plain, "ratio_vs_plain": ms3 / ms3_plain,
             "mlups": m3.n_cells / (ms3 * 1e-3) / 1e6, "kernels_ms": k3}
 
@@ -160,6 +178,7 @@ def main():
     mesh, params, state = workloads.channel()
     n = mesh.n_cells
     tuning = None
+    fused = None  # the library's default fluid pass
     if args.tile:
         from lbfs_amd import autotune
         params = autotune.with_tile(params, tuple(args.tile), args.threads or 256)
@@ -167,21 +186,22 @@ def main():
         # tile shape and workgroup size of the flux/update kernel (results do
         # not depend on them); tuned on the plain step, outside the timed runs
         from lbfs_amd import autotune
-        params, table = autotune.tune(mesh, params, state=state, steps=20, warmup=5)
-        tuning = [{"tile": list(t), "threads": nt, "ms_per_step": ms} for t, nt, ms in table]
-        log("autotuned tile %s, %d threads" % (list(params.tile), params.block_threads))
+        params, fused, table = autotune.tune(mesh, params, state=state, steps=20, warmup=5)
+        tuning = [{"tile": list(t), "threads": nt, "fused": f, "ms_per_step": ms} for t, nt, f, ms in table]
+        log("autotuned tile %s, %d threads, %s fluid pass" % (list(params.tile), params.block_threads,
+                                                              "fused" if fused else "two-pass"))
     if args.only_rbc:
         args.no_batched = args.no_adjoint = True
         args.long_steps = args.tgv_steps = 0
         ms_plain = float("nan")
     else:
-        plain = abi.Solver(mesh, params)
+        plain = new_solver(mesh, params, fused)
         plain.set_state(*state)
         ms_plain = timed(plain, args.steps, args.warmup)
         plain.close()
         log("plain fluid step %.4f ms" % ms_plain)
 
-    solver = abi.Solver(mesh, params)
+    solver = new_solver(mesh, params, fused)
     solver.set_state(*state)
     solver.add_membranes(workloads.single_rbc(mesh), ibm_kernel=abi.IBM_COS4)
     ms_step = timed(solver, args.steps, args.warmup)
@@ -195,7 +215,8 @@ def main():
     dom = max(range(len(names)), key=lambda i: kms[names[i]])
     # the dominant kernel's duration: back-to-back launches between two HIP
     # events on the library stream (no per-launch event overhead)
-    dom_ms = solver.time_stage(names[dom], 50) if names[dom] in ("gradients", "flux_update") else kms[names[dom]]
+    dom_ms = solver.time_stage(names[dom], 50) if names[dom] in ("gradients", "flux_update", "fluid") \
+        else kms[names[dom]]
     achieved = mbytes[dom] / (dom_ms * 1e-3) / 1e9
     step_gbps = step_bytes / (ms_step * 1e-3) / 1e9
     diag = solver.diagnostics(ref_rho=1.0, ref_u=0.01)
@@ -207,7 +228,7 @@ def main():
     long_run = None
     if args.long_steps > 0:
         log("long run: %d steps" % args.long_steps)
-        lr = abi.Solver(mesh, params)
+        lr = new_solver(mesh, params, fused)
         lr.set_state(*state)
         lr.add_membranes(workloads.single_rbc(mesh), ibm_kernel=abi.IBM_COS4)
         d0 = lr.diagnostics(ref_rho=1.0, ref_u=0.01)
@@ -261,7 +282,7 @@ def main():
             mems = workloads.single_rbc(mesh)
             mp = mems[0].params
             mu, kl, kg = mp["shear_modulus"], mp["local_bending_modulus"], mp["global_bending_modulus"]
-            a = abi.Solver(mesh, params)
+            a = new_solver(mesh, params, fused)
             try:
                 a.set_state(*state)
                 a.add_membranes(mems, ibm_kernel=abi.IBM_COS4)
@@ -352,7 +373,8 @@ def main():
                                "body-force Poiseuille, one Evans-Fung RBC 2562 nodes / 5120 triangles, 4-point IBM",
                    "cells": n, "faces": mesh.n_faces, "lattice": "D3Q19",
                    "tile": [params.tile[0], params.tile[1], params.tile[2]],
-                   "block_threads": params.block_threads or 256},
+                   "block_threads": params.block_threads or 256,
+                   "fluid_pass": "library default" if fused is None else ("fused" if fused else "two-pass")},
         "autotune": tuning,
         "achieved_hbm_gbps_model": step_gbps,
         "plain_ms_per_step": None if args.only_rbc else ms_plain,

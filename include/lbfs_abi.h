@@ -121,7 +121,8 @@ typedef struct {
     double max_velocity;/* U of the CFL estimate (LBFS_TS_LOCAL / LBFS_TS_MIN) */
     int tile[3];        /* cells per tile along x, y, z of the logical grid (no counterpart in the
                            reference); 0 0 0 = 8 8 4.  Results do not depend on it. */
-    int block_threads;  /* workgroup size of the flux/update kernel: 0 (= 256), 256 or 512 */
+    int block_threads;  /* workgroup size of the fluid kernels: 0 (= 256), 256, 512 or 576 (three faces per
+                           thread on 8x8x8 tiles: 1728 faces) */
 } lbfs_params;
 
 /* time stepping (gpu_time_stepping, Solver_gpu.cu:767-790; get_cfl_device
@@ -164,6 +165,9 @@ int lbfs_step(lbfs_ctx *ctx, int nsteps);
                                     and the membrane forces and spreading on a second; 2 the fluid
                                     passes on a second stream from the start, the membrane forces
                                     on a third after the interpolation */
+#define LBFS_OPT_FUSED_FLUID 4 /* 1 (default where the tiles fit): whole steps run the fluid passes as one
+                                 launch per tile (the tile's gradients in LDS); 0: gradients, then the
+                                 face pass; the result is the same bit for bit */
 int lbfs_set_option(lbfs_ctx *ctx, int option, int value);
 
 /* per-stage entry points (parity tests) */

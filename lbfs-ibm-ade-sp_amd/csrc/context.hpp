@@ -69,6 +69,8 @@ struct lbfs_ctx {
 
     // hipGraph of two consecutive steps starting from cur == 0
     int schedule_option = 0;        // LBFS_OPT_STEP_SCHEDULE
+    bool fused_ok = false;          // the tiles fit the fused fluid pass's LDS
+    bool fused = false;             // LBFS_OPT_FUSED_FLUID: whole steps run the fused fluid pass
     bool use_graph = true;
     hipGraphExec_t graph_exec = nullptr;
     void drop_graph();
@@ -108,7 +110,9 @@ struct lbfs_ctx {
         std::function<int()> run;
         double model_bytes;
     };
-    std::vector<Stage> step_stages(bool with_visc = false);
+    // fused: the fluid passes as the one fused launch "fluid" (the step's own
+    // launches); otherwise "gradients" and "flux_update" (the stage API)
+    std::vector<Stage> step_stages(bool with_visc = false, bool fused = false);
     // the internal viscosity field is rebuilt at t = 1 (mod 10000) (Solver_gpu.cu:884-889)
     bool visc_due(long long t) const { return visc_factor && (t % 10000) == 1; }
     int run_stages(bool with_visc);

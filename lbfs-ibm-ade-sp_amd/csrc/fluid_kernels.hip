@@ -27,16 +27,11 @@ namespace lbfs {
 // ---------------------------------------------------------------------------
 // gradients
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_gradients(FluidArgs a, const float4 *__restrict__ st,
-                                                   float4 *__restrict__ grad)
+// least-squares gradient of cell i (12 components: rho, u, v, w along x, y, z)
+__device__ __forceinline__ void cell_gradient(const FluidArgs &a, const float4 *__restrict__ st, int i, float R[12])
 {
-    const int nblk = gridDim.x;
-    const int b = xcd_remap(blockIdx.x, nblk);
-    const int i = b * 256 + threadIdx.x;
     const int n = a.n_cells;
-    if (i >= n) return;
     const float4 si = st[i];
-    float R[12];
 #pragma unroll
     for (int t = 0; t < 12; t++) R[t] = 0.0f;
 #pragma unroll
@@ -66,17 +61,49 @@ __global__ __launch_bounds__(256) void k_gradients(FluidArgs a, const float4 *__
         R[6] += wx * dv; R[7] += wy * dv; R[8] += wz * dv;
         R[9] += wx * dw; R[10] += wy * dw; R[11] += wz * dw;
     }
+}
+
+__global__ __launch_bounds__(256) void k_gradients(FluidArgs a, const float4 *__restrict__ st,
+                                                   float4 *__restrict__ grad)
+{
+    const int nblk = gridDim.x;
+    const int b = xcd_remap(blockIdx.x, nblk);
+    const int i = b * 256 + threadIdx.x;
+    if (i >= a.n_cells) return;
+    float R[12];
+    cell_gradient(a, st, i, R);
     grad[3 * (size_t)i + 0] = make_float4(R[0], R[1], R[2], R[3]);
     grad[3 * (size_t)i + 1] = make_float4(R[4], R[5], R[6], R[7]);
     grad[3 * (size_t)i + 2] = make_float4(R[8], R[9], R[10], R[11]);
 }
 
+// Sources of the cell gradients a face reads: the gradient array of the
+// two-pass step, or the fused step's tile (the gradients of the tile's cells
+// and of the cells across its border in LDS, at the face's slots)
+struct GlobalGrad {
+    const float4 *__restrict__ grad;
+    __device__ __forceinline__ void load_owner(int, int i, float g[12]) const { load_grad(grad, i, g); }
+    __device__ __forceinline__ void load_nbr(int, int i, float g[12]) const { load_grad(grad, i, g); }
+};
+struct TileGrad {
+    const float4 *lg;              // LDS: 3 float4 per gradient slot
+    const uint32_t *__restrict__ slots;  // per tile-face: owner slot | neighbour slot << 16
+    __device__ __forceinline__ void at(int q, float g[12]) const
+    {
+        const float4 x = lg[3 * q], y = lg[3 * q + 1], z = lg[3 * q + 2];
+        g[0] = x.x; g[1] = x.y; g[2] = x.z; g[3] = x.w;
+        g[4] = y.x; g[5] = y.y; g[6] = y.z; g[7] = y.w;
+        g[8] = z.x; g[9] = z.y; g[10] = z.z; g[11] = z.w;
+    }
+    __device__ __forceinline__ void load_owner(int tf, int, float g[12]) const { at(slots[tf] & 0xffffu, g); }
+    __device__ __forceinline__ void load_nbr(int tf, int, float g[12]) const { at(slots[tf] >> 16, g); }
+};
+
 // ---------------------------------------------------------------------------
 // face reconstruction and flux
 // ---------------------------------------------------------------------------
-template <class L, bool BAND>
-__device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__restrict__ st,
-                                            const float4 *__restrict__ grad, int tf)
+template <class L, bool BAND, class G>
+__device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__restrict__ st, const G &gsrc, int tf)
 {
     static_assert(opposite_pairs<L>(), "lattice directions must come in opposite pairs after the rest point");
     const FaceRec fr = load_face(a, tf);
@@ -86,12 +113,12 @@ __device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__
     const float nx = fr.nx, ny = fr.ny, nz = fr.nz, A = fr.A, dt = fr.dt;
     const float4 so = st[o];
     float go[12], gn[12];
-    load_grad(grad, o, go);
+    gsrc.load_owner(tf, o, go);
     const float sov[4] = {so.x, so.y, so.z, so.w};
     float po[4], pn[4], p0[4];
     if (!ghost) {
         const float4 sn = st[nb];
-        load_grad(grad, nb, gn);
+        gsrc.load_nbr(tf, nb, gn);
         const float snv[4] = {sn.x, sn.y, sn.z, sn.w};
 #pragma unroll
         for (int v = 0; v < 4; v++) {
@@ -172,19 +199,15 @@ __device__ __forceinline__ float4 face_flux(const FluidArgs &a, const float4 *__
     return r;
 }
 
-template <class L, bool BAND, int NT>
-__global__ __launch_bounds__(NT) void k_flux_update(FluidArgs a, const float4 *__restrict__ st,
-                                                     const float4 *__restrict__ grad, float4 *__restrict__ st_out,
-                                                     float4 *__restrict__ res_out, int do_update)
+// the cell update of a tile from its face fluxes in LDS: each cell gathers
+// its six signed face slots in a fixed order (LBFS.cu:2242-2281), adds the
+// force density and advances (LBFS.cu:2089-2172)
+template <int NT>
+__device__ __forceinline__ void tile_update(const FluidArgs &a, const float4 *lds_flux, int c0, int c1,
+                                            const float4 *__restrict__ st, float4 *__restrict__ st_out,
+                                            float4 *__restrict__ res_out, int do_update)
 {
-    extern __shared__ float4 lds_flux[];
-    const int t = xcd_remap(blockIdx.x, gridDim.x);
-    const int f0 = a.tile_face_start[t], f1 = a.tile_face_start[t + 1];
-    for (int f = f0 + threadIdx.x; f < f1; f += blockDim.x)
-        lds_flux[f - f0] = face_flux<L, BAND>(a, st, grad, f);
-    __syncthreads();
-    const int c0 = a.tile_cell_start[t], c1 = a.tile_cell_start[t + 1];
-    for (int i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
+    for (int i = c0 + threadIdx.x; i < c1; i += NT) {
         const uint32_t *sl = reinterpret_cast<const uint32_t *>(a.cell_slots) + 3 * (size_t)i;
         const uint32_t s01 = sl[0], s23 = sl[1], s45 = sl[2];
         const uint32_t s[6] = {s01 & 0xffffu, s01 >> 16, s23 & 0xffffu, s23 >> 16, s45 & 0xffffu, s45 >> 16};
@@ -225,6 +248,55 @@ __global__ __launch_bounds__(NT) void k_flux_update(FluidArgs a, const float4 *_
     }
 }
 
+// two-pass step, second pass: the tile's faces from the gradient array
+template <class L, bool BAND, int NT>
+__global__ __launch_bounds__(NT) void k_flux_update(FluidArgs a, const float4 *__restrict__ st,
+                                                     const float4 *__restrict__ grad, float4 *__restrict__ st_out,
+                                                     float4 *__restrict__ res_out, int do_update)
+{
+    extern __shared__ float4 lds_flux[];
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int f0 = a.tile_face_start[t], f1 = a.tile_face_start[t + 1];
+    const GlobalGrad gsrc{grad};
+    for (int f = f0 + threadIdx.x; f < f1; f += blockDim.x)
+        lds_flux[f - f0] = face_flux<L, BAND>(a, st, gsrc, f);
+    __syncthreads();
+    tile_update<NT>(a, lds_flux, a.tile_cell_start[t], a.tile_cell_start[t + 1], st, st_out, res_out, do_update);
+}
+
+// Fused step, one workgroup per tile: the gradients of the tile's cells and
+// of the cells across its border that its faces reach into LDS (the same
+// arithmetic as the gradient pass), the tile's faces from them, then the cell
+// update.  Results equal the two-pass step bit for bit; the gradient array is
+// neither written nor read.
+template <class L, bool BAND, int NT>
+__global__ __launch_bounds__(NT) void k_fluid_fused(FluidArgs a, const float4 *__restrict__ st,
+                                                     float4 *__restrict__ st_out, float4 *__restrict__ res_out,
+                                                     int do_update)
+{
+    extern __shared__ float4 lds[];
+    float4 *lg = lds;                                   // 3 per gradient slot
+    float4 *lds_flux = lds + 3 * (size_t)a.max_grad_slots;
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int c0 = a.tile_cell_start[t], c1 = a.tile_cell_start[t + 1], nc = c1 - c0;
+    const int h0 = a.tile_halo_start[t], nq = nc + a.tile_halo_start[t + 1] - h0;
+    for (int q = threadIdx.x; q < nq; q += NT) {
+        const int i = q < nc ? c0 + q : a.halo_ids[h0 + q - nc];
+        float R[12];
+        cell_gradient(a, st, i, R);
+        float4 *o = lg + 3 * q;
+        o[0] = make_float4(R[0], R[1], R[2], R[3]);
+        o[1] = make_float4(R[4], R[5], R[6], R[7]);
+        o[2] = make_float4(R[8], R[9], R[10], R[11]);
+    }
+    __syncthreads();
+    const int f0 = a.tile_face_start[t], f1 = a.tile_face_start[t + 1];
+    const TileGrad gsrc{lg, a.face_gslot};
+    for (int f = f0 + threadIdx.x; f < f1; f += NT) lds_flux[f - f0] = face_flux<L, BAND>(a, st, gsrc, f);
+    __syncthreads();
+    tile_update<NT>(a, lds_flux, c0, c1, st, st_out, res_out, do_update);
+}
+
 template <class L, bool BAND>
 __global__ __launch_bounds__(256) void k_face_flux_all(FluidArgs a, const float4 *__restrict__ st,
                                                        const float4 *__restrict__ grad, float4 *__restrict__ out,
@@ -232,7 +304,7 @@ __global__ __launch_bounds__(256) void k_face_flux_all(FluidArgs a, const float4
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f < n_tile_faces)
-        out[f] = face_flux<L, BAND>(a, st, grad, f);
+        out[f] = face_flux<L, BAND>(a, st, GlobalGrad{grad}, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -333,7 +405,9 @@ struct FluxUpdate {
     static void launch(dim3 g, size_t lds, hipStream_t s, const FluidArgs &a, const float4 *st, const float4 *grad,
                        float4 *st_out, float4 *res_out, int do_update)
     {
-        if (a.block_threads == 512)
+        if (a.block_threads == 576)
+            hipLaunchKernelGGL((k_flux_update<L, BAND, 576>), g, dim3(576), lds, s, a, st, grad, st_out, res_out, do_update);
+        else if (a.block_threads == 512)
             hipLaunchKernelGGL((k_flux_update<L, BAND, 512>), g, dim3(512), lds, s, a, st, grad, st_out, res_out, do_update);
         else
             hipLaunchKernelGGL((k_flux_update<L, BAND, 256>), g, dim3(256), lds, s, a, st, grad, st_out, res_out, do_update);
@@ -354,6 +428,31 @@ hipError_t launch_flux_update(const FluidArgs &a, const float4 *st, const float4
 {
     const size_t lds = sizeof(float4) * (size_t)a.max_tile_faces;
     return dispatch<FluxUpdate>(a, dim3(a.n_tiles), lds, s, st, grad, st_out, res_out, do_update);
+}
+
+template <class L, bool BAND>
+struct FluidFused {
+    static void launch(dim3 g, size_t lds, hipStream_t s, const FluidArgs &a, const float4 *st, float4 *st_out,
+                       float4 *res_out, int do_update)
+    {
+        if (a.block_threads == 576)
+            hipLaunchKernelGGL((k_fluid_fused<L, BAND, 576>), g, dim3(576), lds, s, a, st, st_out, res_out, do_update);
+        else if (a.block_threads == 512)
+            hipLaunchKernelGGL((k_fluid_fused<L, BAND, 512>), g, dim3(512), lds, s, a, st, st_out, res_out, do_update);
+        else
+            hipLaunchKernelGGL((k_fluid_fused<L, BAND, 256>), g, dim3(256), lds, s, a, st, st_out, res_out, do_update);
+    }
+};
+
+size_t fluid_fused_lds(const FluidArgs &a)
+{
+    return sizeof(float4) * (3 * (size_t)a.max_grad_slots + (size_t)a.max_tile_faces);
+}
+
+hipError_t launch_fluid_fused(const FluidArgs &a, const float4 *st, float4 *st_out, float4 *res_out, int do_update,
+                              hipStream_t s)
+{
+    return dispatch<FluidFused>(a, dim3(a.n_tiles), fluid_fused_lds(a), s, st, st_out, res_out, do_update);
 }
 
 hipError_t launch_face_flux_all(const FluidArgs &a, const float4 *st, const float4 *grad, float4 *out, int n_tile_faces,

@@ -10,6 +10,10 @@ namespace lbfs {
 
 struct FluidArgs {
     int n_cells, n_bc, n_tiles, max_tile_faces;
+    int max_tile_cells;            // largest tile
+    int max_grad_slots;            // largest tile plus its halo (LDS of the fused step)
+    const int *tile_halo_start, *halo_ids;  // per tile: cells across its border its faces reach
+    const uint32_t *face_gslot;    // per tile-face: LDS gradient slots of owner | neighbour << 16
     int block_threads;             // workgroup size of the flux/update kernel (256 or 512)
     int lattice, flags;
     float rho0, inv_gamma, visc3_over_gamma, dt;
@@ -40,6 +44,11 @@ struct FluidArgs {
 hipError_t launch_gradients(const FluidArgs &a, const float4 *st, float4 *grad, hipStream_t s);
 hipError_t launch_flux_update(const FluidArgs &a, const float4 *st, const float4 *grad, float4 *st_out,
                               float4 *res_out, int do_update, hipStream_t s);
+// the whole fluid step in one pass per tile (gradients in LDS); equals
+// launch_gradients + launch_flux_update bit for bit
+hipError_t launch_fluid_fused(const FluidArgs &a, const float4 *st, float4 *st_out, float4 *res_out, int do_update,
+                              hipStream_t s);
+size_t fluid_fused_lds(const FluidArgs &a);
 hipError_t launch_face_flux_all(const FluidArgs &a, const float4 *st, const float4 *grad, float4 *out,
                                 int n_tile_faces, hipStream_t s);
 // global minimum of the local steps into *a.dt_min (LBFS_TS_MIN)

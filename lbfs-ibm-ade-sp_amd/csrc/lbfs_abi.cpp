@@ -68,8 +68,8 @@ static int fluid_setup(lbfs_ctx *ctx, const lbfs_mesh_desc &m, const lbfs_params
                              p.tile[1], p.tile[2]);
         for (int t = 0; t < 3; t++) tile[t] = p.tile[t];
     }
-    if (p.block_threads != 0 && p.block_threads != 256 && p.block_threads != 512)
-        return ctx->fail(LBFS_E_INVALID_ARG, "block_threads must be 256 or 512");
+    if (p.block_threads != 0 && p.block_threads != 256 && p.block_threads != 512 && p.block_threads != 576)
+        return ctx->fail(LBFS_E_INVALID_ARG, "block_threads must be 256, 512 or 576");
     rc = mesh_blocks(m, ctx->blocks, err);
     if (rc) return ctx->fail(rc, "mesh blocks: %s", err.c_str());
     rc = build_layout(m, ctx->geo, lhs, rhs, p.rho0, p.body_force, p.lattice == LBFS_D2Q9 ? 2 : 3, tile, ctx->blocks,
@@ -83,9 +83,26 @@ static int fluid_setup(lbfs_ctx *ctx, const lbfs_mesh_desc &m, const lbfs_params
     a.n_bc = nbc;
     a.n_tiles = d.n_tiles;
     a.max_tile_faces = d.max_tile_faces;
-    a.block_threads = p.block_threads == 512 ? 512 : 256;
+    a.block_threads = p.block_threads == 512 || p.block_threads == 576 ? p.block_threads : 256;
     if ((size_t)d.max_tile_faces * 16 > 96 * 1024)
         return ctx->fail(LBFS_E_INVALID_ARG, "tile too large: %d faces need more than 96 KiB of LDS", d.max_tile_faces);
+    for (int t = 0; t < d.n_tiles; t++)
+        a.max_tile_cells = std::max(a.max_tile_cells, d.tile_cell_start[t + 1] - d.tile_cell_start[t]);
+    a.max_grad_slots = d.max_grad_slots;
+    {
+        int *ths, *hid;
+        uint32_t *fgs;
+        if ((rc = dev_upload(ctx, &ths, d.tile_halo_start.data(), d.tile_halo_start.size()))) return rc;
+        if ((rc = dev_upload(ctx, &hid, d.halo_ids.data(), d.halo_ids.size()))) return rc;
+        if ((rc = dev_upload(ctx, &fgs, d.face_gslot.data(), d.face_gslot.size()))) return rc;
+        a.tile_halo_start = ths;
+        a.halo_ids = hid;
+        a.face_gslot = fgs;
+    }
+    // the fused fluid pass keeps the tile's gradients in LDS beside its face
+    // fluxes; larger tiles run the two passes
+    ctx->fused_ok = fluid_fused_lds(a) <= 96 * 1024;
+    ctx->fused = ctx->fused_ok;
     a.lattice = p.lattice;
     a.flags = p.flags;
     a.rho0 = (float)p.rho0;
@@ -423,6 +440,14 @@ extern "C" int lbfs_set_option(lbfs_ctx *ctx, int option, int value)
     if (option == LBFS_OPT_IBM_BINNED) {
         if (value < -1 || value > 2) return ctx->fail(LBFS_E_INVALID_ARG, "LBFS_OPT_IBM_BINNED takes -1, 0, 1 or 2");
         ctx->ibm_binned_option = value;  // applies to the next lbfs_add_membranes
+        return LBFS_OK;
+    }
+    if (option == LBFS_OPT_FUSED_FLUID) {
+        if (value < 0 || value > 1) return ctx->fail(LBFS_E_INVALID_ARG, "LBFS_OPT_FUSED_FLUID takes 0 or 1");
+        if (value && !ctx->fused_ok)
+            return ctx->fail(LBFS_E_UNSUPPORTED, "the tiles are too large for the fused fluid pass");
+        ctx->fused = value != 0;
+        ctx->drop_graph();
         return LBFS_OK;
     }
     if (option == LBFS_OPT_STEP_SCHEDULE) {

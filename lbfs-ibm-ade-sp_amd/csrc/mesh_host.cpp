@@ -381,6 +381,11 @@ int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<d
     d.fA.clear(); d.fB.clear(); d.fC.clear(); d.face_ref.clear();
     d.face_first.assign(g.n_faces, -1);
     d.max_tile_faces = 0;
+    d.tile_halo_start.assign(1, 0);
+    d.halo_ids.clear();
+    d.face_gslot.clear();
+    d.max_grad_slots = 0;
+    std::vector<int> halo_slot(nc, -1), halo;
     std::vector<int> faces;
     for (int t = 0; t < d.n_tiles; t++) {
         faces.clear();
@@ -393,6 +398,26 @@ int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<d
         // caller order puts internal faces (ids < n_int) before boundary faces
         d.tile_bface_start.push_back(base + (int)(std::lower_bound(faces.begin(), faces.end(), g.n_int) - faces.begin()));
         d.max_tile_faces = std::max(d.max_tile_faces, (int)faces.size());
+        const int c0 = d.tile_cell_start[t], c1 = d.tile_cell_start[t + 1], ntc = c1 - c0;
+        halo.clear();
+        auto gslot = [&](int ic) -> uint32_t {
+            if (ic >= c0 && ic < c1) return (uint32_t)(ic - c0);
+            if (halo_slot[ic] < 0) {
+                halo_slot[ic] = ntc + (int)halo.size();
+                halo.push_back(ic);
+            }
+            return (uint32_t)halo_slot[ic];
+        };
+        for (int f : faces) {
+            const int o = d.inv[g.owner[f]], n = g.nbr[f];
+            const uint32_t so = gslot(o), sn = n < nc ? gslot(d.inv[n]) : 0xffffu;
+            d.face_gslot.push_back(so | (sn << 16));
+        }
+        for (int ic : halo) halo_slot[ic] = -1;
+        d.halo_ids.insert(d.halo_ids.end(), halo.begin(), halo.end());
+        d.tile_halo_start.push_back((int)d.halo_ids.size());
+        d.max_grad_slots = std::max(d.max_grad_slots, ntc + (int)halo.size());
+        if (ntc + (int)halo.size() >= 0xffff) { err = "tile and its halo exceed 65534 cells"; return LBFS_E_UNSUPPORTED; }
         for (int f : faces) {
             const int o = g.owner[f], n = g.nbr[f];
             const V3 xf = load3(&g.fc[3 * (size_t)f]);

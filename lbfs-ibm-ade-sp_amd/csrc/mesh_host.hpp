@@ -81,6 +81,13 @@ struct DeviceLayout {
     std::vector<float> fB;        // 4 per face: owner centroid - face centroid, owner id
     std::vector<float> fC;        // 4 per face: neighbour image centroid - face centroid, neighbour id
                                   // (ghost: n_cells + bc)
+    // fused fluid pass: per tile the cells across its border that its faces
+    // reach (their gradients are computed into the tile's LDS after its own),
+    // and per tile-face the LDS gradient slots of owner and neighbour
+    // (tile cells first, then that halo; 0xffff for a ghost)
+    std::vector<int> tile_halo_start, halo_ids;
+    std::vector<uint32_t> face_gslot;
+    int max_grad_slots = 0;
     std::vector<int> face_ref;    // caller face id of each tile-face
     std::vector<int> face_first;  // caller face id -> one tile-face holding it
     // boundary faces

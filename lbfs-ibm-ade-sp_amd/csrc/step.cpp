@@ -16,7 +16,7 @@ using namespace lbfs;
         if (e_ != hipSuccess) return ctx->fail(LBFS_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
-std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
+std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc, bool fused_fluid)
 {
     std::vector<Stage> s;
     lbfs_ctx *ctx = this;
@@ -62,18 +62,20 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
                      },
                      36.0 * 64.0 * nn / 16.0});
     }
-    // pass G: state 16 + stencil 24 + RHS 72 + LS matrix 36 + gradients written 48 (B per cell)
-    s.push_back({"gradients", [ctx]() -> int {
-                     HIPCHK(launch_gradients(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->stream));
-                     return LBFS_OK;
-                 },
-                 196.0 * n});
+    const bool fused = fused_fluid && fused_ok && args.ts_mode != LBFS_TS_MIN;
     if (args.ts_mode == LBFS_TS_MIN)
         s.push_back({"dt_min", [ctx]() -> int {
                          HIPCHK(launch_dt_min(ctx->args, ctx->st[ctx->cur], ctx->stream));
                          return LBFS_OK;
                      },
                      24.0 * n});
+    // pass G: state 16 + stencil 24 + RHS 72 + LS matrix 36 + gradients written 48 (B per cell)
+    if (!fused)
+        s.push_back({"gradients", [ctx]() -> int {
+                         HIPCHK(launch_gradients(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->stream));
+                         return LBFS_OK;
+                     },
+                     196.0 * n});
     if (have_membranes) {
         // The face pass updates the cell with the static force (it does not
         // need the spread force, so it runs beside the coupling chain); the
@@ -82,12 +84,22 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
         // scale 4 + static force 12 + state read and written 32 + force
         // written 12 per touched cell (about 343 cells per 16 nodes)
         const double touched = std::min(n, 343.0 * mem.n_nodes / 16.0);
-        s.push_back({"flux_update", [ctx]() -> int {
-                         HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->st[ctx->cur ^ 1],
-                                                   ctx->res_out(), 1, ctx->stream));
-                         return LBFS_OK;
-                     },
-                     108.0 * n + 40.0 * nf});
+        if (fused)
+            // the fused pass moves the bytes of both passes' model (the
+            // gradients stay in LDS)
+            s.push_back({"fluid", [ctx]() -> int {
+                             HIPCHK(launch_fluid_fused(ctx->args, ctx->st[ctx->cur], ctx->st[ctx->cur ^ 1],
+                                                       ctx->res_out(), 1, ctx->stream));
+                             return LBFS_OK;
+                         },
+                         304.0 * n + 40.0 * nf});
+        else
+            s.push_back({"flux_update", [ctx]() -> int {
+                             HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->st[ctx->cur ^ 1],
+                                                       ctx->res_out(), 1, ctx->stream));
+                             return LBFS_OK;
+                         },
+                         108.0 * n + 40.0 * nf});
         s.push_back({"ibm_apply", [ctx]() -> int {
                          HIPCHK(launch_ibm_apply(ctx->ibm, ctx->apply_args(ctx->st[ctx->cur ^ 1], ctx->cell_force, 0),
                                                  ctx->stream));
@@ -99,23 +111,33 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc)
     }
     // pass F: state 16 + gradients 48 + volume/centroid 16 + force 12 + state written 16 per cell,
     // plus owner/nb 8, normal 12, face centroid 12, area 4, dt 4 per face
-    s.push_back({"flux_update", [ctx]() -> int {
-                     HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->st[ctx->cur ^ 1],
-                                               ctx->res_out(), 1, ctx->stream));
-                     ctx->cur ^= 1;
-                     return LBFS_OK;
-                 },
-                 108.0 * n + 40.0 * nf});
+    if (fused)
+        s.push_back({"fluid", [ctx]() -> int {
+                         HIPCHK(launch_fluid_fused(ctx->args, ctx->st[ctx->cur], ctx->st[ctx->cur ^ 1], ctx->res_out(),
+                                                   1, ctx->stream));
+                         ctx->cur ^= 1;
+                         return LBFS_OK;
+                     },
+                     304.0 * n + 40.0 * nf});
+    else
+        s.push_back({"flux_update", [ctx]() -> int {
+                         HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->st[ctx->cur ^ 1],
+                                                   ctx->res_out(), 1, ctx->stream));
+                         ctx->cur ^= 1;
+                         return LBFS_OK;
+                     },
+                     108.0 * n + 40.0 * nf});
     return s;
 }
 
 int lbfs_ctx::run_stages(bool with_visc)
 {
     lbfs_ctx *ctx = this;
+    const bool fz = fused && fused_ok && args.ts_mode != LBFS_TS_MIN;
     if (!have_membranes || !side || with_visc) {
         // (the internal viscosity field feeds the face pass: in the steps
         // that rebuild it, everything runs in order on one stream)
-        for (auto &st : step_stages(with_visc)) {
+        for (auto &st : step_stages(with_visc, fz)) {
             int rc = st.run();
             if (rc) return rc;
         }
@@ -140,8 +162,12 @@ int lbfs_ctx::run_stages(bool with_visc)
         const bool interp_first = schedule_option == 1;
         if (interp_first) HIPCHK(launch_ibm_interp(mem, ibm, st[cur], stream));
         HIPCHK(hipEventRecord(ev_fork, stream));
-        HIPCHK(launch_gradients(args, st[cur], grad, stream));
-        HIPCHK(launch_flux_update(args, st[cur], grad, st[cur ^ 1], res_out(), 1, stream));
+        if (fz) {
+            HIPCHK(launch_fluid_fused(args, st[cur], st[cur ^ 1], res_out(), 1, stream));
+        } else {
+            HIPCHK(launch_gradients(args, st[cur], grad, stream));
+            HIPCHK(launch_flux_update(args, st[cur], grad, st[cur ^ 1], res_out(), 1, stream));
+        }
         HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
         if (!interp_first) HIPCHK(launch_ibm_interp(mem, ibm, st[cur], side));
         HIPCHK(launch_mem_volume(mem, side));
@@ -156,8 +182,12 @@ int lbfs_ctx::run_stages(bool with_visc)
     }
     HIPCHK(hipEventRecord(ev_fork, stream));
     HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
-    HIPCHK(launch_gradients(args, st[cur], grad, side));
-    HIPCHK(launch_flux_update(args, st[cur], grad, st[cur ^ 1], res_out(), 1, side));
+    if (fz) {
+        HIPCHK(launch_fluid_fused(args, st[cur], st[cur ^ 1], res_out(), 1, side));
+    } else {
+        HIPCHK(launch_gradients(args, st[cur], grad, side));
+        HIPCHK(launch_flux_update(args, st[cur], grad, st[cur ^ 1], res_out(), 1, side));
+    }
     HIPCHK(hipEventRecord(ev_join, side));
     HIPCHK(launch_ibm_interp(mem, ibm, st[cur], stream));
     HIPCHK(hipEventRecord(ev_fork2, stream));
@@ -225,7 +255,7 @@ extern "C" int lbfs_time_kernels(lbfs_ctx *ctx, int nsteps, int max_count, const
     for (int s = 0; s < nsteps; s++)
         if (ctx->visc_due(ctx->t_step + s))
             return ctx->fail(LBFS_E_STATE, "a timed step would rebuild the internal viscosity field");
-    std::vector<lbfs_ctx::Stage> stages = ctx->step_stages();
+    std::vector<lbfs_ctx::Stage> stages = ctx->step_stages(false, ctx->fused);
     const int ns = (int)stages.size();
     EventSet E;
     HIPCHK(E.create((size_t)nsteps * (ns + 1)));
@@ -259,9 +289,10 @@ extern "C" int lbfs_time_stage(lbfs_ctx *ctx, const char *name, int n, double *m
     if (!ctx || !name || n <= 0 || !ms_avg) return LBFS_E_INVALID_ARG;
     // the fluid passes read the state and write scratch or the other state
     // buffer, so back-to-back launches leave the state as it was
-    if (std::strcmp(name, "gradients") != 0 && std::strcmp(name, "flux_update") != 0)
-        return ctx->fail(LBFS_E_INVALID_ARG, "lbfs_time_stage times the fluid passes only (gradients, flux_update)");
-    std::vector<lbfs_ctx::Stage> stages = ctx->step_stages();
+    if (std::strcmp(name, "gradients") != 0 && std::strcmp(name, "flux_update") != 0 && std::strcmp(name, "fluid") != 0)
+        return ctx->fail(LBFS_E_INVALID_ARG,
+                         "lbfs_time_stage times the fluid passes only (gradients, flux_update, fluid)");
+    std::vector<lbfs_ctx::Stage> stages = ctx->step_stages(false, std::strcmp(name, "fluid") == 0);
     const lbfs_ctx::Stage *st = nullptr;
     for (auto &s : stages)
         if (std::strcmp(s.name, name) == 0) st = &s;
@@ -286,7 +317,7 @@ extern "C" int lbfs_time_stage(lbfs_ctx *ctx, const char *name, int n, double *m
 extern "C" int lbfs_model_bytes(lbfs_ctx *ctx, int max_count, double *bytes, int *count)
 {
     if (!ctx || !bytes || !count || max_count <= 0) return LBFS_E_INVALID_ARG;
-    std::vector<lbfs_ctx::Stage> stages = ctx->step_stages();
+    std::vector<lbfs_ctx::Stage> stages = ctx->step_stages(false, ctx->fused);
     *count = (int)stages.size() < max_count ? (int)stages.size() : max_count;
     for (int k = 0; k < *count; k++) bytes[k] = stages[k].model_bytes;
     return LBFS_OK;

@@ -22,7 +22,7 @@ FLAG_GHOST_FACE_GRAD = 64
 # the reference GPU solver's face semantics (tie band, ghost direction, ghost gradient at every point)
 FLAGS_REFERENCE_GPU = FLAG_UPWIND_BAND | FLAG_GHOST_DIR_GPU | FLAG_GHOST_FACE_GRAD
 IBM_LINEAR2, IBM_COS4, IBM_LAGRANGE4 = 0, 1, 2
-OPT_GRAPH, OPT_IBM_BINNED, OPT_STEP_SCHEDULE = 1, 2, 3
+OPT_GRAPH, OPT_IBM_BINNED, OPT_STEP_SCHEDULE, OPT_FUSED_FLUID = 1, 2, 3, 4
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -670,6 +670,11 @@ class Solver:
         g = np.zeros(6 * len(self._membranes))
         check(load().lbfs_get_membrane_globals(self.ctx, ptr(g)), self.ctx)
         return g.reshape(-1, 6)
+
+    def set_fused(self, on):
+        """Whole steps with the fused fluid pass (1, the default where the
+        tiles fit) or the two passes (0); the same bits either way."""
+        check(load().lbfs_set_option(self.ctx, OPT_FUSED_FLUID, 1 if on else 0), self.ctx)
 
     def set_ibm_binned(self, mode):
         """Spreading path of the next add_membranes: -1 by node count, 0 per node, 1 per bin

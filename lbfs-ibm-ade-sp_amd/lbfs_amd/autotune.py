@@ -1,10 +1,12 @@
-"""Tile-shape and workgroup-size autotuner for the flux/update kernel.
+"""Tile-shape, workgroup-size and fluid-pass autotuner.
 
-The cell tiling (cells per tile along x, y, z) and the workgroup size change
-neither the arithmetic nor the summation order of the step, only how faces are
-shared between workgroups and how many workgroups fill the chip.  The tuner
-builds the solver once per candidate on the caller's mesh, times a few whole
-steps on the device, and returns the fastest candidate.
+The cell tiling (cells per tile along x, y, z), the workgroup size and the
+choice between the fused fluid pass (each tile's gradients in LDS) and the
+gradient + face passes change neither the arithmetic nor the summation order
+of the step, only how faces and gradients are shared between workgroups and
+how many workgroups fill the chip.  The tuner builds the solver once per
+candidate on the caller's mesh, times a few whole steps on the device, and
+returns the fastest candidate.
 """
 from . import abi
 
@@ -14,6 +16,7 @@ CANDIDATES = (
     (8, 4, 8, 256),
     (8, 8, 8, 256),
     (8, 8, 8, 512),
+    (8, 8, 8, 576),
     (16, 8, 4, 256),
     (16, 8, 4, 512),
     (16, 8, 8, 512),
@@ -37,31 +40,39 @@ def with_tile(params, tile, threads):
 
 
 def tune(mesh, params, state=None, steps=20, warmup=5, candidates=CANDIDATES, membranes=None, ibm_kernel=None):
-    """Time `steps` whole steps per candidate and return (best_params, table).
+    """Time `steps` whole steps per candidate (tile, threads) with the fused
+    fluid pass where the tiles fit it and with the two passes, and return
+    (best_params, best_fused, table).
 
-    table: list of (tile, threads, ms_per_step).  Candidates whose tiles do not
-    fit the mesh or the LDS budget are skipped."""
+    table: list of (tile, threads, fused, ms_per_step).  Candidates whose
+    tiles do not fit the mesh or the LDS budget are skipped."""
     table = []
     best = None
     for tx, ty, tz, nt in candidates:
         if mesh.nx and (tx > mesh.nx or ty > mesh.ny or tz > mesh.nz):
             continue
         p = with_tile(params, (tx, ty, tz), nt)
-        try:
-            s = abi.Solver(mesh, p)
-        except abi.LbfsError:
-            continue
-        if state is not None:
-            s.set_state(*state)
-        if membranes:
-            s.add_membranes(membranes, ibm_kernel=abi.IBM_COS4 if ibm_kernel is None else ibm_kernel)
-        s.step(warmup)
-        s.sync()
-        ms = s.time_steps(steps) / steps
-        s.close()
-        table.append(((tx, ty, tz), nt, ms))
-        if best is None or ms < best[1]:
-            best = (p, ms)
+        for fused in (True, False):
+            try:
+                s = abi.Solver(mesh, p)
+            except abi.LbfsError:
+                break  # the tile does not fit
+            try:
+                s.set_fused(fused)
+            except abi.LbfsError:
+                s.close()  # too large for the fused pass
+                continue
+            if state is not None:
+                s.set_state(*state)
+            if membranes:
+                s.add_membranes(membranes, ibm_kernel=abi.IBM_COS4 if ibm_kernel is None else ibm_kernel)
+            s.step(warmup)
+            s.sync()
+            ms = s.time_steps(steps) / steps
+            s.close()
+            table.append(((tx, ty, tz), nt, fused, ms))
+            if best is None or ms < best[2]:
+                best = (p, fused, ms)
     if best is None:
         raise abi.LbfsError("no tile candidate fits this mesh")
-    return best[0], table
+    return best[0], best[1], table

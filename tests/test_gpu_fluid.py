@@ -265,10 +265,11 @@ def test_autotuner_picks_a_valid_candidate():
     from lbfs_amd import autotune
     m = cases.channel_mesh(32, 16, 16)
     p = abi.make_params(lattice=abi.D3Q19, visc=VISC, dt=DT)
-    best, table = autotune.tune(m, p, steps=4, warmup=2)
+    best, fused, table = autotune.tune(m, p, steps=4, warmup=2)
     assert len(table) >= 3
-    assert tuple(best.tile) in [t for t, _, _ in table]
+    assert (tuple(best.tile), best.block_threads, fused) in [(t, nt, f) for t, nt, f, _ in table]
     assert best.block_threads in (256, 512)
+    assert any(f for _, _, f, _ in table) and not all(f for _, _, f, _ in table)
 
 
 def test_restart_files_continue_the_run(tmp_path):
@@ -288,3 +289,31 @@ def test_restart_files_continue_the_run(tmp_path):
     assert np.array_equal(np.concatenate(a.get_state()), np.concatenate(b.get_state()))
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("lat_a,lat_o", cases.LAT_PAIRS)
+@pytest.mark.parametrize("flags", [0, abi.FLAGS_REFERENCE_GPU | abi.FLAG_NEUMANN_SIGNED])
+@pytest.mark.parametrize("tile,threads", [((8, 8, 8), 512), ((8, 8, 4), 256), ((4, 4, 16), 256)])
+def test_fused_fluid_pass_equals_two_passes(lat_a, lat_o, flags, tile, threads):
+    """Whole steps with the fused fluid pass (each tile's gradients in LDS,
+    the cells across its border recomputed) equal the gradient pass + face
+    pass bit for bit, with local time steps too."""
+    if lat_a == abi.D2Q9:
+        m = abi.HexMesh(20, 12, 1, periodic=(1, 0, 1), jitter=0.1, seed=2)
+    else:
+        m = cases.channel_mesh(20, 12, 10)
+    st = cases.smooth_state(m, seed=15)
+    outs = []
+    for fused in (True, False):
+        for ts in (0, abi.TS_LOCAL):
+            p = abi.make_params(lattice=lat_a, visc=VISC, dt=DT if ts == 0 else 0.4, flags=flags,
+                                body_force=(2e-5, 0.0, 0.0), time_stepping=ts, max_velocity=0.02,
+                                tile=tile, block_threads=threads)
+            s = abi.Solver(m, p)
+            s.set_fused(fused)
+            s.set_state(*st)
+            s.step(7)
+            outs.append(np.concatenate(s.get_state()))
+            s.close()
+    assert np.array_equal(outs[0], outs[2]) and np.array_equal(outs[1], outs[3])
+    assert np.all(np.isfinite(outs[0])) and not np.array_equal(outs[0], outs[1])

@@ -380,3 +380,26 @@ def test_binned_spreading_across_a_ragged_periodic_axis(nx):
         s.close()
     for a, b in zip(outs[0], outs[1]):
         assert np.array_equal(a, b)
+
+
+def test_fused_fluid_pass_coupled_step_equals_two_passes():
+    """The coupled step with the fused fluid pass (the default) equals the
+    one with the gradient and face passes bit for bit, graph and eager."""
+    outs = []
+    for fused in (True, False):
+        for graph in (True, False):
+            mesh = cases.channel_mesh(24, 16, 16, seed=14)
+            s = abi.Solver(mesh, abi.make_params(visc=VISC, dt=DT, body_force=(2e-5, 0.0, 0.0), tile=(8, 8, 8),
+                                                 block_threads=512))
+            s.set_fused(fused)
+            s.set_graph(graph)
+            s.set_state(*cases.smooth_state(mesh, seed=14, amp=0.005))
+            x, t = abi.rbc(4, 8.0, center=(12.0, 8.0, 8.0))
+            s.add_membranes([abi.Membrane(x, t, **PARAMS)], ibm_kernel=abi.IBM_COS4)
+            s.step(8)
+            X, U, F = s.membrane_nodes()
+            outs.append((np.concatenate(s.get_state()), s.cell_force(), X, F))
+            s.close()
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
