@@ -251,7 +251,13 @@ typedef struct {
     int n_springs;                    /* 0: springs built from the triangles (lagrangian_object.cpp:322-490) */
     const int *springs;               /* [4 n_springs] a, b, c, d as read from an MZ file (mesh_read) */
     int block;                        /* box of the ensemble the membrane lives in (0 for one box) */
+    int spring_law;                   /* LBFS_SPRING_WLC (0) or LBFS_SPRING_WLC_POW */
 } lbfs_membrane_desc;
+/* spring laws: the WLC of get_spring_forces_atomics (spring_network.cu:631-633, 932-936), or
+ * WLC-POW, -(k_s (1/(1-x)^2 - 1 + 4x) + k_p / L^2) (spring_network.cu:221-223) with k_s, k_p from
+ * the shear modulus and the rest length (get_spring_pow_constants, spring_network.cu:1655-1682) */
+#define LBFS_SPRING_WLC 0
+#define LBFS_SPRING_WLC_POW 1
 
 /* attach membranes (replaces any attached before); requires the mesh's logical
  * structure (nx, ny, nz, origin, spacing); ibm_kernel = LBFS_IBM_*.

@@ -92,7 +92,8 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
             const int *q = &s.springs[4 * (size_t)e];
             edges.push_back(make_int4(nb + q[0], nb + q[1], nb + q[2], nb + q[3]));
             edge_mem.push_back(i);
-            edge_const.push_back(make_float4((float)s.contour_length[e], (float)s.spring_constant[e], (float)s.curvature_0[e], 0.0f));
+            edge_const.push_back(make_float4((float)s.contour_length[e], (float)s.spring_constant[e],
+                                             (float)s.curvature_0[e], (float)s.pow_ratio[e]));
             edge_area0.push_back(make_float2((float)s.spring_area_0[2 * (size_t)e], (float)s.spring_area_0[2 * (size_t)e + 1]));
             edge_state.push_back(make_float4((float)s.curvature[e], (float)s.spring_area[2 * (size_t)e],
                                              (float)s.spring_area[2 * (size_t)e + 1], 0.0f));
@@ -104,6 +105,7 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
         MemParamDev &p = params[i];
         p.spring_c = (float)(0.25 * r * r / ((r - 1.0) * (r - 1.0)) - 0.25 + 1.0 / r);
         p.wlc_x0 = (float)(1.0 / r);
+        p.spring_law = mem[i].spring_law;
         p.volume_modulus = (float)mem[i].volume_modulus;
         p.local_area_modulus = (float)mem[i].local_area_modulus;
         p.global_area_modulus = (float)mem[i].global_area_modulus;

@@ -18,6 +18,7 @@ struct MemParamDev {
     int n_mad;                // edges summed for MAD
     float visc_ratio;         // internal viscosity ratio (1 = off)
     float wlc_x0;             // 1 / wlc_ratio: rest length over contour length
+    int spring_law;           // LBFS_SPRING_WLC or LBFS_SPRING_WLC_POW
 };
 
 struct MemArgs {
@@ -31,7 +32,7 @@ struct MemArgs {
     const int *mem_tri_start, *mem_edge_start, *mem_node_start;
     const int4 *edges;        // a, b, c, d (edge a-c)
     const int *edge_mem;
-    const float4 *edge_const; // contour length, spring constant, curvature_0, 0
+    const float4 *edge_const; // contour length, spring constant, curvature_0, k_p / k_s (WLC-POW; 0 with WLC)
     const float2 *edge_area0; // |N1|, |N2| of the reference state
     float4 *edge_state;       // curvature, |N1|, |N2|, 0 (feeds the MAD of the next step)
     float4 *edge_force;       // 4 per edge: forces on a, b, c, d

@@ -153,7 +153,8 @@ __device__ __forceinline__ memf::EdgeConst edge_consts(const MemArgs &m, const M
 {
     const float4 ec = m.edge_const[e];
     const float2 a0 = m.edge_area0[e];
-    return {ec.x, ec.z, a0.x, a0.y, p.local_area_modulus, p.viscosity, p.spring_c};
+    return {ec.x, ec.z, a0.x, a0.y, p.local_area_modulus, p.viscosity, p.spring_c, ec.w,
+            p.spring_law == LBFS_SPRING_WLC_POW ? 1 : 0};
 }
 
 }  // namespace

@@ -440,7 +440,8 @@ __device__ __forceinline__ memf::EdgeConst edge_const(const MemArgs &m, const Me
 {
     const float4 ec = m.edge_const[e];
     const float2 a0 = m.edge_area0[e];
-    return {ec.x, ec.z, a0.x, a0.y, p.local_area_modulus, p.viscosity, p.spring_c};
+    return {ec.x, ec.z, a0.x, a0.y, p.local_area_modulus, p.viscosity, p.spring_c, ec.w,
+            p.spring_law == LBFS_SPRING_WLC_POW ? 1 : 0};
 }
 
 
@@ -656,7 +657,15 @@ __global__ __launch_bounds__(256) void k_mem_energy(MemArgs m, double *out)
         const float4 st = m.edge_state[e];
         const float2 a0 = m.edge_area0[e];
         const double Lm = kc.x, x = (double)sqrtf(dot(l, l)) / Lm;
-        es += 0.5 * (double)kc.y * fabs(0.25 * Lm * (x * x * (3.0 - 2.0 * x) / (1.0 - x) - 4.0 * x * c1 - c2));
+        if (p.spring_law == LBFS_SPRING_WLC_POW) {
+            // no energy routine in the reference: the integral of the WLC-POW
+            // force from the rest length, k_s Lm [g(x) - g(x0)] - k_p (1/L - 1/L0),
+            // g(x) = 1/(1-x) - x + 2x^2
+            const double g = 1.0 / (1.0 - x) - x + 2.0 * x * x, g0 = 1.0 / (1.0 - x0) - x0 + 2.0 * x0 * x0;
+            es += (double)kc.y * Lm * (g - g0) - (double)kc.y * (double)kc.w * (1.0 / (x * Lm) - 1.0 / (x0 * Lm));
+        } else {
+            es += 0.5 * (double)kc.y * fabs(0.25 * Lm * (x * x * (3.0 - 2.0 * x) / (1.0 - x) - 4.0 * x * c1 - c2));
+        }
         const double d1 = (double)st.y - a0.x, d2 = (double)st.z - a0.y;
         ela += (double)p.global_area_modulus * 0.5 * (d1 * d1 / a0.x + d2 * d2 / a0.y);
         const int flat = e - e0;                       // entry of the membrane's 2E |N| array

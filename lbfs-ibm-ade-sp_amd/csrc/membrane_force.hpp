@@ -159,6 +159,8 @@ struct EdgeConst {
     float lmax, kappa0;               // contour length, reference curvature
     float a01, a02;                   // reference |N1|, |N2|
     float la_mod, visc, spring_c;     // local area modulus, membrane viscosity, WLC constant
+    float kp_ks;                      // WLC-POW: k_p / k_s (the POW term scales with the spring constant)
+    int pow_law;                      // 1: WLC-POW spring, 0: WLC
 };
 
 template <class T>
@@ -225,11 +227,14 @@ __device__ __forceinline__ void edge_force(const EdgeIn<T> &in, const EdgeConst 
         Fo[j] = Fo[j] + v3(g.d_area[3 * j], g.d_area[3 * j + 1], g.d_area[3 * j + 2]) * lbc;
         Fo[j] = Fo[j] + v3(d_curve[3 * j], d_curve[3 * j + 1], d_curve[3 * j + 2]) * ccf;
     }
-    // worm-like chain spring
+    // spring
     {
         const T x = length / k.lmax;
         const T om = 1.0f - x;
-        const T fc = -in.ks * (0.25f / (om * om) - 0.25f + x - k.spring_c);
+        // worm-like chain (spring_network.cu:932-936), or WLC-POW
+        // -(k_s (1/(1-x)^2 - 1 + 4x) + k_p / L^2) (spring_network.cu:221-223)
+        const T fc = k.pow_law ? -in.ks * (1.0f / (om * om) - 1.0f + 4.0f * x + k.kp_ks * (il * il))
+                               : -in.ks * (0.25f / (om * om) - 0.25f + x - k.spring_c);
         Fo[2] = Fo[2] + dL * fc;
         Fo[0] = Fo[0] - dL * fc;
     }

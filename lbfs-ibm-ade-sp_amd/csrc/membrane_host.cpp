@@ -166,14 +166,29 @@ int membrane_setup(const lbfs_membrane_desc &m, MembraneSetup &s, std::string &e
     // contour lengths and spring constants (get_contour_lengths, get_spring_constant)
     const double r = m.wlc_ratio;
     const double cons = 0.5 * std::pow(1.0 - 1.0 / r, -3.0) + 1.0;
+    if (m.spring_law != LBFS_SPRING_WLC && m.spring_law != LBFS_SPRING_WLC_POW) {
+        err = "unknown spring law " + std::to_string(m.spring_law);
+        return LBFS_E_INVALID_ARG;
+    }
     s.contour_length.resize(ne);
     s.spring_constant.resize(ne);
+    s.pow_ratio.assign(ne, 0.0);
     for (int e = 0; e < ne; e++) {
         const int *q = &s.springs[4 * (size_t)e];
         const P3 l = sub(at(Xsf, q[2]), at(Xsf, q[0]));
         const double L = std::sqrt(dot(l, l));
         s.spring_constant[e] = 4.0 * r * L * m.shear_modulus / (std::sqrt(3.0) * cons);
         s.contour_length[e] = L * r;
+        if (m.spring_law == LBFS_SPRING_WLC_POW) {
+            // get_spring_pow_constants (spring_network.cu:1655-1682) on the rest length
+            const double x = 1.0 / r;
+            const double alpha = std::sqrt(3.0) / L;
+            const double beta = 0.5 * x / std::pow(1.0 - x, 3.0) - 0.25 / std::pow(1.0 - x, 2.0) + 0.25;
+            const double gamma = 0.75 * std::sqrt(3.0) / (L * L * L);
+            const double beta2 = 1.0 / std::pow(1.0 - x, 2.0) - 1.0 + 4.0 * x;
+            s.spring_constant[e] = m.shear_modulus / (alpha * beta + gamma * L * L * beta2);
+            s.pow_ratio[e] = -L * L * beta2;  // k_p = -L^2 k_s beta2: no force at the rest length
+        }
     }
     // reference curvature (get_reference_curvatures)
     s.curvature_0.assign(ne, m.sphere_reference ? m.curvature_ratio / 3.265483301 : 0.0);

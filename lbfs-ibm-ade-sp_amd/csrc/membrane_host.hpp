@@ -28,6 +28,7 @@ struct MembraneSetup {
     std::vector<int> tri, springs;
     std::vector<double> X;                 // initial positions
     std::vector<double> contour_length, spring_constant, curvature_0;
+    std::vector<double> pow_ratio;         // WLC-POW: k_p / k_s per edge (0 with WLC)
     std::vector<double> spring_area_0;     // 2 per edge
     std::vector<double> curvature, spring_area;  // state after the initial call (2 per edge for areas)
     double V0 = 0, A0 = 0, MAD0 = 0, V = 0, A = 0, MAD = 0;

@@ -65,7 +65,7 @@ class MembraneDesc(C.Structure):
         ("global_bending_modulus", C.c_double), ("local_bending_modulus", C.c_double),
         ("viscosity", C.c_double), ("curvature_ratio", C.c_double), ("volume_ratio", C.c_double),
         ("sphere_reference", C.c_int), ("mad_compat", C.c_int), ("internal_viscosity_ratio", C.c_double),
-        ("n_springs", C.c_int), ("springs", _ip), ("block", C.c_int),
+        ("n_springs", C.c_int), ("springs", _ip), ("block", C.c_int), ("spring_law", C.c_int),
     ]
 
 
@@ -101,13 +101,14 @@ class Case(C.Structure):
 
 MEMBRANE_PARAM_NAMES = ("shear_modulus", "wlc_ratio", "volume_modulus", "local_area_modulus", "global_area_modulus",
                         "global_bending_modulus", "local_bending_modulus", "viscosity", "curvature_ratio",
-                        "volume_ratio", "sphere_reference", "mad_compat", "internal_viscosity_ratio")
+                        "volume_ratio", "sphere_reference", "mad_compat", "internal_viscosity_ratio", "spring_law")
+SPRING_WLC, SPRING_WLC_POW = 0, 1
 
 # lattice-unit membrane parameters for a 20-cell RBC in nu = 1/6 flow
 RBC_PARAMS = dict(shear_modulus=5e-3, wlc_ratio=2.2, volume_modulus=0.1, local_area_modulus=5e-3,
                   global_area_modulus=0.1, global_bending_modulus=1e-3, local_bending_modulus=0.0, viscosity=0.0,
                   curvature_ratio=1.0, volume_ratio=1.0, sphere_reference=0, mad_compat=0,
-                  internal_viscosity_ratio=1.0)
+                  internal_viscosity_ratio=1.0, spring_law=0)
 
 _lib = None
 

@@ -133,6 +133,8 @@ typedef struct {
     double global_bending_modulus, local_bending_modulus, viscosity, curvature_ratio, volume_ratio;
     int sphere_reference;
     int mad_compat;   /* 1: MAD summed over the first n_nodes edges (Solver_gpu.cu:2355) */
+    int spring_law;   /* 0: WLC (get_spring_forces_atomics); 1: WLC-POW (spring_network.cu:221-223, constants
+                         of get_spring_pow_constants :1655-1682) */
 } or_mem_params;
 
 typedef struct {
@@ -144,6 +146,7 @@ typedef struct {
     double *curvature, *spring_area /* [2E] */;
     /* globals */
     double total_volume, total_volume_0, total_area, total_area_0, mad, mad_0;
+    double *pow_constant;  /* [n_edges] POW constant k_p of the WLC-POW law (unused with WLC) */
 } or_membrane;
 
 int or_build_springs(int n_nodes, int n_tri, const int *tri, int *spring_out /* [4 * 3 n_tri / 2] */);

@@ -243,9 +243,15 @@ static void edge_forces(or_membrane *mb, const or_mem_params *mp, const double *
         for (j = 0; j < 4; j++) Fo[j] = add(Fo[j], scl(mk(d_area[3 * j], d_area[3 * j + 1], d_area[3 * j + 2]), lbc));
         cc = -mp->global_bending_modulus * (kappa - mb->curvature_0[e]) + mad_coef;
         for (j = 0; j < 4; j++) Fo[j] = add(Fo[j], scl(mk(d_curve[3 * j], d_curve[3 * j + 1], d_curve[3 * j + 2]), cc));
-        /* worm-like-chain spring */
         x = length / mb->contour_length[e];
-        fcoef = -mb->spring_constant[e] * (0.25 / ((1.0 - x) * (1.0 - x)) - 0.25 + x - spring_c);
+        if (mp->spring_law == 1) {
+            /* WLC-POW spring (spring_network.cu:221-223): -(k_s (1/(1-x)^2 - 1 + 4x) + k_p / L^2) */
+            fcoef = -(mb->spring_constant[e] * (1.0 / ((1.0 - x) * (1.0 - x)) - 1.0 + 4.0 * x) +
+                      mb->pow_constant[e] / (length * length));
+        } else {
+            /* worm-like-chain spring */
+            fcoef = -mb->spring_constant[e] * (0.25 / ((1.0 - x) * (1.0 - x)) - 0.25 + x - spring_c);
+        }
         Fo[2] = add(Fo[2], scl(dL, fcoef));
         Fo[0] = sub(Fo[0], scl(dL, fcoef));
     }
@@ -305,6 +311,19 @@ void or_membrane_setup(or_membrane *mb, const or_mem_params *mp, const double *X
         double L = sqrt(dot(l, l));
         mb->spring_constant[e] = 4.0 * r * L * mp->shear_modulus / (sqrt(3.0) * cons);
         mb->contour_length[e] = L * r;
+        if (mp->spring_law == 1) {
+            /* get_spring_pow_constants (spring_network.cu:1655-1682) on the rest
+             * length L: the linear shear modulus of the network equals the
+             * membrane's, and the force vanishes at L */
+            double x = 1.0 / r;
+            double alpha = sqrt(3.0) / L;
+            double beta = x * 0.5 / pow(1.0 - x, 3.0) - 0.25 / pow(1.0 - x, 2.0) + 0.25;
+            double gamma = sqrt(3.0) * 3.0 * 0.25 / pow(L, 3.0);
+            double beta2 = 1.0 / pow(1.0 - x, 2.0) - 1.0 + 4.0 * x;
+            double ks = mp->shear_modulus / (alpha * beta + gamma * L * L * beta2);
+            mb->spring_constant[e] = ks;
+            mb->pow_constant[e] = -L * L * ks * beta2;
+        }
     }
     for (e = 0; e < mb->n_edges; e++) {
         const int *s = mb->spring + 4 * e;
@@ -361,8 +380,17 @@ void or_membrane_energies(const or_membrane *mb, const or_mem_params *mp, const 
         e_la += mp->global_area_modulus * 0.5 *
                 (a1 * a1 / mb->spring_area_0[2 * e] + a2 * a2 / mb->spring_area_0[2 * e + 1]);
         e_lb += 0.5 * mp->local_bending_modulus * dk * dk / mb->spring_area[e];
-        e_spring += 0.5 * mb->spring_constant[e] *
-                    fabs(0.25 * Lm * (x * x * (3.0 - 2.0 * x) / (1.0 - x) - 4.0 * x * e1 - e2));
+        if (mp->spring_law == 1) {
+            /* WLC-POW (no energy routine in the reference): the integral of
+             * the spring force from the rest length L0 = x0 Lm,
+             * k_s Lm [g(x) - g(x0)] - k_p (1/L - 1/L0), g(x) = 1/(1-x) - x + 2x^2 */
+            double L = x * Lm, L0 = x0 * Lm;
+            double g = 1.0 / (1.0 - x) - x + 2.0 * x * x, g0 = 1.0 / (1.0 - x0) - x0 + 2.0 * x0 * x0;
+            e_spring += mb->spring_constant[e] * Lm * (g - g0) - mb->pow_constant[e] * (1.0 / L - 1.0 / L0);
+        } else {
+            e_spring += 0.5 * mb->spring_constant[e] *
+                        fabs(0.25 * Lm * (x * x * (3.0 - 2.0 * x) / (1.0 - x) - 4.0 * x * e1 - e2));
+        }
     }
     out[0] = e_spring;
     out[1] = e_la;

@@ -53,7 +53,7 @@ class OrMemParams(C.Structure):
                 ("local_area_modulus", C.c_double), ("global_area_modulus", C.c_double),
                 ("global_bending_modulus", C.c_double), ("local_bending_modulus", C.c_double),
                 ("viscosity", C.c_double), ("curvature_ratio", C.c_double), ("volume_ratio", C.c_double),
-                ("sphere_reference", C.c_int), ("mad_compat", C.c_int)]
+                ("sphere_reference", C.c_int), ("mad_compat", C.c_int), ("spring_law", C.c_int)]
 
 
 class OrMembrane(C.Structure):
@@ -61,7 +61,7 @@ class OrMembrane(C.Structure):
                 ("contour_length", _dp), ("spring_constant", _dp), ("curvature_0", _dp), ("spring_angle_0", _dp),
                 ("spring_area_0", _dp), ("curvature", _dp), ("spring_area", _dp),
                 ("total_volume", C.c_double), ("total_volume_0", C.c_double), ("total_area", C.c_double),
-                ("total_area_0", C.c_double), ("mad", C.c_double), ("mad_0", C.c_double)]
+                ("total_area_0", C.c_double), ("mad", C.c_double), ("mad_0", C.c_double), ("pow_constant", _dp)]
 
 
 _lib = None
@@ -256,7 +256,7 @@ class OracleFluid:
 
 MEM_KEYS = ("shear_modulus", "wlc_ratio", "volume_modulus", "local_area_modulus", "global_area_modulus",
             "global_bending_modulus", "local_bending_modulus", "viscosity", "curvature_ratio", "volume_ratio",
-            "sphere_reference", "mad_compat")
+            "sphere_reference", "mad_compat", "spring_law")
 
 
 def build_springs(n_nodes, tri):
@@ -284,6 +284,7 @@ class OracleMembrane:
         self.ne = ne
         self.arr = {k: np.zeros(ne) for k in ("contour_length", "spring_constant", "curvature_0", "spring_angle_0",
                                                "curvature")}
+        self.arr["pow_constant"] = np.zeros(ne)
         self.arr["spring_area_0"] = np.zeros(2 * ne)
         self.arr["spring_area"] = np.zeros(2 * ne)
         mb = OrMembrane()
@@ -294,7 +295,7 @@ class OracleMembrane:
         self.mb = mb
         mp = OrMemParams()
         for k in MEM_KEYS:
-            setattr(mp, k, params[k])
+            setattr(mp, k, params.get(k, 0))
         self.mp = mp
         sf = self.X if nodes_stress_free is None else np.ascontiguousarray(nodes_stress_free, dtype=np.float64).reshape(-1)
         cv = sf if nodes_curvature is None else np.ascontiguousarray(nodes_curvature, dtype=np.float64).reshape(-1)

@@ -403,3 +403,37 @@ def test_fused_fluid_pass_coupled_step_equals_two_passes():
     for o in outs[1:]:
         for a, b in zip(outs[0], o):
             assert np.array_equal(a, b)
+
+
+def test_pow_springs_coupled_parity_and_energy():
+    """WLC-POW springs (spring_network.cu:221-223, 1655-1682): node forces
+    after one step within 1e-4 and 40 coupled steps within 1e-5 (velocity,
+    nodes) of the FP64 oracle; the spring energy within 1e-5."""
+    p = dict(PARAMS, spring_law=abi.SPRING_WLC_POW)
+    mesh = cases.channel_mesh(24, 16, 16, seed=22)
+    xs, tri = abi.rbc(4, 8.0, center=(12.0, 8.0, 8.0), rot=abi.rotation((1, 1, 0), 0.4))
+    c = np.array([12.0, 8.0, 8.0])
+    x0 = c + (xs - c) * np.array([1.08, 0.96, 0.96])   # stretched: the springs carry force
+    st = cases.smooth_state(mesh, seed=22, amp=0.005)
+    s = abi.Solver(mesh, abi.make_params(lattice=abi.D3Q19, visc=VISC, dt=DT, body_force=(2e-5, 0.0, 0.0)))
+    s.set_state(*st)
+    s.add_membranes([abi.Membrane(x0, tri, nodes_stress_free=xs, **p)], ibm_kernel=abi.IBM_COS4)
+    fl = orc.OracleFluid(mesh, orc.OR_D3Q19, visc=VISC, dt=DT)
+    fl.set_state(*st)
+    mo = orc.OracleMembrane(x0, tri, p, nodes_stress_free=xs)
+    cp = orc.OracleCoupled(fl, mesh, [mo], abi.IBM_COS4, body_force=(2e-5, 0.0, 0.0))
+    s.step(1)
+    cp.step(1)
+    X, U, F = s.membrane_nodes()
+    assert cases.rel_l2(F, mo.F.reshape(-1, 3)) < 1e-4
+    eg = s.membrane_energies()[0]
+    eo = mo.energies()
+    assert abs(eg[0] - eo[0]) < 1e-5 * abs(eo[0]), (eg[0], eo[0])
+    s.step(39)
+    cp.step(39)
+    X, _, _ = s.membrane_nodes()
+    _, us, vs, ws = s.get_state()
+    _, uo, vo, wo = fl.state()
+    assert cases.rel_l2(np.concatenate([us, vs, ws]), np.concatenate([uo, vo, wo])) < 1e-5
+    assert cases.rel_l2(X, mo.X.reshape(-1, 3)) < 1e-5
+    assert cases.rel_l2(X - x0, mo.X.reshape(-1, 3) - x0) < 1e-4

@@ -536,3 +536,68 @@ def test_sphere_reference_bending_uses_the_spherical_curvature():
     # without it the spontaneous curvature is zero and the force differs
     m0 = orc.OracleMembrane(x0, t, dict(params, sphere_reference=0))
     assert np.max(np.abs(m0.forces(X).reshape(-1) - F)) > 1e-3 * np.max(np.abs(F))
+
+
+POW = dict(NO_BEND, shear_modulus=5e-3, wlc_ratio=2.2, volume_modulus=0.0, local_area_modulus=0.0,
+           global_area_modulus=0.0, spring_law=1)
+
+
+def test_pow_spring_constants_give_the_published_shear_modulus():
+    """WLC-POW constants (get_spring_pow_constants, spring_network.cu:1655-1682):
+    the force vanishes at the rest length, k_p < 0 (a repulsive POW term in
+    the reference's sign), and the linear shear modulus of the network in the
+    published WLC-POW form (Fedosov, Caswell & Karniadakis 2010, with
+    k_B T / p = 4 k_s and exponent m = 2),
+    mu0 = sqrt3 k_B T / (4 p L_max x0) (x0 / (2 (1-x0)^3) - 1 / (4 (1-x0)^2) + 1/4)
+          + sqrt3 (m + 1) |k_p| / (4 L0^(m+1)),
+    equals the membrane's shear modulus on every spring."""
+    x0, t = small_rbc(freq=4, d0=8.0)
+    m = orc.OracleMembrane(x0, t, POW)
+    sp = m.spring.reshape(-1, 4)
+    L0 = np.linalg.norm(x0[sp[:, 2]] - x0[sp[:, 0]], axis=1)
+    ks, kp, Lm = m.arr["spring_constant"], m.arr["pow_constant"], m.arr["contour_length"]
+    r = POW["wlc_ratio"]
+    x = 1.0 / r
+    np.testing.assert_allclose(Lm, r * L0, rtol=1e-14)
+    assert np.all(kp < 0)
+    kbt_p = 4.0 * ks
+    mu0 = (np.sqrt(3.0) * kbt_p / (4.0 * Lm * x) * (x / (2 * (1 - x) ** 3) - 1 / (4 * (1 - x) ** 2) + 0.25)
+           + np.sqrt(3.0) * 3.0 * np.abs(kp) / (4.0 * L0 ** 3))
+    np.testing.assert_allclose(mu0, POW["shear_modulus"], rtol=1e-12)
+    # no force at the rest length: the stress-free shape is force free
+    F = m.forces(x0)
+    assert np.abs(F).max() < 1e-12 * np.abs(ks).max()
+
+
+def test_pow_spring_force_is_minus_the_energy_gradient():
+    """The WLC-POW force -(k_s (1/(1-x)^2 - 1 + 4x) + k_p / L^2) along the
+    spring (spring_network.cu:221-223) is -dE/dX of its energy, with
+    E = k_s L_max (g(x) - g(x0)) - k_p (1/L - 1/L0), g(x) = 1/(1-x) - x + 2x^2,
+    by central differences; and it differs from the WLC law."""
+    x0, t = small_rbc(freq=3, d0=6.0)
+    m = orc.OracleMembrane(x0, t, POW)
+    X = x0 + 0.03 * np.random.default_rng(16).standard_normal(x0.shape)
+    F = m.forces(X).reshape(-1)
+    sp = m.spring.reshape(-1, 4)
+    ks, kp, Lm = m.arr["spring_constant"], m.arr["pow_constant"], m.arr["contour_length"]
+    L0 = Lm / POW["wlc_ratio"]
+
+    def E(xf):
+        Xr = xf.reshape(-1, 3)
+        L = np.linalg.norm(Xr[sp[:, 2]] - Xr[sp[:, 0]], axis=1)
+        g = lambda s: 1 / (1 - s) - s + 2 * s * s  # noqa: E731
+        return np.sum(ks * Lm * (g(L / Lm) - g(L0 / Lm)) - kp * (1 / L - 1 / L0))
+
+    Xf = X.reshape(-1).copy()
+    h = 1e-6
+    for j in np.random.default_rng(17).choice(len(Xf), 25, replace=False):
+        xp, xm = Xf.copy(), Xf.copy()
+        xp[j] += h
+        xm[j] -= h
+        g = (E(xp) - E(xm)) / (2 * h)
+        assert abs(-g - F[j]) < 1e-11 + 1e-6 * np.abs(F).max(), (j, -g, F[j])
+    # the energy routine reports the same energy
+    m.forces(X)
+    np.testing.assert_allclose(m.energies(X)[0], E(Xf), rtol=1e-10)
+    w = orc.OracleMembrane(x0, t, dict(POW, spring_law=0))
+    assert np.abs(w.forces(X).reshape(-1) - F).max() > 0.05 * np.abs(F).max()
