@@ -15,6 +15,9 @@ REPO = os.path.dirname(ROOT)
 CSRC = os.path.join(ROOT, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG_DIR, "liblbfs_amd.so")
+# host check programs: the C++ mirror of the solver class and the
+# reference-signature step entry, each built from host/<name>.cpp
+HOST_CHECKS = ("gpu_solver_check", "reference_entry_check")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [
@@ -51,17 +54,20 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-8000:]))
     os.replace(tmp, LIB)
-    # the C++ host mirror's check program (plain host C++ against the C ABI)
+    # the C++ host mirror's check programs (plain host C++ against the C ABI)
     host = os.path.join(ROOT, "host")
-    exe = os.path.join(host, "gpu_solver_check")
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + host, "-I" + os.path.join(REPO, "include"),
-           os.path.join(host, "gpu_solver_check.cpp"), "-L" + PKG_DIR, "-llbfs_amd",
-           "-Wl,-rpath,$ORIGIN/../lbfs_amd", "-o", exe]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("g++ failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-8000:]))
+    exes = []
+    for name in HOST_CHECKS:
+        exe = os.path.join(host, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + host, "-I" + os.path.join(REPO, "include"),
+               os.path.join(host, name + ".cpp"), "-L" + PKG_DIR, "-llbfs_amd",
+               "-Wl,-rpath,$ORIGIN/../lbfs_amd", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("g++ failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-8000:]))
+        exes.append(exe)
     if verbose:
-        print("built", LIB, "and", exe)
+        print("built", LIB, "and", ", ".join(exes))
     return LIB
 
 

@@ -186,6 +186,24 @@ def test_cpp_host_mirror_builds_and_fails_cleanly_without_device():
     assert r.returncode in (0, 3), r.stdout + r.stderr
 
 
+def test_reference_signature_entry_compiles_and_writes_its_case(tmp_path):
+    """The reference-signature step entry (host/reference_entry.hpp) compiles
+    against stand-in classes carrying the reference's member names; without a
+    HIP device its check program writes the polyMesh case and the cell's mesh
+    file, then stops at lbfs_create with LBFS_E_NO_DEVICE (exit 3)."""
+    import subprocess
+    exe = os.path.join(REPO, "lbfs-ibm-ade-sp_amd", "host", "reference_entry_check")
+    assert os.path.exists(exe)
+    case = tmp_path / "case"
+    r = subprocess.run([exe, str(case)], capture_output=True, text=True, timeout=120)
+    assert r.returncode in (0, 3), r.stdout + r.stderr
+    for f in ("points", "faces", "owner", "neighbour", "boundary", "lbfsFaceShift", "cell.dat"):
+        assert (case / f).exists(), f
+    m = abi.HexMesh.read_polymesh(case, 16, 12, 12)
+    assert m.n_cells == 16 * 12 * 12 and m.patch_names == ("bottom", "top", "back", "front")
+    assert list(m.periodic) == [1, 0, 0]
+
+
 def test_state_files_in_the_reference_format(tmp_path):
     """Restart files: one "i ,value" line per cell in caller order
     (Solution::output); 17 digits round-trip exactly, the default 6 digits

@@ -1,5 +1,8 @@
 """GPU parity of the coupled step (IBM interpolation/advection, spring-network
 membrane forces, spreading, fluid) against the FP64 oracle, through the C ABI."""
+import os
+import subprocess
+
 import numpy as np
 import pytest
 
@@ -183,9 +186,6 @@ def test_named_stages_equal_whole_steps():
     groups) gives bitwise the same state as lbfs_step; the C++ host mirror's
     check program does the same through gpu_solver::General_Purpose_Solver_mk_i
     and the reference-named group wrappers."""
-    import os
-    import subprocess
-
     def build():
         mesh = cases.channel_mesh(24, 16, 16, seed=9)
         x0, tri = abi.rbc(4, 8.0, center=(12.0, 8.0, 8.0))
@@ -209,6 +209,20 @@ def test_named_stages_equal_whole_steps():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "groups equal whole steps" in r.stdout
+
+
+def test_reference_signature_entry_equals_direct_abi_run(tmp_path):
+    """General_Purpose_Solver_mk_i with the reference's parameter list
+    (host/reference_entry.hpp), called on stand-ins of the reference's classes
+    with a polyMesh case, a cell read from its Osay mesh file and the
+    pressure-gradient body force, gives bit for bit the run made through the C
+    ABI directly, calls the output hook every output_step steps, and stops
+    after one step when the tolerance is above the first residual."""
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lbfs-ibm-ade-sp_amd", "host",
+                       "reference_entry_check")
+    r = subprocess.run([exe, str(tmp_path / "case")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "direct ABI run equal" in r.stdout and "stop after 1 step" in r.stdout
 
 
 def test_ensemble_equals_members_run_alone():
