@@ -12,7 +12,6 @@ a CPU baseline (the FP64 oracle on a bounded sample of the same workload).
 import argparse
 import json
 import os
-import subprocess
 import sys
 import time
 
@@ -28,28 +27,90 @@ METRIC = "MLUPS + achieved HBM GB/s, 1 GPU; batched-RBC and adjoint step vs plai
 HBM_PEAK_GBPS = 8000.0
 
 
-def cpu_baseline(mesh, params, steps):
-    """FP64 oracle (single thread) on `steps` steps of the same mesh."""
+def rel_l2(a, b):
+    a, b = np.asarray(a, dtype=np.float64).ravel(), np.asarray(b, dtype=np.float64).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def oracle_coupled(mesh, params, state, mems, body_force=None):
+    """The FP64 oracle of the coupled step (fluid + membranes + 4-point IBM)."""
+    from lbfs_amd import abi
     from oracle import oracle as orc
-    o = orc.OracleFluid(mesh, orc.OR_D3Q19, visc=params.visc, dt=params.dt)
-    n = mesh.n_cells
-    o.set_state(np.ones(n), np.zeros(n), np.zeros(n), np.zeros(n))
-    force = np.tile(np.array([params.body_force[0], params.body_force[1], params.body_force[2]]), n)
+    fl = orc.OracleFluid(mesh, orc.OR_D3Q19, visc=params.visc, dt=params.dt)
+    fl.set_state(*state)
+    mos = [orc.OracleMembrane(m.nodes, m.tri, m.params, nodes_stress_free=m.sf, nodes_curvature=m.cv) for m in mems]
+    bf = tuple(params.body_force) if body_force is None else tuple(body_force)
+    return fl, mos, orc.OracleCoupled(fl, mesh, mos, abi.IBM_COS4, body_force=bf)
+
+
+def sample_errors(gpu, fl, mos):
+    """Relative L2 of the GPU sample's velocity and node positions against
+    the FP64 oracle's."""
+    _, u, v, w = gpu["state"]
+    _, uo, vo, wo = fl.state()
+    out = {"velocity_rel_l2": rel_l2(np.concatenate([u, v, w]), np.concatenate([uo, vo, wo]))}
+    if mos:
+        out["nodes_rel_l2"] = rel_l2(gpu["nodes"], np.concatenate([m.X.reshape(-1, 3) for m in mos]))
+    return out
+
+
+def cpu_leg(mesh, params, state, mems, c2_sample, tgv, ens_sample):
+    """The CPU baseline and the FP64 errors of the GPU samples.
+
+    Times the FP64 oracle (one thread) on the headline workload's first
+    c2_sample["steps"] coupled steps (the bounded sample) and on the whole
+    Taylor-Green case (1000 steps, the CPU-runnable configuration), and
+    compares each GPU sample taken on the same inputs with the oracle's result:
+    the channel + cell, the Taylor-Green vortex and one instance of the
+    ensemble run alone."""
+    from oracle import oracle as orc
+    steps = c2_sample["steps"]
+    fl, mos, cp = oracle_coupled(mesh, params, state, mems)
     t0 = time.perf_counter()
-    o.step(steps, force=force)
+    cp.step(steps)
     dt = time.perf_counter() - t0
-    return n * steps / dt / 1e6, dt
+    parity = {"channel_rbc": dict(sample_errors(c2_sample, fl, mos),
+                                  sample="%d coupled steps from rest, FP32 GPU vs FP64 oracle" % steps)}
+    c1 = None
+    if tgv is not None:
+        mt, pt, stt = tgv["case"]
+        o = orc.OracleFluid(mt, orc.OR_D2Q9, visc=pt.visc, dt=pt.dt)
+        o.set_state(*stt)
+        t1 = time.perf_counter()
+        o.step(tgv["steps"])
+        dt1 = time.perf_counter() - t1
+        c1 = {"workload": "64x64 D2Q9 Taylor-Green vortex, %d steps, FP64" % tgv["steps"],
+              "seconds": dt1, "mlups": mt.n_cells * tgv["steps"] / dt1 / 1e6,
+              "gpu_ms_total": tgv["gpu_ms"]}
+        parity["taylor_green"] = dict(sample_errors({"state": tgv["state"]}, o, None),
+                                      sample="%d steps, FP32 GPU vs FP64 oracle" % tgv["steps"])
+    if ens_sample is not None:
+        mi, bf, st_i, mm = ens_sample["instance"]
+        efl, emos, ecp = oracle_coupled(mi, params, st_i, mm, body_force=bf)
+        ecp.step(ens_sample["steps"])
+        parity["ensemble_instance"] = dict(sample_errors(ens_sample, efl, emos), sample=ens_sample["label"])
+    return {"value": mesh.n_cells * steps / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+            "sample": "%d coupled steps (fluid + 2562-node cell + 4-point IBM) of the 128x64x64 D3Q19 channel, "
+                      "FP64 oracle, one thread, %.1f s" % (steps, dt),
+            "host": host_cpu(), "c1_full": c1}, parity
 
 
-def cpu_model():
+def host_cpu():
+    """CPU model, logical CPUs of the host and CPUs this process may run on."""
+    model = "unknown"
     try:
-        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
-        for line in out.splitlines():
-            if line.startswith("Model name"):
-                return line.split(":", 1)[1].strip()
-    except Exception:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
         pass
-    return "unknown"
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
 
 
 def measured_traffic(kernel):
@@ -125,7 +186,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=20,
+                    help="coupled steps of the headline workload in the CPU baseline sample (about 1 s each)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -144,8 +206,8 @@ def main():
     ap.add_argument("--adjoint-every", type=int, default=100)
     ap.add_argument("--only-batched", action="store_true",
                     help="time the 256-cell suspension alone (no channel, adjoint or CPU baseline): profiling runs")
-    ap.add_argument("--ensemble", action="store_true",
-                    help="also time the 64-instance ensemble (edges from {32,48,64,96}, 1-8 cells of 642 nodes each)")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 64-instance ensemble (edges from {32,48,64,96}, 1-8 cells of 642 nodes each)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU design is documented, not built); "
@@ -203,7 +265,13 @@ def main():
 
     solver = new_solver(mesh, params, fused)
     solver.set_state(*state)
-    solver.add_membranes(workloads.single_rbc(mesh), ibm_kernel=abi.IBM_COS4)
+    rbc = workloads.single_rbc(mesh)
+    solver.add_membranes(rbc, ibm_kernel=abi.IBM_COS4)
+    # the sample the CPU baseline repeats in FP64: the first steps from rest
+    c2_sample = None
+    if not args.no_cpu_baseline:
+        solver.step(args.cpu_steps)
+        c2_sample = {"steps": args.cpu_steps, "state": solver.get_state(), "nodes": solver.membrane_nodes()[0]}
     ms_step = timed(solver, args.steps, args.warmup)
     mlups = n / (ms_step * 1e-3) / 1e6
     ev_ms = solver.time_steps(args.steps) / args.steps
@@ -254,8 +322,13 @@ def main():
         mt, pt, stt = workloads.tgv_2d()
         tg = abi.Solver(mt, pt)
         tg.set_state(*stt)
+        tg.sync()
+        t0 = time.perf_counter()
         tg.step(args.tgv_steps)
-        _, u1, v1, _ = tg.get_state()
+        tg.sync()
+        tgv_ms = (time.perf_counter() - t0) * 1e3
+        tgv_state = tg.get_state()
+        _, u1, v1, _ = tgv_state
         tg.close()
         k = 2 * np.pi / 64
         tt = args.tgv_steps * pt.dt
@@ -264,7 +337,9 @@ def main():
                     np.dot(np.concatenate([stt[1], stt[2]]), np.concatenate([stt[1], stt[2]])))
         taylor_green = {"workload": "64x64 D2Q9, jitter 0.05, periodic, Re 100, U0 0.01, dt %g, %d steps"
                                     % (pt.dt, args.tgv_steps),
-                        "amplitude": amp, "analytic": decay, "rel_error": amp / decay - 1.0}
+                        "precision": "FP32 on the GPU (the configuration names FP64: its FP64 error is in "
+                                     "parity_fp64.taylor_green)",
+                        "amplitude": amp, "analytic": decay, "rel_error": amp / decay - 1.0, "gpu_ms_total": tgv_ms}
         log("Taylor-Green: amplitude %.6f, analytic %.6f" % (amp, decay))
 
     # --- 256x128x128 suspension of 256 RBCs of mixed resolution ---
@@ -326,35 +401,52 @@ def main():
                 log("adjoint with %d recorded steps failed (%s); fewer records" % (every, e))
         adjoint_ckpt = run_adjoint(args.adjoint_every)
 
-    ensemble = None
-    if args.ensemble:
+    ensemble = ens_sample = None
+    if not args.no_ensemble:
         log("building the 64-instance ensemble")
         m5, p5, s5, mems5 = workloads.ensemble()
         p5.tile[0], p5.tile[1], p5.tile[2] = params.tile[0], params.tile[1], params.tile[2]
         p5.block_threads = params.block_threads
-        e_plain = abi.Solver(m5, p5)
+        e_plain = new_solver(m5, p5, fused)
         e_plain.set_state(*s5)
         ms5_plain = timed(e_plain, max(args.steps // 4, 10), 5)
         e_plain.close()
-        e = abi.Solver(m5, p5)
+        e = new_solver(m5, p5, fused)
         e.set_state(*s5)
         e.add_membranes(mems5, ibm_kernel=abi.IBM_COS4)
+        if not args.no_cpu_baseline:
+            # the instance with the most cells, its first 10 steps from rest
+            counts = np.bincount([mm.block for mm in mems5], minlength=len(m5.members))
+            pick = int(np.argmax(counts))
+            e.step(10)
+            X5 = e.membrane_nodes()[0]
+            starts = np.cumsum([0] + [len(mm.nodes) // 3 for mm in mems5])
+            mine = [k for k, mm in enumerate(mems5) if mm.block == pick]
+            mi = m5.members[pick]
+            ens_sample = {"steps": 10, "state": [m5.split(x)[pick] for x in e.get_state()],
+                          "nodes": np.concatenate([X5[starts[k]:starts[k + 1]] for k in mine]),
+                          "instance": (mi, m5.body_forces[3 * pick: 3 * pick + 3],
+                                       [m5.split(x)[pick] for x in s5],
+                                       [mems5[k] for k in mine]),
+                          "label": "instance %d (%dx%dx%d, %d cells), 10 steps from rest inside the ensemble, "
+                                   "FP32 GPU vs the FP64 oracle on the instance alone"
+                                   % (pick, mi.nx, mi.ny, mi.nz, len(mine))}
         ms5 = timed(e, max(args.steps // 4, 10), 5)
         k5 = e.time_kernels(10)
         e.close()
-        ensemble = {"workload": "64 independent jittered channel boxes (edges from {32,48,64,96}, seeds 5000+i), "
-                                "%d cells of 642 nodes, D3Q19, one mesh" % len(mems5),
+        ensemble = {"workload": "64 independent jittered channel boxes (edges from {32,48,64,96}, seeds 5000+i, "
+                                "each with its own duct body force), %d cells of 642 nodes, D3Q19, one mesh"
+                                % len(mems5),
                     "cells": m5.n_cells, "ms_per_step": ms5, "plain_ms_per_step": ms5_plain,
                     "ratio_vs_plain": ms5 / ms5_plain, "mlups": m5.n_cells / (ms5 * 1e-3) / 1e6, "kernels_ms": k5}
         log("ensemble step %.4f ms (plain %.4f ms)" % (ms5, ms5_plain))
 
-    cpu = None
+    cpu = parity = None
     if not args.no_cpu_baseline:
-        log("CPU baseline (FP64 oracle, %d steps)" % args.cpu_steps)
-        cpu_mlups, cpu_s = cpu_baseline(mesh, params, args.cpu_steps)
-        cpu = {"value": cpu_mlups, "unit": "MLUPS", "cores": 1, "kind": "port",
-               "sample": "%d fluid steps of the same 128x64x64 D3Q19 mesh, FP64, single thread, %.1f s, host %s"
-                         % (args.cpu_steps, cpu_s, cpu_model())}
+        log("CPU baseline (FP64 oracle, %d coupled steps) and FP64 errors of the GPU samples" % args.cpu_steps)
+        tgv = None if taylor_green is None else {"case": (mt, pt, stt), "steps": args.tgv_steps,
+                                                 "state": tgv_state, "gpu_ms": tgv_ms}
+        cpu, parity = cpu_leg(mesh, params, state, rbc, c2_sample, tgv, ens_sample)
 
     out = {
         "metric": METRIC,
@@ -394,6 +486,7 @@ def main():
         "adjoint": adjoint,
         "adjoint_checkpointed": adjoint_ckpt,
         "cpu_baseline": cpu,
+        "parity_fp64": parity,
     }
     print(json.dumps(out))
 

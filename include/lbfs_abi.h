@@ -104,6 +104,10 @@ typedef struct {
     const int *block_dims;       /* [3 n_blocks] */
     const double *block_origin;  /* [3 n_blocks] */
     const double *block_spacing; /* [n_blocks] */
+    /* body force of each box (the same per-cell force term as
+     * lbfs_params.body_force, box by box), or NULL: lbfs_params.body_force in
+     * every box */
+    const double *block_body_force; /* [3 n_blocks] or NULL */
 } lbfs_mesh_desc;
 
 typedef struct {
@@ -288,7 +292,7 @@ int lbfs_get_membrane_energies_noalloc(lbfs_ctx *ctx, double *out /* [7 n_membra
 /* ---- parameters the adjoint differentiates ----
  * Uniform body force (the reference's pressure gradient along its direction,
  * Solver_gpu.cu:834-839, preprocessor.cpp:722-736); replaces lbfs_params.body_force. */
-int lbfs_set_body_force(lbfs_ctx *ctx, const double *f /* [3] */);
+int lbfs_set_body_force(lbfs_ctx *ctx, const double *f /* [3] */);  /* every box of an ensemble */
 /* shear modulus (spring constants k_s are proportional to it, spring_network.cu:1600-1615),
  * local and global bending moduli of membrane `index` (lagrangian_object.h) */
 int lbfs_set_membrane_moduli(lbfs_ctx *ctx, int index, double shear_modulus, double local_bending_modulus,

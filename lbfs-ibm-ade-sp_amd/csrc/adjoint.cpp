@@ -598,6 +598,8 @@ extern "C" int lbfs_set_body_force(lbfs_ctx *ctx, const double *f)
     HIPCHK(hipMemcpyAsync(ctx->cell_force, sf.data(), sizeof(float) * sf.size(), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (int t = 0; t < 3; t++) ctx->params.body_force[t] = f[t];
+    for (lbfs::HostBlock &b : ctx->blocks)
+        for (int t = 0; t < 3; t++) b.force[t] = f[t];
     return LBFS_OK;
 }
 

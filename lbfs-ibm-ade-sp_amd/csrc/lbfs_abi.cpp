@@ -70,7 +70,7 @@ static int fluid_setup(lbfs_ctx *ctx, const lbfs_mesh_desc &m, const lbfs_params
     }
     if (p.block_threads != 0 && p.block_threads != 256 && p.block_threads != 512 && p.block_threads != 576)
         return ctx->fail(LBFS_E_INVALID_ARG, "block_threads must be 256, 512 or 576");
-    rc = mesh_blocks(m, ctx->blocks, err);
+    rc = mesh_blocks(m, p.body_force, ctx->blocks, err);
     if (rc) return ctx->fail(rc, "mesh blocks: %s", err.c_str());
     rc = build_layout(m, ctx->geo, lhs, rhs, p.rho0, p.body_force, p.lattice == LBFS_D2Q9 ? 2 : 3, tile, ctx->blocks,
                       ctx->lay, err);
@@ -536,8 +536,9 @@ extern "C" int lbfs_get_cell_force(lbfs_ctx *ctx, double *f_out)
     // body force plus the spread IBM force (the closure term is not a force of the reference)
     for (int i = 0; i < n; i++) {
         const int c = ctx->lay.perm[i];
+        const double *bf = cell_body_force(ctx->blocks, c, ctx->params.body_force);
         for (int t = 0; t < 3; t++)
-            f_out[3 * (size_t)c + t] = ctx->params.body_force[t] + ((double)cf[3 * (size_t)i + t] - (double)sf[3 * (size_t)i + t]);
+            f_out[3 * (size_t)c + t] = bf[t] + ((double)cf[3 * (size_t)i + t] - (double)sf[3 * (size_t)i + t]);
     }
     return LBFS_OK;
 }

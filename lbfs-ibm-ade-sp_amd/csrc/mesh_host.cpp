@@ -289,9 +289,14 @@ int compute_ls(const Geometry &g, int grad_type, bool det_compat, std::vector<do
 // ---------------------------------------------------------------------------
 // tile-blocked device layout
 // ---------------------------------------------------------------------------
-int mesh_blocks(const lbfs_mesh_desc &m, std::vector<HostBlock> &blocks, std::string &err)
+int mesh_blocks(const lbfs_mesh_desc &m, const double body_force[3], std::vector<HostBlock> &blocks,
+                std::string &err)
 {
     blocks.clear();
+    if (m.block_body_force && m.n_blocks <= 1) {
+        err = "block_body_force needs n_blocks > 1";
+        return LBFS_E_INVALID_ARG;
+    }
     if (m.n_blocks > 1) {
         if (!m.block_dims || !m.block_origin || !m.block_spacing) { err = "block arrays missing"; return LBFS_E_INVALID_ARG; }
         long long c0 = 0;
@@ -302,6 +307,13 @@ int mesh_blocks(const lbfs_mesh_desc &m, std::vector<HostBlock> &blocks, std::st
                 h.origin[t] = m.block_origin[3 * b + t];
             }
             h.spacing = m.block_spacing[b];
+            for (int t = 0; t < 3; t++) {
+                h.force[t] = m.block_body_force ? m.block_body_force[3 * b + t] : body_force[t];
+                if (!std::isfinite(h.force[t])) {
+                    err = "block " + std::to_string(b) + ": non-finite body force";
+                    return LBFS_E_INVALID_ARG;
+                }
+            }
             if (h.n[0] < 1 || h.n[1] < 1 || h.n[2] < 1 || !(h.spacing > 0.0)) {
                 err = "block " + std::to_string(b) + ": dims and spacing must be positive";
                 return LBFS_E_INVALID_ARG;
@@ -319,9 +331,23 @@ int mesh_blocks(const lbfs_mesh_desc &m, std::vector<HostBlock> &blocks, std::st
         for (int t = 0; t < 3; t++) h.origin[t] = m.origin[t];
         h.spacing = m.spacing;
         h.cell0 = 0;
+        for (int t = 0; t < 3; t++) h.force[t] = body_force[t];
         blocks.push_back(h);
     }
     return LBFS_OK;
+}
+
+const double *cell_body_force(const std::vector<HostBlock> &blocks, int c, const double *dflt)
+{
+    // boxes are numbered in cell order: the last box starting at or before c
+    int lo = 0, hi = (int)blocks.size() - 1, b = -1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) / 2;
+        if (blocks[mid].cell0 <= c) { b = mid; lo = mid + 1; } else hi = mid - 1;
+    }
+    if (b < 0) return dflt;
+    const HostBlock &h = blocks[b];
+    return c < h.cell0 + h.n[0] * h.n[1] * h.n[2] ? h.force : dflt;
 }
 
 int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<double> &lhs,
@@ -466,9 +492,10 @@ int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<d
         d.static_force.assign(3 * (size_t)nc, 0.0f);
         for (int i = 0; i < nc; i++) {
             const int c = d.perm[i];
+            const double *bf = cell_body_force(blocks, c, body_force);
             for (int t = 0; t < 3; t++)
                 d.static_force[3 * (size_t)i + t] =
-                    (float)(body_force[t] - (t < dims ? rho0 / 3.0 * S[3 * (size_t)c + t] / g.vol[c] : 0.0));
+                    (float)(bf[t] - (t < dims ? rho0 / 3.0 * S[3 * (size_t)c + t] / g.vol[c] : 0.0));
         }
     }
     d.bc_owner.assign(g.n_bc, 0);

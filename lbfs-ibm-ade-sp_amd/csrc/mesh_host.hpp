@@ -57,8 +57,12 @@ struct HostBlock {
     double origin[3];
     double spacing;
     int cell0;
+    double force[3];  // body force of the box
 };
-int mesh_blocks(const lbfs_mesh_desc &m, std::vector<HostBlock> &blocks, std::string &err);
+int mesh_blocks(const lbfs_mesh_desc &m, const double body_force[3], std::vector<HostBlock> &blocks,
+                std::string &err);
+// body force acting on caller cell c: its box's, or `dflt` outside every box
+const double *cell_body_force(const std::vector<HostBlock> &blocks, int c, const double *dflt);
 
 // Tile-blocked device layout (all indices are internal, tile-ordered ids).
 struct DeviceLayout {

@@ -37,6 +37,7 @@ class MeshDesc(C.Structure):
         ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
         ("origin", C.c_double * 3), ("spacing", C.c_double),
         ("n_blocks", C.c_int), ("block_dims", _ip), ("block_origin", _dp), ("block_spacing", _dp),
+        ("block_body_force", _dp),
     ]
 
 
@@ -353,10 +354,14 @@ class HexMesh:
 
 class Ensemble:
     """Independent boxes stepped together as one mesh (cells numbered box
-    after box, all internal faces first, then all boundary faces)."""
+    after box, all internal faces first, then all boundary faces).
+    body_forces: one (fx, fy, fz) per box, or None for the solver's
+    params.body_force in every box."""
 
-    def __init__(self, members):
+    def __init__(self, members, body_forces=None):
         self.members = list(members)
+        self.body_forces = None if body_forces is None else \
+            np.ascontiguousarray(np.asarray(body_forces, dtype=np.float64).reshape(len(self.members), 3).reshape(-1))
         pts, fv, own, nb, sh, patch = [], [], [], [], [], []
         bfv, bown, bsh, bpatch, brt, bvt, bval = [], [], [], [], [], [], []
         p0 = c0 = 0
@@ -415,6 +420,8 @@ class Ensemble:
         d.block_dims = ptr(self.block_dims, _ip)
         d.block_origin = ptr(self.block_origin)
         d.block_spacing = ptr(self.block_spacing)
+        if self.body_forces is not None:
+            d.block_body_force = ptr(self.body_forces)
         return d
 
     def split(self, arr):

@@ -33,11 +33,12 @@ def coupled_run(mesh, params, st, mems, steps, kernel=abi.IBM_COS4, graph=True):
     return s
 
 
-def oracle_run(mesh, params, st, mems, steps, kernel=abi.IBM_COS4):
+def oracle_run(mesh, params, st, mems, steps, kernel=abi.IBM_COS4, body_force=None):
     fl = orc.OracleFluid(mesh, orc.OR_D3Q19, visc=params.visc, dt=params.dt)
     fl.set_state(*st)
     mos = oracle_membranes(mems)
-    cp = orc.OracleCoupled(fl, mesh, mos, kernel, body_force=tuple(params.body_force))
+    bf = tuple(params.body_force) if body_force is None else tuple(body_force)
+    cp = orc.OracleCoupled(fl, mesh, mos, kernel, body_force=bf)
     cp.step(steps)
     return fl, mos
 
@@ -148,8 +149,9 @@ def test_suspension_full_size_parity_and_replay():
 
 
 def test_ensemble_full_size_sampled_instances():
-    """The 64-instance ensemble (12.1 M cells, 204 cells of 642 nodes) stepped
-    together for 10 steps from a perturbed state; three instances (the
+    """The 64-instance ensemble (12.1 M cells, 204 cells of 642 nodes, each
+    box driven by its own duct body force) stepped together for 10 steps from
+    a perturbed state; three instances (the
     smallest, one with the most cells, the largest box) each run alone
     through the single-instance oracle agree within 1e-5 (velocity, nodes)
     and their node positions within 14 FP32 spacings (one rounding of the
@@ -171,7 +173,7 @@ def test_ensemble_full_size_sampled_instances():
     for i in picks:
         mi = members[i]
         mm = [mems[k] for k in by_block[i]]
-        fl, mos = oracle_run(mi, params, states[i], mm, 10)
+        fl, mos = oracle_run(mi, params, states[i], mm, 10, body_force=ens.body_forces[3 * i: 3 * i + 3])
         _, uo, vo, wo = fl.state()
         vel = cases.rel_l2(np.concatenate([ens.split(us)[i], ens.split(vs)[i], ens.split(ws)[i]]),
                            np.concatenate([uo, vo, wo]))

@@ -225,10 +225,11 @@ def test_reference_signature_entry_equals_direct_abi_run(tmp_path):
     assert "direct ABI run equal" in r.stdout and "stop after 1 step" in r.stdout
 
 
-def test_ensemble_equals_members_run_alone():
+@pytest.mark.parametrize("per_box_force", [False, True])
+def test_ensemble_equals_members_run_alone(per_box_force):
     """Independent boxes stepped together as one mesh (one launch per stage
     over all boxes, membranes in their own box) give bit for bit what each box
-    gives run alone."""
+    gives run alone, with one body force for all boxes or one per box."""
     members = [abi.HexMesh(16, 10, 10, periodic=(1, 0, 0), jitter=0.1, seed=31),
                abi.HexMesh(20, 12, 10, periodic=(1, 0, 0), jitter=0.1, seed=32),
                abi.HexMesh(12, 10, 12, periodic=(0, 0, 0), jitter=0.1, seed=33)]
@@ -243,8 +244,9 @@ def test_ensemble_equals_members_run_alone():
             out.append(abi.Membrane(x, t, block=block, **PARAMS))
         return out
 
-    ens = abi.Ensemble(members)
-    s = abi.Solver(ens, abi.make_params(visc=VISC, dt=DT, body_force=body))
+    forces = [(2e-5 * (k + 1), 1e-6 * k, -1e-6) if per_box_force else body for k in range(len(members))]
+    ens = abi.Ensemble(members, body_forces=forces if per_box_force else None)
+    s = abi.Solver(ens, abi.make_params(visc=VISC, dt=DT, body_force=(0.0, 0.0, 0.0) if per_box_force else body))
     s.set_state(*[np.concatenate([st[v] for st in states]) for v in range(4)])
     s.add_membranes([mm for k in range(3) for mm in mems(k, k)])
     s.step(12)
@@ -252,7 +254,7 @@ def test_ensemble_equals_members_run_alone():
     ens_nodes = s.membrane_nodes()[0]
     node0 = 0
     for k, m in enumerate(members):
-        r = abi.Solver(m, abi.make_params(visc=VISC, dt=DT, body_force=body))
+        r = abi.Solver(m, abi.make_params(visc=VISC, dt=DT, body_force=forces[k]))
         r.set_state(*states[k])
         if cells[k]:
             r.add_membranes(mems(k, 0))
