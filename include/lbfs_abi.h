@@ -157,11 +157,13 @@ int lbfs_step(lbfs_ctx *ctx, int nsteps);
 
 /* options (no counterpart in the reference) */
 #define LBFS_OPT_GRAPH 1 /* 1 (default): replay the step from a captured hipGraph, 0: launch kernels one by one */
-#define LBFS_OPT_IBM_BINNED 2 /* spreading for the next lbfs_add_membranes: -1 (default) by node count
-                                 (binned from 2048 nodes), 0 one wavefront per node into integer cell
-                                 accumulators, 1 nodes binned and spread per bin through LDS (the occupied
-                                 bins listed up to 65536 nodes, every bin of the grid above), 2 binned
-                                 with every bin of the grid; the result is the same bit for bit */
+#define LBFS_OPT_IBM_BINNED 2 /* spreading for the next lbfs_add_membranes: -1 (default) by node density
+                                 (1 from one node per 100 cells, else 2); 0 one wavefront per node into
+                                 integer cell accumulators in global memory; 1 the nodes listed per fluid
+                                 tile, each tile gathering the nodes around it (sums in LDS, no global
+                                 atomics); 2 the nodes listed per 4x4x4 bin, each occupied bin spreading
+                                 into an LDS window added to the accumulators; the result is the same
+                                 bit for bit */
 #define LBFS_OPT_STEP_SCHEDULE 3 /* coupled step schedule (the result is the same bit for bit):
                                     0 (default) the fluid passes on the main stream from the start
                                     of the step, the whole coupling chain on a second; 1 the

@@ -316,7 +316,7 @@ int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next)
     HIPCHK(hipStreamWaitEvent(ctx->side2, ctx->ev_fork, 0));
     HIPCHK(launch_mem_volume(m, ctx->side2));
     HIPCHK(launch_mem_edges(m, ctx->side2));
-    if (!ctx->ibm.binned) HIPCHK(launch_mem_nodes(m, ctx->side2));
+    HIPCHK(launch_mem_nodes(m, ctx->side2));
     HIPCHK(hipEventRecord(ctx->ev_join2, ctx->side2));
     HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join2, 0));
     HIPCHK(launch_ibm_spread(m, ctx->ibm, ctx->stream));

@@ -50,7 +50,7 @@ struct lbfs_ctx {
     std::vector<int> mem_node_start;
     double *mem_energy = nullptr;   // [7 n_mem] output of lbfs_get_membrane_energies
     double *energy_scratch = nullptr; // this device's static staging of lbfs_get_membrane_energies_noalloc
-    int ibm_binned_option = -1;     // LBFS_OPT_IBM_BINNED: -1 by node count, 0 per node, 1 per bin
+    int ibm_binned_option = -1;     // LBFS_OPT_IBM_BINNED: -1 by node density, else IBM_SPREAD_*
     std::vector<double> mem_origin;
     // host copies behind lbfs_set_membrane_moduli: per-membrane parameters,
     // per-edge spring constant per unit shear modulus (FP64), edge ranges

@@ -200,10 +200,22 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     m.err = ctx->dev_err;
     HIPCHK(hipMemsetAsync(ctx->dev_err, 0, sizeof(int), ctx->stream));
     g.kernel = ibm_kernel;
+    // spreading path: dense suspensions (a node per 100 cells or more, where
+    // most fluid tiles hold membrane) gather per fluid tile without atomics;
+    // sparser ones scatter per occupied 4x4x4 bin, whose short work items
+    // spread over the whole chip (measured: one 2562-node cell in a
+    // 128x64x64 channel 13 us against 70 us; 380k nodes in 256x128x128
+    // 0.28 ms against 0.85 ms)
+    const int nc0 = ctx->lay.n_cells;
+    if (ctx->ibm_binned_option >= 0) g.binned = ctx->ibm_binned_option;
+    else g.binned = (double)nn >= 0.01 * nc0 ? IBM_SPREAD_GATHER : IBM_SPREAD_BINS;
+    if (g.binned == IBM_SPREAD_GATHER && spread_tile_lds(ctx->args.max_tile_cells) > 150 * 1024)
+        g.binned = IBM_SPREAD_BINS;  // tiles beyond the LDS budget
     const int nblk = (int)ctx->blocks.size();
     std::vector<IbmBlock> blk(nblk);
-    std::vector<int> bin_block;
+    std::vector<int4> tile_lo;  // first cell of each node-list tile in its box, and the box
     std::vector<int> cell_blk_caller(ctx->lay.n_cells, 0);
+    int ftile0 = 0;
     for (int b = 0; b < nblk; b++) {
         const HostBlock &hb = ctx->blocks[b];
         IbmBlock &B = blk[b];
@@ -212,16 +224,30 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
             B.origin[a] = (float)hb.origin[a];
             B.L[a] = (float)(hb.n[a] * hb.spacing);
             B.per[a] = 0;
-            B.tn[a] = (hb.n[a] + 3) / 4;
+            B.ftw[a] = ctx->lay.tile[a];
+            B.fnt[a] = (hb.n[a] + B.ftw[a] - 1) / B.ftw[a];
+            // node lists per fluid tile (gathering) or per 4x4x4 bin
+            B.tw[a] = g.binned == IBM_SPREAD_GATHER ? B.ftw[a] : 4;
+            B.nt[a] = (hb.n[a] + B.tw[a] - 1) / B.tw[a];
         }
         B.h = (float)hb.spacing;
         B.inv_h = (float)(1.0 / hb.spacing);
         B.cell0 = hb.cell0;
-        B.bin0 = (int)bin_block.size();
-        bin_block.insert(bin_block.end(), (size_t)B.tn[0] * B.tn[1] * B.tn[2], b);
+        // the fluid tiles of the box come in the order of the device layout
+        B.regular = hb.n[0] % B.ftw[0] == 0 && hb.n[1] % B.ftw[1] == 0 && hb.n[2] % B.ftw[2] == 0;
+        B.icell0 = ctx->lay.tile_cell_start[ftile0];
+        ftile0 += B.fnt[0] * B.fnt[1] * B.fnt[2];
+        B.tile0 = (int)tile_lo.size();
+        for (int tz = 0; tz < B.nt[2]; tz++)
+            for (int ty = 0; ty < B.nt[1]; ty++)
+                for (int tx = 0; tx < B.nt[0]; tx++)
+                    tile_lo.push_back(make_int4(tx * B.tw[0], ty * B.tw[1], tz * B.tw[2], b));
         const int nb_cells = hb.n[0] * hb.n[1] * hb.n[2];
         for (int c = 0; c < nb_cells; c++) cell_blk_caller[hb.cell0 + c] = b;
     }
+    if (ftile0 != ctx->lay.n_tiles)
+        return ctx->fail(LBFS_E_UNSUPPORTED, "the boxes' tiles (%d) do not match the layout's (%d)", ftile0,
+                         ctx->lay.n_tiles);
     // periodic axes of each box from the translations of its wrap faces
     for (int f = 0; f < ctx->geo.n_faces; f++)
         for (int a = 0; a < 3; a++)
@@ -230,6 +256,8 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
                 B.per[a] = 1;
                 B.L[a] = (float)std::fabs(ctx->geo.shift[3 * (size_t)f + a]);
             }
+    for (IbmBlock &B : blk)
+        for (int a = 0; a < 3; a++) B.period_step[a] = B.L[a] / (float)B.n[a];
     // 4-point Lagrange kernel: per box and axis the vertex planes along the
     // first grid line (face centroids of its cells, the reference's plateau
     // arrays) and the cell-line centroids
@@ -263,12 +291,7 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     if ((rc = cp_alloc(ctx, &dax, axis.data(), axis.size()))) return rc;
     g.axis = dax;
     g.n_blocks = nblk;
-    g.n_bins = (int)bin_block.size();
-    // few nodes: spread per node straight into the cell accumulators (fewer
-    // launches); from a couple of thousand nodes on, binning them and
-    // spreading per bin through LDS is faster (one 2562-node cell in a
-    // 128x64x64 channel: 28 us against 39 us for the spreading)
-    g.binned = ctx->ibm_binned_option >= 0 ? (ctx->ibm_binned_option >= 1 ? 1 : 0) : (nn >= 2048 ? 1 : 0);
+    g.n_tiles = (int)tile_lo.size();
     g.dt = (float)ctx->params.dt;
     std::vector<int> node_block(nn), mem_block(count);
     for (int i = 0; i < count; i++) {
@@ -276,13 +299,17 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
         for (int k = mem_node_start[i]; k < mem_node_start[i + 1]; k++) node_block[k] = mem[i].block;
     }
     IbmBlock *dblk;
-    int *dbb2, *dnb, *dmb;
+    int4 *dtl;
+    int *dnb, *dmb;
     if ((rc = cp_alloc(ctx, &dblk, blk.data(), blk.size()))) return rc;
-    if ((rc = cp_alloc(ctx, &dbb2, bin_block.data(), bin_block.size()))) return rc;
+    if ((rc = cp_alloc(ctx, &dtl, tile_lo.data(), tile_lo.size()))) return rc;
     if ((rc = cp_alloc(ctx, &dnb, node_block.data(), node_block.size()))) return rc;
     if ((rc = cp_alloc(ctx, &dmb, mem_block.data(), mem_block.size()))) return rc;
     g.blk = dblk;
-    g.bin_block = dbb2;
+    g.blk0 = blk[0];
+    g.tile_lo = dtl;
+    g.tile_cell_start = ctx->args.tile_cell_start;
+    g.max_tile_cells = ctx->args.max_tile_cells;
     g.node_block = dnb;
     ctx->mem.mem_block = dmb;
     if (nblk > 1) {
@@ -313,30 +340,33 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     float *dssc;
     if ((rc = cp_alloc(ctx, &dssc, ssc.data(), ssc.size()))) return rc;
     g.spread_sc = dssc;
-    int *dinv, *bc, *slots, *ov, *nov;
+    int *dinv, *tcount, *slots, *ov, *nov;
     float4 *dcen;
-    float *wn;
+    float *wn, *sf;
     int4 *nbs;
-    if ((rc = cp_alloc(ctx, &dinv, ctx->lay.inv.data(), nc))) return rc;
-    if ((rc = cp_alloc(ctx, &dcen, cen.data(), nc))) return rc;
-    if ((rc = cp_alloc<int>(ctx, &bc, nullptr, g.n_bins))) return rc;
-    if ((rc = cp_alloc<int>(ctx, &slots, nullptr, (size_t)IBM_BIN_CAP * g.n_bins))) return rc;
-    if ((rc = cp_alloc<int>(ctx, &ov, nullptr, nn))) return rc;
-    if ((rc = cp_alloc<int>(ctx, &nov, nullptr, 1))) return rc;
+    // node slots per fluid tile: twice its cells (a membrane sheet crossing a
+    // tile twice at the finest resolution stays below that); per bin 128; the
+    // rest overflow
+    g.tile_cap = g.binned == IBM_SPREAD_GATHER ? std::max(64, 2 * ctx->args.max_tile_cells) : IBM_BIN_CAP;
     int *blist, *nblist;
-    if ((rc = cp_alloc<int>(ctx, &blist, nullptr, g.n_bins))) return rc;
+    if ((rc = cp_alloc<int>(ctx, &blist, nullptr, g.n_tiles))) return rc;
     if ((rc = cp_alloc<int>(ctx, &nblist, nullptr, 1))) return rc;
     g.bin_list = blist;
     g.n_bin_list = nblist;
-    // few nodes occupy few bins: list them (one atomic per occupied bin) and
-    // loop over the list; many nodes: one workgroup per bin of the grid
-    g.use_list = ctx->ibm_binned_option == 2 ? 0 : (nn <= 65536 ? 1 : 0);
+    if ((rc = cp_alloc(ctx, &dinv, ctx->lay.inv.data(), nc))) return rc;
+    if ((rc = cp_alloc(ctx, &dcen, cen.data(), nc))) return rc;
+    if ((rc = cp_alloc<int>(ctx, &tcount, nullptr, g.n_tiles))) return rc;
+    if ((rc = cp_alloc<int>(ctx, &slots, nullptr, (size_t)g.tile_cap * g.n_tiles))) return rc;
+    if ((rc = cp_alloc<int>(ctx, &ov, nullptr, nn))) return rc;
+    if ((rc = cp_alloc<int>(ctx, &nov, nullptr, 1))) return rc;
     unsigned long long *dacc;
     if ((rc = cp_alloc<unsigned long long>(ctx, &dacc, nullptr, 3 * (size_t)nc))) return rc;
     g.acc = dacc;
+    if ((rc = cp_alloc<float>(ctx, &sf, nullptr, 3 * (size_t)nc))) return rc;
+    g.spread_f = sf;
     if ((rc = cp_alloc<float>(ctx, &wn, nullptr, 64 * (size_t)nn))) return rc;
     if ((rc = cp_alloc<int4>(ctx, &nbs, nullptr, nn))) return rc;
-    g.inv = dinv; g.cen = dcen; g.bin_count = bc; g.bin_slots = slots; g.overflow = ov; g.n_overflow = nov;
+    g.inv = dinv; g.cen = dcen; g.tile_count = tcount; g.tile_slots = slots; g.overflow = ov; g.n_overflow = nov;
     g.wnode = wn; g.nbase = nbs;
     g.static_force = ctx->static_force;
     g.cell_force = ctx->cell_force;
@@ -347,12 +377,14 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
         std::vector<int> ct(nc);
         for (int t = 0; t < nt; t++)
             for (int i = ctx->lay.tile_cell_start[t]; i < ctx->lay.tile_cell_start[t + 1]; i++) ct[i] = t;
-        int *dct, *dnow, *dprev;
+        int *dct, *dnow, *dacc2, *dprev;
         if ((rc = cp_alloc(ctx, &dct, ct.data(), ct.size()))) return rc;
         if ((rc = cp_alloc<int>(ctx, &dnow, nullptr, nt))) return rc;
+        if ((rc = cp_alloc<int>(ctx, &dacc2, nullptr, nt))) return rc;
         if ((rc = cp_alloc<int>(ctx, &dprev, nullptr, nt))) return rc;
         g.cell_tile = dct;
         g.tile_now = dnow;
+        g.tile_acc = dacc2;
         g.tile_prev = dprev;
     }
     // internal viscosity field, only when some membrane has a ratio other than 1

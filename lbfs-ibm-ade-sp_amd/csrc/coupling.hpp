@@ -7,9 +7,18 @@
 
 namespace lbfs {
 
-constexpr int IBM_BIN_CAP = 128;  // node slots per 4x4x4 bin (fuller bins overflow to a list)
 // scale of the membranes' 64-bit fixed-point sums (volume, area, MAD numerator)
 constexpr double MEM_SUM_SCALE = 1073741824.0;  // 2^30
+
+// spreading paths; all give the same bits
+enum {
+    IBM_SPREAD_NODES = 0,   // one wavefront per node into per-cell accumulators (global atomics)
+    IBM_SPREAD_GATHER = 1,  // nodes listed per fluid tile; each tile gathers the nodes whose
+                            // stencils reach it and sums their forces in LDS (no global atomics)
+    IBM_SPREAD_BINS = 2,    // nodes listed per 4x4x4 bin; each occupied bin spreads its nodes into an
+                            // LDS window (the bin and S - 1 cells beyond) added to the accumulators
+};
+constexpr int IBM_BIN_CAP = 128;  // node slots per bin of the bin scattering
 
 struct MemParamDev {
     float spring_c;           // WLC constant making F(L0) = 0
@@ -50,52 +59,65 @@ struct MemArgs {
 // one logical box of cells (an ensemble member, or the whole mesh)
 struct IbmBlock {
     int n[3], per[3];
-    int tn[3];                // 4x4x4 bins (= spreading tiles) per axis
+    int tw[3];                // node-list tile size (cells per tile along each axis; the last tile may be smaller)
+    int nt[3];                // node-list tiles per axis
+    int tile0;                // first node-list tile id of the box (tiles x fastest, then y, then z)
     int cell0;                // first caller cell id of the box (cells i + j nx + k nx ny after it)
-    int bin0;                 // first bin id of the box
+    int ftw[3], fnt[3];       // fluid tile size and tiles per axis (the device cell order)
+    int regular;              // 1: every fluid tile of the box is whole, so internal ids follow from i, j, k
+    int icell0;               // internal id of the box's first cell (its first fluid tile's first cell)
     int ax_off[3];            // 4-point Lagrange kernel: offset of the axis data in IbmArgs::axis
     float origin[3], L[3], h, inv_h;
+    float period_step[3];     // L / n: shift of a cell centroid per period of a wrapped stencil index
 };
 
 struct IbmArgs {
     int kernel;               // LBFS_IBM_*
     int n_cells;
-    int n_blocks, n_bins;
-    int binned;               // 1: nodes binned and spread per bin; 0: spread per node with global atomics
+    int n_blocks, n_tiles;
+    int binned;               // IBM_SPREAD_*: how the forces are spread
     const IbmBlock *blk;      // [n_blocks]
-    const int *bin_block;     // [n_bins] box of each bin
+    IbmBlock blk0;            // the box of a single-box mesh, by value (no per-node lookup)
+    const int4 *tile_lo;      // [n_tiles] first cell of each node-list tile in its box (x, y, z) and the box
+    const int *tile_cell_start;  // [n_tiles + 1] internal cell ids of each tile (tile gathering: the
+                              // node-list tiles are the fluid tiles)
+    int max_tile_cells;
+    int *bin_list, *n_bin_list;  // bin scattering: the tiles holding nodes this step (the count is
+                              // cleared by the apply pass)
     const int *node_block;    // [n_nodes] box of each node
     const int *cell_block;    // [n_cells] box of each cell (internal order), null for one box
     const double *axis;       // per box and axis: n + 1 vertex plane coordinates, then n cell-line centroids
     float dt;
     const int *inv;           // caller cell id -> internal id
     const float4 *cen;        // internal order: centroid xyz, volume
-    int *bin_count;           // nodes put in each bin this step (cleared by the spreading, or by the
-                              // apply pass for listed bins)
-    int *bin_slots;           // IBM_BIN_CAP node ids per bin
-    int use_list;             // 1: the interpolation lists the occupied bins and the spreading loops
-                              // over chunks of their nodes (few nodes); 0: one spreading workgroup per bin
-    int *bin_list, *n_bin_list;  // the bins holding nodes this step (the count is cleared by the apply pass)
-    int *overflow, *n_overflow;  // nodes of full bins (the count is cleared by the cell update)
+    int *tile_count;          // [n_tiles] nodes listed in each tile this step (cleared by the apply pass)
+    int *tile_slots;          // tile_cap node ids per tile
+    int tile_cap;
+    int *overflow, *n_overflow;  // nodes of full tiles, gathered by every tile (the count is cleared by the
+                              // apply pass)
     float *wnode;             // 64 per node: normalised stencil weights of its current position
-    int4 *nbase;              // per node: stencil base cell (logical) and bin
+                              // (lane order i + S j + S^2 k)
+    int4 *nbase;              // per node: stencil base cell (logical) of the current position and its tile
     const float *spread_sc;   // [n_cells] force density per unit spread sum: 1/V, or 1/h^3 (2-point)
-    unsigned long long *acc;  // 3 per cell (internal order): fixed-point spread force
+    float *spread_f;          // [3 n_cells] force density of the tiles the tile spreading wrote
+    unsigned long long *acc;  // 3 per cell (internal order): fixed-point spread force of the per-node
+                              // spreading (and scratch of the adjoint)
     const float *static_force;
     float *cell_force;
     const int *cell_tile;     // [n_cells] fluid tile of each cell
-    int *tile_now;            // [n_tiles] 1: the spreading wrote into the tile this step
+    int *tile_now;            // [n_tiles] 1: the tile spreading wrote a nonzero force density this step
+    int *tile_acc;            // [n_tiles] 1: the per-node spreading wrote into the tile's accumulators
     int *tile_prev;           // [n_tiles] 1: the cell force output holds spread force there
     int *err;                 // the context's device error word (ERR_FIX_* bits, fixed_point.hpp)
 };
 
 // The end of the coupled step: adds the spread force density to the updated
 // state (the face pass updated it with the static force), writes the cell
-// force output and clears the accumulators, tile by tile: only the tiles the
-// spreading wrote into (all = 0; the tiles of the previous step get their
-// output reset to the static force), or every tile (all = 1, for per-step
-// force records).  Also closes out the membrane sums of the step and clears
-// the node-binning overflow count.
+// force output and clears the per-node accumulators, tile by tile: only the
+// tiles the spreading wrote into (all = 0; the tiles of the previous step get
+// their output reset to the static force), or every tile (all = 1, for
+// per-step force records).  Also closes out the membrane sums of the step and
+// clears the tile node counts and the overflow count for the next step.
 struct ApplyArgs {
     const int *tile_cell_start;
     int n_tiles;
@@ -108,14 +130,21 @@ struct ApplyArgs {
     int n_mem;
 };
 
+// threads of the tile spreading, also the candidate nodes it tests at a time
+constexpr int SPREAD_NT = 512;
+// LDS of the tile spreading: the staged candidate nodes (base 16 B, force
+// 16 B, id 4 B), then three 64-bit sums per tile cell
+inline size_t spread_tile_lds(int max_tile_cells) { return (size_t)SPREAD_NT * 36 + (size_t)max_tile_cells * 24; }
+
 hipError_t launch_ibm_interp(const MemArgs &m, const IbmArgs &g, const float4 *st, hipStream_t s);
-// weights of the current node positions (after attaching membranes)
+// weights and tiles of the current node positions (after attaching membranes
+// or restoring them)
 hipError_t launch_ibm_weights(const MemArgs &m, const IbmArgs &g, hipStream_t s);
 hipError_t launch_mem_volume(const MemArgs &m, hipStream_t s);
 hipError_t launch_mem_edges(const MemArgs &m, hipStream_t s);
 hipError_t launch_mem_nodes(const MemArgs &m, hipStream_t s);
-// spreading into the fixed-point cell accumulators; launch_ibm_apply turns
-// them into force densities
+// spreading: per fluid tile into spread_f, or per node into the fixed-point
+// cell accumulators; launch_ibm_apply adds the force densities to the state
 hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s);
 hipError_t launch_ibm_apply(const IbmArgs &g, const ApplyArgs &p, hipStream_t s);
 // internal viscosity field (immersed_boundary_method.cu:389-527)

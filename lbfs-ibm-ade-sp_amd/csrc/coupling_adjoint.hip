@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void k_adj_spread(MemArgs m, IbmArgs g, AdjMem
     const int lane = threadIdx.x & 63;
     const int nd = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (nd >= m.n_nodes) return;
-    const IbmBlock B = g.blk[g.node_block[nd]];
+    const IbmBlock B = node_block(g, nd);
     const float4 o = m.origin[m.node_mem[nd]], xl = m.X[nd];
     const float P[3] = {o.x + xl.x, o.y + xl.y, o.z + xl.z};
     float wh, dw[3];
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(256) void k_adj_interp(MemArgs m, IbmArgs g, AdjMem
     const int lane = threadIdx.x & 63;
     const int nd = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (nd >= m.n_nodes) return;
-    const IbmBlock B = g.blk[g.node_block[nd]];
+    const IbmBlock B = node_block(g, nd);
     const float4 o = m.origin[m.node_mem[nd]], xl = j.Xn[nd];
     const float P[3] = {o.x + xl.x, o.y + xl.y, o.z + xl.z};
     float wh, dw[3];

@@ -1,9 +1,9 @@
 // Immersed-boundary coupling and spring-network membrane kernels for gfx950.
 //
-//   k_ibm_interp   per node: fluid velocity at the node (immersed_boundary_method.cu:14-97
-//                  or the 4-point cosine kernel), explicit advection X += U dt
-//                  (immersed_boundary_method.cu:234-264), the advected node put in
-//                  its bin
+//   k_ibm_interp   per node (one per lane): fluid velocity at the node
+//                  (immersed_boundary_method.cu:14-97 or the 4-point kernels),
+//                  explicit advection X += U dt (immersed_boundary_method.cu:234-264),
+//                  the advected node listed in its fluid tile
 //   k_mem_volume   per triangle: volume and area sums (lagrangian_object.cpp:601-650)
 //                  in 64-bit fixed point, wavefront-reduced
 //   k_mem_edges    per spring: all membrane force terms (spring_network.cu:613-971),
@@ -11,10 +11,12 @@
 //                  force call (spring_network.cu:1186-1238)
 //   k_mem_nodes    per node: sum of its edge slots in a fixed order, combined
 //                  with the membrane's global coefficients
-//   k_ibm_spread   per 4x4x4 bin tile: each cell gathers the forces of the nodes
-//                  binned around it (immersed_boundary_method.cu:947-1024); the sum
-//                  is accumulated in 64-bit fixed point so its value does not depend
+//   k_ibm_spread_tiles
+//                  per fluid tile: the forces of the nodes listed around the tile
+//                  spread onto its cells (immersed_boundary_method.cu:947-1024) in
+//                  LDS, summed in 64-bit fixed point so the value does not depend
 //                  on the order the nodes are visited in
+//   k_ibm_apply    per fluid tile: the spread force density added to the updated state
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,6 +25,7 @@
 #include "coupling.hpp"
 #include "ibm_device.hpp"
 #include "membrane_force.hpp"
+#include "xcd.hpp"
 
 namespace lbfs {
 
@@ -84,43 +87,96 @@ __device__ __forceinline__ void mem_coefs(const MemArgs &m, int mem, float &vol,
 
 
 // ---------------------------------------------------------------------------
-// IBM: interpolation, advection, weights, binning.  One wavefront per node,
-// one lane per stencil cell.
+// IBM stencils.  The stencil of a node at P is the S x S x S block of cells
+// from its base index (S = 2 for the 2-point kernel, 4 for the others);
+// indices wrap on periodic axes, cells beyond a wall drop out (weight 0).
+// Each node's normalised weights are computed once per step, at its advected
+// position, and kept (64 floats, lane order i + S j + S^2 k): the spreading
+// of that step and the interpolation of the next read them.
 // ---------------------------------------------------------------------------
-// INTERP: interpolate the cell velocity with the node's cached weights (those
-// of its current position), advect the node and count it into its bin.
-// Always: weights of the (new) position, normalised for the cosine kernel,
-// cached for spreading and the next interpolation.
-// (held to 64 registers: 8 wavefronts per SIMD keep more nodes' loads in flight)
-template <bool INTERP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_ibm_interp(MemArgs m, IbmArgs g,
-                                                                                         const float4 *__restrict__ st)
+template <int K>
+struct Stencil {
+    static constexpr int S = K == LBFS_IBM_LINEAR2 ? 2 : 4;
+};
+
+// unnormalised weight of stencil cell (i, j, k) of a node at P (grid
+// coordinate xr, base index base); w: the wrapped cell index, c its centroid
+template <int K>
+__device__ __forceinline__ float stencil_weight(const IbmArgs &g, const IbmBlock &B, const float P[3], const float xr[3],
+                                                const int base[3], int i, int j, int k, const int w[3], const float4 c)
 {
-    const int lane = threadIdx.x & 63;
-    const int nd = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (nd >= m.n_nodes) return;  // uniform over the wavefront
-    const int S = stencil_size(g);
-    const bool act = lane < S * S * S;
-    int off[3];
-    lane_offset(S, lane, off);
-    const IbmBlock B = g.blk[g.node_block[nd]];
+    if constexpr (K == LBFS_IBM_LINEAR2) {
+        float wt = 1.0f;
+        wt *= linear2_factor(base[0] + i, xr[0]);
+        wt *= linear2_factor(base[1] + j, xr[1]);
+        wt *= linear2_factor(base[2] + k, xr[2]);
+        return wt;
+    } else if constexpr (K == LBFS_IBM_LAGRANGE4) {
+        return (float)(lagrange4(g, B, 0, base[0], i, (double)P[0]) * lagrange4(g, B, 1, base[1], j, (double)P[1]) *
+                       lagrange4(g, B, 2, base[2], k, (double)P[2]));
+    } else {
+        const int d[3] = {base[0] + i - w[0], base[1] + j - w[1], base[2] + k - w[2]};
+        return cos4_cell(B, c, d, P);
+    }
+}
+
+__device__ __forceinline__ int node_tile(const IbmBlock &B, const int base[3])
+{
+    return B.tile0 + tile_coord(B, 0, base[0]) + B.nt[0] * (tile_coord(B, 1, base[1]) + B.nt[1] * tile_coord(B, 2, base[2]));
+}
+
+// sum over the L lanes of a node's lane group (aligned groups of L lanes,
+// butterfly order)
+template <int L>
+__device__ __forceinline__ float group_sum(float v)
+{
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Per node, S x S lanes, lane (j, k) taking the S cells of stencil row (j, k).
+// INTERP: the fluid velocity at the node from its kept weights
+// (immersed_boundary_method.cu:14-97, or the 4-point kernels) and the
+// advection X += U dt (immersed_boundary_method.cu:234-264).  Always: the
+// weights of the (new) position, normalised for the cosine kernel, kept with
+// the stencil base and its fluid tile; INTERP also lists the node in that
+// tile for the spreading.
+template <int K, bool INTERP>
+__global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const float4 *__restrict__ st)
+{
+    constexpr int S = Stencil<K>::S, L = S * S;
+    // consecutive nodes (neighbours on a membrane) on one XCD share its L2
+    const int gid = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int r = gid % L, j = r % S, k = r / S;
+    const bool live = gid / L < m.n_nodes;
+    const int nd = live ? gid / L : m.n_nodes - 1;  // dead lanes shadow a live node for the group sums
+    const IbmBlock B = node_block(g, nd);
     const float4 o = m.origin[m.node_mem[nd]];
     float4 xl = m.X[nd];
-    if (INTERP) {
+    float *row = g.wnode + 64 * (size_t)nd + S * r;  // this lane's S weights
+    if constexpr (INTERP) {
         const int4 nb = g.nbase[nd];
         float ux = 0.0f, uy = 0.0f, uz = 0.0f;
-        if (act) {
-            const int idx[3] = {nb.x + off[0], nb.y + off[1], nb.z + off[2]};
-            int w[3];
-            if (wrap_index(B, 0, idx[0], w[0]) && wrap_index(B, 1, idx[1], w[1]) && wrap_index(B, 2, idx[2], w[2])) {
-                const float wt = g.wnode[64 * (size_t)nd + lane];
+        int w[3];
+        if (wrap_index(B, 1, nb.y + j, w[1]) && wrap_index(B, 2, nb.z + k, w[2])) {
+#pragma unroll
+            for (int i = 0; i < S; i++) {
+                if (!wrap_index(B, 0, nb.x + i, w[0])) continue;
+                const float wt = row[i];
                 const float4 s = st[cell_internal(g, B, w)];
-                ux = wt * s.y; uy = wt * s.z; uz = wt * s.w;
+                ux = fmaf(wt, s.y, ux);
+                uy = fmaf(wt, s.z, uy);
+                uz = fmaf(wt, s.w, uz);
             }
         }
-        ux = wave_sum(ux); uy = wave_sum(uy); uz = wave_sum(uz);
-        xl.x += ux * g.dt; xl.y += uy * g.dt; xl.z += uz * g.dt;
-        if (lane == 0) {
+        ux = group_sum<L>(ux);
+        uy = group_sum<L>(uy);
+        uz = group_sum<L>(uz);
+        xl.x += ux * g.dt;
+        xl.y += uy * g.dt;
+        xl.z += uz * g.dt;
+        if (live && r == 0) {
             m.U[nd] = make_float4(ux, uy, uz, 0.0f);
             m.X[nd] = xl;
         }
@@ -129,58 +185,63 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     float xr[3];
     int base[3];
     node_base(g, B, P, xr, base);
-    float wt = 0.0f;
-    if (act) {
-        const int idx[3] = {base[0] + off[0], base[1] + off[1], base[2] + off[2]};
+    float wt[S];
+    float ws = 0.0f;
+    {
         int w[3];
-        if (wrap_index(B, 0, idx[0], w[0]) && wrap_index(B, 1, idx[1], w[1]) && wrap_index(B, 2, idx[2], w[2])) {
-            if (g.kernel == LBFS_IBM_LAGRANGE4)
-                wt = (float)(lagrange4(g, B, 0, base[0], off[0], (double)P[0]) *
-                             lagrange4(g, B, 1, base[1], off[1], (double)P[1]) *
-                             lagrange4(g, B, 2, base[2], off[2], (double)P[2]));
-            else
-                wt = cell_weight(g, B, idx, w, cell_internal(g, B, w), P, xr);
+        const bool rowok = wrap_index(B, 1, base[1] + j, w[1]) && wrap_index(B, 2, base[2] + k, w[2]);
+#pragma unroll
+        for (int i = 0; i < S; i++) {
+            wt[i] = 0.0f;
+            if (!rowok || !wrap_index(B, 0, base[0] + i, w[0])) continue;
+            const float4 c = K == LBFS_IBM_COS4 ? g.cen[cell_internal(g, B, w)] : float4{};
+            wt[i] = stencil_weight<K>(g, B, P, xr, base, i, j, k, w, c);
+            ws += wt[i];
         }
     }
-    const float ws = wave_sum(wt);
-    const float inv = g.kernel == LBFS_IBM_COS4 ? (ws > 0.0f ? 1.0f / ws : 0.0f) : 1.0f;
-    g.wnode[64 * (size_t)nd + lane] = wt * inv;
-    if (lane == 0) {
-        const int key = B.bin0 + bin_coord(B, 0, base[0]) +
-                        B.tn[0] * (bin_coord(B, 1, base[1]) + B.tn[1] * bin_coord(B, 2, base[2]));
-        g.nbase[nd] = make_int4(base[0], base[1], base[2], key);
-        if (INTERP && g.binned) {
-            // straight into the bin's slots; a full bin sends the node to the
-            // overflow list (spread node by node)
-            const int r = atomicAdd(&g.bin_count[key], 1);
-            if (r == 0 && g.use_list) {  // once per bin and step
-                const int li = atomicAdd(g.n_bin_list, 1);
-                if (li < g.n_bins) g.bin_list[li] = key;
-            }
-            if (r < IBM_BIN_CAP) {
-                g.bin_slots[(size_t)key * IBM_BIN_CAP + r] = nd;
-            } else {
-                const int o = atomicAdd(g.n_overflow, 1);
-                if (o < m.n_nodes) g.overflow[o] = nd;
-            }
+    if constexpr (K == LBFS_IBM_COS4) {
+        ws = group_sum<L>(ws);
+        const float inv = ws > 0.0f ? 1.0f / ws : 0.0f;
+#pragma unroll
+        for (int i = 0; i < S; i++) wt[i] *= inv;
+    }
+    if (!live) return;
+    if constexpr (S == 4) {
+        *reinterpret_cast<float4 *>(row) = make_float4(wt[0], wt[1], wt[2], wt[3]);
+    } else {
+        *reinterpret_cast<float2 *>(row) = make_float2(wt[0], wt[1]);
+    }
+    if (r != 0) return;
+    const int key = node_tile(B, base);
+    g.nbase[nd] = make_int4(base[0], base[1], base[2], key);
+    if (INTERP && g.binned != IBM_SPREAD_NODES) {
+        // listed in its tile (the bin scattering also lists the tile once);
+        // a full tile sends the node to the overflow list
+        const int q = atomicAdd(&g.tile_count[key], 1);
+        if (q == 0 && g.binned == IBM_SPREAD_BINS) {
+            const int li = atomicAdd(g.n_bin_list, 1);
+            if (li < g.n_tiles) g.bin_list[li] = key;
+        }
+        if (q < g.tile_cap) {
+            g.tile_slots[(size_t)key * g.tile_cap + q] = nd;
+        } else {
+            const int e = atomicAdd(g.n_overflow, 1);
+            if (e < m.n_nodes) g.overflow[e] = nd;
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// IBM: spreading.  Per node bin (the 4x4x4 block of cells that holds the
-// nodes' stencil base; its nodes were put in the bin's slots, and the bin in
-// the list of occupied bins, by the interpolation), or per chunk of a bin's
-// nodes: the nodes' stencils stay inside a 7x7x7 window, where the weighted
-// forces are accumulated in LDS (one wavefront per node, one lane per
-// stencil slot), then the window is added to the per-cell accumulators in
-// global memory.  The node forces are gathered from their edge records here
-// (eight lanes per node, the order of k_mem_nodes) and stored.  Nodes of full
-// bins are spread one wavefront each after the bins.  All sums are 64-bit
-// fixed point integer additions, so the result does not depend on the order
-// of the nodes, lanes or workgroups.  Each write flags the cell's fluid tile;
-// k_ibm_apply turns the accumulators of the flagged tiles into force
-// densities and clears them.
+// IBM: spreading (immersed_boundary_method.cu:947-1024).  Tile spreading: one
+// workgroup per fluid tile gathers the nodes whose stencils reach it (those
+// listed in the tiles holding stencil bases from S - 1 cells before the tile
+// to its end, and the overflow list), keeps those that reach it, and spreads
+// them one wavefront each (one lane per stencil cell, the node's kept
+// weights) into per-cell sums of the tile in LDS; it then writes the tile's
+// force densities (no global atomics).  Per-node spreading: one wavefront
+// per node straight into per-cell accumulators in global memory.  All sums
+// are 64-bit fixed-point integer sums, so their value depends neither on the
+// order of the nodes nor on the path: both give the same bits.
 // ---------------------------------------------------------------------------
 // sum of a node's edge slots by eight lanes (sub = lane & 7; 8 or more: no
 // slots), fixed order
@@ -201,120 +262,254 @@ __device__ __forceinline__ float4 node_force8(const MemArgs &m, int nd, int sub)
     return make_float4(fx, fy, fz, 0.0f);
 }
 
-// one node by one wavefront straight into the global accumulators
-__device__ __forceinline__ void spread_node_global(const MemArgs &m, const IbmArgs &g, int nd, float4 F, int lane)
+// one node by one wavefront (lane = stencil cell) into the global accumulators
+template <int K>
+__device__ __forceinline__ void spread_node_global(const MemArgs &m, const IbmArgs &g, int nd, int lane)
 {
-    const int S = stencil_size(g);
+    constexpr int S = Stencil<K>::S;
     if (lane >= S * S * S) return;
     const float wt = g.wnode[64 * (size_t)nd + lane];
     if (wt == 0.0f) return;
-    const IbmBlock B = g.blk[g.node_block[nd]];
+    const IbmBlock B = node_block(g, nd);
     const int4 nb = g.nbase[nd];
-    int off[3];
+    int off[3], w[3];
     lane_offset(S, lane, off);
-    const int idx[3] = {nb.x + off[0], nb.y + off[1], nb.z + off[2]};
-    int w[3];
-    if (!(wrap_index(B, 0, idx[0], w[0]) && wrap_index(B, 1, idx[1], w[1]) && wrap_index(B, 2, idx[2], w[2]))) return;
+    if (!(wrap_index(B, 0, nb.x + off[0], w[0]) && wrap_index(B, 1, nb.y + off[1], w[1]) &&
+          wrap_index(B, 2, nb.z + off[2], w[2])))
+        return;
     const int ic = cell_internal(g, B, w);
+    const float4 F = m.F[nd];
     unsigned long long *acc = g.acc + 3 * (size_t)ic;
-    atomicAdd(acc, fix_term((double)(F.x * wt), FIX_SCALE, g.err, ERR_FIX_IBM));
-    atomicAdd(acc + 1, fix_term((double)(F.y * wt), FIX_SCALE, g.err, ERR_FIX_IBM));
-    atomicAdd(acc + 2, fix_term((double)(F.z * wt), FIX_SCALE, g.err, ERR_FIX_IBM));
-    g.tile_now[g.cell_tile[ic]] = 1;
+    atomicAdd(acc, fix_term_f(F.x * wt, g.err, ERR_FIX_IBM));
+    atomicAdd(acc + 1, fix_term_f(F.y * wt, g.err, ERR_FIX_IBM));
+    atomicAdd(acc + 2, fix_term_f(F.z * wt, g.err, ERR_FIX_IBM));
+    g.tile_acc[g.cell_tile[ic]] = 1;
 }
 
-// LIST (few nodes): the workgroups loop over work items of CH nodes of the
-// listed bins (IBM_BIN_CAP / CH items per bin, empty ones skipped), so a full
-// bin is shared by several workgroups; the apply pass clears the bin counts.
-// Otherwise one workgroup per bin of the grid, its nodes in chunks of CH
-// (small workgroups measured fastest there: 128 threads, 16-node chunks).
-template <int NT, int CH, bool LIST>
-__global__ __launch_bounds__(NT) void k_ibm_spread(MemArgs m, IbmArgs g)
+// offsets along one axis whose (wrapped) cell lies in [lo, lo + dim), and
+// their local index
+template <int S>
+__device__ __forceinline__ unsigned axis_in(const IbmBlock &B, int a, int base, int lo, int dim, int loc[S])
 {
-    constexpr int NW = NT / 64;
-    constexpr int NCH = LIST ? IBM_BIN_CAP / CH : 1;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    __shared__ unsigned long long racc[3][343];
-    __shared__ int cnd[CH];
-    __shared__ int4 cbase[CH];
-    __shared__ float4 cF[CH];
-    __shared__ float cw[CH][64];   // the chunk's cached stencil weights, one row per node
-    const int S = stencil_size(g);
-    int off[3];
-    lane_offset(S, lane, off);
-    const bool act = lane < S * S * S;
-    // uniform over the workgroup
-    const int n_items = LIST ? min(*g.n_bin_list, g.n_bins) * NCH : g.n_bins;
-    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-        const int b = LIST ? g.bin_list[it / NCH] : it;
-        const int cnt_all = min(g.bin_count[b], IBM_BIN_CAP);
-        const int c_begin = LIST ? (it % NCH) * CH : 0;
-        const int c_end = LIST ? min(cnt_all, c_begin + CH) : cnt_all;
-        if (c_begin >= c_end) continue;  // uniform over the workgroup
-        const IbmBlock B = g.blk[g.bin_block[b]];
-        const int lb = b - B.bin0;
-        const int bo[3] = {4 * (lb % B.tn[0]), 4 * ((lb / B.tn[0]) % B.tn[1]), 4 * (lb / (B.tn[0] * B.tn[1]))};
-        __syncthreads();  // the previous item's window is flushed
-        for (int q = tid; q < 3 * 343; q += NT) racc[q / 343][q % 343] = 0ull;
-        const int *slots = g.bin_slots + (size_t)b * IBM_BIN_CAP;
-        for (int c0 = c_begin; c0 < c_end; c0 += CH) {
-            const int cnt = min(CH, c_end - c0);
-            __syncthreads();
-            if (tid < cnt) {
-                const int p = slots[c0 + tid];
-                cnd[tid] = p;
-                cbase[tid] = g.nbase[p];
-            }
-            __syncthreads();
-            // node forces from the edge records: eight lanes per node
-            for (int jn = tid >> 3; jn < cnt; jn += NT / 8) {
-                const int p = cnd[jn];
-                const float4 F = node_force8(m, p, tid & 7);
-                if ((tid & 7) == 0) {
-                    cF[jn] = F;
-                    m.F[p] = F;
-                }
-            }
-            // all weight rows of the chunk in flight at once (one coalesced row per node)
-            for (int q = tid; q < 64 * cnt; q += NT) cw[q >> 6][q & 63] = g.wnode[64 * (size_t)cnd[q >> 6] + (q & 63)];
-            __syncthreads();
-            for (int jn = wave; jn < cnt; jn += NW) {
-                const float wt = cw[jn][lane];
-                if (!act || wt == 0.0f) continue;
-                const int4 nb = cbase[jn];
-                const int bs[3] = {nb.x, nb.y, nb.z};
-                int loc = 0, mul = 1;
-                bool in = true;
+    unsigned in = 0u;
 #pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    int c = bs[a] + off[a];
-                    int l;
-                    if (B.per[a]) {
-                        // window coordinate of the wrapped base, then the slot offset
-                        // (nodes advect freely across periodic boundaries, so
-                        // the base may lie several periods away)
-                        const int wb = ((bs[a] % B.n[a]) + B.n[a]) % B.n[a];
-                        l = wb - bo[a] + off[a];
-                    } else {
-                        in = in && c >= 0 && c < B.n[a];
-                        l = c - bo[a];
-                    }
-                    in = in && l >= 0 && l < 7;
-                    loc += l * mul;
-                    mul *= 7;
+    for (int o = 0; o < S; o++) {
+        int w = 0;
+        const bool ok = wrap_index(B, a, base + o, w);
+        loc[o] = w - lo;
+        if (ok && loc[o] >= 0 && loc[o] < dim) in |= 1u << o;
+    }
+    return in;
+}
+
+template <int K>
+__global__ __launch_bounds__(SPREAD_NT) void k_ibm_spread_tiles(MemArgs m, IbmArgs g)
+{
+    constexpr int S = Stencil<K>::S, C = S * S * S, NPW = 64 / C, NW = SPREAD_NT / 64, NS = S * S * S;
+    extern __shared__ int4 tile_lds[];  // candidates (bases, forces, ids), then 3 x max_tile_cells sums
+    __shared__ int s_tile[NS], s_pre[NS + 1], s_total, s_nov, s_nk;
+    const int T = xcd_remap(blockIdx.x, gridDim.x);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int4 tl = g.tile_lo[T];
+    const IbmBlock B = box(g, tl.w);
+    const int lo[3] = {tl.x, tl.y, tl.z};
+    const int dim[3] = {min(B.tw[0], B.n[0] - lo[0]), min(B.tw[1], B.n[1] - lo[1]), min(B.tw[2], B.n[2] - lo[2])};
+    if (tid == 0) {
+        // tiles holding the bases of nodes that can reach this tile: per axis
+        // those of the cells lo - (S - 1) ... lo (wrapped, or clamped at a wall
+        // as the node tiles are)
+        int tc[3][S], ntc[3];
+        for (int a = 0; a < 3; a++) {
+            ntc[a] = 0;
+            for (int c = lo[a] - (S - 1); c <= lo[a]; c++) {
+                int w = c;
+                if (w < 0) w = B.per[a] ? ((w % B.n[a]) + B.n[a]) % B.n[a] : 0;
+                const int t = w / B.tw[a];
+                bool seen = false;
+                for (int q = 0; q < ntc[a]; q++) seen |= tc[a][q] == t;
+                if (!seen) tc[a][ntc[a]++] = t;
+            }
+        }
+        int k = 0, tot = 0;
+        for (int z = 0; z < ntc[2]; z++)
+            for (int y = 0; y < ntc[1]; y++)
+                for (int x = 0; x < ntc[0]; x++) {
+                    const int id = B.tile0 + tc[0][x] + B.nt[0] * (tc[1][y] + B.nt[1] * tc[2][z]);
+                    const int c = min(g.tile_count[id], g.tile_cap);
+                    if (c == 0) continue;
+                    s_tile[k] = id;
+                    s_pre[k] = tot;
+                    tot += c;
+                    k++;
                 }
-                if (!in) continue;
-                const float4 F = cF[jn];
-                atomicAdd(&racc[0][loc], fix_term((double)(F.x * wt), FIX_SCALE, g.err, ERR_FIX_IBM));
-                atomicAdd(&racc[1][loc], fix_term((double)(F.y * wt), FIX_SCALE, g.err, ERR_FIX_IBM));
-                atomicAdd(&racc[2][loc], fix_term((double)(F.z * wt), FIX_SCALE, g.err, ERR_FIX_IBM));
+        s_pre[k] = tot;
+        s_total = tot;
+        s_nov = min(*g.n_overflow, m.n_nodes);
+    }
+    __syncthreads();
+    const int tot = s_total, nov = s_nov;
+    if (tot + nov == 0) {  // uniform over the workgroup
+        if (tid == 0) g.tile_now[T] = 0;
+        return;
+    }
+    const int cap = g.max_tile_cells;
+    const int nct = dim[0] * dim[1] * dim[2];
+    int4 *cbase = tile_lds;                                              // [SPREAD_NT] stencil bases
+    float4 *cF = reinterpret_cast<float4 *>(cbase + SPREAD_NT);          // [SPREAD_NT] forces
+    int *cand = reinterpret_cast<int *>(cF + SPREAD_NT);                 // [SPREAD_NT] node ids
+    unsigned long long *tacc = reinterpret_cast<unsigned long long *>(cand + SPREAD_NT);  // 3 x cap sums
+    for (int i = tid; i < nct; i += SPREAD_NT) tacc[i] = tacc[cap + i] = tacc[2 * cap + i] = 0ull;
+    for (int q0 = 0; q0 < tot + nov; q0 += SPREAD_NT) {
+        if (tid == 0) s_nk = 0;
+        __syncthreads();  // the previous chunk is spread
+        const int q = q0 + tid;
+        if (q < tot + nov) {
+            int nd;
+            if (q < tot) {
+                int k = 0;
+                while (q >= s_pre[k + 1]) k++;
+                nd = g.tile_slots[(size_t)s_tile[k] * g.tile_cap + (q - s_pre[k])];
+            } else {
+                nd = g.overflow[q - tot];
+            }
+            if (q < tot || g.node_block[nd] == tl.w) {
+                const int4 nb = g.nbase[nd];
+                int loc[S];
+                const unsigned ix = axis_in<S>(B, 0, nb.x, lo[0], dim[0], loc);
+                const unsigned iy = axis_in<S>(B, 1, nb.y, lo[1], dim[1], loc);
+                const unsigned iz = axis_in<S>(B, 2, nb.z, lo[2], dim[2], loc);
+                if (ix && iy && iz) {
+                    // keep it: the spreading pass below reads it from LDS
+                    const int e = atomicAdd(&s_nk, 1);
+                    cand[e] = nd;
+                    cbase[e] = nb;
+                    cF[e] = m.F[nd];
+                }
             }
         }
         __syncthreads();
-        for (int q = tid; q < 343; q += NT) {
+        const int nk = s_nk;
+        int off[3];
+        lane_offset(S, lane % C, off);
+        // UN nodes per wavefront and pass: their weight loads in flight together
+        constexpr int UN = 4;
+        for (int e0 = wave * NPW * UN; e0 < nk; e0 += NW * NPW * UN) {
+            float wt[UN];
+#pragma unroll
+            for (int u = 0; u < UN; u++) {
+                const int e = e0 + u * NPW + lane / C;
+                wt[u] = e < nk ? g.wnode[64 * (size_t)cand[e] + lane % C] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < UN; u++) {
+                if (wt[u] == 0.0f) continue;
+                const int e = e0 + u * NPW + lane / C;
+                const int4 nb = cbase[e];
+                int w[3];
+                if (!(wrap_index(B, 0, nb.x + off[0], w[0]) && wrap_index(B, 1, nb.y + off[1], w[1]) &&
+                      wrap_index(B, 2, nb.z + off[2], w[2])))
+                    continue;
+                const int l0 = w[0] - lo[0], l1 = w[1] - lo[1], l2 = w[2] - lo[2];
+                if (l0 < 0 || l0 >= dim[0] || l1 < 0 || l1 >= dim[1] || l2 < 0 || l2 >= dim[2]) continue;
+                const int li = l0 + dim[0] * (l1 + dim[1] * l2);
+                const float4 F = cF[e];
+                atomicAdd(tacc + li, fix_term_f(F.x * wt[u], g.err, ERR_FIX_IBM));
+                atomicAdd(tacc + cap + li, fix_term_f(F.y * wt[u], g.err, ERR_FIX_IBM));
+                atomicAdd(tacc + 2 * cap + li, fix_term_f(F.z * wt[u], g.err, ERR_FIX_IBM));
+            }
+        }
+    }
+    __syncthreads();
+    const int c0 = g.tile_cell_start[T];
+    int any = 0;
+    for (int i = tid; i < nct; i += SPREAD_NT) {
+        const unsigned long long v[3] = {tacc[i], tacc[cap + i], tacc[2 * cap + i]};
+        const float sc = g.spread_sc[c0 + i];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            g.spread_f[3 * (size_t)(c0 + i) + c] = (float)((double)(long long)v[c] / FIX_SCALE) * sc;
+        any |= (v[0] | v[1] | v[2]) != 0ull;
+    }
+    any = __syncthreads_or(any);
+    if (tid == 0) g.tile_now[T] = any;
+}
+
+// Bin scattering: the workgroups loop over work items of up to 32 nodes of
+// the listed bins (IBM_BIN_CAP / 32 items per bin, empty ones skipped), so a
+// full bin is shared by several workgroups.  Each item spreads its nodes (one
+// wavefront each, one lane per stencil cell, the kept weights) into an LDS
+// window of the bin and the S - 1 cells beyond its upper faces, then adds the
+// window's nonzero sums to the per-cell accumulators.
+constexpr int BIN_NT = 512, BIN_CH = 32;
+
+template <int K>
+__global__ __launch_bounds__(BIN_NT) void k_ibm_spread_bins(MemArgs m, IbmArgs g)
+{
+    constexpr int S = Stencil<K>::S, C = S * S * S, NPW = 64 / C, NW = BIN_NT / 64;
+    constexpr int NCH = IBM_BIN_CAP / BIN_CH, WD = 4 + S - 1, NWIN = WD * WD * WD;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ unsigned long long racc[3][NWIN];
+    __shared__ int cnd[BIN_CH];
+    __shared__ int4 cbase[BIN_CH];
+    __shared__ float4 cF[BIN_CH];
+    __shared__ float cw[BIN_CH][C];  // the item's kept weights, one row per node
+    int off[3];
+    lane_offset(S, lane % C, off);
+    const int n_items = min(*g.n_bin_list, g.n_tiles) * NCH;  // uniform over the workgroup
+    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const int b = g.bin_list[it / NCH];
+        const int cnt_all = min(g.tile_count[b], IBM_BIN_CAP);
+        const int c_begin = (it % NCH) * BIN_CH, cnt = min(cnt_all, c_begin + BIN_CH) - c_begin;
+        if (cnt <= 0) continue;  // uniform over the workgroup
+        const int4 tl = g.tile_lo[b];
+        const IbmBlock B = box(g, tl.w);
+        const int bo[3] = {tl.x, tl.y, tl.z};
+        __syncthreads();  // the previous item's window is flushed
+        for (int q = tid; q < 3 * NWIN; q += BIN_NT) racc[q / NWIN][q % NWIN] = 0ull;
+        if (tid < cnt) {
+            const int p = g.tile_slots[(size_t)b * IBM_BIN_CAP + c_begin + tid];
+            cnd[tid] = p;
+            cbase[tid] = g.nbase[p];
+            cF[tid] = m.F[p];
+        }
+        __syncthreads();
+        // all weight rows of the item in flight at once (one coalesced row per node)
+        for (int q = tid; q < C * cnt; q += BIN_NT) cw[q / C][q % C] = g.wnode[64 * (size_t)cnd[q / C] + q % C];
+        __syncthreads();
+        for (int jn = wave * NPW + lane / C; jn < cnt; jn += NW * NPW) {
+            const float wt = cw[jn][lane % C];
+            if (wt == 0.0f) continue;
+            const int4 nb = cbase[jn];
+            const int bs[3] = {nb.x, nb.y, nb.z};
+            int loc = 0, mul = 1;
+            bool in = true;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                int l;
+                if (B.per[a]) {
+                    // window coordinate of the wrapped base (nodes advect freely
+                    // across periodic boundaries: the base may lie periods away)
+                    l = ((bs[a] % B.n[a]) + B.n[a]) % B.n[a] - bo[a] + off[a];
+                } else {
+                    const int c = bs[a] + off[a];
+                    in = in && c >= 0 && c < B.n[a];
+                    l = c - bo[a];
+                }
+                in = in && l >= 0 && l < WD;
+                loc += l * mul;
+                mul *= WD;
+            }
+            if (!in) continue;
+            const float4 F = cF[jn];
+            atomicAdd(&racc[0][loc], fix_term_f(F.x * wt, g.err, ERR_FIX_IBM));
+            atomicAdd(&racc[1][loc], fix_term_f(F.y * wt, g.err, ERR_FIX_IBM));
+            atomicAdd(&racc[2][loc], fix_term_f(F.z * wt, g.err, ERR_FIX_IBM));
+        }
+        __syncthreads();
+        for (int q = tid; q < NWIN; q += BIN_NT) {
             const unsigned long long v0 = racc[0][q], v1 = racc[1][q], v2 = racc[2][q];
             if ((v0 | v1 | v2) == 0ull) continue;
-            const int l[3] = {q % 7, (q / 7) % 7, q / 49};
+            const int l[3] = {q % WD, (q / WD) % WD, q / (WD * WD)};
             int w[3];
             bool ok = true;
 #pragma unroll
@@ -333,52 +528,41 @@ __global__ __launch_bounds__(NT) void k_ibm_spread(MemArgs m, IbmArgs g)
             if (v0) atomicAdd(acc, v0);
             if (v1) atomicAdd(acc + 1, v1);
             if (v2) atomicAdd(acc + 2, v2);
-            g.tile_now[g.cell_tile[ic]] = 1;
+            g.tile_acc[g.cell_tile[ic]] = 1;
         }
-        // (the grid form) read by this workgroup only; cleared for the next step
-        if (!LIST && tid == 0) g.bin_count[b] = 0;
     }
-    // nodes of full bins, one wavefront each
+    // nodes of full bins: one wavefront each, straight into the accumulators
     const int nov = min(*g.n_overflow, m.n_nodes);
-    for (int k = blockIdx.x * NW + wave; k < nov; k += gridDim.x * NW) {
-        const int nd = g.overflow[k];
-        float4 F = node_force8(m, nd, lane & 7);
-        F.x = __shfl(F.x, 0, 64);
-        F.y = __shfl(F.y, 0, 64);
-        F.z = __shfl(F.z, 0, 64);
-        if (lane == 0) m.F[nd] = F;
-        spread_node_global(m, g, nd, F, lane);
-    }
+    for (int e = blockIdx.x * NW + wave; e < nov; e += gridDim.x * NW) spread_node_global<K>(m, g, g.overflow[e], lane);
 }
 
-// Spreading for small node counts: one wavefront per node, one lane per
-// stencil slot, adding its weighted force straight into the cell's 64-bit
-// fixed-point accumulators (integer additions: order independent).  No node
-// binning is needed.
+// per-node spreading of every node: one wavefront each
+template <int K>
 __global__ __launch_bounds__(256) void k_ibm_spread_nodes(MemArgs m, IbmArgs g)
 {
-    const int lane = threadIdx.x & 63;
     const int nd = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (nd >= m.n_nodes) return;
-    spread_node_global(m, g, nd, m.F[nd], lane);
+    spread_node_global<K>(m, g, nd, threadIdx.x & 63);
 }
 
-// One workgroup per fluid tile: the spread sums of the tile's cells turned
-// into force densities (static force + sum * scale), added to the state the
-// face pass updated with the static force (rho u += dt f, the update's own
-// form: u_new = u* + dt f / rho_new, rho_new unchanged by the force), written
-// to the cell force output and cleared.
+// One workgroup per fluid tile: the tile's spread force densities (written
+// by the tile spreading, or from the per-node accumulators, which are
+// cleared) added to the state the face pass updated with the static force
+// (rho u += dt f, the update's own form: u_new = u* + dt f / rho_new, rho_new
+// unchanged by the force) and written to the cell force output.  Clears the
+// tile's node count for the next step.
 __global__ __launch_bounds__(256) void k_ibm_apply(IbmArgs g, ApplyArgs p)
 {
-    const int t = blockIdx.x, tid = threadIdx.x;
+    const int t = xcd_remap(blockIdx.x, gridDim.x), tid = threadIdx.x;
+    const bool gather = g.binned == IBM_SPREAD_GATHER;
     if (t == 0) {
-        // the listed bins' counts (the spreading shares a bin between workgroups)
-        if (g.binned && g.use_list) {
-            const int nl = min(*g.n_bin_list, g.n_bins);
-            for (int e = tid; e < nl; e += blockDim.x) g.bin_count[g.bin_list[e]] = 0;
+        // the listed bins' counts (the next interpolation lists the nodes anew)
+        if (g.binned == IBM_SPREAD_BINS) {
+            const int nl = min(*g.n_bin_list, g.n_tiles);
+            for (int e = tid; e < nl; e += blockDim.x) g.tile_count[g.bin_list[e]] = 0;
         }
         __syncthreads();
-        if (tid == 0 && g.binned) {
+        if (tid == 0 && g.binned != IBM_SPREAD_NODES) {
             *g.n_overflow = 0;
             *g.n_bin_list = 0;
         }
@@ -397,37 +581,43 @@ __global__ __launch_bounds__(256) void k_ibm_apply(IbmArgs g, ApplyArgs p)
             s[0] = s[1] = s[2] = 0ull;
         }
     }
-    const int now = g.tile_now[t], prev = g.tile_prev[t];
+    if (tid == 0 && gather) g.tile_count[t] = 0;
+    const int now = gather ? g.tile_now[t] : g.tile_acc[t], prev = g.tile_prev[t];
     if (!(now | prev | p.all)) return;  // uniform over the workgroup
+    int any = 0;
     for (int i = p.tile_cell_start[t] + tid; i < p.tile_cell_start[t + 1]; i += blockDim.x) {
         float f[3] = {0.0f, 0.0f, 0.0f};
         if (now) {
-            unsigned long long *a = g.acc + 3 * (size_t)i;
-            const float sc = g.spread_sc[i];
-            bool any = false;
+            if (gather) {
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const long long v = (long long)a[c];
-                f[c] = (float)((double)v / FIX_SCALE) * sc;
-                any |= v != 0;
+                for (int c = 0; c < 3; c++) f[c] = g.spread_f[3 * (size_t)i + c];
+            } else {
+                unsigned long long *a = g.acc + 3 * (size_t)i;
+                const unsigned long long v[3] = {a[0], a[1], a[2]};
+                if ((v[0] | v[1] | v[2]) != 0ull) {
+                    const float sc = g.spread_sc[i];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) f[c] = (float)((double)(long long)v[c] / FIX_SCALE) * sc;
+                    a[0] = a[1] = a[2] = 0ull;
+                }
             }
-            if (any) {
-                a[0] = a[1] = a[2] = 0ull;
+            if (f[0] != 0.0f || f[1] != 0.0f || f[2] != 0.0f) {
                 float4 s = p.st[i];
                 const float q = p.dt / (p.rho0 + s.x);
                 s.y += q * f[0];
                 s.z += q * f[1];
                 s.w += q * f[2];
                 p.st[i] = s;
+                any = 1;
             }
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) p.cell_force_out[3 * (size_t)i + c] = g.static_force[3 * (size_t)i + c] + f[c];
     }
-    __syncthreads();
+    any = __syncthreads_or(any);
     if (tid == 0) {
-        if (now) g.tile_now[t] = 0;
-        if (!p.all) g.tile_prev[t] = now;
+        if (!gather) g.tile_acc[t] = 0;
+        if (!p.all) g.tile_prev[t] = any;
     }
 }
 
@@ -504,15 +694,27 @@ __global__ __launch_bounds__(256) void k_mem_nodes(MemArgs m)
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+template <bool INTERP>
+static hipError_t launch_interp_kernel(const MemArgs &m, const IbmArgs &g, const float4 *st, hipStream_t s)
+{
+    // S x S lanes per node
+    const int lanes = g.kernel == LBFS_IBM_LINEAR2 ? 4 : 16;
+    const dim3 grid((unsigned)(((size_t)m.n_nodes * lanes + 255) / 256)), blk(256);
+    if (g.kernel == LBFS_IBM_LINEAR2)
+        hipLaunchKernelGGL((k_ibm_interp<LBFS_IBM_LINEAR2, INTERP>), grid, blk, 0, s, m, g, st);
+    else if (g.kernel == LBFS_IBM_LAGRANGE4)
+        hipLaunchKernelGGL((k_ibm_interp<LBFS_IBM_LAGRANGE4, INTERP>), grid, blk, 0, s, m, g, st);
+    else
+        hipLaunchKernelGGL((k_ibm_interp<LBFS_IBM_COS4, INTERP>), grid, blk, 0, s, m, g, st);
+    return hipGetLastError();
+}
 hipError_t launch_ibm_interp(const MemArgs &m, const IbmArgs &g, const float4 *st, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ibm_interp<true>, dim3((m.n_nodes + 3) / 4), dim3(256), 0, s, m, g, st);
-    return hipGetLastError();
+    return launch_interp_kernel<true>(m, g, st, s);
 }
 hipError_t launch_ibm_weights(const MemArgs &m, const IbmArgs &g, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ibm_interp<false>, dim3((m.n_nodes + 3) / 4), dim3(256), 0, s, m, g, nullptr);
-    return hipGetLastError();
+    return launch_interp_kernel<false>(m, g, nullptr, s);
 }
 hipError_t launch_mem_edges(const MemArgs &m, hipStream_t s)
 {
@@ -536,15 +738,30 @@ hipError_t launch_mem_nodes(const MemArgs &m, hipStream_t s)
 }
 hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s)
 {
-    // the workgroups loop over the listed bins (or take one bin of the grid
-    // each), then over the nodes of full bins
-    if (g.binned && g.use_list)
-        hipLaunchKernelGGL((k_ibm_spread<512, 32, true>), dim3(std::min(g.n_bins * (IBM_BIN_CAP / 32), 1024)), dim3(512),
-                           0, s, m, g);
-    else if (g.binned)
-        hipLaunchKernelGGL((k_ibm_spread<128, 16, false>), dim3(g.n_bins), dim3(128), 0, s, m, g);
-    else
-        hipLaunchKernelGGL(k_ibm_spread_nodes, dim3((m.n_nodes + 3) / 4), dim3(256), 0, s, m, g);
+    if (g.binned == IBM_SPREAD_BINS) {
+        const dim3 grid(std::min(g.n_tiles * (IBM_BIN_CAP / BIN_CH), 1024)), blk(BIN_NT);
+        if (g.kernel == LBFS_IBM_LINEAR2)
+            hipLaunchKernelGGL(k_ibm_spread_bins<LBFS_IBM_LINEAR2>, grid, blk, 0, s, m, g);
+        else if (g.kernel == LBFS_IBM_LAGRANGE4)
+            hipLaunchKernelGGL(k_ibm_spread_bins<LBFS_IBM_LAGRANGE4>, grid, blk, 0, s, m, g);
+        else
+            hipLaunchKernelGGL(k_ibm_spread_bins<LBFS_IBM_COS4>, grid, blk, 0, s, m, g);
+    } else if (g.binned == IBM_SPREAD_GATHER) {
+        const size_t lds = spread_tile_lds(g.max_tile_cells);
+        const dim3 grid(g.n_tiles), blk(SPREAD_NT);
+        if (g.kernel == LBFS_IBM_LINEAR2)
+            hipLaunchKernelGGL(k_ibm_spread_tiles<LBFS_IBM_LINEAR2>, grid, blk, lds, s, m, g);
+        else if (g.kernel == LBFS_IBM_LAGRANGE4)
+            hipLaunchKernelGGL(k_ibm_spread_tiles<LBFS_IBM_LAGRANGE4>, grid, blk, lds, s, m, g);
+        else
+            hipLaunchKernelGGL(k_ibm_spread_tiles<LBFS_IBM_COS4>, grid, blk, lds, s, m, g);
+    } else {
+        const dim3 grid((m.n_nodes + 3) / 4), blk(256);
+        if (g.kernel == LBFS_IBM_LINEAR2) hipLaunchKernelGGL(k_ibm_spread_nodes<LBFS_IBM_LINEAR2>, grid, blk, 0, s, m, g);
+        else if (g.kernel == LBFS_IBM_LAGRANGE4)
+            hipLaunchKernelGGL(k_ibm_spread_nodes<LBFS_IBM_LAGRANGE4>, grid, blk, 0, s, m, g);
+        else hipLaunchKernelGGL(k_ibm_spread_nodes<LBFS_IBM_COS4>, grid, blk, 0, s, m, g);
+    }
     return hipGetLastError();
 }
 

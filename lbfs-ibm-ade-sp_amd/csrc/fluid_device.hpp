@@ -1,6 +1,6 @@
-// Device helpers shared by the fluid kernels and their adjoints: the XCD
-// remap of workgroup ids, gradient loads, compile-time lattice direction
-// sums and the equilibrium deviation of one lattice point.
+// Device helpers shared by the fluid kernels and their adjoints: gradient
+// loads, compile-time lattice direction sums and the equilibrium deviation of
+// one lattice point.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -9,18 +9,9 @@
 
 #include "fluid_kernels.hpp"
 #include "lattice.hpp"
+#include "xcd.hpp"
 
 namespace lbfs {
-
-__device__ __forceinline__ int xcd_remap(int bid, int nblocks)
-{
-    // consecutive tile ids land on the same XCD (8 XCDs, round-robin dispatch)
-    const int per = (nblocks + 7) >> 3;
-    const int xcd = bid & 7, idx = bid >> 3;
-    const int full = nblocks - (per - 1) * 8;  // XCDs that get `per` blocks
-    int base = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
-    return base + idx;
-}
 
 __device__ __forceinline__ void load_grad(const float4 *__restrict__ grad, int i, float g[12])
 {

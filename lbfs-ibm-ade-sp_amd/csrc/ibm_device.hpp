@@ -26,6 +26,17 @@ __device__ __forceinline__ unsigned long long fix_term(double v, double scale, i
     return 0ull;
 }
 
+// the same term for a float value: v * 2^40 is exact in float, so the range
+// test and the rounding give the bits fix_term gives for (double)v
+__device__ __forceinline__ unsigned long long fix_term_f(float v, int *err, int bit)
+{
+    const float s = v * 1099511627776.0f;  // 2^40
+    if (fabsf(s) < 4503599627370496.0f)    // 2^52 (false for NaN)
+        return (unsigned long long)(long long)rintf(s);
+    atomicOr(err, bit);
+    return 0ull;
+}
+
 struct f3 {
     float x, y, z;
 };
@@ -142,22 +153,37 @@ __device__ __forceinline__ float cos4_dphi(float r)
     return -sn * dsn * (PI_F * 0.25f);
 }
 
+// 2-point kernel factor of stencil index idx along one axis (xr: the node's
+// grid coordinate)
+__device__ __forceinline__ float linear2_factor(int idx, float xr) { return 1.0f - fabsf((float)idx + 0.5f - xr); }
+
+// 4-point cosine weight of a cell with centroid c for a node at P; d: the
+// periods between the stencil index and the wrapped cell along each axis
+__device__ __forceinline__ float cos4_cell(const IbmBlock &B, const float4 c, const int d[3], const float P[3])
+{
+    const float cc[3] = {c.x, c.y, c.z};
+    float wt = 1.0f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float xc = fmaf((float)d[a], B.period_step[a], cc[a]);
+        const float r = (xc - P[a]) * B.inv_h;
+        wt *= fabsf(r) < 2.0f ? cos4_phi(r) : 0.0f;
+    }
+    return wt;
+}
+
+// unnormalised weight of the stencil cell with unwrapped index idx, wrapped
+// index w and internal id `internal`, for a node at P (grid coordinate xr)
 __device__ __forceinline__ float cell_weight(const IbmArgs &g, const IbmBlock &B, const int idx[3], const int w[3],
                                              int internal, const float P[3], const float xr[3])
 {
     float wt = 1.0f;
     if (g.kernel == LBFS_IBM_LINEAR2) {
 #pragma unroll
-        for (int a = 0; a < 3; a++) wt *= 1.0f - fabsf((float)idx[a] + 0.5f - xr[a]);
+        for (int a = 0; a < 3; a++) wt *= linear2_factor(idx[a], xr[a]);
     } else {
-        const float4 c = g.cen[internal];
-        const float cc[3] = {c.x, c.y, c.z};
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            const float xc = cc[a] + (float)(idx[a] - w[a]) * (B.L[a] / (float)B.n[a]);
-            const float r = (xc - P[a]) * B.inv_h;
-            wt *= fabsf(r) < 2.0f ? cos4_phi(r) : 0.0f;
-        }
+        const int d[3] = {idx[0] - w[0], idx[1] - w[1], idx[2] - w[2]};
+        wt = cos4_cell(B, g.cen[internal], d, P);
     }
     return wt;
 }
@@ -171,7 +197,7 @@ __device__ __forceinline__ float cos4_weight_grad(const IbmBlock &B, const float
     float ph[3], dph[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        const float xc = cc[a] + (float)(idx[a] - w[a]) * (B.L[a] / (float)B.n[a]);
+        const float xc = fmaf((float)(idx[a] - w[a]), B.period_step[a], cc[a]);
         const float r = (xc - P[a]) * B.inv_h;
         const bool in = fabsf(r) < 2.0f;
         ph[a] = in ? cos4_phi(r) : 0.0f;
@@ -186,18 +212,35 @@ __device__ __forceinline__ float cos4_weight_grad(const IbmBlock &B, const float
     return wt;
 }
 
+// internal id of the cell with (wrapped) index w: from the tile layout in a
+// box of whole tiles (tiles x fastest, cells x fastest within a tile), else
+// through the id map
+// the box of node nd, and box b (a single box comes by value with the arguments)
+__device__ __forceinline__ IbmBlock node_block(const IbmArgs &g, int nd)
+{
+    return g.n_blocks == 1 ? g.blk0 : g.blk[g.node_block[nd]];
+}
+__device__ __forceinline__ IbmBlock box(const IbmArgs &g, int b) { return g.n_blocks == 1 ? g.blk0 : g.blk[b]; }
+
 __device__ __forceinline__ int cell_internal(const IbmArgs &g, const IbmBlock &B, const int w[3])
 {
+    if (B.regular) {
+        const int t0 = w[0] / B.ftw[0], t1 = w[1] / B.ftw[1], t2 = w[2] / B.ftw[2];
+        const int l = (w[0] - t0 * B.ftw[0]) + B.ftw[0] * ((w[1] - t1 * B.ftw[1]) + B.ftw[1] * (w[2] - t2 * B.ftw[2]));
+        return B.icell0 + (t0 + B.fnt[0] * (t1 + B.fnt[1] * t2)) * (B.ftw[0] * B.ftw[1] * B.ftw[2]) + l;
+    }
     return g.inv[B.cell0 + w[0] + B.n[0] * (w[1] + B.n[1] * w[2])];
 }
 
-__device__ __forceinline__ int bin_coord(const IbmBlock &B, int a, int base)
+// fluid tile coordinate along axis a of a node whose stencil base index is
+// `base` (wrapped on periodic axes, clamped into the box on the others)
+__device__ __forceinline__ int tile_coord(const IbmBlock &B, int a, int base)
 {
     const int n = B.n[a];
     int b = base;
     if (B.per[a]) b = ((b % n) + n) % n;
     else b = b < 0 ? 0 : (b >= n ? n - 1 : b);
-    return b >> 2;
+    return b / B.tw[a];
 }
 
 // ---------------------------------------------------------------------------

@@ -362,6 +362,7 @@ int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<d
     // cell order: TX x TY x TZ blocks of the logical grid when the mesh has
     // one, else consecutive chunks of TX TY TZ caller ids
     const int TX = tile[0], TY = tile[1], TZ = tile[2], TN = TX * TY * TZ;
+    for (int a = 0; a < 3; a++) d.tile[a] = tile[a];
     if (!blocks.empty()) {
         for (const HostBlock &b : blocks) {
             const int nx = b.n[0], ny = b.n[1], nz = b.n[2];

@@ -67,6 +67,7 @@ const double *cell_body_force(const std::vector<HostBlock> &blocks, int c, const
 // Tile-blocked device layout (all indices are internal, tile-ordered ids).
 struct DeviceLayout {
     int n_cells = 0, n_bc = 0, n_tiles = 0, max_tile_faces = 0;
+    int tile[3] = {0, 0, 0};      // cells per tile along x, y, z of the logical boxes
     std::vector<int> perm;        // internal id -> caller id
     std::vector<int> inv;         // caller id -> internal id
     std::vector<int> tile_cell_start, tile_face_start;

@@ -24,13 +24,15 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc, bool fused_fl
     if (have_membranes) {
         // IBM interpolation from the state at the start of the step and node
         // advection, then membrane forces at the advected nodes and spreading
-        // (Solver_gpu.cu:808-915): X 12 r/w, U 12 w, per-node stencil data
+        // (Solver_gpu.cu:808-915).  Interpolation: X read and written 32, U 16,
+        // node indices 8, stencil base and tile 16, weight sum 4, slot 8 per
+        // node (the stencil cells' state, centroids and ids are cache reads)
         const double nn = mem.n_nodes, ne = mem.n_edges, nt = mem.n_tri;
         s.push_back({"ibm_interp", [ctx]() -> int {
                          HIPCHK(launch_ibm_interp(ctx->mem, ctx->ibm, ctx->st[ctx->cur], ctx->stream));
                          return LBFS_OK;
                      },
-                     60.0 * nn + 16.0 * ne});
+                     84.0 * nn});
         // volume and area sums: triangle 16 + nodes 48 per triangle; edge
         // forces: nodes 96 + indices 24 + constants 24 + state 16 + forces
         // written 64 per edge
@@ -44,23 +46,26 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc, bool fused_fl
                          return LBFS_OK;
                      },
                      224.0 * ne});
-        if (!ibm.binned)  // the binned spreading gathers the node forces itself
-            s.push_back({"mem_nodes", [ctx]() -> int {
-                             HIPCHK(launch_mem_nodes(ctx->mem, ctx->stream));
-                             return LBFS_OK;
-                         },
-                         36.0 * nn});
+        s.push_back({"mem_nodes", [ctx]() -> int {
+                         HIPCHK(launch_mem_nodes(ctx->mem, ctx->stream));
+                         return LBFS_OK;
+                     },
+                     36.0 * nn});
         if (with_visc)
             s.push_back({"internal_visc", [ctx]() -> int {
                              HIPCHK(launch_internal_visc(ctx->mem, ctx->ibm, ctx->delta_h, ctx->visc_factor, ctx->stream));
                              return LBFS_OK;
                          },
                          8.0 * n});
+        // spreading: per node read about 2.5 times (nodes listed around
+        // several tiles): base 16, weight sum 4, force 16, X 16, indices 8;
+        // per touched cell the centroid 16 and the force density written 12
+        // (about 64 cells per node, each cell reached by about 8 nodes)
         s.push_back({"ibm_spread", [ctx]() -> int {
                          HIPCHK(launch_ibm_spread(ctx->mem, ctx->ibm, ctx->stream));
                          return LBFS_OK;
                      },
-                     36.0 * 64.0 * nn / 16.0});
+                     2.5 * 60.0 * nn + 28.0 * std::min(n, 8.0 * nn)});
     }
     const bool fused = fused_fluid && fused_ok && args.ts_mode != LBFS_TS_MIN;
     if (args.ts_mode == LBFS_TS_MIN)
@@ -80,10 +85,10 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc, bool fused_fl
         // The face pass updates the cell with the static force (it does not
         // need the spread force, so it runs beside the coupling chain); the
         // apply pass then adds the spread force density in the tiles the
-        // spreading wrote into.  Face pass as below; apply: spread sums 24 +
-        // scale 4 + static force 12 + state read and written 32 + force
-        // written 12 per touched cell (about 343 cells per 16 nodes)
-        const double touched = std::min(n, 343.0 * mem.n_nodes / 16.0);
+        // spreading wrote into.  Face pass as below; apply: force density 12
+        // + static force 12 + state read and written 32 + force written 12
+        // per touched cell (about 8 per node)
+        const double touched = std::min(n, 8.0 * mem.n_nodes);
         if (fused)
             // the fused pass moves the bytes of both passes' model (the
             // gradients stay in LDS)
@@ -106,7 +111,7 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc, bool fused_fl
                          ctx->cur ^= 1;
                          return LBFS_OK;
                      },
-                     84.0 * touched});
+                     68.0 * touched});
         return s;
     }
     // pass F: state 16 + gradients 48 + volume/centroid 16 + force 12 + state written 16 per cell,
@@ -172,7 +177,7 @@ int lbfs_ctx::run_stages(bool with_visc)
         if (!interp_first) HIPCHK(launch_ibm_interp(mem, ibm, st[cur], side));
         HIPCHK(launch_mem_volume(mem, side));
         HIPCHK(launch_mem_edges(mem, side));
-        if (!ibm.binned) HIPCHK(launch_mem_nodes(mem, side));
+        HIPCHK(launch_mem_nodes(mem, side));
         HIPCHK(launch_ibm_spread(mem, ibm, side));
         HIPCHK(hipEventRecord(ev_join, side));
         HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
@@ -194,7 +199,7 @@ int lbfs_ctx::run_stages(bool with_visc)
     HIPCHK(hipStreamWaitEvent(side2, ev_fork2, 0));
     HIPCHK(launch_mem_volume(mem, side2));
     HIPCHK(launch_mem_edges(mem, side2));
-    if (!ibm.binned) HIPCHK(launch_mem_nodes(mem, side2));
+    HIPCHK(launch_mem_nodes(mem, side2));
     HIPCHK(hipEventRecord(ev_join2, side2));
     HIPCHK(hipStreamWaitEvent(stream, ev_join2, 0));
     HIPCHK(launch_ibm_spread(mem, ibm, stream));

@@ -685,8 +685,9 @@ class Solver:
         check(load().lbfs_set_option(self.ctx, OPT_FUSED_FLUID, 1 if on else 0), self.ctx)
 
     def set_ibm_binned(self, mode):
-        """Spreading path of the next add_membranes: -1 by node count, 0 per node, 1 per bin
-        (occupied bins listed for up to 65536 nodes), 2 per bin over the whole bin grid."""
+        """Spreading path of the next add_membranes: -1 (default) by node density, 0 per node
+        into global accumulators, 1 per fluid tile (each tile gathers its nodes, sums in LDS),
+        2 per occupied 4x4x4 bin (LDS windows added to the accumulators)."""
         check(load().lbfs_set_option(self.ctx, OPT_IBM_BINNED, int(mode)), self.ctx)
 
     def set_step_schedule(self, mode):

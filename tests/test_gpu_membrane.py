@@ -293,23 +293,25 @@ def test_lagrange4_coupled_parity():
     assert cases.rel_l2(np.concatenate([us, vs, ws]), np.concatenate([uo, vo, wo])) < 1e-5
 
 
-@pytest.mark.parametrize("kernel", [abi.IBM_COS4, abi.IBM_LINEAR2])
-def test_spreading_paths_are_bitwise_equal(kernel):
-    """Per-node spreading into integer cell accumulators and binned per-bin
-    spreading through LDS windows (over the list of occupied bins, or over the
-    whole bin grid) add the same 64-bit fixed-point terms, so whole coupled
-    steps agree bit for bit."""
+@pytest.mark.parametrize("kernel,tile", [(abi.IBM_COS4, (0, 0, 0)), (abi.IBM_LINEAR2, (0, 0, 0)),
+                                         (abi.IBM_COS4, (3, 5, 2)), (abi.IBM_COS4, (16, 4, 4))])
+def test_spreading_paths_are_bitwise_equal(kernel, tile):
+    """Per-node spreading into integer cell accumulators in global memory,
+    per-tile gathering (each fluid tile gathering the nodes listed around it,
+    sums in LDS) and per-bin scattering (LDS windows added to the
+    accumulators) add the same 64-bit fixed-point terms, so whole coupled
+    steps agree bit for bit; also with tiles narrower than the stencil (3x5x2)
+    and ragged tiles (16 cells along a 24-cell axis)."""
     outs = []
     for mode in (0, 1, 2):
         mesh = cases.channel_mesh(24, 16, 16, seed=12)
-        s = abi.Solver(mesh, abi.make_params(visc=VISC, dt=DT, body_force=(2e-5, 0.0, 0.0)))
+        s = abi.Solver(mesh, abi.make_params(visc=VISC, dt=DT, body_force=(2e-5, 0.0, 0.0), tile=tile))
         s.set_state(*cases.smooth_state(mesh, seed=12, amp=0.005))
         s.set_ibm_binned(mode)
         xa, ta = abi.rbc(4, 8.0, center=(8.0, 8.0, 8.0))
         xb, tb = abi.rbc(3, 6.0, center=(18.0, 8.0, 8.0))
         s.add_membranes([abi.Membrane(xa, ta, **PARAMS), abi.Membrane(xb, tb, **PARAMS)], ibm_kernel=kernel)
-        names = s.stage_names()
-        assert ("mem_nodes" in names) == (mode == 0)  # the binned path gathers node forces itself
+        assert "mem_nodes" in s.stage_names()
         s.step(9)
         X, U, F = s.membrane_nodes()
         outs.append((np.concatenate(s.get_state()), s.cell_force(), X, F))
@@ -319,10 +321,12 @@ def test_spreading_paths_are_bitwise_equal(kernel):
             assert np.array_equal(a, b)
 
 
-def test_full_bins_overflow_to_the_node_list():
-    """A 2562-node cell 6 cells across puts hundreds of nodes in some 4x4x4
-    bins (more than the 128 slots of a bin): those nodes take the per-node
-    path, and the step still equals the per-node spreading bit for bit."""
+def test_full_tiles_overflow_to_the_node_list():
+    """A 2562-node cell 6 cells across puts more than a thousand nodes in one
+    8x8x4 fluid tile (more than its 512 slots) and hundreds in 4x4x4 bins
+    (128 slots): the rest go to the overflow list (gathered by every tile, or
+    spread node by node), and the step still equals the per-node spreading
+    bit for bit."""
     outs = []
     for mode in (0, 1, 2):
         mesh = cases.channel_mesh(24, 16, 16, seed=13)
@@ -376,14 +380,15 @@ def test_membrane_zone_file_from_the_solver(tmp_path):
 
 
 @pytest.mark.parametrize("nx", [22, 9])
-def test_binned_spreading_across_a_ragged_periodic_axis(nx):
+def test_tile_spreading_across_a_ragged_periodic_axis(nx):
     """A cell straddling the periodic boundary of an axis whose length is not a
-    multiple of the bins (22 cells; 9 cells, where an 11-cell window of the
-    8-cell bins covers cells twice): spreading over every bin of the grid
-    reaches the wrapped stencils and equals the per-node spreading bit for
-    bit."""
+    multiple of the 8-cell fluid tiles or 4-cell bins (22 cells; 9 cells,
+    where the last tile is one cell wide, narrower than the stencil, so nodes
+    listed two tiles away reach it, and a 7-cell bin window covers cells
+    twice): the tile gathering and the bin scattering reach the wrapped
+    stencils and equal the per-node spreading bit for bit."""
     outs = []
-    for mode in (0, 2):
+    for mode in (0, 1, 2):
         mesh = cases.channel_mesh(nx, 16, 16, seed=16)
         s = abi.Solver(mesh, abi.make_params(visc=VISC, dt=DT, body_force=(2e-5, 0.0, 0.0)))
         s.set_state(*cases.smooth_state(mesh, seed=16, amp=0.005))
@@ -394,8 +399,9 @@ def test_binned_spreading_across_a_ragged_periodic_axis(nx):
         X, U, F = s.membrane_nodes()
         outs.append((np.concatenate(s.get_state()), s.cell_force(), X, F))
         s.close()
-    for a, b in zip(outs[0], outs[1]):
-        assert np.array_equal(a, b)
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
 
 
 def test_fused_fluid_pass_coupled_step_equals_two_passes():
