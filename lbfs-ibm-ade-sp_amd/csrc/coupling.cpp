@@ -234,7 +234,13 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
         B.inv_h = (float)(1.0 / hb.spacing);
         B.cell0 = hb.cell0;
         // the fluid tiles of the box come in the order of the device layout
-        B.regular = hb.n[0] % B.ftw[0] == 0 && hb.n[1] % B.ftw[1] == 0 && hb.n[2] % B.ftw[2] == 0;
+        auto log2_or = [](int v) { int s = 0; while ((1 << s) < v) s++; return (1 << s) == v ? s : -1; };
+        B.regular = 1;
+        for (int a = 0; a < 3; a++) {
+            B.fsh[a] = log2_or(B.ftw[a]);
+            B.tsh[a] = log2_or(B.tw[a]);
+            B.regular &= B.fsh[a] >= 0 && hb.n[a] % B.ftw[a] == 0;
+        }
         B.icell0 = ctx->lay.tile_cell_start[ftile0];
         ftile0 += B.fnt[0] * B.fnt[1] * B.fnt[2];
         B.tile0 = (int)tile_lo.size();

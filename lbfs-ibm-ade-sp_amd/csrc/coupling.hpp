@@ -64,7 +64,10 @@ struct IbmBlock {
     int tile0;                // first node-list tile id of the box (tiles x fastest, then y, then z)
     int cell0;                // first caller cell id of the box (cells i + j nx + k nx ny after it)
     int ftw[3], fnt[3];       // fluid tile size and tiles per axis (the device cell order)
-    int regular;              // 1: every fluid tile of the box is whole, so internal ids follow from i, j, k
+    int fsh[3];               // log2 of the fluid tile size
+    int tsh[3];               // log2 of the node-list tile size, -1 if not a power of two
+    int regular;              // 1: every fluid tile of the box is whole and a power of two along each
+                              // axis, so internal ids follow from i, j, k by shifts
     int icell0;               // internal id of the box's first cell (its first fluid tile's first cell)
     int ax_off[3];            // 4-point Lagrange kernel: offset of the axis data in IbmArgs::axis
     float origin[3], L[3], h, inv_h;

@@ -78,7 +78,7 @@ __device__ __forceinline__ void mem_coefs(const MemArgs &m, int mem, float &vol,
 {
     const double *gl = m.glob + 8 * (size_t)mem;
     const unsigned long long *acc = m.mem_acc + 4 * (size_t)mem;
-    const double V = (double)(long long)acc[0] / MEM_SUM_SCALE, A = (double)(long long)acc[1] / MEM_SUM_SCALE;
+    const double V = (double)(long long)acc[0] * (1.0 / MEM_SUM_SCALE), A = (double)(long long)acc[1] * (1.0 / MEM_SUM_SCALE);
     const MemParamDev &p = m.params[mem];
     vol = (float)(-(double)p.volume_modulus * (V / gl[3] - 1.0) / 6.0);
     area = (float)(-(double)p.global_area_modulus * (A / gl[4] - 1.0));
@@ -205,19 +205,35 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
 #pragma unroll
         for (int i = 0; i < S; i++) wt[i] *= inv;
     }
-    if (!live) return;
-    if constexpr (S == 4) {
-        *reinterpret_cast<float4 *>(row) = make_float4(wt[0], wt[1], wt[2], wt[3]);
-    } else {
-        *reinterpret_cast<float2 *>(row) = make_float2(wt[0], wt[1]);
+    if (live) {
+        if constexpr (S == 4) {
+            *reinterpret_cast<float4 *>(row) = make_float4(wt[0], wt[1], wt[2], wt[3]);
+        } else {
+            *reinterpret_cast<float2 *>(row) = make_float2(wt[0], wt[1]);
+        }
     }
-    if (r != 0) return;
     const int key = node_tile(B, base);
-    g.nbase[nd] = make_int4(base[0], base[1], base[2], key);
+    if (live && r == 0) g.nbase[nd] = make_int4(base[0], base[1], base[2], key);
     if (INTERP && g.binned != IBM_SPREAD_NODES) {
-        // listed in its tile (the bin scattering also lists the tile once);
-        // a full tile sends the node to the overflow list
-        const int q = atomicAdd(&g.tile_count[key], 1);
+        // listed in its tile, one atomic per tile among the wavefront's nodes
+        // (the bin scattering also lists the tile once); a full tile sends the
+        // node to the overflow list.  Every lane takes part in the shuffles.
+        const int lane = threadIdx.x & 63;
+        const int mykey = live && r == 0 ? key : -1;
+        int rank = 0, cnt = 0, first = lane;
+#pragma unroll
+        for (int p = 0; p < 64; p += L) {
+            const int kp = __shfl(mykey, p, 64);
+            if (kp == mykey) {
+                cnt++;
+                if (p < lane) rank++;
+                first = min(first, p);
+            }
+        }
+        int base_q = 0;
+        if (mykey >= 0 && first == lane) base_q = atomicAdd(&g.tile_count[key], cnt);
+        const int q = __shfl(base_q, first, 64) + rank;
+        if (mykey < 0) return;
         if (q == 0 && g.binned == IBM_SPREAD_BINS) {
             const int li = atomicAdd(g.n_bin_list, 1);
             if (li < g.n_tiles) g.bin_list[li] = key;
@@ -307,7 +323,7 @@ __global__ __launch_bounds__(SPREAD_NT) void k_ibm_spread_tiles(MemArgs m, IbmAr
 {
     constexpr int S = Stencil<K>::S, C = S * S * S, NPW = 64 / C, NW = SPREAD_NT / 64, NS = S * S * S;
     extern __shared__ int4 tile_lds[];  // candidates (bases, forces, ids), then 3 x max_tile_cells sums
-    __shared__ int s_tile[NS], s_pre[NS + 1], s_total, s_nov, s_nk;
+    __shared__ int s_tile[NS], s_pre[NS + 1], s_total, s_nov, s_nk, s_tc[3][S], s_ntc[3];
     const int T = xcd_remap(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int4 tl = g.tile_lo[T];
@@ -317,34 +333,48 @@ __global__ __launch_bounds__(SPREAD_NT) void k_ibm_spread_tiles(MemArgs m, IbmAr
     if (tid == 0) {
         // tiles holding the bases of nodes that can reach this tile: per axis
         // those of the cells lo - (S - 1) ... lo (wrapped, or clamped at a wall
-        // as the node tiles are)
-        int tc[3][S], ntc[3];
+        // as the node tiles are); constant axis indices keep the box in registers
+#pragma unroll
         for (int a = 0; a < 3; a++) {
-            ntc[a] = 0;
-            for (int c = lo[a] - (S - 1); c <= lo[a]; c++) {
-                int w = c;
+            int n = 0;
+#pragma unroll
+            for (int o = S - 1; o >= 0; o--) {
+                int w = lo[a] - o;
                 if (w < 0) w = B.per[a] ? ((w % B.n[a]) + B.n[a]) % B.n[a] : 0;
                 const int t = w / B.tw[a];
                 bool seen = false;
-                for (int q = 0; q < ntc[a]; q++) seen |= tc[a][q] == t;
-                if (!seen) tc[a][ntc[a]++] = t;
+                for (int q = 0; q < n; q++) seen |= s_tc[a][q] == t;
+                if (!seen) s_tc[a][n++] = t;
             }
+            s_ntc[a] = n;
         }
-        int k = 0, tot = 0;
-        for (int z = 0; z < ntc[2]; z++)
-            for (int y = 0; y < ntc[1]; y++)
-                for (int x = 0; x < ntc[0]; x++) {
-                    const int id = B.tile0 + tc[0][x] + B.nt[0] * (tc[1][y] + B.nt[1] * tc[2][z]);
-                    const int c = min(g.tile_count[id], g.tile_cap);
-                    if (c == 0) continue;
-                    s_tile[k] = id;
-                    s_pre[k] = tot;
-                    tot += c;
-                    k++;
-                }
-        s_pre[k] = tot;
-        s_total = tot;
-        s_nov = min(*g.n_overflow, m.n_nodes);
+    }
+    __syncthreads();
+    if (wave == 0) {
+        // one lane per neighbour tile: its node count, then a scan for the
+        // candidates' offsets
+        const int nx = s_ntc[0], ny = s_ntc[1], ncomb = nx * ny * s_ntc[2];
+        int c = 0, id = 0;
+        if (lane < ncomb) {
+            const int x = lane % nx, y = (lane / nx) % ny, z = lane / (nx * ny);
+            id = B.tile0 + s_tc[0][x] + B.nt[0] * (s_tc[1][y] + B.nt[1] * s_tc[2][z]);
+            c = min(g.tile_count[id], g.tile_cap);
+        }
+        int incl = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += v;
+        }
+        if (lane < ncomb) {
+            s_tile[lane] = id;
+            s_pre[lane] = incl - c;
+        }
+        if (lane == ncomb - 1) {
+            s_pre[ncomb] = incl;
+            s_total = incl;
+        }
+        if (lane == 0) s_nov = min(*g.n_overflow, m.n_nodes);
     }
     __syncthreads();
     const int tot = s_total, nov = s_nov;
@@ -427,7 +457,7 @@ __global__ __launch_bounds__(SPREAD_NT) void k_ibm_spread_tiles(MemArgs m, IbmAr
         const float sc = g.spread_sc[c0 + i];
 #pragma unroll
         for (int c = 0; c < 3; c++)
-            g.spread_f[3 * (size_t)(c0 + i) + c] = (float)((double)(long long)v[c] / FIX_SCALE) * sc;
+            g.spread_f[3 * (size_t)(c0 + i) + c] = (float)((double)(long long)v[c] * (1.0 / FIX_SCALE)) * sc;
         any |= (v[0] | v[1] | v[2]) != 0ull;
     }
     any = __syncthreads_or(any);
@@ -571,8 +601,8 @@ __global__ __launch_bounds__(256) void k_ibm_apply(IbmArgs g, ApplyArgs p)
         for (int k = tid; k < p.n_mem; k += blockDim.x) {
             unsigned long long *s = p.mem_acc + 4 * (size_t)k;
             double *gl = p.glob + 8 * (size_t)k;
-            const double V = (double)(long long)s[0] / MEM_SUM_SCALE, A = (double)(long long)s[1] / MEM_SUM_SCALE;
-            const double M = (double)(long long)s[2] / MEM_SUM_SCALE;
+            const double V = (double)(long long)s[0] * (1.0 / MEM_SUM_SCALE), A = (double)(long long)s[1] * (1.0 / MEM_SUM_SCALE);
+            const double M = (double)(long long)s[2] * (1.0 / MEM_SUM_SCALE);
             gl[0] = V;
             gl[1] = A;
             gl[2] = gl[7];
@@ -597,7 +627,7 @@ __global__ __launch_bounds__(256) void k_ibm_apply(IbmArgs g, ApplyArgs p)
                 if ((v[0] | v[1] | v[2]) != 0ull) {
                     const float sc = g.spread_sc[i];
 #pragma unroll
-                    for (int c = 0; c < 3; c++) f[c] = (float)((double)(long long)v[c] / FIX_SCALE) * sc;
+                    for (int c = 0; c < 3; c++) f[c] = (float)((double)(long long)v[c] * (1.0 / FIX_SCALE)) * sc;
                     a[0] = a[1] = a[2] = 0ull;
                 }
             }

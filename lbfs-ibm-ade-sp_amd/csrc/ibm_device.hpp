@@ -114,7 +114,10 @@ __device__ __forceinline__ bool wrap_index(const IbmBlock &B, int a, int i, int 
     const int n = B.n[a];
     if (i >= 0 && i < n) { w = i; return true; }
     if (!B.per[a]) return false;
-    w = ((i % n) + n) % n;
+    // one period away in almost every case; the remainder only beyond that
+    int v = i < 0 ? i + n : i - n;
+    if (__builtin_expect(v < 0 || v >= n, 0)) v = ((i % n) + n) % n;
+    w = v;
     return true;
 }
 
@@ -225,9 +228,10 @@ __device__ __forceinline__ IbmBlock box(const IbmArgs &g, int b) { return g.n_bl
 __device__ __forceinline__ int cell_internal(const IbmArgs &g, const IbmBlock &B, const int w[3])
 {
     if (B.regular) {
-        const int t0 = w[0] / B.ftw[0], t1 = w[1] / B.ftw[1], t2 = w[2] / B.ftw[2];
-        const int l = (w[0] - t0 * B.ftw[0]) + B.ftw[0] * ((w[1] - t1 * B.ftw[1]) + B.ftw[1] * (w[2] - t2 * B.ftw[2]));
-        return B.icell0 + (t0 + B.fnt[0] * (t1 + B.fnt[1] * t2)) * (B.ftw[0] * B.ftw[1] * B.ftw[2]) + l;
+        const int t0 = w[0] >> B.fsh[0], t1 = w[1] >> B.fsh[1], t2 = w[2] >> B.fsh[2];
+        const int l = (w[0] & (B.ftw[0] - 1)) + ((w[1] & (B.ftw[1] - 1)) << B.fsh[0]) +
+                      ((w[2] & (B.ftw[2] - 1)) << (B.fsh[0] + B.fsh[1]));
+        return B.icell0 + ((t0 + B.fnt[0] * (t1 + B.fnt[1] * t2)) << (B.fsh[0] + B.fsh[1] + B.fsh[2])) + l;
     }
     return g.inv[B.cell0 + w[0] + B.n[0] * (w[1] + B.n[1] * w[2])];
 }
@@ -240,7 +244,7 @@ __device__ __forceinline__ int tile_coord(const IbmBlock &B, int a, int base)
     int b = base;
     if (B.per[a]) b = ((b % n) + n) % n;
     else b = b < 0 ? 0 : (b >= n ? n - 1 : b);
-    return b / B.tw[a];
+    return B.tsh[a] >= 0 ? b >> B.tsh[a] : b / B.tw[a];
 }
 
 // ---------------------------------------------------------------------------
