@@ -59,7 +59,11 @@ struct AdjointState {
     bool last_in_rec = false;     // the records hold the last segment (written by the forward pass)
     std::vector<Snapshot> ckpt;   // start states of steps 0, K, 2K, ...
     std::vector<StepRec> rec;     // the steps of one segment
+    std::vector<StepRec> rec2;    // a second segment's steps: the backward pass replays the segment
+                                  // before into one set while it reverses the other
     Snapshot start, end;          // before and after the recorded run
+    hipStream_t rs[3] = {nullptr, nullptr, nullptr};  // streams of the overlapped replays
+    hipEvent_t rev[7] = {};       // replay fork and joins, records ready (2), records reversed (2)
     // fluid adjoint
     float4 *lam_st[2] = {nullptr, nullptr};
     float4 *mu = nullptr, *lcf = nullptr, *ls = nullptr, *ls_acc = nullptr, *lam_g = nullptr;
@@ -73,13 +77,22 @@ struct AdjointState {
     int *echunk = nullptr, *echunk_start = nullptr;
     int n_echunks = 0;
     unsigned int *umax = nullptr;
+    unsigned long long *acc = nullptr;  // fixed-point cell sums of the coupling adjoint
 };
 
 void adjoint_release(lbfs_ctx *ctx)
 {
     if (!ctx->adj) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (void *p : ctx->adj->allocs) (void)hipFree(p);
+    AdjointState &A = *ctx->adj;
+    for (hipStream_t &s : A.rs)
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    for (hipEvent_t &e : A.rev)
+        if (e) (void)hipEventDestroy(e);
+    for (void *p : A.allocs) (void)hipFree(p);
     delete ctx->adj;
     ctx->adj = nullptr;
 }
@@ -113,16 +126,18 @@ int snap_alloc(lbfs_ctx *ctx, Snapshot &s)
     return LBFS_OK;
 }
 
-// save: live state -> s; otherwise s -> live state (state into buffer 0)
-int snap_copy(lbfs_ctx *ctx, Snapshot &s, bool save)
+// save: live state -> s; otherwise s -> live state (state into buffer 0);
+// on stream `q` (the context's stream if null)
+int snap_copy(lbfs_ctx *ctx, Snapshot &s, bool save, hipStream_t q = nullptr)
 {
     const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    hipStream_t const stream = q ? q : ctx->stream;
     const size_t n = (size_t)ctx->lay.n_cells;
     if (save) {
-        HIPCHK(hipMemcpyAsync(s.st, ctx->st[ctx->cur], sizeof(float4) * n, k, ctx->stream));
+        HIPCHK(hipMemcpyAsync(s.st, ctx->st[ctx->cur], sizeof(float4) * n, k, stream));
         s.t = ctx->t_step;
     } else {
-        HIPCHK(hipMemcpyAsync(ctx->st[0], s.st, sizeof(float4) * n, k, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->st[0], s.st, sizeof(float4) * n, k, stream));
         ctx->cur = 0;
         ctx->t_step = s.t;
     }
@@ -131,23 +146,23 @@ int snap_copy(lbfs_ctx *ctx, Snapshot &s, bool save)
     IbmArgs &g = ctx->ibm;
     const size_t nn = (size_t)m.n_nodes, ne = (size_t)m.n_edges, nm = (size_t)m.n_mem;
     if (save) {
-        HIPCHK(hipMemcpyAsync(s.X, m.X, sizeof(float4) * nn, k, ctx->stream));
-        HIPCHK(hipMemcpyAsync(s.edge_state, m.edge_state, sizeof(float4) * ne, k, ctx->stream));
-        HIPCHK(hipMemcpyAsync(s.wnode, g.wnode, sizeof(float) * 64 * nn, k, ctx->stream));
-        HIPCHK(hipMemcpyAsync(s.nbase, g.nbase, sizeof(int4) * nn, k, ctx->stream));
-        HIPCHK(hipMemcpyAsync(s.glob, m.glob, sizeof(double) * 8 * nm, k, ctx->stream));
+        HIPCHK(hipMemcpyAsync(s.X, m.X, sizeof(float4) * nn, k, stream));
+        HIPCHK(hipMemcpyAsync(s.edge_state, m.edge_state, sizeof(float4) * ne, k, stream));
+        HIPCHK(hipMemcpyAsync(s.wnode, g.wnode, sizeof(float) * 64 * nn, k, stream));
+        HIPCHK(hipMemcpyAsync(s.nbase, g.nbase, sizeof(int4) * nn, k, stream));
+        HIPCHK(hipMemcpyAsync(s.glob, m.glob, sizeof(double) * 8 * nm, k, stream));
     } else {
-        HIPCHK(hipMemcpyAsync(m.X, s.X, sizeof(float4) * nn, k, ctx->stream));
-        HIPCHK(hipMemcpyAsync(m.edge_state, s.edge_state, sizeof(float4) * ne, k, ctx->stream));
-        HIPCHK(hipMemcpyAsync(g.wnode, s.wnode, sizeof(float) * 64 * nn, k, ctx->stream));
-        HIPCHK(hipMemcpyAsync(g.nbase, s.nbase, sizeof(int4) * nn, k, ctx->stream));
-        HIPCHK(hipMemcpyAsync(m.glob, s.glob, sizeof(double) * 8 * nm, k, ctx->stream));
+        HIPCHK(hipMemcpyAsync(m.X, s.X, sizeof(float4) * nn, k, stream));
+        HIPCHK(hipMemcpyAsync(m.edge_state, s.edge_state, sizeof(float4) * ne, k, stream));
+        HIPCHK(hipMemcpyAsync(g.wnode, s.wnode, sizeof(float) * 64 * nn, k, stream));
+        HIPCHK(hipMemcpyAsync(g.nbase, s.nbase, sizeof(int4) * nn, k, stream));
+        HIPCHK(hipMemcpyAsync(m.glob, s.glob, sizeof(double) * 8 * nm, k, stream));
     }
     return LBFS_OK;
 }
 
 int save(lbfs_ctx *ctx, Snapshot &s) { return snap_copy(ctx, s, true); }
-int restore(lbfs_ctx *ctx, Snapshot &s) { return snap_copy(ctx, s, false); }
+int restore(lbfs_ctx *ctx, Snapshot &s, hipStream_t q = nullptr) { return snap_copy(ctx, s, false, q); }
 
 // the conditions under which the adjoint is the derivative of the step
 int adjoint_supported(lbfs_ctx *ctx)
@@ -163,6 +178,30 @@ int adjoint_supported(lbfs_ctx *ctx)
             return ctx->fail(LBFS_E_UNSUPPORTED, "the adjoint supports the 4-point cosine IBM kernel");
         if (ctx->visc_factor)
             return ctx->fail(LBFS_E_UNSUPPORTED, "the adjoint needs internal viscosity ratios of 1");
+    }
+    return LBFS_OK;
+}
+
+// the step records of one segment
+int alloc_records(lbfs_ctx *ctx, std::vector<StepRec> &recs, int every)
+{
+    const int n = ctx->lay.n_cells;
+    int rc;
+    recs.resize(every);
+    for (int k = 0; k < every; k++) {
+        StepRec &r = recs[k];
+        if (k > 0 && (rc = adj_alloc(ctx, &r.st, (size_t)n))) return rc;
+        if ((rc = adj_alloc(ctx, &r.grad, 3 * (size_t)n))) return rc;
+        if ((rc = adj_alloc(ctx, &r.res, (size_t)n))) return rc;
+        if ((rc = adj_alloc(ctx, &r.mom, ctx->lay.fA.size() / 4))) return rc;
+        if (ctx->have_membranes && (rc = adj_alloc(ctx, &r.cf, 3 * (size_t)n))) return rc;
+        if (ctx->have_membranes) {
+            const size_t nn = ctx->mem.n_nodes, nm = ctx->mem.n_mem;
+            if ((rc = adj_alloc(ctx, &r.Xn1, nn))) return rc;
+            if ((rc = adj_alloc(ctx, &r.U, nn))) return rc;
+            if ((rc = adj_alloc(ctx, &r.F, nn))) return rc;
+            if ((rc = adj_alloc(ctx, &r.glob, 8 * nm))) return rc;
+        }
     }
     return LBFS_OK;
 }
@@ -186,21 +225,13 @@ int adjoint_setup(lbfs_ctx *ctx, int n_steps, int every)
     A.ckpt.resize(n_ckpt);
     for (auto &s : A.ckpt)
         if ((rc = snap_alloc(ctx, s))) return rc;
-    A.rec.resize(every);
-    for (int k = 0; k < every; k++) {
-        StepRec &r = A.rec[k];
-        if (k > 0 && (rc = adj_alloc(ctx, &r.st, (size_t)n))) return rc;
-        if ((rc = adj_alloc(ctx, &r.grad, 3 * (size_t)n))) return rc;
-        if ((rc = adj_alloc(ctx, &r.res, (size_t)n))) return rc;
-        if ((rc = adj_alloc(ctx, &r.mom, ctx->lay.fA.size() / 4))) return rc;
-        if (ctx->have_membranes && (rc = adj_alloc(ctx, &r.cf, 3 * (size_t)n))) return rc;
-        if (ctx->have_membranes) {
-            const size_t nn = ctx->mem.n_nodes, nm = ctx->mem.n_mem;
-            if ((rc = adj_alloc(ctx, &r.Xn1, nn))) return rc;
-            if ((rc = adj_alloc(ctx, &r.U, nn))) return rc;
-            if ((rc = adj_alloc(ctx, &r.F, nn))) return rc;
-            if ((rc = adj_alloc(ctx, &r.glob, 8 * nm))) return rc;
-        }
+    if ((rc = alloc_records(ctx, A.rec, every))) return rc;
+    if (n_ckpt > 1) {
+        // the overlapped replays: a second record set, their own streams
+        // (default priority) and events
+        if ((rc = alloc_records(ctx, A.rec2, every))) return rc;
+        for (hipStream_t &s : A.rs) HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        for (hipEvent_t &e : A.rev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     if ((rc = snap_alloc(ctx, A.end))) return rc;
     for (auto &b : A.lam_st)
@@ -256,6 +287,7 @@ int adjoint_setup(lbfs_ctx *ctx, int n_steps, int every)
         if ((rc = adj_alloc(ctx, &A.mglob, 8 * nm))) return rc;
         if ((rc = adj_alloc(ctx, &A.gmem, 4 * nm))) return rc;
         if ((rc = adj_alloc(ctx, &A.umax, 1))) return rc;
+        if ((rc = adj_alloc(ctx, &A.acc, 3 * (size_t)n))) return rc;
         // chunks of at most 256 edges per membrane for the two-level sums
         std::vector<int> ch, cs(1, 0);
         for (int i = 0; i < (int)nm; i++) {
@@ -284,15 +316,24 @@ int adjoint_setup(lbfs_ctx *ctx, int n_steps, int every)
 // `st` (the record's start state), keeping what the reverse step reads; the
 // new state goes to `st_next`.  The kernels and their order are those of
 // lbfs_step.
-int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next)
+// streams and events a replay runs on: the context's own, or those of the
+// overlapped replays
+struct Lanes {
+    hipStream_t main, side, side2;
+    hipEvent_t fork, join, join2;
+};
+Lanes ctx_lanes(lbfs_ctx *ctx) { return {ctx->stream, ctx->side, ctx->side2, ctx->ev_fork, ctx->ev_join, ctx->ev_join2}; }
+Lanes replay_lanes(AdjointState &A) { return {A.rs[0], A.rs[1], A.rs[2], A.rev[0], A.rev[1], A.rev[2]}; }
+
+int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next, const Lanes &q)
 {
     const hipMemcpyKind k = hipMemcpyDeviceToDevice;
     const size_t n = (size_t)ctx->lay.n_cells;
     FluidArgs a = ctx->args;
     a.face_moments = r.mom;
     if (!ctx->have_membranes) {
-        HIPCHK(launch_gradients(a, st, r.grad, ctx->stream));
-        HIPCHK(launch_flux_update(a, st, r.grad, st_next, r.res, 1, ctx->stream));
+        HIPCHK(launch_gradients(a, st, r.grad, q.main));
+        HIPCHK(launch_flux_update(a, st, r.grad, st_next, r.res, 1, q.main));
         return LBFS_OK;
     }
     // as in the time step: after the interpolation, the gradients and the
@@ -301,30 +342,30 @@ int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next)
     // in r.Xn1; U^n, F^n and the step's globals land in r.U, r.F, r.glob)
     MemArgs m = ctx->mem;
     const size_t nn = m.n_nodes, nm = m.n_mem;
-    HIPCHK(hipMemcpyAsync(r.Xn1, r.X, sizeof(float4) * nn, k, ctx->stream));
-    HIPCHK(hipMemcpyAsync(r.glob, r.glob_prev, sizeof(double) * 8 * nm, k, ctx->stream));
+    HIPCHK(hipMemcpyAsync(r.Xn1, r.X, sizeof(float4) * nn, k, q.main));
+    HIPCHK(hipMemcpyAsync(r.glob, r.glob_prev, sizeof(double) * 8 * nm, k, q.main));
     m.X = r.Xn1;
     m.U = r.U;
     m.F = r.F;
     m.glob = r.glob;
-    HIPCHK(launch_ibm_interp(m, ctx->ibm, st, ctx->stream));
-    HIPCHK(hipEventRecord(ctx->ev_fork, ctx->stream));
-    HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-    HIPCHK(launch_gradients(a, st, r.grad, ctx->side));
-    HIPCHK(launch_flux_update(a, st, r.grad, st_next, r.res, 1, ctx->side));
-    HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
-    HIPCHK(hipStreamWaitEvent(ctx->side2, ctx->ev_fork, 0));
-    HIPCHK(launch_mem_volume(m, ctx->side2));
-    HIPCHK(launch_mem_edges(m, ctx->side2));
-    HIPCHK(launch_mem_nodes(m, ctx->side2));
-    HIPCHK(hipEventRecord(ctx->ev_join2, ctx->side2));
-    HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join2, 0));
-    HIPCHK(launch_ibm_spread(m, ctx->ibm, ctx->stream));
-    HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+    HIPCHK(launch_ibm_interp(m, ctx->ibm, st, q.main));
+    HIPCHK(hipEventRecord(q.fork, q.main));
+    HIPCHK(hipStreamWaitEvent(q.side, q.fork, 0));
+    HIPCHK(launch_gradients(a, st, r.grad, q.side));
+    HIPCHK(launch_flux_update(a, st, r.grad, st_next, r.res, 1, q.side));
+    HIPCHK(hipEventRecord(q.join, q.side));
+    HIPCHK(hipStreamWaitEvent(q.side2, q.fork, 0));
+    HIPCHK(launch_mem_volume(m, q.side2));
+    HIPCHK(launch_mem_edges(m, q.side2));
+    HIPCHK(launch_mem_nodes(m, q.side2));
+    HIPCHK(hipEventRecord(q.join2, q.side2));
+    HIPCHK(hipStreamWaitEvent(q.main, q.join2, 0));
+    HIPCHK(launch_ibm_spread(m, ctx->ibm, q.main));
+    HIPCHK(hipStreamWaitEvent(q.main, q.join, 0));
     // the step's cell force goes straight into the record (every cell)
     lbfs::ApplyArgs p = ctx->apply_args(st_next, r.cf, 1);
     p.glob = m.glob;
-    HIPCHK(launch_ibm_apply(ctx->ibm, p, ctx->stream));
+    HIPCHK(launch_ibm_apply(ctx->ibm, p, q.main));
     return LBFS_OK;
 }
 
@@ -340,8 +381,8 @@ int reverse_step(lbfs_ctx *ctx, const StepRec &r, bool first, int &cur_l)
     j.mu = A.mu; j.lcf = A.lcf; j.ls = A.ls; j.ls_acc = A.ls_acc; j.lam_g = A.lam_g;
     j.fpart = A.fpart; j.gforce = A.gforce; j.n_fpart = A.n_fpart;
     j.rev_w = A.rev_w;
-    j.acc = ctx->have_membranes ? ctx->ibm.acc : nullptr;
-    j.acc_rw = ctx->have_membranes ? ctx->ibm.acc : nullptr;
+    j.acc = ctx->have_membranes ? A.acc : nullptr;
+    j.acc_rw = ctx->have_membranes ? A.acc : nullptr;
     j.umax = A.umax;
     HIPCHK(launch_adj_update(a, j, r.st, r.res, ctx->stream));
     if (!ctx->have_membranes) {
@@ -362,7 +403,7 @@ int reverse_step(lbfs_ctx *ctx, const StepRec &r, bool first, int &cur_l)
     jm.Xn = r.X; jm.glob_prev = r.glob_prev;
     jm.lam_X = A.lam_X; jm.lam_U = A.lam_U; jm.lam_F = A.lam_F;
     jm.erec = A.erec; jm.grec = A.grec; jm.mrec = A.mrec; jm.mglob = A.mglob; jm.gmem = A.gmem;
-    jm.umax = A.umax; jm.lcf = A.lcf; jm.st_n = r.st; jm.acc = ctx->ibm.acc;
+    jm.umax = A.umax; jm.lcf = A.lcf; jm.st_n = r.st; jm.acc = A.acc;
     jm.mad_lag = first ? 0 : 1;
     jm.echunk = A.echunk; jm.echunk_start = A.echunk_start; jm.epart = A.epart; jm.n_echunks = A.n_echunks;
     HIPCHK(hipMemsetAsync(A.umax, 0, sizeof(unsigned int), ctx->stream));
@@ -375,22 +416,51 @@ int reverse_step(lbfs_ctx *ctx, const StepRec &r, bool first, int &cur_l)
 
 // Replays segment sg from its checkpoint into the step records; the state
 // after the segment is left in ctx->st[1].
-int record_segment(lbfs_ctx *ctx, int sg)
+// a segment replay enqueued step by step (so that it can be interleaved with
+// the reverse of another segment on other streams)
+struct SegReplay {
+    int sg = -1, s0 = 0, s1 = 0, t = 0;
+    std::vector<StepRec> *recs = nullptr;
+    Lanes q{};
+};
+
+int replay_begin(lbfs_ctx *ctx, int sg, std::vector<StepRec> &recs, const Lanes &q, SegReplay &p)
 {
     AdjointState &A = *ctx->adj;
-    const int K = A.every, N = A.n_steps;
-    const int s0 = sg * K, s1 = std::min(s0 + K, N);
+    p.sg = sg;
+    p.s0 = sg * A.every;
+    p.s1 = std::min(p.s0 + A.every, A.n_steps);
+    p.t = p.s0;
+    p.recs = &recs;
+    p.q = q;
+    return restore(ctx, A.ckpt[sg], q.main);
+}
+
+bool replay_pending(const SegReplay &p) { return p.recs && p.t < p.s1; }
+
+int replay_next(lbfs_ctx *ctx, SegReplay &p)
+{
+    AdjointState &A = *ctx->adj;
+    std::vector<StepRec> &recs = *p.recs;
+    const int t = p.t, s0 = p.s0, s1 = p.s1;
+    StepRec &r = recs[t - s0];
+    const StepRec *pr = t > s0 ? &recs[t - 1 - s0] : nullptr;
+    if (!pr) r.st = A.ckpt[p.sg].st;
+    r.X = pr ? pr->Xn1 : A.ckpt[p.sg].X;
+    r.glob_prev = pr ? pr->glob : A.ckpt[p.sg].glob;
+    float4 *next = t + 1 < s1 ? recs[t + 1 - s0].st : ctx->st[1];
+    p.t++;
+    return replay_step(ctx, r, t == s0 ? ctx->st[0] : r.st, next, p.q);
+}
+
+// replays segment sg from its checkpoint into the step records `recs`
+int record_segment(lbfs_ctx *ctx, int sg, std::vector<StepRec> &recs, const Lanes &q)
+{
+    SegReplay p;
     int rc;
-    if ((rc = restore(ctx, A.ckpt[sg]))) return rc;
-    for (int t = s0; t < s1; t++) {
-        StepRec &r = A.rec[t - s0];
-        const StepRec *p = t > s0 ? &A.rec[t - 1 - s0] : nullptr;
-        if (!p) r.st = A.ckpt[sg].st;
-        r.X = p ? p->Xn1 : A.ckpt[sg].X;
-        r.glob_prev = p ? p->glob : A.ckpt[sg].glob;
-        float4 *next = t + 1 < s1 ? A.rec[t + 1 - s0].st : ctx->st[1];
-        if ((rc = replay_step(ctx, r, t == s0 ? ctx->st[0] : r.st, next))) return rc;
-    }
+    if ((rc = replay_begin(ctx, sg, recs, q, p))) return rc;
+    while (replay_pending(p))
+        if ((rc = replay_next(ctx, p))) return rc;
     return LBFS_OK;
 }
 
@@ -417,7 +487,7 @@ extern "C" int lbfs_adjoint_forward(lbfs_ctx *ctx, int n_steps, int checkpoint_e
     // the last segment straight into the step records (the same kernels on
     // the same inputs), so the backward pass starts without replaying it;
     // then the solver takes the end state from the records
-    if ((rc = record_segment(ctx, n_seg - 1))) return rc;
+    if ((rc = record_segment(ctx, n_seg - 1, A.rec, ctx_lanes(ctx)))) return rc;
     {
         const StepRec &r = A.rec[n_steps - 1 - s_last];
         const hipMemcpyKind k = hipMemcpyDeviceToDevice;
@@ -495,16 +565,45 @@ extern "C" int lbfs_adjoint_backward(lbfs_ctx *ctx, const double *lam_nodes, con
     int cur_l = 0;
     const int K = A.every, N = A.n_steps;
     const int n_seg = (N + K - 1) / K;
-    for (int sg = n_seg - 1; sg >= 0; sg--) {
-        const int s0 = sg * K, s1 = std::min(s0 + K, N);
-        // replay the segment into its step records (the forward pass left
-        // the last one there), then reverse it
-        if (!(sg == n_seg - 1 && A.last_in_rec)) {
-            if ((rc = record_segment(ctx, sg))) return rc;
-            A.last_in_rec = false;
+    // Segment sg's records: set (n_seg - 1 - sg) % 2 (the forward pass left
+    // the last segment in set 0).  With several segments the replay of the
+    // segment before runs on its own streams while the reverse of this one
+    // runs on the context's: it waits until the reverse of the segment after
+    // has released its set, and the reverse of a segment waits for its replay.
+    std::vector<StepRec> *sets[2] = {&A.rec, &A.rec2};
+    hipEvent_t *ready = A.rev + 3, *reversed = A.rev + 5;
+    if (!A.last_in_rec) {
+        if (n_seg > 1) {
+            HIPCHK(hipEventRecord(ready[1], ctx->stream));  // the seeds above come first
+            HIPCHK(hipStreamWaitEvent(A.rs[0], ready[1], 0));
+            if ((rc = record_segment(ctx, n_seg - 1, A.rec, replay_lanes(A)))) return rc;
+            HIPCHK(hipEventRecord(ready[0], A.rs[0]));
+        } else if ((rc = record_segment(ctx, n_seg - 1, A.rec, ctx_lanes(ctx)))) {
+            return rc;
         }
-        for (int t = s1 - 1; t >= s0; t--)
-            if ((rc = reverse_step(ctx, A.rec[t - s0], t == 0, cur_l))) return rc;
+    } else if (n_seg > 1) {
+        HIPCHK(hipEventRecord(ready[0], ctx->stream));
+    }
+    A.last_in_rec = false;
+    for (int sg = n_seg - 1; sg >= 0; sg--) {
+        const int s0 = sg * K, s1 = std::min(s0 + K, N), b = (n_seg - 1 - sg) & 1;
+        SegReplay p;
+        if (sg > 0) {
+            // replay the segment before into the other set
+            if (sg + 1 < n_seg) HIPCHK(hipStreamWaitEvent(A.rs[0], reversed[b ^ 1], 0));
+            else HIPCHK(hipStreamWaitEvent(A.rs[0], ready[b], 0));  // the live state is free
+            if ((rc = replay_begin(ctx, sg - 1, *sets[b ^ 1], replay_lanes(A), p))) return rc;
+        }
+        if (n_seg > 1) HIPCHK(hipStreamWaitEvent(ctx->stream, ready[b], 0));
+        // the two streams' steps enqueued in turn, so both have work at once
+        for (int t = s1 - 1; t >= s0; t--) {
+            if (replay_pending(p) && (rc = replay_next(ctx, p))) return rc;
+            if ((rc = reverse_step(ctx, (*sets[b])[t - s0], t == 0, cur_l))) return rc;
+        }
+        while (replay_pending(p))
+            if ((rc = replay_next(ctx, p))) return rc;
+        if (sg > 0) HIPCHK(hipEventRecord(ready[b ^ 1], A.rs[0]));
+        if (n_seg > 1) HIPCHK(hipEventRecord(reversed[b], ctx->stream));
     }
     if (n_seg == 1) A.last_in_rec = true;  // the records still hold the only segment
     {
