@@ -62,7 +62,9 @@ struct AdjointState {
     std::vector<StepRec> rec2;    // a second segment's steps: the backward pass replays the segment
                                   // before into one set while it reverses the other
     Snapshot start, end;          // before and after the recorded run
-    hipStream_t rs[3] = {nullptr, nullptr, nullptr};  // streams of the overlapped replays
+    hipStream_t rs = nullptr;     // stream of the overlapped replays: one, so that the process's
+                                  // streams stay within its hardware queues (four by default) and
+                                  // no replay kernel queues behind a reverse kernel
     hipEvent_t rev[7] = {};       // replay fork and joins, records ready (2), records reversed (2)
     // fluid adjoint
     float4 *lam_st[2] = {nullptr, nullptr};
@@ -73,7 +75,7 @@ struct AdjointState {
     // coupling adjoint
     float4 *lam_X = nullptr, *lam_U = nullptr, *lam_F = nullptr, *grec = nullptr;
     float *erec = nullptr, *mrec = nullptr;
-    double *mglob = nullptr, *gmem = nullptr, *epart = nullptr;
+    double *gmem = nullptr, *epart = nullptr;
     int *echunk = nullptr, *echunk_start = nullptr;
     int n_echunks = 0;
     unsigned int *umax = nullptr;
@@ -85,11 +87,10 @@ void adjoint_release(lbfs_ctx *ctx)
     if (!ctx->adj) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     AdjointState &A = *ctx->adj;
-    for (hipStream_t &s : A.rs)
-        if (s) {
-            (void)hipStreamSynchronize(s);
-            (void)hipStreamDestroy(s);
-        }
+    if (A.rs) {
+        (void)hipStreamSynchronize(A.rs);
+        (void)hipStreamDestroy(A.rs);
+    }
     for (hipEvent_t &e : A.rev)
         if (e) (void)hipEventDestroy(e);
     for (void *p : A.allocs) (void)hipFree(p);
@@ -227,10 +228,10 @@ int adjoint_setup(lbfs_ctx *ctx, int n_steps, int every)
         if ((rc = snap_alloc(ctx, s))) return rc;
     if ((rc = alloc_records(ctx, A.rec, every))) return rc;
     if (n_ckpt > 1) {
-        // the overlapped replays: a second record set, their own streams
+        // the overlapped replays: a second record set, its own stream
         // (default priority) and events
         if ((rc = alloc_records(ctx, A.rec2, every))) return rc;
-        for (hipStream_t &s : A.rs) HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&A.rs, hipStreamNonBlocking));
         for (hipEvent_t &e : A.rev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     if ((rc = snap_alloc(ctx, A.end))) return rc;
@@ -284,7 +285,6 @@ int adjoint_setup(lbfs_ctx *ctx, int n_steps, int every)
         if ((rc = adj_alloc(ctx, &A.grec, 4 * ne))) return rc;
         if ((rc = adj_alloc(ctx, &A.erec, 24 * ne))) return rc;
         if ((rc = adj_alloc(ctx, &A.mrec, 12 * ne))) return rc;
-        if ((rc = adj_alloc(ctx, &A.mglob, 8 * nm))) return rc;
         if ((rc = adj_alloc(ctx, &A.gmem, 4 * nm))) return rc;
         if ((rc = adj_alloc(ctx, &A.umax, 1))) return rc;
         if ((rc = adj_alloc(ctx, &A.acc, 3 * (size_t)n))) return rc;
@@ -323,17 +323,29 @@ struct Lanes {
     hipEvent_t fork, join, join2;
 };
 Lanes ctx_lanes(lbfs_ctx *ctx) { return {ctx->stream, ctx->side, ctx->side2, ctx->ev_fork, ctx->ev_join, ctx->ev_join2}; }
-Lanes replay_lanes(AdjointState &A) { return {A.rs[0], A.rs[1], A.rs[2], A.rev[0], A.rev[1], A.rev[2]}; }
+Lanes replay_lanes(AdjointState &A) { return {A.rs, A.rs, A.rs, A.rev[0], A.rev[1], A.rev[2]}; }
+
+// the fluid pass of a replayed step, keeping its gradients and face moments:
+// the fused pass where the step uses it (it writes the gradients as well),
+// else the gradient pass and the flux pass
+hipError_t replay_fluid(const lbfs_ctx *ctx, FluidArgs a, StepRec &r, const float4 *st, float4 *st_next,
+                        hipStream_t s)
+{
+    a.face_moments = r.mom;
+    if (ctx->fused && ctx->fused_ok && a.ts_mode != LBFS_TS_MIN) {
+        a.grad_out = r.grad;
+        return launch_fluid_fused(a, st, st_next, r.res, 1, s);
+    }
+    const hipError_t e = launch_gradients(a, st, r.grad, s);
+    return e != hipSuccess ? e : launch_flux_update(a, st, r.grad, st_next, r.res, 1, s);
+}
 
 int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next, const Lanes &q)
 {
     const hipMemcpyKind k = hipMemcpyDeviceToDevice;
     const size_t n = (size_t)ctx->lay.n_cells;
-    FluidArgs a = ctx->args;
-    a.face_moments = r.mom;
     if (!ctx->have_membranes) {
-        HIPCHK(launch_gradients(a, st, r.grad, q.main));
-        HIPCHK(launch_flux_update(a, st, r.grad, st_next, r.res, 1, q.main));
+        HIPCHK(replay_fluid(ctx, ctx->args, r, st, st_next, q.main));
         return LBFS_OK;
     }
     // as in the time step: after the interpolation, the gradients and the
@@ -351,8 +363,7 @@ int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next, co
     HIPCHK(launch_ibm_interp(m, ctx->ibm, st, q.main));
     HIPCHK(hipEventRecord(q.fork, q.main));
     HIPCHK(hipStreamWaitEvent(q.side, q.fork, 0));
-    HIPCHK(launch_gradients(a, st, r.grad, q.side));
-    HIPCHK(launch_flux_update(a, st, r.grad, st_next, r.res, 1, q.side));
+    HIPCHK(replay_fluid(ctx, ctx->args, r, st, st_next, q.side));
     HIPCHK(hipEventRecord(q.join, q.side));
     HIPCHK(hipStreamWaitEvent(q.side2, q.fork, 0));
     HIPCHK(launch_mem_volume(m, q.side2));
@@ -402,11 +413,10 @@ int reverse_step(lbfs_ctx *ctx, const StepRec &r, bool first, int &cur_l)
     AdjMemArgs jm{};
     jm.Xn = r.X; jm.glob_prev = r.glob_prev;
     jm.lam_X = A.lam_X; jm.lam_U = A.lam_U; jm.lam_F = A.lam_F;
-    jm.erec = A.erec; jm.grec = A.grec; jm.mrec = A.mrec; jm.mglob = A.mglob; jm.gmem = A.gmem;
+    jm.erec = A.erec; jm.grec = A.grec; jm.mrec = A.mrec; jm.gmem = A.gmem;
     jm.umax = A.umax; jm.lcf = A.lcf; jm.st_n = r.st; jm.acc = A.acc;
     jm.mad_lag = first ? 0 : 1;
     jm.echunk = A.echunk; jm.echunk_start = A.echunk_start; jm.epart = A.epart; jm.n_echunks = A.n_echunks;
-    HIPCHK(hipMemsetAsync(A.umax, 0, sizeof(unsigned int), ctx->stream));
     HIPCHK(launch_adj_coupling(m, ctx->ibm, jm, ctx->stream));
     HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
     HIPCHK(launch_adj_grad(a, j, ctx->stream));
@@ -567,7 +577,7 @@ extern "C" int lbfs_adjoint_backward(lbfs_ctx *ctx, const double *lam_nodes, con
     const int n_seg = (N + K - 1) / K;
     // Segment sg's records: set (n_seg - 1 - sg) % 2 (the forward pass left
     // the last segment in set 0).  With several segments the replay of the
-    // segment before runs on its own streams while the reverse of this one
+    // segment before runs on its own stream while the reverse of this one
     // runs on the context's: it waits until the reverse of the segment after
     // has released its set, and the reverse of a segment waits for its replay.
     std::vector<StepRec> *sets[2] = {&A.rec, &A.rec2};
@@ -575,9 +585,9 @@ extern "C" int lbfs_adjoint_backward(lbfs_ctx *ctx, const double *lam_nodes, con
     if (!A.last_in_rec) {
         if (n_seg > 1) {
             HIPCHK(hipEventRecord(ready[1], ctx->stream));  // the seeds above come first
-            HIPCHK(hipStreamWaitEvent(A.rs[0], ready[1], 0));
+            HIPCHK(hipStreamWaitEvent(A.rs, ready[1], 0));
             if ((rc = record_segment(ctx, n_seg - 1, A.rec, replay_lanes(A)))) return rc;
-            HIPCHK(hipEventRecord(ready[0], A.rs[0]));
+            HIPCHK(hipEventRecord(ready[0], A.rs));
         } else if ((rc = record_segment(ctx, n_seg - 1, A.rec, ctx_lanes(ctx)))) {
             return rc;
         }
@@ -590,8 +600,8 @@ extern "C" int lbfs_adjoint_backward(lbfs_ctx *ctx, const double *lam_nodes, con
         SegReplay p;
         if (sg > 0) {
             // replay the segment before into the other set
-            if (sg + 1 < n_seg) HIPCHK(hipStreamWaitEvent(A.rs[0], reversed[b ^ 1], 0));
-            else HIPCHK(hipStreamWaitEvent(A.rs[0], ready[b], 0));  // the live state is free
+            if (sg + 1 < n_seg) HIPCHK(hipStreamWaitEvent(A.rs, reversed[b ^ 1], 0));
+            else HIPCHK(hipStreamWaitEvent(A.rs, ready[b], 0));  // the live state is free
             if ((rc = replay_begin(ctx, sg - 1, *sets[b ^ 1], replay_lanes(A), p))) return rc;
         }
         if (n_seg > 1) HIPCHK(hipStreamWaitEvent(ctx->stream, ready[b], 0));
@@ -602,7 +612,7 @@ extern "C" int lbfs_adjoint_backward(lbfs_ctx *ctx, const double *lam_nodes, con
         }
         while (replay_pending(p))
             if ((rc = replay_next(ctx, p))) return rc;
-        if (sg > 0) HIPCHK(hipEventRecord(ready[b ^ 1], A.rs[0]));
+        if (sg > 0) HIPCHK(hipEventRecord(ready[b ^ 1], A.rs));
         if (n_seg > 1) HIPCHK(hipEventRecord(reversed[b], ctx->stream));
     }
     if (n_seg == 1) A.last_in_rec = true;  // the records still hold the only segment

@@ -44,7 +44,6 @@ struct AdjMemArgs {
     float *erec;               // [24 edges] adjoint of each edge's differentiated inputs
     float4 *grec;              // [4 edges] volume/area constraint share per edge slot (X^n+1)
     float *mrec;               // [12 edges] MAD share per edge coordinate (X^n)
-    double *mglob;             // [8 mem] lambda_V, lambda_A, lambda_MAD, MAD coefficients
     double *gmem;              // [4 mem] running sums: lambda_ks ks, local bending, global bending
     unsigned int *umax;
     const float4 *lcf;         // [n cells] adjoint of the cell force

@@ -8,12 +8,13 @@
 //   k_adj_mem_edges   per edge: the edge force (spring_network.cu:613-971) in
 //                     forward-mode, one input direction at a time, contracted
 //                     with the adjoint of the four node forces
-//   k_adj_mem_partials, k_adj_mem_reduce
-//                     per chunk of edges, then per membrane (fixed order):
-//                     adjoints of the global volume/area/MAD coefficients and
-//                     the modulus gradients
-//   k_adj_mem_geo     per edge: volume and area constraint terms at X^n+1
-//   k_adj_mem_mad     per edge: the MAD term, a function of X^n (one step lag)
+//   k_adj_mem_partials per chunk of edges: partial sums of the coefficient
+//                     and modulus adjoints
+//   k_adj_mem_coef    per membrane (fixed order, in every workgroup that needs
+//                     it): adjoints of the global volume/area/MAD coefficients
+//                     and the modulus gradients; then per edge the volume and
+//                     area constraint terms at X^n+1 and the MAD term, a
+//                     function of X^n (one step lag)
 //   k_adj_mem_nodes   per node: fixed-order sum of its edge slots; adds the
 //                     advection X^n+1 = X^n + dt U^n in reverse
 //   k_adj_interp      per node at X^n: adjoint of X^n through the weights of
@@ -74,6 +75,9 @@ __global__ __launch_bounds__(256) void k_adj_spread(MemArgs m, IbmArgs g, AdjMem
 {
     const int lane = threadIdx.x & 63;
     const int nd = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // the node-velocity maximum of this reverse step starts from zero (the
+    // previous step's readers ran before this kernel on the same stream)
+    if (blockIdx.x == 0 && threadIdx.x == 0) *j.umax = 0u;
     if (nd >= m.n_nodes) return;
     const IbmBlock B = node_block(g, nd);
     const float4 o = m.origin[m.node_mem[nd]], xl = m.X[nd];
@@ -211,66 +215,78 @@ __global__ __launch_bounds__(256) void k_adj_mem_partials(MemArgs m, AdjMemArgs 
     if (tid < 6) j.epart[6 * (size_t)blockIdx.x + tid] = sh[tid][0];
 }
 
-// second level: one wavefront per membrane over its chunks, then the chain
-// rule through the coefficient formulas
-__global__ __launch_bounds__(64) void k_adj_mem_reduce(MemArgs m, AdjMemArgs j)
-{
-    const int mem = blockIdx.x, lane = threadIdx.x;
-    double s[6] = {0, 0, 0, 0, 0, 0};
-    for (int c = j.echunk_start[mem] + lane; c < j.echunk_start[mem + 1]; c += 64)
-        for (int k = 0; k < 6; k++) s[k] += j.epart[6 * (size_t)c + k];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-        for (int k = 0; k < 6; k++) s[k] += __shfl_xor(s[k], off, 64);
-    if (lane == 0) {
-        const MemParamDev p = m.params[mem];
-        const double *gl = m.glob + 8 * (size_t)mem;
-        const double *gp = j.glob_prev + 8 * (size_t)mem;
-        double *mg = j.mglob + 8 * (size_t)mem;
-        const double lmad = s[2] * (-2.0 * (double)p.global_bending / gl[4]);
-        mg[0] = s[0] * (-(double)p.volume_modulus / (6.0 * gl[3]));
-        mg[1] = s[1] * (-(double)p.global_area_modulus / gl[4]);
-        mg[2] = lmad;
-        // MAD = M / A(X^n), M = sum over the MAD edges of kappa (|N1| + |N2|) / 3
-        mg[3] = lmad / gp[6];
-        mg[4] = lmad * gl[2] / gp[6];
-        double *gm = j.gmem + 4 * (size_t)mem;
-        gm[0] += s[3];
-        gm[1] += s[4];
-        gm[2] += s[5] + s[2] * (-2.0 * (gl[2] - gl[5]) / gl[4]);
-    }
-}
-
-// lambda_V dV/dX + lambda_A dA/dX at X^n+1, per edge slot
-__global__ __launch_bounds__(128) void k_adj_mem_geo(MemArgs m, AdjMemArgs j)
-{
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= m.n_edges) return;
-    const double *mg = j.mglob + 8 * (size_t)m.edge_mem[e];
-    const float lv = (float)mg[0], la = (float)mg[1];
-    const int4 q = m.edges[e];
-    const float4 xa = m.X[q.x], xb = m.X[q.y], xc = m.X[q.z], xd = m.X[q.w];
-    memf::EdgeGeometry<float> g;
-    memf::edge_geometry<float>({xa.x, xa.y, xa.z}, {xb.x, xb.y, xb.z}, {xc.x, xc.y, xc.z}, {xd.x, xd.y, xd.z}, g);
-#pragma unroll
-    for (int s = 0; s < 4; s++)
-        j.grec[4 * (size_t)e + s] = make_float4(lv * g.d_vol[3 * s] + la * g.d_area[3 * s],
-                                                lv * g.d_vol[3 * s + 1] + la * g.d_area[3 * s + 1],
-                                                lv * g.d_vol[3 * s + 2] + la * g.d_area[3 * s + 2], 0.0f);
-}
-
+// Second level, fused with the per-edge terms that use its result.  Every
+// workgroup first sums, one wavefront per membrane of its edges, the chunk
+// partials of those membranes in a fixed order (so all workgroups get the same
+// bits) and chains them through the coefficient formulas; the workgroup
+// holding a membrane's first edge also adds the step's modulus gradients.
+// Then, per edge, lane 12 of its ADJ_COEF_LANES lanes writes
+// lambda_V dV/dX + lambda_A dA/dX at X^n+1 per edge slot, and lanes 0-11 the
 // gradient at X^n of c1 [MAD edge] kappa (|N1| + |N2|) / 3 - c2 (|N1| + |N2|) / 6,
-// c1 = lambda_MAD / A(X^n), c2 = lambda_MAD MAD / A(X^n); the sum of the second
-// term over the edges is c2 A.  The first step's MAD comes from the set-up
-// (a constant of the run, like the reference areas and volume)
-__global__ __launch_bounds__(256) void k_adj_mem_mad(MemArgs m, AdjMemArgs j)
+// c1 = lambda_MAD / A(X^n), c2 = lambda_MAD MAD / A(X^n) (the sum of the second
+// term over the edges is c2 A).  The first step's MAD comes from the set-up (a
+// constant of the run, like the reference areas and volume).
+constexpr int ADJ_COEF_LANES = 13;
+constexpr int ADJ_COEF_NT = 256;
+constexpr int ADJ_COEF_MAX_MEM = ADJ_COEF_NT / ADJ_COEF_LANES + 2;  // membranes a workgroup's edges can span
+
+__global__ __launch_bounds__(ADJ_COEF_NT) void k_adj_mem_coef(MemArgs m, AdjMemArgs j)
 {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int e = gid / 12, d = gid - 12 * e;
+    __shared__ double s_mg[ADJ_COEF_MAX_MEM][5];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int gid = blockIdx.x * ADJ_COEF_NT + tid;
+    const int e_lo = blockIdx.x * ADJ_COEF_NT / ADJ_COEF_LANES;
+    const int e_hi = min((blockIdx.x * ADJ_COEF_NT + ADJ_COEF_NT - 1) / ADJ_COEF_LANES, m.n_edges - 1);
+    const int mem_lo = m.edge_mem[e_lo], n_mem_blk = m.edge_mem[e_hi] - mem_lo + 1;
+    for (int k = wave; k < n_mem_blk; k += ADJ_COEF_NT / 64) {
+        const int mem = mem_lo + k;
+        double s[6] = {0, 0, 0, 0, 0, 0};
+        for (int c = j.echunk_start[mem] + lane; c < j.echunk_start[mem + 1]; c += 64)
+            for (int t = 0; t < 6; t++) s[t] += j.epart[6 * (size_t)c + t];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int t = 0; t < 6; t++) s[t] += __shfl_xor(s[t], off, 64);
+        if (lane == 0) {
+            const MemParamDev p = m.params[mem];
+            const double *gl = m.glob + 8 * (size_t)mem;
+            const double *gp = j.glob_prev + 8 * (size_t)mem;
+            const double lmad = s[2] * (-2.0 * (double)p.global_bending / gl[4]);
+            double *mg = s_mg[k];
+            mg[0] = s[0] * (-(double)p.volume_modulus / (6.0 * gl[3]));
+            mg[1] = s[1] * (-(double)p.global_area_modulus / gl[4]);
+            mg[2] = lmad;
+            // MAD = M / A(X^n), M = sum over the MAD edges of kappa (|N1| + |N2|) / 3
+            mg[3] = lmad / gp[6];
+            mg[4] = lmad * gl[2] / gp[6];
+            const int ef = m.mem_edge_start[mem];
+            if (ef >= e_lo && ef <= e_hi) {
+                double *gm = j.gmem + 4 * (size_t)mem;
+                gm[0] += s[3];
+                gm[1] += s[4];
+                gm[2] += s[5] + s[2] * (-2.0 * (gl[2] - gl[5]) / gl[4]);
+            }
+        }
+    }
+    __syncthreads();
+    const int e = gid / ADJ_COEF_LANES, d = gid - ADJ_COEF_LANES * e;
     if (e >= m.n_edges) return;
     const int mem = m.edge_mem[e];
-    const double *mg = j.mglob + 8 * (size_t)mem;
+    const double *mg = s_mg[mem - mem_lo];
+    if (d == ADJ_COEF_LANES - 1) {
+        const float lv = (float)mg[0], la = (float)mg[1];
+        const int4 q = m.edges[e];
+        const float4 xa = m.X[q.x], xb = m.X[q.y], xc = m.X[q.z], xd = m.X[q.w];
+        memf::EdgeGeometry<float> g;
+        memf::edge_geometry<float>({xa.x, xa.y, xa.z}, {xb.x, xb.y, xb.z}, {xc.x, xc.y, xc.z}, {xd.x, xd.y, xd.z},
+                                   g);
+#pragma unroll
+        for (int sl = 0; sl < 4; sl++)
+            j.grec[4 * (size_t)e + sl] = make_float4(lv * g.d_vol[3 * sl] + la * g.d_area[3 * sl],
+                                                     lv * g.d_vol[3 * sl + 1] + la * g.d_area[3 * sl + 1],
+                                                     lv * g.d_vol[3 * sl + 2] + la * g.d_area[3 * sl + 2], 0.0f);
+        return;
+    }
     float *rec = j.mrec + 12 * (size_t)e + d;
     if (!j.mad_lag || mg[2] == 0.0) {
         *rec = 0.0f;
@@ -375,9 +391,8 @@ hipError_t launch_adj_coupling(const MemArgs &m, const IbmArgs &g, const AdjMemA
     hipLaunchKernelGGL(k_adj_spread, dim3(nb4), dim3(256), 0, s, m, g, j);
     hipLaunchKernelGGL(k_adj_mem_edges, dim3((24 * m.n_edges + 255) / 256), dim3(256), 0, s, m, j);
     hipLaunchKernelGGL(k_adj_mem_partials, dim3(j.n_echunks), dim3(256), 0, s, m, j);
-    hipLaunchKernelGGL(k_adj_mem_reduce, dim3(m.n_mem), dim3(64), 0, s, m, j);
-    hipLaunchKernelGGL(k_adj_mem_geo, dim3((m.n_edges + 127) / 128), dim3(128), 0, s, m, j);
-    hipLaunchKernelGGL(k_adj_mem_mad, dim3((12 * m.n_edges + 255) / 256), dim3(256), 0, s, m, j);
+    hipLaunchKernelGGL(k_adj_mem_coef, dim3((ADJ_COEF_LANES * m.n_edges + ADJ_COEF_NT - 1) / ADJ_COEF_NT),
+                       dim3(ADJ_COEF_NT), 0, s, m, j);
     hipLaunchKernelGGL(k_adj_mem_nodes, dim3((8 * m.n_nodes + 255) / 256), dim3(256), 0, s, m, g, j);
     hipLaunchKernelGGL(k_adj_interp, dim3(nb4), dim3(256), 0, s, m, g, j);
     return hipGetLastError();

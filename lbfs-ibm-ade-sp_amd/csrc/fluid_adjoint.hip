@@ -352,7 +352,8 @@ __device__ __forceinline__ void face_vjp(const FluidArgs &a, const float4 *__res
 }
 
 constexpr int ADJ_THREADS = 512;
-constexpr int ADJ_CHUNK = 512;       // faces per LDS chunk (8 float4 each: 64 KiB)
+constexpr int ADJ_CHUNK = 512;       // faces per LDS chunk (8 float4 each: 64 KiB), held as 8 planes
+                                     // of ADJ_CHUNK float4 so that a wavefront's stores are contiguous
 
 __device__ __forceinline__ void add4(float *acc, float4 v)
 {
@@ -394,7 +395,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_flux(FluidArgs a, AdjFluidA
             if (f < fb) face_vjp<L, false>(a, st, grad, j.mu, f, o);
             else face_vjp<L, true>(a, st, grad, j.mu, f, o);
 #pragma unroll
-            for (int k = 0; k < 8; k++) lds_adj[8 * (f - cs) + k] = o[k];
+            for (int k = 0; k < 8; k++) lds_adj[k * ADJ_CHUNK + (f - cs)] = o[k];
         }
         __syncthreads();
         const uint32_t base = (uint32_t)(cs - f0), n = (uint32_t)(ce - cs);
@@ -407,9 +408,9 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_flux(FluidArgs a, AdjFluidA
                 const uint32_t r = (s & 0x7fffu) - base;
                 if (r >= n) continue;
                 const int side = (s & 0x8000u) ? 0 : 4;  // bit 15 set: the cell owns the face
-                const float4 *src = lds_adj + 8 * r + side;
+                const float4 *src = lds_adj + side * ADJ_CHUNK + r;
 #pragma unroll
-                for (int q = 0; q < 4; q++) add4(&acc[u][4 * q], src[q]);
+                for (int q = 0; q < 4; q++) add4(&acc[u][4 * q], src[q * ADJ_CHUNK]);
             }
         __syncthreads();
     }

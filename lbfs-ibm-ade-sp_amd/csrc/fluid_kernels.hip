@@ -268,7 +268,7 @@ __global__ __launch_bounds__(NT) void k_flux_update(FluidArgs a, const float4 *_
 // of the cells across its border that its faces reach into LDS (the same
 // arithmetic as the gradient pass), the tile's faces from them, then the cell
 // update.  Results equal the two-pass step bit for bit; the gradient array is
-// neither written nor read.
+// not read, and written (the tile's own cells) only when a.grad_out is set.
 template <class L, bool BAND, int NT>
 __global__ __launch_bounds__(NT) void k_fluid_fused(FluidArgs a, const float4 *__restrict__ st,
                                                      float4 *__restrict__ st_out, float4 *__restrict__ res_out,
@@ -288,6 +288,12 @@ __global__ __launch_bounds__(NT) void k_fluid_fused(FluidArgs a, const float4 *_
         o[0] = make_float4(R[0], R[1], R[2], R[3]);
         o[1] = make_float4(R[4], R[5], R[6], R[7]);
         o[2] = make_float4(R[8], R[9], R[10], R[11]);
+        if (a.grad_out && q < nc) {
+            float4 *g = a.grad_out + 3 * (size_t)i;
+            g[0] = o[0];
+            g[1] = o[1];
+            g[2] = o[2];
+        }
     }
     __syncthreads();
     const int f0 = a.tile_face_start[t], f1 = a.tile_face_start[t + 1];

@@ -26,6 +26,8 @@ struct FluidArgs {
     const float *visc_factor;      // [n] or null
     const float *cell_force;       // [3 n] or null (IBM force density)
     float4 *face_moments;          // [tile faces] or null: S0, S1 of each face (kept for the adjoint)
+    float4 *grad_out;              // [3 n] or null: the fused pass also writes the gradients of its
+                                   // tile's cells (kept for the adjoint, the layout of the gradient pass)
     // tiles
     const int *tile_cell_start, *tile_face_start;
     const int *tile_bface_start;   // first boundary face of each tile's list
@@ -45,7 +47,8 @@ hipError_t launch_gradients(const FluidArgs &a, const float4 *st, float4 *grad, 
 hipError_t launch_flux_update(const FluidArgs &a, const float4 *st, const float4 *grad, float4 *st_out,
                               float4 *res_out, int do_update, hipStream_t s);
 // the whole fluid step in one pass per tile (gradients in LDS); equals
-// launch_gradients + launch_flux_update bit for bit
+// launch_gradients + launch_flux_update bit for bit, gradients included when
+// a.grad_out is set
 hipError_t launch_fluid_fused(const FluidArgs &a, const float4 *st, float4 *st_out, float4 *res_out, int do_update,
                               hipStream_t s);
 size_t fluid_fused_lds(const FluidArgs &a);

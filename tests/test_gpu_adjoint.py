@@ -44,8 +44,9 @@ def oracle_fluid_loss(mesh, st, c, steps, fx):
     return float(np.sum(c * np.stack([r, u, v, w], axis=1)))
 
 
-def gpu_fluid_adjoint(mesh, st, c, steps, every):
+def gpu_fluid_adjoint(mesh, st, c, steps, every, fused=True):
     s = abi.Solver(mesh, abi.make_params(lattice=abi.D3Q19, visc=VISC, dt=DT, body_force=BODY))
+    s.set_fused(fused)
     s.set_state(*st)
     s.adjoint_forward(steps, every)
     g = s.adjoint_backward(lam_state=c)
@@ -78,9 +79,12 @@ def test_fluid_adjoint_matches_oracle_differences():
 
 def test_adjoint_independent_of_checkpoint_interval():
     """Replays recompute the recorded trajectory bit for bit, so the
-    gradients do not depend on the checkpoint interval."""
+    gradients do not depend on the checkpoint interval, nor on whether the
+    steps and replays run the fused fluid pass (which then also writes the
+    recorded gradients) or the gradient and flux passes."""
     mesh, st, c = fluid_case()
     outs = [gpu_fluid_adjoint(mesh, st, c, 7, k) for k in (1, 3, 7)]
+    outs.append(gpu_fluid_adjoint(mesh, st, c, 7, 3, fused=False))
     for o in outs[1:]:
         assert np.array_equal(o["body_force"], outs[0]["body_force"])
         assert np.array_equal(o["state0"], outs[0]["state0"])
@@ -223,3 +227,56 @@ def test_coupled_adjoint_pow_springs():
     e = rel(g["moduli"][0, 0], fd)
     print("POW adjoint dL/dmu %.9e vs FD %.9e (rel %.2e)" % (g["moduli"][0, 0], fd, e))
     assert e < 1e-4, e
+
+
+def test_coupled_adjoint_two_membranes():
+    """Two cells with their own moduli in one run: each cell's modulus
+    gradients and the body-force gradient against central differences of the
+    FP64 oracle stepping both cells (the per-membrane sums of the adjoint are
+    split between workgroups that hold edges of both cells)."""
+    mesh = cases.channel_mesh(24, 16, 16, seed=23)
+    st = cases.smooth_state(mesh, seed=23, amp=0.005)
+    cells = []
+    for k, (c, ax) in enumerate((((7.0, 8.0, 8.0), (1, 1, 0)), ((17.0, 8.5, 7.5), (0, 1, 1)))):
+        xs, tri = abi.rbc(4, 6.0, center=c, rot=abi.rotation(ax, 0.3 + 0.2 * k))
+        xs = xs.reshape(-1, 3)
+        x0 = np.array(c) + (xs - np.array(c)) * np.array([1.05, 0.97, 0.98])
+        cells.append((x0, xs, tri))
+    pars = [dict(PARAMS), dict(PARAMS, shear_modulus=1.5 * PARAMS["shear_modulus"],
+                                global_bending_modulus=2.0 * PARAMS["global_bending_modulus"])]
+    steps = 8
+    s = abi.Solver(mesh, abi.make_params(lattice=abi.D3Q19, visc=VISC, dt=DT, body_force=BODY))
+    s.set_state(*st)
+    s.add_membranes([abi.Membrane(x0, tri, nodes_stress_free=xs, nodes_curvature=xs, **p)
+                     for (x0, xs, tri), p in zip(cells, pars)], ibm_kernel=abi.IBM_COS4)
+    s.adjoint_forward(steps, 3)
+    X, _, _ = s.membrane_nodes()
+    target = np.concatenate([xs for _, xs, _ in cells])
+    g = s.adjoint_backward(lam_nodes=2.0 * (X - target))
+    s.close()
+
+    def loss(ps, fx=BODY[0]):
+        fl = orc.OracleFluid(mesh, orc.OR_D3Q19, visc=VISC, dt=DT)
+        fl.set_state(*st)
+        mos = [orc.OracleMembrane(x0, tri, q, nodes_stress_free=xs, nodes_curvature=xs)
+               for (x0, xs, tri), q in zip(cells, ps)]
+        orc.OracleCoupled(fl, mesh, mos, abi.IBM_COS4, body_force=(fx, 0.0, 0.0)).step(steps)
+        return sum(float(np.sum((mo.X.reshape(-1, 3) - xs) ** 2)) for mo, (_, xs, _) in zip(mos, cells))
+
+    errs = {}
+    for k in range(2):
+        for col, key in ((0, "shear_modulus"), (2, "global_bending_modulus")):
+            v0 = pars[k][key]
+            h = 1e-3 * v0
+            up = [dict(p) for p in pars]
+            dn = [dict(p) for p in pars]
+            up[k][key] = v0 + h
+            dn[k][key] = v0 - h
+            fd = (loss(up) - loss(dn)) / (2 * h)
+            errs["cell%d %s" % (k, key)] = (g["moduli"][k, col], fd)
+    h = 1e-3 * BODY[0]
+    errs["body force x"] = (g["body_force"][0], (loss(pars, BODY[0] + h) - loss(pars, BODY[0] - h)) / (2 * h))
+    for k, (a, f) in errs.items():
+        print("two-cell adjoint %-32s %.9e vs FD %.9e (rel %.2e)" % (k, a, f, rel(a, f)))
+    for k, (a, f) in errs.items():
+        assert rel(a, f) < 1e-4, (k, a, f)
