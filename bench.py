@@ -113,17 +113,42 @@ def host_cpu():
     return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
 
 
+def profile_record(kernel):
+    """The committed rocprofv3 counter summary of `kernel` (a stage name) on
+    this workload, or None."""
+    path = os.path.join(REPO, "profiles", "traffic_channel_128x64x64.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` on this workload from the committed
     rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE per dispatch, each
     scaled by the same counter's reading on a copy of known size), or None."""
-    path = os.path.join(REPO, "profiles", "traffic_channel_128x64x64.json")
-    try:
-        with open(path) as f:
-            rec = json.load(f).get(kernel)
-    except (OSError, ValueError):
-        return None
+    rec = profile_record(kernel)
     return None if rec is None else rec.get("bytes_per_launch")
+
+
+# FP32 vector issue rate of the MI355X: 256 CUs x 4 SIMDs at 2.4 GHz, a
+# wave64 VALU instruction occupying its 16-lane SIMD for 4 cycles (no packed
+# FP32: the build turns it off)
+VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 4
+
+
+def valu_roofline(kernel, launch_ms):
+    """The dominant kernel against its vector-ALU floor: wave-level VALU
+    instructions per launch (SQ_INSTS_VALU from the committed counter summary)
+    over the chip's issue rate, as a fraction of the measured launch time."""
+    rec = profile_record(kernel)
+    insts = None if rec is None else rec.get("valu_insts_per_launch")
+    if not insts:
+        return None
+    floor_ms = insts / VALU_WAVE_INSTS_PER_S * 1e3
+    return {"insts_per_launch": insts, "peak_insts_per_s": VALU_WAVE_INSTS_PER_S, "floor_ms": floor_ms,
+            "frac": floor_ms / launch_ms}
 
 
 def timed(solver, steps, warmup):
@@ -156,7 +181,7 @@ def run_suspension(args, params, log):
     p3.block_threads = params.block_threads
     # the fluid pass (fused or two passes, the same bits) by the plain step of this mesh
     times = {}
-    for fz in (True, False):
+    for fz in ((True, False) if args.fused is None else (bool(args.fused),)):
         try:
             b_plain = new_solver(m3, p3, fz)
         except abi.LbfsError:
@@ -194,6 +219,8 @@ def main():
     ap.add_argument("--tile", type=int, nargs=3, default=None,
                     help="fixed tile shape of the flux/update kernel (skips the autotuner)")
     ap.add_argument("--threads", type=int, default=0, help="workgroup size with --tile (256 or 512)")
+    ap.add_argument("--fused", type=int, choices=(0, 1), default=None,
+                    help="with --tile: 1 the fused fluid pass, 0 the gradient and flux passes (default: the library's)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 1 GiB device-to-device copies (the reference for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -201,7 +228,8 @@ def main():
     ap.add_argument("--long-steps", type=int, default=5000, help="steps of the long channel + RBC run (0: none)")
     ap.add_argument("--tgv-steps", type=int, default=1000, help="steps of the Taylor-Green check (0: none)")
     ap.add_argument("--only-rbc", action="store_true",
-                    help="time the channel + RBC step alone (no plain step, suspension or adjoint): profiling runs")
+                    help="time the channel + RBC step alone (no plain step, suspension, ensemble or adjoint): "
+                         "profiling runs")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-every", type=int, default=100)
     ap.add_argument("--only-batched", action="store_true",
@@ -244,6 +272,7 @@ def main():
     if args.tile:
         from lbfs_amd import autotune
         params = autotune.with_tile(params, tuple(args.tile), args.threads or 256)
+        fused = None if args.fused is None else bool(args.fused)
     elif not args.no_autotune:
         # tile shape and workgroup size of the flux/update kernel (results do
         # not depend on them); tuned on the plain step, outside the timed runs
@@ -253,7 +282,7 @@ def main():
         log("autotuned tile %s, %d threads, %s fluid pass" % (list(params.tile), params.block_threads,
                                                               "fused" if fused else "two-pass"))
     if args.only_rbc:
-        args.no_batched = args.no_adjoint = True
+        args.no_batched = args.no_adjoint = args.no_ensemble = True
         args.long_steps = args.tgv_steps = 0
         ms_plain = float("nan")
     else:
@@ -474,7 +503,8 @@ def main():
         "rbc_step_vs_plain": None if args.only_rbc else ms_step / ms_plain,
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(names[dom]),
-                     "model_bytes_per_launch": mbytes[dom], "avg_launch_ms": dom_ms},
+                     "model_bytes_per_launch": mbytes[dom], "avg_launch_ms": dom_ms,
+                     "valu": valu_roofline(names[dom], dom_ms)},
         "kernels_ms": kms,
         "event_ms_per_step": ev_ms,
         "max_speed": diag["max_speed"],

@@ -24,15 +24,18 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc, bool fused_fl
     if (have_membranes) {
         // IBM interpolation from the state at the start of the step and node
         // advection, then membrane forces at the advected nodes and spreading
-        // (Solver_gpu.cu:808-915).  Interpolation: X read and written 32, U 16,
-        // node indices 8, stencil base and tile 16, weight sum 4, slot 8 per
-        // node (the stencil cells' state, centroids and ids are cache reads)
+        // (Solver_gpu.cu:808-915).  Interpolation per node: X read and written
+        // 32, U 16, node indices 8, stencil base read and written 32, the kept
+        // weight row read and the new one written 512, list slot 8; per cell
+        // the stencils reach (about 8 per node, each cell reached by about 8
+        // nodes) state 16 and centroid 16
         const double nn = mem.n_nodes, ne = mem.n_edges, nt = mem.n_tri;
+        const double touched = std::min(n, 8.0 * nn);
         s.push_back({"ibm_interp", [ctx]() -> int {
                          HIPCHK(launch_ibm_interp(ctx->mem, ctx->ibm, ctx->st[ctx->cur], ctx->stream));
                          return LBFS_OK;
                      },
-                     84.0 * nn});
+                     608.0 * nn + 32.0 * touched});
         // volume and area sums: triangle 16 + nodes 48 per triangle; edge
         // forces: nodes 96 + indices 24 + constants 24 + state 16 + forces
         // written 64 per edge
@@ -46,26 +49,28 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc, bool fused_fl
                          return LBFS_OK;
                      },
                      224.0 * ne});
+        // node forces: per edge its four force slots 64 and their incidence
+        // ids 16; per node the incidence range 4 and the force written 16
         s.push_back({"mem_nodes", [ctx]() -> int {
                          HIPCHK(launch_mem_nodes(ctx->mem, ctx->stream));
                          return LBFS_OK;
                      },
-                     36.0 * nn});
+                     80.0 * ne + 20.0 * nn});
         if (with_visc)
             s.push_back({"internal_visc", [ctx]() -> int {
                              HIPCHK(launch_internal_visc(ctx->mem, ctx->ibm, ctx->delta_h, ctx->visc_factor, ctx->stream));
                              return LBFS_OK;
                          },
                          8.0 * n});
-        // spreading: per node read about 2.5 times (nodes listed around
-        // several tiles): base 16, weight sum 4, force 16, X 16, indices 8;
-        // per touched cell the centroid 16 and the force density written 12
-        // (about 64 cells per node, each cell reached by about 8 nodes)
+        // spreading per node: weight row 256, force 16, stencil base 16; per
+        // touched cell the force density written 12 (tile gathering) or the
+        // fixed-point sums read and written 48 (per-node and bin scattering)
+        const double cell_bytes = ibm.binned == IBM_SPREAD_GATHER ? 12.0 : 48.0;
         s.push_back({"ibm_spread", [ctx]() -> int {
                          HIPCHK(launch_ibm_spread(ctx->mem, ctx->ibm, ctx->stream));
                          return LBFS_OK;
                      },
-                     2.5 * 60.0 * nn + 28.0 * std::min(n, 8.0 * nn)});
+                     288.0 * nn + cell_bytes * touched});
     }
     const bool fused = fused_fluid && fused_ok && args.ts_mode != LBFS_TS_MIN;
     if (args.ts_mode == LBFS_TS_MIN)
@@ -88,16 +93,16 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc, bool fused_fl
         // spreading wrote into.  Face pass as below; apply: force density 12
         // + static force 12 + state read and written 32 + force written 12
         // per touched cell (about 8 per node)
-        const double touched = std::min(n, 8.0 * mem.n_nodes);
+        const double touched = std::min(n, 8.0 * (double)mem.n_nodes);
         if (fused)
-            // the fused pass moves the bytes of both passes' model (the
-            // gradients stay in LDS)
+            // the fused pass: both passes' bytes less the gradients written
+            // and read back (they stay in LDS) and the second read of the state
             s.push_back({"fluid", [ctx]() -> int {
                              HIPCHK(launch_fluid_fused(ctx->args, ctx->st[ctx->cur], ctx->st[ctx->cur ^ 1],
                                                        ctx->res_out(), 1, ctx->stream));
                              return LBFS_OK;
                          },
-                         304.0 * n + 40.0 * nf});
+                         192.0 * n + 40.0 * nf});
         else
             s.push_back({"flux_update", [ctx]() -> int {
                              HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->st[ctx->cur ^ 1],
@@ -115,7 +120,9 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc, bool fused_fl
         return s;
     }
     // pass F: state 16 + gradients 48 + volume/centroid 16 + force 12 + state written 16 per cell,
-    // plus owner/nb 8, normal 12, face centroid 12, area 4, dt 4 per face
+    // plus owner/nb 8, normal 12, face centroid 12, area 4, dt 4 per face; the
+    // fused pass moves G + F less the gradients written and read back (they
+    // stay in LDS) and the second read of the state: 192 per cell
     if (fused)
         s.push_back({"fluid", [ctx]() -> int {
                          HIPCHK(launch_fluid_fused(ctx->args, ctx->st[ctx->cur], ctx->st[ctx->cur ^ 1], ctx->res_out(),
@@ -123,7 +130,7 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc, bool fused_fl
                          ctx->cur ^= 1;
                          return LBFS_OK;
                      },
-                     304.0 * n + 40.0 * nf});
+                     192.0 * n + 40.0 * nf});
     else
         s.push_back({"flux_update", [ctx]() -> int {
                          HIPCHK(launch_flux_update(ctx->args, ctx->st[ctx->cur], ctx->grad, ctx->st[ctx->cur ^ 1],
