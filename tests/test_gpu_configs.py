@@ -121,7 +121,7 @@ def test_three_resolutions_in_one_launch_parity():
 
 
 def test_suspension_full_size_parity_and_replay():
-    """The 256-cell suspension (256x128x128, 405642 nodes of three
+    """The 256-cell suspension (256x128x128, 380052 nodes of three
     resolutions at 20 % hematocrit) from a perturbed state: 2 steps against
     the oracle holding the same 256 membranes (velocity and node positions
     within 1e-5, node positions within 6 FP32 spacings), then 10 steps with
