@@ -171,6 +171,35 @@ def new_solver(mesh, params, fused=None):
     return s
 
 
+def coupled_pass(args, mesh, params, state, mems, steps):
+    """The fluid pass (fused, or the gradient and flux passes: the same bits)
+    for a coupled step: --fused when given, else the faster of the two over
+    `steps` whole coupled steps (the passes share the GPU with the coupling
+    chain differently than in the plain step).  Returns (fused, {fused: ms})."""
+    from lbfs_amd import abi
+    if args.fused is not None:
+        return bool(args.fused), {}
+    times = {}
+    for fz in (True, False):
+        try:
+            s = new_solver(mesh, params, fz)
+        except abi.LbfsError:
+            continue  # the tiles do not fit the fused pass
+        try:
+            s.set_state(*state)
+            s.add_membranes(mems, ibm_kernel=abi.IBM_COS4)
+            s.step(5)
+            s.sync()
+            times[fz] = s.time_steps(steps) / steps
+        finally:
+            s.close()
+    return min(times, key=times.get), times
+
+
+def pass_name(fz):
+    return "library default" if fz is None else ("fused" if fz else "two-pass")
+
+
 def run_suspension(args, params, log):
     """The 256-cell suspension (256x128x128, three resolutions, 20 %
     hematocrit): its step and its plain step on the same mesh, per kernel."""
@@ -189,8 +218,8 @@ def run_suspension(args, params, log):
         b_plain.set_state(*s3)
         times[fz] = timed(b_plain, max(args.steps // 4, 10), 5)
         b_plain.close()
-    fused = min(times, key=times.get)
-    ms3_plain = times[fused]
+    ms3_plain = min(times.values())
+    fused, ctimes = coupled_pass(args, m3, p3, s3, mems, 10)
     b = new_solver(m3, p3, fused)
     b.set_state(*s3)
     b.add_membranes(mems, ibm_kernel=abi.IBM_COS4)
@@ -201,7 +230,7 @@ def run_suspension(args, params, log):
     log("suspension step %.4f ms (plain %.4f ms)" % (ms3, ms3_plain))
     return {"workload": "256x128x128 jittered hex, 256 Evans-Fung cells (d0 %.1f, %d nodes), 20%% hematocrit, "
                         "D3Q19" % (d0, nodes),
-            "fluid_pass": "fused" if fused else "two-pass",
+            "fluid_pass": pass_name(fused), "plain_fluid_pass": pass_name(min(times, key=times.get)),
             "ms_per_step": ms3, "plain_ms_per_step": ms3_plain, "ratio_vs_plain": ms3 / ms3_plain,
             "mlups": m3.n_cells / (ms3 * 1e-3) / 1e6, "kernels_ms": k3}
 
@@ -292,6 +321,12 @@ def main():
         plain.close()
         log("plain fluid step %.4f ms" % ms_plain)
 
+    # the coupled step's own fluid pass (the plain step keeps the one tuned above)
+    plain_fused = fused
+    if not args.tile or args.fused is not None:
+        fused, coupled_times = coupled_pass(args, mesh, params, state, workloads.single_rbc(mesh), 50)
+        log("coupled step: %s fluid pass %s" % (pass_name(fused), {pass_name(k): round(v, 5)
+                                                                   for k, v in coupled_times.items()}))
     solver = new_solver(mesh, params, fused)
     solver.set_state(*state)
     rbc = workloads.single_rbc(mesh)
@@ -436,11 +471,12 @@ def main():
         m5, p5, s5, mems5 = workloads.ensemble()
         p5.tile[0], p5.tile[1], p5.tile[2] = params.tile[0], params.tile[1], params.tile[2]
         p5.block_threads = params.block_threads
-        e_plain = new_solver(m5, p5, fused)
+        e_plain = new_solver(m5, p5, plain_fused)
         e_plain.set_state(*s5)
         ms5_plain = timed(e_plain, max(args.steps // 4, 10), 5)
         e_plain.close()
-        e = new_solver(m5, p5, fused)
+        fused5, _ = coupled_pass(args, m5, p5, s5, mems5, 10)
+        e = new_solver(m5, p5, fused5)
         e.set_state(*s5)
         e.add_membranes(mems5, ibm_kernel=abi.IBM_COS4)
         if not args.no_cpu_baseline:
@@ -466,7 +502,8 @@ def main():
         ensemble = {"workload": "64 independent jittered channel boxes (edges from {32,48,64,96}, seeds 5000+i, "
                                 "each with its own duct body force), %d cells of 642 nodes, D3Q19, one mesh"
                                 % len(mems5),
-                    "cells": m5.n_cells, "ms_per_step": ms5, "plain_ms_per_step": ms5_plain,
+                    "cells": m5.n_cells, "fluid_pass": pass_name(fused5), "plain_fluid_pass": pass_name(plain_fused),
+                    "ms_per_step": ms5, "plain_ms_per_step": ms5_plain,
                     "ratio_vs_plain": ms5 / ms5_plain, "mlups": m5.n_cells / (ms5 * 1e-3) / 1e6, "kernels_ms": k5}
         log("ensemble step %.4f ms (plain %.4f ms)" % (ms5, ms5_plain))
 
@@ -495,7 +532,7 @@ def main():
                    "cells": n, "faces": mesh.n_faces, "lattice": "D3Q19",
                    "tile": [params.tile[0], params.tile[1], params.tile[2]],
                    "block_threads": params.block_threads or 256,
-                   "fluid_pass": "library default" if fused is None else ("fused" if fused else "two-pass")},
+                   "fluid_pass": pass_name(fused), "plain_fluid_pass": pass_name(plain_fused)},
         "autotune": tuning,
         "achieved_hbm_gbps_model": step_gbps,
         "plain_ms_per_step": None if args.only_rbc else ms_plain,

@@ -368,7 +368,7 @@ int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next, co
     HIPCHK(hipStreamWaitEvent(q.side2, q.fork, 0));
     HIPCHK(launch_mem_volume(m, q.side2));
     HIPCHK(launch_mem_edges(m, q.side2));
-    HIPCHK(launch_mem_nodes(m, q.side2));
+    if (!spread_gathers_forces(ctx->ibm)) HIPCHK(launch_mem_nodes(m, q.side2));
     HIPCHK(hipEventRecord(q.join2, q.side2));
     HIPCHK(hipStreamWaitEvent(q.main, q.join2, 0));
     HIPCHK(launch_ibm_spread(m, ctx->ibm, q.main));

@@ -121,6 +121,11 @@ struct IbmArgs {
 // their output reset to the static force), or every tile (all = 1, for
 // per-step force records).  Also closes out the membrane sums of the step and
 // clears the tile node counts and the overflow count for the next step.
+// the bin scattering gathers the node forces from the edge records itself
+// (every node is in one bin slot or on the overflow list), so the node-force
+// pass does not run before it
+inline bool spread_gathers_forces(const IbmArgs &g) { return g.binned == IBM_SPREAD_BINS; }
+
 struct ApplyArgs {
     const int *tile_cell_start;
     int n_tiles;

@@ -280,7 +280,7 @@ __device__ __forceinline__ float4 node_force8(const MemArgs &m, int nd, int sub)
 
 // one node by one wavefront (lane = stencil cell) into the global accumulators
 template <int K>
-__device__ __forceinline__ void spread_node_global(const MemArgs &m, const IbmArgs &g, int nd, int lane)
+__device__ __forceinline__ void spread_node_global(const MemArgs &m, const IbmArgs &g, int nd, int lane, float4 F)
 {
     constexpr int S = Stencil<K>::S;
     if (lane >= S * S * S) return;
@@ -294,7 +294,6 @@ __device__ __forceinline__ void spread_node_global(const MemArgs &m, const IbmAr
           wrap_index(B, 2, nb.z + off[2], w[2])))
         return;
     const int ic = cell_internal(g, B, w);
-    const float4 F = m.F[nd];
     unsigned long long *acc = g.acc + 3 * (size_t)ic;
     atomicAdd(acc, fix_term_f(F.x * wt, g.err, ERR_FIX_IBM));
     atomicAdd(acc + 1, fix_term_f(F.y * wt, g.err, ERR_FIX_IBM));
@@ -466,7 +465,10 @@ __global__ __launch_bounds__(SPREAD_NT) void k_ibm_spread_tiles(MemArgs m, IbmAr
 
 // Bin scattering: the workgroups loop over work items of up to 32 nodes of
 // the listed bins (IBM_BIN_CAP / 32 items per bin, empty ones skipped), so a
-// full bin is shared by several workgroups.  Each item spreads its nodes (one
+// full bin is shared by several workgroups.  Each item gathers its nodes'
+// forces from the edge records (eight lanes per node, the fixed order of
+// the node-force pass, which this path replaces: every node is in one bin
+// slot or on the overflow list) and writes them out, spreads its nodes (one
 // wavefront each, one lane per stencil cell, the kept weights) into an LDS
 // window of the bin and the S - 1 cells beyond its upper faces, then adds the
 // window's nonzero sums to the per-cell accumulators.
@@ -500,9 +502,17 @@ __global__ __launch_bounds__(BIN_NT) void k_ibm_spread_bins(MemArgs m, IbmArgs g
             const int p = g.tile_slots[(size_t)b * IBM_BIN_CAP + c_begin + tid];
             cnd[tid] = p;
             cbase[tid] = g.nbase[p];
-            cF[tid] = m.F[p];
         }
         __syncthreads();
+        if (tid < 8 * BIN_CH) {  // whole wavefronts: the eight-lane sums shuffle
+            const int jn = tid >> 3;
+            const bool ok = jn < cnt;
+            const float4 F = node_force8(m, ok ? cnd[jn] : 0, ok ? (tid & 7) : 8);
+            if (ok && (tid & 7) == 0) {
+                cF[jn] = F;
+                m.F[cnd[jn]] = F;
+            }
+        }
         // all weight rows of the item in flight at once (one coalesced row per node)
         for (int q = tid; q < C * cnt; q += BIN_NT) cw[q / C][q % C] = g.wnode[64 * (size_t)cnd[q / C] + q % C];
         __syncthreads();
@@ -561,9 +571,16 @@ __global__ __launch_bounds__(BIN_NT) void k_ibm_spread_bins(MemArgs m, IbmArgs g
             g.tile_acc[g.cell_tile[ic]] = 1;
         }
     }
-    // nodes of full bins: one wavefront each, straight into the accumulators
+    // nodes of full bins: one wavefront each (its force from the edge records
+    // by lanes 0-7), straight into the accumulators
     const int nov = min(*g.n_overflow, m.n_nodes);
-    for (int e = blockIdx.x * NW + wave; e < nov; e += gridDim.x * NW) spread_node_global<K>(m, g, g.overflow[e], lane);
+    for (int e = blockIdx.x * NW + wave; e < nov; e += gridDim.x * NW) {
+        const int nd = g.overflow[e];
+        float4 F = node_force8(m, nd, lane < 8 ? lane : 8);
+        F = make_float4(__shfl(F.x, 0, 64), __shfl(F.y, 0, 64), __shfl(F.z, 0, 64), 0.0f);
+        if (lane == 0) m.F[nd] = F;
+        spread_node_global<K>(m, g, nd, lane, F);
+    }
 }
 
 // per-node spreading of every node: one wavefront each
@@ -572,7 +589,7 @@ __global__ __launch_bounds__(256) void k_ibm_spread_nodes(MemArgs m, IbmArgs g)
 {
     const int nd = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (nd >= m.n_nodes) return;
-    spread_node_global<K>(m, g, nd, threadIdx.x & 63);
+    spread_node_global<K>(m, g, nd, threadIdx.x & 63, m.F[nd]);
 }
 
 // One workgroup per fluid tile: the tile's spread force densities (written

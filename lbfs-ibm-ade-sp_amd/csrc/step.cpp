@@ -50,12 +50,16 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc, bool fused_fl
                      },
                      224.0 * ne});
         // node forces: per edge its four force slots 64 and their incidence
-        // ids 16; per node the incidence range 4 and the force written 16
-        s.push_back({"mem_nodes", [ctx]() -> int {
-                         HIPCHK(launch_mem_nodes(ctx->mem, ctx->stream));
-                         return LBFS_OK;
-                     },
-                     80.0 * ne + 20.0 * nn});
+        // ids 16; per node the incidence range 4 and the force written 16 (in
+        // the spreading when it gathers them itself)
+        const double node_force_bytes = 80.0 * ne + 20.0 * nn;
+        const bool gather_forces = spread_gathers_forces(ibm);
+        if (!gather_forces)
+            s.push_back({"mem_nodes", [ctx]() -> int {
+                             HIPCHK(launch_mem_nodes(ctx->mem, ctx->stream));
+                             return LBFS_OK;
+                         },
+                         node_force_bytes});
         if (with_visc)
             s.push_back({"internal_visc", [ctx]() -> int {
                              HIPCHK(launch_internal_visc(ctx->mem, ctx->ibm, ctx->delta_h, ctx->visc_factor, ctx->stream));
@@ -70,7 +74,7 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc, bool fused_fl
                          HIPCHK(launch_ibm_spread(ctx->mem, ctx->ibm, ctx->stream));
                          return LBFS_OK;
                      },
-                     288.0 * nn + cell_bytes * touched});
+                     288.0 * nn + cell_bytes * touched + (gather_forces ? node_force_bytes : 0.0)});
     }
     const bool fused = fused_fluid && fused_ok && args.ts_mode != LBFS_TS_MIN;
     if (args.ts_mode == LBFS_TS_MIN)
@@ -184,7 +188,7 @@ int lbfs_ctx::run_stages(bool with_visc)
         if (!interp_first) HIPCHK(launch_ibm_interp(mem, ibm, st[cur], side));
         HIPCHK(launch_mem_volume(mem, side));
         HIPCHK(launch_mem_edges(mem, side));
-        HIPCHK(launch_mem_nodes(mem, side));
+        if (!spread_gathers_forces(ibm)) HIPCHK(launch_mem_nodes(mem, side));
         HIPCHK(launch_ibm_spread(mem, ibm, side));
         HIPCHK(hipEventRecord(ev_join, side));
         HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
@@ -206,7 +210,7 @@ int lbfs_ctx::run_stages(bool with_visc)
     HIPCHK(hipStreamWaitEvent(side2, ev_fork2, 0));
     HIPCHK(launch_mem_volume(mem, side2));
     HIPCHK(launch_mem_edges(mem, side2));
-    HIPCHK(launch_mem_nodes(mem, side2));
+    if (!spread_gathers_forces(ibm)) HIPCHK(launch_mem_nodes(mem, side2));
     HIPCHK(hipEventRecord(ev_join2, side2));
     HIPCHK(hipStreamWaitEvent(stream, ev_join2, 0));
     HIPCHK(launch_ibm_spread(mem, ibm, stream));
