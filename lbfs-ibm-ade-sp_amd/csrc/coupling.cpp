@@ -148,6 +148,10 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     if ((rc = cp_alloc<float4>(ctx, &dF, nullptr, nn))) return rc;
     if ((rc = cp_alloc(ctx, &dorig, origin.data(), count))) return rc;
     if ((rc = cp_alloc(ctx, &dnm, node_mem.data(), nn))) return rc;
+    std::vector<float4> node_origin(nn);
+    for (size_t i = 0; i < nn; i++) node_origin[i] = origin[node_mem[i]];
+    float4 *dnorig;
+    if ((rc = cp_alloc(ctx, &dnorig, node_origin.data(), nn))) return rc;
     if ((rc = cp_alloc(ctx, &dtri, tri.data(), tri.size()))) return rc;
     if ((rc = cp_alloc(ctx, &dts, mem_tri_start.data(), mem_tri_start.size()))) return rc;
     if ((rc = cp_alloc(ctx, &des0, mem_edge_start.data(), mem_edge_start.size()))) return rc;
@@ -170,7 +174,7 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
     if ((rc = cp_alloc<double>(ctx, &ctx->mem_energy, nullptr, 7 * (size_t)count))) return rc;
     m.mem_node_start = dns;
     m.bbox = dbb;
-    m.X = dX; m.U = dU; m.F = dF; m.node_mem = dnm; m.origin = dorig; m.tri = dtri;
+    m.X = dX; m.U = dU; m.F = dF; m.node_mem = dnm; m.origin = dorig; m.node_origin = dnorig; m.tri = dtri;
     m.mem_tri_start = dts; m.mem_edge_start = des0; m.edges = dedg; m.edge_mem = dem;
     m.edge_const = dec; m.edge_area0 = dea; m.edge_state = des; m.edge_force = def;
     m.inc_start = dis; m.inc = dinc; m.params = dpar; m.glob = dgl; m.mem_acc = dmacc;

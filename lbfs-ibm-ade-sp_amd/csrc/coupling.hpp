@@ -37,6 +37,7 @@ struct MemArgs {
     float4 *F;                // node force
     const int *node_mem;
     const float4 *origin;     // per membrane
+    const float4 *node_origin;  // per node: its membrane's origin (one load where origin[node_mem] takes two)
     const int4 *tri;          // a, b, c, membrane
     const int *mem_tri_start, *mem_edge_start, *mem_node_start;
     const int4 *edges;        // a, b, c, d (edge a-c)

@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
     const bool live = gid / L < m.n_nodes;
     const int nd = live ? gid / L : m.n_nodes - 1;  // dead lanes shadow a live node for the group sums
     const IbmBlock B = node_block(g, nd);
-    const float4 o = m.origin[m.node_mem[nd]];
+    const float4 o = m.node_origin[nd];
     float4 xl = m.X[nd];
     float *row = g.wnode + 64 * (size_t)nd + S * r;  // this lane's S weights
     if constexpr (INTERP) {
