@@ -64,6 +64,9 @@ extern "C" {
                                         GPU reconstruction sets grad_*_1 = grad_*_2 there (LBFS.cu:391-394);
                                         without it the owner-side points take the owner's own gradient
                                         (Solver.cpp:918-1000) */
+#define LBFS_FLAG_HOST_GEOMETRY 128 /* set-up only (no counterpart in the reference): the FP64 mesh geometry and
+                                        least-squares weights on the host instead of the GPU; both give the
+                                        same bits */
 /* the reference GPU solver's semantics (calc_face_flux / calc_face_flux_bc, LBFS.cu:328-936, 1933-2048) */
 #define LBFS_FLAGS_REFERENCE_GPU (LBFS_FLAG_UPWIND_BAND | LBFS_FLAG_GHOST_DIR_GPU | LBFS_FLAG_GHOST_FACE_GRAD)
 

@@ -13,6 +13,7 @@
 
 #include "context.hpp"
 #include "fluid_kernels.hpp"
+#include "geometry_gpu.hpp"
 #include "mesh_host.hpp"
 
 using namespace lbfs;
@@ -56,11 +57,20 @@ extern "C" const char *lbfs_last_error(const lbfs_ctx *ctx)
 static int fluid_setup(lbfs_ctx *ctx, const lbfs_mesh_desc &m, const lbfs_params &p)
 {
     std::string err;
-    int rc = compute_geometry(m, p.visc, p.gamma, p.fneq_min, (p.flags & LBFS_FLAG_CLOSED_FACES) != 0, ctx->geo, err);
-    if (rc) return ctx->fail(rc, "geometry: %s", err.c_str());
+    const bool closed = (p.flags & LBFS_FLAG_CLOSED_FACES) != 0, det_compat = (p.flags & LBFS_FLAG_LS_DET_COMPAT) != 0;
     std::vector<double> lhs, rhs;
-    rc = compute_ls(ctx->geo, p.gradient_type, (p.flags & LBFS_FLAG_LS_DET_COMPAT) != 0, lhs, rhs, err);
-    if (rc) return ctx->fail(rc, "least squares: %s", err.c_str());
+    int rc;
+    if (p.flags & LBFS_FLAG_HOST_GEOMETRY) {
+        rc = compute_geometry(m, p.visc, p.gamma, p.fneq_min, closed, ctx->geo, err);
+        if (rc) return ctx->fail(rc, "geometry: %s", err.c_str());
+        rc = compute_ls(ctx->geo, p.gradient_type, det_compat, lhs, rhs, err);
+        if (rc) return ctx->fail(rc, "least squares: %s", err.c_str());
+    } else {
+        // the face and cell passes on the GPU, the same bits as the host's
+        rc = gpu_geometry(m, p.visc, p.gamma, p.fneq_min, closed, p.gradient_type, det_compat, ctx->stream, ctx->geo,
+                          lhs, rhs, err);
+        if (rc) return ctx->fail(rc, "geometry: %s", err.c_str());
+    }
     int tile[3] = {8, 8, 4};
     if (p.tile[0] || p.tile[1] || p.tile[2]) {
         if (p.tile[0] < 1 || p.tile[1] < 1 || p.tile[2] < 1 || p.tile[0] * p.tile[1] * p.tile[2] > 2048)

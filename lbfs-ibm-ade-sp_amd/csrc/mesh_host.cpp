@@ -4,11 +4,39 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
 
 namespace lbfs {
 
 namespace {
+
+// Runs body(begin, end) over [0, n) in contiguous ranges of at least
+// `grain` items on up to 16 host threads (LBFS_HOST_THREADS overrides the
+// count), the calling thread taking the first range.  The ranges must not
+// write to the same elements; the result then does not depend on the split.
+template <class F>
+void parallel_for(int n, F body, int grain = 1024)
+{
+    int nth = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
+    if (const char *e = std::getenv("LBFS_HOST_THREADS")) nth = std::max(1, std::atoi(e));
+    nth = std::min(nth, std::max(1, n / std::max(1, grain)));
+    if (nth <= 1) {
+        body(0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const int chunk = (n + nth - 1) / nth;
+    for (int t = 1; t < nth; t++) {
+        const int b = t * chunk, e = std::min(n, b + chunk);
+        if (b < e) pool.emplace_back([&body, b, e]() { body(b, e); });
+    }
+    body(0, std::min(n, chunk));
+    for (std::thread &th : pool) th.join();
+}
 
 struct V3 {
     double x, y, z;
@@ -384,120 +412,167 @@ int build_layout(const lbfs_mesh_desc &m, const Geometry &g, const std::vector<d
     }
     d.n_tiles = (int)d.tile_cell_start.size() - 1;
     d.inv.assign(nc, -1);
-    for (int i = 0; i < nc; i++) d.inv[d.perm[i]] = i;
+    parallel_for(nc, [&](int i0, int i1) {
+        for (int i = i0; i < i1; i++) d.inv[d.perm[i]] = i;
+    });
     // per-cell arrays
     d.stencil.assign(6 * (size_t)nc, 0);
     d.gw.assign(18 * (size_t)nc, 0.0f);
     d.inv_vol.assign(nc, 0.0f);
     d.cell_slots.assign(6 * (size_t)nc, 0);
-    for (int i = 0; i < nc; i++) {
-        const int c = d.perm[i];
-        d.inv_vol[i] = (float)(1.0 / g.vol[c]);
-        const double *L = &lhs[9 * (size_t)c];
-        for (int k = 0; k < 6; k++) {
-            const int nb = g.gcell[6 * (size_t)c + k];
-            d.stencil[(size_t)k * nc + i] = nb < nc ? d.inv[nb] : -1 - (nb - nc);
-            const double *R = &rhs[18 * (size_t)c + 3 * k];
-            for (int t = 0; t < 3; t++)
-                d.gw[(size_t)(k * 3 + t) * nc + i] = (float)(L[3 * t] * R[0] + L[3 * t + 1] * R[1] + L[3 * t + 2] * R[2]);
-        }
-    }
-    // tile face lists: every face touching a cell of the tile, in caller face order
-    d.tile_face_start.assign(1, 0);
-    d.tile_bface_start.clear();
-    d.fA.clear(); d.fB.clear(); d.fC.clear(); d.face_ref.clear();
-    d.face_first.assign(g.n_faces, -1);
-    d.max_tile_faces = 0;
-    d.tile_halo_start.assign(1, 0);
-    d.halo_ids.clear();
-    d.face_gslot.clear();
-    d.max_grad_slots = 0;
-    std::vector<int> halo_slot(nc, -1), halo;
-    std::vector<int> faces;
-    for (int t = 0; t < d.n_tiles; t++) {
-        faces.clear();
-        for (int i = d.tile_cell_start[t]; i < d.tile_cell_start[t + 1]; i++)
-            for (int k = 0; k < 6; k++) faces.push_back(g.gface[6 * (size_t)d.perm[i] + k]);
-        std::sort(faces.begin(), faces.end());
-        faces.erase(std::unique(faces.begin(), faces.end()), faces.end());
-        const int base = (int)d.face_ref.size();
-        if ((int)faces.size() > 32767) { err = "tile has too many faces"; return LBFS_E_UNSUPPORTED; }
-        // caller order puts internal faces (ids < n_int) before boundary faces
-        d.tile_bface_start.push_back(base + (int)(std::lower_bound(faces.begin(), faces.end(), g.n_int) - faces.begin()));
-        d.max_tile_faces = std::max(d.max_tile_faces, (int)faces.size());
-        const int c0 = d.tile_cell_start[t], c1 = d.tile_cell_start[t + 1], ntc = c1 - c0;
-        halo.clear();
-        auto gslot = [&](int ic) -> uint32_t {
-            if (ic >= c0 && ic < c1) return (uint32_t)(ic - c0);
-            if (halo_slot[ic] < 0) {
-                halo_slot[ic] = ntc + (int)halo.size();
-                halo.push_back(ic);
-            }
-            return (uint32_t)halo_slot[ic];
-        };
-        for (int f : faces) {
-            const int o = d.inv[g.owner[f]], n = g.nbr[f];
-            const uint32_t so = gslot(o), sn = n < nc ? gslot(d.inv[n]) : 0xffffu;
-            d.face_gslot.push_back(so | (sn << 16));
-        }
-        for (int ic : halo) halo_slot[ic] = -1;
-        d.halo_ids.insert(d.halo_ids.end(), halo.begin(), halo.end());
-        d.tile_halo_start.push_back((int)d.halo_ids.size());
-        d.max_grad_slots = std::max(d.max_grad_slots, ntc + (int)halo.size());
-        if (ntc + (int)halo.size() >= 0xffff) { err = "tile and its halo exceed 65534 cells"; return LBFS_E_UNSUPPORTED; }
-        for (int f : faces) {
-            const int o = g.owner[f], n = g.nbr[f];
-            const V3 xf = load3(&g.fc[3 * (size_t)f]);
-            const V3 d1 = load3(&g.cc[3 * (size_t)o]) - xf;
-            const V3 d2 = load3(&g.cc[3 * (size_t)n]) + load3(&g.shift[3 * (size_t)f]) - xf;
-            const double A = g.area[f];
-            const int on[2] = {d.inv[o], n < nc ? d.inv[n] : n};
-            float ob, nbb;
-            std::memcpy(&ob, &on[0], 4);
-            std::memcpy(&nbb, &on[1], 4);
-            d.fA.insert(d.fA.end(), {(float)(g.normal[3 * (size_t)f] * A), (float)(g.normal[3 * (size_t)f + 1] * A),
-                                     (float)(g.normal[3 * (size_t)f + 2] * A), (float)g.fdt[f]});
-            d.fB.insert(d.fB.end(), {(float)d1.x, (float)d1.y, (float)d1.z, ob});
-            d.fC.insert(d.fC.end(), {(float)d2.x, (float)d2.y, (float)d2.z, nbb});
-            if (d.face_first[f] < 0) d.face_first[f] = (int)d.face_ref.size();
-            d.face_ref.push_back(f);
-        }
-        for (int i = d.tile_cell_start[t]; i < d.tile_cell_start[t + 1]; i++) {
+    parallel_for(nc, [&](int i0, int i1) {
+        for (int i = i0; i < i1; i++) {
             const int c = d.perm[i];
+            d.inv_vol[i] = (float)(1.0 / g.vol[c]);
+            const double *L = &lhs[9 * (size_t)c];
             for (int k = 0; k < 6; k++) {
-                const int f = g.gface[6 * (size_t)c + k];
-                const int slot = (int)(std::lower_bound(faces.begin(), faces.end(), f) - faces.begin());
-                const int own = g.grole[6 * (size_t)c + k] == 0 ? 0x8000 : 0;
-                d.cell_slots[6 * (size_t)i + k] = (int16_t)(uint16_t)(slot | own);
+                const int nb = g.gcell[6 * (size_t)c + k];
+                d.stencil[(size_t)k * nc + i] = nb < nc ? d.inv[nb] : -1 - (nb - nc);
+                const double *R = &rhs[18 * (size_t)c + 3 * k];
+                for (int t = 0; t < 3; t++)
+                    d.gw[(size_t)(k * 3 + t) * nc + i] =
+                        (float)(L[3 * t] * R[0] + L[3 * t + 1] * R[1] + L[3 * t + 2] * R[2]);
             }
         }
-        d.tile_face_start.push_back(base + (int)faces.size());
+    });
+    // Tile face lists: every face touching a cell of the tile, in caller face
+    // order; and the tile's halo, the cells across its border in the order
+    // its faces first reach them.  Tiles are independent: first each tile's
+    // lists (on worker threads), then their offsets, then the records.
+    const int nt = d.n_tiles;
+    std::vector<std::vector<int>> tfaces(nt), thalo(nt);
+    std::string terr;
+    std::mutex terr_lock;
+    parallel_for(nt, [&](int t0, int t1) {
+        std::unordered_map<int, int> slot;
+        for (int t = t0; t < t1; t++) {
+            std::vector<int> &faces = tfaces[t];
+            for (int i = d.tile_cell_start[t]; i < d.tile_cell_start[t + 1]; i++)
+                for (int k = 0; k < 6; k++) faces.push_back(g.gface[6 * (size_t)d.perm[i] + k]);
+            std::sort(faces.begin(), faces.end());
+            faces.erase(std::unique(faces.begin(), faces.end()), faces.end());
+            const int c0 = d.tile_cell_start[t], c1 = d.tile_cell_start[t + 1], ntc = c1 - c0;
+            slot.clear();
+            std::vector<int> &halo = thalo[t];
+            auto visit = [&](int ic) {
+                if (ic >= c0 && ic < c1) return;
+                if (slot.emplace(ic, ntc + (int)halo.size()).second) halo.push_back(ic);
+            };
+            for (int f : faces) {
+                visit(d.inv[g.owner[f]]);
+                if (g.nbr[f] < nc) visit(d.inv[g.nbr[f]]);
+            }
+            if ((int)faces.size() > 32767 || ntc + (int)halo.size() >= 0xffff) {
+                std::lock_guard<std::mutex> lk(terr_lock);
+                if (terr.empty())
+                    terr = (int)faces.size() > 32767 ? "tile has too many faces" : "tile and its halo exceed 65534 cells";
+            }
+        }
+    }, 4);
+    if (!terr.empty()) { err = terr; return LBFS_E_UNSUPPORTED; }
+    d.tile_face_start.assign(nt + 1, 0);
+    d.tile_halo_start.assign(nt + 1, 0);
+    d.tile_bface_start.assign(nt, 0);
+    d.max_tile_faces = 0;
+    d.max_grad_slots = 0;
+    for (int t = 0; t < nt; t++) {
+        const int nft = (int)tfaces[t].size(), ntc = d.tile_cell_start[t + 1] - d.tile_cell_start[t];
+        d.tile_face_start[t + 1] = d.tile_face_start[t] + nft;
+        d.tile_halo_start[t + 1] = d.tile_halo_start[t] + (int)thalo[t].size();
+        // caller order puts internal faces (ids < n_int) before boundary faces
+        d.tile_bface_start[t] = d.tile_face_start[t] +
+            (int)(std::lower_bound(tfaces[t].begin(), tfaces[t].end(), g.n_int) - tfaces[t].begin());
+        d.max_tile_faces = std::max(d.max_tile_faces, nft);
+        d.max_grad_slots = std::max(d.max_grad_slots, ntc + (int)thalo[t].size());
     }
+    const size_t ntf = (size_t)d.tile_face_start[nt];
+    d.face_gslot.assign(ntf, 0u);
+    d.halo_ids.assign((size_t)d.tile_halo_start[nt], 0);
+    d.fA.assign(4 * ntf, 0.0f); d.fB.assign(4 * ntf, 0.0f); d.fC.assign(4 * ntf, 0.0f);
+    d.face_ref.assign(ntf, 0);
+    parallel_for(nt, [&](int t0, int t1) {
+        std::unordered_map<int, int> slot;
+        for (int t = t0; t < t1; t++) {
+            const std::vector<int> &faces = tfaces[t], &halo = thalo[t];
+            const int base = d.tile_face_start[t], c0 = d.tile_cell_start[t], c1 = d.tile_cell_start[t + 1];
+            const int ntc = c1 - c0;
+            slot.clear();
+            for (size_t h = 0; h < halo.size(); h++) slot.emplace(halo[h], ntc + (int)h);
+            std::copy(halo.begin(), halo.end(), d.halo_ids.begin() + d.tile_halo_start[t]);
+            auto gslot = [&](int ic) -> uint32_t {
+                if (ic >= c0 && ic < c1) return (uint32_t)(ic - c0);
+                return (uint32_t)slot.at(ic);
+            };
+            for (size_t q = 0; q < faces.size(); q++) {
+                const int f = faces[q];
+                const int o = g.owner[f], n = g.nbr[f];
+                const uint32_t so = gslot(d.inv[o]), sn = n < nc ? gslot(d.inv[n]) : 0xffffu;
+                const size_t r = (size_t)base + q;
+                d.face_gslot[r] = so | (sn << 16);
+                const V3 xf = load3(&g.fc[3 * (size_t)f]);
+                const V3 d1 = load3(&g.cc[3 * (size_t)o]) - xf;
+                const V3 d2 = load3(&g.cc[3 * (size_t)n]) + load3(&g.shift[3 * (size_t)f]) - xf;
+                const double A = g.area[f];
+                const int on[2] = {d.inv[o], n < nc ? d.inv[n] : n};
+                float ob, nbb;
+                std::memcpy(&ob, &on[0], 4);
+                std::memcpy(&nbb, &on[1], 4);
+                const float a4[4] = {(float)(g.normal[3 * (size_t)f] * A), (float)(g.normal[3 * (size_t)f + 1] * A),
+                                     (float)(g.normal[3 * (size_t)f + 2] * A), (float)g.fdt[f]};
+                const float b4[4] = {(float)d1.x, (float)d1.y, (float)d1.z, ob};
+                const float c4[4] = {(float)d2.x, (float)d2.y, (float)d2.z, nbb};
+                std::copy(a4, a4 + 4, d.fA.begin() + 4 * r);
+                std::copy(b4, b4 + 4, d.fB.begin() + 4 * r);
+                std::copy(c4, c4 + 4, d.fC.begin() + 4 * r);
+                d.face_ref[r] = f;
+            }
+            for (int i = c0; i < c1; i++) {
+                const int c = d.perm[i];
+                for (int k = 0; k < 6; k++) {
+                    const int f = g.gface[6 * (size_t)c + k];
+                    const int sl = (int)(std::lower_bound(faces.begin(), faces.end(), f) - faces.begin());
+                    const int own = g.grole[6 * (size_t)c + k] == 0 ? 0x8000 : 0;
+                    d.cell_slots[6 * (size_t)i + k] = (int16_t)(uint16_t)(sl | own);
+                }
+            }
+        }
+    }, 4);
+    d.face_first.assign(g.n_faces, -1);
+    for (size_t r = 0; r < ntf; r++)
+        if (d.face_first[d.face_ref[r]] < 0) d.face_first[d.face_ref[r]] = (int)r;
     // Static cell force: body force plus the pressure term rho0/3 sum_f n_f A_f
     // that the deviation-form face flux leaves out.  It vanishes on closed
     // cells; with the reference's face areas (sum of triangle areas) warped
     // faces leave a defect, kept here exactly in FP64.
     {
+        // per cell over its faces in face order (an owner's term before a
+        // neighbour's on a face the cell is both of): the sums of a face loop
         std::vector<double> &S = d.closure;
         S.assign(3 * (size_t)nc, 0.0);
-        for (int f = 0; f < g.n_faces; f++) {
-            // faces of empty patches carry no flux at all
-            if (f >= g.n_int && m.bc_vel_type && m.bc_vel_type[f - g.n_int] == LBFS_BC_EMPTY) continue;
-            const double A = g.area[f];
-            for (int t = 0; t < 3; t++) {
-                const double v = g.normal[3 * (size_t)f + t] * A;
-                S[3 * (size_t)g.owner[f] + t] += v;
-                if (g.nbr[f] < nc) S[3 * (size_t)g.nbr[f] + t] -= v;
-            }
-        }
+        parallel_for(nc, [&](int c0, int c1) {
+            for (int c = c0; c < c1; c++)
+                for (int k = 0; k < 6; k++) {
+                    const int f = g.gface[6 * (size_t)c + k];
+                    // faces of empty patches carry no flux at all
+                    if (f >= g.n_int && m.bc_vel_type && m.bc_vel_type[f - g.n_int] == LBFS_BC_EMPTY) continue;
+                    const double A = g.area[f];
+                    for (int t = 0; t < 3; t++) {
+                        const double v = g.normal[3 * (size_t)f + t] * A;
+                        if (g.grole[6 * (size_t)c + k] == 0) S[3 * (size_t)c + t] += v;
+                        else S[3 * (size_t)c + t] -= v;
+                    }
+                }
+        });
         d.static_force.assign(3 * (size_t)nc, 0.0f);
-        for (int i = 0; i < nc; i++) {
-            const int c = d.perm[i];
-            const double *bf = cell_body_force(blocks, c, body_force);
-            for (int t = 0; t < 3; t++)
-                d.static_force[3 * (size_t)i + t] =
-                    (float)(bf[t] - (t < dims ? rho0 / 3.0 * S[3 * (size_t)c + t] / g.vol[c] : 0.0));
-        }
+        parallel_for(nc, [&](int i0, int i1) {
+            for (int i = i0; i < i1; i++) {
+                const int c = d.perm[i];
+                const double *bf = cell_body_force(blocks, c, body_force);
+                for (int t = 0; t < 3; t++)
+                    d.static_force[3 * (size_t)i + t] =
+                        (float)(bf[t] - (t < dims ? rho0 / 3.0 * S[3 * (size_t)c + t] / g.vol[c] : 0.0));
+            }
+        });
     }
     d.bc_owner.assign(g.n_bc, 0);
     for (int b = 0; b < g.n_bc; b++) d.bc_owner[b] = d.inv[g.owner[g.n_int + b]];

@@ -19,6 +19,7 @@ TS_LOCAL, TS_CONSTANT, TS_MIN = 1, 2, 3
 MESH_OSAY, MESH_MZ = 0, 1
 FLAG_UPWIND_BAND, FLAG_NEUMANN_SIGNED, FLAG_GHOST_DIR_GPU, FLAG_LS_DET_COMPAT, FLAG_CLOSED_FACES = 1, 2, 4, 16, 32
 FLAG_GHOST_FACE_GRAD = 64
+FLAG_HOST_GEOMETRY = 128  # set-up: FP64 geometry and least-squares weights on the host (the same bits)
 # the reference GPU solver's face semantics (tie band, ghost direction, ghost gradient at every point)
 FLAGS_REFERENCE_GPU = FLAG_UPWIND_BAND | FLAG_GHOST_DIR_GPU | FLAG_GHOST_FACE_GRAD
 IBM_LINEAR2, IBM_COS4, IBM_LAGRANGE4 = 0, 1, 2
