@@ -220,9 +220,12 @@ def run_suspension(args, params, log):
         b_plain.close()
     ms3_plain = min(times.values())
     fused, ctimes = coupled_pass(args, m3, p3, s3, mems, 10)
+    t0 = time.perf_counter()
     b = new_solver(m3, p3, fused)
+    t1 = time.perf_counter()
     b.set_state(*s3)
     b.add_membranes(mems, ibm_kernel=abi.IBM_COS4)
+    setup = {"create_s": t1 - t0, "membranes_s": time.perf_counter() - t1}
     ms3 = timed(b, max(args.steps // 4, 10), 5)
     k3 = b.time_kernels(10)
     b.close()
@@ -232,7 +235,7 @@ def run_suspension(args, params, log):
                         "D3Q19" % (d0, nodes),
             "fluid_pass": pass_name(fused), "plain_fluid_pass": pass_name(min(times, key=times.get)),
             "ms_per_step": ms3, "plain_ms_per_step": ms3_plain, "ratio_vs_plain": ms3 / ms3_plain,
-            "mlups": m3.n_cells / (ms3 * 1e-3) / 1e6, "kernels_ms": k3}
+            "mlups": m3.n_cells / (ms3 * 1e-3) / 1e6, "kernels_ms": k3, "setup": setup}
 
 
 def main():
@@ -327,10 +330,13 @@ def main():
         fused, coupled_times = coupled_pass(args, mesh, params, state, workloads.single_rbc(mesh), 50)
         log("coupled step: %s fluid pass %s" % (pass_name(fused), {pass_name(k): round(v, 5)
                                                                    for k, v in coupled_times.items()}))
+    t0 = time.perf_counter()
     solver = new_solver(mesh, params, fused)
+    t1 = time.perf_counter()
     solver.set_state(*state)
     rbc = workloads.single_rbc(mesh)
     solver.add_membranes(rbc, ibm_kernel=abi.IBM_COS4)
+    setup = {"create_s": t1 - t0, "membranes_s": time.perf_counter() - t1}
     # the sample the CPU baseline repeats in FP64: the first steps from rest
     c2_sample = None
     if not args.no_cpu_baseline:
@@ -476,9 +482,12 @@ def main():
         ms5_plain = timed(e_plain, max(args.steps // 4, 10), 5)
         e_plain.close()
         fused5, _ = coupled_pass(args, m5, p5, s5, mems5, 10)
+        t0 = time.perf_counter()
         e = new_solver(m5, p5, fused5)
+        t1 = time.perf_counter()
         e.set_state(*s5)
         e.add_membranes(mems5, ibm_kernel=abi.IBM_COS4)
+        setup5 = {"create_s": t1 - t0, "membranes_s": time.perf_counter() - t1}
         if not args.no_cpu_baseline:
             # the instance with the most cells, its first 10 steps from rest
             counts = np.bincount([mm.block for mm in mems5], minlength=len(m5.members))
@@ -504,7 +513,8 @@ def main():
                                 % len(mems5),
                     "cells": m5.n_cells, "fluid_pass": pass_name(fused5), "plain_fluid_pass": pass_name(plain_fused),
                     "ms_per_step": ms5, "plain_ms_per_step": ms5_plain,
-                    "ratio_vs_plain": ms5 / ms5_plain, "mlups": m5.n_cells / (ms5 * 1e-3) / 1e6, "kernels_ms": k5}
+                    "ratio_vs_plain": ms5 / ms5_plain, "mlups": m5.n_cells / (ms5 * 1e-3) / 1e6, "kernels_ms": k5,
+                    "setup": setup5}
         log("ensemble step %.4f ms (plain %.4f ms)" % (ms5, ms5_plain))
 
     cpu = parity = None
@@ -543,6 +553,7 @@ def main():
                      "model_bytes_per_launch": mbytes[dom], "avg_launch_ms": dom_ms,
                      "valu": valu_roofline(names[dom], dom_ms)},
         "kernels_ms": kms,
+        "setup": setup,
         "event_ms_per_step": ev_ms,
         "max_speed": diag["max_speed"],
         "membrane_volume_rel_change": mglob[0] / mglob[3] - 1.0,
