@@ -177,6 +177,9 @@ int lbfs_step(lbfs_ctx *ctx, int nsteps);
 #define LBFS_OPT_FUSED_FLUID 4 /* 1 (default where the tiles fit): whole steps run the fluid passes as one
                                  launch per tile (the tile's gradients in LDS); 0: gradients, then the
                                  face pass; the result is the same bit for bit */
+#define LBFS_OPT_VISC_SEARCH 5 /* 1 (default): the internal-viscosity field searches binned triangles and
+                                 nodes; 0: every triangle of each membrane (immersed_boundary_method.cu:
+                                 389-527); the field is the same bit for bit */
 int lbfs_set_option(lbfs_ctx *ctx, int option, int value);
 
 /* per-stage entry points (parity tests) */

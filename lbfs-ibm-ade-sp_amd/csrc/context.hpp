@@ -61,6 +61,8 @@ struct lbfs_ctx {
     std::vector<double> mem_shear;
     std::vector<void *> coupling_allocs;
     float *visc_factor = nullptr;   // [n] internal viscosity field, null while every ratio is 1
+    lbfs::ViscGrid visc_grid{};     // its search structure (allocated with it)
+    int visc_search = 1;            // LBFS_OPT_VISC_SEARCH
     float delta_h = 0.1f;           // smoothing width of the field (unstructured_mesh.cpp:263-264)
     long long t_step = 0;           // steps since the membranes were attached
 

@@ -40,6 +40,7 @@ void lbfs_ctx::release_extensions()
     mem_energy = nullptr;
     visc_factor = nullptr;
     args.visc_factor = nullptr;
+    visc_grid = ViscGrid{};
 }
 
 extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, int count, int ibm_kernel)
@@ -406,6 +407,17 @@ extern "C" int lbfs_add_membranes(lbfs_ctx *ctx, const lbfs_membrane_desc *mem, 
         if ((rc = cp_alloc(ctx, &vf, ones.data(), nc))) return rc;
         ctx->visc_factor = vf;
         ctx->args.visc_factor = vf;
+        ViscGrid &v = ctx->visc_grid;
+        v = ViscGrid{};
+        if ((rc = cp_alloc<float4>(ctx, &v.geo, nullptr, 3 * (size_t)count))) return rc;
+        if ((rc = cp_alloc<int>(ctx, &v.tri_off, nullptr, (size_t)count * (VISC_G2 * VISC_G2 + 1)))) return rc;
+        if ((rc = cp_alloc<int>(ctx, &v.tri_ent, nullptr, (size_t)VISC_TRI_CAP * tri.size()))) return rc;
+        if ((rc = cp_alloc<int>(ctx, &v.ovf, nullptr, (size_t)count))) return rc;
+        if ((rc = cp_alloc<int>(ctx, &v.node_off, nullptr, (size_t)count * (VISC_G3 * VISC_G3 * VISC_G3 + 1))))
+            return rc;
+        if ((rc = cp_alloc<int>(ctx, &v.node_ent, nullptr, nn))) return rc;
+        if ((rc = cp_alloc<unsigned>(ctx, &v.qx, nullptr, nn))) return rc;
+        v.on = ctx->visc_search;
     }
     double hmin = ctx->blocks[0].spacing;
     for (const HostBlock &hb : ctx->blocks) hmin = std::min(hmin, hb.spacing);

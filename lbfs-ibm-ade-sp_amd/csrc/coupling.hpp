@@ -156,8 +156,26 @@ hipError_t launch_mem_nodes(const MemArgs &m, hipStream_t s);
 // cell accumulators; launch_ibm_apply adds the force densities to the state
 hipError_t launch_ibm_spread(const MemArgs &m, const IbmArgs &g, hipStream_t s);
 hipError_t launch_ibm_apply(const IbmArgs &g, const ApplyArgs &p, hipStream_t s);
+// Binned search structure of the internal-viscosity field, rebuilt with it:
+// per membrane its triangles by the y-z bins their projections overlap (a ray
+// along +x from a cell can only cross those of the cell's bin) and its nodes
+// by 3-D bins (the nearest-vertex distance only matters within 2 delta_h).
+constexpr int VISC_G2 = 16;       // y-z bins per axis of a membrane's triangle grid
+constexpr int VISC_G3 = 16;       // bins per axis of its node grid
+constexpr int VISC_TRI_CAP = 8;   // triangle-bin entries per triangle; a membrane needing more is searched whole
+struct ViscGrid {
+    int on;                       // 0: every triangle of each membrane (the definition; the same bits)
+    float4 *geo;                  // 3 per membrane: (y0, z0, bins/len y, bins/len z), (x0, y0, z0, -), bins/len xyz
+    int *tri_off;                 // [n_mem (VISC_G2^2 + 1)] entry offsets of each y-z bin
+    int *tri_ent;                 // [VISC_TRI_CAP n_tri] triangle ids (a membrane's from VISC_TRI_CAP tri_start)
+    int *ovf;                     // [n_mem] 1: the membrane's entries did not fit (searched whole)
+    int *node_off;                // [n_mem (VISC_G3^3 + 1)]
+    int *node_ent;                // [n_nodes] node ids (a membrane's from its node_start)
+    unsigned *qx;                 // [n_nodes] order key of the largest smallest-x of the node's triangles
+};
 // internal viscosity field (immersed_boundary_method.cu:389-527)
-hipError_t launch_internal_visc(const MemArgs &m, const IbmArgs &g, float delta_h, float *visc_factor, hipStream_t s);
+hipError_t launch_internal_visc(const MemArgs &m, const IbmArgs &g, const ViscGrid &v, float delta_h,
+                                float *visc_factor, hipStream_t s);
 // output-step energies, 7 doubles per membrane (Solver_gpu.cu:1060-1139)
 hipError_t launch_mem_energy(const MemArgs &m, double *out, hipStream_t s);
 // device address of the static energy staging of the current device (taken

@@ -847,8 +847,198 @@ __global__ __launch_bounds__(256) void k_mem_bbox(MemArgs m)
     }
 }
 
-__global__ __launch_bounds__(256) void k_internal_visc(MemArgs m, IbmArgs g, float delta_h, float *visc_factor)
+// Per triangle and cell, shared by both searches so they give the same bits
+// (no product fused into an add): the Plücker sidedness of the ray from the
+// cell along +x against the triangle's edges (all of one strict sign: the
+// ray crosses it), and the distance from the cell to a vertex.
+__device__ __forceinline__ bool visc_ray_crosses(const float p[3][3], float pc0, float pc1, float pc2)
 {
+#pragma clang fp contract(off)
+    float side[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float *u = p[k], *v = p[(k + 1) % 3];
+        const float e0 = u[1] * v[2] - v[1] * u[2], e1 = u[2] - v[2], e2 = v[1] - u[1];
+        side[k] = pc0 * e1 + pc1 * e2 + pc2 * e0;
+    }
+    return (side[0] < 0 && side[1] < 0 && side[2] < 0) || (side[0] > 0 && side[1] > 0 && side[2] > 0);
+}
+
+__device__ __forceinline__ float visc_dist(const float4 c, const float p[3])
+{
+#pragma clang fp contract(off)
+    const float dx = c.x - p[0], dy = c.y - p[1], dz = c.z - p[2];
+    return sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
+// a triangle's vertices relative to the membrane origin (as stored): the
+// sidedness products then stay small and exact to a few ulps of the cell size
+__device__ __forceinline__ void visc_tri(const MemArgs &m, int t, float p[3][3])
+{
+    const int4 q = m.tri[t];
+    const int ids[3] = {q.x, q.y, q.z};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float4 x = m.X[ids[k]];
+        p[k][0] = x.x; p[k][1] = x.y; p[k][2] = x.z;
+    }
+}
+
+// order-preserving keys of floats (NaN aside) for an integer maximum
+__device__ __forceinline__ unsigned fkey(float f)
+{
+    const unsigned b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float fkey_value(unsigned k)
+{
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+__device__ __forceinline__ int visc_bin(float v, float v0, float inv, int n)
+{
+    const int b = (int)floorf((v - v0) * inv);
+    return b < 0 ? 0 : (b >= n ? n - 1 : b);
+}
+
+// One workgroup per membrane with a viscosity ratio other than 1: its
+// triangles into the y-z bins their (slightly widened) projections overlap,
+// the largest smallest-x over each node's triangles (a node counts for the
+// nearest-vertex distance of a cell exactly when one of its triangles lies
+// wholly at x beyond the cell), and its nodes into 3-D bins.  Bin counts
+// and cursors in LDS; entries within a bin in any order (the field does not
+// depend on it).
+__global__ __launch_bounds__(256) void k_visc_build(MemArgs m, ViscGrid v, float delta_h)
+{
+    constexpr int NB2 = VISC_G2 * VISC_G2, NB3 = VISC_G3 * VISC_G3 * VISC_G3, PER = NB3 / 256;
+    const int mem = blockIdx.x, tid = threadIdx.x;
+    if (m.params[mem].visc_ratio == 1.0f) return;  // uniform over the workgroup
+    __shared__ int c2[NB2], c3[NB3], part[256], s_tot;
+    const float4 lo = m.bbox[2 * mem], hi = m.bbox[2 * mem + 1], o = m.origin[mem];
+    const float pad = 2.0f * delta_h;
+    // grids in membrane coordinates (relative to its origin), over its box
+    // widened by 2 delta_h
+    const float y0 = lo.y - o.y - pad, z0 = lo.z - o.z - pad, x0 = lo.x - o.x - pad;
+    const float ey = hi.y - o.y + pad - y0, ez = hi.z - o.z + pad - z0, ex = hi.x - o.x + pad - x0;
+    const float iy = ey > 0.0f ? VISC_G2 / ey : 0.0f, iz = ez > 0.0f ? VISC_G2 / ez : 0.0f;
+    const float jx = ex > 0.0f ? VISC_G3 / ex : 0.0f, jy = ey > 0.0f ? VISC_G3 / ey : 0.0f,
+                jz = ez > 0.0f ? VISC_G3 / ez : 0.0f;
+    // a triangle joins every bin its projection, widened by a small margin
+    // (far beyond rounding), overlaps
+    const float margin = 1e-4f * (ey + ez);
+    if (tid == 0) {
+        v.geo[3 * (size_t)mem] = make_float4(y0, z0, iy, iz);
+        v.geo[3 * (size_t)mem + 1] = make_float4(x0, y0, z0, 0.0f);
+        v.geo[3 * (size_t)mem + 2] = make_float4(jx, jy, jz, 0.0f);
+    }
+    const int t0 = m.mem_tri_start[mem], t1 = m.mem_tri_start[mem + 1];
+    const int n0 = m.mem_node_start[mem], n1 = m.mem_node_start[mem + 1];
+    for (int b = tid; b < NB2; b += 256) c2[b] = 0;
+    for (int b = tid; b < NB3; b += 256) c3[b] = 0;
+    for (int n = n0 + tid; n < n1; n += 256) v.qx[n] = fkey(-INFINITY);
+    __syncthreads();
+    auto tri_bins = [&](const float p[3][3], int &by0, int &by1, int &bz0, int &bz1) {
+        const float ylo = fminf(fminf(p[0][1], p[1][1]), p[2][1]), yhi = fmaxf(fmaxf(p[0][1], p[1][1]), p[2][1]);
+        const float zlo = fminf(fminf(p[0][2], p[1][2]), p[2][2]), zhi = fmaxf(fmaxf(p[0][2], p[1][2]), p[2][2]);
+        by0 = visc_bin(ylo - margin, y0, iy, VISC_G2); by1 = visc_bin(yhi + margin, y0, iy, VISC_G2);
+        bz0 = visc_bin(zlo - margin, z0, iz, VISC_G2); bz1 = visc_bin(zhi + margin, z0, iz, VISC_G2);
+    };
+    for (int t = t0 + tid; t < t1; t += 256) {
+        float p[3][3];
+        visc_tri(m, t, p);
+        int by0, by1, bz0, bz1;
+        tri_bins(p, by0, by1, bz0, bz1);
+        for (int bz = bz0; bz <= bz1; bz++)
+            for (int by = by0; by <= by1; by++) atomicAdd(&c2[by + VISC_G2 * bz], 1);
+        const int4 q = m.tri[t];
+        const unsigned kx = fkey(fminf(fminf(p[0][0], p[1][0]), p[2][0]));
+        atomicMax(v.qx + q.x, kx);
+        atomicMax(v.qx + q.y, kx);
+        atomicMax(v.qx + q.z, kx);
+    }
+    __syncthreads();
+    // exclusive offsets of the y-z bins (one per thread)
+    {
+        const int c = tid < NB2 ? c2[tid] : 0;
+        part[tid] = c;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const int add = tid >= off ? part[tid - off] : 0;
+            __syncthreads();
+            part[tid] += add;
+            __syncthreads();
+        }
+        if (tid == 255) s_tot = part[255];
+        const int excl = part[tid] - c;
+        int *off = v.tri_off + (size_t)mem * (NB2 + 1);
+        if (tid < NB2) {
+            off[tid] = excl;
+            c2[tid] = excl;  // cursor
+        }
+        __syncthreads();
+        if (tid == 0) off[NB2] = s_tot;
+    }
+    const bool fits = s_tot <= VISC_TRI_CAP * (t1 - t0);
+    if (tid == 0) v.ovf[mem] = fits ? 0 : 1;
+    if (fits)
+        for (int t = t0 + tid; t < t1; t += 256) {
+            float p[3][3];
+            visc_tri(m, t, p);
+            int by0, by1, bz0, bz1;
+            tri_bins(p, by0, by1, bz0, bz1);
+            for (int bz = bz0; bz <= bz1; bz++)
+                for (int by = by0; by <= by1; by++) {
+                    const int slot = atomicAdd(&c2[by + VISC_G2 * bz], 1);
+                    v.tri_ent[(size_t)VISC_TRI_CAP * t0 + slot] = t;
+                }
+        }
+    // nodes into 3-D bins
+    auto node_bin = [&](int n) {
+        const float4 x = m.X[n];
+        return visc_bin(x.x, x0, jx, VISC_G3) + VISC_G3 * (visc_bin(x.y, y0, jy, VISC_G3) + VISC_G3 * visc_bin(x.z, z0, jz, VISC_G3));
+    };
+    for (int n = n0 + tid; n < n1; n += 256) atomicAdd(&c3[node_bin(n)], 1);
+    __syncthreads();
+    {
+        int sum = 0;
+        for (int k = 0; k < PER; k++) sum += c3[tid * PER + k];
+        part[tid] = sum;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const int add = tid >= off ? part[tid - off] : 0;
+            __syncthreads();
+            part[tid] += add;
+            __syncthreads();
+        }
+        int run = part[tid] - sum;
+        int *off = v.node_off + (size_t)mem * (NB3 + 1);
+        for (int k = 0; k < PER; k++) {
+            const int b = tid * PER + k, c = c3[b];
+            off[b] = run;
+            c3[b] = run;
+            run += c;
+        }
+        if (tid == 255) off[NB3] = part[255];
+        __syncthreads();
+    }
+    for (int n = n0 + tid; n < n1; n += 256) {
+        const int slot = atomicAdd(&c3[node_bin(n)], 1);
+        v.node_ent[n0 + slot] = n;
+    }
+}
+
+// Per cell: for every membrane with a viscosity ratio other than 1 whose
+// bounding box (widened by 2 delta_h) holds the cell, the parity of the
+// crossings of the ray from the cell along +x with the triangles lying
+// wholly at larger x (inside or outside) and the distance to the nearest
+// vertex of those triangles give the smoothed factor; several membranes:
+// the value furthest from 1.  With the search structure the crossings come
+// from the triangles of the cell's y-z bin and the distance from the nodes
+// within 2 delta_h (beyond 1.5 delta_h the smoothed factor is exactly 1 in
+// float, as it is with no vertex at all): the same bits as every triangle.
+__global__ __launch_bounds__(256) void k_internal_visc(MemArgs m, IbmArgs g, ViscGrid v, float delta_h, float *visc_factor)
+{
+    constexpr int NB2 = VISC_G2 * VISC_G2, NB3 = VISC_G3 * VISC_G3 * VISC_G3;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.n_cells) return;
     const float4 c = g.cen[i];
@@ -863,35 +1053,54 @@ __global__ __launch_bounds__(256) void k_internal_visc(MemArgs m, IbmArgs g, flo
         if (c.x < xl || c.x > xh || c.y < lo.y - 2.0f * delta_h || c.y > hi.y + 2.0f * delta_h ||
             c.z < lo.z - 2.0f * delta_h || c.z > hi.z + 2.0f * delta_h)
             continue;
+        // membrane coordinates: the cell relative to the membrane origin
         const float4 o = m.origin[mem];
-        const float pc0 = c.x * c.y - xh * c.y, pc1 = c.x * c.z - xh * c.z, pc2 = c.x - xh;
+        const float4 cr = make_float4(c.x - o.x, c.y - o.y, c.z - o.z, 0.0f);
+        const float xr = xh - o.x;
+        const float pc0 = cr.x * cr.y - xr * cr.y, pc1 = cr.x * cr.z - xr * cr.z, pc2 = cr.x - xr;
         int count = 0;
         float dmin = 1e9f;
-        for (int t = m.mem_tri_start[mem]; t < m.mem_tri_start[mem + 1]; t++) {
-            const int4 q = m.tri[t];
-            const int ids[3] = {q.x, q.y, q.z};
-            float p[3][3];
-            for (int k = 0; k < 3; k++) {
-                const float4 x = m.X[ids[k]];
-                p[k][0] = o.x + x.x; p[k][1] = o.y + x.y; p[k][2] = o.z + x.z;
+        if (!v.on || v.ovf[mem]) {
+            for (int t = m.mem_tri_start[mem]; t < m.mem_tri_start[mem + 1]; t++) {
+                float p[3][3];
+                visc_tri(m, t, p);
+                if (!(p[0][0] > cr.x && p[1][0] > cr.x && p[2][0] > cr.x)) continue;
+                if (visc_ray_crosses(p, pc0, pc1, pc2)) count++;
+                for (int k = 0; k < 3; k++) dmin = fminf(dmin, visc_dist(cr, p[k]));
             }
-            if (!(p[0][0] > c.x && p[1][0] > c.x && p[2][0] > c.x)) continue;
-            float side[3];
-            for (int k = 0; k < 3; k++) {
-                const float *u = p[k], *v = p[(k + 1) % 3];
-                const float e0 = u[1] * v[2] - v[1] * u[2], e1 = u[2] - v[2], e2 = v[1] - u[1];
-                side[k] = pc0 * e1 + pc1 * e2 + pc2 * e0;
+        } else {
+            const float4 g0 = v.geo[3 * (size_t)mem], g1 = v.geo[3 * (size_t)mem + 1], g2 = v.geo[3 * (size_t)mem + 2];
+            const int b2 = visc_bin(cr.y, g0.x, g0.z, VISC_G2) + VISC_G2 * visc_bin(cr.z, g0.y, g0.w, VISC_G2);
+            const int *toff = v.tri_off + (size_t)mem * (NB2 + 1);
+            const int *tent = v.tri_ent + (size_t)VISC_TRI_CAP * m.mem_tri_start[mem];
+            for (int e = toff[b2]; e < toff[b2 + 1]; e++) {
+                float p[3][3];
+                visc_tri(m, tent[e], p);
+                if (!(p[0][0] > cr.x && p[1][0] > cr.x && p[2][0] > cr.x)) continue;
+                if (visc_ray_crosses(p, pc0, pc1, pc2)) count++;
             }
-            if ((side[0] < 0 && side[1] < 0 && side[2] < 0) || (side[0] > 0 && side[1] > 0 && side[2] > 0)) count++;
-            for (int k = 0; k < 3; k++) {
-                const float dx = c.x - p[k][0], dy = c.y - p[k][1], dz = c.z - p[k][2];
-                dmin = fminf(dmin, sqrtf(dx * dx + dy * dy + dz * dz));
-            }
+            const float R = 2.02f * delta_h;
+            const int xa = visc_bin(cr.x - R, g1.x, g2.x, VISC_G3), xb = visc_bin(cr.x + R, g1.x, g2.x, VISC_G3);
+            const int ya = visc_bin(cr.y - R, g1.y, g2.y, VISC_G3), yb = visc_bin(cr.y + R, g1.y, g2.y, VISC_G3);
+            const int za = visc_bin(cr.z - R, g1.z, g2.z, VISC_G3), zb = visc_bin(cr.z + R, g1.z, g2.z, VISC_G3);
+            const int *noff = v.node_off + (size_t)mem * (NB3 + 1);
+            for (int bz = za; bz <= zb; bz++)
+                for (int by = ya; by <= yb; by++)
+                    for (int bx = xa; bx <= xb; bx++) {
+                        const int b = bx + VISC_G3 * (by + VISC_G3 * bz);
+                        for (int e = noff[b]; e < noff[b + 1]; e++) {
+                            const int n = v.node_ent[m.mem_node_start[mem] + e];
+                            if (!(fkey_value(v.qx[n]) > cr.x)) continue;
+                            const float4 x = m.X[n];
+                            const float p[3] = {x.x, x.y, x.z};
+                            dmin = fminf(dmin, visc_dist(cr, p));
+                        }
+                    }
         }
         const float factor = 2.0f * fminf(0.5f, dmin * 0.5f / delta_h + 1.0f / PI_F * sinf(PI_F * dmin / delta_h) * 0.5f);
-        const float v = 0.5f * ((count & 1) ? (r + 1.0f) + (r - 1.0f) * factor : (r + 1.0f) + (1.0f - r) * factor);
+        const float val = 0.5f * ((count & 1) ? (r + 1.0f) + (r - 1.0f) * factor : (r + 1.0f) + (1.0f - r) * factor);
         // several membranes: keep the value furthest from 1
-        if (!any || fabsf(v - 1.0f) > fabsf(out - 1.0f)) out = v;
+        if (!any || fabsf(val - 1.0f) > fabsf(out - 1.0f)) out = val;
         any = true;
     }
     visc_factor[i] = out;
@@ -993,10 +1202,12 @@ hipError_t energy_scratch_setup(int device, double **out)
     return hipSuccess;
 }
 
-hipError_t launch_internal_visc(const MemArgs &m, const IbmArgs &g, float delta_h, float *visc_factor, hipStream_t s)
+hipError_t launch_internal_visc(const MemArgs &m, const IbmArgs &g, const ViscGrid &v, float delta_h,
+                                float *visc_factor, hipStream_t s)
 {
     hipLaunchKernelGGL(k_mem_bbox, dim3(m.n_mem), dim3(256), 0, s, m);
-    hipLaunchKernelGGL(k_internal_visc, dim3((g.n_cells + 255) / 256), dim3(256), 0, s, m, g, delta_h, visc_factor);
+    if (v.on) hipLaunchKernelGGL(k_visc_build, dim3(m.n_mem), dim3(256), 0, s, m, v, delta_h);
+    hipLaunchKernelGGL(k_internal_visc, dim3((g.n_cells + 255) / 256), dim3(256), 0, s, m, g, v, delta_h, visc_factor);
     return hipGetLastError();
 }
 

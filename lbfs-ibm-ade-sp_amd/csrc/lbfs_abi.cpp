@@ -460,6 +460,12 @@ extern "C" int lbfs_set_option(lbfs_ctx *ctx, int option, int value)
         ctx->drop_graph();
         return LBFS_OK;
     }
+    if (option == LBFS_OPT_VISC_SEARCH) {
+        if (value < 0 || value > 1) return ctx->fail(LBFS_E_INVALID_ARG, "LBFS_OPT_VISC_SEARCH takes 0 or 1");
+        ctx->visc_search = value;
+        if (ctx->visc_factor) ctx->visc_grid.on = value;
+        return LBFS_OK;
+    }
     if (option == LBFS_OPT_STEP_SCHEDULE) {
         if (value < 0 || value > 2) return ctx->fail(LBFS_E_INVALID_ARG, "LBFS_OPT_STEP_SCHEDULE takes 0, 1 or 2");
         ctx->schedule_option = value;

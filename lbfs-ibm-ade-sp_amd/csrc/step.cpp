@@ -62,7 +62,8 @@ std::vector<lbfs_ctx::Stage> lbfs_ctx::step_stages(bool with_visc, bool fused_fl
                          node_force_bytes});
         if (with_visc)
             s.push_back({"internal_visc", [ctx]() -> int {
-                             HIPCHK(launch_internal_visc(ctx->mem, ctx->ibm, ctx->delta_h, ctx->visc_factor, ctx->stream));
+                             HIPCHK(launch_internal_visc(ctx->mem, ctx->ibm, ctx->visc_grid, ctx->delta_h, ctx->visc_factor,
+                                                          ctx->stream));
                              return LBFS_OK;
                          },
                          8.0 * n});

@@ -23,7 +23,7 @@ FLAG_HOST_GEOMETRY = 128  # set-up: FP64 geometry and least-squares weights on t
 # the reference GPU solver's face semantics (tie band, ghost direction, ghost gradient at every point)
 FLAGS_REFERENCE_GPU = FLAG_UPWIND_BAND | FLAG_GHOST_DIR_GPU | FLAG_GHOST_FACE_GRAD
 IBM_LINEAR2, IBM_COS4, IBM_LAGRANGE4 = 0, 1, 2
-OPT_GRAPH, OPT_IBM_BINNED, OPT_STEP_SCHEDULE, OPT_FUSED_FLUID = 1, 2, 3, 4
+OPT_GRAPH, OPT_IBM_BINNED, OPT_STEP_SCHEDULE, OPT_FUSED_FLUID, OPT_VISC_SEARCH = 1, 2, 3, 4, 5
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -679,6 +679,12 @@ class Solver:
         g = np.zeros(6 * len(self._membranes))
         check(load().lbfs_get_membrane_globals(self.ctx, ptr(g)), self.ctx)
         return g.reshape(-1, 6)
+
+    def set_visc_search(self, on):
+        """The internal-viscosity field from binned triangles and nodes (1,
+        the default) or from every triangle of each membrane (0); the same
+        bits either way."""
+        check(load().lbfs_set_option(self.ctx, OPT_VISC_SEARCH, 1 if on else 0), self.ctx)
 
     def set_fused(self, on):
         """Whole steps with the fused fluid pass (1, the default where the

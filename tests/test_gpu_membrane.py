@@ -461,3 +461,49 @@ def test_pow_springs_coupled_parity_and_energy():
     assert cases.rel_l2(np.concatenate([us, vs, ws]), np.concatenate([uo, vo, wo])) < 1e-5
     assert cases.rel_l2(X, mo.X.reshape(-1, 3)) < 1e-5
     assert cases.rel_l2(X - x0, mo.X.reshape(-1, 3) - x0) < 1e-4
+
+
+@pytest.mark.parametrize("layout", ["three cells", "far from the origin", "ensemble"])
+def test_internal_viscosity_search_equals_every_triangle(layout):
+    """The internal-viscosity field from the binned search (triangles by the
+    y-z bins their projections overlap, nodes by 3-D bins within 2 delta_h)
+    equals the field from every triangle of each membrane bit for bit: three
+    cells of different ratios and resolutions close together (one with a
+    bounding box overlapping another's), two cells near (250, 120, 120)
+    where the crossing test in absolute coordinates rounds by more than the
+    bin margin, or two boxes of an ensemble."""
+    outs = []
+    for search in (1, 0):
+        if layout != "ensemble":
+            if layout == "three cells":
+                mesh = cases.channel_mesh(28, 18, 16, seed=41)
+                shapes = [abi.rbc(4, 8.0, center=(8.0, 9.0, 8.0), rot=abi.rotation((0, 1, 1), 0.5)),
+                          abi.rbc(3, 6.0, center=(15.5, 8.0, 8.5), rot=abi.rotation((1, 0, 1), 1.1)),
+                          abi.rbc(4, 7.0, center=(22.0, 9.5, 7.5))]
+                ratios = (5.0, 3.0, 0.5)
+            else:
+                mesh = abi.HexMesh(26, 18, 18, origin=(238.0, 111.0, 111.0), periodic=(1, 0, 0), jitter=0.1, seed=45)
+                shapes = [abi.rbc(4, 8.0, center=(246.0, 120.0, 119.5), rot=abi.rotation((0, 1, 1), 0.7)),
+                          abi.rbc(4, 7.0, center=(256.5, 119.0, 120.5), rot=abi.rotation((1, 0, 1), 0.4))]
+                ratios = (5.0, 3.0)
+            mems = [abi.Membrane(x, t, **dict(PARAMS, internal_viscosity_ratio=r)) for (x, t), r in zip(shapes, ratios)]
+            s = abi.Solver(mesh, abi.make_params(visc=VISC, dt=DT, body_force=(2e-5, 0.0, 0.0)))
+            s.set_state(*cases.smooth_state(mesh, seed=41, amp=0.005))
+        else:
+            boxes = [cases.channel_mesh(16, 14, 14, seed=42), cases.channel_mesh(18, 16, 14, seed=43)]
+            mesh = abi.Ensemble(boxes)
+            mems = []
+            for k, (c, r) in enumerate((((8.0, 7.0, 7.0), 5.0), ((9.0, 8.0, 7.0), 2.0))):
+                x, t = abi.rbc(4, 7.0, center=c, rot=abi.rotation((1, 1, 0), 0.3 * k))
+                mems.append(abi.Membrane(x, t, block=k, **dict(PARAMS, internal_viscosity_ratio=r)))
+            s = abi.Solver(mesh, abi.make_params(visc=VISC, dt=DT, body_force=(2e-5, 0.0, 0.0)))
+            parts = [cases.smooth_state(m, seed=44 + k, amp=0.005) for k, m in enumerate(boxes)]
+            s.set_state(*[np.concatenate([p[v] for p in parts]) for v in range(4)])
+        s.add_membranes(mems, ibm_kernel=abi.IBM_COS4)
+        s.set_visc_search(search)
+        s.step(5)  # the field is built at t = 1
+        X, _, F = s.membrane_nodes()
+        outs.append((np.concatenate(s.get_state()), X, F))
+        s.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
