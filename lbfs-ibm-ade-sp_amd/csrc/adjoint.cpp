@@ -380,7 +380,11 @@ int replay_step(lbfs_ctx *ctx, StepRec &r, const float4 *st, float4 *st_next, co
     return LBFS_OK;
 }
 
-int reverse_step(lbfs_ctx *ctx, const StepRec &r, bool first, int &cur_l)
+// One reverse step.  updated: the update adjoint of this step already ran
+// (fused into the gradient transpose of the step after); prev: the record of
+// the step before, whose update adjoint then runs fused into this step's
+// gradient transpose (null: it runs at the start of that step).
+int reverse_step(lbfs_ctx *ctx, const StepRec &r, bool first, int &cur_l, bool updated, const StepRec *prev)
 {
     AdjointState &A = *ctx->adj;
     FluidArgs a = ctx->args;
@@ -395,10 +399,12 @@ int reverse_step(lbfs_ctx *ctx, const StepRec &r, bool first, int &cur_l)
     j.acc = ctx->have_membranes ? A.acc : nullptr;
     j.acc_rw = ctx->have_membranes ? A.acc : nullptr;
     j.umax = A.umax;
-    HIPCHK(launch_adj_update(a, j, r.st, r.res, ctx->stream));
+    AdjPrevStep pv{};
+    if (prev) pv = {prev->st, prev->res, ctx->have_membranes ? prev->cf : ctx->args.cell_force};
+    if (!updated) HIPCHK(launch_adj_update(a, j, r.st, r.res, ctx->stream));
     if (!ctx->have_membranes) {
         HIPCHK(launch_adj_flux(a, j, r.st, r.grad, A.max_tile_cells, ctx->stream));
-        HIPCHK(launch_adj_grad(a, j, ctx->stream));
+        HIPCHK(launch_adj_grad(a, j, prev ? &pv : nullptr, ctx->stream));
         cur_l ^= 1;
         return LBFS_OK;
     }
@@ -419,7 +425,7 @@ int reverse_step(lbfs_ctx *ctx, const StepRec &r, bool first, int &cur_l)
     jm.echunk = A.echunk; jm.echunk_start = A.echunk_start; jm.epart = A.epart; jm.n_echunks = A.n_echunks;
     HIPCHK(launch_adj_coupling(m, ctx->ibm, jm, ctx->stream));
     HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
-    HIPCHK(launch_adj_grad(a, j, ctx->stream));
+    HIPCHK(launch_adj_grad(a, j, prev ? &pv : nullptr, ctx->stream));
     cur_l ^= 1;
     return LBFS_OK;
 }
@@ -605,10 +611,15 @@ extern "C" int lbfs_adjoint_backward(lbfs_ctx *ctx, const double *lam_nodes, con
             if ((rc = replay_begin(ctx, sg - 1, *sets[b ^ 1], replay_lanes(A), p))) return rc;
         }
         if (n_seg > 1) HIPCHK(hipStreamWaitEvent(ctx->stream, ready[b], 0));
-        // the two streams' steps enqueued in turn, so both have work at once
+        // the two streams' steps enqueued in turn, so both have work at once;
+        // within the segment each step's update adjoint runs fused into the
+        // gradient transpose of the step after (the segment's first step
+        // takes the record of the segment before, which is still being
+        // replayed: its update adjoint runs on its own)
         for (int t = s1 - 1; t >= s0; t--) {
             if (replay_pending(p) && (rc = replay_next(ctx, p))) return rc;
-            if ((rc = reverse_step(ctx, (*sets[b])[t - s0], t == 0, cur_l))) return rc;
+            const StepRec *prev = t > s0 ? &(*sets[b])[t - 1 - s0] : nullptr;
+            if ((rc = reverse_step(ctx, (*sets[b])[t - s0], t == 0, cur_l, t < s1 - 1, prev))) return rc;
         }
         while (replay_pending(p))
             if ((rc = replay_next(ctx, p))) return rc;

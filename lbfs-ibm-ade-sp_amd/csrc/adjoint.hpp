@@ -81,7 +81,14 @@ hipError_t launch_adj_update(const FluidArgs &a, const AdjFluidArgs &j, const fl
 // a.face_moments holds the forward moments of the step; tiles of at most 1024 cells
 hipError_t launch_adj_flux(const FluidArgs &a, const AdjFluidArgs &j, const float4 *st, const float4 *grad,
                            int max_tile_cells, hipStream_t s);
-hipError_t launch_adj_grad(const FluidArgs &a, const AdjFluidArgs &j, hipStream_t s);
+// the record of the step before, for the update adjoint fused into the
+// gradient transpose: its start state, residual and cell force
+struct AdjPrevStep {
+    const float4 *st, *res;
+    const float *cf;
+};
+// prev: run the update adjoint of the step before in the same pass (null: not)
+hipError_t launch_adj_grad(const FluidArgs &a, const AdjFluidArgs &j, const AdjPrevStep *prev, hipStream_t s);
 // gforce = sum of the per-block partials, in a fixed order
 hipError_t launch_adj_force_sum(const AdjFluidArgs &j, hipStream_t s);
 

@@ -29,36 +29,39 @@ namespace lbfs {
 // ---------------------------------------------------------------------------
 // forward: drho' = drho + dt r0, rho' = rho0 + drho',
 //          u'_t  = u_t + dt (r_t + f_t - u_t r0) / rho'
-__global__ __launch_bounds__(256) void k_adj_update(FluidArgs a, AdjFluidArgs j, const float4 *__restrict__ st,
-                                                    const float4 *__restrict__ res)
+// Cell i of the update's adjoint from the step's state adjoint L, its start
+// state s0, residual r and cell force cf; p: the cell's body-force terms.
+__device__ __forceinline__ void adj_update_cell(const FluidArgs &a, const AdjFluidArgs &j, const float *cf_all, int i,
+                                                float4 L, float4 s0, float4 r, double p[3])
+{
+    float fx = a.body_force[0], fy = a.body_force[1], fz = a.body_force[2];
+    if (cf_all) {
+        const float *cf = cf_all + 3 * (size_t)i;
+        fx += cf[0]; fy += cf[1]; fz += cf[2];
+    }
+    const float r0 = r.x, r1 = r.y + fx, r2 = r.z + fy, r3 = r.w + fz;
+    const float dt = a.dt;
+    const float rho = a.rho0 + (s0.x + dt * r0);
+    const float inv = 1.0f / rho;
+    const float a1 = dt * (r1 - s0.y * r0), a2 = dt * (r2 - s0.z * r0), a3 = dt * (r3 - s0.w * r0);
+    const float lrho = -inv * inv * (L.y * a1 + L.z * a2 + L.w * a3);
+    const float ld = L.x + lrho;
+    const float lr0 = dt * ld - dt * inv * (L.y * s0.y + L.z * s0.z + L.w * s0.w);
+    const float c = dt * inv;
+    const float lr1 = L.y * c, lr2 = L.z * c, lr3 = L.w * c;
+    const float keep = 1.0f - dt * r0 * inv;
+    const float iv = a.inv_vol[i];
+    j.mu[i] = make_float4(lr0 * iv, lr1 * iv, lr2 * iv, lr3 * iv);
+    j.lcf[i] = make_float4(lr1, lr2, lr3, 0.0f);
+    j.ls[i] = make_float4(ld, L.y * keep, L.z * keep, L.w * keep);
+    p[0] = lr1; p[1] = lr2; p[2] = lr3;
+}
+
+// the workgroup's body-force terms (256 threads, one cell each) added to its
+// own partials: the running sums need no atomics
+__device__ __forceinline__ void adj_force_partials(const AdjFluidArgs &j, const double p[3])
 {
     __shared__ double sh[3][256];
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    double p[3] = {0.0, 0.0, 0.0};
-    if (i < a.n_cells) {
-        const float4 L = j.lam_in[i], s0 = st[i], r = res[i];
-        float fx = a.body_force[0], fy = a.body_force[1], fz = a.body_force[2];
-        if (a.cell_force) {
-            const float *cf = a.cell_force + 3 * (size_t)i;
-            fx += cf[0]; fy += cf[1]; fz += cf[2];
-        }
-        const float r0 = r.x, r1 = r.y + fx, r2 = r.z + fy, r3 = r.w + fz;
-        const float dt = a.dt;
-        const float rho = a.rho0 + (s0.x + dt * r0);
-        const float inv = 1.0f / rho;
-        const float a1 = dt * (r1 - s0.y * r0), a2 = dt * (r2 - s0.z * r0), a3 = dt * (r3 - s0.w * r0);
-        const float lrho = -inv * inv * (L.y * a1 + L.z * a2 + L.w * a3);
-        const float ld = L.x + lrho;
-        const float lr0 = dt * ld - dt * inv * (L.y * s0.y + L.z * s0.z + L.w * s0.w);
-        const float c = dt * inv;
-        const float lr1 = L.y * c, lr2 = L.z * c, lr3 = L.w * c;
-        const float keep = 1.0f - dt * r0 * inv;
-        const float iv = a.inv_vol[i];
-        j.mu[i] = make_float4(lr0 * iv, lr1 * iv, lr2 * iv, lr3 * iv);
-        j.lcf[i] = make_float4(lr1, lr2, lr3, 0.0f);
-        j.ls[i] = make_float4(ld, L.y * keep, L.z * keep, L.w * keep);
-        p[0] = lr1; p[1] = lr2; p[2] = lr3;
-    }
     for (int t = 0; t < 3; t++) sh[t][threadIdx.x] = p[t];
     __syncthreads();
     for (int stp = 128; stp > 0; stp >>= 1) {
@@ -66,8 +69,16 @@ __global__ __launch_bounds__(256) void k_adj_update(FluidArgs a, AdjFluidArgs j,
             for (int t = 0; t < 3; t++) sh[t][threadIdx.x] += sh[t][threadIdx.x + stp];
         __syncthreads();
     }
-    // every block owns its partials: the running sums need no atomics
     if (threadIdx.x < 3) j.fpart[3 * (size_t)blockIdx.x + threadIdx.x] += sh[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_adj_update(FluidArgs a, AdjFluidArgs j, const float4 *__restrict__ st,
+                                                    const float4 *__restrict__ res)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double p[3] = {0.0, 0.0, 0.0};
+    if (i < a.n_cells) adj_update_cell(a, j, a.cell_force, i, j.lam_in[i], st[i], res[i], p);
+    adj_force_partials(j, p);
 }
 
 // fixed-order sum of the partials into the body-force gradient
@@ -439,37 +450,54 @@ __device__ __forceinline__ void lam_grad(const float4 *__restrict__ lg, int i, f
     g[8] = c.x; g[9] = c.y; g[10] = c.z; g[11] = c.w;
 }
 
-__global__ __launch_bounds__(256) void k_adj_grad(FluidArgs a, AdjFluidArgs j)
+// The transpose of the gradients for cell i: its state adjoint (written to
+// lam_out and returned).
+__device__ __forceinline__ float4 adj_grad_cell(const FluidArgs &a, const AdjFluidArgs &j, int i)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
     const int n = a.n_cells;
-    if (i >= n) return;
     const float4 l0 = j.ls_acc[i];
     float acc[4] = {l0.x, l0.y, l0.z, l0.w};
     float g[12];
     lam_grad(j.lam_g, i, g);
+    // every stencil index first, then the neighbours' rows and adjoint
+    // gradients three entries at a time (a boundary entry loads the cell's
+    // own, unused), so the loads are in flight together rather than one
+    // index and one neighbour at a time
+    int jk[6];
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-        const int jn = a.stencil[(size_t)k * n + i];
-        const float wx = a.gw[(size_t)(3 * k) * n + i];
-        const float wy = a.gw[(size_t)(3 * k + 1) * n + i];
-        const float wz = a.gw[(size_t)(3 * k + 2) * n + i];
-        float c[4] = {1.0f, 1.0f, 1.0f, 1.0f};
-        if (jn < 0) {
-            const int2 ty = a.bc_type[-1 - jn];
-            const bool mv = ty.y == 1 || ty.y == 3;
-            c[0] = ty.x == 1 ? 2.0f : 0.0f;
-            c[1] = c[2] = c[3] = mv ? 2.0f : 0.0f;
+    for (int k = 0; k < 6; k++) jk[k] = a.stencil[(size_t)k * n + i];
+#pragma unroll
+    for (int k0 = 0; k0 < 6; k0 += 3) {
+        float4 rk[3];
+        float gk[3][12];
+#pragma unroll
+        for (int u = 0; u < 3; u++) {
+            const int jn = jk[k0 + u];
+            rk[u] = j.rev_w[(size_t)(k0 + u) * n + i];
+            lam_grad(j.lam_g, jn >= 0 ? jn : i, gk[u]);
         }
 #pragma unroll
-        for (int v = 0; v < 4; v++) acc[v] -= c[v] * (wx * g[3 * v] + wy * g[3 * v + 1] + wz * g[3 * v + 2]);
-        if (jn >= 0) {
-            // the neighbour's stencil row that points back here
-            const float4 r = j.rev_w[(size_t)k * n + i];
-            float gc[12];
-            lam_grad(j.lam_g, jn, gc);
+        for (int u = 0; u < 3; u++) {
+            const int k = k0 + u, jn = jk[k];
+            const float wx = a.gw[(size_t)(3 * k) * n + i];
+            const float wy = a.gw[(size_t)(3 * k + 1) * n + i];
+            const float wz = a.gw[(size_t)(3 * k + 2) * n + i];
+            float c[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+            if (jn < 0) {
+                const int2 ty = a.bc_type[-1 - jn];
+                const bool mv = ty.y == 1 || ty.y == 3;
+                c[0] = ty.x == 1 ? 2.0f : 0.0f;
+                c[1] = c[2] = c[3] = mv ? 2.0f : 0.0f;
+            }
 #pragma unroll
-            for (int v = 0; v < 4; v++) acc[v] += r.x * gc[3 * v] + r.y * gc[3 * v + 1] + r.z * gc[3 * v + 2];
+            for (int v = 0; v < 4; v++) acc[v] -= c[v] * (wx * g[3 * v] + wy * g[3 * v + 1] + wz * g[3 * v + 2]);
+            if (jn >= 0) {
+                // the neighbour's stencil row that points back here
+                const float4 r = rk[u];
+                const float *gc = gk[u];
+#pragma unroll
+                for (int v = 0; v < 4; v++) acc[v] += r.x * gc[3 * v] + r.y * gc[3 * v + 1] + r.z * gc[3 * v + 2];
+            }
         }
     }
     if (j.acc_rw) {
@@ -486,7 +514,24 @@ __global__ __launch_bounds__(256) void k_adj_grad(FluidArgs a, AdjFluidArgs j)
             }
         }
     }
-    j.lam_out[i] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+    const float4 L = make_float4(acc[0], acc[1], acc[2], acc[3]);
+    j.lam_out[i] = L;
+    return L;
+}
+
+// UPD: the update adjoint of the step before follows in the same pass (its
+// state adjoint is the one just computed; every reader of the update
+// adjoint's outputs of this step has finished): one launch fewer per step
+template <bool UPD>
+__global__ __launch_bounds__(256) void k_adj_grad(FluidArgs a, AdjFluidArgs j, AdjPrevStep prev)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double p[3] = {0.0, 0.0, 0.0};
+    if (i < a.n_cells) {
+        const float4 L = adj_grad_cell(a, j, i);
+        if (UPD) adj_update_cell(a, j, prev.cf, i, L, prev.st[i], prev.res[i], p);
+    }
+    if (UPD) adj_force_partials(j, p);
 }
 
 // ---------------------------------------------------------------------------
@@ -530,9 +575,11 @@ hipError_t launch_adj_flux(const FluidArgs &a, const AdjFluidArgs &j, const floa
     return hipGetLastError();
 }
 
-hipError_t launch_adj_grad(const FluidArgs &a, const AdjFluidArgs &j, hipStream_t s)
+hipError_t launch_adj_grad(const FluidArgs &a, const AdjFluidArgs &j, const AdjPrevStep *prev, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_adj_grad, dim3((a.n_cells + 255) / 256), dim3(256), 0, s, a, j);
+    const dim3 grid((a.n_cells + 255) / 256), blk(256);
+    if (prev) hipLaunchKernelGGL(k_adj_grad<true>, grid, blk, 0, s, a, j, *prev);
+    else hipLaunchKernelGGL(k_adj_grad<false>, grid, blk, 0, s, a, j, AdjPrevStep{});
     return hipGetLastError();
 }
 

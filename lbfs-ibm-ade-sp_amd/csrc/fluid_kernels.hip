@@ -27,19 +27,37 @@ namespace lbfs {
 // ---------------------------------------------------------------------------
 // gradients
 // ---------------------------------------------------------------------------
-// least-squares gradient of cell i (12 components: rho, u, v, w along x, y, z)
-__device__ __forceinline__ void cell_gradient(const FluidArgs &a, const float4 *__restrict__ st, int i, float R[12])
+// least-squares gradient of cell i (12 components: rho, u, v, w along x, y, z):
+// the loads of its stencil (every index, then every neighbour state, a ghost
+// entry loading the cell's own, and every weight, so they are in flight
+// together rather than one index and one state at a time), then the sums
+struct CellStencil {
+    float4 si, sk[6];
+    int jk[6];
+    float wk[18];
+};
+__device__ __forceinline__ void load_stencil(const FluidArgs &a, const float4 *__restrict__ st, int i, CellStencil &c)
 {
     const int n = a.n_cells;
-    const float4 si = st[i];
+    c.si = st[i];
+#pragma unroll
+    for (int k = 0; k < 6; k++) c.jk[k] = a.stencil[(size_t)k * n + i];
+#pragma unroll
+    for (int k = 0; k < 6; k++) c.sk[k] = st[c.jk[k] >= 0 ? c.jk[k] : i];
+#pragma unroll
+    for (int k = 0; k < 18; k++) c.wk[k] = a.gw[(size_t)k * n + i];
+}
+__device__ __forceinline__ void stencil_gradient(const FluidArgs &a, const CellStencil &c, float R[12])
+{
+    const float4 si = c.si;
 #pragma unroll
     for (int t = 0; t < 12; t++) R[t] = 0.0f;
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-        const int j = a.stencil[(size_t)k * n + i];
+        const int j = c.jk[k];
         float dr, du, dv, dw;
         if (j >= 0) {
-            const float4 sj = st[j];
+            const float4 sj = c.sk[k];
             dr = sj.x - si.x; du = sj.y - si.y; dv = sj.z - si.z; dw = sj.w - si.w;
         } else {
             // ghost: Dirichlet mirror 2 bc - owner (velocity codes 1 and 3),
@@ -53,14 +71,18 @@ __device__ __forceinline__ void cell_gradient(const FluidArgs &a, const float4 *
             dv = mv ? 2.0f * (bv.z - si.z) : 0.0f;
             dw = mv ? 2.0f * (bv.w - si.w) : 0.0f;
         }
-        const float wx = a.gw[(size_t)(3 * k) * n + i];
-        const float wy = a.gw[(size_t)(3 * k + 1) * n + i];
-        const float wz = a.gw[(size_t)(3 * k + 2) * n + i];
+        const float wx = c.wk[3 * k], wy = c.wk[3 * k + 1], wz = c.wk[3 * k + 2];
         R[0] += wx * dr; R[1] += wy * dr; R[2] += wz * dr;
         R[3] += wx * du; R[4] += wy * du; R[5] += wz * du;
         R[6] += wx * dv; R[7] += wy * dv; R[8] += wz * dv;
         R[9] += wx * dw; R[10] += wy * dw; R[11] += wz * dw;
     }
+}
+__device__ __forceinline__ void cell_gradient(const FluidArgs &a, const float4 *__restrict__ st, int i, float R[12])
+{
+    CellStencil c;
+    load_stencil(a, st, i, c);
+    stencil_gradient(a, c, R);
 }
 
 __global__ __launch_bounds__(256) void k_gradients(FluidArgs a, const float4 *__restrict__ st,
