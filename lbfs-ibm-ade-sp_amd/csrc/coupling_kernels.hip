@@ -159,16 +159,30 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
         const int4 nb = g.nbase[nd];
         float ux = 0.0f, uy = 0.0f, uz = 0.0f;
         int w[3];
-        if (wrap_index(B, 1, nb.y + j, w[1]) && wrap_index(B, 2, nb.z + k, w[2])) {
+        const bool rowok = wrap_index(B, 1, nb.y + j, w[1]) && wrap_index(B, 2, nb.z + k, w[2]);
+        // the row's cell indices first, then its states and kept weights (a
+        // cell off a wall loads cell 0, unused), so the loads are in flight
+        // together rather than one cell at a time
+        bool ok[S];
+        int ci[S];
 #pragma unroll
-            for (int i = 0; i < S; i++) {
-                if (!wrap_index(B, 0, nb.x + i, w[0])) continue;
-                const float wt = row[i];
-                const float4 s = st[cell_internal(g, B, w)];
-                ux = fmaf(wt, s.y, ux);
-                uy = fmaf(wt, s.z, uy);
-                uz = fmaf(wt, s.w, uz);
-            }
+        for (int i = 0; i < S; i++) {
+            ok[i] = rowok && wrap_index(B, 0, nb.x + i, w[0]);
+            ci[i] = ok[i] ? cell_internal(g, B, w) : 0;
+        }
+        float4 sv[S];
+        float rw[S];
+#pragma unroll
+        for (int i = 0; i < S; i++) {
+            sv[i] = st[ci[i]];
+            rw[i] = row[i];
+        }
+#pragma unroll
+        for (int i = 0; i < S; i++) {
+            if (!ok[i]) continue;
+            ux = fmaf(rw[i], sv[i].y, ux);
+            uy = fmaf(rw[i], sv[i].z, uy);
+            uz = fmaf(rw[i], sv[i].w, uz);
         }
         ux = group_sum<L>(ux);
         uy = group_sum<L>(uy);
@@ -190,12 +204,27 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
     {
         int w[3];
         const bool rowok = wrap_index(B, 1, base[1] + j, w[1]) && wrap_index(B, 2, base[2] + k, w[2]);
+        // the row's wrapped indices, then (cosine kernel) every centroid at
+        // once, then the weights
+        bool ok[S];
+        int wx[S];
+#pragma unroll
+        for (int i = 0; i < S; i++) ok[i] = rowok && wrap_index(B, 0, base[0] + i, wx[i]);
+        float4 c[S];
+#pragma unroll
+        for (int i = 0; i < S; i++) {
+            c[i] = float4{};
+            if constexpr (K == LBFS_IBM_COS4) {
+                const int wi[3] = {wx[i], w[1], w[2]};
+                c[i] = g.cen[ok[i] ? cell_internal(g, B, wi) : 0];
+            }
+        }
 #pragma unroll
         for (int i = 0; i < S; i++) {
             wt[i] = 0.0f;
-            if (!rowok || !wrap_index(B, 0, base[0] + i, w[0])) continue;
-            const float4 c = K == LBFS_IBM_COS4 ? g.cen[cell_internal(g, B, w)] : float4{};
-            wt[i] = stencil_weight<K>(g, B, P, xr, base, i, j, k, w, c);
+            if (!ok[i]) continue;
+            const int wi[3] = {wx[i], w[1], w[2]};
+            wt[i] = stencil_weight<K>(g, B, P, xr, base, i, j, k, wi, c[i]);
             ws += wt[i];
         }
     }
