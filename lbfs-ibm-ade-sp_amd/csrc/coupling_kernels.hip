@@ -293,11 +293,18 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
 __device__ __forceinline__ float4 node_force8(const MemArgs &m, int nd, int sub)
 {
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    if (sub < 8)
-        for (int j = m.inc_start[nd] + sub; j < m.inc_start[nd + 1]; j += 8) {
-            const float4 f = m.edge_force[m.inc[j]];
-            fx += f.x; fy += f.y; fz += f.z;
+    if (sub < 8) {
+        // two incidences per pass, their loads in flight together (the sum
+        // keeps the order j, j + 8, j + 16, ...)
+        const int j1 = m.inc_start[nd + 1];
+        for (int j = m.inc_start[nd] + sub; j < j1; j += 16) {
+            const bool two = j + 8 < j1;
+            const int e0 = m.inc[j], e1 = m.inc[two ? j + 8 : j];
+            const float4 f0 = m.edge_force[e0], f1 = m.edge_force[e1];
+            fx += f0.x; fy += f0.y; fz += f0.z;
+            if (two) { fx += f1.x; fy += f1.y; fz += f1.z; }
         }
+    }
 #pragma unroll
     for (int off = 4; off > 0; off >>= 1) {
         fx += __shfl_xor(fx, off, 64);
@@ -508,6 +515,7 @@ __global__ __launch_bounds__(BIN_NT) void k_ibm_spread_bins(MemArgs m, IbmArgs g
 {
     constexpr int S = Stencil<K>::S, C = S * S * S, NPW = 64 / C, NW = BIN_NT / 64;
     constexpr int NCH = IBM_BIN_CAP / BIN_CH, WD = 4 + S - 1, NWIN = WD * WD * WD;
+    constexpr int CWN = (C * BIN_CH + BIN_NT - 1) / BIN_NT;  // weight loads per thread and item
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     __shared__ unsigned long long racc[3][NWIN];
     __shared__ int cnd[BIN_CH];
@@ -533,6 +541,14 @@ __global__ __launch_bounds__(BIN_NT) void k_ibm_spread_bins(MemArgs m, IbmArgs g
             cbase[tid] = g.nbase[p];
         }
         __syncthreads();
+        // all weight rows of the item in flight at once (one coalesced row
+        // per node), held in registers while the node forces are gathered
+        float wv[CWN];
+#pragma unroll
+        for (int u = 0; u < CWN; u++) {
+            const int q = tid + u * BIN_NT;
+            wv[u] = q < C * cnt ? g.wnode[64 * (size_t)cnd[q / C] + q % C] : 0.0f;
+        }
         if (tid < 8 * BIN_CH) {  // whole wavefronts: the eight-lane sums shuffle
             const int jn = tid >> 3;
             const bool ok = jn < cnt;
@@ -542,8 +558,11 @@ __global__ __launch_bounds__(BIN_NT) void k_ibm_spread_bins(MemArgs m, IbmArgs g
                 m.F[cnd[jn]] = F;
             }
         }
-        // all weight rows of the item in flight at once (one coalesced row per node)
-        for (int q = tid; q < C * cnt; q += BIN_NT) cw[q / C][q % C] = g.wnode[64 * (size_t)cnd[q / C] + q % C];
+#pragma unroll
+        for (int u = 0; u < CWN; u++) {
+            const int q = tid + u * BIN_NT;
+            if (q < C * cnt) cw[q / C][q % C] = wv[u];
+        }
         __syncthreads();
         for (int jn = wave * NPW + lane / C; jn < cnt; jn += NW * NPW) {
             const float wt = cw[jn][lane % C];
@@ -663,6 +682,12 @@ __global__ __launch_bounds__(256) void k_ibm_apply(IbmArgs g, ApplyArgs p)
     int any = 0;
     for (int i = p.tile_cell_start[t] + tid; i < p.tile_cell_start[t + 1]; i += blockDim.x) {
         float f[3] = {0.0f, 0.0f, 0.0f};
+        // the cell's state, static force and (per-cell sums) scale loaded
+        // with its spread force rather than after it
+        const float4 s0 = p.st[i];
+        float sfc[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) sfc[c] = g.static_force[3 * (size_t)i + c];
         if (now) {
             if (gather) {
 #pragma unroll
@@ -670,15 +695,15 @@ __global__ __launch_bounds__(256) void k_ibm_apply(IbmArgs g, ApplyArgs p)
             } else {
                 unsigned long long *a = g.acc + 3 * (size_t)i;
                 const unsigned long long v[3] = {a[0], a[1], a[2]};
+                const float sc = g.spread_sc[i];
                 if ((v[0] | v[1] | v[2]) != 0ull) {
-                    const float sc = g.spread_sc[i];
 #pragma unroll
                     for (int c = 0; c < 3; c++) f[c] = (float)((double)(long long)v[c] * (1.0 / FIX_SCALE)) * sc;
                     a[0] = a[1] = a[2] = 0ull;
                 }
             }
             if (f[0] != 0.0f || f[1] != 0.0f || f[2] != 0.0f) {
-                float4 s = p.st[i];
+                float4 s = s0;
                 const float q = p.dt / (p.rho0 + s.x);
                 s.y += q * f[0];
                 s.z += q * f[1];
@@ -688,7 +713,7 @@ __global__ __launch_bounds__(256) void k_ibm_apply(IbmArgs g, ApplyArgs p)
             }
         }
 #pragma unroll
-        for (int c = 0; c < 3; c++) p.cell_force_out[3 * (size_t)i + c] = g.static_force[3 * (size_t)i + c] + f[c];
+        for (int c = 0; c < 3; c++) p.cell_force_out[3 * (size_t)i + c] = sfc[c] + f[c];
     }
     any = __syncthreads_or(any);
     if (tid == 0) {
