@@ -232,6 +232,13 @@ __device__ __forceinline__ void tile_update(const FluidArgs &a, const float4 *ld
     for (int i = c0 + threadIdx.x; i < c1; i += NT) {
         const uint32_t *sl = reinterpret_cast<const uint32_t *>(a.cell_slots) + 3 * (size_t)i;
         const uint32_t s01 = sl[0], s23 = sl[1], s45 = sl[2];
+        // the cell's state and force loaded with its slots, not after the sums
+        const float4 s0 = do_update ? st[i] : float4{};
+        float cfv[3] = {0.0f, 0.0f, 0.0f};
+        if (do_update && a.cell_force) {
+            const float *cf = a.cell_force + 3 * (size_t)i;
+            cfv[0] = cf[0]; cfv[1] = cf[1]; cfv[2] = cf[2];
+        }
         const uint32_t s[6] = {s01 & 0xffffu, s01 >> 16, s23 & 0xffffu, s23 >> 16, s45 & 0xffffu, s45 >> 16};
         float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f;
 #pragma unroll
@@ -246,11 +253,9 @@ __device__ __forceinline__ void tile_update(const FluidArgs &a, const float4 *ld
         if (!do_update) continue;
         float fx = a.body_force[0], fy = a.body_force[1], fz = a.body_force[2];
         if (a.cell_force) {
-            const float *cf = a.cell_force + 3 * (size_t)i;
-            fx += cf[0]; fy += cf[1]; fz += cf[2];
+            fx += cfv[0]; fy += cfv[1]; fz += cfv[2];
         }
         r1 += fx; r2 += fy; r3 += fz;
-        const float4 s0 = st[i];
         float dt = a.dt;
         if (a.ts_mode == LBFS_TS_LOCAL) {
             const float2 c = a.cfl_coef[i];
