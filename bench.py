@@ -306,10 +306,20 @@ def main():
         params = autotune.with_tile(params, tuple(args.tile), args.threads or 256)
         fused = None if args.fused is None else bool(args.fused)
     elif not args.no_autotune:
-        # tile shape and workgroup size of the flux/update kernel (results do
-        # not depend on them); tuned on the plain step, outside the timed runs
+        # tile shape, workgroup size and fluid pass (results do not depend on
+        # them); tuned on the coupled step this line reports (channel + RBC),
+        # outside the timed runs, among the 512-thread workgroups: the
+        # suspension and the adjoint reported beside it run on the same tiles,
+        # and their tile spreading and face transpose are built for 8x8x8
+        # tiles (on 256-cell tiles they run 20-25 % slower, and the adjoint's
+        # backward some 5 % slower on 16x8x4, while the channel step differs
+        # by under 1 %); a candidate displaces 8x8x8 (listed first) only when
+        # over 1 % faster
         from lbfs_amd import autotune
-        params, fused, table = autotune.tune(mesh, params, state=state, steps=20, warmup=5)
+        params, fused, table = autotune.tune(mesh, params, state=state, steps=50, warmup=10,
+                                             candidates=[c for c in autotune.CANDIDATES if c[3] == 512],
+                                             membranes=workloads.single_rbc(mesh), ibm_kernel=abi.IBM_COS4,
+                                             margin=0.01)
         tuning = [{"tile": list(t), "threads": nt, "fused": f, "ms_per_step": ms} for t, nt, f, ms in table]
         log("autotuned tile %s, %d threads, %s fluid pass" % (list(params.tile), params.block_threads,
                                                               "fused" if fused else "two-pass"))

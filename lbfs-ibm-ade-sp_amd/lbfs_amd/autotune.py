@@ -39,10 +39,13 @@ def with_tile(params, tile, threads):
     return p
 
 
-def tune(mesh, params, state=None, steps=20, warmup=5, candidates=CANDIDATES, membranes=None, ibm_kernel=None):
+def tune(mesh, params, state=None, steps=20, warmup=5, candidates=CANDIDATES, membranes=None, ibm_kernel=None,
+         margin=0.0):
     """Time `steps` whole steps per candidate (tile, threads) with the fused
     fluid pass where the tiles fit it and with the two passes, and return
-    (best_params, best_fused, table).
+    (best_params, best_fused, table).  A later candidate replaces the best so
+    far only when it is faster by more than `margin` (relative), so that
+    timings within the run-to-run spread keep the earlier candidate.
 
     table: list of (tile, threads, fused, ms_per_step).  Candidates whose
     tiles do not fit the mesh or the LDS budget are skipped."""
@@ -71,7 +74,7 @@ def tune(mesh, params, state=None, steps=20, warmup=5, candidates=CANDIDATES, me
             ms = s.time_steps(steps) / steps
             s.close()
             table.append(((tx, ty, tz), nt, fused, ms))
-            if best is None or ms < best[2]:
+            if best is None or ms < best[2] * (1.0 - margin):
                 best = (p, fused, ms)
     if best is None:
         raise abi.LbfsError("no tile candidate fits this mesh")
