@@ -113,22 +113,28 @@ def host_cpu():
     return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
 
 
-def profile_record(kernel):
+def profile_record(kernel, tile=None):
     """The committed rocprofv3 counter summary of `kernel` (a stage name) on
-    this workload, or None."""
+    this workload: the entry for the tile shape run ("fluid@16x8x4") where
+    there is one, else the 8x8x8 entry, or None."""
     path = os.path.join(REPO, "profiles", "traffic_channel_128x64x64.json")
     try:
         with open(path) as f:
-            return json.load(f).get(kernel)
+            d = json.load(f)
     except (OSError, ValueError):
         return None
+    if tile is not None:
+        rec = d.get("%s@%dx%dx%d" % (kernel, tile[0], tile[1], tile[2]))
+        if rec is not None:
+            return rec
+    return d.get(kernel)
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, tile=None):
     """HBM bytes per launch of `kernel` on this workload from the committed
     rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE per dispatch, each
     scaled by the same counter's reading on a copy of known size), or None."""
-    rec = profile_record(kernel)
+    rec = profile_record(kernel, tile)
     return None if rec is None else rec.get("bytes_per_launch")
 
 
@@ -138,11 +144,11 @@ def measured_traffic(kernel):
 VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 4
 
 
-def valu_roofline(kernel, launch_ms):
+def valu_roofline(kernel, launch_ms, tile=None):
     """The dominant kernel against its vector-ALU floor: wave-level VALU
     instructions per launch (SQ_INSTS_VALU from the committed counter summary)
     over the chip's issue rate, as a fraction of the measured launch time."""
-    rec = profile_record(kernel)
+    rec = profile_record(kernel, tile)
     insts = None if rec is None else rec.get("valu_insts_per_launch")
     if not insts:
         return None
@@ -559,9 +565,9 @@ def main():
         "plain_mlups": None if args.only_rbc else n / (ms_plain * 1e-3) / 1e6,
         "rbc_step_vs_plain": None if args.only_rbc else ms_step / ms_plain,
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(names[dom]),
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(names[dom], tuple(params.tile)),
                      "model_bytes_per_launch": mbytes[dom], "avg_launch_ms": dom_ms,
-                     "valu": valu_roofline(names[dom], dom_ms)},
+                     "valu": valu_roofline(names[dom], dom_ms, tuple(params.tile))},
         "kernels_ms": kms,
         "setup": setup,
         "event_ms_per_step": ev_ms,
