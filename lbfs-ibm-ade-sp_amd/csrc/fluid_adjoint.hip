@@ -459,13 +459,16 @@ __device__ __forceinline__ float4 adj_grad_cell(const FluidArgs &a, const AdjFlu
     float acc[4] = {l0.x, l0.y, l0.z, l0.w};
     float g[12];
     lam_grad(j.lam_g, i, g);
-    // every stencil index first, then the neighbours' rows and adjoint
-    // gradients three entries at a time (a boundary entry loads the cell's
-    // own, unused), so the loads are in flight together rather than one
-    // index and one neighbour at a time
+    // every stencil index and weight first, then the neighbours' rows and
+    // adjoint gradients three entries at a time (a boundary entry loads the
+    // cell's own, unused), so the loads are in flight together rather than
+    // one index and one neighbour at a time
     int jk[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) jk[k] = a.stencil[(size_t)k * n + i];
+    float wk[18];
+#pragma unroll
+    for (int k = 0; k < 18; k++) wk[k] = a.gw[(size_t)k * n + i];
 #pragma unroll
     for (int k0 = 0; k0 < 6; k0 += 3) {
         float4 rk[3];
@@ -479,9 +482,7 @@ __device__ __forceinline__ float4 adj_grad_cell(const FluidArgs &a, const AdjFlu
 #pragma unroll
         for (int u = 0; u < 3; u++) {
             const int k = k0 + u, jn = jk[k];
-            const float wx = a.gw[(size_t)(3 * k) * n + i];
-            const float wy = a.gw[(size_t)(3 * k + 1) * n + i];
-            const float wz = a.gw[(size_t)(3 * k + 2) * n + i];
+            const float wx = wk[3 * k], wy = wk[3 * k + 1], wz = wk[3 * k + 2];
             float c[4] = {1.0f, 1.0f, 1.0f, 1.0f};
             if (jn < 0) {
                 const int2 ty = a.bc_type[-1 - jn];
