@@ -158,7 +158,7 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
     if constexpr (INTERP) {
         const int4 nb = g.nbase[nd];
         float ux = 0.0f, uy = 0.0f, uz = 0.0f;
-        int w[3];
+        int w[3] = {0, 0, 0};
         const bool rowok = wrap_index(B, 1, nb.y + j, w[1]) && wrap_index(B, 2, nb.z + k, w[2]);
         // the row's cell indices first, then its states and kept weights (a
         // cell off a wall loads cell 0, unused), so the loads are in flight
@@ -202,14 +202,17 @@ __global__ __launch_bounds__(256) void k_ibm_interp(MemArgs m, IbmArgs g, const 
     float wt[S];
     float ws = 0.0f;
     {
-        int w[3];
+        int w[3] = {0, 0, 0};
         const bool rowok = wrap_index(B, 1, base[1] + j, w[1]) && wrap_index(B, 2, base[2] + k, w[2]);
         // the row's wrapped indices, then (cosine kernel) every centroid at
         // once, then the weights
         bool ok[S];
         int wx[S];
 #pragma unroll
-        for (int i = 0; i < S; i++) ok[i] = rowok && wrap_index(B, 0, base[0] + i, wx[i]);
+        for (int i = 0; i < S; i++) {
+            wx[i] = 0;
+            ok[i] = rowok && wrap_index(B, 0, base[0] + i, wx[i]);
+        }
         float4 c[S];
 #pragma unroll
         for (int i = 0; i < S; i++) {
